@@ -1,0 +1,193 @@
+// Host f64 dense linear algebra of the solver's small problems: the symmetric eigensolver, LU and Cholesky, and the
+// generalised Rayleigh–Ritz step of LOBPCG (m <= a few hundred). These run on the CPU in f64 in every precision mode.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "lira_internal.hpp"
+
+namespace lira {
+
+// Cyclic Jacobi (Golub & Van Loan §8.5; the reference's fallback eigensolver, crates/qm/src/eigen.rs:175-211):
+// rotations until the off-diagonal norm is below 1e-15 ||A||; pairs returned ascending. a is m x m row-major.
+void sym_eigen(int m, std::vector<double> &a, std::vector<double> &evals, std::vector<double> &evecs) {
+    std::vector<double> v((size_t)m * m, 0.0);
+    for (int i = 0; i < m; i++) v[(size_t)i * m + i] = 1.0;
+    double norm = 0.0;
+    for (double x : a) norm += x * x;
+    norm = std::sqrt(norm);
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0;
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < m; j++)
+                if (i != j) off += a[(size_t)i * m + j] * a[(size_t)i * m + j];
+        if (!(std::sqrt(off) > 1e-15 * norm)) break;
+        for (int p = 0; p < m; p++)
+            for (int q = p + 1; q < m; q++) {
+                const double apq = a[(size_t)p * m + q];
+                if (apq == 0.0) continue;
+                const double theta = (a[(size_t)q * m + q] - a[(size_t)p * m + p]) / (2.0 * apq);
+                double t = (theta > 0 ? 1.0 : (theta < 0 ? -1.0 : 0.0)) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                if (theta == 0.0) t = 1.0;
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < m; k++) {
+                    const double akp = a[(size_t)k * m + p], akq = a[(size_t)k * m + q];
+                    a[(size_t)k * m + p] = c * akp - s * akq;
+                    a[(size_t)k * m + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < m; k++) {
+                    const double apk = a[(size_t)p * m + k], aqk = a[(size_t)q * m + k];
+                    a[(size_t)p * m + k] = c * apk - s * aqk;
+                    a[(size_t)q * m + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < m; k++) {
+                    const double vkp = v[(size_t)k * m + p], vkq = v[(size_t)k * m + q];
+                    v[(size_t)k * m + p] = c * vkp - s * vkq;
+                    v[(size_t)k * m + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    std::vector<int> order(m);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return a[(size_t)x * m + x] < a[(size_t)y * m + y]; });
+    evals.assign(m, 0.0);
+    evecs.assign((size_t)m * m, 0.0);
+    for (int c = 0; c < m; c++) {
+        evals[c] = a[(size_t)order[c] * m + order[c]];
+        for (int r = 0; r < m; r++) evecs[(size_t)r * m + c] = v[(size_t)r * m + order[c]];
+    }
+}
+
+// LU with partial pivoting; false on a zero pivot (nalgebra's LU::solve returning None, crates/qm/src/scf.rs:696)
+bool lu_solve(int m, std::vector<double> a, std::vector<double> &b) {
+    for (int c = 0; c < m; c++) {
+        int piv = c;
+        double best = std::fabs(a[(size_t)c * m + c]);
+        for (int r = c + 1; r < m; r++)
+            if (std::fabs(a[(size_t)r * m + c]) > best) {
+                best = std::fabs(a[(size_t)r * m + c]);
+                piv = r;
+            }
+        if (best == 0.0) return false;
+        if (piv != c) {
+            for (int k = 0; k < m; k++) std::swap(a[(size_t)c * m + k], a[(size_t)piv * m + k]);
+            std::swap(b[c], b[piv]);
+        }
+        for (int r = c + 1; r < m; r++) {
+            const double f = a[(size_t)r * m + c] / a[(size_t)c * m + c];
+            for (int k = c; k < m; k++) a[(size_t)r * m + k] -= f * a[(size_t)c * m + k];
+            b[r] -= f * b[c];
+        }
+    }
+    for (int r = m - 1; r >= 0; r--) {
+        double s = b[r];
+        for (int k = r + 1; k < m; k++) s -= a[(size_t)r * m + k] * b[k];
+        b[r] = s / a[(size_t)r * m + r];
+    }
+    return true;
+}
+
+bool cholesky(int m, std::vector<double> &a) {
+    for (int j = 0; j < m; j++) {
+        double d = a[(size_t)j * m + j];
+        for (int k = 0; k < j; k++) d -= a[(size_t)j * m + k] * a[(size_t)j * m + k];
+        if (!(d > 0.0)) return false;
+        d = std::sqrt(d);
+        a[(size_t)j * m + j] = d;
+        for (int i = j + 1; i < m; i++) {
+            double s = a[(size_t)i * m + j];
+            for (int k = 0; k < j; k++) s -= a[(size_t)i * m + k] * a[(size_t)j * m + k];
+            a[(size_t)i * m + j] = s / d;
+        }
+        for (int k = j + 1; k < m; k++) a[(size_t)j * m + k] = 0.0;
+    }
+    return true;
+}
+
+// solves (L L^T) X = B for nrhs columns of B (row-major m x nrhs), in place
+void chol_solve(int m, const std::vector<double> &l, std::vector<double> &b, int nrhs) {
+    for (int c = 0; c < nrhs; c++) {
+        for (int i = 0; i < m; i++) {
+            double s = b[(size_t)i * nrhs + c];
+            for (int k = 0; k < i; k++) s -= l[(size_t)i * m + k] * b[(size_t)k * nrhs + c];
+            b[(size_t)i * nrhs + c] = s / l[(size_t)i * m + i];
+        }
+        for (int i = m - 1; i >= 0; i--) {
+            double s = b[(size_t)i * nrhs + c];
+            for (int k = i + 1; k < m; k++) s -= l[(size_t)k * m + i] * b[(size_t)k * nrhs + c];
+            b[(size_t)i * nrhs + c] = s / l[(size_t)i * m + i];
+        }
+    }
+}
+
+// Generalised Rayleigh–Ritz A c = theta B c on a raw (not orthonormal) subspace (crates/qm-gpu/src/eigen.rs:515-542):
+// D-scaling to unit diagonal, the eigen-decomposition of B~ with directions below drop * sigma_max removed,
+// T = U Sigma^{-1/2}, the standard problem T^T A~ T, and C = D T Y. Returns alpha = max_i ||Sigma^{-1/2} y_i||, the
+// coefficient amplification of the step (1 for an orthonormal subspace).
+double rayleigh_ritz(int m, const std::vector<double> &A, const std::vector<double> &B, int nk, double drop,
+                     std::vector<double> &theta, std::vector<double> &C) {
+    for (int i = 0; i < m; i++)
+        if (!(B[(size_t)i * m + i] > 0.0))
+            throw NumericalError("lobpcg: column " + std::to_string(i) + " of the subspace has a non-positive norm");
+    std::vector<double> d(m);
+    for (int i = 0; i < m; i++) d[i] = 1.0 / std::sqrt(B[(size_t)i * m + i]);
+    std::vector<double> an((size_t)m * m), bn((size_t)m * m);
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < m; j++) {
+            an[(size_t)i * m + j] = A[(size_t)i * m + j] * d[i] * d[j];
+            bn[(size_t)i * m + j] = B[(size_t)i * m + j] * d[i] * d[j];
+        }
+    std::vector<double> eb, ub;
+    sym_eigen(m, bn, eb, ub);
+    const double smax = *std::max_element(eb.begin(), eb.end());
+    std::vector<int> keep;
+    for (int i = 0; i < m; i++)
+        if (eb[i] > drop * smax) keep.push_back(i);
+    const int nkeep = (int)keep.size();
+    if (nkeep < nk)
+        throw NumericalError("lobpcg: the subspace lost rank (" + std::to_string(nkeep) + " of " + std::to_string(m) +
+                             " directions kept, " + std::to_string(nk) + " needed)");
+    std::vector<double> t((size_t)m * nkeep);
+    for (int i = 0; i < m; i++)
+        for (int c = 0; c < nkeep; c++) t[(size_t)i * nkeep + c] = ub[(size_t)i * m + keep[c]] / std::sqrt(eb[keep[c]]);
+    // ah = T^T an T
+    std::vector<double> at((size_t)m * nkeep, 0.0);
+    for (int i = 0; i < m; i++)
+        for (int k = 0; k < m; k++) {
+            const double aik = an[(size_t)i * m + k];
+            if (aik == 0.0) continue;
+            for (int c = 0; c < nkeep; c++) at[(size_t)i * nkeep + c] += aik * t[(size_t)k * nkeep + c];
+        }
+    std::vector<double> ah((size_t)nkeep * nkeep, 0.0);
+    for (int r = 0; r < nkeep; r++)
+        for (int c = 0; c < nkeep; c++) {
+            double s = 0.0;
+            for (int i = 0; i < m; i++) s += t[(size_t)i * nkeep + r] * at[(size_t)i * nkeep + c];
+            ah[(size_t)r * nkeep + c] = s;
+        }
+    for (int r = 0; r < nkeep; r++)
+        for (int c = r + 1; c < nkeep; c++) {
+            const double s = 0.5 * (ah[(size_t)r * nkeep + c] + ah[(size_t)c * nkeep + r]);
+            ah[(size_t)r * nkeep + c] = ah[(size_t)c * nkeep + r] = s;
+        }
+    std::vector<double> ea, ya;
+    sym_eigen(nkeep, ah, ea, ya);
+    theta.assign(ea.begin(), ea.begin() + nk);
+    double alpha = 0.0;
+    for (int c = 0; c < nk; c++) {
+        double s = 0.0;
+        for (int dd = 0; dd < nkeep; dd++) s += ya[(size_t)dd * nkeep + c] * ya[(size_t)dd * nkeep + c] / eb[keep[dd]];
+        alpha = std::max(alpha, std::sqrt(s));
+    }
+    C.assign((size_t)m * nk, 0.0);
+    for (int i = 0; i < m; i++)
+        for (int c = 0; c < nk; c++) {
+            double s = 0.0;
+            for (int k = 0; k < nkeep; k++) s += t[(size_t)i * nkeep + k] * ya[(size_t)k * nkeep + c];
+            C[(size_t)i * nk + c] = d[i] * s;
+        }
+    return alpha;
+}
+
+}  // namespace lira

@@ -1,0 +1,828 @@
+// gfx950 kernels on the dense n^3 grid: the block stencil H.X, reductions, the tall-skinny Gram S^T U, block
+// linear combinations, density accumulation, LSDA, nuclear potential, orbital guess, DIIS mix and HF forces.
+// All kernels are written for 64-wide wavefronts; reductions are deterministic (fixed partial counts, fixed order).
+#include "lira_internal.hpp"
+
+#include <cmath>
+#include <cstdio>
+
+namespace lira {
+
+void check_hip(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void DevBuf::alloc(size_t b) {
+    release();
+    if (b == 0) b = 256;
+    LIRA_HIP(hipMalloc(&ptr, b));
+    bytes = b;
+}
+
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+// 8th-order second-derivative coefficients c1..c4 (qm/src/stencil.rs:24; c0 = -205/72 is folded into the
+// differences: sum_m c_m[(f+ - f) + (f- - f)] per axis equals c0 f + sum_m c_m (f+ + f-) since c0 = -2 sum c_m).
+#define C1 (8.0 / 5.0)
+#define C2 (-1.0 / 5.0)
+#define C3 (8.0 / 315.0)
+#define C4 (-1.0 / 560.0)
+
+// ------------------------------------------------------------------------------------------ stencil ----
+// 2.5-D z-streaming: a workgroup owns a 64 x TY tile of (x, y) columns of one band and a chunk of z planes.
+// Each thread keeps its RPT columns' 9-plane z-window in registers; the x/y taps of the current plane come from an
+// LDS tile with a 4-point halo (double-buffered: one barrier per plane). The potential V is read once per point.
+constexpr int ST_TX = 64;
+
+template <typename T, int MODE, int RPT>
+__global__ __launch_bounds__(256) void k_stencil(const T *__restrict__ x, T *__restrict__ out, const T *__restrict__ v,
+                                                 const T *__restrict__ r, const double *__restrict__ lam, int n,
+                                                 long N, int zc, T inv_h2, T scale, T omega, T diag_t) {
+    constexpr int TY = 4 * RPT;
+    constexpr int LX = ST_TX + 8;
+    __shared__ T tile[2][TY + 8][LX];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int t = ty * 64 + tx;
+    const int x0 = blockIdx.x * ST_TX, y0 = blockIdx.y * TY;
+    const int nzc = (n + zc - 1) / zc;
+    const int band = blockIdx.z / nzc, zi = blockIdx.z - band * nzc;
+    const int z0 = zi * zc;
+    const int z1 = min(n, z0 + zc);
+    const T *xb = x + (long)band * N;
+    T *ob = out + (long)band * N;
+    const T *rb = (MODE == ST_JAC) ? r + (long)band * N : nullptr;
+    const T lamb = (MODE == ST_JAC) ? (T)lam[band] : T(0);
+    const int gx = x0 + tx;
+    const bool xin = gx < n;
+    const long nn = (long)n * n;
+
+    auto ld = [&](int xx, int yy, int zz) -> T {
+        return (xx >= 0 && xx < n && yy >= 0 && yy < n && zz >= 0 && zz < n) ? xb[xx + (long)n * yy + nn * zz] : T(0);
+    };
+
+    T q[9][RPT];
+#pragma unroll
+    for (int d = 0; d < 8; d++)
+#pragma unroll
+        for (int rr = 0; rr < RPT; rr++) q[d][rr] = ld(gx, y0 + ty + 4 * rr, z0 - 4 + d);
+
+    int buf = 0;
+    for (int z = z0; z < z1; z++) {
+#pragma unroll
+        for (int rr = 0; rr < RPT; rr++) q[8][rr] = ld(gx, y0 + ty + 4 * rr, z + 4);
+        // interior of plane z from the queue
+#pragma unroll
+        for (int rr = 0; rr < RPT; rr++) tile[buf][ty + 4 * rr + 4][tx + 4] = q[4][rr];
+        // y halo: 8 rows x 64 columns
+#pragma unroll
+        for (int s2 = 0; s2 < 2; s2++) {
+            int e = t + 256 * s2;
+            int row = e >> 6;
+            int ly = row < 4 ? row : TY + row;
+            int gy = y0 - 4 + ly;
+            tile[buf][ly][tx + 4] = ld(gx, gy, z);
+        }
+        // x halo: (TY + 8) rows x 8 columns
+        if (t < 8 * (TY + 8)) {
+            int row = t >> 3, c = t & 7;
+            int lx = c < 4 ? c : ST_TX + c;
+            tile[buf][row][lx] = ld(x0 - 4 + lx, y0 - 4 + row, z);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int rr = 0; rr < RPT; rr++) {
+            const int ly = ty + 4 * rr + 4, lx = tx + 4;
+            const T c = q[4][rr];
+            T ax = C1 * ((tile[buf][ly][lx + 1] - c) + (tile[buf][ly][lx - 1] - c)) +
+                   C2 * ((tile[buf][ly][lx + 2] - c) + (tile[buf][ly][lx - 2] - c)) +
+                   C3 * ((tile[buf][ly][lx + 3] - c) + (tile[buf][ly][lx - 3] - c)) +
+                   C4 * ((tile[buf][ly][lx + 4] - c) + (tile[buf][ly][lx - 4] - c));
+            T ay = C1 * ((tile[buf][ly + 1][lx] - c) + (tile[buf][ly - 1][lx] - c)) +
+                   C2 * ((tile[buf][ly + 2][lx] - c) + (tile[buf][ly - 2][lx] - c)) +
+                   C3 * ((tile[buf][ly + 3][lx] - c) + (tile[buf][ly - 3][lx] - c)) +
+                   C4 * ((tile[buf][ly + 4][lx] - c) + (tile[buf][ly - 4][lx] - c));
+            T az = C1 * ((q[5][rr] - c) + (q[3][rr] - c)) + C2 * ((q[6][rr] - c) + (q[2][rr] - c)) +
+                   C3 * ((q[7][rr] - c) + (q[1][rr] - c)) + C4 * ((q[8][rr] - c) + (q[0][rr] - c));
+            const T lap = ((ax + ay) + az) * inv_h2;
+            const int gy = y0 + ty + 4 * rr;
+            if (xin && gy < n) {
+                const long id = gx + (long)n * gy + nn * z;
+                if (MODE == ST_LAP) {
+                    ob[id] = scale * lap;
+                } else if (MODE == ST_HAM) {
+                    ob[id] = T(-0.5) * lap + v[id] * c;
+                } else {
+                    const T vv = v[id];
+                    const T hw = T(-0.5) * lap + vv * c - lamb * c;
+                    T d = diag_t + vv - lamb;
+                    d = d > T(0.1) ? d : T(0.1);
+                    ob[id] = c + omega / d * (rb[id] - hw);
+                }
+            }
+        }
+        buf ^= 1;
+#pragma unroll
+        for (int d = 0; d < 8; d++)
+#pragma unroll
+            for (int rr = 0; rr < RPT; rr++) q[d][rr] = q[d + 1][rr];
+    }
+}
+
+struct StVariant {
+    int rpt;
+    int zc;  // 0 = whole column
+    const char *name;
+};
+static const StVariant ST_VARIANTS[] = {
+    {4, 0, "64x16 tile, full z column"}, {4, 32, "64x16 tile, z chunks of 32"}, {4, 16, "64x16 tile, z chunks of 16"},
+    {2, 0, "64x8 tile, full z column"},  {2, 16, "64x8 tile, z chunks of 16"},  {2, 8, "64x8 tile, z chunks of 8"},
+};
+static int g_default_variant[2] = {-1, -1};
+
+int stencil_variants() { return (int)(sizeof(ST_VARIANTS) / sizeof(ST_VARIANTS[0])); }
+const char *stencil_variant_name(int v) { return (v >= 0 && v < stencil_variants()) ? ST_VARIANTS[v].name : "auto"; }
+void stencil_set_default_variant(int dtype_size, int variant) { g_default_variant[dtype_size == 8 ? 1 : 0] = variant; }
+
+// Heuristic when no tuned choice exists: whole columns when the band x tile count alone fills the chip, z chunks
+// otherwise (small grids, few bands).
+static int pick_variant(int n, int nb) {
+    long tiles16 = (long)((n + 63) / 64) * ((n + 15) / 16) * nb;
+    if (tiles16 >= 1024) return 0;
+    if (tiles16 * ((n + 31) / 32) >= 1024) return 1;
+    return 2;
+}
+
+template <typename T, int MODE, int RPT>
+static void launch_stencil(const StencilParams &p, int zc, const T *x, T *out, const T *v, const T *r, const double *lam,
+                           hipStream_t s) {
+    const int n = p.n;
+    const long N = (long)n * n * n;
+    if (zc <= 0 || zc > n) zc = n;
+    const int nzc = (n + zc - 1) / zc;
+    dim3 grid((n + ST_TX - 1) / ST_TX, (n + 4 * RPT - 1) / (4 * RPT), nzc * p.nb);
+    dim3 block(64, 4, 1);
+    hipLaunchKernelGGL((k_stencil<T, MODE, RPT>), grid, block, 0, s, x, out, v, r, lam, n, N, zc,
+                       (T)(1.0 / (p.h * p.h)), (T)p.scale, (T)p.omega, (T)p.diag_t);
+    LIRA_HIP(hipGetLastError());
+}
+
+template <typename T>
+void stencil(int mode, const StencilParams &p, const T *x, T *out, const T *v, const T *r, const double *lam,
+             hipStream_t s, int variant) {
+    if (p.nb <= 0) return;
+    if (x == out) throw InvalidError("stencil: output aliases the input");
+    if (variant < 0) variant = g_default_variant[sizeof(T) == 8 ? 1 : 0];
+    if (variant < 0 || variant >= stencil_variants()) variant = pick_variant(p.n, p.nb);
+    const StVariant &sv = ST_VARIANTS[variant];
+#define LIRA_ST(MODE_, RPT_) launch_stencil<T, MODE_, RPT_>(p, sv.zc, x, out, v, r, lam, s)
+    if (sv.rpt == 4) {
+        if (mode == ST_LAP) LIRA_ST(ST_LAP, 4);
+        else if (mode == ST_HAM) LIRA_ST(ST_HAM, 4);
+        else LIRA_ST(ST_JAC, 4);
+    } else {
+        if (mode == ST_LAP) LIRA_ST(ST_LAP, 2);
+        else if (mode == ST_HAM) LIRA_ST(ST_HAM, 2);
+        else LIRA_ST(ST_JAC, 2);
+    }
+#undef LIRA_ST
+}
+
+// ------------------------------------------------------------------------------------------ reductions ----
+__device__ inline double block_sum256(double v, double *sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+#pragma unroll
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ inline double block_max256(double v, double *sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+#pragma unroll
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_reduce(RedJobs<T> jobs, long N, int n, double *partials) {
+    __shared__ double sh[256];
+    const int j = blockIdx.y;
+    const int kind = jobs.kind[j];
+    const T *a = jobs.a[j];
+    const T *b = jobs.b[j];
+    double acc = 0.0;
+    const long stride = (long)gridDim.x * 256;
+    if (kind == RED_DOT) {
+        for (long p = blockIdx.x * 256L + threadIdx.x; p < N; p += stride) acc += (double)a[p] * (double)b[p];
+    } else if (kind == RED_SUM) {
+        for (long p = blockIdx.x * 256L + threadIdx.x; p < N; p += stride) acc += (double)a[p];
+    } else if (kind == RED_ABSDIFF) {
+        for (long p = blockIdx.x * 256L + threadIdx.x; p < N; p += stride) acc += fabs((double)a[p] - (double)b[p]);
+    } else if (kind == RED_FACEMAX) {
+        const long nn = (long)n * n;
+        for (long p = blockIdx.x * 256L + threadIdx.x; p < N; p += stride) {
+            const long k = p / nn, rem = p - k * nn, jj = rem / n, i = rem - jj * n;
+            if (i == 0 || jj == 0 || k == 0 || i == n - 1 || jj == n - 1 || k == n - 1) acc = fmax(acc, fabs((double)a[p]));
+        }
+    } else {
+        for (long p = blockIdx.x * 256L + threadIdx.x; p < N; p += stride) acc = fmax(acc, fabs((double)a[p]));
+    }
+    const bool is_max = kind == RED_FACEMAX || kind == RED_MAX_ABS;
+    const double r = is_max ? block_max256(acc, sh) : block_sum256(acc, sh);
+    if (threadIdx.x == 0) partials[(long)j * RED_BLOCKS + blockIdx.x] = r;
+}
+
+struct KindList {
+    int kind[MAX_JOBS];
+};
+
+__global__ __launch_bounds__(256) void k_reduce_finish(KindList kinds, const double *partials, double *out,
+                                                        double scale) {
+    __shared__ double sh[256];
+    const int j = blockIdx.x;
+    const bool is_max = kinds.kind[j] == RED_FACEMAX || kinds.kind[j] == RED_MAX_ABS;
+    double acc = 0.0;
+    for (int p = threadIdx.x; p < RED_BLOCKS; p += 256) {
+        const double v = partials[(long)j * RED_BLOCKS + p];
+        acc = is_max ? fmax(acc, v) : acc + v;
+    }
+    const double r = is_max ? block_max256(acc, sh) : block_sum256(acc, sh);
+    if (threadIdx.x == 0) out[j] = is_max ? r : r * scale;
+}
+
+template <typename T>
+void reduce_jobs(const RedJobs<T> &jobs, int n, double scale, double *partials, double *out, hipStream_t s) {
+    if (jobs.count <= 0) return;
+    if (jobs.count > MAX_JOBS) throw InvalidError("reduce_jobs: too many jobs");
+    const long N = (long)n * n * n;
+    hipLaunchKernelGGL(k_reduce<T>, dim3(RED_BLOCKS, jobs.count), dim3(256), 0, s, jobs, N, n, partials);
+    LIRA_HIP(hipGetLastError());
+    KindList kl;
+    for (int j = 0; j < MAX_JOBS; j++) kl.kind[j] = j < jobs.count ? jobs.kind[j] : RED_SUM;
+    hipLaunchKernelGGL(k_reduce_finish, dim3(jobs.count), dim3(256), 0, s, kl, partials, out, scale);
+    LIRA_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------------------------- Gram ----
+// G = scale S^T U over N points. A workgroup computes a 32 (S) x 64 (U) output block over one chunk of points,
+// staging 64 points of its 96 columns in LDS at a time; each thread owns a 2 x 4 register tile accumulated in f64.
+constexpr int GR_BI = 32, GR_BJ = 64, GR_P = 64;
+
+struct ColPtrs {
+    const void *base[MAX_SEGS];
+    int start[MAX_SEGS + 1];
+    int nseg;
+};
+
+__device__ inline const void *col_base(const ColPtrs &c, int col, long N, int esize) {
+    int s = 0;
+#pragma unroll
+    for (int i = 1; i < MAX_SEGS; i++)
+        if (i < c.nseg && col >= c.start[i]) s = i;
+    return static_cast<const char *>(c.base[s]) + (size_t)(col - c.start[s]) * (size_t)N * (size_t)esize;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q, long N, long chunk,
+                                              double *partials) {
+    __shared__ T ss[GR_BI][GR_P + 1];
+    __shared__ T us[GR_BJ][GR_P + 1];
+    const int nbj = (q + GR_BJ - 1) / GR_BJ;
+    const int bi = blockIdx.y / nbj, bj = blockIdx.y - bi * nbj;
+    const int i0 = bi * GR_BI, j0 = bj * GR_BJ;
+    const int t = threadIdx.x;
+    const int ti = t >> 4, tj = t & 15;  // 16 x 16 threads, tile 2 x 4
+    double acc[2][4] = {};
+    const long p_begin = (long)blockIdx.x * chunk;
+    const long p_end = min(N, p_begin + chunk);
+    for (long p0 = p_begin; p0 < p_end; p0 += GR_P) {
+        // stage: 96 columns x 64 points; thread loads column rows in strides
+        for (int e = t; e < (GR_BI + GR_BJ) * GR_P; e += 256) {
+            const int col = e / GR_P, pp = e - col * GR_P;
+            const long p = p0 + pp;
+            if (col < GR_BI) {
+                const int ci = i0 + col;
+                T val = T(0);
+                if (ci < m && p < p_end) val = static_cast<const T *>(col_base(S, ci, N, sizeof(T)))[p];
+                ss[col][pp] = val;
+            } else {
+                const int cj = j0 + col - GR_BI;
+                T val = T(0);
+                if (cj < q && p < p_end) val = static_cast<const T *>(col_base(U, cj, N, sizeof(T)))[p];
+                us[col - GR_BI][pp] = val;
+            }
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int pp = 0; pp < GR_P; pp++) {
+            const double s0 = (double)ss[2 * ti][pp], s1 = (double)ss[2 * ti + 1][pp];
+            double u[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) u[c] = (double)us[4 * tj + c][pp];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                acc[0][c] += s0 * u[c];
+                acc[1][c] += s1 * u[c];
+            }
+        }
+        __syncthreads();
+    }
+    double *dst = partials + (long)blockIdx.x * m * q;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int i = i0 + 2 * ti + a, j = j0 + 4 * tj + c;
+            if (i < m && j < q) dst[(long)i * q + j] = acc[a][c];
+        }
+}
+
+__global__ __launch_bounds__(256) void k_gram_finish(const double *partials, int chunks, long mq, double scale,
+                                                      double *G) {
+    const long idx = blockIdx.x * 256L + threadIdx.x;
+    if (idx >= mq) return;
+    double acc = 0.0;
+    for (int c = 0; c < chunks; c++) acc += partials[(long)c * mq + idx];
+    G[idx] = acc * scale;
+}
+
+static int gram_chunks(int m, int q, long N) {
+    const int blocks = ((m + GR_BI - 1) / GR_BI) * ((q + GR_BJ - 1) / GR_BJ);
+    int chunks = (1024 + blocks - 1) / blocks;
+    const long max_chunks = (N + GR_P - 1) / GR_P;
+    if (chunks > max_chunks) chunks = (int)max_chunks;
+    if (chunks < 1) chunks = 1;
+    return chunks;
+}
+
+size_t gram_partials_needed(int m, int q, long N) { return (size_t)gram_chunks(m, q, N) * m * q; }
+
+static ColPtrs to_ptrs(const Cols &c) {
+    ColPtrs p{};
+    p.nseg = c.nseg;
+    int acc = 0;
+    for (int i = 0; i < c.nseg; i++) {
+        p.base[i] = c.seg[i].base;
+        p.start[i] = acc;
+        acc += c.seg[i].count;
+    }
+    p.start[c.nseg] = acc;
+    return p;
+}
+
+template <typename T>
+void gram(const Cols &S, const Cols &U, long N, double scale, double *partials, size_t partial_cap, double *G,
+          hipStream_t s) {
+    const int m = S.total(), q = U.total();
+    if (m == 0 || q == 0) return;
+    const int chunks = gram_chunks(m, q, N);
+    if ((size_t)chunks * m * q > partial_cap) throw InvalidError("gram: partial buffer too small");
+    long chunk = (N + chunks - 1) / chunks;
+    chunk = ((chunk + GR_P - 1) / GR_P) * GR_P;
+    const int blocks = ((m + GR_BI - 1) / GR_BI) * ((q + GR_BJ - 1) / GR_BJ);
+    hipLaunchKernelGGL(k_gram<T>, dim3(chunks, blocks), dim3(256), 0, s, to_ptrs(S), to_ptrs(U), m, q, N, chunk,
+                       partials);
+    LIRA_HIP(hipGetLastError());
+    const long mq = (long)m * q;
+    hipLaunchKernelGGL(k_gram_finish, dim3((unsigned)((mq + 255) / 256)), dim3(256), 0, s, partials, chunks, mq, scale,
+                       G);
+    LIRA_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------------------------ lincomb ----
+// Y_j = sum_i S_i C_ij for up to 16 outputs per pass (grid.y groups of 16 outputs); f64 accumulation.
+constexpr int LC_Q = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_lincomb(ColPtrs S, int m, const double *__restrict__ C, int q,
+                                                 T *__restrict__ Y, int qsplit, T *__restrict__ Y2, long N) {
+    const long p = blockIdx.x * 256L + threadIdx.x;
+    const int j0 = blockIdx.y * LC_Q;
+    if (p >= N) return;
+    double acc[LC_Q];
+#pragma unroll
+    for (int c = 0; c < LC_Q; c++) acc[c] = 0.0;
+    const int qn = min(LC_Q, q - j0);
+    for (int i = 0; i < m; i++) {
+        const double sv = (double)static_cast<const T *>(col_base(S, i, N, sizeof(T)))[p];
+        const double *row = C + (long)i * q + j0;
+#pragma unroll
+        for (int c = 0; c < LC_Q; c++)
+            if (c < qn) acc[c] += sv * row[c];
+    }
+#pragma unroll
+    for (int c = 0; c < LC_Q; c++)
+        if (c < qn) {
+            const int j = j0 + c;
+            T *dst = j < qsplit ? Y + (long)j * N : Y2 + (long)(j - qsplit) * N;
+            dst[p] = (T)acc[c];
+        }
+}
+
+template <typename T>
+void lincomb(const Cols &S, const double *C, int q, T *Y, long N, hipStream_t s, int qsplit, T *Y2) {
+    const int m = S.total();
+    if (q <= 0) return;
+    if (qsplit < 0 || qsplit > q) qsplit = q;
+    hipLaunchKernelGGL(k_lincomb<T>, dim3((unsigned)((N + 255) / 256), (q + LC_Q - 1) / LC_Q), dim3(256), 0, s,
+                       to_ptrs(S), m, C, q, Y, qsplit, Y2, N);
+    LIRA_HIP(hipGetLastError());
+}
+
+// ----------------------------------------------------------------------------------------- pointwise ----
+#define LIRA_GRID1(N) dim3((unsigned)(((N) + 255) / 256)), dim3(256)
+
+template <typename T> __global__ void k_fill(T *y, long n, T a) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i < n) y[i] = a;
+}
+template <typename T> void fill(T *y, long count, double a, hipStream_t s) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_fill<T>, LIRA_GRID1(count), 0, s, y, count, (T)a);
+    LIRA_HIP(hipGetLastError());
+}
+
+template <typename T> __global__ void k_axpy(T *y, const T *x, long n, T a) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i < n) y[i] += a * x[i];
+}
+template <typename T> void axpy(T *y, const T *x, long count, double a, hipStream_t s) {
+    hipLaunchKernelGGL(k_axpy<T>, LIRA_GRID1(count), 0, s, y, x, count, (T)a);
+    LIRA_HIP(hipGetLastError());
+}
+
+template <typename T>
+__global__ void k_residual(T *r, const T *hx, const T *x, const double *theta, int nb, long N) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    const int b = blockIdx.y;
+    if (i >= N) return;
+    const long id = (long)b * N + i;
+    r[id] = (T)((double)hx[id] - theta[b] * (double)x[id]);
+}
+template <typename T> void residual(T *r, const T *hx, const T *x, const double *theta, int nb, long N, hipStream_t s) {
+    if (nb <= 0) return;
+    hipLaunchKernelGGL(k_residual<T>, dim3((unsigned)((N + 255) / 256), nb), dim3(256), 0, s, r, hx, x, theta, nb, N);
+    LIRA_HIP(hipGetLastError());
+}
+
+template <typename T>
+__global__ void k_precond_init(T *w, const T *r, const T *v, const double *lam, T diag_t, int nb, long N) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    const int b = blockIdx.y;
+    if (i >= N) return;
+    T d = diag_t + v[i] - (T)lam[b];
+    d = d > T(0.1) ? d : T(0.1);
+    w[(long)b * N + i] = r[(long)b * N + i] / d;
+}
+template <typename T>
+void precond_init(T *w, const T *r, const T *v, const double *lam, double diag_t, int nb, long N, hipStream_t s) {
+    if (nb <= 0) return;
+    hipLaunchKernelGGL(k_precond_init<T>, dim3((unsigned)((N + 255) / 256), nb), dim3(256), 0, s, w, r, v, lam,
+                       (T)diag_t, nb, N);
+    LIRA_HIP(hipGetLastError());
+}
+
+template <typename T> __global__ void k_density(T *rho, const T *psi, const double *f, int nb, long N) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= N) return;
+    double acc = 0.0;
+    for (int b = 0; b < nb; b++) {
+        const double fb = f[b];
+        if (fb == 0.0) continue;
+        const double p = (double)psi[(long)b * N + i];
+        acc += fb * p * p;
+    }
+    rho[i] = (T)acc;
+}
+template <typename T> void density(T *rho, const T *psi, const double *f, int nb, long N, hipStream_t s) {
+    hipLaunchKernelGGL(k_density<T>, LIRA_GRID1(N), 0, s, rho, psi, f, nb, N);
+    LIRA_HIP(hipGetLastError());
+}
+
+template <typename T> __global__ void k_sum3(T *o, const T *a, const T *b, const T *c, long N) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i < N) o[i] = (T)(((double)a[i] + (double)b[i]) + (double)c[i]);
+}
+template <typename T> void sum3(T *out, const T *a, const T *b, const T *c, long N, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum3<T>, LIRA_GRID1(N), 0, s, out, a, b, c, N);
+    LIRA_HIP(hipGetLastError());
+}
+template <typename T> __global__ void k_add2(T *o, const T *a, const T *b, long N) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i < N) o[i] = a[i] + b[i];
+}
+template <typename T> void add2(T *out, const T *a, const T *b, long N, hipStream_t s) {
+    hipLaunchKernelGGL(k_add2<T>, LIRA_GRID1(N), 0, s, out, a, b, N);
+    LIRA_HIP(hipGetLastError());
+}
+template <typename T> __global__ void k_sub2(T *o, const T *a, const T *b, long N) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i < N) o[i] = a[i] - b[i];
+}
+template <typename T> void sub2(T *out, const T *a, const T *b, long N, hipStream_t s) {
+    hipLaunchKernelGGL(k_sub2<T>, LIRA_GRID1(N), 0, s, out, a, b, N);
+    LIRA_HIP(hipGetLastError());
+}
+template <typename T> void copy(T *out, const T *a, long N, hipStream_t s) {
+    LIRA_HIP(hipMemcpyAsync(out, a, (size_t)N * sizeof(T), hipMemcpyDeviceToDevice, s));
+}
+template <typename Td, typename Ts> __global__ void k_convert(Td *o, const Ts *a, long N) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i < N) o[i] = (Td)a[i];
+}
+template <typename Td, typename Ts> void convert(Td *out, const Ts *a, long N, hipStream_t s) {
+    hipLaunchKernelGGL((k_convert<Td, Ts>), LIRA_GRID1(N), 0, s, out, a, N);
+    LIRA_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------------- LSDA ----
+// Dirac exchange + PW92 correlation (qm/src/xc.rs:78-187), evaluated in f64 at every point.
+__device__ inline void pw92_g(double rs, double a, double a1, double b1, double b2, double b3, double b4, double &g,
+                              double &dg) {
+    const double srs = sqrt(rs);
+    const double q0 = -2.0 * a * (1.0 + a1 * rs);
+    const double q1 = 2.0 * a * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs);
+    const double q1p = a * (b1 / srs + 2.0 * b2 + 3.0 * b3 * srs + 4.0 * b4 * rs);
+    const double ln = log(1.0 + 1.0 / q1);
+    g = q0 * ln;
+    dg = -2.0 * a * a1 * ln - q0 * q1p / (q1 * q1 + q1);
+}
+
+__device__ inline void exc_point(double nu, double nd, double &e, double &vu, double &vd) {
+    nu = nu > 0.0 ? nu : 0.0;
+    nd = nd > 0.0 ? nd : 0.0;
+    const double n = nu + nd;
+    if (n < 1e-20) {
+        e = vu = vd = 0.0;
+        return;
+    }
+    const double third = 1.0 / 3.0;
+    const double cx = 0.75 * cbrt(3.0 / M_PI) * cbrt(2.0);
+    const double cnu = cbrt(nu), cnd = cbrt(nd);
+    const double ex = -cx * (nu * cnu + nd * cnd);
+    const double vxu = -cbrt(6.0 / M_PI) * cnu, vxd = -cbrt(6.0 / M_PI) * cnd;
+    // correlation
+    const double rs = cbrt(3.0 / (4.0 * M_PI * n));
+    double z = (nu - nd) / n;
+    z = z < -1.0 ? -1.0 : (z > 1.0 ? 1.0 : z);
+    double ec0, dec0, ec1, dec1, mac, dmac;
+    pw92_g(rs, 0.031091, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294, ec0, dec0);
+    pw92_g(rs, 0.015545, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517, ec1, dec1);
+    pw92_g(rs, 0.016887, 0.11125, 10.357, 3.6231, 0.88026, 0.49671, mac, dmac);
+    const double ac = -mac, dac = -dmac;
+    const double denom = cbrt(16.0) - 2.0;  // 2^{4/3} - 2
+    const double opz = cbrt(1.0 + z), omz = cbrt(1.0 - z);
+    const double f = ((1.0 + z) * opz + (1.0 - z) * omz - 2.0) / denom;
+    const double fp = (4.0 * third) * (opz - omz) / denom;
+    const double FPP0 = 1.709921;
+    const double z3 = z * z * z, z4 = z3 * z;
+    const double ec = ec0 + ac * f / FPP0 * (1.0 - z4) + (ec1 - ec0) * f * z4;
+    const double decdrs = dec0 + dac * f / FPP0 * (1.0 - z4) + (dec1 - dec0) * f * z4;
+    const double decdz = ac / FPP0 * (fp * (1.0 - z4) - 4.0 * z3 * f) + (ec1 - ec0) * (fp * z4 + 4.0 * z3 * f);
+    const double common = ec - rs * third * decdrs;
+    e = ex + n * ec;
+    vu = vxu + common - (z - 1.0) * decdz;
+    vd = vxd + common - (z + 1.0) * decdz;
+}
+
+template <typename T> __global__ void k_xc(const T *nu, const T *nd, T *e, T *vu, T *vd, long N) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= N) return;
+    double ee, a, b;
+    exc_point((double)nu[i], (double)nd[i], ee, a, b);
+    e[i] = (T)ee;
+    vu[i] = (T)a;
+    vd[i] = (T)b;
+}
+template <typename T> void xc_lsda(const T *nu, const T *nd, T *e, T *vu, T *vd, long N, hipStream_t s) {
+    hipLaunchKernelGGL(k_xc<T>, LIRA_GRID1(N), 0, s, nu, nd, e, vu, vd, N);
+    LIRA_HIP(hipGetLastError());
+}
+
+// --------------------------------------------------------------------------------------------- nuclei ----
+template <typename T>
+__global__ void k_nuclear(T *out, int n, double h, const double *pos, const double *zf, int nat, double sigma) {
+    const long N = (long)n * n * n;
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= N) return;
+    const long nn = (long)n * n;
+    const long k = i / nn, rem = i - k * nn, j = rem / n, ii = rem - j * n;
+    const double o = -((double)n - 1.0) * h / 2.0;
+    const double x = o + (double)ii * h, y = o + (double)j * h, z = o + (double)k * h;
+    const double s2 = sigma * sqrt(2.0), on_site = sqrt(2.0 / M_PI) / sigma;
+    double acc = 0.0;
+    for (int a = 0; a < nat; a++) {
+        const double dx = x - pos[3 * a], dy = y - pos[3 * a + 1], dz = z - pos[3 * a + 2];
+        const double d = sqrt(dx * dx + dy * dy + dz * dz);
+        acc += d < 1e-10 ? -zf[a] * on_site : -zf[a] * erf(d / s2) / d;
+    }
+    out[i] = (T)acc;
+}
+template <typename T>
+void nuclear_potential(T *out, int n, double h, const double *pos, const double *zf, int nat, double sigma,
+                       hipStream_t s) {
+    const long N = (long)n * n * n;
+    hipLaunchKernelGGL(k_nuclear<T>, LIRA_GRID1(N), 0, s, out, n, h, pos, zf, nat, sigma);
+    LIRA_HIP(hipGetLastError());
+}
+
+template <typename T>
+__global__ void k_deposit(T *out, int n, double h, const double *pos, const double *zf, int nat, double sigma,
+                          double scale) {
+    const long N = (long)n * n * n;
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= N) return;
+    const long nn = (long)n * n;
+    const long k = i / nn, rem = i - k * nn, j = rem / n, ii = rem - j * n;
+    const double o = -((double)n - 1.0) * h / 2.0;
+    const double x = o + (double)ii * h, y = o + (double)j * h, z = o + (double)k * h;
+    const double norm = 1.0 / pow(2.0 * M_PI * sigma * sigma, 1.5), inv = 1.0 / (2.0 * sigma * sigma);
+    double acc = 0.0;
+    for (int a = 0; a < nat; a++) {
+        const double dx = x - pos[3 * a], dy = y - pos[3 * a + 1], dz = z - pos[3 * a + 2];
+        acc += zf[a] * norm * exp(-(dx * dx + dy * dy + dz * dz) * inv);
+    }
+    out[i] = (T)(acc * scale);
+}
+template <typename T>
+void deposit_charge(T *out, int n, double h, const double *pos, const double *zf, int nat, double sigma, double scale,
+                    hipStream_t s) {
+    const long N = (long)n * n * n;
+    hipLaunchKernelGGL(k_deposit<T>, LIRA_GRID1(N), 0, s, out, n, h, pos, zf, nat, sigma, scale);
+    LIRA_HIP(hipGetLastError());
+}
+
+// The LCG x -> a x + c (mod 2^64) advanced t steps: (A_t, C_t) with x_t = A_t x_0 + C_t, by binary powering.
+__device__ inline void lcg_jump(uint64_t t, uint64_t &A, uint64_t &Cc) {
+    uint64_t a = 6364136223846793005ULL, c = 1442695040888963407ULL;
+    A = 1;
+    Cc = 0;
+    while (t) {
+        if (t & 1) {
+            A = A * a;
+            Cc = Cc * a + c;
+        }
+        c = c * a + c;
+        a = a * a;
+        t >>= 1;
+    }
+}
+
+constexpr int IO_RUN = 64;
+template <typename T>
+__global__ void k_initial(T *out, int n, double h, const double *pos, int nat, int k, uint64_t seed0) {
+    const long N = (long)n * n * n;
+    const long total = N * k;
+    const long e0 = (blockIdx.x * 256L + threadIdx.x) * IO_RUN;
+    if (e0 >= total) return;
+    uint64_t A, Cc;
+    lcg_jump((uint64_t)e0, A, Cc);
+    uint64_t seed = A * seed0 + Cc;
+    const double o = -((double)n - 1.0) * h / 2.0;
+    const long nn = (long)n * n;
+    for (long e = e0; e < min(total, e0 + IO_RUN); e++) {
+        seed = seed * 6364136223846793005ULL + 1442695040888963407ULL;
+        const double u = (double)(seed >> 11) / 9007199254740992.0;
+        const long i = e % N;
+        const long kk = i / nn, rem = i - kk * nn, j = rem / n, ii = rem - j * n;
+        const double x = o + (double)ii * h, y = o + (double)j * h, z = o + (double)kk * h;
+        double env = 0.0;
+        for (int a = 0; a < nat; a++) {
+            const double dx = x - pos[3 * a], dy = y - pos[3 * a + 1], dz = z - pos[3 * a + 2];
+            env += exp(-(dx * dx + dy * dy + dz * dz) / 4.0);
+        }
+        out[e] = (T)((u + 0.5) * env);
+    }
+}
+template <typename T>
+void initial_orbitals(T *out, int n, double h, const double *pos, int nat, int k, uint64_t seed0, hipStream_t s) {
+    const long total = (long)n * n * n * k;
+    const long threads = (total + IO_RUN - 1) / IO_RUN;
+    hipLaunchKernelGGL(k_initial<T>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, out, n, h, pos, nat, k,
+                       seed0);
+    LIRA_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------------- DIIS ----
+constexpr int DIIS_MAX = 16;
+template <typename T> struct PtrList {
+    const T *p[DIIS_MAX];
+};
+template <typename T>
+__global__ void k_diis_mix(T *out, PtrList<T> rho, PtrList<T> res, const double *c, int m, double alpha, long N) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= N) return;
+    double acc = 0.0;
+    for (int q = 0; q < m; q++) acc += c[q] * ((double)rho.p[q][i] + alpha * (double)res.p[q][i]);
+    out[i] = (T)(acc > 0.0 ? acc : 0.0);
+}
+template <typename T>
+void diis_mix(T *out, const T *const *rho, const T *const *res, const double *c, int m, double alpha, long N,
+              hipStream_t s) {
+    if (m > DIIS_MAX) throw InvalidError("diis_mix: history too long");
+    PtrList<T> a{}, b{};
+    for (int q = 0; q < m; q++) {
+        a.p[q] = rho[q];
+        b.p[q] = res[q];
+    }
+    hipLaunchKernelGGL(k_diis_mix<T>, LIRA_GRID1(N), 0, s, out, a, b, c, m, alpha, N);
+    LIRA_HIP(hipGetLastError());
+}
+
+// -------------------------------------------------------------------------------------------- forces ----
+template <typename T>
+__global__ __launch_bounds__(256) void k_hf_forces(const T *rho, int n, double h, const double *pos, const double *zf,
+                                                   double sigma, double *partials) {
+    __shared__ double sh[256];
+    const int I = blockIdx.y;
+    const long N = (long)n * n * n, nn = (long)n * n;
+    const double o = -((double)n - 1.0) * h / 2.0;
+    const double a = sigma * sqrt(2.0);
+    const double px = pos[3 * I], py = pos[3 * I + 1], pz = pos[3 * I + 2], z = zf[I];
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (long p = blockIdx.x * 256L + threadIdx.x; p < N; p += (long)gridDim.x * 256) {
+        const long k = p / nn, rem = p - k * nn, j = rem / n, i = rem - j * n;
+        const double dx = o + (double)i * h - px, dy = o + (double)j * h - py, dz = o + (double)k * h - pz;
+        const double d = sqrt(dx * dx + dy * dy + dz * dz);
+        if (d < 1e-8) continue;
+        const double u = d / a;
+        const double gp = ((2.0 / sqrt(M_PI)) * u * exp(-u * u) - erf(u)) / (d * d);
+        const double w = (double)rho[p] * (-z * gp / d);
+        f0 += w * dx;
+        f1 += w * dy;
+        f2 += w * dz;
+    }
+    const double r0 = block_sum256(f0, sh), r1 = block_sum256(f1, sh), r2 = block_sum256(f2, sh);
+    if (threadIdx.x == 0) {
+        partials[((long)I * 3 + 0) * RED_BLOCKS + blockIdx.x] = r0;
+        partials[((long)I * 3 + 1) * RED_BLOCKS + blockIdx.x] = r1;
+        partials[((long)I * 3 + 2) * RED_BLOCKS + blockIdx.x] = r2;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sum_partials(const double *partials, double *out, double scale) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (int p = threadIdx.x; p < RED_BLOCKS; p += 256) acc += partials[(long)blockIdx.x * RED_BLOCKS + p];
+    const double r = block_sum256(acc, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = r * scale;
+}
+
+template <typename T>
+void hf_forces(const T *rho, int n, double h, const double *pos, const double *zf, int nat, double sigma,
+               double *partials, double *out, hipStream_t s) {
+    if (nat <= 0) return;
+    hipLaunchKernelGGL(k_hf_forces<T>, dim3(RED_BLOCKS, nat), dim3(256), 0, s, rho, n, h, pos, zf, sigma, partials);
+    LIRA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_partials, dim3(3 * nat), dim3(256), 0, s, partials, out, h * h * h);
+    LIRA_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------------- explicit instances ----
+#define LIRA_INST(T)                                                                                                  \
+    template void stencil<T>(int, const StencilParams &, const T *, T *, const T *, const T *, const double *,       \
+                             hipStream_t, int);                                                                       \
+    template void reduce_jobs<T>(const RedJobs<T> &, int, double, double *, double *, hipStream_t);                 \
+    template void gram<T>(const Cols &, const Cols &, long, double, double *, size_t, double *, hipStream_t);       \
+    template void lincomb<T>(const Cols &, const double *, int, T *, long, hipStream_t, int, T *);                             \
+    template void fill<T>(T *, long, double, hipStream_t);                                                           \
+    template void axpy<T>(T *, const T *, long, double, hipStream_t);                                                \
+    template void residual<T>(T *, const T *, const T *, const double *, int, long, hipStream_t);                    \
+    template void precond_init<T>(T *, const T *, const T *, const double *, double, int, long, hipStream_t);        \
+    template void density<T>(T *, const T *, const double *, int, long, hipStream_t);                                \
+    template void sum3<T>(T *, const T *, const T *, const T *, long, hipStream_t);                                  \
+    template void add2<T>(T *, const T *, const T *, long, hipStream_t);                                             \
+    template void sub2<T>(T *, const T *, const T *, long, hipStream_t);                                             \
+    template void copy<T>(T *, const T *, long, hipStream_t);                                                        \
+    template void xc_lsda<T>(const T *, const T *, T *, T *, T *, long, hipStream_t);                                \
+    template void nuclear_potential<T>(T *, int, double, const double *, const double *, int, double, hipStream_t); \
+    template void deposit_charge<T>(T *, int, double, const double *, const double *, int, double, double,          \
+                                    hipStream_t);                                                                     \
+    template void initial_orbitals<T>(T *, int, double, const double *, int, int, uint64_t, hipStream_t);            \
+    template void diis_mix<T>(T *, const T *const *, const T *const *, const double *, int, double, long,            \
+                              hipStream_t);                                                                           \
+    template void hf_forces<T>(const T *, int, double, const double *, const double *, int, double, double *,        \
+                               double *, hipStream_t);
+LIRA_INST(float)
+LIRA_INST(double)
+template void convert<float, double>(float *, const double *, long, hipStream_t);
+template void convert<double, float>(double *, const float *, long, hipStream_t);
+template void convert<double, double>(double *, const double *, long, hipStream_t);
+template void convert<float, float>(float *, const float *, long, hipStream_t);
+
+}  // namespace lira

@@ -1,0 +1,180 @@
+// Internal declarations of liblira: device buffers, launch wrappers of the gfx950 kernels, and the host-side
+// numerics shared by the Poisson, LOBPCG and SCF drivers. Nothing here is part of the C ABI (include/lira.h).
+//
+// Layout in HBM: every grid field is a dense cube n^3 in the reference's qm layout idx = i + n(j + n k)
+// (qm/src/grid.rs:16-17), zero outside the box (Dirichlet). A block of nb bands is nb such fields back to back
+// (band-major), so band b of a block starts at b * N.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace lira {
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct NumericalError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct InvalidError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void check_hip(hipError_t e, const char *what);
+#define LIRA_HIP(call) ::lira::check_hip((call), #call)
+
+// A device allocation owned by the solver (hipMalloc'd, freed on destruction).
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t b) { alloc(b); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); ptr = o.ptr; bytes = o.bytes; o.ptr = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t b);
+    void release();
+    template <typename T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// ------------------------------------------------------------------ reductions ----
+// Deterministic two-level sums: a fixed number of workgroup partials in f64, then one fixed-order finish.
+constexpr int RED_BLOCKS = 512;
+constexpr int MAX_JOBS = 24;
+
+enum RedKind : int { RED_DOT = 0, RED_SUM = 1, RED_ABSDIFF = 2, RED_FACEMAX = 3, RED_MAX_ABS = 4 };
+
+template <typename T> struct RedJobs {
+    const T *a[MAX_JOBS];
+    const T *b[MAX_JOBS];
+    int kind[MAX_JOBS];
+    int count;
+};
+
+// out[j] = scale * reduction j over N points (FACEMAX: max |a| on the 6 faces of the n^3 cube; MAX_ABS: max |a|).
+// partials: RED_BLOCKS * MAX_JOBS doubles of scratch. Results stay on the device (out).
+template <typename T>
+void reduce_jobs(const RedJobs<T> &jobs, int n, double scale, double *partials, double *out, hipStream_t s);
+
+// G[i * q + j] = scale * sum_p S_i[p] * U_j[p] for i < m, j < q; S columns are segments of band-major blocks.
+struct Seg {
+    const void *base;
+    int count;
+};
+constexpr int MAX_SEGS = 4;
+struct Cols {
+    Seg seg[MAX_SEGS];
+    int nseg;
+    int total() const { int t = 0; for (int i = 0; i < nseg; i++) t += seg[i].count; return t; }
+};
+template <typename T>
+void gram(const Cols &S, const Cols &U, long N, double scale, double *partials, size_t partial_cap, double *G,
+          hipStream_t s);
+size_t gram_partials_needed(int m, int q, long N);
+
+// Y_j = sum_i S_i * C[i * q + j] (C f64 on the device, m x q), accumulated in f64. Outputs j < qsplit go to
+// Y + j N, the others to Y2 + (j - qsplit) N.
+template <typename T>
+void lincomb(const Cols &S, const double *C, int q, T *Y, long N, hipStream_t s, int qsplit = -1, T *Y2 = nullptr);
+
+// ------------------------------------------------------------------- stencil ----
+enum StencilMode : int { ST_LAP = 0, ST_HAM = 1, ST_JAC = 2 };
+struct StencilParams {
+    int n, nb;
+    double h;
+    double scale;   // ST_LAP: out = scale * lap x
+    double omega;   // ST_JAC damping
+    double diag_t;  // ST_JAC: diagonal of -1/2 lap, -1.5 c0 / h^2
+};
+// ST_HAM: out_b = -1/2 lap x_b + v x_b. ST_JAC: out_b = x_b + omega/max(diag_t + v - lam_b, 0.1) (r_b - (H - lam_b) x_b).
+template <typename T>
+void stencil(int mode, const StencilParams &p, const T *x, T *out, const T *v, const T *r, const double *lam,
+             hipStream_t s, int variant = -1);
+int stencil_variants();
+const char *stencil_variant_name(int v);
+void stencil_set_default_variant(int dtype_size, int variant);
+
+// -------------------------------------------------------------- pointwise ----
+template <typename T> void fill(T *y, long count, double a, hipStream_t s);
+template <typename T> void axpy(T *y, const T *x, long count, double a, hipStream_t s);  // y += a x
+// r_b = hx_b - theta_b x_b, theta on the device (f64)
+template <typename T> void residual(T *r, const T *hx, const T *x, const double *theta, int nb, long N, hipStream_t s);
+// w_b = r_b / max(diag_t + v - lam_b, 0.1)
+template <typename T>
+void precond_init(T *w, const T *r, const T *v, const double *lam, double diag_t, int nb, long N, hipStream_t s);
+// rho = sum_b f_b psi_b^2 (f on the device, f64)
+template <typename T> void density(T *rho, const T *psi, const double *f, int nb, long N, hipStream_t s);
+// v_sigma = vnuc + vh + vxc_sigma
+template <typename T> void sum3(T *out, const T *a, const T *b, const T *c, long N, hipStream_t s);
+template <typename T> void add2(T *out, const T *a, const T *b, long N, hipStream_t s);
+template <typename T> void sub2(T *out, const T *a, const T *b, long N, hipStream_t s);
+template <typename T> void copy(T *out, const T *a, long N, hipStream_t s);
+template <typename Td, typename Ts> void convert(Td *out, const Ts *a, long N, hipStream_t s);
+// LSDA (Dirac + PW92), math in f64 (qm/src/xc.rs:120-210)
+template <typename T>
+void xc_lsda(const T *nu, const T *nd, T *e, T *vu, T *vd, long N, hipStream_t s);
+// V = -sum_I Z_I erf(d/(sigma sqrt2))/d (qm/src/nuclear.rs:258-273); pos/z on the device
+template <typename T>
+void nuclear_potential(T *out, int n, double h, const double *pos, const double *zf, int nat, double sigma,
+                       hipStream_t s);
+template <typename T>
+void deposit_charge(T *out, int n, double h, const double *pos, const double *zf, int nat, double sigma, double scale,
+                    hipStream_t s);
+// the reference's LCG orbital guess (qm/src/scf.rs:626-634), generated on the device by jump-ahead
+template <typename T>
+void initial_orbitals(T *out, int n, double h, const double *pos, int nat, int k, uint64_t seed0, hipStream_t s);
+// DIIS mix: out = max(0, sum_i c_i (rho_i + alpha r_i)) with c on the device
+template <typename T>
+void diis_mix(T *out, const T *const *rho, const T *const *res, const double *c, int m, double alpha, long N,
+              hipStream_t s);
+// HF force partials: per nucleus I and axis a, sum_p rho_p (-Z g'(d)/d) (r_p - R_I)_a (qm/src/forces.rs:189-236)
+template <typename T>
+void hf_forces(const T *rho, int n, double h, const double *pos, const double *zf, int nat, double sigma,
+               double *partials, double *out, hipStream_t s);
+
+// --------------------------------------------------------------- Poisson ----
+// Q_lm = dv * sum_p rho_p R_lm(r_p) for the 49 real solid harmonics l <= 6 (qm/src/poisson.rs:97-162)
+template <typename T> void multipole_moments(const T *rho, int n, double h, double *partials, double *q, hipStream_t s);
+// b = 4 pi rho + L_bc (qm/src/poisson.rs:175-214) with the moments q on the device
+template <typename T> void poisson_rhs(double *b, const T *rho, const double *q, int n, double h, hipStream_t s);
+// multigrid level operators (7-point A2 with boundary-distance ghost factors)
+template <typename T>
+void mg_jacobi(T *dst, const T *x, const T *b, int n, double h2, double omega, double g_low, double g_high,
+               hipStream_t s);
+template <typename T>
+void mg_residual(T *dst, const T *x, const T *b, int n, double h2, double g_low, double g_high, hipStream_t s);
+template <typename T> void mg_restrict(T *dst, const T *fine, int nc, int nf, hipStream_t s);
+template <typename T> void mg_prolong_add(T *fine, const T *coarse, int nf, int nc, hipStream_t s);
+// device-side CG scalar updates
+// x += (num/den) p ; r -= (num/den) q    (alpha = sc[ia]/sc[ib])
+template <typename Tv, typename Tr>
+void cg_update(Tv *x, const Tr *p, Tr *r, const Tr *q, const double *sc, int ia, int ib, long N, hipStream_t s);
+// p = z + (sc[ia]/sc[ib]) p
+template <typename T> void cg_direction(T *p, const T *z, const double *sc, int ia, int ib, long N, hipStream_t s);
+// r = b + lap(v) computed outside; r64 -> r32 etc. via convert
+// x += e (f64 += T)
+template <typename T> void add_into_f64(double *x, const T *e, long N, hipStream_t s);
+
+// ------------------------------------------------------------- host utils ----
+void sym_eigen(int m, std::vector<double> &a, std::vector<double> &evals, std::vector<double> &evecs);
+bool lu_solve(int m, std::vector<double> a, std::vector<double> &b);
+bool cholesky(int m, std::vector<double> &a);  // in place, lower; false if not SPD
+void chol_solve(int m, const std::vector<double> &l, std::vector<double> &b, int nrhs);
+
+// generalised Rayleigh-Ritz on (A, B) m x m, lowest nk pairs; C is m x nk (row-major), returns the amplification
+// alpha = max_i ||T y_i||. Throws NumericalError on rank loss.
+double rayleigh_ritz(int m, const std::vector<double> &A, const std::vector<double> &B, int nk, double drop,
+                     std::vector<double> &theta, std::vector<double> &C);
+
+}  // namespace lira

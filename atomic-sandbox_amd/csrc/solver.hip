@@ -1,0 +1,1532 @@
+// liblira host driver: the Poisson MG-PCG solve, LOBPCG, the Kohn–Sham SCF loop with DIIS and Fermi–Dirac
+// occupations, energies and Hellmann–Feynman forces, and the C ABI of include/lira.h.
+//
+// Precision: Solver<double> keeps every grid field in f64 (LIRA_FP64); Solver<float> keeps the grid fields in f32
+// (stencil applies, V-cycles, block products) while V_H, the Poisson right-hand side and residual corrections, every
+// reduction, the Rayleigh–Ritz and Cholesky steps, DIIS coefficients and the energy sums are f64 (LIRA_MIXED).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <numeric>
+
+#include "lira.h"
+#include "lira_internal.hpp"
+
+namespace lira {
+
+static const double COEFF0 = -205.0 / 72.0;
+static const double COEFF[5] = {-205.0 / 72.0, 8.0 / 5.0, -1.0 / 5.0, 8.0 / 315.0, -1.0 / 560.0};
+
+// qm/src/eigen.rs:218-222: omega = 4 / (3 rho), rho = symbol(pi) / c0
+static double jacobi_omega() {
+    const double symbol_pi = COEFF[0] + 2.0 * (-COEFF[1] + COEFF[2] - COEFF[3] + COEFF[4]);
+    return 4.0 / (3.0 * (symbol_pi / COEFF[0]));
+}
+
+// f32 residual floor of ||(H - lambda) x|| for ||x|| = 1 (crates/qm-gpu/src/eigen.rs:313-318): the RMS rounding of a
+// stored f32 vector (2.4e-8) filtered by the kinetic symbol.
+static double residual_floor_f32(double h) {
+    const double s = 1.0 / (2.0 * h * h);
+    const double mean = 3.0 * std::fabs(COEFF[0]) * s;
+    double c2 = 0.0;
+    for (int i = 1; i < 5; i++) c2 += COEFF[i] * COEFF[i];
+    const double var = 3.0 * 2.0 * c2 * s * s;
+    return 2.4e-8 * std::sqrt(mean * mean + var);
+}
+
+using Clock = std::chrono::steady_clock;
+static double secs(Clock::time_point a) { return std::chrono::duration<double>(Clock::now() - a).count(); }
+
+// Multigrid constants (crates/qm-gpu/src/multigrid.rs:70-78): V(2,2), 12 sweeps on the coarsest level (n <= 2),
+// omega = 6/7 for the 7-point operator.
+constexpr int MG_PRE = 2, MG_POST = 2, MG_COARSE = 12, MG_COARSEST_MAX = 2;
+constexpr double MG_OMEGA = 6.0 / 7.0;
+// LOBPCG constants (crates/qm-gpu/src/eigen.rs:231-262)
+constexpr double RR_DROP = 1e-6;
+constexpr double ALPHA_MAX = 2.0;
+constexpr double ALPHA_RESIDUAL_FRACTION = 0.1;
+// SCF constants (crates/qm-gpu/src/scf.rs:357-386; crates/qm/src/scf.rs:368-704)
+constexpr double EIG_TOL_FLOOR_FACTOR = 3.0;
+constexpr double EIG_TOL_FRACTION = 0.0015;
+constexpr double EIG_TOL_MAX = 1e-3;
+constexpr double DIIS_REG = 1e-12;
+constexpr int MAX_WITNESS_GROWTH = 32;
+constexpr double WITNESS_GAP = 0.1, WITNESS_TAIL_KT = 36.0, WITNESS_GUARD_TOL = 1e-3, WITNESS_RESIDUAL_FACTOR = 1.5;
+constexpr double STALL_LEVEL_PER_ELECTRON = 1e-3;
+
+// ------------------------------------------------------------------------------------- occupations ----
+static double occupation(double x) {
+    if (x >= 0.0) {
+        const double e = std::exp(-x);
+        return e / (1.0 + e);
+    }
+    return 1.0 / (1.0 + std::exp(x));
+}
+static double neumaier(const std::vector<double> &v) {
+    double sum = 0.0, comp = 0.0;
+    for (double x : v) {
+        const double t = sum + x;
+        if (std::fabs(sum) >= std::fabs(x)) comp += (sum - t) + x;
+        else comp += (x - t) + sum;
+        sum = t;
+    }
+    return sum + comp;
+}
+static double fd_entropy(double x) {
+    const double a = std::fabs(x);
+    if (std::isinf(a)) return 0.0;
+    return std::log1p(std::exp(-a)) + a / (1.0 + std::exp(a));
+}
+struct Fd {
+    std::vector<double> f;
+    double mu = NAN, entropy = 0.0;
+};
+// qm/src/occupations.rs:166-213: mu by two tie-directed bisections, reported at their midpoint
+static Fd fermi_dirac(const std::vector<double> &ev, int n, double kt) {
+    Fd r;
+    const int m = (int)ev.size();
+    r.f.assign(m, 0.0);
+    if (n == 0) return r;
+    if (m < n + 1) throw NumericalError("Fermi-Dirac needs at least n + 1 levels");
+    for (double e : ev)
+        if (!std::isfinite(e)) throw NumericalError("non-finite eigenvalue in the Fermi-Dirac occupations");
+    if (!(std::isfinite(kt) && kt > 0.0)) throw NumericalError("Fermi-Dirac kT is not finite and > 0");
+    auto total = [&](double mu) {
+        std::vector<double> f(m);
+        for (int i = 0; i < m; i++) f[i] = occupation((ev[i] - mu) / kt);
+        return neumaier(f);
+    };
+    const double nf = n;
+    double lo = *std::min_element(ev.begin(), ev.end()) - 50.0 * kt;
+    double hi = *std::max_element(ev.begin(), ev.end()) + 50.0 * kt;
+    if (!(total(lo) < nf && total(hi) > nf)) throw NumericalError("the Fermi-Dirac mu bracket does not bracket n");
+    auto bisect = [&](double a, double b, bool ge) {
+        for (int hv = 0; hv < 2048; hv++) {
+            const double mid = 0.5 * a + 0.5 * b;
+            if (mid <= a || mid >= b) break;
+            const double d = total(mid) - nf;
+            if (ge ? d >= 0.0 : d > 0.0) b = mid;
+            else a = mid;
+        }
+        return ge ? b : a;
+    };
+    const double mu = 0.5 * bisect(lo, hi, true) + 0.5 * bisect(lo, hi, false);
+    std::vector<double> s(m);
+    for (int i = 0; i < m; i++) {
+        r.f[i] = occupation((ev[i] - mu) / kt);
+        s[i] = fd_entropy((ev[i] - mu) / kt);
+    }
+    r.mu = mu;
+    r.entropy = neumaier(s);
+    return r;
+}
+static std::vector<int> ascending(const std::vector<double> &ev) {
+    std::vector<int> o(ev.size());
+    std::iota(o.begin(), o.end(), 0);
+    std::stable_sort(o.begin(), o.end(), [&](int a, int b) { return ev[a] < ev[b]; });
+    return o;
+}
+// qm/src/occupations.rs:244-257
+static bool witness_bound_reached(const std::vector<double> &ev, const std::vector<double> &res, int n, double mu,
+                                  double kt) {
+    const int m = (int)ev.size();
+    if (n == 0) return true;
+    if (m <= n) return false;
+    auto o = ascending(ev);
+    const double eps_n = ev[o[n - 1]];
+    const int top = o[m - 1];
+    const double lower = ev[top] - res[top];
+    return lower >= eps_n + WITNESS_GAP && lower - mu >= WITNESS_TAIL_KT * kt;
+}
+// qm/src/occupations.rs:322-340
+static int witness_keep(const std::vector<double> &ev, const std::vector<double> &res, int n, double mu, double kt) {
+    const int m = (int)ev.size();
+    if (n == 0 || m <= n) return m;
+    auto o = ascending(ev);
+    const double eps_n = ev[o[n - 1]];
+    auto passes = [&](int rank) {
+        const double lower = ev[o[rank]] - res[o[rank]];
+        return lower >= eps_n + WITNESS_GAP && lower - mu >= WITNESS_TAIL_KT * kt;
+    };
+    for (int t = n; t < m; t++)
+        if (passes(t)) return (t + 1 < m && passes(t + 1)) ? t + 2 : t + 1;
+    return m;
+}
+
+// ------------------------------------------------------------------------------------ solver state ----
+struct PoissonOut {
+    int iterations = 0, outer = 0;
+    double residual = 0.0, seconds = 0.0;
+};
+struct EigenOut {
+    int iterations = 0, fallbacks = 0;
+    double max_residual = 0.0;
+};
+
+struct SolverIface {
+    virtual ~SolverIface() = default;
+    virtual lira_status solve(const lira_system *sys, const lira_settings *st, int warm, lira_abort_fn abort,
+                              void *user, lira_scf_stats *out) = 0;
+    virtual void forces(const lira_nucleus *nuc, int nat, double sigma, double *out) = 0;
+    virtual void get_field(int which, double *out) = 0;
+    virtual void get_levels(int spin, double *evals, double *occ, int cap, int *count) = 0;
+    virtual void get_orbitals(int spin, double *out, int cap, int *count) = 0;
+    virtual PoissonOut poisson_host(const double *rho, double *v, double tol, int max_iter) = 0;
+    virtual EigenOut lobpcg_host(const double *v, int k, double *x, double *evals, double lam, double tol,
+                                 int max_iter) = 0;
+    virtual double effective_eig_tol(double requested) const = 0;
+    virtual void reset() = 0;
+    virtual int precision() const = 0;
+};
+
+template <typename T> struct Channel {
+    int kcap = 0, k = 0;
+    DevBuf X, HX, R, W, HW, P, HP, XN, PN, TMP;
+    std::vector<double> evals, occ, res;
+};
+
+template <typename T> class Solver final : public SolverIface {
+  public:
+    Solver(int n, double h);
+    ~Solver() override {
+        if (hbuf_) (void)hipHostFree(hbuf_);
+        if (stream_) (void)hipStreamDestroy(stream_);
+    }
+    lira_status solve(const lira_system *sys, const lira_settings *st, int warm, lira_abort_fn abort, void *user,
+                      lira_scf_stats *out) override;
+    void forces(const lira_nucleus *nuc, int nat, double sigma, double *out) override;
+    void get_field(int which, double *out) override;
+    void get_levels(int spin, double *evals, double *occ, int cap, int *count) override;
+    void get_orbitals(int spin, double *out, int cap, int *count) override;
+    PoissonOut poisson_host(const double *rho, double *v, double tol, int max_iter) override;
+    EigenOut lobpcg_host(const double *v, int k, double *x, double *evals, double lam, double tol,
+                         int max_iter) override;
+    double effective_eig_tol(double requested) const override {
+        return sizeof(T) == 4 ? std::max(requested, EIG_TOL_FLOOR_FACTOR * residual_floor_f32(h_)) : requested;
+    }
+    void reset() override { valid_ = false; }
+    int precision() const override { return sizeof(T) == 8 ? LIRA_FP64 : LIRA_MIXED; }
+
+  private:
+    int n_;
+    double h_, dv_;
+    long N_;
+    hipStream_t stream_ = nullptr;
+    double *hbuf_ = nullptr;
+    size_t hcap_ = 0;
+    DevBuf partials_, dres_, gpart_, dG_, dC_, dsmall_;
+    size_t gpart_cap_ = 0, dG_cap_ = 0, dC_cap_ = 0;
+    int readbacks_ = 0;
+    // Poisson
+    struct Level {
+        int n;
+        double h, d_low, d_high;
+        DevBuf x, b, res, tmp;
+    };
+    std::vector<Level> levels_;
+    DevBuf pb_, pr64_, ptmp64_, pr_, pz_, pp_, pq_, pe_, psc_;
+    bool poisson_ready_ = false;
+    int cg_total_ = 0;
+    // SCF
+    DevBuf vnuc_, vh_, vh_t_, rho_in_[2], rho_out_[2], vxc_[2], exc_, veff_[2], rho_t_, rho64_, pos_, zf_;
+    struct Slot {
+        DevBuf rho[2], res[2];
+    };
+    std::vector<Slot> slots_;
+    Channel<T> ch_[2];
+    bool valid_ = false;
+    int nat_ = 0;
+    std::vector<double> pos_h_, z_h_;
+    double sigma_ = 0.0;
+    int occ_scheme_ = -1;
+    double kt_ = 0.0;
+    int nel_[2] = {0, 0};
+    int lobpcg_total_ = 0;
+
+    void ensure_host(size_t count) {
+        if (count <= hcap_) return;
+        if (hbuf_) LIRA_HIP(hipHostFree(hbuf_));
+        hcap_ = std::max(count, (size_t)4096);
+        LIRA_HIP(hipHostMalloc((void **)&hbuf_, hcap_ * sizeof(double), hipHostMallocDefault));
+    }
+    // copies count doubles from the device and waits (the solve's only sync points)
+    const double *readback(const double *d, size_t count) {
+        ensure_host(count);
+        LIRA_HIP(hipMemcpyAsync(hbuf_, d, count * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        LIRA_HIP(hipStreamSynchronize(stream_));
+        readbacks_++;
+        return hbuf_;
+    }
+    void upload(double *d, const double *h, size_t count) {
+        LIRA_HIP(hipMemcpyAsync(d, h, count * sizeof(double), hipMemcpyHostToDevice, stream_));
+        LIRA_HIP(hipStreamSynchronize(stream_));
+    }
+    T *fld(DevBuf &b) { return b.template as<T>(); }
+    void alloc_field(DevBuf &b, int count = 1) {
+        if (b.bytes < (size_t)N_ * count * sizeof(T)) b.alloc((size_t)N_ * count * sizeof(T));
+    }
+    void ensure_gram(int m, int q) {
+        const size_t need = gram_partials_needed(m, q, N_);
+        if (need > gpart_cap_) {
+            gpart_.alloc(need * sizeof(double));
+            gpart_cap_ = need;
+        }
+        const size_t g = (size_t)m * q + m + 64;
+        if (g > dG_cap_) {
+            dG_.alloc(g * sizeof(double));
+            dG_cap_ = g;
+        }
+    }
+    void ensure_coeff(size_t count) {
+        if (count > dC_cap_) {
+            dC_.alloc(count * sizeof(double));
+            dC_cap_ = count;
+        }
+    }
+    void ensure_channel(Channel<T> &c, int kc);
+    void hamiltonian(const T *v, const T *x, T *out, int nb) {
+        StencilParams sp{n_, nb, h_, 1.0, 0.0, 0.0};
+        stencil<T>(ST_HAM, sp, x, out, v, nullptr, nullptr, stream_);
+    }
+    std::vector<double> reduce(RedJobs<T> &jobs, double scale) {
+        reduce_jobs<T>(jobs, n_, scale, partials_.as<double>(), dres_.as<double>(), stream_);
+        const double *r = readback(dres_.as<double>(), jobs.count);
+        return std::vector<double>(r, r + jobs.count);
+    }
+    // Poisson
+    void poisson_setup();
+    void vcycle(int l, T *x, const T *rhs);
+    PoissonOut poisson(const T *rho, double *v, double tol, int max_iter);
+    int pcg(const T *r0, T *e, double rel_tol, int max_iter, double *rel_out);
+    // LOBPCG on a channel's X (k columns) for H = -1/2 lap + v
+    EigenOut lobpcg(Channel<T> &c, const T *v, int k, double lam, double tol, int max_iter, double tail_tol, int head);
+    void solve_channel(int spin, const T *v, int nel, bool fd, double kt, double tol, EigenOut &eo, Fd &occ);
+    void upload_nuclei(const lira_nucleus *nuc, int nat);
+};
+
+template <typename T> Solver<T>::Solver(int n, double h) : n_(n), h_(h), dv_(h * h * h), N_((long)n * n * n) {
+    LIRA_HIP(hipStreamCreate(&stream_));
+    partials_.alloc((size_t)RED_BLOCKS * std::max(MAX_JOBS, 49) * sizeof(double) * 4);
+    dres_.alloc(1024 * sizeof(double));
+    dsmall_.alloc(4096 * sizeof(double));
+    ensure_host(1 << 16);
+}
+
+template <typename T> void Solver<T>::ensure_channel(Channel<T> &c, int kc) {
+    if (kc <= c.kcap) return;
+    if (c.kcap > 0) kc = std::max(kc, std::min(2 * c.kcap, kc + 8));  // headroom for witness growth
+    const size_t bytes = (size_t)N_ * kc * sizeof(T);
+    DevBuf nx(bytes);
+    if (c.k > 0) LIRA_HIP(hipMemcpyAsync(nx.ptr, c.X.ptr, (size_t)N_ * c.k * sizeof(T), hipMemcpyDeviceToDevice, stream_));
+    c.X = std::move(nx);
+    for (DevBuf *b : {&c.HX, &c.R, &c.W, &c.HW, &c.P, &c.HP, &c.XN, &c.PN, &c.TMP}) b->alloc(bytes);
+    c.kcap = kc;
+}
+
+// -------------------------------------------------------------------------------------------- Poisson ----
+template <typename T> void Solver<T>::poisson_setup() {
+    if (poisson_ready_) return;
+    pb_.alloc(N_ * sizeof(double));
+    pr64_.alloc(N_ * sizeof(double));
+    ptmp64_.alloc(N_ * sizeof(double));
+    for (DevBuf *b : {&pr_, &pz_, &pp_, &pq_, &pe_}) alloc_field(*b);
+    psc_.alloc(64 * sizeof(double));
+    // level 0 works in pz_ (x) with rhs pr_; coarse levels own x and b
+    levels_.clear();
+    Level l0{n_, h_, 1.0, 1.0, {}, {}, {}, {}};
+    l0.res.alloc(N_ * sizeof(T));
+    l0.tmp.alloc(N_ * sizeof(T));
+    levels_.push_back(std::move(l0));
+    while (levels_.back().n > MG_COARSEST_MAX) {
+        const Level &f = levels_.back();
+        const int nc = f.n / 2;
+        Level c{nc, 2.0 * f.h, (f.d_low + 1.0) / 2.0, (f.d_high + (double)(f.n - 2 * nc)) / 2.0, {}, {}, {}, {}};
+        const size_t bytes = (size_t)nc * nc * nc * sizeof(T);
+        c.x.alloc(bytes);
+        c.b.alloc(bytes);
+        c.res.alloc(bytes);
+        c.tmp.alloc(bytes);
+        levels_.push_back(std::move(c));
+    }
+    poisson_ready_ = true;
+}
+
+// one V-cycle from level l: x = M_l^{-1} rhs (crates/qm-gpu/src/multigrid.rs:165-181)
+template <typename T> void Solver<T>::vcycle(int l, T *x, const T *rhs) {
+    Level &L = levels_[l];
+    const long Nl = (long)L.n * L.n * L.n;
+    const double h2 = L.h * L.h, gl = 1.0 - 1.0 / L.d_low, gh = 1.0 - 1.0 / L.d_high;
+    fill<T>(x, Nl, 0.0, stream_);
+    T *tmp = L.tmp.template as<T>();
+    auto smooth = [&](int sweeps) {
+        for (int s = 0; s < sweeps; s += 2) {
+            mg_jacobi<T>(tmp, x, rhs, L.n, h2, MG_OMEGA, gl, gh, stream_);
+            mg_jacobi<T>(x, tmp, rhs, L.n, h2, MG_OMEGA, gl, gh, stream_);
+        }
+    };
+    if (l + 1 == (int)levels_.size()) {
+        smooth(MG_COARSE);
+        return;
+    }
+    smooth(MG_PRE);
+    mg_residual<T>(L.res.template as<T>(), x, rhs, L.n, h2, gl, gh, stream_);
+    Level &C = levels_[l + 1];
+    mg_restrict<T>(C.b.template as<T>(), L.res.template as<T>(), C.n, L.n, stream_);
+    vcycle(l + 1, C.x.template as<T>(), C.b.template as<T>());
+    mg_prolong_add<T>(x, C.x.template as<T>(), L.n, C.n, stream_);
+    smooth(MG_POST);
+}
+
+// Preconditioned CG in precision T on A e = r0 (A = -lap, 8th order, Dirichlet) from e = 0, with the V-cycle as M.
+// The CG scalars live on the device (alpha, beta never cross to the host); the residual norm is read back every
+// iteration with a one-iteration lag, so the host never waits on the iteration it just queued.
+template <typename T> int Solver<T>::pcg(const T *r0, T *e, double rel_tol, int max_iter, double *rel_out) {
+    T *r = pr_.template as<T>(), *z = pz_.template as<T>(), *p = pp_.template as<T>(), *q = pq_.template as<T>();
+    double *sc = psc_.as<double>();
+    copy<T>(r, r0, N_, stream_);
+    fill<T>(e, N_, 0.0, stream_);
+    // sc: [0],[1] rz ping-pong, [2] pq, [3 + i % 2] rr of iteration i, [8] r0.r0
+    RedJobs<T> j0{};
+    j0.a[0] = r;
+    j0.b[0] = r;
+    j0.kind[0] = RED_DOT;
+    j0.count = 1;
+    reduce_jobs<T>(j0, n_, 1.0, partials_.as<double>(), sc + 8, stream_);
+    vcycle(0, z, r);
+    copy<T>(p, z, N_, stream_);
+    RedJobs<T> jrz{};
+    jrz.a[0] = r;
+    jrz.b[0] = z;
+    jrz.kind[0] = RED_DOT;
+    jrz.count = 1;
+    reduce_jobs<T>(jrz, n_, 1.0, partials_.as<double>(), sc + 0, stream_);
+    const double r00 = readback(sc + 8, 1)[0];
+    if (!(r00 > 0.0)) {
+        *rel_out = 0.0;
+        return 0;
+    }
+    const double r0n = std::sqrt(r00);
+    StencilParams sp{n_, 1, h_, -1.0, 0.0, 0.0};
+    int cur = 0, it = 0;
+    double rel = 1.0;
+    hipEvent_t ev[2];
+    LIRA_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    LIRA_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    double *hrr = hbuf_;  // two lagged slots
+    bool pending[2] = {false, false};
+    bool done = false;
+    while (!done && it < max_iter) {
+        stencil<T>(ST_LAP, sp, p, q, nullptr, nullptr, nullptr, stream_);  // q = A p = -lap p
+        RedJobs<T> jpq{};
+        jpq.a[0] = p;
+        jpq.b[0] = q;
+        jpq.kind[0] = RED_DOT;
+        jpq.count = 1;
+        reduce_jobs<T>(jpq, n_, 1.0, partials_.as<double>(), sc + 2, stream_);
+        cg_update<T, T>(e, p, r, q, sc, cur, 2, N_, stream_);
+        vcycle(0, z, r);
+        RedJobs<T> jn{};
+        jn.a[0] = r;
+        jn.b[0] = z;
+        jn.kind[0] = RED_DOT;
+        jn.a[1] = r;
+        jn.b[1] = r;
+        jn.kind[1] = RED_DOT;
+        jn.count = 2;
+        // rz_new into sc[1 - cur], rr into sc[3 + it % 2]: reduce writes consecutive slots, so route via dres
+        reduce_jobs<T>(jn, n_, 1.0, partials_.as<double>(), dres_.as<double>() + 16, stream_);
+        LIRA_HIP(hipMemcpyAsync(sc + (1 - cur), dres_.as<double>() + 16, sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        cg_direction<T>(p, z, sc, 1 - cur, cur, N_, stream_);
+        const int slot = it & 1;
+        LIRA_HIP(hipMemcpyAsync(hrr + slot, dres_.as<double>() + 17, sizeof(double), hipMemcpyDeviceToHost, stream_));
+        LIRA_HIP(hipEventRecord(ev[slot], stream_));
+        pending[slot] = true;
+        cur = 1 - cur;
+        it++;
+        // check the previous iteration's residual while this one runs
+        const int prev = slot ^ 1;
+        if (pending[prev]) {
+            LIRA_HIP(hipEventSynchronize(ev[prev]));
+            pending[prev] = false;
+            rel = std::sqrt(std::max(hrr[prev], 0.0)) / r0n;
+            if (!std::isfinite(rel)) throw NumericalError("poisson: non-finite CG residual");
+            if (rel <= rel_tol) done = true;
+        }
+    }
+    LIRA_HIP(hipStreamSynchronize(stream_));
+    const int last = (it - 1) & 1;
+    if (it > 0) rel = std::sqrt(std::max(hrr[last], 0.0)) / r0n;
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    readbacks_ += it;
+    *rel_out = rel;
+    cg_total_ += it;
+    return it;
+}
+
+// del^2 V = -4 pi rho with the multipole boundary (qm/src/poisson.rs:200-255): b = 4 pi rho + L_bc in f64, then
+// f64 residual corrections V += e with e from the precision-T MG-PCG, until ||b - A V|| / ||b|| <= tol.
+template <typename T> PoissonOut Solver<T>::poisson(const T *rho, double *v, double tol, int max_iter) {
+    auto t0 = Clock::now();
+    poisson_setup();
+    PoissonOut out;
+    double *q = dsmall_.as<double>();
+    multipole_moments<T>(rho, n_, h_, partials_.as<double>(), q, stream_);
+    double *b = pb_.as<double>(), *r64 = pr64_.as<double>(), *t64 = ptmp64_.as<double>();
+    poisson_rhs<T>(b, rho, q, n_, h_, stream_);
+    RedJobs<double> jb{};
+    jb.a[0] = b;
+    jb.b[0] = b;
+    jb.kind[0] = RED_DOT;
+    jb.count = 1;
+    reduce_jobs<double>(jb, n_, 1.0, partials_.as<double>(), dres_.as<double>(), stream_);
+    double bb = readback(dres_.as<double>(), 1)[0];
+    const double bnorm = std::max(std::sqrt(bb), 1e-300);
+    StencilParams sp{n_, 1, h_, 1.0, 0.0, 0.0};
+    const double inner_floor = sizeof(T) == 4 ? 2e-5 : 1e-14;
+    for (int outer = 0; outer < 12; outer++) {
+        stencil<double>(ST_LAP, sp, v, t64, nullptr, nullptr, nullptr, stream_);  // lap V
+        add2<double>(r64, b, t64, N_, stream_);                                   // r = b - A V = b + lap V
+        RedJobs<double> jr{};
+        jr.a[0] = r64;
+        jr.b[0] = r64;
+        jr.kind[0] = RED_DOT;
+        jr.count = 1;
+        reduce_jobs<double>(jr, n_, 1.0, partials_.as<double>(), dres_.as<double>(), stream_);
+        const double rr = readback(dres_.as<double>(), 1)[0];
+        out.residual = std::sqrt(std::max(rr, 0.0)) / bnorm;
+        if (!std::isfinite(out.residual)) throw NumericalError("poisson: non-finite residual (NaN in rho?)");
+        if (out.residual <= tol || out.iterations >= max_iter) break;
+        out.outer++;
+        const double rel = std::max(inner_floor, 0.3 * tol / out.residual);
+        T *e = pe_.template as<T>();
+        const T *r0;
+        if (sizeof(T) == 8) {
+            r0 = reinterpret_cast<const T *>(r64);
+        } else {
+            convert<T, double>(pe_.template as<T>(), r64, N_, stream_);
+            copy<T>(pq_.template as<T>(), pe_.template as<T>(), N_, stream_);
+            r0 = pq_.template as<T>();
+        }
+        double rel_out = 0.0;
+        // r0 may alias pq_ (the CG's q); pcg copies it into r before q is written
+        out.iterations += pcg(r0, e, rel, max_iter - out.iterations, &rel_out);
+        add_into_f64<T>(v, e, N_, stream_);
+    }
+    LIRA_HIP(hipStreamSynchronize(stream_));
+    out.seconds = secs(t0);
+    return out;
+}
+
+// ------------------------------------------------------------------------------------------- LOBPCG ----
+// Block LOBPCG with one readback per iteration (crates/qm-gpu/src/eigen.rs:685-943, restated): R = HX - theta X;
+// W = 4-sweep damped Jacobi on (H - lam) w = r; one Gram of S = [X, W, P] against [S, R, HW, HP] gives
+// B = S^T S and A = S^T H S (A's X columns restored from R); host f64 generalised Rayleigh–Ritz with the
+// conditioning guard; X, P updated by one block lincomb on the device; HX recomputed by the stencil.
+template <typename T>
+EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double tol, int max_iter, double tail_tol,
+                           int head) {
+    EigenOut out;
+    if (k <= 0) return out;
+    const double diag_t = -1.5 * COEFF0 / (h_ * h_);
+    const double omega = jacobi_omega();
+    const double floor = sizeof(T) == 4 ? residual_floor_f32(h_) : 0.0;
+    double *dth = dsmall_.as<double>() + 256, *dlam = dsmall_.as<double>() + 512;
+    {
+        std::vector<double> l(k, lam);
+        upload(dlam, l.data(), k);
+    }
+    auto X = [&]() { return c.X.template as<T>(); };
+    // initial Rayleigh–Ritz on X
+    hamiltonian(v, X(), c.HX.template as<T>(), k);
+    ensure_gram(3 * k, 6 * k);
+    {
+        Cols S{{{X(), k}}, 1}, U{{{X(), k}, {c.HX.ptr, k}}, 2};
+        gram<T>(S, U, N_, dv_, gpart_.as<double>(), gpart_cap_, dG_.as<double>(), stream_);
+    }
+    std::vector<double> G;
+    {
+        const double *g = readback(dG_.as<double>(), (size_t)k * 2 * k);
+        G.assign(g, g + (size_t)k * 2 * k);
+    }
+    std::vector<double> A((size_t)k * k), B((size_t)k * k);
+    for (int i = 0; i < k; i++)
+        for (int j = 0; j <= i; j++) {
+            B[(size_t)i * k + j] = B[(size_t)j * k + i] = G[(size_t)i * 2 * k + j];
+            A[(size_t)i * k + j] = A[(size_t)j * k + i] = G[(size_t)i * 2 * k + k + j];
+        }
+    for (double x : G)
+        if (!std::isfinite(x)) throw NumericalError("lobpcg: non-finite Gram of the guess (NaN in V or X?)");
+    std::vector<double> theta, C;
+    rayleigh_ritz(k, A, B, k, 1e-10, theta, C);
+    ensure_coeff((size_t)9 * k * k + 64);
+    upload(dC_.as<double>(), C.data(), (size_t)k * k);
+    {
+        Cols S{{{X(), k}}, 1};
+        lincomb<T>(S, dC_.as<double>(), k, c.XN.template as<T>(), N_, stream_);
+    }
+    std::swap(c.X, c.XN);
+    hamiltonian(v, X(), c.HX.template as<T>(), k);
+    int np = 0;
+    std::vector<double> lambda(k), res(k);
+    StencilParams jp{n_, k, h_, 1.0, omega, diag_t};
+    for (;;) {
+        upload(dth, theta.data(), k);
+        residual<T>(c.R.template as<T>(), c.HX.template as<T>(), X(), dth, k, N_, stream_);
+        precond_init<T>(c.W.template as<T>(), c.R.template as<T>(), v, dlam, diag_t, k, N_, stream_);
+        for (int s = 0; s < 3; s++) {
+            stencil<T>(ST_JAC, jp, c.W.template as<T>(), c.TMP.template as<T>(), v, c.R.template as<T>(), dlam, stream_);
+            std::swap(c.W, c.TMP);
+        }
+        hamiltonian(v, c.W.template as<T>(), c.HW.template as<T>(), k);
+        if (np > 0) hamiltonian(v, c.P.template as<T>(), c.HP.template as<T>(), np);
+        const int m = 2 * k + np, q = 2 * m;
+        ensure_gram(m, q);
+        Cols S{{{X(), k}, {c.W.ptr, k}, {c.P.ptr, np}}, np > 0 ? 3 : 2};
+        // B = S^T S and S^T [R, HW, HP] as two Gram products (a column list holds at most 4 segments)
+        Cols Ss = S;
+        Cols Tt{{{c.R.ptr, k}, {c.HW.ptr, k}, {c.HP.ptr, np}}, np > 0 ? 3 : 2};
+        double *dG = dG_.as<double>();
+        gram<T>(S, Ss, N_, dv_, gpart_.as<double>(), gpart_cap_, dG, stream_);
+        gram<T>(S, Tt, N_, dv_, gpart_.as<double>(), gpart_cap_, dG + (size_t)m * m, stream_);
+        RedJobs<T> jr{};
+        for (int i = 0; i < k; i++) {
+            jr.a[i % MAX_JOBS] = c.R.template as<T>() + (long)i * N_;
+            jr.b[i % MAX_JOBS] = jr.a[i % MAX_JOBS];
+            jr.kind[i % MAX_JOBS] = RED_DOT;
+            jr.count = i % MAX_JOBS + 1;
+            if (jr.count == MAX_JOBS || i == k - 1) {
+                reduce_jobs<T>(jr, n_, dv_, partials_.as<double>(), dG + (size_t)2 * m * m + (i / MAX_JOBS) * MAX_JOBS,
+                               stream_);
+                jr.count = 0;
+            }
+        }
+        const double *g = readback(dG, (size_t)2 * m * m + k);
+        std::vector<double> Bm((size_t)m * m), Am((size_t)m * m);
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j <= i; j++) {
+                const double bij = g[(size_t)i * m + j];
+                double aij = g[(size_t)m * m + (size_t)i * m + j];
+                if (j < k) aij += theta[j] * bij;
+                Bm[(size_t)i * m + j] = Bm[(size_t)j * m + i] = bij;
+                Am[(size_t)i * m + j] = Am[(size_t)j * m + i] = aij;
+            }
+        double max_res = 0.0, head_res = 0.0, tail_res = 0.0;
+        bool finite = true;
+        for (int i = 0; i < k; i++) {
+            const double xx = Bm[(size_t)i * m + i];
+            lambda[i] = Am[(size_t)i * m + i] / xx;
+            res[i] = std::sqrt(std::max(g[(size_t)2 * m * m + i], 0.0) / xx);
+            if (!std::isfinite(lambda[i]) || !std::isfinite(res[i])) finite = false;
+            max_res = std::max(max_res, res[i]);
+            if (i < head) head_res = std::max(head_res, res[i]);
+            else tail_res = std::max(tail_res, res[i]);
+        }
+        for (double x : Am)
+            if (!std::isfinite(x)) finite = false;
+        if (!finite) throw NumericalError("lobpcg: non-finite Gram entry or residual (NaN in a vector or in V)");
+        out.max_residual = max_res;
+        if ((head_res < tol && tail_res < tail_tol) || out.iterations >= max_iter) break;
+        out.iterations++;
+        // Rayleigh–Ritz with the guard: a step whose coefficient amplification exceeds the budget drops P
+        int mm = m;
+        std::vector<double> th, Cc;
+        double alpha;
+        const double alpha_budget =
+            sizeof(T) == 4 ? std::max(ALPHA_MAX, ALPHA_RESIDUAL_FRACTION * max_res / std::max(floor, 1e-300)) : 1e300;
+        try {
+            alpha = rayleigh_ritz(m, Am, Bm, k, RR_DROP, th, Cc);
+        } catch (const NumericalError &) {
+            if (np == 0) throw;
+            alpha = 1e300;
+        }
+        if (alpha > alpha_budget && np > 0) {
+            out.fallbacks++;
+            mm = 2 * k;
+            std::vector<double> As((size_t)mm * mm), Bs((size_t)mm * mm);
+            for (int i = 0; i < mm; i++)
+                for (int j = 0; j < mm; j++) {
+                    As[(size_t)i * mm + j] = Am[(size_t)i * m + j];
+                    Bs[(size_t)i * mm + j] = Bm[(size_t)i * m + j];
+                }
+            rayleigh_ritz(mm, As, Bs, k, RR_DROP, th, Cc);
+            Am.swap(As);
+            Bm.swap(Bs);
+        }
+        // P = the Ritz vector minus its X component: C_p = C - [Bxx^{-1} Bxs C ; 0]
+        std::vector<double> M((size_t)mm * 2 * k, 0.0);
+        for (int i = 0; i < mm; i++)
+            for (int j = 0; j < k; j++) M[(size_t)i * 2 * k + j] = Cc[(size_t)i * k + j];
+        {
+            std::vector<double> bxx((size_t)k * k), rhs((size_t)k * k, 0.0);
+            for (int i = 0; i < k; i++)
+                for (int j = 0; j < k; j++) bxx[(size_t)i * k + j] = Bm[(size_t)i * mm + j];
+            for (int i = 0; i < k; i++)
+                for (int j = 0; j < k; j++) {
+                    double s = 0.0;
+                    for (int a = 0; a < mm; a++) s += Bm[(size_t)i * mm + a] * Cc[(size_t)a * k + j];
+                    rhs[(size_t)i * k + j] = s;
+                }
+            if (!cholesky(k, bxx)) throw NumericalError("lobpcg: Cholesky of X^T X failed (X lost rank)");
+            chol_solve(k, bxx, rhs, k);
+            for (int i = 0; i < mm; i++)
+                for (int j = 0; j < k; j++)
+                    M[(size_t)i * 2 * k + k + j] = Cc[(size_t)i * k + j] - (i < k ? rhs[(size_t)i * k + j] : 0.0);
+        }
+        ensure_coeff(M.size() + 64);
+        upload(dC_.as<double>(), M.data(), M.size());
+        Cols Sm{{{X(), k}, {c.W.ptr, k}, {c.P.ptr, mm > 2 * k ? np : 0}}, mm > 2 * k ? 3 : 2};
+        lincomb<T>(Sm, dC_.as<double>(), 2 * k, c.XN.template as<T>(), N_, stream_, k, c.PN.template as<T>());
+        std::swap(c.X, c.XN);
+        std::swap(c.P, c.PN);
+        np = k;
+        theta = th;
+        hamiltonian(v, X(), c.HX.template as<T>(), k);
+    }
+    c.evals.assign(lambda.begin(), lambda.end());
+    c.res.assign(res.begin(), res.end());
+    lobpcg_total_ += out.iterations;
+    return out;
+}
+
+// one spin channel under Aufbau or Fermi–Dirac (qm/src/scf.rs:774-877; every column held to tol)
+template <typename T>
+void Solver<T>::solve_channel(int spin, const T *v, int nel, bool fd, double kt, double tol, EigenOut &eo, Fd &occ) {
+    Channel<T> &c = ch_[spin];
+    if (c.k == 0) {
+        occ = Fd{};
+        return;
+    }
+    const double guard = std::max(WITNESS_GUARD_TOL, tol);
+    for (;;) {
+        const double lam = c.evals.empty() ? -1.0 : c.evals[0];
+        EigenOut e = lobpcg(c, v, c.k, lam, tol, 200, tol, c.k);
+        eo.iterations += e.iterations;
+        eo.fallbacks += e.fallbacks;
+        eo.max_residual = std::max(eo.max_residual, e.max_residual);
+        if (!fd) {
+            occ.f.assign(c.k, 1.0);
+            occ.mu = NAN;
+            occ.entropy = 0.0;
+            c.occ = occ.f;
+            return;
+        }
+        occ = fermi_dirac(c.evals, nel, kt);
+        const int extras = c.k - nel;
+        auto o = ascending(c.evals);
+        const double top_r = c.res[o[c.k - 1]];
+        const bool witness = witness_bound_reached(c.evals, c.res, nel, occ.mu, kt);
+        auto add_column = [&]() {
+            ensure_channel(c, c.k + 1);
+            initial_orbitals<T>(c.X.template as<T>() + (long)c.k * N_, n_, h_, pos_.as<double>(), nat_, 1,
+                                1000ULL + 2ULL * (uint64_t)c.k + (uint64_t)spin, stream_);
+            c.evals.push_back(c.evals.back());
+            c.res.push_back(0.0);
+            c.k++;
+        };
+        if (witness && top_r > WITNESS_RESIDUAL_FACTOR * guard && extras < MAX_WITNESS_GROWTH) {
+            add_column();
+            continue;
+        }
+        if (witness) {
+            const int keep = witness_keep(c.evals, c.res, nel, occ.mu, kt);
+            if (keep < c.k) {
+                // keep the lowest `keep` levels, in their column order
+                std::vector<int> kept(o.begin(), o.begin() + keep);
+                std::sort(kept.begin(), kept.end());
+                for (int j = 0; j < keep; j++)
+                    LIRA_HIP(hipMemcpyAsync(c.XN.template as<T>() + (long)j * N_, c.X.template as<T>() + (long)kept[j] * N_,
+                                            (size_t)N_ * sizeof(T), hipMemcpyDeviceToDevice, stream_));
+                std::swap(c.X, c.XN);
+                std::vector<double> ev2, r2;
+                for (int j : kept) {
+                    ev2.push_back(c.evals[j]);
+                    r2.push_back(c.res[j]);
+                }
+                c.evals = ev2;
+                c.res = r2;
+                c.k = keep;
+                occ = fermi_dirac(c.evals, nel, kt);
+            }
+            c.occ = occ.f;
+            return;
+        }
+        if (extras >= MAX_WITNESS_GROWTH)
+            throw NumericalError("no gap witness within " + std::to_string(MAX_WITNESS_GROWTH) +
+                                 " extra levels of spin " + std::to_string(spin));
+        const int target = std::min(std::max(2 * extras, extras + 1), MAX_WITNESS_GROWTH);
+        while (c.k - nel < target) add_column();
+    }
+}
+
+template <typename T> void Solver<T>::upload_nuclei(const lira_nucleus *nuc, int nat) {
+    nat_ = nat;
+    pos_h_.assign(3 * nat, 0.0);
+    z_h_.assign(nat, 0.0);
+    for (int i = 0; i < nat; i++) {
+        for (int a = 0; a < 3; a++) pos_h_[3 * i + a] = nuc[i].pos[a];
+        z_h_[i] = nuc[i].z;
+    }
+    if (pos_.bytes < (size_t)(3 * nat + 1) * sizeof(double)) pos_.alloc((size_t)(3 * nat + 1) * sizeof(double));
+    if (zf_.bytes < (size_t)(nat + 1) * sizeof(double)) zf_.alloc((size_t)(nat + 1) * sizeof(double));
+    if (nat > 0) {
+        upload(pos_.as<double>(), pos_h_.data(), 3 * nat);
+        upload(zf_.as<double>(), z_h_.data(), nat);
+    }
+}
+
+// ----------------------------------------------------------------------------------------------- SCF ----
+template <typename T>
+lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, int warm, lira_abort_fn abort,
+                             void *user, lira_scf_stats *out) {
+    auto t0 = Clock::now();
+    std::memset(out, 0, sizeof *out);
+    readbacks_ = 0;
+    cg_total_ = 0;
+    lobpcg_total_ = 0;
+    double t_poisson = 0.0, t_eigen = 0.0;
+    const int nat = sys->n_nuclei;
+    if (nat < 0 || (nat > 0 && !sys->nuclei) || sys->n_up < 0 || sys->n_dn < 0)
+        throw InvalidError("lira_solve: bad system");
+    if (!(st->sigma > 0.0) || !(st->tol > 0.0) || st->max_iter < 1 || st->diis_history > 15)
+        throw InvalidError("lira_solve: bad settings");
+    // nucleus-nucleus repulsion and the coincidence check (qm/src/nuclear.rs:277-289): the warm state is kept
+    double e_nn = 0.0;
+    for (int i = 0; i < nat; i++)
+        for (int j = i + 1; j < nat; j++) {
+            double d2 = 0.0;
+            for (int a = 0; a < 3; a++) d2 += std::pow(sys->nuclei[i].pos[a] - sys->nuclei[j].pos[a], 2);
+            const double d = std::sqrt(d2);
+            if (d < 1e-8) throw std::pair<int, int>(i, j);
+            e_nn += (double)sys->nuclei[i].z * sys->nuclei[j].z / d;
+        }
+    const bool fd = st->occupations == LIRA_FERMI_DIRAC;
+    const double kt = fd ? st->kt : 0.0;
+    const bool reuse = warm && valid_ && nel_[0] == sys->n_up && nel_[1] == sys->n_dn && occ_scheme_ == st->occupations &&
+                       (!fd || kt_ == st->kt) && nat_ == nat;
+    upload_nuclei(sys->nuclei, nat);
+    sigma_ = st->sigma;
+    const int nel[2] = {sys->n_up, sys->n_dn};
+    const bool restricted = sys->n_up == sys->n_dn;
+    valid_ = false;
+    // fields
+    for (DevBuf *b : {&vnuc_, &vh_t_, &rho_in_[0], &rho_in_[1], &rho_out_[0], &rho_out_[1], &vxc_[0], &vxc_[1], &exc_,
+                      &veff_[0], &veff_[1], &rho_t_})
+        alloc_field(*b);
+    if (vh_.bytes < N_ * sizeof(double)) {
+        vh_.alloc(N_ * sizeof(double));
+        fill<double>(vh_.as<double>(), N_, 0.0, stream_);
+    }
+    if (rho64_.bytes < N_ * sizeof(double)) rho64_.alloc(N_ * sizeof(double));
+    nuclear_potential<T>(vnuc_.template as<T>(), n_, h_, pos_.as<double>(), zf_.as<double>(), nat, st->sigma, stream_);
+    if (!reuse) {
+        // cold start (qm/src/scf.rs:948-965): sigma = 1 blobs scaled to the electron count, seeded orbitals, V_H = 0
+        double ztot = 0.0;
+        for (int i = 0; i < nat; i++) ztot += sys->nuclei[i].z;
+        const double total = nel[0] + nel[1];
+        const double scale = ztot > 0.0 ? total / ztot : 0.0;
+        for (int s = 0; s < 2; s++) {
+            const double f = total > 0.0 ? nel[s] / total : 0.0;
+            deposit_charge<T>(rho_in_[s].template as<T>(), n_, h_, pos_.as<double>(), zf_.as<double>(), nat, 1.0, scale * f,
+                              stream_);
+            Channel<T> &c = ch_[s];
+            const int k = nel[s] + (fd && nel[s] > 0 ? 1 : 0);
+            c.k = 0;
+            ensure_channel(c, std::max(k, 1));
+            c.k = k;
+            if (k > 0) initial_orbitals<T>(c.X.template as<T>(), n_, h_, pos_.as<double>(), nat, k, (uint64_t)(s + 1), stream_);
+            c.evals.assign(k, -1.0);
+            c.res.assign(k, 0.0);
+            c.occ.assign(k, 1.0);
+        }
+        fill<double>(vh_.as<double>(), N_, 0.0, stream_);
+    } else {
+        for (int s = 0; s < 2; s++) copy<T>(rho_in_[s].template as<T>(), rho_out_[s].template as<T>(), N_, stream_);
+    }
+    const double ptol = std::min(1e-10, 1e-3 * st->tol);  // qm/src/scf.rs:519
+    const double lo = effective_eig_tol(st->eig_tol), hi = std::max(EIG_TOL_MAX, lo);
+    const int H = std::max(st->diis_history, 1);
+    if ((int)slots_.size() < H) slots_.resize(H);
+    for (auto &sl : slots_)
+        for (int s = 0; s < 2; s++) {
+            alloc_field(sl.rho[s]);
+            alloc_field(sl.res[s]);
+        }
+    std::deque<int> order;       // DIIS history, oldest first (slot indices)
+    std::vector<double> gram((size_t)H * H, 0.0);  // <r_i, r_j> by slot
+    double delta_rho = INFINITY, delta_prev = INFINITY;
+    double anchor = INFINITY, best = INFINITY;
+    int anchor_at = 0;
+    const double level = STALL_LEVEL_PER_ELECTRON * (nel[0] + nel[1]);
+    Fd occ[2];
+    EigenOut eo;
+    double max_pres = 0.0;
+    int max_pit = 0;
+    int it = 0;
+    bool converged = false, stalled = false;
+    while (it < st->max_iter) {
+        if (abort && abort(user)) {
+            out->iterations = it;
+            return LIRA_ABORTED;
+        }
+        it++;
+        add2<T>(rho_t_.template as<T>(), rho_in_[0].template as<T>(), rho_in_[1].template as<T>(), N_, stream_);
+        PoissonOut po = poisson(rho_t_.template as<T>(), vh_.as<double>(), ptol, 5000);
+        t_poisson += po.seconds;
+        max_pres = std::max(max_pres, po.residual);
+        max_pit = std::max(max_pit, po.iterations);
+        convert<T, double>(vh_t_.template as<T>(), vh_.as<double>(), N_, stream_);
+        xc_lsda<T>(rho_in_[0].template as<T>(), rho_in_[1].template as<T>(), exc_.template as<T>(), vxc_[0].template as<T>(), vxc_[1].template as<T>(), N_, stream_);
+        sum3<T>(veff_[0].template as<T>(), vnuc_.template as<T>(), vh_t_.template as<T>(), vxc_[0].template as<T>(), N_, stream_);
+        if (!restricted) sum3<T>(veff_[1].template as<T>(), vnuc_.template as<T>(), vh_t_.template as<T>(), vxc_[1].template as<T>(), N_, stream_);
+        // the eigen tolerance follows delta_rho (crates/qm-gpu/src/scf.rs:383-386); a loose first step
+        const double tol_step = std::isfinite(delta_prev) ? std::min(std::max(EIG_TOL_FRACTION * delta_prev, lo), hi) : hi;
+        const bool tight = tol_step <= lo;
+        auto te = Clock::now();
+        solve_channel(0, veff_[0].template as<T>(), nel[0], fd, kt, tol_step, eo, occ[0]);
+        if (restricted) {
+            Channel<T> &u = ch_[0], &d = ch_[1];
+            ensure_channel(d, std::max(u.k, 1));
+            if (u.k > 0)
+                LIRA_HIP(hipMemcpyAsync(d.X.ptr, u.X.ptr, (size_t)N_ * u.k * sizeof(T), hipMemcpyDeviceToDevice, stream_));
+            d.k = u.k;
+            d.evals = u.evals;
+            d.res = u.res;
+            d.occ = u.occ;
+            occ[1] = occ[0];
+        } else {
+            solve_channel(1, veff_[1].template as<T>(), nel[1], fd, kt, tol_step, eo, occ[1]);
+        }
+        t_eigen += secs(te);
+        // output densities and delta_rho
+        double *df = dsmall_.as<double>() + 1024;
+        for (int s = 0; s < 2; s++) {
+            Channel<T> &c = ch_[s];
+            if (c.k > 0) {
+                upload(df + 256 * s, c.occ.data(), c.k);
+                density<T>(rho_out_[s].template as<T>(), c.X.template as<T>(), df + 256 * s, c.k, N_, stream_);
+            } else {
+                fill<T>(rho_out_[s].template as<T>(), N_, 0.0, stream_);
+            }
+        }
+        RedJobs<T> jd{};
+        jd.a[0] = rho_out_[0].template as<T>();
+        jd.b[0] = rho_in_[0].template as<T>();
+        jd.kind[0] = RED_ABSDIFF;
+        jd.a[1] = rho_out_[1].template as<T>();
+        jd.b[1] = rho_in_[1].template as<T>();
+        jd.kind[1] = RED_ABSDIFF;
+        jd.count = 2;
+        auto dr = reduce(jd, dv_);
+        delta_rho = dr[0] + dr[1];
+        // stall watch (qm/src/scf.rs:433-469)
+        if (delta_rho < best) best = delta_rho;
+        if (std::isfinite(delta_rho) && (!std::isfinite(anchor) || delta_rho <= 0.5 * anchor)) {
+            anchor = delta_rho;
+            anchor_at = it;
+        }
+        const int since = it - anchor_at;
+        if (!std::isfinite(delta_rho)) break;
+        if (delta_rho < st->tol && tight) {
+            converged = true;
+            break;
+        }
+        if (delta_rho >= st->tol && st->stall_window > 0 && since >= st->stall_window && best >= level) {
+            stalled = true;
+            break;
+        }
+        delta_prev = delta_rho;
+        // DIIS (qm/src/scf.rs:681-718): push (rho_in, r = rho_out - rho_in), drop the oldest beyond the history
+        int slot;
+        if ((int)order.size() < H) {
+            std::vector<bool> used(H, false);
+            for (int o : order) used[o] = true;
+            slot = 0;
+            while (used[slot]) slot++;
+        } else {
+            slot = order.front();
+            order.pop_front();
+        }
+        for (int s = 0; s < 2; s++) {
+            copy<T>(slots_[slot].rho[s].template as<T>(), rho_in_[s].template as<T>(), N_, stream_);
+            sub2<T>(slots_[slot].res[s].template as<T>(), rho_out_[s].template as<T>(), rho_in_[s].template as<T>(), N_, stream_);
+        }
+        order.push_back(slot);
+        {
+            RedJobs<T> jg{};
+            int cnt = 0;
+            for (int o : order)
+                for (int s = 0; s < 2; s++) {
+                    jg.a[cnt] = slots_[slot].res[s].template as<T>();
+                    jg.b[cnt] = slots_[o].res[s].template as<T>();
+                    jg.kind[cnt] = RED_DOT;
+                    cnt++;
+                }
+            jg.count = cnt;
+            auto gv = reduce(jg, dv_);
+            int idx = 0;
+            for (int o : order) {
+                const double val = gv[idx] + gv[idx + 1];
+                idx += 2;
+                gram[(size_t)slot * H + o] = gram[(size_t)o * H + slot] = val;
+            }
+        }
+        std::vector<double> cvec;
+        for (;;) {
+            const int m = (int)order.size();
+            if (m == 1) {
+                cvec.assign(1, 1.0);
+                break;
+            }
+            const int mm = m + 1;
+            std::vector<double> Bm((size_t)mm * mm, 0.0), rhs(mm, 0.0);
+            double scale = 0.0;
+            for (int i = 0; i < m; i++) scale = std::max(scale, gram[(size_t)order[i] * H + order[i]]);
+            bool ok = false;
+            if (scale > 0.0) {
+                for (int i = 0; i < m; i++) {
+                    for (int j = 0; j < m; j++) Bm[(size_t)i * mm + j] = gram[(size_t)order[i] * H + order[j]] / scale;
+                    Bm[(size_t)i * mm + i] += DIIS_REG;
+                    Bm[(size_t)i * mm + m] = Bm[(size_t)m * mm + i] = 1.0;
+                }
+                rhs[m] = 1.0;
+                if (lu_solve(mm, Bm, rhs)) {
+                    ok = true;
+                    for (int i = 0; i < m; i++)
+                        if (!std::isfinite(rhs[i])) ok = false;
+                }
+            }
+            if (ok) {
+                cvec.assign(rhs.begin(), rhs.begin() + m);
+                break;
+            }
+            order.pop_front();
+        }
+        double *dcv = dsmall_.as<double>() + 2048;
+        upload(dcv, cvec.data(), cvec.size());
+        for (int s = 0; s < 2; s++) {
+            std::vector<const T *> rp, rr;
+            for (int o : order) {
+                rp.push_back(slots_[o].rho[s].template as<T>());
+                rr.push_back(slots_[o].res[s].template as<T>());
+            }
+            diis_mix<T>(rho_in_[s].template as<T>(), rp.data(), rr.data(), dcv, (int)order.size(), st->mixing, N_, stream_);
+        }
+    }
+    out->iterations = it;
+    out->delta_rho = delta_rho;
+    out->max_eigen_residual = eo.max_residual;
+    out->lobpcg_iterations = lobpcg_total_;
+    out->restricted = restricted ? 1 : 0;
+    out->tol_used = st->tol;
+    if (stalled) {
+        out->seconds_total = secs(t0);
+        return LIRA_STALLED;
+    }
+    if (!converged) {
+        out->seconds_total = secs(t0);
+        return LIRA_NOT_CONVERGED;
+    }
+    // energy from the output density and orbitals (qm/src/scf.rs:1048-1081)
+    add2<T>(rho_t_.template as<T>(), rho_out_[0].template as<T>(), rho_out_[1].template as<T>(), N_, stream_);
+    PoissonOut po = poisson(rho_t_.template as<T>(), vh_.as<double>(), ptol, 5000);
+    t_poisson += po.seconds;
+    max_pres = std::max(max_pres, po.residual);
+    max_pit = std::max(max_pit, po.iterations);
+    xc_lsda<T>(rho_out_[0].template as<T>(), rho_out_[1].template as<T>(), exc_.template as<T>(), vxc_[0].template as<T>(), vxc_[1].template as<T>(), N_, stream_);
+    double kinetic = 0.0;
+    {
+        std::vector<double> terms;
+        StencilParams sp{n_, 0, h_, -0.5, 0.0, 0.0};
+        for (int s = 0; s < 2; s++) {
+            Channel<T> &c = ch_[s];
+            if (c.k == 0) continue;
+            sp.nb = c.k;
+            stencil<T>(ST_LAP, sp, c.X.template as<T>(), c.TMP.template as<T>(), nullptr, nullptr, nullptr, stream_);
+            for (int b0 = 0; b0 < c.k; b0 += MAX_JOBS) {
+                RedJobs<T> jk{};
+                for (int b = b0; b < std::min(c.k, b0 + MAX_JOBS); b++) {
+                    jk.a[b - b0] = c.X.template as<T>() + (long)b * N_;
+                    jk.b[b - b0] = c.TMP.template as<T>() + (long)b * N_;
+                    jk.kind[b - b0] = RED_DOT;
+                    jk.count = b - b0 + 1;
+                }
+                auto tv = reduce(jk, dv_);
+                for (int b = b0; b < std::min(c.k, b0 + MAX_JOBS); b++) {
+                    const double f = fd ? c.occ[b] : 1.0;
+                    if (fd && f == 0.0) continue;
+                    terms.push_back(fd ? f * tv[b - b0] : tv[b - b0]);
+                }
+            }
+        }
+        kinetic = neumaier(terms);
+    }
+    convert<double, T>(rho64_.as<double>(), rho_t_.template as<T>(), N_, stream_);
+    RedJobs<double> je{};
+    je.a[0] = rho64_.as<double>();
+    je.b[0] = vh_.as<double>();
+    je.kind[0] = RED_DOT;
+    je.count = 1;
+    reduce_jobs<double>(je, n_, dv_, partials_.as<double>(), dres_.as<double>() + 32, stream_);
+    RedJobs<T> jt{};
+    jt.a[0] = rho_t_.template as<T>();
+    jt.b[0] = vnuc_.template as<T>();
+    jt.kind[0] = RED_DOT;
+    jt.a[1] = exc_.template as<T>();
+    jt.kind[1] = RED_SUM;
+    jt.a[2] = rho_out_[0].template as<T>();
+    jt.kind[2] = RED_SUM;
+    jt.a[3] = rho_out_[1].template as<T>();
+    jt.kind[3] = RED_SUM;
+    jt.a[4] = rho_t_.template as<T>();
+    jt.kind[4] = RED_FACEMAX;
+    jt.count = 5;
+    for (int i = 1; i < 5; i++) jt.b[i] = jt.a[i];
+    reduce_jobs<T>(jt, n_, dv_, partials_.as<double>(), dres_.as<double>(), stream_);
+    const double *ev = readback(dres_.as<double>(), 33);
+    const double external_ = ev[0], xc_e = ev[1], hartree = 0.5 * ev[32];
+    out->electrons[0] = ev[2];
+    out->electrons[1] = ev[3];
+    out->boundary_density = ev[4];
+    const double internal = kinetic + external_ + hartree + xc_e + e_nn;
+    const double entropy = neumaier({occ[0].entropy, occ[1].entropy});
+    const double ts = kt * entropy;
+    out->energy = lira_energy{kinetic, external_, hartree, xc_e, e_nn, internal, ts, internal - ts};
+    out->entropy = entropy;
+    out->mu[0] = occ[0].mu;
+    out->mu[1] = occ[1].mu;
+    out->poisson_residual = max_pres;
+    out->poisson_iterations = max_pit;
+    out->n_stored[0] = ch_[0].k;
+    out->n_stored[1] = ch_[1].k;
+    out->cg_iterations = cg_total_;
+    out->readbacks = readbacks_;
+    out->seconds_poisson = t_poisson;
+    out->seconds_eigen = t_eigen;
+    out->seconds_total = secs(t0);
+    out->seconds_other = out->seconds_total - t_poisson - t_eigen;
+    valid_ = true;
+    nel_[0] = nel[0];
+    nel_[1] = nel[1];
+    occ_scheme_ = st->occupations;
+    kt_ = st->kt;
+    return LIRA_OK;
+}
+
+template <typename T> void Solver<T>::forces(const lira_nucleus *nuc, int nat, double sigma, double *out) {
+    if (!valid_) throw InvalidError("lira_forces: no converged state");
+    if (std::fabs(sigma - sigma_) > 1e-15) throw InvalidError("lira_forces: sigma differs from the solve's");
+    upload_nuclei(nuc, nat);
+    add2<T>(rho_t_.template as<T>(), rho_out_[0].template as<T>(), rho_out_[1].template as<T>(), N_, stream_);
+    hf_forces<T>(rho_t_.template as<T>(), n_, h_, pos_.as<double>(), zf_.as<double>(), nat, sigma, partials_.as<double>(),
+                 dres_.as<double>(), stream_);
+    const double *f = readback(dres_.as<double>(), (size_t)3 * nat);
+    std::vector<double> fo(f, f + 3 * nat);
+    // nucleus-nucleus part (qm/src/forces.rs:224-234)
+    for (int i = 0; i < nat; i++)
+        for (int j = 0; j < nat; j++) {
+            if (i == j) continue;
+            double d[3], d2 = 0.0;
+            for (int a = 0; a < 3; a++) {
+                d[a] = nuc[i].pos[a] - nuc[j].pos[a];
+                d2 += d[a] * d[a];
+            }
+            const double dd = std::sqrt(d2);
+            if (dd < 1e-8) throw std::pair<int, int>(std::min(i, j), std::max(i, j));
+            const double s = (double)nuc[i].z * nuc[j].z / (dd * dd * dd);
+            for (int a = 0; a < 3; a++) fo[3 * i + a] += s * d[a];
+        }
+    std::memcpy(out, fo.data(), fo.size() * sizeof(double));
+}
+
+template <typename T> void Solver<T>::get_field(int which, double *out) {
+    if (!valid_ && which != 4) throw InvalidError("lira_get_field: no converged state");
+    if (rho64_.bytes < N_ * sizeof(double)) rho64_.alloc(N_ * sizeof(double));
+    double *d = rho64_.as<double>();
+    switch (which) {
+        case 0: convert<double, T>(d, rho_out_[0].template as<T>(), N_, stream_); break;
+        case 1: convert<double, T>(d, rho_out_[1].template as<T>(), N_, stream_); break;
+        case 2:
+            add2<T>(rho_t_.template as<T>(), rho_out_[0].template as<T>(), rho_out_[1].template as<T>(), N_, stream_);
+            convert<double, T>(d, rho_t_.template as<T>(), N_, stream_);
+            break;
+        case 3: copy<double>(d, vh_.as<double>(), N_, stream_); break;
+        case 4:
+            if (!vnuc_.ptr) throw InvalidError("lira_get_field: no potential yet");
+            convert<double, T>(d, vnuc_.template as<T>(), N_, stream_);
+            break;
+        default: throw InvalidError("lira_get_field: unknown field");
+    }
+    LIRA_HIP(hipMemcpyAsync(out, d, N_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    LIRA_HIP(hipStreamSynchronize(stream_));
+}
+
+template <typename T> void Solver<T>::get_levels(int spin, double *evals, double *occ, int cap, int *count) {
+    if (spin < 0 || spin > 1) throw InvalidError("lira_get_levels: spin must be 0 or 1");
+    if (!valid_) throw InvalidError("lira_get_levels: no converged state");
+    Channel<T> &c = ch_[spin];
+    *count = c.k;
+    for (int i = 0; i < std::min(cap, c.k); i++) {
+        if (evals) evals[i] = c.evals[i];
+        if (occ) occ[i] = c.occ[i];
+    }
+}
+
+template <typename T> void Solver<T>::get_orbitals(int spin, double *out, int cap, int *count) {
+    if (spin < 0 || spin > 1) throw InvalidError("lira_get_orbitals: spin must be 0 or 1");
+    if (!valid_) throw InvalidError("lira_get_orbitals: no converged state");
+    Channel<T> &c = ch_[spin];
+    *count = c.k;
+    if (rho64_.bytes < N_ * sizeof(double)) rho64_.alloc(N_ * sizeof(double));
+    for (int b = 0; b < std::min(cap, c.k); b++) {
+        convert<double, T>(rho64_.as<double>(), c.X.template as<T>() + (long)b * N_, N_, stream_);
+        LIRA_HIP(hipMemcpyAsync(out + (long)b * N_, rho64_.ptr, N_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        LIRA_HIP(hipStreamSynchronize(stream_));
+    }
+}
+
+template <typename T> PoissonOut Solver<T>::poisson_host(const double *rho, double *v, double tol, int max_iter) {
+    alloc_field(rho_t_);
+    if (vh_.bytes < N_ * sizeof(double)) vh_.alloc(N_ * sizeof(double));
+    if (rho64_.bytes < N_ * sizeof(double)) rho64_.alloc(N_ * sizeof(double));
+    LIRA_HIP(hipMemcpyAsync(rho64_.ptr, rho, N_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+    convert<T, double>(rho_t_.template as<T>(), rho64_.as<double>(), N_, stream_);
+    LIRA_HIP(hipMemcpyAsync(vh_.ptr, v, N_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+    cg_total_ = 0;
+    PoissonOut po = poisson(rho_t_.template as<T>(), vh_.as<double>(), tol, max_iter);
+    LIRA_HIP(hipMemcpyAsync(v, vh_.ptr, N_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    LIRA_HIP(hipStreamSynchronize(stream_));
+    valid_ = false;
+    return po;
+}
+
+template <typename T>
+EigenOut Solver<T>::lobpcg_host(const double *v, int k, double *x, double *evals, double lam, double tol,
+                                int max_iter) {
+    if (k <= 0) throw InvalidError("lira_lobpcg: k must be positive");
+    alloc_field(veff_[0]);
+    if (rho64_.bytes < N_ * sizeof(double)) rho64_.alloc(N_ * sizeof(double));
+    LIRA_HIP(hipMemcpyAsync(rho64_.ptr, v, N_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+    convert<T, double>(veff_[0].template as<T>(), rho64_.as<double>(), N_, stream_);
+    Channel<T> &c = ch_[0];
+    c.k = 0;
+    ensure_channel(c, k);
+    for (int b = 0; b < k; b++) {
+        LIRA_HIP(hipMemcpyAsync(rho64_.ptr, x + (long)b * N_, N_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+        convert<T, double>(c.X.template as<T>() + (long)b * N_, rho64_.as<double>(), N_, stream_);
+    }
+    c.k = k;
+    valid_ = false;
+    EigenOut eo = lobpcg(c, veff_[0].template as<T>(), k, lam, tol, max_iter, tol, k);
+    for (int b = 0; b < k; b++) {
+        convert<double, T>(rho64_.as<double>(), c.X.template as<T>() + (long)b * N_, N_, stream_);
+        LIRA_HIP(hipMemcpyAsync(x + (long)b * N_, rho64_.ptr, N_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        LIRA_HIP(hipStreamSynchronize(stream_));
+    }
+    for (int i = 0; i < k; i++) evals[i] = c.evals[i];
+    return eo;
+}
+
+}  // namespace lira
+
+// ========================================================================================== C ABI ====
+using namespace lira;
+
+struct lira_solver {
+    std::unique_ptr<SolverIface> impl;
+    std::string err;
+    int n = 0;
+    double h = 0.0;
+};
+
+static thread_local std::string g_create_error;
+
+template <typename F> static lira_status guarded(lira_solver *s, F &&f) {
+    try {
+        f();
+        if (s) s->err.clear();
+        return LIRA_OK;
+    } catch (const std::pair<int, int> &p) {
+        if (s) s->err = "nuclei " + std::to_string(p.first) + " and " + std::to_string(p.second) + " coincide";
+        return LIRA_COINCIDENT_NUCLEI;
+    } catch (const NumericalError &e) {
+        if (s) {
+            s->err = std::string("numerical failure in the solver: ") + e.what();
+            if (s->impl) s->impl->reset();
+        }
+        return LIRA_NUMERICAL;
+    } catch (const DeviceError &e) {
+        if (s) {
+            s->err = std::string("device failure: ") + e.what();
+            if (s->impl) s->impl->reset();
+        }
+        return LIRA_DEVICE;
+    } catch (const InvalidError &e) {
+        if (s) s->err = e.what();
+        return LIRA_INVALID;
+    } catch (const std::exception &e) {
+        if (s) {
+            s->err = e.what();
+            if (s->impl) s->impl->reset();
+        }
+        return LIRA_DEVICE;
+    }
+}
+
+extern "C" {
+
+const char *lira_version(void) { return "liblira 0.1 (gfx950)"; }
+
+void lira_default_settings(double h, lira_settings *out) {
+    out->sigma = h;
+    out->tol = 1e-6;
+    out->max_iter = 200;
+    out->mixing = 0.3;
+    out->eig_tol = 1e-6;
+    out->diis_history = 6;
+    out->occupations = LIRA_FERMI_DIRAC;
+    out->kt = 5e-3;
+    out->stall_window = 50;
+}
+
+lira_status lira_solver_create(int n, double h, int precision, lira_solver **out) {
+    if (!out) return LIRA_INVALID;
+    *out = nullptr;
+    if (n < 3 || n > 2048 || !(h > 0.0) || (precision != LIRA_FP64 && precision != LIRA_MIXED)) {
+        g_create_error = "lira_solver_create: bad grid or precision";
+        return LIRA_INVALID;
+    }
+    auto s = std::make_unique<lira_solver>();
+    s->n = n;
+    s->h = h;
+    lira_status st = guarded(s.get(), [&] {
+        if (precision == LIRA_FP64) s->impl = std::make_unique<Solver<double>>(n, h);
+        else s->impl = std::make_unique<Solver<float>>(n, h);
+    });
+    if (st != LIRA_OK) {
+        g_create_error = s->err;
+        return st;
+    }
+    *out = s.release();
+    return LIRA_OK;
+}
+
+void lira_solver_destroy(lira_solver *s) { delete s; }
+
+const char *lira_last_error(const lira_solver *s) { return s ? s->err.c_str() : g_create_error.c_str(); }
+
+lira_status lira_solve(lira_solver *s, const lira_system *sys, const lira_settings *settings, int warm,
+                       lira_abort_fn abort, void *user, lira_scf_stats *stats) {
+    if (!s || !sys || !settings || !stats) return LIRA_INVALID;
+    lira_status result = LIRA_OK;
+    lira_status g = guarded(s, [&] { result = s->impl->solve(sys, settings, warm, abort, user, stats); });
+    if (g != LIRA_OK) return g;
+    if (result == LIRA_NOT_CONVERGED)
+        s->err = "SCF did not converge after " + std::to_string(stats->iterations) + " iterations";
+    else if (result == LIRA_STALLED)
+        s->err = "SCF stalled after " + std::to_string(stats->iterations) + " iterations";
+    else if (result == LIRA_ABORTED)
+        s->err = "aborted";
+    return result;
+}
+
+lira_status lira_forces(lira_solver *s, const lira_nucleus *nuclei, int n_nuclei, double sigma, double *out) {
+    if (!s || !nuclei || !out || n_nuclei <= 0) return LIRA_INVALID;
+    return guarded(s, [&] { s->impl->forces(nuclei, n_nuclei, sigma, out); });
+}
+
+lira_status lira_get_field(lira_solver *s, int which, double *out) {
+    if (!s || !out) return LIRA_INVALID;
+    return guarded(s, [&] { s->impl->get_field(which, out); });
+}
+
+lira_status lira_get_levels(lira_solver *s, int spin, double *evals, double *occ, int cap, int *count) {
+    if (!s || !count) return LIRA_INVALID;
+    return guarded(s, [&] { s->impl->get_levels(spin, evals, occ, cap, count); });
+}
+
+lira_status lira_get_orbitals(lira_solver *s, int spin, double *out, int cap, int *count) {
+    if (!s || !out || !count) return LIRA_INVALID;
+    return guarded(s, [&] { s->impl->get_orbitals(spin, out, cap, count); });
+}
+
+double lira_effective_eig_tol(const lira_solver *s, double requested) {
+    return s ? s->impl->effective_eig_tol(requested) : requested;
+}
+
+void lira_reset(lira_solver *s) {
+    if (s) s->impl->reset();
+}
+
+lira_status lira_poisson(lira_solver *s, const double *rho, double *v, double tol, int max_iter,
+                         lira_poisson_stats *stats) {
+    if (!s || !rho || !v || !(tol > 0.0) || max_iter < 1) return LIRA_INVALID;
+    return guarded(s, [&] {
+        PoissonOut po = s->impl->poisson_host(rho, v, tol, max_iter);
+        if (stats) *stats = lira_poisson_stats{po.iterations, po.outer, po.residual, po.seconds};
+    });
+}
+
+lira_status lira_lobpcg(lira_solver *s, const double *v, int k, double *x, double *evals, double precond_lambda,
+                        double tol, int max_iter, lira_eigen_stats *stats) {
+    if (!s || !v || !x || !evals || k < 1 || !(tol > 0.0)) return LIRA_INVALID;
+    return guarded(s, [&] {
+        EigenOut eo = s->impl->lobpcg_host(v, k, x, evals, precond_lambda, tol, max_iter);
+        if (stats) *stats = lira_eigen_stats{eo.iterations, eo.max_residual, eo.fallbacks};
+    });
+}
+
+lira_status lira_rayleigh_ritz(int m, const double *a, const double *b, int nk, double *theta, double *c) {
+    if (m < 1 || nk < 1 || nk > m || !a || !b || !theta || !c) return LIRA_INVALID;
+    try {
+        std::vector<double> A(a, a + (size_t)m * m), B(b, b + (size_t)m * m), th, C;
+        rayleigh_ritz(m, A, B, nk, 1e-6, th, C);
+        std::memcpy(theta, th.data(), nk * sizeof(double));
+        std::memcpy(c, C.data(), (size_t)m * nk * sizeof(double));
+        return LIRA_OK;
+    } catch (const NumericalError &) {
+        return LIRA_NUMERICAL;
+    }
+}
+
+static hipStream_t as_stream(void *s) { return static_cast<hipStream_t>(s); }
+
+lira_status lira_hamiltonian(int dtype, int n, double h, int nbands, const void *v, const void *x, void *hx,
+                             void *stream) {
+    if ((dtype != 4 && dtype != 8) || n < 1 || nbands < 0 || !v || !x || !hx || x == hx) return LIRA_INVALID;
+    return guarded(nullptr, [&] {
+        StencilParams sp{n, nbands, h, 1.0, 0.0, 0.0};
+        if (dtype == 4)
+            stencil<float>(ST_HAM, sp, (const float *)x, (float *)hx, (const float *)v, nullptr, nullptr, as_stream(stream));
+        else
+            stencil<double>(ST_HAM, sp, (const double *)x, (double *)hx, (const double *)v, nullptr, nullptr,
+                            as_stream(stream));
+    });
+}
+
+lira_status lira_laplacian(int dtype, int n, double h, int nbands, double scale, const void *x, void *out,
+                           void *stream) {
+    if ((dtype != 4 && dtype != 8) || n < 1 || nbands < 0 || !x || !out || x == out) return LIRA_INVALID;
+    return guarded(nullptr, [&] {
+        StencilParams sp{n, nbands, h, scale, 0.0, 0.0};
+        if (dtype == 4)
+            stencil<float>(ST_LAP, sp, (const float *)x, (float *)out, nullptr, nullptr, nullptr, as_stream(stream));
+        else
+            stencil<double>(ST_LAP, sp, (const double *)x, (double *)out, nullptr, nullptr, nullptr, as_stream(stream));
+    });
+}
+
+lira_status lira_jacobi_sweep(int dtype, int n, double h, int nbands, const void *v, const void *w, const void *r,
+                              const double *lam, void *out, void *stream) {
+    if ((dtype != 4 && dtype != 8) || n < 1 || nbands < 0 || !v || !w || !r || !lam || !out || w == out)
+        return LIRA_INVALID;
+    return guarded(nullptr, [&] {
+        StencilParams sp{n, nbands, h, 1.0, jacobi_omega(), -1.5 * COEFF0 / (h * h)};
+        if (dtype == 4)
+            stencil<float>(ST_JAC, sp, (const float *)w, (float *)out, (const float *)v, (const float *)r, lam,
+                           as_stream(stream));
+        else
+            stencil<double>(ST_JAC, sp, (const double *)w, (double *)out, (const double *)v, (const double *)r, lam,
+                            as_stream(stream));
+    });
+}
+
+lira_status lira_xc_lsda(int dtype, long count, const void *nu, const void *nd, void *e, void *vu, void *vd,
+                         void *stream) {
+    if ((dtype != 4 && dtype != 8) || count < 0 || !nu || !nd || !e || !vu || !vd) return LIRA_INVALID;
+    return guarded(nullptr, [&] {
+        if (dtype == 4)
+            xc_lsda<float>((const float *)nu, (const float *)nd, (float *)e, (float *)vu, (float *)vd, count,
+                           as_stream(stream));
+        else
+            xc_lsda<double>((const double *)nu, (const double *)nd, (double *)e, (double *)vu, (double *)vd, count,
+                            as_stream(stream));
+    });
+}
+
+lira_status lira_nuclear_potential(int dtype, int n, double h, const lira_nucleus *nuclei, int n_nuclei, double sigma,
+                                   void *out, void *stream) {
+    if ((dtype != 4 && dtype != 8) || n < 1 || n_nuclei < 0 || !out || !(sigma > 0.0)) return LIRA_INVALID;
+    return guarded(nullptr, [&] {
+        std::vector<double> ph(3 * n_nuclei + 1), zh(n_nuclei + 1);
+        for (int i = 0; i < n_nuclei; i++) {
+            for (int a = 0; a < 3; a++) ph[3 * i + a] = nuclei[i].pos[a];
+            zh[i] = nuclei[i].z;
+        }
+        DevBuf dp(ph.size() * sizeof(double)), dz(zh.size() * sizeof(double));
+        LIRA_HIP(hipMemcpy(dp.ptr, ph.data(), ph.size() * sizeof(double), hipMemcpyHostToDevice));
+        LIRA_HIP(hipMemcpy(dz.ptr, zh.data(), zh.size() * sizeof(double), hipMemcpyHostToDevice));
+        if (dtype == 4)
+            nuclear_potential<float>((float *)out, n, h, dp.as<double>(), dz.as<double>(), n_nuclei, sigma,
+                                     as_stream(stream));
+        else
+            nuclear_potential<double>((double *)out, n, h, dp.as<double>(), dz.as<double>(), n_nuclei, sigma,
+                                      as_stream(stream));
+        LIRA_HIP(hipStreamSynchronize(as_stream(stream)));
+    });
+}
+
+lira_status lira_gram(int dtype, int n, double h, int m, const void *s_block, int q, const void *u_block, double *g,
+                      void *stream) {
+    if ((dtype != 4 && dtype != 8) || n < 1 || m < 1 || q < 1 || !s_block || !u_block || !g) return LIRA_INVALID;
+    return guarded(nullptr, [&] {
+        const long N = (long)n * n * n;
+        const size_t need = gram_partials_needed(m, q, N);
+        DevBuf part(need * sizeof(double)), dg((size_t)m * q * sizeof(double));
+        Cols S{{{s_block, m}}, 1}, U{{{u_block, q}}, 1};
+        if (dtype == 4) gram<float>(S, U, N, h * h * h, part.as<double>(), need, dg.as<double>(), as_stream(stream));
+        else gram<double>(S, U, N, h * h * h, part.as<double>(), need, dg.as<double>(), as_stream(stream));
+        LIRA_HIP(hipMemcpyAsync(g, dg.ptr, (size_t)m * q * sizeof(double), hipMemcpyDeviceToHost, as_stream(stream)));
+        LIRA_HIP(hipStreamSynchronize(as_stream(stream)));
+    });
+}
+
+int lira_stencil_variant_count(void) { return stencil_variants(); }
+const char *lira_stencil_variant_name(int v) { return stencil_variant_name(v); }
+void lira_set_stencil_variant(int dtype, int variant) { stencil_set_default_variant(dtype, variant); }
+
+int lira_autotune_stencil(int dtype, int n, int nbands, double *ms_out) {
+    if ((dtype != 4 && dtype != 8) || n < 1 || nbands < 1) return -1;
+    int best = -1;
+    double best_ms = 1e300;
+    lira_status st = guarded(nullptr, [&] {
+        const long N = (long)n * n * n;
+        const size_t es = dtype;
+        DevBuf x(N * nbands * es), o(N * nbands * es), v(N * es);
+        LIRA_HIP(hipMemset(x.ptr, 0, N * nbands * es));
+        LIRA_HIP(hipMemset(v.ptr, 0, N * es));
+        hipEvent_t a, b;
+        LIRA_HIP(hipEventCreate(&a));
+        LIRA_HIP(hipEventCreate(&b));
+        StencilParams sp{n, nbands, 0.2, 1.0, 0.0, 0.0};
+        for (int var = 0; var < stencil_variants(); var++) {
+            auto run = [&] {
+                if (dtype == 4) stencil<float>(ST_HAM, sp, x.as<float>(), o.as<float>(), v.as<float>(), nullptr, nullptr, nullptr, var);
+                else stencil<double>(ST_HAM, sp, x.as<double>(), o.as<double>(), v.as<double>(), nullptr, nullptr, nullptr, var);
+            };
+            run();
+            LIRA_HIP(hipEventRecord(a, nullptr));
+            for (int r = 0; r < 3; r++) run();
+            LIRA_HIP(hipEventRecord(b, nullptr));
+            LIRA_HIP(hipEventSynchronize(b));
+            float ms = 0.f;
+            LIRA_HIP(hipEventElapsedTime(&ms, a, b));
+            if (ms / 3.0 < best_ms) {
+                best_ms = ms / 3.0;
+                best = var;
+            }
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        stencil_set_default_variant(dtype, best);
+    });
+    if (st != LIRA_OK) return -1;
+    if (ms_out) *ms_out = best_ms;
+    return best;
+}
+
+}  // extern "C"

@@ -1,0 +1,337 @@
+"""Python binding of liblira (ctypes), mirroring the reference's `qm_gpu::GpuSolver` interface.
+
+`GpuSolver(n, h, precision)` wraps `lira_solver_create` (crates/qm-gpu/src/scf.rs:1083 `GpuSolver::new`);
+`solve` is `solve_polled` (scf.rs:1195-1235), `forces` is `GpuSolver::forces` (forces.rs:239), `state()` returns the
+converged densities, potentials and levels (scf.rs:1172, 1763-1851). Errors raise `QmError` subclasses named after
+the reference's variants (crates/qm/src/error.rs:1-33). All compute runs in liblira's HIP kernels on the GPU; this
+module only marshals arguments. It refuses to run without the built library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblira.so")
+_lib = None
+
+FP64, MIXED = 0, 1
+AUFBAU, FERMI_DIRAC = 0, 1
+OK, COINCIDENT_NUCLEI, NOT_CONVERGED, STALLED, NUMERICAL, DEVICE, INVALID, ABORTED = range(8)
+
+#: every symbol include/lira.h declares
+EXPORTS = [
+    "lira_default_settings", "lira_solver_create", "lira_solver_destroy", "lira_last_error", "lira_solve",
+    "lira_forces", "lira_get_field", "lira_get_levels", "lira_get_orbitals", "lira_effective_eig_tol", "lira_reset",
+    "lira_poisson", "lira_lobpcg", "lira_rayleigh_ritz", "lira_hamiltonian", "lira_laplacian", "lira_jacobi_sweep",
+    "lira_xc_lsda", "lira_nuclear_potential", "lira_gram", "lira_stencil_variant_count", "lira_stencil_variant_name",
+    "lira_autotune_stencil", "lira_set_stencil_variant", "lira_version",
+]
+
+
+class QmError(RuntimeError):
+    status = -1
+
+
+class CoincidentNuclei(QmError):
+    status = COINCIDENT_NUCLEI
+
+
+class ScfNotConverged(QmError):
+    status = NOT_CONVERGED
+
+
+class ScfStalled(QmError):
+    status = STALLED
+
+
+class Numerical(QmError):
+    status = NUMERICAL
+
+
+class Device(QmError):
+    status = DEVICE
+
+
+class Invalid(QmError):
+    status = INVALID
+
+
+class Aborted(QmError):
+    status = ABORTED
+
+
+_ERRORS = {c.status: c for c in (CoincidentNuclei, ScfNotConverged, ScfStalled, Numerical, Device, Invalid, Aborted)}
+
+
+class Nucleus(C.Structure):
+    _fields_ = [("pos", C.c_double * 3), ("z", C.c_int)]
+
+
+class System(C.Structure):
+    _fields_ = [("nuclei", C.POINTER(Nucleus)), ("n_nuclei", C.c_int), ("n_up", C.c_int), ("n_dn", C.c_int)]
+
+
+class Settings(C.Structure):
+    _fields_ = [("sigma", C.c_double), ("tol", C.c_double), ("max_iter", C.c_int), ("mixing", C.c_double),
+                ("eig_tol", C.c_double), ("diis_history", C.c_int), ("occupations", C.c_int), ("kt", C.c_double),
+                ("stall_window", C.c_int)]
+
+
+class Energy(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("kinetic", "external", "hartree", "xc", "nuclear", "internal", "ts",
+                                           "total")]
+
+
+class ScfStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("delta_rho", C.c_double), ("energy", Energy),
+                ("boundary_density", C.c_double), ("max_eigen_residual", C.c_double),
+                ("poisson_residual", C.c_double), ("poisson_iterations", C.c_int), ("entropy", C.c_double),
+                ("mu", C.c_double * 2), ("electrons", C.c_double * 2), ("tol_used", C.c_double),
+                ("n_stored", C.c_int * 2), ("restricted", C.c_int), ("lobpcg_iterations", C.c_int),
+                ("cg_iterations", C.c_int), ("readbacks", C.c_int), ("seconds_total", C.c_double),
+                ("seconds_poisson", C.c_double), ("seconds_eigen", C.c_double), ("seconds_other", C.c_double)]
+
+
+class PoissonStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("outer", C.c_int), ("residual", C.c_double), ("seconds", C.c_double)]
+
+
+class EigenStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("max_residual", C.c_double), ("fallbacks", C.c_int)]
+
+
+ABORT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    """Loads liblira.so; raises if it has not been built (there is no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"liblira.so is not built at {LIB_PATH}: run __graft_entry__.build() (make -C "
+                               f"atomic-sandbox_amd); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.lira_default_settings.argtypes = [C.c_double, C.POINTER(Settings)]
+        L.lira_default_settings.restype = None
+        L.lira_solver_create.argtypes = [C.c_int, C.c_double, C.c_int, C.POINTER(vp)]
+        L.lira_solver_destroy.argtypes = [vp]
+        L.lira_solver_destroy.restype = None
+        L.lira_last_error.argtypes = [vp]
+        L.lira_last_error.restype = C.c_char_p
+        L.lira_solve.argtypes = [vp, C.POINTER(System), C.POINTER(Settings), C.c_int, ABORT_FN, vp,
+                                 C.POINTER(ScfStats)]
+        L.lira_forces.argtypes = [vp, C.POINTER(Nucleus), C.c_int, C.c_double, _dp]
+        L.lira_get_field.argtypes = [vp, C.c_int, _dp]
+        L.lira_get_levels.argtypes = [vp, C.c_int, _dp, _dp, C.c_int, C.POINTER(C.c_int)]
+        L.lira_get_orbitals.argtypes = [vp, C.c_int, _dp, C.c_int, C.POINTER(C.c_int)]
+        L.lira_effective_eig_tol.argtypes = [vp, C.c_double]
+        L.lira_effective_eig_tol.restype = C.c_double
+        L.lira_reset.argtypes = [vp]
+        L.lira_reset.restype = None
+        L.lira_poisson.argtypes = [vp, _dp, _dp, C.c_double, C.c_int, C.POINTER(PoissonStats)]
+        L.lira_lobpcg.argtypes = [vp, _dp, C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_int, C.POINTER(EigenStats)]
+        L.lira_rayleigh_ritz.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp]
+        L.lira_hamiltonian.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, vp, vp, vp, vp]
+        L.lira_laplacian.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, vp, vp, vp]
+        L.lira_jacobi_sweep.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, vp, vp, vp, vp, vp, vp]
+        L.lira_xc_lsda.argtypes = [C.c_int, C.c_long, vp, vp, vp, vp, vp, vp]
+        L.lira_nuclear_potential.argtypes = [C.c_int, C.c_int, C.c_double, C.POINTER(Nucleus), C.c_int, C.c_double,
+                                             vp, vp]
+        L.lira_gram.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, vp, C.c_int, vp, _dp, vp]
+        L.lira_stencil_variant_count.restype = C.c_int
+        L.lira_stencil_variant_name.argtypes = [C.c_int]
+        L.lira_stencil_variant_name.restype = C.c_char_p
+        L.lira_autotune_stencil.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.lira_autotune_stencil.restype = C.c_int
+        L.lira_set_stencil_variant.argtypes = [C.c_int, C.c_int]
+        L.lira_set_stencil_variant.restype = None
+        L.lira_version.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _raise(status: int, handle=None, what: str = ""):
+    if status == OK:
+        return
+    msg = lib().lira_last_error(handle).decode() if handle is not None else ""
+    raise _ERRORS.get(status, QmError)(f"{what}: {msg}" if what else msg)
+
+
+def default_settings(h: float, occupations: int = FERMI_DIRAC, **kw) -> Settings:
+    """ScfSettings::for_grid (crates/qm/src/scf.rs:380-385); keyword arguments override fields."""
+    s = Settings()
+    lib().lira_default_settings(h, C.byref(s))
+    s.occupations = occupations
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+def nuclei_array(pos, z):
+    pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+    arr = (Nucleus * len(pos))()
+    for i, (p, zz) in enumerate(zip(pos, z)):
+        arr[i].pos[:] = [float(p[0]), float(p[1]), float(p[2])]
+        arr[i].z = int(zz)
+    return arr
+
+
+@dataclass
+class State:
+    """A converged Kohn–Sham state copied to the host (f64, qm layout)."""
+    rho_up: np.ndarray
+    rho_dn: np.ndarray
+    v_h: np.ndarray
+    evals_up: np.ndarray
+    evals_dn: np.ndarray
+    occ_up: np.ndarray
+    occ_dn: np.ndarray
+    extra: dict = field(default_factory=dict)
+
+
+class GpuSolver:
+    """One n^3 grid at spacing h on the current HIP device; precision FP64 or MIXED."""
+
+    def __init__(self, n: int, h: float, precision: int = MIXED):
+        handle = C.c_void_p()
+        st = lib().lira_solver_create(n, h, precision, C.byref(handle))
+        if st != OK:
+            _raise(st, None, "lira_solver_create")
+        self._h = handle
+        self.n, self.h, self.precision = n, h, precision
+        self.N = n ** 3
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lira_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, pos, z, n_up, n_dn, settings: Settings | None = None, warm=False, abort=None) -> ScfStats:
+        nuc = nuclei_array(pos, z)
+        sysd = System(C.cast(nuc, C.POINTER(Nucleus)), len(nuc), n_up, n_dn)
+        s = settings if settings is not None else default_settings(self.h)
+        stats = ScfStats()
+        cb = ABORT_FN(lambda _u: int(bool(abort()))) if abort else ABORT_FN(0)
+        st = lib().lira_solve(self._h, C.byref(sysd), C.byref(s), int(warm), cb, None, C.byref(stats))
+        _raise(st, self._h, "lira_solve")
+        return stats
+
+    def forces(self, pos, z, sigma) -> np.ndarray:
+        nuc = nuclei_array(pos, z)
+        out = np.zeros(3 * len(nuc))
+        _raise(lib().lira_forces(self._h, nuc, len(nuc), sigma, out), self._h, "lira_forces")
+        return out.reshape(-1, 3)
+
+    def field(self, which: int) -> np.ndarray:
+        out = np.zeros(self.N)
+        _raise(lib().lira_get_field(self._h, which, out), self._h, "lira_get_field")
+        return out
+
+    def levels(self, spin: int):
+        ev, oc = np.zeros(256), np.zeros(256)
+        cnt = C.c_int()
+        _raise(lib().lira_get_levels(self._h, spin, ev, oc, 256, C.byref(cnt)), self._h, "lira_get_levels")
+        return ev[:cnt.value].copy(), oc[:cnt.value].copy()
+
+    def orbitals(self, spin: int, cap: int = 64) -> np.ndarray:
+        out = np.zeros(cap * self.N)
+        cnt = C.c_int()
+        _raise(lib().lira_get_orbitals(self._h, spin, out, cap, C.byref(cnt)), self._h, "lira_get_orbitals")
+        return out[:min(cap, cnt.value) * self.N].reshape(-1, self.N)
+
+    def state(self) -> State:
+        eu, ou = self.levels(0)
+        ed, od = self.levels(1)
+        return State(self.field(0), self.field(1), self.field(3), eu, ed, ou, od)
+
+    def effective_eig_tol(self, requested: float) -> float:
+        return lib().lira_effective_eig_tol(self._h, requested)
+
+    def reset(self):
+        lib().lira_reset(self._h)
+
+    def poisson(self, rho, v0=None, tol=1e-10, max_iter=5000):
+        v = np.zeros(self.N) if v0 is None else np.ascontiguousarray(v0, dtype=np.float64).copy()
+        st = PoissonStats()
+        _raise(lib().lira_poisson(self._h, np.ascontiguousarray(rho, dtype=np.float64), v, tol, max_iter,
+                                  C.byref(st)), self._h, "lira_poisson")
+        return v, st
+
+    def lobpcg(self, v, x0, tol, max_iter=500, precond_lambda=-1.0):
+        x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+        k = x.shape[0]
+        ev = np.zeros(k)
+        st = EigenStats()
+        _raise(lib().lira_lobpcg(self._h, np.ascontiguousarray(v, dtype=np.float64), k, x, ev, precond_lambda, tol,
+                                 max_iter, C.byref(st)), self._h, "lira_lobpcg")
+        return x, ev, st
+
+
+def rayleigh_ritz(a, b, nk):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    m = a.shape[0]
+    th, c = np.zeros(nk), np.zeros(m * nk)
+    _raise(lib().lira_rayleigh_ritz(m, a, b, nk, th, c), None, "lira_rayleigh_ritz")
+    return th, c.reshape(m, nk)
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (torch is only the allocator here)."""
+    return C.c_void_p(t.data_ptr())
+
+
+def _dtype(t):
+    return 4 if t.element_size() == 4 else 8
+
+
+def hamiltonian(n, h, v, x, hx, nbands, stream=None):
+    _raise(lib().lira_hamiltonian(_dtype(x), n, h, nbands, _ptr(v), _ptr(x), _ptr(hx), stream), None, "hamiltonian")
+
+
+def laplacian(n, h, x, out, nbands, scale=1.0, stream=None):
+    _raise(lib().lira_laplacian(_dtype(x), n, h, nbands, scale, _ptr(x), _ptr(out), stream), None, "laplacian")
+
+
+def jacobi_sweep(n, h, v, w, r, lam, out, nbands, stream=None):
+    _raise(lib().lira_jacobi_sweep(_dtype(w), n, h, nbands, _ptr(v), _ptr(w), _ptr(r), _ptr(lam), _ptr(out), stream),
+           None, "jacobi_sweep")
+
+
+def xc_lsda(nu, nd, e, vu, vd, stream=None):
+    _raise(lib().lira_xc_lsda(_dtype(nu), nu.numel(), _ptr(nu), _ptr(nd), _ptr(e), _ptr(vu), _ptr(vd), stream), None,
+           "xc_lsda")
+
+
+def nuclear_potential(n, h, pos, z, sigma, out, stream=None):
+    nuc = nuclei_array(pos, z)
+    _raise(lib().lira_nuclear_potential(_dtype(out), n, h, nuc, len(nuc), sigma, _ptr(out), stream), None,
+           "nuclear_potential")
+
+
+def gram(n, h, s_block, m, u_block, q, stream=None):
+    g = np.zeros(m * q)
+    _raise(lib().lira_gram(_dtype(s_block), n, h, m, _ptr(s_block), q, _ptr(u_block), g, stream), None, "gram")
+    return g.reshape(m, q)
+
+
+def autotune_stencil(dtype_size, n, nbands):
+    ms = C.c_double()
+    v = lib().lira_autotune_stencil(dtype_size, n, nbands, C.byref(ms))
+    return v, ms.value
+
+
+def stencil_variants():
+    return [lib().lira_stencil_variant_name(i).decode() for i in range(lib().lira_stencil_variant_count())]

@@ -1,0 +1,189 @@
+"""Benchmark of the real-space Kohn–Sham SCF hot path on one MI355X.
+
+One step = one cold SCF of water (H2O, all-electron, O at the origin, O-H 0.96 A, H-O-H 104.5 deg) on a 95^3 grid at
+h = 0.15 Bohr, LSDA, Fermi–Dirac occupations at kT = 5 mHa, converged to delta_rho < 1e-6 in MIXED precision (f32
+grid fields, f64 Rayleigh–Ritz / DIIS / energies). The same SCF in FP64 is timed once for the mixed-vs-FP64 ratio
+and the per-result errors. The roofline object is the block stencil H.X on 192^3 x 64 bands (f32, V ~ U[-5, 0] Ha,
+X ~ N(0, 1)), timed with HIP events on the stream it runs on. The CPU baseline times the f64 oracle (a restatement of
+the reference's single-threaded CPU solver) on a bounded sample of the same water problem.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "atomic-sandbox_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+BOHR_PER_ANGSTROM = 1.8897261  # scenes/aspirin.scene:3
+METRIC = "SCF s to 1e-6 Ha (water 95³) mixed vs FP64 on 1 GPU; H·X GB/s % of HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def water():
+    r = 0.96 * BOHR_PER_ANGSTROM
+    half = math.radians(104.5 / 2)
+    pos = [[0.0, 0.0, 0.0], [r * math.sin(half), r * math.cos(half), 0.0], [-r * math.sin(half), r * math.cos(half), 0.0]]
+    return pos, [8, 1, 1], 5, 5
+
+
+def hx_roofline(torch, lira, n=192, nb=64, reps=20, dtype="float32"):
+    """Times lira_hamiltonian on n^3 x nb bands with HIP events on torch's current stream."""
+    dt = getattr(torch, dtype)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    x = torch.randn((nb, n ** 3), generator=g, device="cuda", dtype=torch.float32).to(dt)
+    v = (torch.rand(n ** 3, generator=g, device="cuda", dtype=torch.float32) * -5.0).to(dt)
+    out = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    esize = x.element_size()
+    tuned, tuned_ms = lira.autotune_stencil(esize, n, nb)
+    for _ in range(3):
+        lira.hamiltonian(n, 0.2, v, x, out, nb, stream=sp)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(reps):
+        lira.hamiltonian(n, 0.2, v, x, out, nb, stream=sp)
+    stop.record(stream)
+    stop.synchronize()
+    ms = start.elapsed_time(stop) / reps
+    N = n ** 3
+    algo_bytes = N * (2 * esize * nb + esize)  # x in, HX out per band, V once per block
+    achieved = algo_bytes / (ms * 1e-3) / 1e9
+    del x, v, out
+    torch.cuda.empty_cache()
+    return {"ms": ms, "bytes": algo_bytes, "gbs": achieved, "variant": lira.stencil_variants()[tuned],
+            "frac": achieved / HBM_PEAK_GBS}
+
+
+def cpu_baseline_sample(state, gpu_fp64, n, h, pos, z, threads):
+    """Bounded oracle sample on the same water problem: one f64 Poisson solve of the converged density and two
+    LOBPCG iterations on the converged potential, extrapolated with the SCF's own step and iteration counts."""
+    import ctypes
+
+    import oracle as O
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    rho, vh, vnuc = state
+    t0 = time.perf_counter()
+    _, it_p, _ = O.poisson_solve(n, h, rho, v0=vh * 0.0, tol=1e-10, max_iter=5000)
+    t_poisson = time.perf_counter() - t0
+    v = vnuc + vh
+    k = gpu_fp64["k"]
+    guess = O.initial_orbitals(n, h, pos, k, 1)
+    t0 = time.perf_counter()
+    O.lobpcg(n, h, v, guess, 1e-30, 0, precond_lambda=-1.0)
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.lobpcg(n, h, v, guess, 1e-30, 2, precond_lambda=-1.0)
+    t_two = time.perf_counter() - t0
+    t_iter = max(t_two - t_setup, 0.0) / 2
+    steps = gpu_fp64["steps"]
+    iters = gpu_fp64["lobpcg_iterations"] * 2  # the CPU solver solves both spin channels
+    est = steps * (t_poisson + 2 * t_setup) + iters * t_iter
+    return {"value": est, "unit": "s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C f64 restatement of crates/qm) on the same water 95^3 problem: one Poisson solve "
+                      f"({it_p} CG it, {t_poisson:.2f} s) and 2 LOBPCG iterations of k={k} ({t_iter:.2f} s/it, "
+                      f"{t_setup:.2f} s setup), extrapolated to {steps} SCF steps x 2 spins and {iters} LOBPCG "
+                      f"iterations (the FP64 GPU run's counts)",
+            "measured_seconds": t_poisson + t_setup + t_two}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-roofline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs the SCF on a single MI355X (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    torch.cuda.set_device(0)
+    import lira
+
+    n, h = 95, 0.15
+    pos, z, nu, nd = water()
+    settings = lira.default_settings(h, lira.FERMI_DIRAC)
+    solver = lira.GpuSolver(n, h, lira.MIXED)
+    for _ in range(args.warmup):
+        solver.solve(pos, z, nu, nd, settings)
+    torch.cuda.synchronize()
+    times, stats = [], None
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stats = solver.solve(pos, z, nu, nd, settings)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        print(f"# mixed SCF: {times[-1]:.3f} s, {stats.iterations} steps, E {stats.energy.total:.8f}, "
+              f"lobpcg {stats.lobpcg_iterations}, cg {stats.cg_iterations}, readbacks {stats.readbacks}",
+              file=sys.stderr, flush=True)
+    mixed_s = float(np.mean(times))
+    f_mixed = solver.forces(pos, z, h)
+    ev_mixed = solver.levels(0)[0]
+    result = {"metric": METRIC, "value": mixed_s, "unit": "s", "n_gpus": 1, "steps": args.steps,
+              "warmup": args.warmup, "ms_per_step": mixed_s * 1e3, "higher_is_better": False, "scaling": "strong",
+              "vs_baseline": None, "dtype": "f32 grid / f64 control", "data": "synthetic",
+              "config": {"workload": "water H2O all-electron, 95^3 grid at h = 0.15 Bohr, LSDA, FD kT 5 mHa, "
+                                     "delta_rho < 1e-6", "grid": n, "h_bohr": h, "nuclei": 3,
+                         "electrons": nu + nd, "precision": "mixed", "parallelism": "single GPU"},
+              "mixed": {"seconds": mixed_s, "steps": stats.iterations, "energy": stats.energy.total,
+                        "lobpcg_iterations": stats.lobpcg_iterations, "cg_iterations": stats.cg_iterations,
+                        "readbacks": stats.readbacks, "seconds_poisson": stats.seconds_poisson,
+                        "seconds_eigen": stats.seconds_eigen, "stored": list(stats.n_stored)}}
+    gpu_fp64 = None
+    if not args.no_fp64:
+        s64 = lira.GpuSolver(n, h, lira.FP64)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st64 = s64.solve(pos, z, nu, nd, settings)
+        torch.cuda.synchronize()
+        fp64_s = time.perf_counter() - t0
+        f64 = s64.forces(pos, z, h)
+        ev64 = s64.levels(0)[0]
+        kk = min(len(ev64), len(ev_mixed))
+        gpu_fp64 = {"seconds": fp64_s, "steps": st64.iterations, "energy": st64.energy.total,
+                    "lobpcg_iterations": st64.lobpcg_iterations, "k": int(st64.n_stored[0])}
+        result["fp64"] = gpu_fp64
+        result["mixed_vs_fp64"] = {"speedup": fp64_s / mixed_s, "dE": stats.energy.total - st64.energy.total,
+                                   "max_deps": float(np.abs(ev_mixed[:kk] - ev64[:kk]).max()),
+                                   "max_dF": float(np.abs(f_mixed - f64).max())}
+        print(f"# fp64 SCF: {fp64_s:.3f} s, {st64.iterations} steps, E {st64.energy.total:.8f}", file=sys.stderr,
+              flush=True)
+        state = (s64.field(2), s64.field(3), s64.field(4))
+        del s64
+    del solver
+    torch.cuda.empty_cache()
+    if not args.no_roofline:
+        r = hx_roofline(torch, lira)
+        result["roofline"] = {"bound": "hbm", "achieved": r["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": r["frac"], "traffic": None,
+                              "kernel": "k_stencil<float, ST_HAM> on 192^3 x 64 bands",
+                              "ms_per_launch": r["ms"], "algorithmic_bytes": r["bytes"], "variant": r["variant"]}
+        r64 = hx_roofline(torch, lira, dtype="float64")
+        result["roofline_fp64"] = {"achieved": r64["gbs"], "frac": r64["frac"], "ms_per_launch": r64["ms"]}
+        print(f"# H.X 192^3 x 64 f32: {r['ms']:.3f} ms, {r['gbs']:.0f} GB/s ({100 * r['frac']:.1f} % of peak); "
+              f"f64 {r64['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
+    if not args.no_cpu and gpu_fp64 is not None:
+        threads = min(16, os.cpu_count() or 1)
+        result["cpu_baseline"] = cpu_baseline_sample(state, gpu_fp64, n, h, pos, z, threads)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,197 @@
+/*
+ * lira.h — C ABI of liblira, the MI355X-native real-space Kohn–Sham SCF hot path.
+ *
+ * It replaces the reference's `qm_gpu::GpuSolver` Rust API (crates/qm-gpu/src/scf.rs:1044-1842,
+ * crates/qm-gpu/src/forces.rs:239-297) with plain pointers and sizes, so a Rust `-sys` crate (or ctypes) binds it
+ * without any GPU types. Units: Bohr and Hartree. Grids are cubic n^3 with spacing h, centred at the origin
+ * (origin = -(n-1)h/2, crates/qm/src/grid.rs:11-14); host arrays use the qm layout idx = i + n(j + n k).
+ *
+ * Threading: a solver is used by one thread at a time (the reference's GpuSolver is Send, not Sync,
+ * crates/qm-gpu/src/scf.rs:1042-1043). Errors: every call returns a lira_status; lira_last_error() gives the message
+ * (the reference's QmError variants, crates/qm/src/error.rs:1-33). After any error except LIRA_COINCIDENT_NUCLEI
+ * the solver drops its warm state (the next solve is cold), as crates/qm-gpu/src/scf.rs:1177-1192.
+ */
+#ifndef LIRA_H
+#define LIRA_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    LIRA_OK = 0,
+    LIRA_COINCIDENT_NUCLEI = 1, /* QmError::CoincidentNuclei */
+    LIRA_NOT_CONVERGED = 2,     /* QmError::ScfNotConverged */
+    LIRA_STALLED = 3,           /* QmError::ScfStalled */
+    LIRA_NUMERICAL = 4,         /* QmError::Numerical */
+    LIRA_DEVICE = 5,            /* QmError::Device (a failed HIP call or readback) */
+    LIRA_INVALID = 6,           /* bad arguments (a programming error in the reference: a panic) */
+    LIRA_ABORTED = 7            /* the abort callback returned nonzero (the reference's Ok(None)) */
+} lira_status;
+
+typedef enum {
+    LIRA_FP64 = 0,  /* every grid quantity in f64 */
+    LIRA_MIXED = 1  /* f32 grid fields (stencil, V-cycles, block products), f64 Rayleigh–Ritz, DIIS, energies */
+} lira_precision;
+
+typedef enum { LIRA_AUFBAU = 0, LIRA_FERMI_DIRAC = 1 } lira_occupations;
+
+typedef struct lira_solver lira_solver;
+
+/* qm::Nucleus (crates/qm/src/nuclear.rs:100-104) */
+typedef struct {
+    double pos[3];
+    int z;
+} lira_nucleus;
+
+/* qm::System (crates/qm/src/scf.rs:329) without an ionic setup (all-electron) */
+typedef struct {
+    const lira_nucleus *nuclei;
+    int n_nuclei;
+    int n_up, n_dn;
+} lira_system;
+
+/* qm::ScfSettings (crates/qm/src/scf.rs:375-385); lira_default_settings fills ScfSettings::for_grid */
+typedef struct {
+    double sigma;     /* nuclear smearing width (Bohr); for_grid: h */
+    double tol;       /* on delta_rho = int |rho_out - rho_in|; 1e-6 */
+    int max_iter;     /* 200 */
+    double mixing;    /* DIIS step alpha; 0.3 */
+    double eig_tol;   /* LOBPCG residual tolerance; 1e-6 */
+    int diis_history; /* 6 */
+    int occupations;  /* lira_occupations; for_grid: Fermi–Dirac */
+    double kt;        /* Fermi–Dirac kT (Ha); 5e-3 */
+    int stall_window; /* 50; 0 = off */
+} lira_settings;
+
+/* qm::Energy (crates/qm/src/scf.rs:546-548) */
+typedef struct {
+    double kinetic, external_, hartree, xc, nuclear, internal, ts, total;
+} lira_energy;
+
+/* qm::ScfStats (crates/qm/src/scf.rs:558-574), plus the solve's profile (SolveProfile, crates/qm-gpu/src/scf.rs:420-514) */
+typedef struct {
+    int iterations;
+    double delta_rho;
+    lira_energy energy;
+    double boundary_density;
+    double max_eigen_residual;
+    double poisson_residual;
+    int poisson_iterations;
+    double entropy;
+    double mu[2];
+    double electrons[2];
+    double tol_used;
+    int n_stored[2];  /* orbitals stored per spin */
+    int restricted;   /* 1 when the spin-down channel mirrored the spin-up one (n_up == n_dn) */
+    /* profile */
+    int lobpcg_iterations;
+    int cg_iterations;
+    int readbacks;
+    double seconds_total, seconds_poisson, seconds_eigen, seconds_other;
+} lira_scf_stats;
+
+/* Poisson solve statistics (PoissonStats, crates/qm/src/poisson.rs:136) */
+typedef struct {
+    int iterations;      /* inner (preconditioned CG) iterations */
+    int outer;           /* f64 residual-correction cycles */
+    double residual;     /* ||b - A V|| / ||b|| in f64 at exit */
+    double seconds;
+} lira_poisson_stats;
+
+/* LOBPCG statistics (EigenStats, crates/qm/src/eigen.rs:132-139) */
+typedef struct {
+    int iterations;
+    double max_residual;
+    int fallbacks;
+} lira_eigen_stats;
+
+typedef int (*lira_abort_fn)(void *user);
+
+/* ------------------------------------------------------------------------------------------- solver ---- */
+
+/* Fills ScfSettings::for_grid (crates/qm/src/scf.rs:380-385) for spacing h. */
+void lira_default_settings(double h, lira_settings *out);
+
+/* GpuSolver::new (crates/qm-gpu/src/scf.rs:1083-1101): device buffers for an n^3 grid at spacing h on the current
+ * HIP device. precision: lira_precision. */
+lira_status lira_solver_create(int n, double h, int precision, lira_solver **out);
+void lira_solver_destroy(lira_solver *s);
+const char *lira_last_error(const lira_solver *s);
+
+/* GpuSolver::solve_polled (crates/qm-gpu/src/scf.rs:1195-1235): runs the SCF. warm != 0 reuses the previous converged
+ * state when it fits (same grid, counts and scheme). abort (may be NULL) is polled before each SCF step. */
+lira_status lira_solve(lira_solver *s, const lira_system *sys, const lira_settings *settings, int warm,
+                       lira_abort_fn abort, void *user, lira_scf_stats *stats);
+
+/* GpuSolver::forces (crates/qm-gpu/src/forces.rs:239-266): Hellmann–Feynman forces of the last converged state,
+ * out[3 * n_nuclei]. sigma must be the solve's. */
+lira_status lira_forces(lira_solver *s, const lira_nucleus *nuclei, int n_nuclei, double sigma, double *out);
+
+/* state getters (GpuSolver::state / ks_state / density_f32, crates/qm-gpu/src/scf.rs:1172, 1763-1851) as f64 in the
+ * qm layout. which: 0 = rho_up, 1 = rho_dn, 2 = rho, 3 = V_H, 4 = V_nuc. */
+lira_status lira_get_field(lira_solver *s, int which, double *out);
+/* eigenvalues / occupations of a spin channel (0 up, 1 down); *count receives the stored count */
+lira_status lira_get_levels(lira_solver *s, int spin, double *evals, double *occ, int cap, int *count);
+/* orbitals of a spin channel, cap * n^3 doubles (band-major) */
+lira_status lira_get_orbitals(lira_solver *s, int spin, double *out, int cap, int *count);
+/* GpuSolver::effective_eig_tol (crates/qm-gpu/src/scf.rs:1713): the eigen tolerance the precision can reach */
+double lira_effective_eig_tol(const lira_solver *s, double requested);
+/* drops the warm state (the next solve is cold) */
+void lira_reset(lira_solver *s);
+
+/* --------------------------------------------------------------------------------- sub-solvers ---- */
+
+/* GpuPoisson::solve (crates/qm-gpu/src/poisson.rs:284-342): del^2 V = -4 pi rho with the multipole boundary
+ * (l <= 6); rho and v are host f64 arrays (v: initial guess in, solution out). */
+lira_status lira_poisson(lira_solver *s, const double *rho, double *v, double tol, int max_iter,
+                         lira_poisson_stats *stats);
+
+/* lobpcg (crates/qm-gpu/src/eigen.rs:595-653) for H = -1/2 lap + V with the damped-Jacobi preconditioner shifted by
+ * precond_lambda: V (n^3), x (k * n^3, guess in, orthonormal eigenvectors out), evals (k). Host f64 arrays. */
+lira_status lira_lobpcg(lira_solver *s, const double *v, int k, double *x, double *evals, double precond_lambda,
+                        double tol, int max_iter, lira_eigen_stats *stats);
+
+/* Rayleigh–Ritz on a measured pair (A, B), m x m row-major: the lowest nk Ritz values theta and B-orthonormal
+ * coefficient columns C (m x nk, row-major) (crates/qm-gpu/src/eigen.rs:515-542). Host-only. */
+lira_status lira_rayleigh_ritz(int m, const double *a, const double *b, int nk, double *theta, double *c);
+
+/* ----------------------------------------------------------------- kernels on device pointers ---- */
+/* dtype: 4 = f32, 8 = f64. Pointers are device pointers; stream is a hipStream_t (NULL = default stream).
+ * Band-major blocks: band b of a block starts at b * n^3. Outputs must not alias inputs. */
+
+/* hamiltonian/laplacian (crates/qm-gpu/src/stencil.rs:43-121): hx_b = -1/2 lap x_b + v x_b */
+lira_status lira_hamiltonian(int dtype, int n, double h, int nbands, const void *v, const void *x, void *hx,
+                             void *stream);
+/* out_b = scale * lap x_b (8th order, Dirichlet) */
+lira_status lira_laplacian(int dtype, int n, double h, int nbands, double scale, const void *x, void *out,
+                           void *stream);
+/* one damped-Jacobi sweep of the preconditioner (crates/qm-gpu/src/stencil.rs:62-73):
+ * out_b = w_b + omega / max(diag_t + v - lam_b, 0.1) (r_b - (H - lam_b) w_b), lam device f64[nbands] */
+lira_status lira_jacobi_sweep(int dtype, int n, double h, int nbands, const void *v, const void *w, const void *r,
+                              const double *lam, void *out, void *stream);
+/* xc_lsda (crates/qm-gpu/src/physics.rs:86-114): e = n eps_xc, vu, vd from (nu, nd), count points */
+lira_status lira_xc_lsda(int dtype, long count, const void *nu, const void *nd, void *e, void *vu, void *vd,
+                         void *stream);
+/* nuclear_potential (crates/qm-gpu/src/physics.rs:48-76), nuclei on the host */
+lira_status lira_nuclear_potential(int dtype, int n, double h, const lira_nucleus *nuclei, int n_nuclei, double sigma,
+                                   void *out, void *stream);
+/* tall-skinny Gram G = h^3 S^T U for band-major blocks S (m bands) and U (q bands); G is host f64 m x q. */
+lira_status lira_gram(int dtype, int n, double h, int m, const void *s_block, int q, const void *u_block, double *g,
+                      void *stream);
+
+/* stencil tile variants and the autotuner (the tile shape is picked per grid by timing) */
+int lira_stencil_variant_count(void);
+const char *lira_stencil_variant_name(int v);
+/* times every variant on (dtype, n, nbands) and makes the fastest the default for that dtype; returns its index */
+int lira_autotune_stencil(int dtype, int n, int nbands, double *ms_out);
+void lira_set_stencil_variant(int dtype, int variant);
+
+const char *lira_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,211 @@
+"""GPU parity of the Poisson, LOBPCG and SCF solvers against the f64 oracle and the reference's own pins.
+
+Tolerances: FP64 against the oracle's converged answers to the solvers' tolerances; MIXED (f32 grid, f64 control)
+to the reference's GPU-vs-CPU parity bounds (crates/validate/tests/gpu_vs_cpu.rs:152-160, 220-221:
+|dE| <= 2e-6 Ha for H2, 3e-6 for LiH; |dF| <= 5e-6 Ha/Bohr for H2).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lira_mod():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    import lira
+    lira.lib()
+    return lira
+
+
+def _gaussian(n, h, q, sigma, c):
+    x, y, z = O.grid_points(n, h)
+    r2 = (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2
+    rho = q / (2 * math.pi * sigma * sigma) ** 1.5 * np.exp(-r2 / (2 * sigma * sigma))
+    from scipy.special import erf
+    r = np.sqrt(r2)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        exact = np.where(r < 1e-12, q * math.sqrt(2 / math.pi) / sigma, q * erf(r / (sigma * math.sqrt(2))) / r)
+    return rho, exact
+
+
+# ------------------------------------------------------------------------------------------- Poisson ----
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_poisson_matches_oracle_and_closed_form(lira_mod, precision):
+    lira = lira_mod
+    n, h = 49, 0.25
+    rho, exact = _gaussian(n, h, 1.0, 0.5, [1.3, -0.7, 0.4])
+    s = lira.GpuSolver(n, h, precision)
+    v, st = s.poisson(rho, tol=1e-10)
+    print(f"poisson precision {precision}: {st.iterations} PCG iterations, {st.outer} corrections, "
+          f"residual {st.residual:.2e}, {st.seconds * 1e3:.1f} ms")
+    assert st.residual <= 1e-10
+    vo, _, res = O.poisson_solve(n, h, rho, tol=1e-10, max_iter=3000)
+    # both solve the same discrete system to 1e-10: the solutions agree to the system's conditioning
+    assert np.abs(v - vo).max() < 5e-8
+    assert np.abs(v - exact).max() < 1e-4  # qm/src/poisson.rs:304-307
+    eh = 0.5 * np.sum(rho * v) * h ** 3
+    # MIXED stores rho in f32 (relative rounding 6e-8 of the right-hand side)
+    assert abs(eh - O.dot(n, h, rho, vo) * 0.5) < (1e-9 if precision == 0 else 1e-7)
+    # a warm start from the solution needs no correction
+    v2, st2 = s.poisson(rho, v0=v, tol=1e-10)
+    assert st2.iterations == 0 and np.array_equal(v2, v)
+
+
+def test_poisson_hartree_energy_of_gaussian(lira_mod):
+    lira = lira_mod
+    n, h = 49, 0.25
+    rho, _ = _gaussian(n, h, 1.0, 0.5, [0, 0, 0])
+    v, st = lira.GpuSolver(n, h, 1).poisson(rho, tol=1e-10)
+    eh = 0.5 * np.sum(rho * v) * h ** 3
+    assert abs(eh - 1.0 / (2 * 0.5 * math.sqrt(math.pi))) < 1e-4
+
+
+def test_poisson_nan_density_is_numerical_error(lira_mod):
+    lira = lira_mod
+    n, h = 17, 0.3
+    rho = np.zeros(n ** 3)
+    rho[100] = float("nan")
+    s = lira.GpuSolver(n, h, 0)
+    with pytest.raises(lira.Numerical):
+        s.poisson(rho)
+
+
+# -------------------------------------------------------------------------------------------- LOBPCG ----
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-6), (1, 2e-5)])
+def test_lobpcg_oscillator(lira_mod, precision, tol):
+    # 3D harmonic oscillator: 1.5, 2.5 (x3) (crates/qm-gpu/src/eigen.rs:1008-1033)
+    lira = lira_mod
+    n, h = 41, 0.25
+    x, y, z = O.grid_points(n, h)
+    v = 0.5 * (x * x + y * y + z * z)
+    guess = O.initial_orbitals(n, h, [[0.0, 0.0, 0.0]], 4, 12345)
+    s = lira.GpuSolver(n, h, precision)
+    xs, ev, st = s.lobpcg(v, guess, tol, 500, precond_lambda=1.5)
+    print(f"lobpcg precision {precision}: {st.iterations} iterations, residual {st.max_residual:.2e}, "
+          f"fallbacks {st.fallbacks}, evals {ev}")
+    assert st.max_residual < tol
+    assert abs(ev[0] - 1.5) < 1e-4
+    assert np.abs(ev[1:] - 2.5).max() < 1e-4
+    g = xs @ xs.T * h ** 3
+    assert np.abs(g - np.eye(4)).max() < (1e-8 if precision == 0 else 1e-5)
+    # the same discrete problem in the oracle: eigenvalues agree to the residual^2 / gap scale
+    _, evo, _, _, _ = O.lobpcg(n, h, v, guess, 1e-8, 500, precond_lambda=1.5)
+    assert np.abs(ev - evo).max() < (1e-10 if precision == 0 else 2e-7)
+
+
+# ----------------------------------------------------------------------------------------------- SCF ----
+
+H2 = ([[0.7, 0.0, 0.0], [-0.7, 0.0, 0.0]], [1, 1], 1, 1)
+LIH = ([[-0.8, 0.1, 0.0], [2.2, 0.0, 0.0]], [3, 1], 2, 2)
+
+
+def _scf(lira, precision, system, occ=0, n=33, h=0.25):
+    pos, z, nu, nd = system
+    s = lira.GpuSolver(n, h, precision)
+    st = s.solve(pos, z, nu, nd, lira.default_settings(h, occ))
+    f = s.forces(pos, z, h)
+    return s, st, f
+
+
+@pytest.mark.parametrize("precision,e_tol,f_tol", [(0, 1e-9, 1e-7), (1, 2e-6, 5e-6)])
+def test_scf_h2_matches_reference_pins(lira_mod, precision, e_tol, f_tol):
+    # qm/src/scf.rs:1276-1277: E = -0.952449176828, F_x = +0.128755968665 (33^3 x 0.25, Aufbau, 10 CPU steps)
+    lira = lira_mod
+    s, st, f = _scf(lira, precision, H2)
+    e = st.energy.total
+    print(f"H2 precision {precision}: E {e:.12f} F_x {f[0, 0]:+.12f} steps {st.iterations} lobpcg "
+          f"{st.lobpcg_iterations} cg {st.cg_iterations} readbacks {st.readbacks} {st.seconds_total * 1e3:.1f} ms")
+    assert abs(e - (-0.952449176828)) < e_tol
+    assert abs(f[0, 0] - 0.128755968665) < f_tol
+    assert abs(f[0, 0] + f[1, 0]) < 10 * f_tol
+    assert abs(st.electrons[0] - 1) < 1e-6 and abs(st.electrons[1] - 1) < 1e-6
+    assert st.restricted == 1
+
+
+# forces and eigenvalues of a solve stopped at delta_rho < 1e-6 are first order in its residual (the reference's own
+# Aufbau and FD H2 forces differ by 6.3e-7, qm/src/scf.rs:1277, 1291), energies second order: f_tol 2e-6 for FP64
+@pytest.mark.parametrize("precision,e_tol,f_tol", [(0, 1e-9, 2e-6), (1, 3e-6, 2e-5)])
+def test_scf_lih_matches_reference_pins(lira_mod, precision, e_tol, f_tol):
+    # qm/src/scf.rs:1278-1279: E = -5.219911350989, F_x(Li) = -0.022680156352 (13 CPU steps)
+    lira = lira_mod
+    s, st, f = _scf(lira, precision, LIH)
+    print(f"LiH precision {precision}: E {st.energy.total:.12f} F_x {f[0, 0]:+.12f} steps {st.iterations} "
+          f"{st.seconds_total * 1e3:.1f} ms")
+    assert abs(st.energy.total - (-5.219911350989)) < e_tol
+    assert abs(f[0, 0] - (-0.022680156352)) < f_tol
+
+
+@pytest.mark.parametrize("system", [H2, LIH], ids=["H2", "LiH"])
+def test_scf_fp64_tight_state_matches_oracle(lira_mod, system):
+    # both solvers driven to delta_rho < 1e-9 with eig_tol 1e-9: the same discrete fixed point, so energies,
+    # eigenvalues, densities and forces agree to the tolerances, not to the first-order 1e-6 floor
+    lira = lira_mod
+    pos, z, nu, nd = system
+    n, h = 33, 0.25
+    s = lira.GpuSolver(n, h, 0)
+    st = s.solve(pos, z, nu, nd, lira.default_settings(h, 0, tol=1e-9, eig_tol=1e-9))
+    f = s.forces(pos, z, h)
+    so, out = O.scf(n, h, pos, z, nu, nd, settings=O.default_settings(h, "aufbau", tol=1e-9, eig_tol=1e-9))
+    assert so.status == O.OK
+    state = s.state()
+    print(f"tight {nu + nd} e-: dE {st.energy.total - so.total:.2e} dF {np.abs(f - out['forces']).max():.2e} "
+          f"deps {np.abs(state.evals_up - np.sort(out['evals_up'])).max():.2e}")
+    assert abs(st.energy.total - so.total) < 1e-10
+    assert np.abs(state.evals_up - np.sort(out["evals_up"])).max() < 1e-8
+    assert np.abs(f - out["forces"]).max() < 1e-8
+    rho = state.rho_up + state.rho_dn
+    rho_o = out["rho_up"] + out["rho_dn"]
+    assert np.abs(rho - rho_o).max() < 1e-7 * rho_o.max()
+    for name in ("kinetic", "external", "hartree", "xc", "nuclear"):
+        assert abs(getattr(st.energy, name) - getattr(so, name)) < 1e-8, name
+
+
+def test_scf_fermi_dirac_h2(lira_mod):
+    # FD_H2_E = -0.952449176839 (qm/src/scf.rs:1290): gapped H2, TS ~ 0, two stored levels per spin
+    lira = lira_mod
+    s, st, f = _scf(lira, 0, H2, occ=1)
+    assert abs(st.energy.total - (-0.952449176839)) < 1e-9
+    assert st.n_stored[0] == 2 and st.energy.ts < 1e-12
+
+
+def test_scf_coincident_nuclei_keeps_state(lira_mod):
+    lira = lira_mod
+    s, st, f = _scf(lira, 1, H2)
+    with pytest.raises(lira.CoincidentNuclei):
+        s.solve([[0, 0, 0], [0, 0, 0]], [1, 1], 1, 1)
+    # the converged state survives (the reference keeps its warm state on CoincidentNuclei)
+    ev, _ = s.levels(0)
+    assert ev.size == 1
+
+
+def test_scf_warm_start_is_cheaper_and_lands_on_the_cold_energy(lira_mod):
+    lira = lira_mod
+    n, h = 33, 0.25
+    s = lira.GpuSolver(n, h, 1)
+    st = lira.default_settings(h, 0)
+    s.solve([[0.7, 0, 0], [-0.7, 0, 0]], [1, 1], 1, 1, st)
+    warm = s.solve([[0.72, 0, 0], [-0.7, 0, 0]], [1, 1], 1, 1, st, warm=True)
+    cold = lira.GpuSolver(n, h, 1).solve([[0.72, 0, 0], [-0.7, 0, 0]], [1, 1], 1, 1, st)
+    assert warm.iterations < cold.iterations
+    assert abs(warm.energy.total - cold.energy.total) < 1e-5
+
+
+def test_scf_abort_returns_aborted(lira_mod):
+    lira = lira_mod
+    s = lira.GpuSolver(25, 0.3, 1)
+    calls = []
+
+    def abort():
+        calls.append(1)
+        return len(calls) > 2
+
+    with pytest.raises(lira.Aborted):
+        s.solve(H2[0], H2[1], 1, 1, lira.default_settings(0.3, 0), abort=abort)
