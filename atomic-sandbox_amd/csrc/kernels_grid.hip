@@ -44,6 +44,9 @@ __global__ __launch_bounds__(256) void k_stencil(const T *__restrict__ x, T *__r
                                                  long N, int zc, T inv_h2, T scale, T omega, T diag_t) {
     constexpr int TY = 4 * RPT;
     constexpr int LX = ST_TX + 8;
+    constexpr int PF = 2;  // planes of look-ahead for the halo, V and R loads
+    constexpr bool USE_V = MODE != ST_LAP;
+    constexpr bool USE_R = MODE == ST_JAC;
     __shared__ T tile[2][TY + 8][LX];
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int t = ty * 64 + tx;
@@ -54,44 +57,53 @@ __global__ __launch_bounds__(256) void k_stencil(const T *__restrict__ x, T *__r
     const int z1 = min(n, z0 + zc);
     const T *xb = x + (long)band * N;
     T *ob = out + (long)band * N;
-    const T *rb = (MODE == ST_JAC) ? r + (long)band * N : nullptr;
-    const T lamb = (MODE == ST_JAC) ? (T)lam[band] : T(0);
+    const T *rb = USE_R ? r + (long)band * N : nullptr;
+    const T lamb = USE_R ? (T)lam[band] : T(0);
     const int gx = x0 + tx;
     const bool xin = gx < n;
     const long nn = (long)n * n;
 
-    auto ld = [&](int xx, int yy, int zz) -> T {
-        return (xx >= 0 && xx < n && yy >= 0 && yy < n && zz >= 0 && zz < n) ? xb[xx + (long)n * yy + nn * zz] : T(0);
+    auto inb = [&](int xx, int yy, int zz) { return xx >= 0 && xx < n && yy >= 0 && yy < n && zz >= 0 && zz < n; };
+    auto ld = [&](const T *base, int xx, int yy, int zz) -> T {
+        return inb(xx, yy, zz) ? base[xx + (long)n * yy + nn * zz] : T(0);
     };
+    // this thread's fixed halo cells of a plane: two y-halo cells and (for t < 8 (TY + 8)) one x-halo cell
+    const int hy_row0 = t >> 6, hy_row1 = (t + 256) >> 6;
+    const int hly0 = hy_row0 < 4 ? hy_row0 : TY + hy_row0, hly1 = hy_row1 < 4 ? hy_row1 : TY + hy_row1;
+    const bool has_xh = t < 8 * (TY + 8);
+    const int hxr = t >> 3, hxc = t & 7;
+    const int hlx = hxc < 4 ? hxc : ST_TX + hxc;
 
-    T q[9][RPT];
+    T q[9][RPT];      // interior z-window: q[d] is plane z - 4 + d
+    T hq[PF + 1][3];  // halo cells of planes z .. z + PF
+    T vq[PF + 1][RPT], rq[PF + 1][RPT];
 #pragma unroll
     for (int d = 0; d < 8; d++)
 #pragma unroll
-        for (int rr = 0; rr < RPT; rr++) q[d][rr] = ld(gx, y0 + ty + 4 * rr, z0 - 4 + d);
+        for (int rr = 0; rr < RPT; rr++) q[d][rr] = ld(xb, gx, y0 + ty + 4 * rr, z0 - 4 + d);
+    auto load_side = [&](int slot, int zz) {
+        hq[slot][0] = ld(xb, gx, y0 - 4 + hly0, zz);
+        hq[slot][1] = ld(xb, gx, y0 - 4 + hly1, zz);
+        hq[slot][2] = has_xh ? ld(xb, x0 - 4 + hlx, y0 - 4 + hxr, zz) : T(0);
+#pragma unroll
+        for (int rr = 0; rr < RPT; rr++) {
+            if (USE_V) vq[slot][rr] = ld(v, gx, y0 + ty + 4 * rr, zz);
+            if (USE_R) rq[slot][rr] = ld(rb, gx, y0 + ty + 4 * rr, zz);
+        }
+    };
+#pragma unroll
+    for (int d = 0; d < PF; d++) load_side(d, z0 + d);
 
     int buf = 0;
     for (int z = z0; z < z1; z++) {
 #pragma unroll
-        for (int rr = 0; rr < RPT; rr++) q[8][rr] = ld(gx, y0 + ty + 4 * rr, z + 4);
-        // interior of plane z from the queue
+        for (int rr = 0; rr < RPT; rr++) q[8][rr] = ld(xb, gx, y0 + ty + 4 * rr, z + 4);
+        load_side(PF, z + PF);
 #pragma unroll
         for (int rr = 0; rr < RPT; rr++) tile[buf][ty + 4 * rr + 4][tx + 4] = q[4][rr];
-        // y halo: 8 rows x 64 columns
-#pragma unroll
-        for (int s2 = 0; s2 < 2; s2++) {
-            int e = t + 256 * s2;
-            int row = e >> 6;
-            int ly = row < 4 ? row : TY + row;
-            int gy = y0 - 4 + ly;
-            tile[buf][ly][tx + 4] = ld(gx, gy, z);
-        }
-        // x halo: (TY + 8) rows x 8 columns
-        if (t < 8 * (TY + 8)) {
-            int row = t >> 3, c = t & 7;
-            int lx = c < 4 ? c : ST_TX + c;
-            tile[buf][row][lx] = ld(x0 - 4 + lx, y0 - 4 + row, z);
-        }
+        tile[buf][hly0][tx + 4] = hq[0][0];
+        tile[buf][hly1][tx + 4] = hq[0][1];
+        if (has_xh) tile[buf][hxr][hlx] = hq[0][2];
         __syncthreads();
 #pragma unroll
         for (int rr = 0; rr < RPT; rr++) {
@@ -114,13 +126,13 @@ __global__ __launch_bounds__(256) void k_stencil(const T *__restrict__ x, T *__r
                 if (MODE == ST_LAP) {
                     ob[id] = scale * lap;
                 } else if (MODE == ST_HAM) {
-                    ob[id] = T(-0.5) * lap + v[id] * c;
+                    ob[id] = T(-0.5) * lap + vq[0][rr] * c;
                 } else {
-                    const T vv = v[id];
+                    const T vv = vq[0][rr];
                     const T hw = T(-0.5) * lap + vv * c - lamb * c;
                     T d = diag_t + vv - lamb;
                     d = d > T(0.1) ? d : T(0.1);
-                    ob[id] = c + omega / d * (rb[id] - hw);
+                    ob[id] = c + omega / d * (rq[0][rr] - hw);
                 }
             }
         }
@@ -129,6 +141,16 @@ __global__ __launch_bounds__(256) void k_stencil(const T *__restrict__ x, T *__r
         for (int d = 0; d < 8; d++)
 #pragma unroll
             for (int rr = 0; rr < RPT; rr++) q[d][rr] = q[d + 1][rr];
+#pragma unroll
+        for (int d = 0; d < PF; d++) {
+#pragma unroll
+            for (int e = 0; e < 3; e++) hq[d][e] = hq[d + 1][e];
+#pragma unroll
+            for (int rr = 0; rr < RPT; rr++) {
+                if (USE_V) vq[d][rr] = vq[d + 1][rr];
+                if (USE_R) rq[d][rr] = rq[d + 1][rr];
+            }
+        }
     }
 }
 
@@ -281,7 +303,6 @@ void reduce_jobs(const RedJobs<T> &jobs, int n, double scale, double *partials, 
 // ------------------------------------------------------------------------------------------- Gram ----
 // G = scale S^T U over N points. A workgroup computes a 32 (S) x 64 (U) output block over one chunk of points,
 // staging 64 points of its 96 columns in LDS at a time; each thread owns a 2 x 4 register tile accumulated in f64.
-constexpr int GR_BI = 32, GR_BJ = 64, GR_P = 64;
 
 struct ColPtrs {
     const void *base[MAX_SEGS];
@@ -297,74 +318,95 @@ __device__ inline const void *col_base(const ColPtrs &c, int col, long N, int es
     return static_cast<const char *>(c.base[s]) + (size_t)(col - c.start[s]) * (size_t)N * (size_t)esize;
 }
 
+// G = scale S^T U on the f64 matrix cores: v_mfma_f64_16x16x4f64 with A = S-columns x 4 points, B = 4 points x
+// U-columns (lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]; the 16 x 16 result lands as D[l / 16 + 4 r][l % 16]
+// in register r, measured on gfx950). A workgroup (4 waves) owns a 32 x 64 output tile over one chunk of points; each wave two 16 x 16
+// sub-tiles. Columns are staged 32 points at a time in LDS as f64 (f32 inputs convert exactly), so the products are
+// exact and the accumulation is f64 in every precision mode.
+typedef double lira_d4 __attribute__((ext_vector_type(4)));
+constexpr int G2_TI = 32, G2_TJ = 64, G2_P = 32;
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q, long N, long chunk,
                                               double *partials) {
-    __shared__ T ss[GR_BI][GR_P + 1];
-    __shared__ T us[GR_BJ][GR_P + 1];
-    const int nbj = (q + GR_BJ - 1) / GR_BJ;
+    __shared__ double sa[G2_TI][G2_P + 1];
+    __shared__ double sb[G2_TJ][G2_P + 1];
+    __shared__ const T *scol[G2_TI];
+    __shared__ const T *ucol[G2_TJ];
+    const int nbj = (q + G2_TJ - 1) / G2_TJ;
     const int bi = blockIdx.y / nbj, bj = blockIdx.y - bi * nbj;
-    const int i0 = bi * GR_BI, j0 = bj * GR_BJ;
-    const int t = threadIdx.x;
-    const int ti = t >> 4, tj = t & 15;  // 16 x 16 threads, tile 2 x 4
-    double acc[2][4] = {};
+    const int i0 = bi * G2_TI, j0 = bj * G2_TJ;
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    if (t < G2_TI) scol[t] = i0 + t < m ? static_cast<const T *>(col_base(S, i0 + t, N, sizeof(T))) : nullptr;
+    if (t >= 64 && t < 64 + G2_TJ) {
+        const int c = t - 64;
+        ucol[c] = j0 + c < q ? static_cast<const T *>(col_base(U, j0 + c, N, sizeof(T))) : nullptr;
+    }
+    const int isub = wave & 1, jsub0 = (wave >> 1) * 2;
+    lira_d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
     const long p_begin = (long)blockIdx.x * chunk;
     const long p_end = min(N, p_begin + chunk);
-    for (long p0 = p_begin; p0 < p_end; p0 += GR_P) {
-        // stage: 96 columns x 64 points; thread loads column rows in strides
-        for (int e = t; e < (GR_BI + GR_BJ) * GR_P; e += 256) {
-            const int col = e / GR_P, pp = e - col * GR_P;
+    __syncthreads();
+    for (long p0 = p_begin; p0 < p_end; p0 += G2_P) {
+        for (int e = t; e < (G2_TI + G2_TJ) * G2_P; e += 256) {
+            const int col = e / G2_P, pp = e - col * G2_P;
             const long p = p0 + pp;
-            if (col < GR_BI) {
-                const int ci = i0 + col;
-                T val = T(0);
-                if (ci < m && p < p_end) val = static_cast<const T *>(col_base(S, ci, N, sizeof(T)))[p];
-                ss[col][pp] = val;
+            if (col < G2_TI) {
+                const T *c = scol[col];
+                sa[col][pp] = (c && p < p_end) ? (double)c[p] : 0.0;
             } else {
-                const int cj = j0 + col - GR_BI;
-                T val = T(0);
-                if (cj < q && p < p_end) val = static_cast<const T *>(col_base(U, cj, N, sizeof(T)))[p];
-                us[col - GR_BI][pp] = val;
+                const T *c = ucol[col - G2_TI];
+                sb[col - G2_TI][pp] = (c && p < p_end) ? (double)c[p] : 0.0;
             }
         }
         __syncthreads();
-#pragma unroll 4
-        for (int pp = 0; pp < GR_P; pp++) {
-            const double s0 = (double)ss[2 * ti][pp], s1 = (double)ss[2 * ti + 1][pp];
-            double u[4];
+        const int r16 = lane & 15, k4 = lane >> 4;
 #pragma unroll
-            for (int c = 0; c < 4; c++) u[c] = (double)us[4 * tj + c][pp];
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-                acc[0][c] += s0 * u[c];
-                acc[1][c] += s1 * u[c];
-            }
+        for (int kk = 0; kk < G2_P; kk += 4) {
+            const double a = sa[isub * 16 + r16][kk + k4];
+            const double b0 = sb[jsub0 * 16 + r16][kk + k4];
+            const double b1 = sb[(jsub0 + 1) * 16 + r16][kk + k4];
+            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc1, 0, 0, 0);
         }
         __syncthreads();
     }
     double *dst = partials + (long)blockIdx.x * m * q;
+    const int jl = lane & 15, ib = lane >> 4;
 #pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const int i = i0 + 2 * ti + a, j = j0 + 4 * tj + c;
-            if (i < m && j < q) dst[(long)i * q + j] = acc[a][c];
+    for (int r = 0; r < 4; r++) {
+        const int i = i0 + isub * 16 + ib + 4 * r;
+        const int ja = j0 + jsub0 * 16 + jl, jb = ja + 16;
+        if (i < m) {
+            if (ja < q) dst[(long)i * q + ja] = acc0[r];
+            if (jb < q) dst[(long)i * q + jb] = acc1[r];
         }
+    }
 }
 
+// sums the per-chunk partials of every output in a fixed order: 8 chunk groups x 32 outputs per workgroup
 __global__ __launch_bounds__(256) void k_gram_finish(const double *partials, int chunks, long mq, double scale,
                                                       double *G) {
-    const long idx = blockIdx.x * 256L + threadIdx.x;
-    if (idx >= mq) return;
+    __shared__ double sh[8][33];
+    const int o = threadIdx.x & 31, g = threadIdx.x >> 5;
+    const long out = blockIdx.x * 32L + o;
     double acc = 0.0;
-    for (int c = 0; c < chunks; c++) acc += partials[(long)c * mq + idx];
-    G[idx] = acc * scale;
+    if (out < mq)
+        for (int c = g; c < chunks; c += 8) acc += partials[(long)c * mq + out];
+    sh[g][o] = acc;
+    __syncthreads();
+    if (g == 0 && out < mq) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) s += sh[k][o];
+        G[out] = s * scale;
+    }
 }
 
 static int gram_chunks(int m, int q, long N) {
-    const int blocks = ((m + GR_BI - 1) / GR_BI) * ((q + GR_BJ - 1) / GR_BJ);
+    const int blocks = ((m + G2_TI - 1) / G2_TI) * ((q + G2_TJ - 1) / G2_TJ);
     int chunks = (1024 + blocks - 1) / blocks;
-    const long max_chunks = (N + GR_P - 1) / GR_P;
+    const long max_chunks = (N + G2_P - 1) / G2_P;
     if (chunks > max_chunks) chunks = (int)max_chunks;
     if (chunks < 1) chunks = 1;
     return chunks;
@@ -393,40 +435,46 @@ void gram(const Cols &S, const Cols &U, long N, double scale, double *partials, 
     const int chunks = gram_chunks(m, q, N);
     if ((size_t)chunks * m * q > partial_cap) throw InvalidError("gram: partial buffer too small");
     long chunk = (N + chunks - 1) / chunks;
-    chunk = ((chunk + GR_P - 1) / GR_P) * GR_P;
-    const int blocks = ((m + GR_BI - 1) / GR_BI) * ((q + GR_BJ - 1) / GR_BJ);
+    chunk = ((chunk + G2_P - 1) / G2_P) * G2_P;
+    const int blocks = ((m + G2_TI - 1) / G2_TI) * ((q + G2_TJ - 1) / G2_TJ);
     hipLaunchKernelGGL(k_gram<T>, dim3(chunks, blocks), dim3(256), 0, s, to_ptrs(S), to_ptrs(U), m, q, N, chunk,
                        partials);
     LIRA_HIP(hipGetLastError());
     const long mq = (long)m * q;
-    hipLaunchKernelGGL(k_gram_finish, dim3((unsigned)((mq + 255) / 256)), dim3(256), 0, s, partials, chunks, mq, scale,
+    hipLaunchKernelGGL(k_gram_finish, dim3((unsigned)((mq + 31) / 32)), dim3(256), 0, s, partials, chunks, mq, scale,
                        G);
     LIRA_HIP(hipGetLastError());
 }
 
 // ------------------------------------------------------------------------------------------ lincomb ----
-// Y_j = sum_i S_i C_ij for up to 16 outputs per pass (grid.y groups of 16 outputs); f64 accumulation.
-constexpr int LC_Q = 16;
-
-template <typename T>
+// Y_j = sum_i S_i C_ij: one thread per point holds QB outputs in f64 registers; C (m <= 64 rows) and the column
+// pointers are staged in LDS once per workgroup; grid.y covers q in groups of QB.
+template <typename T, int QB>
 __global__ __launch_bounds__(256) void k_lincomb(ColPtrs S, int m, const double *__restrict__ C, int q,
                                                  T *__restrict__ Y, int qsplit, T *__restrict__ Y2, long N) {
+    __shared__ double cs[64 * QB];
+    __shared__ const T *cols[64];
+    const int j0 = blockIdx.y * QB;
+    const int qn = min(QB, q - j0);
+    for (int e = threadIdx.x; e < m * QB; e += 256) {
+        const int i = e / QB, c = e - i * QB;
+        cs[e] = c < qn ? C[(long)i * q + j0 + c] : 0.0;
+    }
+    for (int i = threadIdx.x; i < m; i += 256) cols[i] = static_cast<const T *>(col_base(S, i, N, sizeof(T)));
+    __syncthreads();
     const long p = blockIdx.x * 256L + threadIdx.x;
-    const int j0 = blockIdx.y * LC_Q;
     if (p >= N) return;
-    double acc[LC_Q];
+    double acc[QB];
 #pragma unroll
-    for (int c = 0; c < LC_Q; c++) acc[c] = 0.0;
-    const int qn = min(LC_Q, q - j0);
+    for (int c = 0; c < QB; c++) acc[c] = 0.0;
     for (int i = 0; i < m; i++) {
-        const double sv = (double)static_cast<const T *>(col_base(S, i, N, sizeof(T)))[p];
-        const double *row = C + (long)i * q + j0;
+        const double sv = (double)cols[i][p];
+        const double *row = cs + i * QB;
 #pragma unroll
-        for (int c = 0; c < LC_Q; c++)
-            if (c < qn) acc[c] += sv * row[c];
+        for (int c = 0; c < QB; c++) acc[c] += sv * row[c];
     }
 #pragma unroll
-    for (int c = 0; c < LC_Q; c++)
+    for (int c = 0; c < QB; c++)
         if (c < qn) {
             const int j = j0 + c;
             T *dst = j < qsplit ? Y + (long)j * N : Y2 + (long)(j - qsplit) * N;
@@ -438,9 +486,17 @@ template <typename T>
 void lincomb(const Cols &S, const double *C, int q, T *Y, long N, hipStream_t s, int qsplit, T *Y2) {
     const int m = S.total();
     if (q <= 0) return;
+    if (m > 64) throw InvalidError("lincomb: more than 64 input columns");
     if (qsplit < 0 || qsplit > q) qsplit = q;
-    hipLaunchKernelGGL(k_lincomb<T>, dim3((unsigned)((N + 255) / 256), (q + LC_Q - 1) / LC_Q), dim3(256), 0, s,
-                       to_ptrs(S), m, C, q, Y, qsplit, Y2, N);
+    const unsigned gx = (unsigned)((N + 255) / 256);
+    if (q <= 8) {
+        hipLaunchKernelGGL((k_lincomb<T, 8>), dim3(gx, 1), dim3(256), 0, s, to_ptrs(S), m, C, q, Y, qsplit, Y2, N);
+    } else if (q <= 16) {
+        hipLaunchKernelGGL((k_lincomb<T, 16>), dim3(gx, 1), dim3(256), 0, s, to_ptrs(S), m, C, q, Y, qsplit, Y2, N);
+    } else {
+        hipLaunchKernelGGL((k_lincomb<T, 32>), dim3(gx, (q + 31) / 32), dim3(256), 0, s, to_ptrs(S), m, C, q, Y,
+                           qsplit, Y2, N);
+    }
     LIRA_HIP(hipGetLastError());
 }
 

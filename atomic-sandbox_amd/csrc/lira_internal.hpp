@@ -72,7 +72,7 @@ struct Seg {
     const void *base;
     int count;
 };
-constexpr int MAX_SEGS = 4;
+constexpr int MAX_SEGS = 8;
 struct Cols {
     Seg seg[MAX_SEGS];
     int nseg;

@@ -157,6 +157,25 @@ static int witness_keep(const std::vector<double> &ev, const std::vector<double>
     return m;
 }
 
+// qm/src/occupations.rs:271-291: the columns whose upper-bound occupation f(theta - r; mu) the tail sum needs to stay
+// below LOOSE_F_FACTOR * tol are tight; returns 1 + the largest column position among them (0 if none)
+static int tight_columns(const std::vector<double> &ev, const std::vector<double> &res, double mu, double kt,
+                         double tol) {
+    if (std::isnan(mu)) return 0;
+    auto o = ascending(ev);
+    const double limit = 1e-2 * tol;
+    double tail = 0.0;
+    int loose_from = (int)o.size();
+    for (int rank = (int)o.size() - 1; rank >= 0; rank--) {
+        tail += occupation((ev[o[rank]] - res[o[rank]] - mu) / kt);
+        if (!(tail < limit)) break;
+        loose_from = rank;
+    }
+    int need = 0;
+    for (int r = 0; r < loose_from; r++) need = std::max(need, o[r] + 1);
+    return need;
+}
+
 // ------------------------------------------------------------------------------------ solver state ----
 struct PoissonOut {
     int iterations = 0, outer = 0;
@@ -185,6 +204,7 @@ struct SolverIface {
 
 template <typename T> struct Channel {
     int kcap = 0, k = 0;
+    int tight = 0;  // Fermi-Dirac: the leading columns held to the eigen tolerance (the rest to the guard tolerance)
     DevBuf X, HX, R, W, HW, P, HP, XN, PN, TMP;
     std::vector<double> evals, occ, res;
 };
@@ -193,7 +213,9 @@ template <typename T> class Solver final : public SolverIface {
   public:
     Solver(int n, double h);
     ~Solver() override {
+        if (stream_) (void)hipStreamSynchronize(stream_);
         if (hbuf_) (void)hipHostFree(hbuf_);
+        if (up_) (void)hipHostFree(up_);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
     lira_status solve(const lira_system *sys, const lira_settings *st, int warm, lira_abort_fn abort, void *user,
@@ -218,6 +240,8 @@ template <typename T> class Solver final : public SolverIface {
     hipStream_t stream_ = nullptr;
     double *hbuf_ = nullptr;
     size_t hcap_ = 0;
+    double *up_ = nullptr;  // pinned upload arena: small host->device copies stay asynchronous
+    size_t up_cap_ = 0, up_pos_ = 0;
     DevBuf partials_, dres_, gpart_, dG_, dC_, dsmall_;
     size_t gpart_cap_ = 0, dG_cap_ = 0, dC_cap_ = 0;
     int readbacks_ = 0;
@@ -258,12 +282,26 @@ template <typename T> class Solver final : public SolverIface {
         ensure_host(count);
         LIRA_HIP(hipMemcpyAsync(hbuf_, d, count * sizeof(double), hipMemcpyDeviceToHost, stream_));
         LIRA_HIP(hipStreamSynchronize(stream_));
+        up_pos_ = 0;  // the stream is idle: every queued upload has been consumed
         readbacks_++;
         return hbuf_;
     }
+    // queues a host->device copy of count doubles through the pinned arena (no wait)
     void upload(double *d, const double *h, size_t count) {
-        LIRA_HIP(hipMemcpyAsync(d, h, count * sizeof(double), hipMemcpyHostToDevice, stream_));
-        LIRA_HIP(hipStreamSynchronize(stream_));
+        if (count == 0) return;
+        if (up_pos_ + count > up_cap_) {
+            LIRA_HIP(hipStreamSynchronize(stream_));
+            up_pos_ = 0;
+            if (count > up_cap_) {
+                if (up_) LIRA_HIP(hipHostFree(up_));
+                up_cap_ = std::max(count, (size_t)1 << 18);
+                LIRA_HIP(hipHostMalloc((void **)&up_, up_cap_ * sizeof(double), hipHostMallocDefault));
+            }
+        }
+        double *src = up_ + up_pos_;
+        std::memcpy(src, h, count * sizeof(double));
+        LIRA_HIP(hipMemcpyAsync(d, src, count * sizeof(double), hipMemcpyHostToDevice, stream_));
+        up_pos_ += count;
     }
     T *fld(DevBuf &b) { return b.template as<T>(); }
     void alloc_field(DevBuf &b, int count = 1) {
@@ -288,9 +326,40 @@ template <typename T> class Solver final : public SolverIface {
         }
     }
     void ensure_channel(Channel<T> &c, int kc);
+    int variant_ = -1;  // the stencil tile variant the autotuner picked for this grid
     void hamiltonian(const T *v, const T *x, T *out, int nb) {
         StencilParams sp{n_, nb, h_, 1.0, 0.0, 0.0};
-        stencil<T>(ST_HAM, sp, x, out, v, nullptr, nullptr, stream_);
+        stencil<T>(ST_HAM, sp, x, out, v, nullptr, nullptr, stream_, variant_);
+    }
+    // times every stencil tile variant on this grid with nb bands (the block size of the solve) and keeps the fastest
+    void tune_stencil(int nb) {
+        if (variant_ >= 0 || nb <= 0) return;
+        DevBuf x((size_t)N_ * nb * sizeof(T)), o((size_t)N_ * nb * sizeof(T)), v((size_t)N_ * sizeof(T));
+        fill<T>(x.template as<T>(), N_ * nb, 0.0, stream_);
+        fill<T>(v.template as<T>(), N_, 0.0, stream_);
+        hipEvent_t a, b;
+        LIRA_HIP(hipEventCreate(&a));
+        LIRA_HIP(hipEventCreate(&b));
+        StencilParams sp{n_, nb, h_, 1.0, 0.0, 0.0};
+        float best = 1e30f;
+        int pick = 0;
+        for (int var = 0; var < stencil_variants(); var++) {
+            stencil<T>(ST_HAM, sp, x.template as<T>(), o.template as<T>(), v.template as<T>(), nullptr, nullptr, stream_, var);
+            LIRA_HIP(hipEventRecord(a, stream_));
+            for (int r = 0; r < 3; r++)
+                stencil<T>(ST_HAM, sp, x.template as<T>(), o.template as<T>(), v.template as<T>(), nullptr, nullptr, stream_, var);
+            LIRA_HIP(hipEventRecord(b, stream_));
+            LIRA_HIP(hipEventSynchronize(b));
+            float ms = 0.f;
+            LIRA_HIP(hipEventElapsedTime(&ms, a, b));
+            if (ms < best) {
+                best = ms;
+                pick = var;
+            }
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        variant_ = pick;
     }
     std::vector<double> reduce(RedJobs<T> &jobs, double scale) {
         reduce_jobs<T>(jobs, n_, scale, partials_.as<double>(), dres_.as<double>(), stream_);
@@ -304,12 +373,15 @@ template <typename T> class Solver final : public SolverIface {
     int pcg(const T *r0, T *e, double rel_tol, int max_iter, double *rel_out);
     // LOBPCG on a channel's X (k columns) for H = -1/2 lap + v
     EigenOut lobpcg(Channel<T> &c, const T *v, int k, double lam, double tol, int max_iter, double tail_tol, int head);
-    void solve_channel(int spin, const T *v, int nel, bool fd, double kt, double tol, EigenOut &eo, Fd &occ);
+    void solve_channel(int spin, const T *v, int nel, bool fd, double kt, double tol, double scf_tol, EigenOut &eo,
+                       Fd &occ);
     void upload_nuclei(const lira_nucleus *nuc, int nat);
 };
 
 template <typename T> Solver<T>::Solver(int n, double h) : n_(n), h_(h), dv_(h * h * h), N_((long)n * n * n) {
     LIRA_HIP(hipStreamCreate(&stream_));
+    up_cap_ = (size_t)1 << 18;
+    LIRA_HIP(hipHostMalloc((void **)&up_, up_cap_ * sizeof(double), hipHostMallocDefault));
     partials_.alloc((size_t)RED_BLOCKS * std::max(MAX_JOBS, 49) * sizeof(double) * 4);
     dres_.alloc(1024 * sizeof(double));
     dsmall_.alloc(4096 * sizeof(double));
@@ -579,7 +651,7 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
         residual<T>(c.R.template as<T>(), c.HX.template as<T>(), X(), dth, k, N_, stream_);
         precond_init<T>(c.W.template as<T>(), c.R.template as<T>(), v, dlam, diag_t, k, N_, stream_);
         for (int s = 0; s < 3; s++) {
-            stencil<T>(ST_JAC, jp, c.W.template as<T>(), c.TMP.template as<T>(), v, c.R.template as<T>(), dlam, stream_);
+            stencil<T>(ST_JAC, jp, c.W.template as<T>(), c.TMP.template as<T>(), v, c.R.template as<T>(), dlam, stream_, variant_);
             std::swap(c.W, c.TMP);
         }
         hamiltonian(v, c.W.template as<T>(), c.HW.template as<T>(), k);
@@ -587,12 +659,11 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
         const int m = 2 * k + np, q = 2 * m;
         ensure_gram(m, q);
         Cols S{{{X(), k}, {c.W.ptr, k}, {c.P.ptr, np}}, np > 0 ? 3 : 2};
-        // B = S^T S and S^T [R, HW, HP] as two Gram products (a column list holds at most 4 segments)
-        Cols Ss = S;
-        Cols Tt{{{c.R.ptr, k}, {c.HW.ptr, k}, {c.HP.ptr, np}}, np > 0 ? 3 : 2};
+        // one Gram product: G = S^T [S, R, HW, HP] (m x 2m); its left half is B = S^T S, its right half S^T T
+        Cols U{{{X(), k}, {c.W.ptr, k}, {c.P.ptr, np}, {c.R.ptr, k}, {c.HW.ptr, k}, {c.HP.ptr, np}}, 6};
+        if (np == 0) U = Cols{{{X(), k}, {c.W.ptr, k}, {c.R.ptr, k}, {c.HW.ptr, k}}, 4};
         double *dG = dG_.as<double>();
-        gram<T>(S, Ss, N_, dv_, gpart_.as<double>(), gpart_cap_, dG, stream_);
-        gram<T>(S, Tt, N_, dv_, gpart_.as<double>(), gpart_cap_, dG + (size_t)m * m, stream_);
+        gram<T>(S, U, N_, dv_, gpart_.as<double>(), gpart_cap_, dG, stream_);
         RedJobs<T> jr{};
         for (int i = 0; i < k; i++) {
             jr.a[i % MAX_JOBS] = c.R.template as<T>() + (long)i * N_;
@@ -600,17 +671,17 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
             jr.kind[i % MAX_JOBS] = RED_DOT;
             jr.count = i % MAX_JOBS + 1;
             if (jr.count == MAX_JOBS || i == k - 1) {
-                reduce_jobs<T>(jr, n_, dv_, partials_.as<double>(), dG + (size_t)2 * m * m + (i / MAX_JOBS) * MAX_JOBS,
+                reduce_jobs<T>(jr, n_, dv_, partials_.as<double>(), dG + (size_t)m * q + (i / MAX_JOBS) * MAX_JOBS,
                                stream_);
                 jr.count = 0;
             }
         }
-        const double *g = readback(dG, (size_t)2 * m * m + k);
+        const double *g = readback(dG, (size_t)m * q + k);
         std::vector<double> Bm((size_t)m * m), Am((size_t)m * m);
         for (int i = 0; i < m; i++)
             for (int j = 0; j <= i; j++) {
-                const double bij = g[(size_t)i * m + j];
-                double aij = g[(size_t)m * m + (size_t)i * m + j];
+                const double bij = g[(size_t)i * q + j];
+                double aij = g[(size_t)i * q + m + j];
                 if (j < k) aij += theta[j] * bij;
                 Bm[(size_t)i * m + j] = Bm[(size_t)j * m + i] = bij;
                 Am[(size_t)i * m + j] = Am[(size_t)j * m + i] = aij;
@@ -620,7 +691,7 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
         for (int i = 0; i < k; i++) {
             const double xx = Bm[(size_t)i * m + i];
             lambda[i] = Am[(size_t)i * m + i] / xx;
-            res[i] = std::sqrt(std::max(g[(size_t)2 * m * m + i], 0.0) / xx);
+            res[i] = std::sqrt(std::max(g[(size_t)m * q + i], 0.0) / xx);
             if (!std::isfinite(lambda[i]) || !std::isfinite(res[i])) finite = false;
             max_res = std::max(max_res, res[i]);
             if (i < head) head_res = std::max(head_res, res[i]);
@@ -693,9 +764,11 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
     return out;
 }
 
-// one spin channel under Aufbau or Fermi–Dirac (qm/src/scf.rs:774-877; every column held to tol)
+// one spin channel under Aufbau or Fermi–Dirac (qm/src/scf.rs:774-877): under Fermi–Dirac the witness columns whose
+// occupation is negligible against the SCF tolerance are held to the guard tolerance (the reference's Tiers::Loose)
 template <typename T>
-void Solver<T>::solve_channel(int spin, const T *v, int nel, bool fd, double kt, double tol, EigenOut &eo, Fd &occ) {
+void Solver<T>::solve_channel(int spin, const T *v, int nel, bool fd, double kt, double tol, double scf_tol,
+                              EigenOut &eo, Fd &occ) {
     Channel<T> &c = ch_[spin];
     if (c.k == 0) {
         occ = Fd{};
@@ -704,10 +777,12 @@ void Solver<T>::solve_channel(int spin, const T *v, int nel, bool fd, double kt,
     const double guard = std::max(WITNESS_GUARD_TOL, tol);
     for (;;) {
         const double lam = c.evals.empty() ? -1.0 : c.evals[0];
-        EigenOut e = lobpcg(c, v, c.k, lam, tol, 200, tol, c.k);
+        const int head = fd ? std::min(std::max(c.tight, 0), c.k) : c.k;
+        EigenOut e = lobpcg(c, v, c.k, lam, tol, 200, fd ? guard : tol, head);
         eo.iterations += e.iterations;
         eo.fallbacks += e.fallbacks;
-        eo.max_residual = std::max(eo.max_residual, e.max_residual);
+        for (int i = 0; i < c.k; i++)
+            if (i < head) eo.max_residual = std::max(eo.max_residual, c.res[i]);
         if (!fd) {
             occ.f.assign(c.k, 1.0);
             occ.mu = NAN;
@@ -716,6 +791,9 @@ void Solver<T>::solve_channel(int spin, const T *v, int nel, bool fd, double kt,
             return;
         }
         occ = fermi_dirac(c.evals, nel, kt);
+        const int need = tight_columns(c.evals, c.res, occ.mu, kt, scf_tol);
+        c.tight = need;
+        if (need > head) continue;  // a loose column needs the eigen tolerance: re-solve with it tight
         const int extras = c.k - nel;
         auto o = ascending(c.evals);
         const double top_r = c.res[o[c.k - 1]];
@@ -842,11 +920,13 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
             c.evals.assign(k, -1.0);
             c.res.assign(k, 0.0);
             c.occ.assign(k, 1.0);
+            c.tight = nel[s];
         }
         fill<double>(vh_.as<double>(), N_, 0.0, stream_);
     } else {
         for (int s = 0; s < 2; s++) copy<T>(rho_in_[s].template as<T>(), rho_out_[s].template as<T>(), N_, stream_);
     }
+    tune_stencil(std::max(ch_[0].k, 1));
     const double ptol = std::min(1e-10, 1e-3 * st->tol);  // qm/src/scf.rs:519
     const double lo = effective_eig_tol(st->eig_tol), hi = std::max(EIG_TOL_MAX, lo);
     const int H = std::max(st->diis_history, 1);
@@ -887,7 +967,7 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
         const double tol_step = std::isfinite(delta_prev) ? std::min(std::max(EIG_TOL_FRACTION * delta_prev, lo), hi) : hi;
         const bool tight = tol_step <= lo;
         auto te = Clock::now();
-        solve_channel(0, veff_[0].template as<T>(), nel[0], fd, kt, tol_step, eo, occ[0]);
+        solve_channel(0, veff_[0].template as<T>(), nel[0], fd, kt, tol_step, st->tol, eo, occ[0]);
         if (restricted) {
             Channel<T> &u = ch_[0], &d = ch_[1];
             ensure_channel(d, std::max(u.k, 1));
@@ -899,7 +979,7 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
             d.occ = u.occ;
             occ[1] = occ[0];
         } else {
-            solve_channel(1, veff_[1].template as<T>(), nel[1], fd, kt, tol_step, eo, occ[1]);
+            solve_channel(1, veff_[1].template as<T>(), nel[1], fd, kt, tol_step, st->tol, eo, occ[1]);
         }
         t_eigen += secs(te);
         // output densities and delta_rho
