@@ -1,6 +1,7 @@
 // Host f64 dense linear algebra of the solver's small problems: the symmetric eigensolver, LU and Cholesky, and the
 // generalised Rayleigh–Ritz step of LOBPCG (m <= a few hundred). These run on the CPU in f64 in every precision mode.
 #include <algorithm>
+#include <limits>
 #include <cmath>
 #include <numeric>
 
@@ -10,7 +11,7 @@ namespace lira {
 
 // Cyclic Jacobi (Golub & Van Loan §8.5; the reference's fallback eigensolver, crates/qm/src/eigen.rs:175-211):
 // rotations until the off-diagonal norm is below 1e-15 ||A||; pairs returned ascending. a is m x m row-major.
-void sym_eigen(int m, std::vector<double> &a, std::vector<double> &evals, std::vector<double> &evecs) {
+static void jacobi_eigen(int m, std::vector<double> &a, std::vector<double> &evals, std::vector<double> &evecs) {
     std::vector<double> v((size_t)m * m, 0.0);
     for (int i = 0; i < m; i++) v[(size_t)i * m + i] = 1.0;
     double norm = 0.0;
@@ -57,6 +58,143 @@ void sym_eigen(int m, std::vector<double> &a, std::vector<double> &evals, std::v
         evals[c] = a[(size_t)order[c] * m + order[c]];
         for (int r = 0; r < m; r++) evecs[(size_t)r * m + c] = v[(size_t)r * m + order[c]];
     }
+}
+
+// Householder reduction to tridiagonal form followed by the implicit-shift QL iteration (Golub & Van Loan §8.3),
+// accumulating the orthogonal transformations. Returns false if the QL iteration does not converge.
+static bool tridiagonal_ql_eigen(int m, std::vector<double> a, std::vector<double> &evals, std::vector<double> &evecs) {
+    std::vector<double> q((size_t)m * m, 0.0), v(m), p(m), w(m), qv(m);
+    for (int i = 0; i < m; i++) q[(size_t)i * m + i] = 1.0;
+    for (int k = 0; k + 2 < m; k++) {
+        const int len = m - k - 1;
+        double xn = 0.0;
+        for (int i = 0; i < len; i++) {
+            v[i] = a[(size_t)(k + 1 + i) * m + k];
+            xn += v[i] * v[i];
+        }
+        if (xn == 0.0) continue;
+        double alpha = std::sqrt(xn);
+        if (v[0] > 0) alpha = -alpha;
+        v[0] -= alpha;  // v = x - alpha e1, H x = alpha e1
+        double vv = 0.0;
+        for (int i = 0; i < len; i++) vv += v[i] * v[i];
+        if (vv == 0.0) continue;
+        const double beta = 2.0 / vv;
+        // trailing block B <- H B H: p = beta B v, K = beta/2 v.p, w = p - K v, B -= v w^T + w v^T
+        for (int i = 0; i < len; i++) {
+            double sum = 0.0;
+            const double *row = &a[(size_t)(k + 1 + i) * m + k + 1];
+            for (int j = 0; j < len; j++) sum += row[j] * v[j];
+            p[i] = beta * sum;
+        }
+        double kk = 0.0;
+        for (int i = 0; i < len; i++) kk += v[i] * p[i];
+        kk *= 0.5 * beta;
+        for (int i = 0; i < len; i++) w[i] = p[i] - kk * v[i];
+        for (int i = 0; i < len; i++) {
+            double *row = &a[(size_t)(k + 1 + i) * m + k + 1];
+            for (int j = 0; j < len; j++) row[j] -= v[i] * w[j] + w[i] * v[j];
+        }
+        a[(size_t)(k + 1) * m + k] = a[(size_t)k * m + k + 1] = alpha;
+        for (int i = 1; i < len; i++) a[(size_t)(k + 1 + i) * m + k] = a[(size_t)k * m + k + 1 + i] = 0.0;
+        // Q <- Q H on columns k+1..
+        for (int r = 0; r < m; r++) {
+            const double *qr = &q[(size_t)r * m + k + 1];
+            double sum = 0.0;
+            for (int j = 0; j < len; j++) sum += qr[j] * v[j];
+            qv[r] = beta * sum;
+        }
+        for (int r = 0; r < m; r++) {
+            double *qr = &q[(size_t)r * m + k + 1];
+            for (int j = 0; j < len; j++) qr[j] -= qv[r] * v[j];
+        }
+    }
+    std::vector<double> d(m), e(m, 0.0);
+    for (int i = 0; i < m; i++) d[i] = a[(size_t)i * m + i];
+    for (int i = 0; i + 1 < m; i++) e[i] = a[(size_t)(i + 1) * m + i];
+    // implicit QL on (d, e): e[i] couples rows i and i + 1
+    const double eps = std::numeric_limits<double>::epsilon();
+    for (int l = 0; l < m; l++) {
+        int iter = 0, mm;
+        do {
+            for (mm = l; mm < m - 1; mm++) {
+                const double dd = std::fabs(d[mm]) + std::fabs(d[mm + 1]);
+                if (std::fabs(e[mm]) <= eps * dd) break;
+            }
+            if (mm != l) {
+                if (iter++ == 60) return false;
+                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                double r = std::hypot(g, 1.0);
+                g = d[mm] - d[l] + e[l] / (g + std::copysign(r, g));
+                double s = 1.0, c = 1.0, pp = 0.0;
+                int i;
+                for (i = mm - 1; i >= l; i--) {
+                    double f = s * e[i];
+                    const double b = c * e[i];
+                    r = std::hypot(f, g);
+                    e[i + 1] = r;
+                    if (r == 0.0) {
+                        d[i + 1] -= pp;
+                        e[mm] = 0.0;
+                        break;
+                    }
+                    s = f / r;
+                    c = g / r;
+                    g = d[i + 1] - pp;
+                    r = (d[i] - g) * s + 2.0 * c * b;
+                    pp = s * r;
+                    d[i + 1] = g + pp;
+                    g = c * r - b;
+                    for (int k = 0; k < m; k++) {
+                        f = q[(size_t)k * m + i + 1];
+                        q[(size_t)k * m + i + 1] = s * q[(size_t)k * m + i] + c * f;
+                        q[(size_t)k * m + i] = c * q[(size_t)k * m + i] - s * f;
+                    }
+                }
+                if (r == 0.0 && i >= l) continue;
+                d[l] -= pp;
+                e[l] = g;
+                e[mm] = 0.0;
+            }
+        } while (mm != l);
+    }
+    std::vector<int> order(m);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return d[x] < d[y]; });
+    evals.assign(m, 0.0);
+    evecs.assign((size_t)m * m, 0.0);
+    for (int c = 0; c < m; c++) {
+        evals[c] = d[order[c]];
+        for (int r = 0; r < m; r++) evecs[(size_t)r * m + c] = q[(size_t)r * m + order[c]];
+    }
+    return true;
+}
+
+// The checked symmetric eigen-decomposition (crates/qm/src/eigen.rs:141-170): Householder + implicit QL, then every
+// pair's residual ||A v - lambda v|| against 1e-11 (1 + max |lambda|); a decomposition that fails the check (or a QL
+// that does not converge) is replaced by cyclic Jacobi. Pairs ascending.
+void sym_eigen(int m, std::vector<double> &a, std::vector<double> &evals, std::vector<double> &evecs) {
+    if (m <= 0) {
+        evals.clear();
+        evecs.clear();
+        return;
+    }
+    if (tridiagonal_ql_eigen(m, a, evals, evecs)) {
+        double scale = 1.0;
+        for (double x : evals) scale = std::max(scale, 1.0 + std::fabs(x));
+        bool ok = true;
+        for (int c = 0; c < m && ok; c++) {
+            double r2 = 0.0;
+            for (int i = 0; i < m; i++) {
+                double s = -evals[c] * evecs[(size_t)i * m + c];
+                for (int j = 0; j < m; j++) s += a[(size_t)i * m + j] * evecs[(size_t)j * m + c];
+                r2 += s * s;
+            }
+            ok = std::sqrt(r2) <= 1e-11 * scale;
+        }
+        if (ok) return;
+    }
+    jacobi_eigen(m, a, evals, evecs);
 }
 
 // LU with partial pivoting; false on a zero pivot (nalgebra's LU::solve returning None, crates/qm/src/scf.rs:696)

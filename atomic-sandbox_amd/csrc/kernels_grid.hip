@@ -38,130 +38,458 @@ void DevBuf::release() {
 // LDS tile with a 4-point halo (double-buffered: one barrier per plane). The potential V is read once per point.
 constexpr int ST_TX = 64;
 
-template <typename T, int MODE, int RPT>
-__global__ __launch_bounds__(256) void k_stencil(const T *__restrict__ x, T *__restrict__ out, const T *__restrict__ v,
-                                                 const T *__restrict__ r, const double *__restrict__ lam, int n,
-                                                 long N, int zc, T inv_h2, T scale, T omega, T diag_t) {
-    constexpr int TY = 4 * RPT;
-    constexpr int LX = ST_TX + 8;
-    constexpr int PF = 2;  // planes of look-ahead for the halo, V and R loads
-    constexpr bool USE_V = MODE != ST_LAP;
-    constexpr bool USE_R = MODE == ST_JAC;
-    __shared__ T tile[2][TY + 8][LX];
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int t = ty * 64 + tx;
-    const int x0 = blockIdx.x * ST_TX, y0 = blockIdx.y * TY;
-    const int nzc = (n + zc - 1) / zc;
-    const int band = blockIdx.z / nzc, zi = blockIdx.z - band * nzc;
-    const int z0 = zi * zc;
-    const int z1 = min(n, z0 + zc);
-    const T *xb = x + (long)band * N;
-    T *ob = out + (long)band * N;
-    const T *rb = USE_R ? r + (long)band * N : nullptr;
-    const T lamb = USE_R ? (T)lam[band] : T(0);
-    const int gx = x0 + tx;
-    const bool xin = gx < n;
-    const long nn = (long)n * n;
+// Register state of one thread: the 9-plane z window of its RPT columns (plane p in slot (p - z0 + 4) mod 9) and the
+// look-ahead queues of its halo cells, V and R (plane p in slot (p - z0) mod 3). The z loop is unrolled by 9 so every
+// slot index is a compile-time constant: registers rotate by name and are never copied, so a look-ahead load is only
+// waited for when its plane is used.
+// XCD-aware workgroup order. The dispatcher hands consecutive workgroup ids to the 8 XCDs round-robin (id mod 8);
+// each XCD has its own L2. The stencil launches a 1-D grid and gives XCD x the contiguous logical range
+// [x P, (x + 1) P) of (band, z chunk, tile) work items, tiles fastest: the tiles of one band (whose halos overlap) and
+// neighbouring bands (which share V) then run together on one XCD and meet in its L2.
+struct StWork {
+    int tile_x, tile_y, band, zchunk;
+    bool valid;
+};
+// order 0: tiles fastest (halo sharing between tiles of a band); order 1: bands fastest (the bands of one tile share
+// V and their x planes stream in step).
+__device__ __forceinline__ StWork st_decode(int ntx, int nty, int nzc, int nb, int order) {
+    const int tiles = ntx * nty;
+    const int total = tiles * nzc * nb;
+    const int per = (total + 7) / 8;
+    const int lin = blockIdx.x;
+    const int L = (lin & 7) * per + (lin >> 3);
+    StWork w;
+    w.valid = L < total;
+    int tile, rest;
+    if (order == 0) {
+        tile = L % tiles;
+        rest = L / tiles;
+        w.zchunk = rest % nzc;
+        w.band = rest / nzc;
+    } else {
+        w.band = L % nb;
+        rest = L / nb;
+        tile = rest % tiles;
+        w.zchunk = rest / tiles;
+    }
+    w.tile_x = tile % ntx;
+    w.tile_y = tile / ntx;
+    return w;
+}
+static inline unsigned st_grid(int ntx, int nty, int nzc, int nb) {
+    const long total = (long)ntx * nty * nzc * nb;
+    return (unsigned)(8 * ((total + 7) / 8));
+}
 
-    auto inb = [&](int xx, int yy, int zz) { return xx >= 0 && xx < n && yy >= 0 && yy < n && zz >= 0 && zz < n; };
-    auto ld = [&](const T *base, int xx, int yy, int zz) -> T {
+template <typename T, int RPT> struct StRegs {
+    T q[9][RPT];
+    T hq[3][3];
+    T vq[3][RPT];
+    T rq[3][RPT];
+};
+
+template <typename T, int RPT> struct StCtx {
+    const T *xb;
+    T *ob;
+    const T *v;
+    const T *rb;
+    int n;
+    long nn;
+    int gx, x0, y0, tx, ty;
+    bool xin;
+    int hly0, hly1, hxr, hlx;
+    bool has_xh;
+    T inv_h2, scale, omega, diag_t, lamb;
+    __device__ bool inb(int xx, int yy, int zz) const {
+        return xx >= 0 && xx < n && yy >= 0 && yy < n && zz >= 0 && zz < n;
+    }
+    __device__ T ld(const T *base, int xx, int yy, int zz) const {
         return inb(xx, yy, zz) ? base[xx + (long)n * yy + nn * zz] : T(0);
-    };
-    // this thread's fixed halo cells of a plane: two y-halo cells and (for t < 8 (TY + 8)) one x-halo cell
-    const int hy_row0 = t >> 6, hy_row1 = (t + 256) >> 6;
-    const int hly0 = hy_row0 < 4 ? hy_row0 : TY + hy_row0, hly1 = hy_row1 < 4 ? hy_row1 : TY + hy_row1;
-    const bool has_xh = t < 8 * (TY + 8);
-    const int hxr = t >> 3, hxc = t & 7;
-    const int hlx = hxc < 4 ? hxc : ST_TX + hxc;
+    }
+};
 
-    T q[9][RPT];      // interior z-window: q[d] is plane z - 4 + d
-    T hq[PF + 1][3];  // halo cells of planes z .. z + PF
-    T vq[PF + 1][RPT], rq[PF + 1][RPT];
+template <typename T, int MODE, int RPT, int SLOT>
+__device__ __forceinline__ void st_side(StRegs<T, RPT> &R, const StCtx<T, RPT> &c, int zz) {
+    constexpr int QS = SLOT % 3;
+    R.hq[QS][0] = c.ld(c.xb, c.gx, c.y0 - 4 + c.hly0, zz);
+    R.hq[QS][1] = c.ld(c.xb, c.gx, c.y0 - 4 + c.hly1, zz);
+    R.hq[QS][2] = c.has_xh ? c.ld(c.xb, c.x0 - 4 + c.hlx, c.y0 - 4 + c.hxr, zz) : T(0);
 #pragma unroll
-    for (int d = 0; d < 8; d++)
-#pragma unroll
-        for (int rr = 0; rr < RPT; rr++) q[d][rr] = ld(xb, gx, y0 + ty + 4 * rr, z0 - 4 + d);
-    auto load_side = [&](int slot, int zz) {
-        hq[slot][0] = ld(xb, gx, y0 - 4 + hly0, zz);
-        hq[slot][1] = ld(xb, gx, y0 - 4 + hly1, zz);
-        hq[slot][2] = has_xh ? ld(xb, x0 - 4 + hlx, y0 - 4 + hxr, zz) : T(0);
-#pragma unroll
-        for (int rr = 0; rr < RPT; rr++) {
-            if (USE_V) vq[slot][rr] = ld(v, gx, y0 + ty + 4 * rr, zz);
-            if (USE_R) rq[slot][rr] = ld(rb, gx, y0 + ty + 4 * rr, zz);
-        }
-    };
-#pragma unroll
-    for (int d = 0; d < PF; d++) load_side(d, z0 + d);
+    for (int rr = 0; rr < RPT; rr++) {
+        if (MODE != ST_LAP) R.vq[QS][rr] = c.ld(c.v, c.gx, c.y0 + c.ty + 4 * rr, zz);
+        if (MODE == ST_JAC) R.rq[QS][rr] = c.ld(c.rb, c.gx, c.y0 + c.ty + 4 * rr, zz);
+    }
+}
 
-    int buf = 0;
-    for (int z = z0; z < z1; z++) {
+// one plane z of the stream; S = (z - z0) mod 9
+template <typename T, int MODE, int RPT, int S>
+__device__ __forceinline__ void st_plane(StRegs<T, RPT> &R, const StCtx<T, RPT> &c, T (*tile)[4 * RPT + 8][ST_TX + 8],
+                                         int z, int buf) {
+    constexpr int C0 = (S + 4) % 9;
+    constexpr int NEW = (S + 8) % 9;
+    constexpr int HS = S % 3;
 #pragma unroll
-        for (int rr = 0; rr < RPT; rr++) q[8][rr] = ld(xb, gx, y0 + ty + 4 * rr, z + 4);
-        load_side(PF, z + PF);
+    for (int rr = 0; rr < RPT; rr++) R.q[NEW][rr] = c.ld(c.xb, c.gx, c.y0 + c.ty + 4 * rr, z + 4);
+    st_side<T, MODE, RPT, S + 2>(R, c, z + 2);
+    T(*tl)[ST_TX + 8] = tile[buf];
 #pragma unroll
-        for (int rr = 0; rr < RPT; rr++) tile[buf][ty + 4 * rr + 4][tx + 4] = q[4][rr];
-        tile[buf][hly0][tx + 4] = hq[0][0];
-        tile[buf][hly1][tx + 4] = hq[0][1];
-        if (has_xh) tile[buf][hxr][hlx] = hq[0][2];
-        __syncthreads();
+    for (int rr = 0; rr < RPT; rr++) tl[c.ty + 4 * rr + 4][c.tx + 4] = R.q[C0][rr];
+    tl[c.hly0][c.tx + 4] = R.hq[HS][0];
+    tl[c.hly1][c.tx + 4] = R.hq[HS][1];
+    if (c.has_xh) tl[c.hxr][c.hlx] = R.hq[HS][2];
+    __syncthreads();
 #pragma unroll
-        for (int rr = 0; rr < RPT; rr++) {
-            const int ly = ty + 4 * rr + 4, lx = tx + 4;
-            const T c = q[4][rr];
-            T ax = C1 * ((tile[buf][ly][lx + 1] - c) + (tile[buf][ly][lx - 1] - c)) +
-                   C2 * ((tile[buf][ly][lx + 2] - c) + (tile[buf][ly][lx - 2] - c)) +
-                   C3 * ((tile[buf][ly][lx + 3] - c) + (tile[buf][ly][lx - 3] - c)) +
-                   C4 * ((tile[buf][ly][lx + 4] - c) + (tile[buf][ly][lx - 4] - c));
-            T ay = C1 * ((tile[buf][ly + 1][lx] - c) + (tile[buf][ly - 1][lx] - c)) +
-                   C2 * ((tile[buf][ly + 2][lx] - c) + (tile[buf][ly - 2][lx] - c)) +
-                   C3 * ((tile[buf][ly + 3][lx] - c) + (tile[buf][ly - 3][lx] - c)) +
-                   C4 * ((tile[buf][ly + 4][lx] - c) + (tile[buf][ly - 4][lx] - c));
-            T az = C1 * ((q[5][rr] - c) + (q[3][rr] - c)) + C2 * ((q[6][rr] - c) + (q[2][rr] - c)) +
-                   C3 * ((q[7][rr] - c) + (q[1][rr] - c)) + C4 * ((q[8][rr] - c) + (q[0][rr] - c));
-            const T lap = ((ax + ay) + az) * inv_h2;
-            const int gy = y0 + ty + 4 * rr;
-            if (xin && gy < n) {
-                const long id = gx + (long)n * gy + nn * z;
-                if (MODE == ST_LAP) {
-                    ob[id] = scale * lap;
-                } else if (MODE == ST_HAM) {
-                    ob[id] = T(-0.5) * lap + vq[0][rr] * c;
-                } else {
-                    const T vv = vq[0][rr];
-                    const T hw = T(-0.5) * lap + vv * c - lamb * c;
-                    T d = diag_t + vv - lamb;
-                    d = d > T(0.1) ? d : T(0.1);
-                    ob[id] = c + omega / d * (rq[0][rr] - hw);
-                }
-            }
-        }
-        buf ^= 1;
-#pragma unroll
-        for (int d = 0; d < 8; d++)
-#pragma unroll
-            for (int rr = 0; rr < RPT; rr++) q[d][rr] = q[d + 1][rr];
-#pragma unroll
-        for (int d = 0; d < PF; d++) {
-#pragma unroll
-            for (int e = 0; e < 3; e++) hq[d][e] = hq[d + 1][e];
-#pragma unroll
-            for (int rr = 0; rr < RPT; rr++) {
-                if (USE_V) vq[d][rr] = vq[d + 1][rr];
-                if (USE_R) rq[d][rr] = rq[d + 1][rr];
+    for (int rr = 0; rr < RPT; rr++) {
+        const int ly = c.ty + 4 * rr + 4, lx = c.tx + 4;
+        const T ce = R.q[C0][rr];
+        T ax = C1 * ((tl[ly][lx + 1] - ce) + (tl[ly][lx - 1] - ce)) + C2 * ((tl[ly][lx + 2] - ce) + (tl[ly][lx - 2] - ce)) +
+               C3 * ((tl[ly][lx + 3] - ce) + (tl[ly][lx - 3] - ce)) + C4 * ((tl[ly][lx + 4] - ce) + (tl[ly][lx - 4] - ce));
+        T ay = C1 * ((tl[ly + 1][lx] - ce) + (tl[ly - 1][lx] - ce)) + C2 * ((tl[ly + 2][lx] - ce) + (tl[ly - 2][lx] - ce)) +
+               C3 * ((tl[ly + 3][lx] - ce) + (tl[ly - 3][lx] - ce)) + C4 * ((tl[ly + 4][lx] - ce) + (tl[ly - 4][lx] - ce));
+        T az = C1 * ((R.q[(S + 5) % 9][rr] - ce) + (R.q[(S + 3) % 9][rr] - ce)) +
+               C2 * ((R.q[(S + 6) % 9][rr] - ce) + (R.q[(S + 2) % 9][rr] - ce)) +
+               C3 * ((R.q[(S + 7) % 9][rr] - ce) + (R.q[(S + 1) % 9][rr] - ce)) +
+               C4 * ((R.q[(S + 8) % 9][rr] - ce) + (R.q[S % 9][rr] - ce));
+        const T lap = ((ax + ay) + az) * c.inv_h2;
+        const int gy = c.y0 + c.ty + 4 * rr;
+        if (c.xin && gy < c.n) {
+            const long id = c.gx + (long)c.n * gy + c.nn * z;
+            if (MODE == ST_LAP) {
+                c.ob[id] = c.scale * lap;
+            } else if (MODE == ST_HAM) {
+                c.ob[id] = T(-0.5) * lap + R.vq[HS][rr] * ce;
+            } else {
+                const T vv = R.vq[HS][rr];
+                const T hw = T(-0.5) * lap + vv * ce - c.lamb * ce;
+                T d = c.diag_t + vv - c.lamb;
+                d = d > T(0.1) ? d : T(0.1);
+                c.ob[id] = ce + c.omega / d * (R.rq[HS][rr] - hw);
             }
         }
     }
 }
 
+template <typename T, int MODE, int RPT>
+__global__ __launch_bounds__(256) void k_stencil(const T *__restrict__ x, T *__restrict__ out, const T *__restrict__ v,
+                                                 const T *__restrict__ r, const double *__restrict__ lam, int n,
+                                                 long N, int zc, int nb, int order, T inv_h2, T scale, T omega,
+                                                 T diag_t) {
+    constexpr int TY = 4 * RPT;
+    __shared__ T tile[2][TY + 8][ST_TX + 8];
+    StCtx<T, RPT> c;
+    c.tx = threadIdx.x;
+    c.ty = threadIdx.y;
+    const int t = c.ty * 64 + c.tx;
+    const int nzc = (n + zc - 1) / zc;
+    const StWork wk = st_decode((n + ST_TX - 1) / ST_TX, (n + TY - 1) / TY, nzc, nb, order);
+    if (!wk.valid) return;
+    c.x0 = wk.tile_x * ST_TX;
+    c.y0 = wk.tile_y * TY;
+    const int band = wk.band;
+    const int z0 = wk.zchunk * zc;
+    const int z1 = min(n, z0 + zc);
+    c.xb = x + (long)band * N;
+    c.ob = out + (long)band * N;
+    c.v = v;
+    c.rb = MODE == ST_JAC ? r + (long)band * N : nullptr;
+    c.lamb = MODE == ST_JAC ? (T)lam[band] : T(0);
+    c.n = n;
+    c.nn = (long)n * n;
+    c.gx = c.x0 + c.tx;
+    c.xin = c.gx < n;
+    const int hy0 = t >> 6, hy1 = (t + 256) >> 6;
+    c.hly0 = hy0 < 4 ? hy0 : TY + hy0;
+    c.hly1 = hy1 < 4 ? hy1 : TY + hy1;
+    c.has_xh = t < 8 * (TY + 8);
+    c.hxr = t >> 3;
+    const int hxc = t & 7;
+    c.hlx = hxc < 4 ? hxc : ST_TX + hxc;
+    c.inv_h2 = inv_h2;
+    c.scale = scale;
+    c.omega = omega;
+    c.diag_t = diag_t;
+
+    StRegs<T, RPT> R;
+#pragma unroll
+    for (int d = 0; d < 8; d++)
+#pragma unroll
+        for (int rr = 0; rr < RPT; rr++) R.q[d][rr] = c.ld(c.xb, c.gx, c.y0 + c.ty + 4 * rr, z0 - 4 + d);
+    st_side<T, MODE, RPT, 0>(R, c, z0);
+    st_side<T, MODE, RPT, 1>(R, c, z0 + 1);
+
+    // consecutive planes alternate the two LDS buffers (plane p uses buffer (p - z0) & 1): one barrier per plane then
+    // separates every write of a buffer from the previous reads of it
+    int z = z0;
+    for (; z + 9 <= z1; z += 9) {
+        const int b = (z - z0) & 1;
+        st_plane<T, MODE, RPT, 0>(R, c, tile, z + 0, b);
+        st_plane<T, MODE, RPT, 1>(R, c, tile, z + 1, b ^ 1);
+        st_plane<T, MODE, RPT, 2>(R, c, tile, z + 2, b);
+        st_plane<T, MODE, RPT, 3>(R, c, tile, z + 3, b ^ 1);
+        st_plane<T, MODE, RPT, 4>(R, c, tile, z + 4, b);
+        st_plane<T, MODE, RPT, 5>(R, c, tile, z + 5, b ^ 1);
+        st_plane<T, MODE, RPT, 6>(R, c, tile, z + 6, b);
+        st_plane<T, MODE, RPT, 7>(R, c, tile, z + 7, b ^ 1);
+        st_plane<T, MODE, RPT, 8>(R, c, tile, z + 8, b);
+    }
+    // remainder (< 9 planes) with the same slot sequence
+    const int b = (z - z0) & 1;
+    if (z < z1) st_plane<T, MODE, RPT, 0>(R, c, tile, z + 0, b);
+    if (z + 1 < z1) st_plane<T, MODE, RPT, 1>(R, c, tile, z + 1, b ^ 1);
+    if (z + 2 < z1) st_plane<T, MODE, RPT, 2>(R, c, tile, z + 2, b);
+    if (z + 3 < z1) st_plane<T, MODE, RPT, 3>(R, c, tile, z + 3, b ^ 1);
+    if (z + 4 < z1) st_plane<T, MODE, RPT, 4>(R, c, tile, z + 4, b);
+    if (z + 5 < z1) st_plane<T, MODE, RPT, 5>(R, c, tile, z + 5, b ^ 1);
+    if (z + 6 < z1) st_plane<T, MODE, RPT, 6>(R, c, tile, z + 6, b);
+    if (z + 7 < z1) st_plane<T, MODE, RPT, 7>(R, c, tile, z + 7, b ^ 1);
+}
+
+// ---- vectorised stencil: each lane owns VX consecutive x points (16-byte loads and stores: float4 / double2) ----
+// Requires n % VX == 0 so every vector is aligned and lies wholly inside or outside the box. Workgroup 16 x 16 lanes:
+// tile TX = 16 VX columns x TY = 16 RPT rows (row ty + 16 rr). z window, halo/V/R look-ahead and LDS double buffering
+// as the scalar kernel; x taps are read from LDS as three 16-byte vectors, y taps as one vector per row.
+template <typename T> struct Vec16;
+template <> struct Vec16<float> {
+    typedef float4 type;
+    static constexpr int N = 4;
+};
+template <> struct Vec16<double> {
+    typedef double2 type;
+    static constexpr int N = 2;
+};
+
+template <typename T, int RPT> struct SvRegs {
+    typedef typename Vec16<T>::type V;
+    V q[9][RPT];
+    V hq[3];
+    V vq[3][RPT];
+    V rq[3][RPT];
+};
+
+template <typename T> __device__ __forceinline__ T vget(const typename Vec16<T>::type &v, int i);
+template <> __device__ __forceinline__ float vget<float>(const float4 &v, int i) {
+    return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
+}
+template <> __device__ __forceinline__ double vget<double>(const double2 &v, int i) { return i == 0 ? v.x : v.y; }
+template <typename T> __device__ __forceinline__ void vset(typename Vec16<T>::type &v, int i, T x);
+template <> __device__ __forceinline__ void vset<float>(float4 &v, int i, float x) {
+    if (i == 0) v.x = x; else if (i == 1) v.y = x; else if (i == 2) v.z = x; else v.w = x;
+}
+template <> __device__ __forceinline__ void vset<double>(double2 &v, int i, double x) {
+    if (i == 0) v.x = x; else v.y = x;
+}
+
+template <typename T, int RPT> struct SvCtx {
+    typedef typename Vec16<T>::type V;
+    static constexpr int VX = Vec16<T>::N;
+    const T *xb;
+    T *ob;
+    const T *v;
+    const T *rb;
+    int n;
+    long nn;
+    int x0, y0, tx, ty, gx;
+    bool xin;
+    int h_ly, h_lx, h_gx, h_gy;  // this lane's halo vector (LDS row/col, global x/y), if any
+    bool has_h;
+    T inv_h2, scale, omega, diag_t, lamb;
+    __device__ V ldv(const T *base, int xx, int yy, int zz) const {
+        if (xx >= 0 && xx < n && yy >= 0 && yy < n && zz >= 0 && zz < n)
+            return *reinterpret_cast<const V *>(base + xx + (long)n * yy + nn * zz);
+        V z0;
+        for (int i = 0; i < VX; i++) vset<T>(z0, i, T(0));
+        return z0;
+    }
+};
+
+template <typename T, int MODE, int RPT, int SLOT>
+__device__ __forceinline__ void sv_side(SvRegs<T, RPT> &R, const SvCtx<T, RPT> &c, int zz) {
+    constexpr int QS = SLOT % 3;
+    if (c.has_h) R.hq[QS] = c.ldv(c.xb, c.h_gx, c.h_gy, zz);
+#pragma unroll
+    for (int rr = 0; rr < RPT; rr++) {
+        if (MODE != ST_LAP) R.vq[QS][rr] = c.ldv(c.v, c.gx, c.y0 + c.ty + 16 * rr, zz);
+        if (MODE == ST_JAC) R.rq[QS][rr] = c.ldv(c.rb, c.gx, c.y0 + c.ty + 16 * rr, zz);
+    }
+}
+
+template <typename T, int MODE, int RPT, int S, int LXP>
+__device__ __forceinline__ void sv_plane(SvRegs<T, RPT> &R, const SvCtx<T, RPT> &c, T (*tile)[16 * RPT + 8][LXP],
+                                         int z, int buf) {
+    typedef typename Vec16<T>::type V;
+    constexpr int VX = Vec16<T>::N;
+    constexpr int C0 = (S + 4) % 9;
+    constexpr int NEW = (S + 8) % 9;
+    constexpr int HS = S % 3;
+#pragma unroll
+    for (int rr = 0; rr < RPT; rr++) R.q[NEW][rr] = c.ldv(c.xb, c.gx, c.y0 + c.ty + 16 * rr, z + 4);
+    sv_side<T, MODE, RPT, S + 2>(R, c, z + 2);
+    T(*tl)[LXP] = tile[buf];
+    const int lx = 4 + VX * c.tx;  // LDS column of this lane's first point
+#pragma unroll
+    for (int rr = 0; rr < RPT; rr++) *reinterpret_cast<V *>(&tl[c.ty + 16 * rr + 4][lx]) = R.q[C0][rr];
+    if (c.has_h) *reinterpret_cast<V *>(&tl[c.h_ly][c.h_lx]) = R.hq[HS];
+    __syncthreads();
+#pragma unroll
+    for (int rr = 0; rr < RPT; rr++) {
+        const int ly = c.ty + 16 * rr + 4;
+        // x window: 4 + VX + 4 values starting at lx - 4
+        T w[VX + 8];
+#pragma unroll
+        for (int b = 0; b < (VX + 8) / VX; b++) {
+            const V vv = *reinterpret_cast<const V *>(&tl[ly][lx - 4 + VX * b]);
+#pragma unroll
+            for (int i = 0; i < VX; i++) w[VX * b + i] = vget<T>(vv, i);
+        }
+        V ym[4], yp[4];
+#pragma unroll
+        for (int m = 1; m <= 4; m++) {
+            ym[m - 1] = *reinterpret_cast<const V *>(&tl[ly - m][lx]);
+            yp[m - 1] = *reinterpret_cast<const V *>(&tl[ly + m][lx]);
+        }
+        V outv;
+        const int gy = c.y0 + c.ty + 16 * rr;
+#pragma unroll
+        for (int i = 0; i < VX; i++) {
+            const T ce = w[4 + i];
+            const T ax = C1 * ((w[5 + i] - ce) + (w[3 + i] - ce)) + C2 * ((w[6 + i] - ce) + (w[2 + i] - ce)) +
+                         C3 * ((w[7 + i] - ce) + (w[1 + i] - ce)) + C4 * ((w[8 + i] - ce) + (w[0 + i] - ce));
+            const T ay = C1 * ((vget<T>(yp[0], i) - ce) + (vget<T>(ym[0], i) - ce)) +
+                         C2 * ((vget<T>(yp[1], i) - ce) + (vget<T>(ym[1], i) - ce)) +
+                         C3 * ((vget<T>(yp[2], i) - ce) + (vget<T>(ym[2], i) - ce)) +
+                         C4 * ((vget<T>(yp[3], i) - ce) + (vget<T>(ym[3], i) - ce));
+            const T az = C1 * ((vget<T>(R.q[(S + 5) % 9][rr], i) - ce) + (vget<T>(R.q[(S + 3) % 9][rr], i) - ce)) +
+                         C2 * ((vget<T>(R.q[(S + 6) % 9][rr], i) - ce) + (vget<T>(R.q[(S + 2) % 9][rr], i) - ce)) +
+                         C3 * ((vget<T>(R.q[(S + 7) % 9][rr], i) - ce) + (vget<T>(R.q[(S + 1) % 9][rr], i) - ce)) +
+                         C4 * ((vget<T>(R.q[(S + 8) % 9][rr], i) - ce) + (vget<T>(R.q[S % 9][rr], i) - ce));
+            const T lap = ((ax + ay) + az) * c.inv_h2;
+            T o;
+            if (MODE == ST_LAP) {
+                o = c.scale * lap;
+            } else if (MODE == ST_HAM) {
+                o = T(-0.5) * lap + vget<T>(R.vq[HS][rr], i) * ce;
+            } else {
+                const T vv = vget<T>(R.vq[HS][rr], i);
+                const T hw = T(-0.5) * lap + vv * ce - c.lamb * ce;
+                T d = c.diag_t + vv - c.lamb;
+                d = d > T(0.1) ? d : T(0.1);
+                o = ce + c.omega / d * (vget<T>(R.rq[HS][rr], i) - hw);
+            }
+            vset<T>(outv, i, o);
+        }
+        if (c.xin && gy < c.n) *reinterpret_cast<V *>(c.ob + c.gx + (long)c.n * gy + c.nn * z) = outv;
+    }
+}
+
+template <typename T, int MODE, int RPT>
+__global__ __launch_bounds__(256) void k_stencil_vec(const T *__restrict__ x, T *__restrict__ out,
+                                                     const T *__restrict__ v, const T *__restrict__ r,
+                                                     const double *__restrict__ lam, int n, long N, int zc, int nb,
+                                                     int order, T inv_h2, T scale, T omega, T diag_t) {
+    typedef typename Vec16<T>::type V;
+    constexpr int VX = Vec16<T>::N;
+    constexpr int TX = 16 * VX, TY = 16 * RPT;
+    constexpr int LXP = TX + 8;
+    __shared__ __align__(16) T tile[2][TY + 8][LXP];
+    SvCtx<T, RPT> c;
+    c.tx = threadIdx.x;
+    c.ty = threadIdx.y;
+    const int t = c.ty * 16 + c.tx;
+    const int nzc = (n + zc - 1) / zc;
+    const StWork wk = st_decode((n + TX - 1) / TX, (n + TY - 1) / TY, nzc, nb, order);
+    if (!wk.valid) return;
+    c.x0 = wk.tile_x * TX;
+    c.y0 = wk.tile_y * TY;
+    const int band = wk.band;
+    const int z0 = wk.zchunk * zc;
+    const int z1 = min(n, z0 + zc);
+    c.xb = x + (long)band * N;
+    c.ob = out + (long)band * N;
+    c.v = v;
+    c.rb = MODE == ST_JAC ? r + (long)band * N : nullptr;
+    c.lamb = MODE == ST_JAC ? (T)lam[band] : T(0);
+    c.n = n;
+    c.nn = (long)n * n;
+    c.gx = c.x0 + VX * c.tx;
+    c.xin = c.gx < n;
+    // halo vectors: 8 y-halo rows x 16 vectors (lanes 0..127), then (TY + 8) rows x 2 sides (lanes 128..)
+    if (t < 128) {
+        const int row = t >> 4, col = t & 15;
+        c.h_ly = row < 4 ? row : TY + row;
+        c.h_lx = 4 + VX * col;
+        c.h_gx = c.x0 + VX * col;
+        c.h_gy = c.y0 - 4 + c.h_ly;
+        c.has_h = true;
+    } else if (t < 128 + 2 * (TY + 8)) {
+        const int e = t - 128, row = e >> 1, side = e & 1;
+        c.h_ly = row;
+        c.h_lx = side ? 4 + TX : 0;
+        c.h_gx = side ? c.x0 + TX : c.x0 - 4;
+        c.h_gy = c.y0 - 4 + row;
+        c.has_h = true;
+    } else {
+        c.h_ly = c.h_lx = c.h_gx = c.h_gy = 0;
+        c.has_h = false;
+    }
+    c.inv_h2 = inv_h2;
+    c.scale = scale;
+    c.omega = omega;
+    c.diag_t = diag_t;
+    SvRegs<T, RPT> R;
+#pragma unroll
+    for (int d = 0; d < 8; d++)
+#pragma unroll
+        for (int rr = 0; rr < RPT; rr++) R.q[d][rr] = c.ldv(c.xb, c.gx, c.y0 + c.ty + 16 * rr, z0 - 4 + d);
+    sv_side<T, MODE, RPT, 0>(R, c, z0);
+    sv_side<T, MODE, RPT, 1>(R, c, z0 + 1);
+    int z = z0;
+    for (; z + 9 <= z1; z += 9) {
+        const int b = (z - z0) & 1;
+        sv_plane<T, MODE, RPT, 0, LXP>(R, c, tile, z + 0, b);
+        sv_plane<T, MODE, RPT, 1, LXP>(R, c, tile, z + 1, b ^ 1);
+        sv_plane<T, MODE, RPT, 2, LXP>(R, c, tile, z + 2, b);
+        sv_plane<T, MODE, RPT, 3, LXP>(R, c, tile, z + 3, b ^ 1);
+        sv_plane<T, MODE, RPT, 4, LXP>(R, c, tile, z + 4, b);
+        sv_plane<T, MODE, RPT, 5, LXP>(R, c, tile, z + 5, b ^ 1);
+        sv_plane<T, MODE, RPT, 6, LXP>(R, c, tile, z + 6, b);
+        sv_plane<T, MODE, RPT, 7, LXP>(R, c, tile, z + 7, b ^ 1);
+        sv_plane<T, MODE, RPT, 8, LXP>(R, c, tile, z + 8, b);
+    }
+    const int b = (z - z0) & 1;
+    if (z < z1) sv_plane<T, MODE, RPT, 0, LXP>(R, c, tile, z + 0, b);
+    if (z + 1 < z1) sv_plane<T, MODE, RPT, 1, LXP>(R, c, tile, z + 1, b ^ 1);
+    if (z + 2 < z1) sv_plane<T, MODE, RPT, 2, LXP>(R, c, tile, z + 2, b);
+    if (z + 3 < z1) sv_plane<T, MODE, RPT, 3, LXP>(R, c, tile, z + 3, b ^ 1);
+    if (z + 4 < z1) sv_plane<T, MODE, RPT, 4, LXP>(R, c, tile, z + 4, b);
+    if (z + 5 < z1) sv_plane<T, MODE, RPT, 5, LXP>(R, c, tile, z + 5, b ^ 1);
+    if (z + 6 < z1) sv_plane<T, MODE, RPT, 6, LXP>(R, c, tile, z + 6, b);
+    if (z + 7 < z1) sv_plane<T, MODE, RPT, 7, LXP>(R, c, tile, z + 7, b ^ 1);
+}
+
 struct StVariant {
     int rpt;
-    int zc;  // 0 = whole column
+    int zc;     // 0 = whole column
+    int vec;    // 1: 16-byte vector lanes (needs n divisible by the vector width)
+    int order;  // XCD work order: 0 tiles fastest, 1 bands fastest
     const char *name;
 };
 static const StVariant ST_VARIANTS[] = {
-    {4, 0, "64x16 tile, full z column"}, {4, 32, "64x16 tile, z chunks of 32"}, {4, 16, "64x16 tile, z chunks of 16"},
-    {2, 0, "64x8 tile, full z column"},  {2, 16, "64x8 tile, z chunks of 16"},  {2, 8, "64x8 tile, z chunks of 8"},
+    {4, 0, 0, 0, "64x16 tile, full z column"},
+    {4, 32, 0, 0, "64x16 tile, z chunks of 32"},
+    {4, 16, 0, 0, "64x16 tile, z chunks of 16"},
+    {2, 0, 0, 0, "64x8 tile, full z column"},
+    {2, 16, 0, 0, "64x8 tile, z chunks of 16"},
+    {2, 8, 0, 0, "64x8 tile, z chunks of 8"},
+    {1, 0, 1, 0, "vector lanes, 16 rows, full z column"},
+    {1, 32, 1, 0, "vector lanes, 16 rows, z chunks of 32"},
+    {1, 16, 1, 0, "vector lanes, 16 rows, z chunks of 16"},
+    {2, 0, 1, 0, "vector lanes, 32 rows, full z column"},
+    {2, 32, 1, 0, "vector lanes, 32 rows, z chunks of 32"},
+    {1, 0, 1, 1, "vector lanes, 16 rows, full z column, bands fastest"},
+    {1, 32, 1, 1, "vector lanes, 16 rows, z chunks of 32, bands fastest"},
+    {2, 0, 1, 1, "vector lanes, 32 rows, full z column, bands fastest"},
+    {2, 0, 0, 1, "64x8 tile, full z column, bands fastest"},
+    {2, 16, 0, 1, "64x8 tile, z chunks of 16, bands fastest"},
 };
 static int g_default_variant[2] = {-1, -1};
 
@@ -179,17 +507,37 @@ static int pick_variant(int n, int nb) {
 }
 
 template <typename T, int MODE, int RPT>
-static void launch_stencil(const StencilParams &p, int zc, const T *x, T *out, const T *v, const T *r, const double *lam,
-                           hipStream_t s) {
+static void launch_stencil(const StencilParams &p, int zc, int order, const T *x, T *out, const T *v, const T *r,
+                           const double *lam, hipStream_t s) {
     const int n = p.n;
     const long N = (long)n * n * n;
     if (zc <= 0 || zc > n) zc = n;
     const int nzc = (n + zc - 1) / zc;
-    dim3 grid((n + ST_TX - 1) / ST_TX, (n + 4 * RPT - 1) / (4 * RPT), nzc * p.nb);
+    const unsigned grid = st_grid((n + ST_TX - 1) / ST_TX, (n + 4 * RPT - 1) / (4 * RPT), nzc, p.nb);
     dim3 block(64, 4, 1);
-    hipLaunchKernelGGL((k_stencil<T, MODE, RPT>), grid, block, 0, s, x, out, v, r, lam, n, N, zc,
+    hipLaunchKernelGGL((k_stencil<T, MODE, RPT>), dim3(grid), block, 0, s, x, out, v, r, lam, n, N, zc, p.nb, order,
                        (T)(1.0 / (p.h * p.h)), (T)p.scale, (T)p.omega, (T)p.diag_t);
     LIRA_HIP(hipGetLastError());
+}
+
+template <typename T, int MODE, int RPT>
+static void launch_stencil_vec(const StencilParams &p, int zc, int order, const T *x, T *out, const T *v, const T *r,
+                               const double *lam, hipStream_t s) {
+    constexpr int VX = Vec16<T>::N;
+    const int n = p.n;
+    const long N = (long)n * n * n;
+    if (zc <= 0 || zc > n) zc = n;
+    const int nzc = (n + zc - 1) / zc;
+    const unsigned grid = st_grid((n + 16 * VX - 1) / (16 * VX), (n + 16 * RPT - 1) / (16 * RPT), nzc, p.nb);
+    hipLaunchKernelGGL((k_stencil_vec<T, MODE, RPT>), dim3(grid), dim3(16, 16, 1), 0, s, x, out, v, r, lam, n, N, zc,
+                       p.nb, order, (T)(1.0 / (p.h * p.h)), (T)p.scale, (T)p.omega, (T)p.diag_t);
+    LIRA_HIP(hipGetLastError());
+}
+
+bool stencil_variant_usable(int variant, int n, int dtype_size) {
+    if (variant < 0 || variant >= stencil_variants()) return false;
+    if (!ST_VARIANTS[variant].vec) return true;
+    return n % (16 / dtype_size) == 0;
 }
 
 template <typename T>
@@ -198,9 +546,28 @@ void stencil(int mode, const StencilParams &p, const T *x, T *out, const T *v, c
     if (p.nb <= 0) return;
     if (x == out) throw InvalidError("stencil: output aliases the input");
     if (variant < 0) variant = g_default_variant[sizeof(T) == 8 ? 1 : 0];
-    if (variant < 0 || variant >= stencil_variants()) variant = pick_variant(p.n, p.nb);
+    if (!stencil_variant_usable(variant, p.n, (int)sizeof(T))) variant = pick_variant(p.n, p.nb);
+    if (ST_VARIANTS[variant].vec) {
+        // vector lanes need 16-byte aligned bases (every band then stays aligned since n is a multiple of the width)
+        auto misaligned = [](const void *q) { return q && (reinterpret_cast<uintptr_t>(q) & 15) != 0; };
+        if (misaligned(x) || misaligned(out) || misaligned(v) || misaligned(r)) variant = pick_variant(p.n, p.nb);
+    }
     const StVariant &sv = ST_VARIANTS[variant];
-#define LIRA_ST(MODE_, RPT_) launch_stencil<T, MODE_, RPT_>(p, sv.zc, x, out, v, r, lam, s)
+    if (sv.vec) {
+#define LIRA_SV(MODE_, RPT_) launch_stencil_vec<T, MODE_, RPT_>(p, sv.zc, sv.order, x, out, v, r, lam, s)
+        if (sv.rpt == 1) {
+            if (mode == ST_LAP) LIRA_SV(ST_LAP, 1);
+            else if (mode == ST_HAM) LIRA_SV(ST_HAM, 1);
+            else LIRA_SV(ST_JAC, 1);
+        } else {
+            if (mode == ST_LAP) LIRA_SV(ST_LAP, 2);
+            else if (mode == ST_HAM) LIRA_SV(ST_HAM, 2);
+            else LIRA_SV(ST_JAC, 2);
+        }
+#undef LIRA_SV
+        return;
+    }
+#define LIRA_ST(MODE_, RPT_) launch_stencil<T, MODE_, RPT_>(p, sv.zc, sv.order, x, out, v, r, lam, s)
     if (sv.rpt == 4) {
         if (mode == ST_LAP) LIRA_ST(ST_LAP, 4);
         else if (mode == ST_HAM) LIRA_ST(ST_HAM, 4);

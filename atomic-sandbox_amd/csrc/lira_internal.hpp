@@ -102,6 +102,7 @@ template <typename T>
 void stencil(int mode, const StencilParams &p, const T *x, T *out, const T *v, const T *r, const double *lam,
              hipStream_t s, int variant = -1);
 int stencil_variants();
+bool stencil_variant_usable(int variant, int n, int dtype_size);
 const char *stencil_variant_name(int v);
 void stencil_set_default_variant(int dtype_size, int variant);
 

@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -39,6 +40,14 @@ static double residual_floor_f32(double h) {
 }
 
 using Clock = std::chrono::steady_clock;
+// LIRA_TRACE=1 prints one line per SCF step and per eigen solve to stderr (the reference's SCF_TRACE / LOBPCG_TRACE)
+static bool trace_on() {
+    static const bool on = [] {
+        const char *e = std::getenv("LIRA_TRACE");
+        return e && *e && *e != '0';
+    }();
+    return on;
+}
 static double secs(Clock::time_point a) { return std::chrono::duration<double>(Clock::now() - a).count(); }
 
 // Multigrid constants (crates/qm-gpu/src/multigrid.rs:70-78): V(2,2), 12 sweeps on the coarsest level (n <= 2),
@@ -155,6 +164,24 @@ static int witness_keep(const std::vector<double> &ev, const std::vector<double>
     for (int t = n; t < m; t++)
         if (passes(t)) return (t + 1 < m && passes(t + 1)) ? t + 2 : t + 1;
     return m;
+}
+
+// qm/src/occupations.rs:352-375: the lowest rank t >= n whose lower bound clears both witness arms with r_t <= r_max;
+// returns t + 1 (t + 2 with a spare above), or -1 when none qualifies
+static int witness_keep_within(const std::vector<double> &ev, const std::vector<double> &res, int n, double mu,
+                               double kt, double r_max) {
+    const int m = (int)ev.size();
+    if (n == 0) return m;
+    if (m <= n) return -1;
+    auto o = ascending(ev);
+    const double eps_n = ev[o[n - 1]];
+    auto passes = [&](int rank) {
+        const double lower = ev[o[rank]] - res[o[rank]];
+        return lower >= eps_n + WITNESS_GAP && lower - mu >= WITNESS_TAIL_KT * kt;
+    };
+    for (int t = n; t < m; t++)
+        if (passes(t) && res[o[t]] <= r_max) return (t + 1 < m && passes(t + 1)) ? t + 2 : t + 1;
+    return -1;
 }
 
 // qm/src/occupations.rs:271-291: the columns whose upper-bound occupation f(theta - r; mu) the tail sum needs to stay
@@ -372,7 +399,8 @@ template <typename T> class Solver final : public SolverIface {
     PoissonOut poisson(const T *rho, double *v, double tol, int max_iter);
     int pcg(const T *r0, T *e, double rel_tol, int max_iter, double *rel_out);
     // LOBPCG on a channel's X (k columns) for H = -1/2 lap + v
-    EigenOut lobpcg(Channel<T> &c, const T *v, int k, double lam, double tol, int max_iter, double tail_tol, int head);
+    EigenOut lobpcg(Channel<T> &c, const T *v, int k, double lam, double tol, int max_iter, double tail_tol, int head,
+                    int checked);
     void solve_channel(int spin, const T *v, int nel, bool fd, double kt, double tol, double scf_tol, EigenOut &eo,
                        Fd &occ);
     void upload_nuclei(const lira_nucleus *nuc, int nat);
@@ -601,7 +629,7 @@ template <typename T> PoissonOut Solver<T>::poisson(const T *rho, double *v, dou
 // conditioning guard; X, P updated by one block lincomb on the device; HX recomputed by the stencil.
 template <typename T>
 EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double tol, int max_iter, double tail_tol,
-                           int head) {
+                           int head, int checked) {
     EigenOut out;
     if (k <= 0) return out;
     const double diag_t = -1.5 * COEFF0 / (h_ * h_);
@@ -693,6 +721,7 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
             lambda[i] = Am[(size_t)i * m + i] / xx;
             res[i] = std::sqrt(std::max(g[(size_t)m * q + i], 0.0) / xx);
             if (!std::isfinite(lambda[i]) || !std::isfinite(res[i])) finite = false;
+            if (i >= checked) continue;  // an untested buffer column (it only speeds up the columns below it)
             max_res = std::max(max_res, res[i]);
             if (i < head) head_res = std::max(head_res, res[i]);
             else tail_res = std::max(tail_res, res[i]);
@@ -764,8 +793,11 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
     return out;
 }
 
-// one spin channel under Aufbau or Fermi–Dirac (qm/src/scf.rs:774-877): under Fermi–Dirac the witness columns whose
-// occupation is negligible against the SCF tolerance are held to the guard tolerance (the reference's Tiers::Loose)
+// one spin channel under Aufbau or Fermi–Dirac (qm/src/scf.rs:774-877; crates/qm-gpu/src/scf.rs:874-1033). Under
+// Fermi–Dirac the block carries, above the tested levels, one untested buffer column (it speeds up the column below
+// it, f = 0); the witness columns whose occupation is negligible against the SCF tolerance are held to the guard
+// tolerance; a top tested witness with r above 1.5 guard yields to the lowest tested level that is a witness at
+// r <= 1.5 guard, else the block grows by one.
 template <typename T>
 void Solver<T>::solve_channel(int spin, const T *v, int nel, bool fd, double kt, double tol, double scf_tol,
                               EigenOut &eo, Fd &occ) {
@@ -775,14 +807,28 @@ void Solver<T>::solve_channel(int spin, const T *v, int nel, bool fd, double kt,
         return;
     }
     const double guard = std::max(WITNESS_GUARD_TOL, tol);
+    const int cap = MAX_WITNESS_GROWTH;
+    auto buffered = [&](int len) { return fd && len >= nel + 2 && len - nel < cap; };
+    auto add_column = [&]() {
+        ensure_channel(c, c.k + 1);
+        initial_orbitals<T>(c.X.template as<T>() + (long)c.k * N_, n_, h_, pos_.as<double>(), nat_, 1,
+                            1000ULL + 2ULL * (uint64_t)c.k + (uint64_t)spin, stream_);
+        c.evals.push_back(c.evals.back());
+        c.res.push_back(0.0);
+        c.k++;
+    };
+    if (fd && c.k == nel + 1 && cap >= 2) add_column();
     for (;;) {
         const double lam = c.evals.empty() ? -1.0 : c.evals[0];
-        const int head = fd ? std::min(std::max(c.tight, 0), c.k) : c.k;
-        EigenOut e = lobpcg(c, v, c.k, lam, tol, 200, fd ? guard : tol, head);
+        const int checked = buffered(c.k) ? c.k - 1 : c.k;
+        const int head = fd ? std::min(std::max(c.tight, 0), checked) : c.k;
+        EigenOut e = lobpcg(c, v, c.k, lam, tol, 200, fd ? guard : tol, head, checked);
+        if (trace_on())
+            std::fprintf(stderr, "lira:   spin %d k %d head %d tested %d tol %.1e: %d iterations, max residual %.2e, fallbacks %d\n",
+                         spin, c.k, head, checked, tol, e.iterations, e.max_residual, e.fallbacks);
         eo.iterations += e.iterations;
         eo.fallbacks += e.fallbacks;
-        for (int i = 0; i < c.k; i++)
-            if (i < head) eo.max_residual = std::max(eo.max_residual, c.res[i]);
+        for (int i = 0; i < std::min(head, c.k); i++) eo.max_residual = std::max(eo.max_residual, c.res[i]);
         if (!fd) {
             occ.f.assign(c.k, 1.0);
             occ.mu = NAN;
@@ -790,53 +836,44 @@ void Solver<T>::solve_channel(int spin, const T *v, int nel, bool fd, double kt,
             c.occ = occ.f;
             return;
         }
-        occ = fermi_dirac(c.evals, nel, kt);
-        const int need = tight_columns(c.evals, c.res, occ.mu, kt, scf_tol);
+        // LOBPCG returns its columns in ascending Ritz order: the tested levels are the first `checked`
+        std::vector<double> ev(c.evals.begin(), c.evals.begin() + checked), rs(c.res.begin(), c.res.begin() + checked);
+        occ = fermi_dirac(ev, nel, kt);
+        const int need = tight_columns(ev, rs, occ.mu, kt, scf_tol);
         c.tight = need;
         if (need > head) continue;  // a loose column needs the eigen tolerance: re-solve with it tight
-        const int extras = c.k - nel;
-        auto o = ascending(c.evals);
-        const double top_r = c.res[o[c.k - 1]];
-        const bool witness = witness_bound_reached(c.evals, c.res, nel, occ.mu, kt);
-        auto add_column = [&]() {
-            ensure_channel(c, c.k + 1);
-            initial_orbitals<T>(c.X.template as<T>() + (long)c.k * N_, n_, h_, pos_.as<double>(), nat_, 1,
-                                1000ULL + 2ULL * (uint64_t)c.k + (uint64_t)spin, stream_);
-            c.evals.push_back(c.evals.back());
-            c.res.push_back(0.0);
-            c.k++;
-        };
-        if (witness && top_r > WITNESS_RESIDUAL_FACTOR * guard && extras < MAX_WITNESS_GROWTH) {
-            add_column();
-            continue;
-        }
-        if (witness) {
-            const int keep = witness_keep(c.evals, c.res, nel, occ.mu, kt);
-            if (keep < c.k) {
-                // keep the lowest `keep` levels, in their column order
-                std::vector<int> kept(o.begin(), o.begin() + keep);
-                std::sort(kept.begin(), kept.end());
-                for (int j = 0; j < keep; j++)
-                    LIRA_HIP(hipMemcpyAsync(c.XN.template as<T>() + (long)j * N_, c.X.template as<T>() + (long)kept[j] * N_,
-                                            (size_t)N_ * sizeof(T), hipMemcpyDeviceToDevice, stream_));
-                std::swap(c.X, c.XN);
-                std::vector<double> ev2, r2;
-                for (int j : kept) {
-                    ev2.push_back(c.evals[j]);
-                    r2.push_back(c.res[j]);
-                }
-                c.evals = ev2;
-                c.res = r2;
-                c.k = keep;
-                occ = fermi_dirac(c.evals, nel, kt);
+        const bool witness = witness_bound_reached(ev, rs, nel, occ.mu, kt);
+        auto o = ascending(ev);
+        const double top_r = rs[o[checked - 1]];
+        const double r_w = WITNESS_RESIDUAL_FACTOR * WITNESS_GUARD_TOL;
+        int keep = -1;
+        if (witness && top_r > r_w && c.k - nel < cap) {
+            keep = witness_keep_within(ev, rs, nel, occ.mu, kt, r_w);
+            if (keep < 0) {
+                add_column();
+                continue;
             }
+        } else if (witness) {
+            keep = witness_keep(ev, rs, nel, occ.mu, kt);
+        }
+        if (keep >= 0) {
+            // keep the lowest `keep` tested levels (the first columns) plus a buffer above them when one fits
+            const int len = std::min(c.k, buffered(keep + 1) ? keep + 1 : keep);
+            c.k = len;
+            c.evals.resize(len);
+            c.res.resize(len);
+            const int chk = buffered(len) ? len - 1 : len;
+            std::vector<double> ev2(c.evals.begin(), c.evals.begin() + chk);
+            occ = fermi_dirac(ev2, nel, kt);
+            occ.f.resize(len, 0.0);  // the buffer holds no charge
             c.occ = occ.f;
             return;
         }
-        if (extras >= MAX_WITNESS_GROWTH)
-            throw NumericalError("no gap witness within " + std::to_string(MAX_WITNESS_GROWTH) +
-                                 " extra levels of spin " + std::to_string(spin));
-        const int target = std::min(std::max(2 * extras, extras + 1), MAX_WITNESS_GROWTH);
+        if (c.k - nel >= cap)
+            throw NumericalError("no gap witness within " + std::to_string(cap) + " extra levels of spin " +
+                                 std::to_string(spin));
+        const int extras = c.k - nel;
+        const int target = std::min(std::max(2 * extras, extras + 1), cap);
         while (c.k - nel < target) add_column();
     }
 }
@@ -1003,6 +1040,9 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
         jd.count = 2;
         auto dr = reduce(jd, dv_);
         delta_rho = dr[0] + dr[1];
+        if (trace_on())
+            std::fprintf(stderr, "lira: SCF step %d delta_rho %.3e eig tol %.1e%s, stored %d/%d, %.1f ms\n", it, delta_rho,
+                         tol_step, tight ? " (tight)" : "", ch_[0].k, ch_[1].k, 1e3 * secs(t0));
         // stall watch (qm/src/scf.rs:433-469)
         if (delta_rho < best) best = delta_rho;
         if (std::isfinite(delta_rho) && (!std::isfinite(anchor) || delta_rho <= 0.5 * anchor)) {
@@ -1301,7 +1341,7 @@ EigenOut Solver<T>::lobpcg_host(const double *v, int k, double *x, double *evals
     }
     c.k = k;
     valid_ = false;
-    EigenOut eo = lobpcg(c, veff_[0].template as<T>(), k, lam, tol, max_iter, tol, k);
+    EigenOut eo = lobpcg(c, veff_[0].template as<T>(), k, lam, tol, max_iter, tol, k, k);
     for (int b = 0; b < k; b++) {
         convert<double, T>(rho64_.as<double>(), c.X.template as<T>() + (long)b * N_, N_, stream_);
         LIRA_HIP(hipMemcpyAsync(x + (long)b * N_, rho64_.ptr, N_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
