@@ -38,6 +38,51 @@ class ScfStats(C.Structure):
                 ("seconds_poisson", C.c_double), ("seconds_eigen", C.c_double), ("seconds_total", C.c_double)]
 
 
+class _Projector(C.Structure):
+    _fields_ = [("l", C.c_int), ("e_kb", C.c_double), ("radius", C.c_double), ("t_u", C.c_void_p), ("n_t", C.c_int)]
+
+
+class _Species(C.Structure):
+    _fields_ = [("z", C.c_int), ("z_v", C.c_double), ("sigma_lr", C.c_double), ("du", C.c_double),
+                ("v_sr_u", C.c_void_p), ("n_v_sr", C.c_int), ("r_sr", C.c_double), ("rho_pc_u", C.c_void_p),
+                ("n_pc", C.c_int), ("pc_radius", C.c_double), ("proj", C.c_void_p), ("n_proj", C.c_int)]
+
+
+class _Ionic(C.Structure):
+    _fields_ = [("species", C.c_void_p), ("n_species", C.c_int), ("sigma_ae", C.c_double)]
+
+
+class Ionic:
+    """An or_ionic built from objects with the Species / Projector attributes of synthetic_tables (keeps the
+    arrays alive while the oracle reads them)."""
+
+    def __init__(self, species, sigma_ae):
+        self._keep = []
+        arr = (_Species * max(1, len(species)))()
+        for i, sp in enumerate(species):
+            projs = (_Projector * max(1, len(sp.projectors)))()
+            for j, pr in enumerate(sp.projectors):
+                t = _f64(pr.t_u)
+                self._keep.append(t)
+                projs[j] = _Projector(pr.l, pr.e_kb, pr.radius, t.ctypes.data, t.size)
+            self._keep.append(projs)
+            v = _f64(sp.v_sr_u)
+            self._keep.append(v)
+            pc = None if sp.rho_pc_u is None else _f64(sp.rho_pc_u)
+            if pc is not None:
+                self._keep.append(pc)
+            arr[i] = _Species(sp.z, sp.z_v, sp.sigma_lr, sp.du, v.ctypes.data, v.size, sp.r_sr,
+                              None if pc is None else pc.ctypes.data, 0 if pc is None else pc.size,
+                              sp.pc_radius if pc is not None else 0.0, C.cast(projs, C.c_void_p).value,
+                              len(sp.projectors))
+        self._keep.append(arr)
+        self.c = _Ionic(C.cast(arr, C.c_void_p).value, len(species), sigma_ae)
+
+    @property
+    def ref(self):
+        return C.byref(self.c)
+
+
 def build() -> str:
     """Compiles the oracle with its Makefile (gcc, no fast-math)."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
@@ -105,6 +150,46 @@ def lib():
                              C.c_void_p, C.c_void_p]
         L.or_forces.restype = C.c_int
         L.or_forces.argtypes = [C.c_int, C.c_double, _dp, C.c_int, _dp, _ip, C.c_double, _dp]
+        L.or_table_read.restype = None
+        L.or_table_read.argtypes = [_dp, C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]
+        L.or_points_within.restype = C.c_long
+        L.or_points_within.argtypes = [C.c_int, C.c_double, _dp, C.c_double, C.c_void_p]
+        L.or_ionic_local_potential.restype = None
+        L.or_ionic_local_potential.argtypes = [C.c_int, C.c_double, C.c_int, _dp, _ip, C.c_void_p, _dp]
+        L.or_partial_core_field.restype = C.c_int
+        L.or_partial_core_field.argtypes = [C.c_int, C.c_double, C.c_int, _dp, _ip, C.c_void_p, _dp]
+        L.or_ionic_repulsion.restype = C.c_int
+        L.or_ionic_repulsion.argtypes = [C.c_int, _dp, _ip, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int)]
+        L.or_solid_harmonic_gradients.restype = None
+        L.or_solid_harmonic_gradients.argtypes = [_dp, _dp]
+        L.or_nl_build.restype = C.c_void_p
+        L.or_nl_build.argtypes = [C.c_int, C.c_double, C.c_int, _dp, _ip, C.c_void_p, C.c_int]
+        L.or_nl_free.restype = None
+        L.or_nl_free.argtypes = [C.c_void_p]
+        for f in (L.or_nl_rows, L.or_nl_spheres):
+            f.restype = C.c_int
+            f.argtypes = [C.c_void_p]
+        L.or_nl_sphere_points.restype = C.c_long
+        L.or_nl_sphere_points.argtypes = [C.c_void_p, C.c_int]
+        for f in (L.or_nl_project, L.or_nl_apply_add):
+            f.restype = None
+            f.argtypes = [C.c_void_p, _dp, _dp]
+        L.or_nl_energy.restype = C.c_double
+        L.or_nl_energy.argtypes = [C.c_void_p, C.c_int, _dp, C.c_void_p]
+        L.or_pseudo_hamiltonian.restype = None
+        L.or_pseudo_hamiltonian.argtypes = [C.c_int, C.c_double, _dp, C.c_void_p, _dp, _dp]
+        L.or_lobpcg_nl.restype = C.c_int
+        L.or_lobpcg_nl.argtypes = [C.c_int, C.c_double, _dp, C.c_void_p, C.c_int, _dp, _dp, C.c_double, C.c_double,
+                                   C.c_int, C.POINTER(C.c_double), _dp]
+        L.or_pseudo_force_terms.restype = C.c_int
+        L.or_pseudo_force_terms.argtypes = [C.c_int, C.c_double, _dp, _dp, C.c_int, _dp, _dp, C.c_int, _dp, _ip,
+                                            C.c_void_p, _dp]
+        L.or_scf_ionic.restype = C.c_int
+        L.or_scf_ionic.argtypes = [C.c_int, C.c_double, C.c_int, _dp, _ip, C.c_int, C.c_int, C.POINTER(Settings),
+                                   C.c_void_p, C.POINTER(ScfStats), C.c_int, _dp, _dp, _dp, _dp, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_mg_apply_a2.restype = None
         L.or_mg_apply_a2.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp]
         for f in (L.or_mg_restrict, L.or_mg_prolong):
@@ -270,8 +355,8 @@ def default_settings(h, occupations="aufbau", tol=1e-6, eig_tol=1e-6, max_iter=2
                     occupations=0 if occupations == "aufbau" else 1, stall_window=50)
 
 
-def scf(n, h, pos, z, n_up, n_dn, settings=None, cap=40, want_rho=True, want_orbitals=False):
-    """Runs the oracle SCF; returns (stats, dict of arrays)."""
+def scf(n, h, pos, z, n_up, n_dn, settings=None, cap=40, want_rho=True, want_orbitals=False, species=None):
+    """Runs the oracle SCF; returns (stats, dict of arrays). species: pseudised ions (the rest all-electron)."""
     s = settings if settings is not None else default_settings(h)
     p, zz = _nuc(pos, z)
     st = ScfStats()
@@ -286,8 +371,9 @@ def scf(n, h, pos, z, n_up, n_dn, settings=None, cap=40, want_rho=True, want_orb
     def ptr(a):
         return None if a is None else a.ctypes.data_as(C.c_void_p)
 
-    lib().or_scf(n, h, len(zz), p, zz, n_up, n_dn, C.byref(s), C.byref(st), cap, ev_u, ev_d, oc_u, oc_d,
-                 ptr(ru), ptr(rd), ptr(forces), ptr(ou), ptr(od))
+    io = None if species is None else Ionic(species, s.sigma)
+    lib().or_scf_ionic(n, h, len(zz), p, zz, n_up, n_dn, C.byref(s), None if io is None else io.ref, C.byref(st),
+                       cap, ev_u, ev_d, oc_u, oc_d, ptr(ru), ptr(rd), ptr(forces), ptr(ou), ptr(od))
     out = {"evals_up": ev_u[:st.k_up].copy(), "evals_dn": ev_d[:st.k_dn].copy(),
            "occ_up": oc_u[:st.k_up].copy(), "occ_dn": oc_d[:st.k_dn].copy(),
            "forces": forces.reshape(-1, 3), "rho_up": ru, "rho_dn": rd}
@@ -320,3 +406,115 @@ def mg_prolong(n, c):
     out = np.empty(n ** 3)
     lib().or_mg_prolong(n, _f64(c), out)
     return out
+
+
+# ------------------------------------------------------------------------------------------ pseudised ions ----
+
+def table_read(f, du, radius, d):
+    f = _f64(f)
+    v, dv = C.c_double(), C.c_double()
+    lib().or_table_read(f, f.size, du, radius, d, C.byref(v), C.byref(dv))
+    return v.value, dv.value
+
+
+def points_within(n, h, c, radius):
+    cnt = lib().or_points_within(n, h, _f64(c), radius, None)
+    out = np.zeros(max(cnt, 1), dtype=np.int64)
+    lib().or_points_within(n, h, _f64(c), radius, out.ctypes.data_as(C.c_void_p))
+    return out[:cnt]
+
+
+def ionic_local_potential(n, h, pos, z, species, sigma_ae):
+    p, zz = _nuc(pos, z)
+    io = Ionic(species, sigma_ae)
+    out = np.empty(n ** 3)
+    lib().or_ionic_local_potential(n, h, len(zz), p, zz, io.ref, out)
+    return out
+
+
+def partial_core_field(n, h, pos, z, species):
+    p, zz = _nuc(pos, z)
+    io = Ionic(species, 1.0)
+    out = np.empty(n ** 3)
+    any_core = lib().or_partial_core_field(n, h, len(zz), p, zz, io.ref, out)
+    return out, bool(any_core)
+
+
+def ionic_repulsion(pos, z, species):
+    p, zz = _nuc(pos, z)
+    io = Ionic(species, 1.0)
+    e, i, j = C.c_double(), C.c_int(), C.c_int()
+    st = lib().or_ionic_repulsion(len(zz), p, zz, io.ref, C.byref(e), C.byref(i), C.byref(j))
+    return (e.value if st == OK else None), st, (i.value, j.value)
+
+
+def solid_harmonic_gradients(r):
+    out = np.empty(27)
+    lib().or_solid_harmonic_gradients(_f64(r), out)
+    return out.reshape(9, 3)
+
+
+class Nonlocal:
+    """The oracle's NonlocalOp (qm/src/nonlocal.rs:81-174) at one geometry."""
+
+    def __init__(self, n, h, pos, z, species, with_grad=False):
+        p, zz = _nuc(pos, z)
+        self.n, self.h = n, h
+        self._io = Ionic(species, 1.0)
+        self._p = C.c_void_p(lib().or_nl_build(n, h, len(zz), p, zz, self._io.ref, int(with_grad)))
+
+    def __del__(self):
+        if getattr(self, "_p", None):
+            lib().or_nl_free(self._p)
+            self._p = None
+
+    @property
+    def rows(self):
+        return lib().or_nl_rows(self._p)
+
+    @property
+    def spheres(self):
+        return lib().or_nl_spheres(self._p)
+
+    def sphere_points(self, s):
+        return lib().or_nl_sphere_points(self._p, s)
+
+    def project(self, psi):
+        c = np.empty(max(self.rows, 1))
+        lib().or_nl_project(self._p, _f64(psi), c)
+        return c[:self.rows]
+
+    def apply_add(self, psi, out):
+        lib().or_nl_apply_add(self._p, _f64(psi), out)
+        return out
+
+    def energy(self, orbitals, f=None):
+        orbs = _f64(orbitals).reshape(-1)
+        k = orbs.size // self.n ** 3
+        fa = None if f is None else _f64(f)
+        return lib().or_nl_energy(self._p, k, orbs, None if fa is None else fa.ctypes.data_as(C.c_void_p))
+
+    def hamiltonian(self, v, x):
+        out = np.empty(self.n ** 3)
+        lib().or_pseudo_hamiltonian(self.n, self.h, _f64(v), self._p, _f64(x), out)
+        return out
+
+    def lobpcg(self, v, x0, tol, max_iter=500, precond_lambda=None):
+        x = _f64(x0).copy()
+        k = x.shape[0]
+        ev, res, mr = np.empty(k), np.empty(k), C.c_double()
+        lam = float("nan") if precond_lambda is None else float(precond_lambda)
+        it = lib().or_lobpcg_nl(self.n, self.h, _f64(v), self._p, k, x, ev, lam, tol, max_iter, C.byref(mr), res)
+        return x, ev, it, mr.value, res
+
+
+def pseudo_force_terms(n, h, rho_up, rho_dn, orbitals, occ, pos, z, species, sigma_ae):
+    """[ion-ion, local LR, local SR, nonlocal, partial core] x nucleus x 3 (qm/src/forces.rs:259-350)."""
+    p, zz = _nuc(pos, z)
+    io = Ionic(species, sigma_ae)
+    orbs = _f64(orbitals).reshape(-1)
+    k = orbs.size // n ** 3
+    terms = np.zeros(15 * len(zz))
+    st = lib().or_pseudo_force_terms(n, h, _f64(rho_up), _f64(rho_dn), k, orbs, _f64(occ), len(zz), p, zz, io.ref,
+                                     terms)
+    return st, terms.reshape(5, len(zz), 3)

@@ -549,9 +549,15 @@ static int orthonormalise(int n, double h, double *s, int cnt) {
     return kept;
 }
 
-/* qm/src/eigen.rs:290-369 (lobpcg_impl with head = k) */
 int or_lobpcg(int n, double h, const double *v, int k, double *x, double *evals, double precond_lambda, double tol,
               int max_iter, double *max_residual, double *residuals) {
+    return or_lobpcg_nl(n, h, v, NULL, k, x, evals, precond_lambda, tol, max_iter, max_residual, residuals);
+}
+
+/* qm/src/eigen.rs:290-369 (lobpcg_impl with head = k); H = PseudoHamiltonian when nl is given (nonlocal.rs:165-174),
+ * the preconditioner stays the local Jacobi one (scf.rs:775-791) */
+int or_lobpcg_nl(int n, double h, const double *v, const or_nl *nl, int k, double *x, double *evals,
+                 double precond_lambda, double tol, int max_iter, double *max_residual, double *residuals) {
     long len = (long)n * n * n;
     double *xs = alloc0(k * len), *p = alloc0(k * len), *hx = alloc0(k * len), *w = alloc0(k * len);
     double *r = alloc0(len), *s = alloc0(3 * k * len), *hs = alloc0(3 * k * len);
@@ -566,7 +572,7 @@ int or_lobpcg(int n, double h, const double *v, int k, double *x, double *evals,
     double max_res = INFINITY;
     for (;;) {
         for (int i = 0; i < k; i++) {
-            or_hamiltonian(n, h, v, xs + i * len, hx + i * len);
+            or_pseudo_hamiltonian(n, h, v, nl, xs + i * len, hx + i * len);
             evals[i] = or_dot(n, h, xs + i * len, hx + i * len);
         }
         max_res = 0.0;
@@ -590,7 +596,7 @@ int or_lobpcg(int n, double h, const double *v, int k, double *x, double *evals,
             cnt += k;
         }
         int m = orthonormalise(n, h, s, cnt);
-        for (int i = 0; i < m; i++) or_hamiltonian(n, h, v, s + i * len, hs + i * len);
+        for (int i = 0; i < m; i++) or_pseudo_hamiltonian(n, h, v, nl, s + i * len, hs + i * len);
         double *a = alloc0((long)m * m), *ev = alloc0(m), *vec = alloc0((long)m * m);
         for (int i = 0; i < m; i++)
             for (int j = 0; j <= i; j++) {
@@ -835,9 +841,9 @@ static double max_on_faces(int n, const double *f) {
 
 /* one channel under Aufbau or Fermi–Dirac (qm/src/scf.rs:774-877, all columns tight). orb: *k vectors with room for
  * cap_total; grows *k. Returns status. */
-static int solve_channel(int n, double h, const double *v_eff, int nat, const double *pos, double **orb, double *ev,
-                         double *res, int *k, int nel, const or_settings *s, int spin, double *occ, double *mu,
-                         double *entropy, double *max_eig_res, int cap_total) {
+static int solve_channel(int n, double h, const double *v_eff, const or_nl *nl, int nat, const double *pos,
+                         double **orb, double *ev, double *res, int *k, int nel, const or_settings *s, int spin,
+                         double *occ, double *mu, double *entropy, double *max_eig_res, int cap_total) {
     long len = (long)n * n * n;
     if (*k == 0) {
         *mu = NAN;
@@ -847,7 +853,7 @@ static int solve_channel(int n, double h, const double *v_eff, int nat, const do
     for (;;) {
         double lambda = ev[0];
         double mr;
-        int it = or_lobpcg(n, h, v_eff, *k, *orb, ev, lambda, s->eig_tol, 200, &mr, res);
+        int it = or_lobpcg_nl(n, h, v_eff, nl, *k, *orb, ev, lambda, s->eig_tol, 200, &mr, res);
         if (it < 0) return OR_NUMERICAL;
         if (mr > *max_eig_res || isnan(mr)) *max_eig_res = mr;
         if (s->occupations == 0) {
@@ -955,23 +961,68 @@ int or_scf(int n, double h, int nat, const double *pos, const int *z, int n_up, 
            or_scf_stats *st, int cap, double *evals_up, double *evals_dn, double *occ_up, double *occ_dn,
            double *rho_up_out, double *rho_dn_out, double *forces_out, double *orbitals_up_out,
            double *orbitals_dn_out) {
+    return or_scf_ionic(n, h, nat, pos, z, n_up, n_dn, s, NULL, st, cap, evals_up, evals_dn, occ_up, occ_dn, rho_up_out,
+                        rho_dn_out, forces_out, orbitals_up_out, orbitals_dn_out);
+}
+
+/* XC on (rho_up + pc/2, rho_dn + pc/2), plain LSDA without a partial core (scf.rs:757-769) */
+static void xc_with_core(long len, const double *up, const double *dn, const double *pc, double *e, double *vu,
+                         double *vd) {
+    if (!pc) {
+        or_lsda(len, up, dn, e, vu, vd);
+        return;
+    }
+    double *u = alloc0(len), *d = alloc0(len);
+    for (long i = 0; i < len; i++) {
+        u[i] = up[i] + 0.5 * pc[i];
+        d[i] = dn[i] + 0.5 * pc[i];
+    }
+    or_lsda(len, u, d, e, vu, vd);
+    free(u);
+    free(d);
+}
+
+/* qm/src/scf.rs:928-1098 with the ionic terms of scf.rs:720-755 (io NULL: all-electron, the same code path) */
+int or_scf_ionic(int n, double h, int nat, const double *pos, const int *z, int n_up, int n_dn, const or_settings *s,
+                 const or_ionic *io, or_scf_stats *st, int cap, double *evals_up, double *evals_dn, double *occ_up,
+                 double *occ_dn, double *rho_up_out, double *rho_dn_out, double *forces_out, double *orbitals_up_out,
+                 double *orbitals_dn_out) {
     double t_start = now_s();
     memset(st, 0, sizeof *st);
     long len = (long)n * n * n;
     double e_nn;
     int ci, cj;
-    if (or_nuclear_repulsion(nat, pos, z, &e_nn, &ci, &cj) != OR_OK) {
+    int rep = io ? or_ionic_repulsion(nat, pos, z, io, &e_nn, &ci, &cj) : or_nuclear_repulsion(nat, pos, z, &e_nn, &ci, &cj);
+    if (rep != OR_OK) {
         st->status = OR_COINCIDENT;
         return OR_COINCIDENT;
     }
     double *v_nuc = alloc0(len);
-    or_nuclear_potential(n, h, nat, pos, z, s->sigma, v_nuc);
+    double *pc = NULL;
+    or_nl *nl = NULL;
+    if (io) {
+        or_ionic_local_potential(n, h, nat, pos, z, io, v_nuc);
+        pc = alloc0(len);
+        if (!or_partial_core_field(n, h, nat, pos, z, io, pc)) {
+            free(pc);
+            pc = NULL;
+        }
+        nl = or_nl_build(n, h, nat, pos, z, io, 0);
+        if (or_nl_spheres(nl) == 0) {
+            or_nl_free(nl);
+            nl = NULL;
+        }
+    } else {
+        or_nuclear_potential(n, h, nat, pos, z, s->sigma, v_nuc);
+    }
     double ptol = 1e-3 * s->tol < 1e-10 ? 1e-3 * s->tol : 1e-10; /* scf.rs:519 */
     int fd = s->occupations == 1;
     int cap_total = cap;
     int ku = n_up + (fd && n_up > 0 ? 1 : 0), kd = n_dn + (fd && n_dn > 0 ? 1 : 0);
     if (ku > cap_total || kd > cap_total) {
         free(v_nuc);
+        free(pc);
+        if (nl) or_nl_free(nl);
         st->status = OR_NUMERICAL;
         return OR_NUMERICAL;
     }
@@ -983,9 +1034,25 @@ int or_scf(int n, double h, int nat, const double *pos, const int *z, int n_up, 
     for (int i = 0; i < cap_total; i++) ev_up[i] = ev_dn[i] = -1.0;
     /* cold guess: sigma = 1 blobs scaled to the electron count (scf.rs:951-964; nuclear.rs:204-208) */
     double *guess = alloc0(len);
-    or_deposit_charge(n, h, nat, pos, z, 1.0, guess);
     double z_tot = 0.0;
-    for (int a = 0; a < nat; a++) z_tot += (double)z[a];
+    if (io) {
+        /* sigma = 1 blobs of each centre's charge q (Z_v pseudised, Z all-electron; nuclear.rs:221-226, 304-307) */
+        const double norm = 1.0 / pow(2.0 * M_PI, 1.5);
+        for (long i = 0; i < len; i++) {
+            double p[3];
+            grid_point(n, h, i, p);
+            double acc = 0.0;
+            for (int a = 0; a < nat; a++) {
+                double dx = p[0] - pos[3 * a], dy = p[1] - pos[3 * a + 1], dz = p[2] - pos[3 * a + 2];
+                acc += or_valence_charge(io, z[a]) * norm * exp(-(dx * dx + dy * dy + dz * dz) / 2.0);
+            }
+            guess[i] = acc;
+        }
+        for (int a = 0; a < nat; a++) z_tot += or_valence_charge(io, z[a]);
+    } else {
+        or_deposit_charge(n, h, nat, pos, z, 1.0, guess);
+        for (int a = 0; a < nat; a++) z_tot += (double)z[a];
+    }
     double total = (double)(n_up + n_dn);
     double scale = z_tot > 0.0 ? total / z_tot : 0.0;
     double fu = total > 0.0 ? (double)n_up / total : 0.0, fdn = total > 0.0 ? (double)n_dn / total : 0.0;
@@ -1019,17 +1086,17 @@ int or_scf(int n, double h, int nat, const double *pos, const int *z, int n_up, 
         t_poisson += now_s() - tp;
         if (pres > st->poisson_residual) st->poisson_residual = pres;
         if (pit > st->poisson_iterations) st->poisson_iterations = pit;
-        or_lsda(len, rin_up, rin_dn, xe, xu, xd);
+        xc_with_core(len, rin_up, rin_dn, pc, xe, xu, xd);
         for (long i = 0; i < len; i++) {
             vu[i] = v_nuc[i] + v_h[i] + xu[i];
             vd[i] = v_nuc[i] + v_h[i] + xd[i];
         }
         double te = now_s();
-        status = solve_channel(n, h, vu, nat, pos, &orb_up, ev_up, res_up, &ku, n_up, s, 0, oc_up, &mu_up, &s_up,
+        status = solve_channel(n, h, vu, nl, nat, pos, &orb_up, ev_up, res_up, &ku, n_up, s, 0, oc_up, &mu_up, &s_up,
                                &max_eig, cap_total);
         if (status == OR_OK)
-            status = solve_channel(n, h, vd, nat, pos, &orb_dn, ev_dn, res_dn, &kd, n_dn, s, 1, oc_dn, &mu_dn, &s_dn,
-                                   &max_eig, cap_total);
+            status = solve_channel(n, h, vd, nl, nat, pos, &orb_dn, ev_dn, res_dn, &kd, n_dn, s, 1, oc_dn, &mu_dn,
+                                   &s_dn, &max_eig, cap_total);
         t_eigen += now_s() - te;
         if (status != OR_OK) break;
         memset(rout_up, 0, (size_t)len * sizeof(double));
@@ -1133,8 +1200,10 @@ int or_scf(int n, double h, int nat, const double *pos, const int *z, int n_up, 
                 kin += fd ? f * t : t;
             }
         }
-        or_lsda(len, rout_up, rout_dn, xe, xu, xd);
+        xc_with_core(len, rout_up, rout_dn, pc, xe, xu, xd);
         double external_ = or_dot(n, h, rho, v_nuc);
+        if (nl) /* + E_NL, f-weighted under Fermi–Dirac (scf.rs:1066-1072) */
+            external_ += or_nl_energy(nl, ku, orb_up, fd ? oc_up : NULL) + or_nl_energy(nl, kd, orb_dn, fd ? oc_dn : NULL);
         double hartree = 0.5 * or_dot(n, h, rho, v_h);
         double xc_e = or_integrate(n, h, xe);
         double internal = kin + external_ + hartree + xc_e + e_nn;
@@ -1173,7 +1242,26 @@ int or_scf(int n, double h, int nat, const double *pos, const int *z, int n_up, 
         if (rho_dn_out) memcpy(rho_dn_out, rout_dn, (size_t)len * sizeof(double));
         if (orbitals_up_out) memcpy(orbitals_up_out, orb_up, (size_t)(kc_u * len) * sizeof(double));
         if (orbitals_dn_out) memcpy(orbitals_dn_out, orb_dn, (size_t)(kc_d * len) * sizeof(double));
-        if (forces_out) status = or_forces(n, h, rho, nat, pos, z, s->sigma, forces_out);
+        if (forces_out && !io) status = or_forces(n, h, rho, nat, pos, z, s->sigma, forces_out);
+        if (forces_out && io) {
+            /* forces.rs:249-257: the per-component Neumaier sum of the five terms, in order */
+            int kt = ku + kd;
+            double *orbs = alloc0((long)kt * len), *oc = alloc0(kt), *terms = alloc0(15L * nat);
+            memcpy(orbs, orb_up, (size_t)ku * len * sizeof(double));
+            memcpy(orbs + (long)ku * len, orb_dn, (size_t)kd * len * sizeof(double));
+            for (int i = 0; i < ku; i++) oc[i] = fd ? oc_up[i] : 1.0;
+            for (int i = 0; i < kd; i++) oc[ku + i] = fd ? oc_dn[i] : 1.0;
+            status = or_pseudo_force_terms(n, h, rout_up, rout_dn, kt, orbs, oc, nat, pos, z, io, terms);
+            for (int I = 0; I < nat; I++)
+                for (int a = 0; a < 3; a++) {
+                    double t5[5];
+                    for (int t = 0; t < 5; t++) t5[t] = terms[(t * nat + I) * 3 + a];
+                    forces_out[3 * I + a] = or_neumaier(t5, 5);
+                }
+            free(orbs);
+            free(oc);
+            free(terms);
+        }
     }
     st->status = status;
     st->seconds_poisson = t_poisson;
@@ -1185,7 +1273,455 @@ int or_scf(int n, double h, int nat, const double *pos, const int *z, int n_up, 
     free(v_nuc); free(orb_up); free(orb_dn); free(ev_up); free(ev_dn); free(res_up); free(res_dn);
     free(oc_up); free(oc_dn); free(rin_up); free(rin_dn); free(v_h); free(rout_up); free(rout_dn); free(rho);
     free(xe); free(xu); free(xd); free(vu); free(vd);
+    free(pc);
+    if (nl) or_nl_free(nl);
     return status;
+}
+
+/* ---------------------------------------------------------- pseudised ions ---- */
+
+/* qm/src/radial/filter.rs:302-318: 4-point Lagrange on a uniform table of an even function (index -1 mirrors 1) */
+static void uniform_lagrange(const double *f, int len, double du, double d, double *val, double *der) {
+    double sv = d / du;
+    long i0 = (long)floor(sv) - 1;
+    if (i0 + 3 >= (long)len) {
+        *val = 0.0;
+        *der = 0.0;
+        return;
+    }
+    double t = sv - (double)i0;
+    double a = f[labs(i0)], b = f[labs(i0 + 1)], c = f[labs(i0 + 2)], e = f[labs(i0 + 3)];
+    double l0 = -(t - 1.0) * (t - 2.0) * (t - 3.0) / 6.0, l1 = t * (t - 2.0) * (t - 3.0) / 2.0;
+    double l2 = -t * (t - 1.0) * (t - 3.0) / 2.0, l3 = t * (t - 1.0) * (t - 2.0) / 6.0;
+    double d0 = -((t - 2.0) * (t - 3.0) + (t - 1.0) * (t - 3.0) + (t - 1.0) * (t - 2.0)) / 6.0;
+    double d1 = ((t - 2.0) * (t - 3.0) + t * (t - 3.0) + t * (t - 2.0)) / 2.0;
+    double d2 = -((t - 1.0) * (t - 3.0) + t * (t - 3.0) + t * (t - 1.0)) / 2.0;
+    double d3 = ((t - 1.0) * (t - 2.0) + t * (t - 2.0) + t * (t - 1.0)) / 6.0;
+    *val = a * l0 + b * l1 + c * l2 + e * l3;
+    *der = (a * d0 + b * d1 + c * d2 + e * d3) / du;
+}
+
+/* filter.rs:355 (read): 0 beyond the table's radius */
+void or_table_read(const double *f, int len, double du, double radius, double d, double *val, double *der) {
+    if (d > radius || !f) {
+        *val = 0.0;
+        *der = 0.0;
+        return;
+    }
+    uniform_lagrange(f, len, du, d, val, der);
+}
+
+static const or_species *find_species(const or_ionic *io, int z) {
+    if (!io) return NULL;
+    for (int i = 0; i < io->n_species; i++)
+        if (io->species[i].z == z) return &io->species[i];
+    return NULL;
+}
+
+/* ionic.rs:179-182 */
+double or_valence_charge(const or_ionic *io, int z) {
+    const or_species *sp = find_species(io, z);
+    return sp ? sp->z_v : (double)z;
+}
+
+static double dist2(const double a[3], const double b[3]) {
+    return (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]);
+}
+
+/* nuclear.rs:322-337 */
+long or_points_within(int n, double h, const double c[3], double radius, long *out) {
+    double o = origin_of(n, h);
+    long lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {
+        double l = floor((c[a] - radius - o) / h);
+        double u = ceil((c[a] + radius - o) / h);
+        if (l < 0.0) l = 0.0;
+        if (u > (double)(n - 1)) u = (double)(n - 1);
+        if (!(u >= l)) return 0;
+        lo[a] = (long)l;
+        hi[a] = (long)u;
+    }
+    long cnt = 0;
+    for (long k = lo[2]; k <= hi[2]; k++)
+        for (long j = lo[1]; j <= hi[1]; j++)
+            for (long i = lo[0]; i <= hi[0]; i++) {
+                long p = i + (long)n * (j + (long)n * k);
+                double r[3];
+                grid_point(n, h, p, r);
+                if (dist2(r, c) <= radius * radius) {
+                    if (out) out[cnt] = p;
+                    cnt++;
+                }
+            }
+    return cnt;
+}
+
+static long *points_alloc(int n, double h, const double c[3], double radius, long *cnt) {
+    *cnt = or_points_within(n, h, c, radius, NULL);
+    long *pts = (long *)malloc(sizeof(long) * (size_t)(*cnt > 0 ? *cnt : 1));
+    or_points_within(n, h, c, radius, pts);
+    return pts;
+}
+
+/* nuclear.rs:290-302 (smeared_potential over centres (q, sigma)) + 342-350 (the V_SR tables) */
+void or_ionic_local_potential(int n, double h, int nat, const double *pos, const int *z, const or_ionic *io,
+                              double *out) {
+    long len = (long)n * n * n;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < len; i++) {
+        double p[3];
+        grid_point(n, h, i, p);
+        double acc = 0.0;
+        for (int a = 0; a < nat; a++) {
+            const or_species *sp = find_species(io, z[a]);
+            double q = sp ? sp->z_v : (double)z[a], sg = sp ? sp->sigma_lr : io->sigma_ae;
+            double d = sqrt(dist2(p, pos + 3 * a));
+            acc += d < 1e-10 ? -q * sqrt(2.0 / M_PI) / sg : -q * erf(d / (sg * sqrt(2.0))) / d;
+        }
+        out[i] = acc;
+    }
+    for (int a = 0; a < nat; a++) {
+        const or_species *sp = find_species(io, z[a]);
+        if (!sp) continue;
+        long cnt;
+        long *pts = points_alloc(n, h, pos + 3 * a, sp->r_sr, &cnt);
+        for (long q = 0; q < cnt; q++) {
+            double r[3], v, dv;
+            grid_point(n, h, pts[q], r);
+            or_table_read(sp->v_sr_u, sp->n_v_sr, sp->du, sp->r_sr, sqrt(dist2(r, pos + 3 * a)), &v, &dv);
+            out[pts[q]] += v;
+        }
+        free(pts);
+    }
+}
+
+/* nuclear.rs:354-368 */
+int or_partial_core_field(int n, double h, int nat, const double *pos, const int *z, const or_ionic *io,
+                          double *out) {
+    long len = (long)n * n * n;
+    memset(out, 0, (size_t)len * sizeof(double));
+    int any = 0;
+    for (int a = 0; a < nat; a++) {
+        const or_species *sp = find_species(io, z[a]);
+        if (!sp || sp->pc_radius <= 0.0) continue;
+        any = 1;
+        long cnt;
+        long *pts = points_alloc(n, h, pos + 3 * a, sp->pc_radius, &cnt);
+        for (long q = 0; q < cnt; q++) {
+            double r[3], v, dv;
+            grid_point(n, h, pts[q], r);
+            or_table_read(sp->rho_pc_u, sp->n_pc, sp->du, sp->pc_radius, sqrt(dist2(r, pos + 3 * a)), &v, &dv);
+            out[pts[q]] += v;
+        }
+        free(pts);
+    }
+    return any;
+}
+
+/* nuclear.rs:371-382 */
+int or_ionic_repulsion(int nat, const double *pos, const int *z, const or_ionic *io, double *e, int *pi, int *pj) {
+    double acc = 0.0;
+    for (int i = 0; i < nat; i++)
+        for (int j = i + 1; j < nat; j++) {
+            double d = sqrt(dist2(pos + 3 * i, pos + 3 * j));
+            if (d < 1e-8) {
+                *pi = i;
+                *pj = j;
+                return OR_COINCIDENT;
+            }
+            acc += or_valence_charge(io, z[i]) * or_valence_charge(io, z[j]) / d;
+        }
+    *e = acc;
+    return OR_OK;
+}
+
+/* nonlocal.rs:63-73 */
+void or_solid_harmonic_gradients(const double r[3], double out[9][3]) {
+    double x = r[0], y = r[1], zz = r[2], s3 = sqrt(3.0);
+    double g[9][3] = {{0.0, 0.0, 0.0},         {0.0, 1.0, 0.0},           {0.0, 0.0, 1.0},
+                      {1.0, 0.0, 0.0},         {s3 * y, s3 * x, 0.0},     {0.0, s3 * zz, s3 * y},
+                      {-x, -y, 2.0 * zz},      {s3 * zz, 0.0, s3 * x},    {s3 * x, -s3 * y, 0.0}};
+    memcpy(out, g, sizeof g);
+}
+
+typedef struct {
+    int ion, l;
+    long npts;
+    long *points;
+    double *beta; /* (2l+1) x npts */
+    double *grad; /* (2l+1) x npts x 3, or NULL */
+    double e_kb;
+} or_sphere;
+
+struct or_nl {
+    int ns, rows;
+    or_sphere *s;
+    double dv;
+    long len; /* n^3: the length of one orbital */
+};
+
+/* nonlocal.rs:94-128 */
+or_nl *or_nl_build(int n, double h, int nat, const double *pos, const int *z, const or_ionic *io, int with_grad) {
+    or_nl *nl = (or_nl *)calloc(1, sizeof(or_nl));
+    nl->dv = h * h * h;
+    nl->len = (long)n * n * n;
+    int cap = 0;
+    for (int a = 0; a < nat; a++) {
+        const or_species *sp = find_species(io, z[a]);
+        if (sp) cap += sp->n_proj;
+    }
+    nl->s = (or_sphere *)calloc((size_t)(cap > 0 ? cap : 1), sizeof(or_sphere));
+    for (int a = 0; a < nat; a++) {
+        const or_species *sp = find_species(io, z[a]);
+        if (!sp) continue;
+        for (int k = 0; k < sp->n_proj; k++) {
+            const or_projector *pr = &sp->proj[k];
+            or_sphere *sph = &nl->s[nl->ns++];
+            sph->ion = a;
+            sph->l = pr->l;
+            sph->e_kb = pr->e_kb;
+            sph->points = points_alloc(n, h, pos + 3 * a, pr->radius, &sph->npts);
+            int nm = 2 * pr->l + 1;
+            nl->rows += nm;
+            sph->beta = alloc0((long)nm * sph->npts);
+            sph->grad = with_grad ? alloc0(3L * nm * sph->npts) : NULL;
+            double norm = sqrt((double)(2 * pr->l + 1) / (4.0 * M_PI));
+            for (long q = 0; q < sph->npts; q++) {
+                double r[3], d[3], sh[OR_N_MOMENTS];
+                grid_point(n, h, sph->points[q], r);
+                for (int c = 0; c < 3; c++) d[c] = r[c] - pos[3 * a + c];
+                double dd = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                or_solid_harmonics(d, sh);
+                double tv, dt;
+                or_table_read(pr->t_u, pr->n_t, sp->du, pr->radius, dd, &tv, &dt);
+                double tn = tv * norm;
+                for (int m = -pr->l; m <= pr->l; m++)
+                    sph->beta[(long)(m + pr->l) * sph->npts + q] = tn * sh[lm_index(pr->l, m)];
+                if (with_grad) {
+                    double gr[9][3];
+                    or_solid_harmonic_gradients(d, gr);
+                    for (int m = -pr->l; m <= pr->l; m++) {
+                        int lm = lm_index(pr->l, m);
+                        double radial = dd > 1e-12 ? dt * norm * sh[lm] / dd : 0.0;
+                        for (int c = 0; c < 3; c++)
+                            sph->grad[((long)(m + pr->l) * sph->npts + q) * 3 + c] = radial * d[c] + tn * gr[lm][c];
+                    }
+                }
+            }
+        }
+    }
+    return nl;
+}
+
+void or_nl_free(or_nl *nl) {
+    if (!nl) return;
+    for (int i = 0; i < nl->ns; i++) {
+        free(nl->s[i].points);
+        free(nl->s[i].beta);
+        free(nl->s[i].grad);
+    }
+    free(nl->s);
+    free(nl);
+}
+
+int or_nl_rows(const or_nl *nl) { return nl ? nl->rows : 0; }
+int or_nl_spheres(const or_nl *nl) { return nl ? nl->ns : 0; }
+long or_nl_sphere_points(const or_nl *nl, int s) { return nl->s[s].npts; }
+
+/* nonlocal.rs:133-135: c[sphere][m + l] = h^3 sum_p beta_lm(r_p) psi(r_p) (Neumaier) */
+void or_nl_project(const or_nl *nl, const double *psi, double *c) {
+    int row = 0;
+    for (int i = 0; i < nl->ns; i++) {
+        const or_sphere *sph = &nl->s[i];
+        double *tmp = alloc0(sph->npts);
+        for (int m = 0; m < 2 * sph->l + 1; m++) {
+            const double *b = sph->beta + (long)m * sph->npts;
+            for (long q = 0; q < sph->npts; q++) tmp[q] = b[q] * psi[sph->points[q]];
+            c[row++] = or_neumaier(tmp, sph->npts) * nl->dv;
+        }
+        free(tmp);
+    }
+}
+
+/* nonlocal.rs:138-145 */
+void or_nl_apply_add(const or_nl *nl, const double *psi, double *out) {
+    double *c = alloc0(nl->rows);
+    or_nl_project(nl, psi, c);
+    int row = 0;
+    for (int i = 0; i < nl->ns; i++) {
+        const or_sphere *sph = &nl->s[i];
+        for (int m = 0; m < 2 * sph->l + 1; m++) {
+            double f = sph->e_kb * c[row++];
+            const double *b = sph->beta + (long)m * sph->npts;
+            for (long q = 0; q < sph->npts; q++) out[sph->points[q]] += f * b[q];
+        }
+    }
+    free(c);
+}
+
+/* nonlocal.rs:148-172 (energy / energy_weighted): one Neumaier sum over every (orbital, sphere, m) term */
+double or_nl_energy(const or_nl *nl, int k, const double *orbitals, const double *f) {
+    if (!nl || nl->rows == 0 || k == 0) return 0.0;
+    const long len = nl->len;
+    double *terms = alloc0((long)k * nl->rows), *c = alloc0(nl->rows);
+    long nt = 0;
+    for (int b = 0; b < k; b++) {
+        double fi = f ? f[b] : 1.0;
+        if (f && fi == 0.0) continue;
+        or_nl_project(nl, orbitals + (long)b * len, c);
+        int row = 0;
+        for (int i = 0; i < nl->ns; i++)
+            for (int m = 0; m < 2 * nl->s[i].l + 1; m++, row++)
+                terms[nt++] = f ? fi * nl->s[i].e_kb * c[row] * c[row] : nl->s[i].e_kb * c[row] * c[row];
+    }
+    double e = or_neumaier(terms, nt);
+    free(terms);
+    free(c);
+    return e;
+}
+
+/* nonlocal.rs:165-174 */
+void or_pseudo_hamiltonian(int n, double h, const double *v, const or_nl *nl, const double *x, double *out) {
+    or_hamiltonian(n, h, v, x, out);
+    if (nl && nl->ns > 0) or_nl_apply_add(nl, x, out);
+}
+
+/* forces.rs:232-246: sum_p w_p V'(d_p)(r_p - R)/d_p h^3 over the points (every point when pts is NULL), on-site
+ * points (d < 1e-8) skipped; kind 0: -q g'(d, sigma) (the smeared erf), 1: a table's interpolant derivative */
+typedef struct {
+    int kind;
+    double q, sigma;
+    const double *tab;
+    int len;
+    double du, radius;
+} radial_deriv;
+
+static double radial_dvdd(const radial_deriv *rd, double d) {
+    if (rd->kind == 0) {
+        /* FilteredTable::dv_lr (filter.rs:347-352) and -Z g'(d) (forces.rs:177-181): one formula */
+        if (d < 1e-10) return 0.0;
+        double x = d / (rd->sigma * sqrt(2.0));
+        return -rd->q * ((2.0 / sqrt(M_PI)) * x * exp(-x * x) - erf(x)) / (d * d);
+    }
+    double v, dv;
+    or_table_read(rd->tab, rd->len, rd->du, rd->radius, d, &v, &dv);
+    return dv;
+}
+
+static void radial_force(int n, double h, const long *pts, long npts, const double c[3], const double *w,
+                         const radial_deriv *rd, double out[3]) {
+    long len = (long)n * n * n;
+    long cnt = pts ? npts : len;
+    double *comp = alloc0(3 * cnt);
+    long m = 0;
+    for (long q = 0; q < cnt; q++) {
+        long p = pts ? pts[q] : q;
+        double r[3];
+        grid_point(n, h, p, r);
+        double d[3] = {r[0] - c[0], r[1] - c[1], r[2] - c[2]};
+        double dd = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        if (dd < 1e-8) continue;
+        double s = w[p] * radial_dvdd(rd, dd) / dd;
+        for (int a = 0; a < 3; a++) comp[a * cnt + m] = s * d[a];
+        m++;
+    }
+    for (int a = 0; a < 3; a++) out[a] = or_neumaier(comp + a * cnt, m) * h * h * h;
+    free(comp);
+}
+
+/* forces.rs:259-350 */
+int or_pseudo_force_terms(int n, double h, const double *rho_up, const double *rho_dn, int k, const double *orbitals,
+                          const double *occ, int nat, const double *pos, const int *z, const or_ionic *io,
+                          double *terms) {
+    long len = (long)n * n * n;
+    memset(terms, 0, sizeof(double) * 15 * (size_t)nat);
+    double *t_ii = terms, *t_lr = terms + 3 * nat, *t_sr = terms + 6 * nat, *t_nl = terms + 9 * nat,
+           *t_pc = terms + 12 * nat;
+    /* ion-ion (forces.rs:335-350) */
+    for (int i = 0; i < nat; i++)
+        for (int j = 0; j < nat; j++) {
+            if (i == j) continue;
+            double dv[3] = {pos[3 * i] - pos[3 * j], pos[3 * i + 1] - pos[3 * j + 1], pos[3 * i + 2] - pos[3 * j + 2]};
+            double d = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+            if (d < 1e-8) return OR_COINCIDENT;
+            double sc = or_valence_charge(io, z[i]) * or_valence_charge(io, z[j]) / (d * d * d);
+            for (int a = 0; a < 3; a++) t_ii[3 * i + a] += sc * dv[a];
+        }
+    double *rho = alloc0(len);
+    for (long p = 0; p < len; p++) rho[p] = rho_up[p] + rho_dn[p];
+    /* local LR and SR (forces.rs:281-297) */
+    for (int a = 0; a < nat; a++) {
+        const or_species *sp = find_species(io, z[a]);
+        radial_deriv lr = {0, sp ? sp->z_v : (double)z[a], sp ? sp->sigma_lr : io->sigma_ae, NULL, 0, 0.0, 0.0};
+        radial_force(n, h, NULL, 0, pos + 3 * a, rho, &lr, t_lr + 3 * a);
+        if (sp) {
+            long cnt;
+            long *pts = points_alloc(n, h, pos + 3 * a, sp->r_sr, &cnt);
+            radial_deriv sr = {1, 0.0, 0.0, sp->v_sr_u, sp->n_v_sr, sp->du, sp->r_sr};
+            radial_force(n, h, pts, cnt, pos + 3 * a, rho, &sr, t_sr + 3 * a);
+            free(pts);
+        }
+    }
+    /* nonlocal (forces.rs:299-318) */
+    or_nl *nl = or_nl_build(n, h, nat, pos, z, io, 1);
+    long maxn = 0;
+    for (int i = 0; i < nl->ns; i++)
+        if (nl->s[i].npts > maxn) maxn = nl->s[i].npts;
+    double **per = (double **)calloc((size_t)nat * 3, sizeof(double *));
+    long *per_n = (long *)calloc((size_t)nat * 3, sizeof(long));
+    long per_cap = (long)k * 9 * nl->ns + 1;
+    for (int i = 0; i < nat * 3; i++) per[i] = alloc0(per_cap);
+    double *tmp = alloc0(maxn > 0 ? maxn : 1);
+    for (int i = 0; i < nl->ns; i++) {
+        const or_sphere *sph = &nl->s[i];
+        for (int b = 0; b < k; b++) {
+            double f = occ[b];
+            if (f == 0.0) continue;
+            const double *psi = orbitals + (long)b * len;
+            for (int m = 0; m < 2 * sph->l + 1; m++) {
+                const double *bv = sph->beta + (long)m * sph->npts;
+                for (long q = 0; q < sph->npts; q++) tmp[q] = bv[q] * psi[sph->points[q]];
+                double c = or_neumaier(tmp, sph->npts) * nl->dv;
+                for (int a = 0; a < 3; a++) {
+                    for (long q = 0; q < sph->npts; q++)
+                        tmp[q] = sph->grad[((long)m * sph->npts + q) * 3 + a] * psi[sph->points[q]];
+                    double gpsi = or_neumaier(tmp, sph->npts) * nl->dv;
+                    int slot = sph->ion * 3 + a;
+                    per[slot][per_n[slot]++] = 2.0 * f * sph->e_kb * c * gpsi;
+                }
+            }
+        }
+    }
+    for (int i = 0; i < nat * 3; i++) {
+        t_nl[i] = or_neumaier(per[i], per_n[i]);
+        free(per[i]);
+    }
+    free(per);
+    free(per_n);
+    free(tmp);
+    or_nl_free(nl);
+    /* partial core (forces.rs:320-333) */
+    double *pc = alloc0(len);
+    if (or_partial_core_field(n, h, nat, pos, z, io, pc)) {
+        double *e = alloc0(len), *vu = alloc0(len), *vd = alloc0(len);
+        xc_with_core(len, rho_up, rho_dn, pc, e, vu, vd);
+        for (long p = 0; p < len; p++) e[p] = 0.5 * (vu[p] + vd[p]);
+        for (int a = 0; a < nat; a++) {
+            const or_species *sp = find_species(io, z[a]);
+            if (!sp || sp->pc_radius <= 0.0) continue;
+            long cnt;
+            long *pts = points_alloc(n, h, pos + 3 * a, sp->pc_radius, &cnt);
+            radial_deriv pd = {1, 0.0, 0.0, sp->rho_pc_u, sp->n_pc, sp->du, sp->pc_radius};
+            radial_force(n, h, pts, cnt, pos + 3 * a, e, &pd, t_pc + 3 * a);
+            free(pts);
+        }
+        free(e);
+        free(vu);
+        free(vd);
+    }
+    free(pc);
+    free(rho);
+    return OR_OK;
 }
 
 /* ---------------------------------------------------------- multigrid model ---- */

@@ -107,6 +107,83 @@ int or_scf(int n, double h, int nat, const double *pos, const int *z, int n_up, 
 int or_forces(int n, double h, const double *rho, int nat, const double *pos, const int *z, double sigma,
               double *out);
 
+/* ------------------------------------------------------------------------------------------------------------
+ * Pseudised ions: the Kleinman–Bylander nonlocal operator, the ionic local potential and the partial core
+ * (qm/src/nonlocal.rs, qm/src/ionic.rs, qm/src/nuclear.rs:290-382, qm/src/radial/filter.rs:302-363).
+ * A species carries the uniform tables of qm::radial::filter::FilteredTable (spacing du, index 0 at d = 0);
+ * a Z with no species entry is all-electron at sigma_ae (Ionic::AllElectron).
+ * ------------------------------------------------------------------------------------------------------------ */
+typedef struct {
+    int l;            /* 0..2 */
+    double e_kb;      /* Ha */
+    double radius;    /* t_u is 0 beyond (FilteredProjector::radius) */
+    const double *t_u; /* t_l = F_l / d^l on the uniform table */
+    int n_t;
+} or_projector;
+
+typedef struct {
+    int z;
+    double z_v, sigma_lr, du;
+    const double *v_sr_u;
+    int n_v_sr;
+    double r_sr;
+    const double *rho_pc_u; /* may be NULL when pc_radius <= 0 */
+    int n_pc;
+    double pc_radius; /* <= 0: no partial core */
+    const or_projector *proj;
+    int n_proj;
+} or_species;
+
+typedef struct {
+    const or_species *species;
+    int n_species;
+    double sigma_ae;
+} or_ionic;
+
+/* radial/filter.rs:302-318, 355-363: (value, d/dd) of the 4-point Lagrange interpolant, 0 beyond radius */
+void or_table_read(const double *f, int len, double du, double radius, double d, double *val, double *der);
+/* nuclear.rs:322-337: grid points with |r_p - c| <= radius, ascending; returns the count (out may be NULL) */
+long or_points_within(int n, double h, const double c[3], double radius, long *out);
+/* nuclear.rs:342-350 */
+void or_ionic_local_potential(int n, double h, int nat, const double *pos, const int *z, const or_ionic *io,
+                              double *out);
+/* nuclear.rs:354-363; returns 1 if any centre carries a partial core (out is filled either way) */
+int or_partial_core_field(int n, double h, int nat, const double *pos, const int *z, const or_ionic *io,
+                          double *out);
+/* nuclear.rs:371-382 */
+int or_ionic_repulsion(int nat, const double *pos, const int *z, const or_ionic *io, double *e, int *pi, int *pj);
+double or_valence_charge(const or_ionic *io, int z);
+/* nonlocal.rs:63-73: gradients of R_lm for l <= 2 in lm_index order */
+void or_solid_harmonic_gradients(const double r[3], double out[9][3]);
+
+/* nonlocal.rs:81-174: spheres in (nucleus, projector) order; c has or_nl_rows() entries, (sphere, m) order */
+typedef struct or_nl or_nl;
+or_nl *or_nl_build(int n, double h, int nat, const double *pos, const int *z, const or_ionic *io, int with_grad);
+void or_nl_free(or_nl *nl);
+int or_nl_rows(const or_nl *nl);
+int or_nl_spheres(const or_nl *nl);
+long or_nl_sphere_points(const or_nl *nl, int s);
+void or_nl_project(const or_nl *nl, const double *psi, double *c);
+void or_nl_apply_add(const or_nl *nl, const double *psi, double *out);
+/* sum_i f_i sum_lm E_KB c_lm[psi_i]^2, f == 0 orbitals skipped (f NULL: every f = 1) */
+double or_nl_energy(const or_nl *nl, int k, const double *orbitals, const double *f);
+/* H x = -1/2 lap x + v x (+ V_NL x when nl is not NULL) (nonlocal.rs:165-174) */
+void or_pseudo_hamiltonian(int n, double h, const double *v, const or_nl *nl, const double *x, double *out);
+int or_lobpcg_nl(int n, double h, const double *v, const or_nl *nl, int k, double *x, double *evals,
+                 double precond_lambda, double tol, int max_iter, double *max_residual, double *residuals);
+
+/* forces.rs:259-350: terms[t][I][a], t in [ion-ion, local LR, local SR, nonlocal, partial core]; orbitals are the
+ * k spin orbitals (up then down) with occupations occ */
+int or_pseudo_force_terms(int n, double h, const double *rho_up, const double *rho_dn, int k, const double *orbitals,
+                          const double *occ, int nat, const double *pos, const int *z, const or_ionic *io,
+                          double *terms);
+
+/* or_scf under an ionic setup (scf.rs:115-135, 720-770, 1060-1075); io NULL is or_scf */
+int or_scf_ionic(int n, double h, int nat, const double *pos, const int *z, int n_up, int n_dn, const or_settings *s,
+                 const or_ionic *io, or_scf_stats *st, int cap, double *evals_up, double *evals_dn, double *occ_up,
+                 double *occ_dn, double *rho_up_out, double *rho_dn_out, double *forces_out, double *orbitals_up_out,
+                 double *orbitals_dn_out);
+
 /* multigrid f64 helpers (qm-gpu/src/multigrid.rs:207-258): the CPU model of the GPU V-cycle */
 void or_mg_apply_a2(int n, double h, double d_low, double d_high, const double *x, double *out);
 void or_mg_restrict(int n, const double *f, double *out);
