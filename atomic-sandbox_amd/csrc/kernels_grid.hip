@@ -262,6 +262,7 @@ template <typename T, int RPT> struct SvRegs {
     typedef typename Vec16<T>::type V;
     V q[9][RPT];
     V hq[3];
+    V hq2[3];  // second x-halo vector when a vector holds fewer than the 4 halo points (double2)
     V vq[3][RPT];
     V rq[3][RPT];
 };
@@ -291,7 +292,7 @@ template <typename T, int RPT> struct SvCtx {
     int x0, y0, tx, ty, gx;
     bool xin;
     int h_ly, h_lx, h_gx, h_gy;  // this lane's halo vector (LDS row/col, global x/y), if any
-    bool has_h;
+    bool has_h, has_h2;          // has_h2: an x-halo lane that also loads the vector at h_gx + VX
     T inv_h2, scale, omega, diag_t, lamb;
     __device__ V ldv(const T *base, int xx, int yy, int zz) const {
         if (xx >= 0 && xx < n && yy >= 0 && yy < n && zz >= 0 && zz < n)
@@ -306,6 +307,7 @@ template <typename T, int MODE, int RPT, int SLOT>
 __device__ __forceinline__ void sv_side(SvRegs<T, RPT> &R, const SvCtx<T, RPT> &c, int zz) {
     constexpr int QS = SLOT % 3;
     if (c.has_h) R.hq[QS] = c.ldv(c.xb, c.h_gx, c.h_gy, zz);
+    if (SvCtx<T, RPT>::VX < 4 && c.has_h2) R.hq2[QS] = c.ldv(c.xb, c.h_gx + SvCtx<T, RPT>::VX, c.h_gy, zz);
 #pragma unroll
     for (int rr = 0; rr < RPT; rr++) {
         if (MODE != ST_LAP) R.vq[QS][rr] = c.ldv(c.v, c.gx, c.y0 + c.ty + 16 * rr, zz);
@@ -329,6 +331,7 @@ __device__ __forceinline__ void sv_plane(SvRegs<T, RPT> &R, const SvCtx<T, RPT> 
 #pragma unroll
     for (int rr = 0; rr < RPT; rr++) *reinterpret_cast<V *>(&tl[c.ty + 16 * rr + 4][lx]) = R.q[C0][rr];
     if (c.has_h) *reinterpret_cast<V *>(&tl[c.h_ly][c.h_lx]) = R.hq[HS];
+    if (VX < 4 && c.has_h2) *reinterpret_cast<V *>(&tl[c.h_ly][c.h_lx + VX]) = R.hq2[HS];
     __syncthreads();
 #pragma unroll
     for (int rr = 0; rr < RPT; rr++) {
@@ -412,7 +415,8 @@ __global__ __launch_bounds__(256) void k_stencil_vec(const T *__restrict__ x, T 
     c.nn = (long)n * n;
     c.gx = c.x0 + VX * c.tx;
     c.xin = c.gx < n;
-    // halo vectors: 8 y-halo rows x 16 vectors (lanes 0..127), then (TY + 8) rows x 2 sides (lanes 128..)
+    // halo vectors: 8 y-halo rows x 16 vectors (lanes 0..127), then (TY + 8) rows x 2 sides (lanes 128..); a side
+    // holds 4 points, i.e. 4 / VX vectors per x-halo lane
     if (t < 128) {
         const int row = t >> 4, col = t & 15;
         c.h_ly = row < 4 ? row : TY + row;
@@ -431,6 +435,7 @@ __global__ __launch_bounds__(256) void k_stencil_vec(const T *__restrict__ x, T 
         c.h_ly = c.h_lx = c.h_gx = c.h_gy = 0;
         c.has_h = false;
     }
+    c.has_h2 = c.has_h && t >= 128;
     c.inv_h2 = inv_h2;
     c.scale = scale;
     c.omega = omega;
@@ -466,6 +471,213 @@ __global__ __launch_bounds__(256) void k_stencil_vec(const T *__restrict__ x, T 
     if (z + 7 < z1) sv_plane<T, MODE, RPT, 7, LXP>(R, c, tile, z + 7, b ^ 1);
 }
 
+// ---- vector lanes carrying NB bands: lane (tx, ty) owns VX x points of row y0 + ty for bands b0 .. b0 + NB - 1 ----
+// V (and R in ST_JAC per band) is loaded once per point for the NB bands; each band has its own LDS plane.
+template <typename T, int NB> struct VbRegs {
+    typedef typename Vec16<T>::type V;
+    V q[9][NB];
+    V hq[3][NB];
+    V hq2[3][NB];
+    V vq[3];
+    V rq[3][NB];
+};
+
+template <typename T, int NB> struct VbCtx {
+    typedef typename Vec16<T>::type V;
+    static constexpr int VX = Vec16<T>::N;
+    const T *xb[NB];
+    T *ob[NB];
+    const T *rb[NB];
+    T lamb[NB];
+    bool bok[NB];
+    const T *v;
+    int n;
+    long nn;
+    int x0, y0, tx, ty, gx, gy;
+    bool xin;
+    int h_ly, h_lx, h_gx, h_gy;
+    bool has_h, has_h2;
+    T inv_h2, scale, omega, diag_t;
+    __device__ V ldv(const T *base, int xx, int yy, int zz) const {
+        if (xx >= 0 && xx < n && yy >= 0 && yy < n && zz >= 0 && zz < n)
+            return *reinterpret_cast<const V *>(base + xx + (long)n * yy + nn * zz);
+        V z0;
+        for (int i = 0; i < VX; i++) vset<T>(z0, i, T(0));
+        return z0;
+    }
+};
+
+template <typename T, int MODE, int NB, int SLOT>
+__device__ __forceinline__ void vb_side(VbRegs<T, NB> &R, const VbCtx<T, NB> &c, int zz) {
+    constexpr int QS = SLOT % 3;
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        if (c.has_h) R.hq[QS][b] = c.ldv(c.xb[b], c.h_gx, c.h_gy, zz);
+        if (VbCtx<T, NB>::VX < 4 && c.has_h2) R.hq2[QS][b] = c.ldv(c.xb[b], c.h_gx + VbCtx<T, NB>::VX, c.h_gy, zz);
+        if (MODE == ST_JAC) R.rq[QS][b] = c.ldv(c.rb[b], c.gx, c.gy, zz);
+    }
+    if (MODE != ST_LAP) R.vq[QS] = c.ldv(c.v, c.gx, c.gy, zz);
+}
+
+template <typename T, int MODE, int NB, int S, int TYP, int LXP>
+__device__ __forceinline__ void vb_plane(VbRegs<T, NB> &R, const VbCtx<T, NB> &c, T (*tile)[NB][TYP][LXP], int z,
+                                         int buf) {
+    typedef typename Vec16<T>::type V;
+    constexpr int VX = Vec16<T>::N;
+    constexpr int C0 = (S + 4) % 9;
+    constexpr int NEW = (S + 8) % 9;
+    constexpr int HS = S % 3;
+#pragma unroll
+    for (int b = 0; b < NB; b++) R.q[NEW][b] = c.ldv(c.xb[b], c.gx, c.gy, z + 4);
+    vb_side<T, MODE, NB, S + 2>(R, c, z + 2);
+    const int lx = 4 + VX * c.tx, ly = c.ty + 4;
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        *reinterpret_cast<V *>(&tile[buf][b][ly][lx]) = R.q[C0][b];
+        if (c.has_h) *reinterpret_cast<V *>(&tile[buf][b][c.h_ly][c.h_lx]) = R.hq[HS][b];
+        if (VX < 4 && c.has_h2) *reinterpret_cast<V *>(&tile[buf][b][c.h_ly][c.h_lx + VX]) = R.hq2[HS][b];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        T(*tl)[LXP] = tile[buf][b];
+        T w[VX + 8];
+#pragma unroll
+        for (int q = 0; q < (VX + 8) / VX; q++) {
+            const V vv = *reinterpret_cast<const V *>(&tl[ly][lx - 4 + VX * q]);
+#pragma unroll
+            for (int i = 0; i < VX; i++) w[VX * q + i] = vget<T>(vv, i);
+        }
+        V ym[4], yp[4];
+#pragma unroll
+        for (int m = 1; m <= 4; m++) {
+            ym[m - 1] = *reinterpret_cast<const V *>(&tl[ly - m][lx]);
+            yp[m - 1] = *reinterpret_cast<const V *>(&tl[ly + m][lx]);
+        }
+        V outv;
+#pragma unroll
+        for (int i = 0; i < VX; i++) {
+            const T ce = w[4 + i];
+            const T ax = C1 * ((w[5 + i] - ce) + (w[3 + i] - ce)) + C2 * ((w[6 + i] - ce) + (w[2 + i] - ce)) +
+                         C3 * ((w[7 + i] - ce) + (w[1 + i] - ce)) + C4 * ((w[8 + i] - ce) + (w[0 + i] - ce));
+            const T ay = C1 * ((vget<T>(yp[0], i) - ce) + (vget<T>(ym[0], i) - ce)) +
+                         C2 * ((vget<T>(yp[1], i) - ce) + (vget<T>(ym[1], i) - ce)) +
+                         C3 * ((vget<T>(yp[2], i) - ce) + (vget<T>(ym[2], i) - ce)) +
+                         C4 * ((vget<T>(yp[3], i) - ce) + (vget<T>(ym[3], i) - ce));
+            const T az = C1 * ((vget<T>(R.q[(S + 5) % 9][b], i) - ce) + (vget<T>(R.q[(S + 3) % 9][b], i) - ce)) +
+                         C2 * ((vget<T>(R.q[(S + 6) % 9][b], i) - ce) + (vget<T>(R.q[(S + 2) % 9][b], i) - ce)) +
+                         C3 * ((vget<T>(R.q[(S + 7) % 9][b], i) - ce) + (vget<T>(R.q[(S + 1) % 9][b], i) - ce)) +
+                         C4 * ((vget<T>(R.q[(S + 8) % 9][b], i) - ce) + (vget<T>(R.q[S % 9][b], i) - ce));
+            const T lap = ((ax + ay) + az) * c.inv_h2;
+            T o;
+            if (MODE == ST_LAP) {
+                o = c.scale * lap;
+            } else if (MODE == ST_HAM) {
+                o = T(-0.5) * lap + vget<T>(R.vq[HS], i) * ce;
+            } else {
+                const T vv = vget<T>(R.vq[HS], i);
+                const T hw = T(-0.5) * lap + vv * ce - c.lamb[b] * ce;
+                T d = c.diag_t + vv - c.lamb[b];
+                d = d > T(0.1) ? d : T(0.1);
+                o = ce + c.omega / d * (vget<T>(R.rq[HS][b], i) - hw);
+            }
+            vset<T>(outv, i, o);
+        }
+        if (c.bok[b] && c.xin && c.gy < c.n) *reinterpret_cast<V *>(c.ob[b] + c.gx + (long)c.n * c.gy + c.nn * z) = outv;
+    }
+}
+
+template <typename T, int MODE, int NB>
+__global__ __launch_bounds__(256) void k_stencil_vb(const T *__restrict__ x, T *__restrict__ out,
+                                                    const T *__restrict__ v, const T *__restrict__ r,
+                                                    const double *__restrict__ lam, int n, long N, int zc, int nb,
+                                                    int order, T inv_h2, T scale, T omega, T diag_t) {
+    constexpr int VX = Vec16<T>::N;
+    constexpr int TX = 16 * VX, TY = 16;
+    constexpr int LXP = TX + 8, TYP = TY + 8;
+    __shared__ __align__(16) T tile[2][NB][TYP][LXP];
+    VbCtx<T, NB> c;
+    c.tx = threadIdx.x;
+    c.ty = threadIdx.y;
+    const int t = c.ty * 16 + c.tx;
+    const int nzc = (n + zc - 1) / zc;
+    const int ngroups = (nb + NB - 1) / NB;
+    const StWork wk = st_decode((n + TX - 1) / TX, (n + TY - 1) / TY, nzc, ngroups, order);
+    if (!wk.valid) return;
+    c.x0 = wk.tile_x * TX;
+    c.y0 = wk.tile_y * TY;
+    const int z0 = wk.zchunk * zc;
+    const int z1 = min(n, z0 + zc);
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const int band = wk.band * NB + b;
+        c.bok[b] = band < nb;
+        const int bb = c.bok[b] ? band : wk.band * NB;  // a missing band reads a valid one and stores nothing
+        c.xb[b] = x + (long)bb * N;
+        c.ob[b] = out + (long)bb * N;
+        c.rb[b] = MODE == ST_JAC ? r + (long)bb * N : nullptr;
+        c.lamb[b] = MODE == ST_JAC ? (T)lam[bb] : T(0);
+    }
+    c.v = v;
+    c.n = n;
+    c.nn = (long)n * n;
+    c.gx = c.x0 + VX * c.tx;
+    c.gy = c.y0 + c.ty;
+    c.xin = c.gx < n;
+    if (t < 128) {
+        const int row = t >> 4, col = t & 15;
+        c.h_ly = row < 4 ? row : TY + row;
+        c.h_lx = 4 + VX * col;
+        c.h_gx = c.x0 + VX * col;
+        c.h_gy = c.y0 - 4 + c.h_ly;
+        c.has_h = true;
+    } else if (t < 128 + 2 * TYP) {
+        const int e = t - 128, row = e >> 1, side = e & 1;
+        c.h_ly = row;
+        c.h_lx = side ? 4 + TX : 0;
+        c.h_gx = side ? c.x0 + TX : c.x0 - 4;
+        c.h_gy = c.y0 - 4 + row;
+        c.has_h = true;
+    } else {
+        c.h_ly = c.h_lx = c.h_gx = c.h_gy = 0;
+        c.has_h = false;
+    }
+    c.has_h2 = c.has_h && t >= 128;
+    c.inv_h2 = inv_h2;
+    c.scale = scale;
+    c.omega = omega;
+    c.diag_t = diag_t;
+    VbRegs<T, NB> R;
+#pragma unroll
+    for (int d = 0; d < 8; d++)
+#pragma unroll
+        for (int b = 0; b < NB; b++) R.q[d][b] = c.ldv(c.xb[b], c.gx, c.gy, z0 - 4 + d);
+    vb_side<T, MODE, NB, 0>(R, c, z0);
+    vb_side<T, MODE, NB, 1>(R, c, z0 + 1);
+    int z = z0;
+    for (; z + 9 <= z1; z += 9) {
+        const int b = (z - z0) & 1;
+        vb_plane<T, MODE, NB, 0, TYP, LXP>(R, c, tile, z + 0, b);
+        vb_plane<T, MODE, NB, 1, TYP, LXP>(R, c, tile, z + 1, b ^ 1);
+        vb_plane<T, MODE, NB, 2, TYP, LXP>(R, c, tile, z + 2, b);
+        vb_plane<T, MODE, NB, 3, TYP, LXP>(R, c, tile, z + 3, b ^ 1);
+        vb_plane<T, MODE, NB, 4, TYP, LXP>(R, c, tile, z + 4, b);
+        vb_plane<T, MODE, NB, 5, TYP, LXP>(R, c, tile, z + 5, b ^ 1);
+        vb_plane<T, MODE, NB, 6, TYP, LXP>(R, c, tile, z + 6, b);
+        vb_plane<T, MODE, NB, 7, TYP, LXP>(R, c, tile, z + 7, b ^ 1);
+        vb_plane<T, MODE, NB, 8, TYP, LXP>(R, c, tile, z + 8, b);
+    }
+    const int b = (z - z0) & 1;
+    if (z < z1) vb_plane<T, MODE, NB, 0, TYP, LXP>(R, c, tile, z + 0, b);
+    if (z + 1 < z1) vb_plane<T, MODE, NB, 1, TYP, LXP>(R, c, tile, z + 1, b ^ 1);
+    if (z + 2 < z1) vb_plane<T, MODE, NB, 2, TYP, LXP>(R, c, tile, z + 2, b);
+    if (z + 3 < z1) vb_plane<T, MODE, NB, 3, TYP, LXP>(R, c, tile, z + 3, b ^ 1);
+    if (z + 4 < z1) vb_plane<T, MODE, NB, 4, TYP, LXP>(R, c, tile, z + 4, b);
+    if (z + 5 < z1) vb_plane<T, MODE, NB, 5, TYP, LXP>(R, c, tile, z + 5, b ^ 1);
+    if (z + 6 < z1) vb_plane<T, MODE, NB, 6, TYP, LXP>(R, c, tile, z + 6, b);
+    if (z + 7 < z1) vb_plane<T, MODE, NB, 7, TYP, LXP>(R, c, tile, z + 7, b ^ 1);
+}
+
 struct StVariant {
     int rpt;
     int zc;     // 0 = whole column
@@ -490,6 +702,10 @@ static const StVariant ST_VARIANTS[] = {
     {2, 0, 1, 1, "vector lanes, 32 rows, full z column, bands fastest"},
     {2, 0, 0, 1, "64x8 tile, full z column, bands fastest"},
     {2, 16, 0, 1, "64x8 tile, z chunks of 16, bands fastest"},
+    {2, 0, 2, 0, "vector lanes, 2 bands per lane, full z column"},
+    {2, 32, 2, 0, "vector lanes, 2 bands per lane, z chunks of 32"},
+    {2, 0, 2, 1, "vector lanes, 2 bands per lane, full z column, bands fastest"},
+    {4, 0, 2, 0, "vector lanes, 4 bands per lane, full z column"},
 };
 static int g_default_variant[2] = {-1, -1};
 
@@ -534,6 +750,20 @@ static void launch_stencil_vec(const StencilParams &p, int zc, int order, const 
     LIRA_HIP(hipGetLastError());
 }
 
+template <typename T, int MODE, int NB>
+static void launch_stencil_vb(const StencilParams &p, int zc, int order, const T *x, T *out, const T *v, const T *r,
+                              const double *lam, hipStream_t s) {
+    constexpr int VX = Vec16<T>::N;
+    const int n = p.n;
+    const long N = (long)n * n * n;
+    if (zc <= 0 || zc > n) zc = n;
+    const int nzc = (n + zc - 1) / zc;
+    const unsigned grid = st_grid((n + 16 * VX - 1) / (16 * VX), (n + 15) / 16, nzc, (p.nb + NB - 1) / NB);
+    hipLaunchKernelGGL((k_stencil_vb<T, MODE, NB>), dim3(grid), dim3(16, 16, 1), 0, s, x, out, v, r, lam, n, N, zc,
+                       p.nb, order, (T)(1.0 / (p.h * p.h)), (T)p.scale, (T)p.omega, (T)p.diag_t);
+    LIRA_HIP(hipGetLastError());
+}
+
 bool stencil_variant_usable(int variant, int n, int dtype_size) {
     if (variant < 0 || variant >= stencil_variants()) return false;
     if (!ST_VARIANTS[variant].vec) return true;
@@ -553,6 +783,20 @@ void stencil(int mode, const StencilParams &p, const T *x, T *out, const T *v, c
         if (misaligned(x) || misaligned(out) || misaligned(v) || misaligned(r)) variant = pick_variant(p.n, p.nb);
     }
     const StVariant &sv = ST_VARIANTS[variant];
+    if (sv.vec == 2) {
+#define LIRA_VB(MODE_, NB_) launch_stencil_vb<T, MODE_, NB_>(p, sv.zc, sv.order, x, out, v, r, lam, s)
+        if (sv.rpt == 2) {
+            if (mode == ST_LAP) LIRA_VB(ST_LAP, 2);
+            else if (mode == ST_HAM) LIRA_VB(ST_HAM, 2);
+            else LIRA_VB(ST_JAC, 2);
+        } else {
+            if (mode == ST_LAP) LIRA_VB(ST_LAP, 4);
+            else if (mode == ST_HAM) LIRA_VB(ST_HAM, 4);
+            else LIRA_VB(ST_JAC, 4);
+        }
+#undef LIRA_VB
+        return;
+    }
     if (sv.vec) {
 #define LIRA_SV(MODE_, RPT_) launch_stencil_vec<T, MODE_, RPT_>(p, sv.zc, sv.order, x, out, v, r, lam, s)
         if (sv.rpt == 1) {

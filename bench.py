@@ -63,7 +63,72 @@ def hx_roofline(torch, lira, n=192, nb=64, reps=20, dtype="float32"):
     del x, v, out
     torch.cuda.empty_cache()
     return {"ms": ms, "bytes": algo_bytes, "gbs": achieved, "variant": lira.stencil_variants()[tuned],
-            "frac": achieved / HBM_PEAK_GBS}
+            "variant_index": tuned, "frac": achieved / HBM_PEAK_GBS}
+
+
+def traffic_probe(esize, variant, n=192, nb=64):
+    """Child of hbm_traffic, run under rocprofv3 --pmc: two copies of a known size (the counter calibration), then
+    two H.X launches of the given stencil variant on the roofline problem."""
+    import torch
+    torch.cuda.set_device(0)
+    import lira
+    dt = torch.float32 if esize == 4 else torch.float64
+    g = torch.Generator(device="cuda").manual_seed(0)
+    x = torch.randn((nb, n ** 3), generator=g, device="cuda", dtype=torch.float32).to(dt)
+    v = (torch.rand(n ** 3, generator=g, device="cuda", dtype=torch.float32) * -5.0).to(dt)
+    out = torch.empty_like(x)
+    sp = torch.cuda.current_stream().cuda_stream
+    for _ in range(2):
+        out.copy_(x)
+    torch.cuda.synchronize()
+    lira.lib().lira_set_stencil_variant(esize, variant)
+    for _ in range(2):
+        lira.hamiltonian(n, 0.2, v, x, out, nb, stream=sp)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": x.numel() * esize}), flush=True)
+
+
+def hbm_traffic(esize, variant, n=192, nb=64):
+    """HBM bytes per H.X launch from two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each
+    scaled by what the same counter reads for a device copy of known size in the same process. None if rocprofv3 is
+    absent or a pass fails."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None
+    copy_bytes = x_bytes = n ** 3 * nb * esize
+    per = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="lira_pmc_")
+        cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "pmc", "--", sys.executable,
+               os.path.abspath(__file__), "--traffic-probe", str(esize), str(variant)]
+        try:
+            subprocess.run(cmd, timeout=90, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, check=False)
+        except subprocess.TimeoutExpired:
+            shutil.rmtree(d, ignore_errors=True)
+            return None
+        rows = []
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(f, newline="") as fh:
+                rows += [r for r in csv.DictReader(fh) if r.get("Counter_Name") == counter]
+        shutil.rmtree(d, ignore_errors=True)
+        copy = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
+        sten = [float(r["Counter_Value"]) for r in rows if "lira::k_stencil" in r["Kernel_Name"]]
+        if not copy or not sten:
+            return None
+        kib_copy = sum(copy) / len(copy)
+        kib_sten = sum(sten) / len(sten)
+        # a copy reads and writes copy_bytes: the counter's scale is copy_bytes / (KiB counted * 1024)
+        factor = copy_bytes / (kib_copy * 1024.0)
+        per[counter] = {"kib_per_dispatch": kib_sten, "copy_kib": kib_copy, "factor": factor,
+                        "bytes": kib_sten * 1024.0 * factor}
+    del x_bytes
+    return {"bytes": per["FETCH_SIZE"]["bytes"] + per["WRITE_SIZE"]["bytes"], "fetch": per["FETCH_SIZE"],
+            "write": per["WRITE_SIZE"]}
 
 
 def cpu_baseline_sample(state, gpu_fp64, n, h, pos, z, threads):
@@ -106,7 +171,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc passes")
+    ap.add_argument("--traffic-probe", nargs=2, type=int, metavar=("ESIZE", "VARIANT"), help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.traffic_probe:
+        traffic_probe(*args.traffic_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the SCF on a single MI355X (asked for --gpus {args.gpus})",
               file=sys.stderr)
@@ -171,10 +241,12 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_roofline:
         r = hx_roofline(torch, lira)
+        traffic = None if args.no_traffic else hbm_traffic(4, r["variant_index"])
         result["roofline"] = {"bound": "hbm", "achieved": r["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                              "frac": r["frac"], "traffic": None,
-                              "kernel": "k_stencil<float, ST_HAM> on 192^3 x 64 bands",
-                              "ms_per_launch": r["ms"], "algorithmic_bytes": r["bytes"], "variant": r["variant"]}
+                              "frac": r["frac"], "traffic": None if traffic is None else traffic["bytes"],
+                              "kernel": "lira::k_stencil* (ST_HAM, f32) H.X on 192^3 x 64 bands",
+                              "ms_per_launch": r["ms"], "algorithmic_bytes": r["bytes"], "variant": r["variant"],
+                              "traffic_detail": traffic}
         r64 = hx_roofline(torch, lira, dtype="float64")
         result["roofline_fp64"] = {"achieved": r64["gbs"], "frac": r64["frac"], "ms_per_launch": r64["ms"]}
         print(f"# H.X 192^3 x 64 f32: {r['ms']:.3f} ms, {r['gbs']:.0f} GB/s ({100 * r['frac']:.1f} % of peak); "

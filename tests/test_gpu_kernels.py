@@ -89,25 +89,42 @@ def test_laplacian_dirichlet_closed_form_and_kinetic_scale(dev):
 
 
 @pytest.mark.parametrize("dtype_name", ["float32", "float64"])
-def test_every_stencil_variant_is_bitwise_identical(dev, dtype_name):
+@pytest.mark.parametrize("n,nb", [(41, 3), (44, 5), (64, 4)])
+def test_every_stencil_variant_is_bitwise_identical(dev, dtype_name, n, nb):
+    # n = 41 takes the scalar tiles only; n = 44 and 64 are multiples of the 16-byte vector width, so the vector-lane
+    # and multi-band variants run too (44: ragged x/y tiles; nb = 5: a partial band group). Every mode (H.X, scaled
+    # Laplacian, damped Jacobi) must give the same bits whichever variant the autotuner picks.
     import lira
     torch = dev
     dt = getattr(torch, dtype_name)
-    n, h, nb = 41, 0.2, 3
+    h = 0.2
     rng = np.random.default_rng(7)
     xd = _t(torch, rng.standard_normal((nb, n ** 3)), dt)
+    rd = _t(torch, rng.standard_normal((nb, n ** 3)), dt)
     vd = _t(torch, rng.uniform(-5, 0, n ** 3), dt)
-    outs = []
+    lam = _t(torch, rng.uniform(-1, 0, nb), torch.float64)
     size = 4 if dt == torch.float32 else 8
-    for var in range(len(lira.stencil_variants())):
-        lira.lib().lira_set_stencil_variant(size, var)
-        o = torch.empty_like(xd)
-        lira.hamiltonian(n, h, vd, xd, o, nb)
-        torch.cuda.synchronize()
-        outs.append(o.cpu().numpy())
-    lira.lib().lira_set_stencil_variant(size, -1)
-    for o in outs[1:]:
-        assert np.array_equal(o, outs[0])
+    outs = []
+    try:
+        for var in range(len(lira.stencil_variants())):
+            lira.lib().lira_set_stencil_variant(size, var)
+            o = [torch.empty_like(xd) for _ in range(3)]
+            lira.hamiltonian(n, h, vd, xd, o[0], nb)
+            lira.laplacian(n, h, xd, o[1], nb, scale=-0.5)
+            lira.jacobi_sweep(n, h, vd, xd, rd, lam, o[2], nb)
+            torch.cuda.synchronize()
+            outs.append([t.cpu().numpy() for t in o])
+    finally:
+        lira.lib().lira_set_stencil_variant(size, -1)
+    for var, o in enumerate(outs[1:], 1):
+        for mode in range(3):
+            assert np.array_equal(o[mode], outs[0][mode]), (lira.stencil_variants()[var], mode)
+    if dt == torch.float64:
+        x = xd.cpu().numpy()
+        v = vd.cpu().numpy()
+        for b in range(nb):
+            want = O.hamiltonian(n, h, v, x[b])
+            assert np.abs(outs[0][0][b] - want).max() < 1e-12 * np.abs(want).max()
 
 
 def test_jacobi_sweep_matches_oracle_step(dev):
