@@ -36,8 +36,9 @@ def water():
     return pos, [8, 1, 1], 5, 5
 
 
-def hx_roofline(torch, lira, n=192, nb=64, reps=20, dtype="float32"):
-    """Times lira_hamiltonian on n^3 x nb bands with HIP events on torch's current stream."""
+def hx_roofline(torch, lira, n=192, nb=64, reps=20, dtype="float32", variant=None):
+    """Times lira_hamiltonian on n^3 x nb bands with HIP events on torch's current stream (the tile variant the
+    autotuner picks, or the given one)."""
     dt = getattr(torch, dtype)
     g = torch.Generator(device="cuda").manual_seed(0)
     x = torch.randn((nb, n ** 3), generator=g, device="cuda", dtype=torch.float32).to(dt)
@@ -46,7 +47,11 @@ def hx_roofline(torch, lira, n=192, nb=64, reps=20, dtype="float32"):
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
     esize = x.element_size()
-    tuned, tuned_ms = lira.autotune_stencil(esize, n, nb)
+    if variant is None:
+        tuned, _ = lira.autotune_stencil(esize, n, nb)
+    else:
+        tuned = variant
+        lira.lib().lira_set_stencil_variant(esize, variant)
     for _ in range(3):
         lira.hamiltonian(n, 0.2, v, x, out, nb, stream=sp)
     torch.cuda.synchronize()
@@ -172,6 +177,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc passes")
+    ap.add_argument("--roofline-only", action="store_true", help="only the H.X roofline measurement (for profiling)")
+    ap.add_argument("--variant", type=int, default=None, help="stencil tile variant of the roofline (default: tuned)")
     ap.add_argument("--traffic-probe", nargs=2, type=int, metavar=("ESIZE", "VARIANT"), help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.traffic_probe:
@@ -186,6 +193,11 @@ def main():
     torch.cuda.set_device(0)
     import lira
 
+    if args.roofline_only:
+        r = hx_roofline(torch, lira, variant=args.variant)
+        print(json.dumps({"roofline_only": True, "ms_per_launch": r["ms"], "achieved_gbs": r["gbs"],
+                          "variant": r["variant"], "variant_index": r["variant_index"]}))
+        return
     n, h = 95, 0.15
     pos, z, nu, nd = water()
     settings = lira.default_settings(h, lira.FERMI_DIRAC)
@@ -240,7 +252,7 @@ def main():
     del solver
     torch.cuda.empty_cache()
     if not args.no_roofline:
-        r = hx_roofline(torch, lira)
+        r = hx_roofline(torch, lira, variant=args.variant)
         traffic = None if args.no_traffic else hbm_traffic(4, r["variant_index"])
         result["roofline"] = {"bound": "hbm", "achieved": r["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": r["frac"], "traffic": None if traffic is None else traffic["bytes"],

@@ -130,8 +130,52 @@ lira_status lira_solve(lira_solver *s, const lira_system *sys, const lira_settin
  * out[3 * n_nuclei]. sigma must be the solve's. */
 lira_status lira_forces(lira_solver *s, const lira_nucleus *nuclei, int n_nuclei, double sigma, double *out);
 
+/* ---- pseudised ions (qm::ionic::IonicSetup, crates/qm/src/ionic.rs:60-70; its tables qm::radial::filter::FilteredTable
+ * and FilteredProjector, crates/qm/src/radial/filter.rs:320-338). Tables are uniform in d with spacing du from d = 0,
+ * read by 4-point Lagrange with index -1 mirrored (filter.rs:302-318) and 0 beyond their radius. */
+typedef struct {
+    int l;             /* 0..2 */
+    double e_kb;       /* Kleinman-Bylander energy, Ha */
+    double radius;     /* t_u is 0 beyond (Bohr) */
+    const double *t_u; /* t_l = F_l / d^l */
+    int n_t;
+} lira_projector;
+
+typedef struct {
+    int z;                /* the species' atomic number */
+    double z_v;           /* valence charge */
+    double sigma_lr;      /* width of the long-range erf (Bohr) */
+    double du;            /* table spacing (Bohr) */
+    const double *v_sr_u; /* short-range local table */
+    int n_v_sr;
+    double r_sr;
+    const double *rho_pc_u; /* partial core (NULL: none) */
+    int n_rho_pc;
+    double pc_radius;
+    const lira_projector *projectors; /* at most 3, one per non-local channel */
+    int n_projectors;
+} lira_species;
+
+/* The ionic setup of the next solves (System::ionic, crates/qm/src/scf.rs:329): species listed here are pseudised,
+ * every other Z is all-electron at settings.sigma (Ionic::AllElectron). n_species = 0 clears it. The tables are
+ * copied. A warm state converged under another setup is not reused (scf.rs:771-777). The caller passes the valence
+ * electron counts in lira_system (System::with_ionic, scf.rs:362-365). */
+lira_status lira_set_ionic(lira_solver *s, const lira_species *species, int n_species);
+/* The five pseudised force terms of the converged state, out[(t * n_nuclei + I) * 3 + a] for t in [ion-ion, local LR,
+ * local SR, nonlocal, partial core] (GpuSolver::force_terms, crates/qm-gpu/src/forces.rs:276; qm::forces::
+ * pseudo_force_terms, crates/qm/src/forces.rs:259). lira_forces is their per-component sum. */
+lira_status lira_force_terms(lira_solver *s, const lira_nucleus *nuclei, int n_nuclei, double sigma, double *out);
+/* The ionic terms of a geometry under the current setup (qm/src/nuclear.rs:342-368): V_loc and rho_pc (n^3 each,
+ * f64, either may be NULL); they also become the solver's operator for lira_nonlocal_apply. */
+lira_status lira_ionic_fields(lira_solver *s, const lira_nucleus *nuclei, int n_nuclei, double sigma, double *v_loc,
+                              double *rho_pc);
+/* V_NL x for nb host vectors (band-major n^3, f64) and their projections c (rows x nb, rows = *n_rows) under the
+ * geometry of the last lira_ionic_fields or solve (NonlocalOp::apply_add / project, crates/qm/src/nonlocal.rs:133-145). */
+lira_status lira_nonlocal_apply(lira_solver *s, const double *x, int nb, double *out, double *c, int c_cap,
+                                int *n_rows);
+
 /* state getters (GpuSolver::state / ks_state / density_f32, crates/qm-gpu/src/scf.rs:1172, 1763-1851) as f64 in the
- * qm layout. which: 0 = rho_up, 1 = rho_dn, 2 = rho, 3 = V_H, 4 = V_nuc. */
+ * qm layout. which: 0 = rho_up, 1 = rho_dn, 2 = rho, 3 = V_H, 4 = V_nuc (V_loc under an ionic setup), 5 = rho_pc. */
 lira_status lira_get_field(lira_solver *s, int which, double *out);
 /* eigenvalues / occupations of a spin channel (0 up, 1 down); *count receives the stored count */
 lira_status lira_get_levels(lira_solver *s, int spin, double *evals, double *occ, int cap, int *count);
