@@ -2,6 +2,7 @@
 // linear combinations, density accumulation, LSDA, nuclear potential, orbital guess, DIIS mix and HF forces.
 // All kernels are written for 64-wide wavefronts; reductions are deterministic (fixed partial counts, fixed order).
 #include "lira_internal.hpp"
+#include "lsda.hpp"
 
 #include <cmath>
 #include <cstdio>
@@ -937,48 +938,69 @@ __device__ inline const void *col_base(const ColPtrs &c, int col, long N, int es
 typedef double lira_d4 __attribute__((ext_vector_type(4)));
 constexpr int G2_TI = 32, G2_TJ = 64, G2_P = 32;
 
+// Loads: the 96 columns x 32 points of a tile are 12 elements per thread, consecutive threads on consecutive points
+// of one column (coalesced). They are issued for tile t + 1 into registers before the MFMAs of tile t run out of LDS,
+// so their latency hides behind the matrix work; 16 x 16 output sub-tiles wholly outside m x q are skipped.
+constexpr int G2_LOADS = (G2_TI + G2_TJ) * G2_P / 256;
+
+template <typename T>
+__device__ inline void gram_fetch(const T *const *cols, long p0, long p_end, double (&v)[G2_LOADS]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int u = 0; u < G2_LOADS; u++) {
+        const int e = t + 256 * u;
+        const int col = e / G2_P, pp = e - col * G2_P;
+        const long p = p0 + pp;
+        const T *c = cols[col];
+        v[u] = (c && p < p_end) ? (double)c[p] : 0.0;
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q, long N, long chunk,
                                               double *partials) {
-    __shared__ double sa[G2_TI][G2_P + 1];
-    __shared__ double sb[G2_TJ][G2_P + 1];
-    __shared__ const T *scol[G2_TI];
-    __shared__ const T *ucol[G2_TJ];
+    __shared__ double sab[G2_TI + G2_TJ][G2_P + 1];  // rows 0..31: S columns, 32..95: U columns
+    __shared__ const T *cols[G2_TI + G2_TJ];
     const int nbj = (q + G2_TJ - 1) / G2_TJ;
     const int bi = blockIdx.y / nbj, bj = blockIdx.y - bi * nbj;
     const int i0 = bi * G2_TI, j0 = bj * G2_TJ;
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    if (t < G2_TI) scol[t] = i0 + t < m ? static_cast<const T *>(col_base(S, i0 + t, N, sizeof(T))) : nullptr;
+    if (t < G2_TI) cols[t] = i0 + t < m ? static_cast<const T *>(col_base(S, i0 + t, N, sizeof(T))) : nullptr;
     if (t >= 64 && t < 64 + G2_TJ) {
         const int c = t - 64;
-        ucol[c] = j0 + c < q ? static_cast<const T *>(col_base(U, j0 + c, N, sizeof(T))) : nullptr;
+        cols[G2_TI + c] = j0 + c < q ? static_cast<const T *>(col_base(U, j0 + c, N, sizeof(T))) : nullptr;
     }
     const int isub = wave & 1, jsub0 = (wave >> 1) * 2;
+    // wave-uniform: does this wave's pair of 16 x 16 sub-tiles hold any output?
+    const bool live_i = i0 + isub * 16 < m;
+    const bool live0 = live_i && j0 + jsub0 * 16 < q, live1 = live_i && j0 + (jsub0 + 1) * 16 < q;
     lira_d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
     const long p_begin = (long)blockIdx.x * chunk;
     const long p_end = min(N, p_begin + chunk);
     __syncthreads();
+    double v[G2_LOADS];
+    if (p_begin < p_end) gram_fetch<T>(cols, p_begin, p_end, v);
+    const int r16 = lane & 15, k4 = lane >> 4;
     for (long p0 = p_begin; p0 < p_end; p0 += G2_P) {
-        for (int e = t; e < (G2_TI + G2_TJ) * G2_P; e += 256) {
+#pragma unroll
+        for (int u = 0; u < G2_LOADS; u++) {
+            const int e = t + 256 * u;
             const int col = e / G2_P, pp = e - col * G2_P;
-            const long p = p0 + pp;
-            if (col < G2_TI) {
-                const T *c = scol[col];
-                sa[col][pp] = (c && p < p_end) ? (double)c[p] : 0.0;
-            } else {
-                const T *c = ucol[col - G2_TI];
-                sb[col - G2_TI][pp] = (c && p < p_end) ? (double)c[p] : 0.0;
-            }
+            sab[col][pp] = v[u];
         }
         __syncthreads();
-        const int r16 = lane & 15, k4 = lane >> 4;
+        if (p0 + G2_P < p_end) gram_fetch<T>(cols, p0 + G2_P, p_end, v);
+        if (live0) {
 #pragma unroll
-        for (int kk = 0; kk < G2_P; kk += 4) {
-            const double a = sa[isub * 16 + r16][kk + k4];
-            const double b0 = sb[jsub0 * 16 + r16][kk + k4];
-            const double b1 = sb[(jsub0 + 1) * 16 + r16][kk + k4];
-            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc1, 0, 0, 0);
+            for (int kk = 0; kk < G2_P; kk += 4) {
+                const double a = sab[isub * 16 + r16][kk + k4];
+                const double b0 = sab[G2_TI + jsub0 * 16 + r16][kk + k4];
+                acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc0, 0, 0, 0);
+                if (live1) {
+                    const double b1 = sab[G2_TI + (jsub0 + 1) * 16 + r16][kk + k4];
+                    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc1, 0, 0, 0);
+                }
+            }
         }
         __syncthreads();
     }
@@ -995,28 +1017,23 @@ __global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q
     }
 }
 
-// sums the per-chunk partials of every output in a fixed order: 8 chunk groups x 32 outputs per workgroup
+// sums the per-chunk partials of every output in a fixed order: 4 outputs per workgroup, 64 lanes per output each
+// summing a strided subset of the chunks, then a fixed shuffle tree
 __global__ __launch_bounds__(256) void k_gram_finish(const double *partials, int chunks, long mq, double scale,
                                                       double *G) {
-    __shared__ double sh[8][33];
-    const int o = threadIdx.x & 31, g = threadIdx.x >> 5;
-    const long out = blockIdx.x * 32L + o;
+    const int lane = threadIdx.x & 63;
+    const long out = blockIdx.x * 4L + (threadIdx.x >> 6);
     double acc = 0.0;
     if (out < mq)
-        for (int c = g; c < chunks; c += 8) acc += partials[(long)c * mq + out];
-    sh[g][o] = acc;
-    __syncthreads();
-    if (g == 0 && out < mq) {
-        double s = 0.0;
+        for (int c = lane; c < chunks; c += 64) acc += partials[(long)c * mq + out];
 #pragma unroll
-        for (int k = 0; k < 8; k++) s += sh[k][o];
-        G[out] = s * scale;
-    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0 && out < mq) G[out] = acc * scale;
 }
 
 static int gram_chunks(int m, int q, long N) {
     const int blocks = ((m + G2_TI - 1) / G2_TI) * ((q + G2_TJ - 1) / G2_TJ);
-    int chunks = (1024 + blocks - 1) / blocks;
+    int chunks = (512 + blocks - 1) / blocks;
     const long max_chunks = (N + G2_P - 1) / G2_P;
     if (chunks > max_chunks) chunks = (int)max_chunks;
     if (chunks < 1) chunks = 1;
@@ -1052,37 +1069,49 @@ void gram(const Cols &S, const Cols &U, long N, double scale, double *partials, 
                        partials);
     LIRA_HIP(hipGetLastError());
     const long mq = (long)m * q;
-    hipLaunchKernelGGL(k_gram_finish, dim3((unsigned)((mq + 31) / 32)), dim3(256), 0, s, partials, chunks, mq, scale,
-                       G);
+    hipLaunchKernelGGL(k_gram_finish, dim3((unsigned)((mq + 3) / 4)), dim3(256), 0, s, partials, chunks, mq, scale, G);
     LIRA_HIP(hipGetLastError());
 }
 
 // ------------------------------------------------------------------------------------------ lincomb ----
 // Y_j = sum_i S_i C_ij: one thread per point holds QB outputs in f64 registers; C (m <= 64 rows) and the column
 // pointers are staged in LDS once per workgroup; grid.y covers q in groups of QB.
+constexpr int LC_MAX_M = 192;  // input columns of one lincomb (3 x the largest block LOBPCG forms)
+
 template <typename T, int QB>
 __global__ __launch_bounds__(256) void k_lincomb(ColPtrs S, int m, const double *__restrict__ C, int q,
                                                  T *__restrict__ Y, int qsplit, T *__restrict__ Y2, long N) {
-    __shared__ double cs[64 * QB];
-    __shared__ const T *cols[64];
+    extern __shared__ double cs[];  // mp x QB coefficients (dynamic: m is a launch parameter)
+    __shared__ const T *cols[LC_MAX_M];
     const int j0 = blockIdx.y * QB;
     const int qn = min(QB, q - j0);
-    for (int e = threadIdx.x; e < m * QB; e += 256) {
+    const int mp = (m + 7) & ~7;  // rows m .. mp - 1 are zero: the input loop runs in unguarded groups of 8
+    for (int e = threadIdx.x; e < mp * QB; e += 256) {
         const int i = e / QB, c = e - i * QB;
-        cs[e] = c < qn ? C[(long)i * q + j0 + c] : 0.0;
+        cs[e] = (i < m && c < qn) ? C[(long)i * q + j0 + c] : 0.0;
     }
-    for (int i = threadIdx.x; i < m; i += 256) cols[i] = static_cast<const T *>(col_base(S, i, N, sizeof(T)));
+    for (int i = threadIdx.x; i < mp; i += 256)
+        cols[i] = i < m ? static_cast<const T *>(col_base(S, i, N, sizeof(T))) : nullptr;
     __syncthreads();
     const long p = blockIdx.x * 256L + threadIdx.x;
     if (p >= N) return;
     double acc[QB];
 #pragma unroll
     for (int c = 0; c < QB; c++) acc[c] = 0.0;
-    for (int i = 0; i < m; i++) {
-        const double sv = (double)cols[i][p];
-        const double *row = cs + i * QB;
+    for (int i0 = 0; i0 < mp; i0 += 8) {
+        // the 8 loads of a group are independent and issued together
+        double sv[8];
 #pragma unroll
-        for (int c = 0; c < QB; c++) acc[c] += sv * row[c];
+        for (int u = 0; u < 8; u++) {
+            const T *c = cols[i0 + u];
+            sv[u] = c ? (double)c[p] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const double *row = cs + (i0 + u) * QB;
+#pragma unroll
+            for (int c = 0; c < QB; c++) acc[c] += sv[u] * row[c];
+        }
     }
 #pragma unroll
     for (int c = 0; c < QB; c++)
@@ -1097,16 +1126,26 @@ template <typename T>
 void lincomb(const Cols &S, const double *C, int q, T *Y, long N, hipStream_t s, int qsplit, T *Y2) {
     const int m = S.total();
     if (q <= 0) return;
-    if (m > 64) throw InvalidError("lincomb: more than 64 input columns");
+    if (m > LC_MAX_M) throw InvalidError("lincomb: more than 192 input columns");
     if (qsplit < 0 || qsplit > q) qsplit = q;
     const unsigned gx = (unsigned)((N + 255) / 256);
-    if (q <= 8) {
-        hipLaunchKernelGGL((k_lincomb<T, 8>), dim3(gx, 1), dim3(256), 0, s, to_ptrs(S), m, C, q, Y, qsplit, Y2, N);
-    } else if (q <= 16) {
-        hipLaunchKernelGGL((k_lincomb<T, 16>), dim3(gx, 1), dim3(256), 0, s, to_ptrs(S), m, C, q, Y, qsplit, Y2, N);
+    // the output group QB: the smallest of 8, 16, 24, 32 that splits q into equal-count launches
+    const int groups = (q + 31) / 32;
+    const int per = (q + groups - 1) / groups;
+    const ColPtrs sp = to_ptrs(S);
+    const size_t mp = (size_t)((m + 7) & ~7);
+    if (per <= 8) {
+        hipLaunchKernelGGL((k_lincomb<T, 8>), dim3(gx, (q + 7) / 8), dim3(256), mp * 8 * sizeof(double), s, sp, m, C,
+                           q, Y, qsplit, Y2, N);
+    } else if (per <= 16) {
+        hipLaunchKernelGGL((k_lincomb<T, 16>), dim3(gx, (q + 15) / 16), dim3(256), mp * 16 * sizeof(double), s, sp, m,
+                           C, q, Y, qsplit, Y2, N);
+    } else if (per <= 24) {
+        hipLaunchKernelGGL((k_lincomb<T, 24>), dim3(gx, (q + 23) / 24), dim3(256), mp * 24 * sizeof(double), s, sp, m,
+                           C, q, Y, qsplit, Y2, N);
     } else {
-        hipLaunchKernelGGL((k_lincomb<T, 32>), dim3(gx, (q + 31) / 32), dim3(256), 0, s, to_ptrs(S), m, C, q, Y,
-                           qsplit, Y2, N);
+        hipLaunchKernelGGL((k_lincomb<T, 32>), dim3(gx, (q + 31) / 32), dim3(256), mp * 32 * sizeof(double), s, sp, m,
+                           C, q, Y, qsplit, Y2, N);
     }
     LIRA_HIP(hipGetLastError());
 }
@@ -1219,65 +1258,20 @@ template <typename Td, typename Ts> void convert(Td *out, const Ts *a, long N, h
 
 // ---------------------------------------------------------------------------------------------- LSDA ----
 // Dirac exchange + PW92 correlation (qm/src/xc.rs:78-187), evaluated in f64 at every point.
-__device__ inline void pw92_g(double rs, double a, double a1, double b1, double b2, double b3, double b4, double &g,
-                              double &dg) {
-    const double srs = sqrt(rs);
-    const double q0 = -2.0 * a * (1.0 + a1 * rs);
-    const double q1 = 2.0 * a * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs);
-    const double q1p = a * (b1 / srs + 2.0 * b2 + 3.0 * b3 * srs + 4.0 * b4 * rs);
-    const double ln = log(1.0 + 1.0 / q1);
-    g = q0 * ln;
-    dg = -2.0 * a * a1 * ln - q0 * q1p / (q1 * q1 + q1);
-}
-
-__device__ inline void exc_point(double nu, double nd, double &e, double &vu, double &vd) {
-    nu = nu > 0.0 ? nu : 0.0;
-    nd = nd > 0.0 ? nd : 0.0;
-    const double n = nu + nd;
-    if (n < 1e-20) {
-        e = vu = vd = 0.0;
-        return;
-    }
-    const double third = 1.0 / 3.0;
-    const double cx = 0.75 * cbrt(3.0 / M_PI) * cbrt(2.0);
-    const double cnu = cbrt(nu), cnd = cbrt(nd);
-    const double ex = -cx * (nu * cnu + nd * cnd);
-    const double vxu = -cbrt(6.0 / M_PI) * cnu, vxd = -cbrt(6.0 / M_PI) * cnd;
-    // correlation
-    const double rs = cbrt(3.0 / (4.0 * M_PI * n));
-    double z = (nu - nd) / n;
-    z = z < -1.0 ? -1.0 : (z > 1.0 ? 1.0 : z);
-    double ec0, dec0, ec1, dec1, mac, dmac;
-    pw92_g(rs, 0.031091, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294, ec0, dec0);
-    pw92_g(rs, 0.015545, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517, ec1, dec1);
-    pw92_g(rs, 0.016887, 0.11125, 10.357, 3.6231, 0.88026, 0.49671, mac, dmac);
-    const double ac = -mac, dac = -dmac;
-    const double denom = cbrt(16.0) - 2.0;  // 2^{4/3} - 2
-    const double opz = cbrt(1.0 + z), omz = cbrt(1.0 - z);
-    const double f = ((1.0 + z) * opz + (1.0 - z) * omz - 2.0) / denom;
-    const double fp = (4.0 * third) * (opz - omz) / denom;
-    const double FPP0 = 1.709921;
-    const double z3 = z * z * z, z4 = z3 * z;
-    const double ec = ec0 + ac * f / FPP0 * (1.0 - z4) + (ec1 - ec0) * f * z4;
-    const double decdrs = dec0 + dac * f / FPP0 * (1.0 - z4) + (dec1 - dec0) * f * z4;
-    const double decdz = ac / FPP0 * (fp * (1.0 - z4) - 4.0 * z3 * f) + (ec1 - ec0) * (fp * z4 + 4.0 * z3 * f);
-    const double common = ec - rs * third * decdrs;
-    e = ex + n * ec;
-    vu = vxu + common - (z - 1.0) * decdz;
-    vd = vxd + common - (z + 1.0) * decdz;
-}
-
-template <typename T> __global__ void k_xc(const T *nu, const T *nd, T *e, T *vu, T *vd, long N) {
+template <typename T> __global__ void k_xc(const T *nu, const T *nd, T *e, T *vu, T *vd, long N, const T *pc) {
     const long i = blockIdx.x * 256L + threadIdx.x;
     if (i >= N) return;
     double ee, a, b;
-    exc_point((double)nu[i], (double)nd[i], ee, a, b);
+    // with a partial core the LSDA sees n_sigma = rho_sigma + rho_pc / 2 (qm/src/scf.rs:757-769)
+    const double c = pc ? 0.5 * (double)pc[i] : 0.0;
+    if (pc) exc_point((double)nu[i] + c, (double)nd[i] + c, ee, a, b);
+    else exc_point((double)nu[i], (double)nd[i], ee, a, b);
     e[i] = (T)ee;
     vu[i] = (T)a;
     vd[i] = (T)b;
 }
-template <typename T> void xc_lsda(const T *nu, const T *nd, T *e, T *vu, T *vd, long N, hipStream_t s) {
-    hipLaunchKernelGGL(k_xc<T>, LIRA_GRID1(N), 0, s, nu, nd, e, vu, vd, N);
+template <typename T> void xc_lsda(const T *nu, const T *nd, T *e, T *vu, T *vd, long N, hipStream_t s, const T *pc) {
+    hipLaunchKernelGGL(k_xc<T>, LIRA_GRID1(N), 0, s, nu, nd, e, vu, vd, N, pc);
     LIRA_HIP(hipGetLastError());
 }
 
@@ -1476,7 +1470,7 @@ void hf_forces(const T *rho, int n, double h, const double *pos, const double *z
     template void add2<T>(T *, const T *, const T *, long, hipStream_t);                                             \
     template void sub2<T>(T *, const T *, const T *, long, hipStream_t);                                             \
     template void copy<T>(T *, const T *, long, hipStream_t);                                                        \
-    template void xc_lsda<T>(const T *, const T *, T *, T *, T *, long, hipStream_t);                                \
+    template void xc_lsda<T>(const T *, const T *, T *, T *, T *, long, hipStream_t, const T *);                                \
     template void nuclear_potential<T>(T *, int, double, const double *, const double *, int, double, hipStream_t); \
     template void deposit_charge<T>(T *, int, double, const double *, const double *, int, double, double,          \
                                     hipStream_t);                                                                     \

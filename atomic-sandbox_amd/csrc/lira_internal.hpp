@@ -122,9 +122,9 @@ template <typename T> void add2(T *out, const T *a, const T *b, long N, hipStrea
 template <typename T> void sub2(T *out, const T *a, const T *b, long N, hipStream_t s);
 template <typename T> void copy(T *out, const T *a, long N, hipStream_t s);
 template <typename Td, typename Ts> void convert(Td *out, const Ts *a, long N, hipStream_t s);
-// LSDA (Dirac + PW92), math in f64 (qm/src/xc.rs:120-210)
+// LSDA (Dirac + PW92), math in f64 (qm/src/xc.rs:120-210); with a partial core pc (nullable) on n_sigma = rho_sigma + pc/2
 template <typename T>
-void xc_lsda(const T *nu, const T *nd, T *e, T *vu, T *vd, long N, hipStream_t s);
+void xc_lsda(const T *nu, const T *nd, T *e, T *vu, T *vd, long N, hipStream_t s, const T *pc = nullptr);
 // V = -sum_I Z_I erf(d/(sigma sqrt2))/d (qm/src/nuclear.rs:258-273); pos/z on the device
 template <typename T>
 void nuclear_potential(T *out, int n, double h, const double *pos, const double *zf, int nat, double sigma,
@@ -144,6 +144,49 @@ template <typename T>
 void hf_forces(const T *rho, int n, double h, const double *pos, const double *zf, int nat, double sigma,
                double *partials, double *out, hipStream_t s);
 
+// ------------------------------------------------------------ pseudised ions ----
+// One nucleus as the local potential sees it (qm/src/nuclear.rs:304-311 Centre + the tables of a pseudised ion):
+// the smeared charge q at width sigma (Z, sigma_AE all-electron; Z_v, sigma_LR pseudised), the V_SR and partial-core
+// tables (offsets into a f64 table arena, length 0 = none) and their radii.
+struct IonicCentre {
+    double pos[3];
+    double q, sigma, du;
+    int sr_off, sr_len;
+    double r_sr;
+    int pc_off, pc_len;
+    double pc_r;
+};
+constexpr int NL_MAX_PROJ = 3;  // one projector per non-local channel, l <= 2 (qm/src/nonlocal.rs:100)
+constexpr int NL_MAX_ROWS = 9;  // sum of 2l+1 over the projectors of one ion
+// One nucleus with projectors: its box (largest projector radius, clipped), beta rows and tables.
+struct NlIon {
+    double pos[3];
+    double du;
+    int lo[3], dim[3];
+    long pts, beta_off;
+    int rows, row_off, nproj;
+    int l[NL_MAX_PROJ], t_off[NL_MAX_PROJ], t_len[NL_MAX_PROJ];
+    double radius[NL_MAX_PROJ], e_kb[NL_MAX_PROJ];
+};
+template <typename T>
+void ionic_fields(T *vloc, T *pc, int n, double h, const IonicCentre *c, int nat, const double *tab, hipStream_t s);
+template <typename T> void nl_beta(T *beta, const NlIon &ion, int n, double h, const double *tab, hipStream_t s);
+// C[(row_off + r) * nb + b] = dv sum_box beta_r x_b for every ion with projectors
+template <typename T>
+void nl_project(const T *x, int nb, long N, int n, const T *beta, const NlIon *ions, int nions, double dv, double *C,
+                hipStream_t s);
+// out_b += sum_r erow_r C_rb beta_r on the boxes of the listed ions (boxes must be disjoint)
+template <typename T>
+void nl_apply(T *out, int nb, long N, int n, const T *beta, const NlIon *ions, const int *members, int nmembers,
+              long max_pts, const double *C, const double *erow, hipStream_t s);
+// out[(term * nat + I) * 3 + a], term 0 local LR, 1 local SR, 2 partial core (qm/src/forces.rs:281-333)
+template <typename T>
+void ionic_forces(const T *rho, const T *vu, const T *vd, int n, double h, const IonicCentre *cs, const double *tab,
+                  int nat, double *partials, double *out, hipStream_t s);
+// CG[((row_off + r) * 4 + q) * nb + b]: q = 0 the projection c, q = 1..3 <grad_a beta_r | psi_b>
+template <typename T>
+void nl_force(const T *psi, int nb, long N, int n, double h, const NlIon *ions, int nions, const double *tab,
+              double *CG, hipStream_t s);
 // --------------------------------------------------------------- Poisson ----
 // Q_lm = dv * sum_p rho_p R_lm(r_p) for the 49 real solid harmonics l <= 6 (qm/src/poisson.rs:97-162)
 template <typename T> void multipole_moments(const T *rho, int n, double h, double *partials, double *q, hipStream_t s);
@@ -167,6 +210,26 @@ template <typename T> void cg_direction(T *p, const T *z, const double *sc, int 
 // x += e (f64 += T)
 template <typename T> void add_into_f64(double *x, const T *e, long N, hipStream_t s);
 
+// ------------------------------------------------------------- free atoms ----
+// The sigma-smeared spherical LSDA free-atom density of Z on the log grid r_i = e^{x0 + i dx} (cached per (Z, sigma));
+// ok = false when the atom does not solve (the caller keeps its blob).
+struct RadialProfile {
+    double x0 = 0.0, dx = 0.0;
+    std::vector<double> rho;
+    bool ok = false;
+};
+RadialProfile free_atom_profile(int z, double sigma);
+// one free atom on the device: its profile at tab + off (len values), weight w, position
+struct AtomProfile {
+    double pos[3];
+    double x0, dx, w;
+    int off, len;
+};
+// out = scale * (sum_A w_A rho_A(|r - R_A|) + sum_B q_B (2 pi)^{-3/2} e^{-|r - R_B|^2 / 2}); blobs = (x, y, z, q) rows
+template <typename T>
+void atom_density(T *out, int n, double h, const AtomProfile *atoms, int natoms, const double *tab,
+                  const double *blobs, int nblobs, double scale, hipStream_t s);
+bool free_atom_levels(int z, std::vector<double> &eps_out, double &electrons);
 // ------------------------------------------------------------- host utils ----
 void sym_eigen(int m, std::vector<double> &a, std::vector<double> &evals, std::vector<double> &evecs);
 bool lu_solve(int m, std::vector<double> a, std::vector<double> &b);

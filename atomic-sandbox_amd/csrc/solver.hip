@@ -227,6 +227,20 @@ struct SolverIface {
     virtual double effective_eig_tol(double requested) const = 0;
     virtual void reset() = 0;
     virtual int precision() const = 0;
+    virtual void set_ionic(const lira_species *sp, int nsp) = 0;
+    virtual void force_terms(const lira_nucleus *nuc, int nat, double sigma, double *out) = 0;
+    virtual void ionic_fields_host(const lira_nucleus *nuc, int nat, double sigma, double *vloc, double *pc) = 0;
+    virtual void nonlocal_apply(const double *x, int nb, double *out, double *c, int c_cap, int *rows) = 0;
+};
+
+// One pseudised species as the solver keeps it: scalars plus offsets of its tables in the f64 table arena.
+struct SpeciesHost {
+    int z;
+    double z_v, sigma_lr, du, r_sr, pc_r;
+    int sr_off, sr_len, pc_off, pc_len;
+    int nproj;
+    int l[NL_MAX_PROJ], t_off[NL_MAX_PROJ], t_len[NL_MAX_PROJ];
+    double radius[NL_MAX_PROJ], e_kb[NL_MAX_PROJ];
 };
 
 template <typename T> struct Channel {
@@ -259,6 +273,10 @@ template <typename T> class Solver final : public SolverIface {
     }
     void reset() override { valid_ = false; }
     int precision() const override { return sizeof(T) == 8 ? LIRA_FP64 : LIRA_MIXED; }
+    void set_ionic(const lira_species *sp, int nsp) override;
+    void force_terms(const lira_nucleus *nuc, int nat, double sigma, double *out) override;
+    void ionic_fields_host(const lira_nucleus *nuc, int nat, double sigma, double *vloc, double *pc) override;
+    void nonlocal_apply(const double *x, int nb, double *out, double *c, int c_cap, int *rows) override;
 
   private:
     int n_;
@@ -297,6 +315,33 @@ template <typename T> class Solver final : public SolverIface {
     double kt_ = 0.0;
     int nel_[2] = {0, 0};
     int lobpcg_total_ = 0;
+    // pseudised ions: the setup (tables copied at lira_set_ionic) and the terms of the current geometry
+    std::vector<SpeciesHost> species_;
+    std::vector<double> tab_h_;
+    DevBuf tab_, centres_, nl_ions_, beta_, nl_C_, nl_erow_, nl_members_, pc_, qf_, fpart_, fout_;
+    size_t nl_C_cap_ = 0;
+    int ionic_version_ = 0, state_ionic_version_ = -1;
+    std::vector<NlIon> nl_h_;
+    std::vector<int> nl_atom_;  // the nucleus of each NlIon
+    std::vector<double> nl_e_h_;
+    struct Colour {
+        int off, count;
+        long max_pts;
+    };
+    std::vector<Colour> colours_;
+    int nl_rows_ = 0;
+    bool has_pc_ = false;
+    std::vector<double> q_h_;  // the charge each nucleus carries (Z_v pseudised, Z all-electron)
+    DevBuf atom_tab_, atoms_dev_, blobs_dev_;
+    void cold_density(bool free_atoms, double sigma, const double *charges, double scale0, double scale1);
+    const SpeciesHost *species_of(int z) const {
+        for (const auto &sp : species_)
+            if (sp.z == z) return &sp;
+        return nullptr;
+    }
+    void ionic_geometry(double sigma);
+    void nl_apply_block(const T *x, T *out, int nb);
+    std::vector<double> nl_energy_terms(Channel<T> &c, bool fd);
 
     void ensure_host(size_t count) {
         if (count <= hcap_) return;
@@ -357,6 +402,7 @@ template <typename T> class Solver final : public SolverIface {
     void hamiltonian(const T *v, const T *x, T *out, int nb) {
         StencilParams sp{n_, nb, h_, 1.0, 0.0, 0.0};
         stencil<T>(ST_HAM, sp, x, out, v, nullptr, nullptr, stream_, variant_);
+        if (nl_rows_ > 0) nl_apply_block(x, out, nb);  // + V_NL (qm/src/nonlocal.rs:165-174)
     }
     // times every stencil tile variant on this grid with nb bands (the block size of the solve) and keeps the fastest
     void tune_stencil(int nb) {
@@ -894,6 +940,326 @@ template <typename T> void Solver<T>::upload_nuclei(const lira_nucleus *nuc, int
     }
 }
 
+// ------------------------------------------------------------------------------------ pseudised ions ----
+template <typename T> void Solver<T>::set_ionic(const lira_species *sp, int nsp) {
+    if (nsp < 0 || (nsp > 0 && !sp)) throw InvalidError("lira_set_ionic: bad species list");
+    std::vector<SpeciesHost> out;
+    std::vector<double> tab;
+    auto put = [&](const double *f, int len, const char *what) {
+        if (len < 0 || (len > 0 && !f)) throw InvalidError(std::string("lira_set_ionic: bad table ") + what);
+        for (int i = 0; i < len; i++)
+            if (!std::isfinite(f[i])) throw InvalidError(std::string("lira_set_ionic: non-finite table ") + what);
+        const int off = (int)tab.size();
+        tab.insert(tab.end(), f, f + len);
+        return off;
+    };
+    for (int i = 0; i < nsp; i++) {
+        const lira_species &s = sp[i];
+        if (s.z < 1 || s.z > 118 || !(s.z_v > 0.0) || !(s.sigma_lr > 0.0) || !(s.du > 0.0) || !(s.r_sr >= 0.0))
+            throw InvalidError("lira_set_ionic: bad species scalars");
+        for (const auto &o : out)
+            if (o.z == s.z) throw InvalidError("lira_set_ionic: a Z listed twice");
+        if (s.n_projectors < 0 || s.n_projectors > NL_MAX_PROJ || (s.n_projectors > 0 && !s.projectors))
+            throw InvalidError("lira_set_ionic: at most 3 projectors per species");
+        SpeciesHost h{};
+        h.z = s.z;
+        h.z_v = s.z_v;
+        h.sigma_lr = s.sigma_lr;
+        h.du = s.du;
+        h.r_sr = s.r_sr;
+        h.sr_len = s.n_v_sr;
+        h.sr_off = put(s.v_sr_u, s.n_v_sr, "v_sr");
+        const bool core = s.rho_pc_u && s.n_rho_pc > 0 && s.pc_radius > 0.0;
+        h.pc_r = core ? s.pc_radius : 0.0;
+        h.pc_len = core ? s.n_rho_pc : 0;
+        h.pc_off = core ? put(s.rho_pc_u, s.n_rho_pc, "rho_pc") : 0;
+        h.nproj = s.n_projectors;
+        int rows = 0;
+        for (int j = 0; j < s.n_projectors; j++) {
+            const lira_projector &pr = s.projectors[j];
+            if (pr.l < 0 || pr.l > 2 || !(pr.radius >= 0.0) || !std::isfinite(pr.e_kb))
+                throw InvalidError("lira_set_ionic: projectors need 0 <= l <= 2, radius >= 0 and a finite E_KB");
+            h.l[j] = pr.l;
+            h.e_kb[j] = pr.e_kb;
+            h.radius[j] = pr.radius;
+            h.t_len[j] = pr.n_t;
+            h.t_off[j] = put(pr.t_u, pr.n_t, "t_u");
+            rows += 2 * pr.l + 1;
+        }
+        if (rows > NL_MAX_ROWS) throw InvalidError("lira_set_ionic: more than 9 projector rows on one species");
+        out.push_back(h);
+    }
+    LIRA_HIP(hipStreamSynchronize(stream_));
+    species_ = std::move(out);
+    tab_h_ = std::move(tab);
+    if (!tab_h_.empty()) {
+        if (tab_.bytes < tab_h_.size() * sizeof(double)) tab_.alloc(tab_h_.size() * sizeof(double));
+        LIRA_HIP(hipMemcpy(tab_.ptr, tab_h_.data(), tab_h_.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    ionic_version_++;
+    nl_rows_ = 0;
+    has_pc_ = false;
+}
+
+// The ionic terms of the uploaded nuclei (qm/src/scf.rs:720-745): V_loc into vnuc_, rho_pc into pc_, the projector
+// boxes and beta rows, and the colouring of the boxes into disjoint sets for the scatter.
+template <typename T> void Solver<T>::ionic_geometry(double sigma) {
+    const int nat = nat_;
+    q_h_.assign(nat, 0.0);
+    std::vector<IonicCentre> cs(std::max(nat, 1));
+    has_pc_ = false;
+    nl_h_.clear();
+    nl_atom_.clear();
+    nl_e_h_.clear();
+    colours_.clear();
+    nl_rows_ = 0;
+    long beta_total = 0;
+    const double o = -((double)n_ - 1.0) * h_ / 2.0;
+    for (int a = 0; a < nat; a++) {
+        const SpeciesHost *sp = species_of((int)z_h_[a]);
+        IonicCentre c{};
+        for (int d = 0; d < 3; d++) c.pos[d] = pos_h_[3 * a + d];
+        c.q = sp ? sp->z_v : z_h_[a];
+        c.sigma = sp ? sp->sigma_lr : sigma;
+        c.du = sp ? sp->du : 1.0;
+        if (sp) {
+            c.sr_off = sp->sr_off;
+            c.sr_len = sp->sr_len;
+            c.r_sr = sp->r_sr;
+            c.pc_off = sp->pc_off;
+            c.pc_len = sp->pc_len;
+            c.pc_r = sp->pc_r;
+            has_pc_ = has_pc_ || sp->pc_len > 0;
+        }
+        q_h_[a] = c.q;
+        cs[a] = c;
+        if (!sp || sp->nproj == 0) continue;
+        // the box of the largest projector ball, clipped to the grid (qm/src/nuclear.rs:322-337)
+        double rmax = 0.0;
+        for (int j = 0; j < sp->nproj; j++) rmax = std::max(rmax, sp->radius[j]);
+        NlIon ion{};
+        bool empty = false;
+        for (int d = 0; d < 3; d++) {
+            double lo = std::floor((c.pos[d] - rmax - o) / h_), hi = std::ceil((c.pos[d] + rmax - o) / h_);
+            lo = std::max(lo, 0.0);
+            hi = std::min(hi, (double)(n_ - 1));
+            if (!(hi >= lo)) empty = true;
+            ion.lo[d] = empty ? 0 : (int)lo;
+            ion.dim[d] = empty ? 0 : (int)hi - (int)lo + 1;
+        }
+        if (empty) continue;
+        for (int d = 0; d < 3; d++) ion.pos[d] = c.pos[d];
+        ion.du = sp->du;
+        ion.pts = (long)ion.dim[0] * ion.dim[1] * ion.dim[2];
+        ion.beta_off = beta_total;
+        ion.nproj = sp->nproj;
+        ion.rows = 0;
+        for (int j = 0; j < sp->nproj; j++) {
+            ion.l[j] = sp->l[j];
+            ion.t_off[j] = sp->t_off[j];
+            ion.t_len[j] = sp->t_len[j];
+            ion.radius[j] = sp->radius[j];
+            ion.e_kb[j] = sp->e_kb[j];
+            for (int m = 0; m < 2 * sp->l[j] + 1; m++) nl_e_h_.push_back(sp->e_kb[j]);
+            ion.rows += 2 * sp->l[j] + 1;
+        }
+        ion.row_off = nl_rows_;
+        nl_rows_ += ion.rows;
+        beta_total += (long)ion.rows * ion.pts;
+        nl_h_.push_back(ion);
+        nl_atom_.push_back(a);
+    }
+    LIRA_HIP(hipStreamSynchronize(stream_));
+    if (centres_.bytes < cs.size() * sizeof(IonicCentre)) centres_.alloc(cs.size() * sizeof(IonicCentre));
+    LIRA_HIP(hipMemcpy(centres_.ptr, cs.data(), cs.size() * sizeof(IonicCentre), hipMemcpyHostToDevice));
+    if (qf_.bytes < (size_t)(nat + 1) * sizeof(double)) qf_.alloc((size_t)(nat + 1) * sizeof(double));
+    if (nat > 0) LIRA_HIP(hipMemcpy(qf_.ptr, q_h_.data(), nat * sizeof(double), hipMemcpyHostToDevice));
+    alloc_field(vnuc_);
+    if (has_pc_) alloc_field(pc_);
+    ionic_fields<T>(vnuc_.template as<T>(), has_pc_ ? pc_.template as<T>() : nullptr, n_, h_,
+                    centres_.as<IonicCentre>(), nat, tab_.as<double>(), stream_);
+    if (nl_rows_ == 0) return;
+    // colours: greedy, an ion joins the first colour none of whose boxes its box overlaps
+    std::vector<std::vector<int>> groups;
+    auto overlap = [&](const NlIon &a, const NlIon &b) {
+        for (int d = 0; d < 3; d++)
+            if (a.lo[d] + a.dim[d] <= b.lo[d] || b.lo[d] + b.dim[d] <= a.lo[d]) return false;
+        return true;
+    };
+    for (int i = 0; i < (int)nl_h_.size(); i++) {
+        bool placed = false;
+        for (auto &g : groups) {
+            bool ok = true;
+            for (int j : g) ok = ok && !overlap(nl_h_[i], nl_h_[j]);
+            if (ok) {
+                g.push_back(i);
+                placed = true;
+                break;
+            }
+        }
+        if (!placed) groups.push_back({i});
+    }
+    std::vector<int> members;
+    for (auto &g : groups) {
+        Colour c{(int)members.size(), (int)g.size(), 0};
+        for (int i : g) {
+            members.push_back(i);
+            c.max_pts = std::max(c.max_pts, nl_h_[i].pts);
+        }
+        colours_.push_back(c);
+    }
+    if (nl_ions_.bytes < nl_h_.size() * sizeof(NlIon)) nl_ions_.alloc(nl_h_.size() * sizeof(NlIon));
+    LIRA_HIP(hipMemcpy(nl_ions_.ptr, nl_h_.data(), nl_h_.size() * sizeof(NlIon), hipMemcpyHostToDevice));
+    if (nl_members_.bytes < members.size() * sizeof(int)) nl_members_.alloc(members.size() * sizeof(int));
+    LIRA_HIP(hipMemcpy(nl_members_.ptr, members.data(), members.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (nl_erow_.bytes < nl_e_h_.size() * sizeof(double)) nl_erow_.alloc(nl_e_h_.size() * sizeof(double));
+    LIRA_HIP(hipMemcpy(nl_erow_.ptr, nl_e_h_.data(), nl_e_h_.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (beta_.bytes < (size_t)beta_total * sizeof(T)) beta_.alloc((size_t)beta_total * sizeof(T));
+    for (const NlIon &ion : nl_h_) nl_beta<T>(beta_.template as<T>(), ion, n_, h_, tab_.as<double>(), stream_);
+}
+
+// out_b += V_NL x_b for a block: the projections C (f64, rows x nb), then the scatter colour by colour
+template <typename T> void Solver<T>::nl_apply_block(const T *x, T *out, int nb) {
+    const size_t need = (size_t)nl_rows_ * nb * 4;
+    if (need > nl_C_cap_) {
+        nl_C_.alloc(need * sizeof(double));
+        nl_C_cap_ = need;
+    }
+    nl_project<T>(x, nb, N_, n_, beta_.template as<T>(), nl_ions_.as<NlIon>(), (int)nl_h_.size(), dv_,
+                  nl_C_.as<double>(), stream_);
+    for (const Colour &c : colours_)
+        nl_apply<T>(out, nb, N_, n_, beta_.template as<T>(), nl_ions_.as<NlIon>(), nl_members_.as<int>() + c.off,
+                    c.count, c.max_pts, nl_C_.as<double>(), nl_erow_.as<double>(), stream_);
+}
+
+// E_NL terms f_b E_r c_rb^2 of a channel in (orbital, sphere, m) order; Aufbau: every orbital at f = 1
+// (qm/src/nonlocal.rs:148-172)
+template <typename T> std::vector<double> Solver<T>::nl_energy_terms(Channel<T> &c, bool fd) {
+    std::vector<double> terms;
+    if (nl_rows_ == 0 || c.k == 0) return terms;
+    const size_t need = (size_t)nl_rows_ * c.k * 4;
+    if (need > nl_C_cap_) {
+        nl_C_.alloc(need * sizeof(double));
+        nl_C_cap_ = need;
+    }
+    nl_project<T>(c.X.template as<T>(), c.k, N_, n_, beta_.template as<T>(), nl_ions_.as<NlIon>(), (int)nl_h_.size(),
+                  dv_, nl_C_.as<double>(), stream_);
+    const double *C = readback(nl_C_.as<double>(), (size_t)nl_rows_ * c.k);
+    for (int b = 0; b < c.k; b++) {
+        const double f = fd ? c.occ[b] : 1.0;
+        if (fd && f == 0.0) continue;
+        for (int r = 0; r < nl_rows_; r++) {
+            const double cr = C[(long)r * c.k + b];
+            terms.push_back(fd ? f * nl_e_h_[r] * cr * cr : nl_e_h_[r] * cr * cr);
+        }
+    }
+    return terms;
+}
+
+template <typename T>
+void Solver<T>::ionic_fields_host(const lira_nucleus *nuc, int nat, double sigma, double *vloc, double *pc) {
+    if (nat < 0 || (nat > 0 && !nuc) || !(sigma > 0.0)) throw InvalidError("lira_ionic_fields: bad arguments");
+    upload_nuclei(nuc, nat);
+    if (species_.empty()) throw InvalidError("lira_ionic_fields: no ionic setup");
+    ionic_geometry(sigma);
+    if (rho64_.bytes < N_ * sizeof(double)) rho64_.alloc(N_ * sizeof(double));
+    for (int w = 0; w < 2; w++) {
+        double *dst = w == 0 ? vloc : pc;
+        if (!dst) continue;
+        if (w == 1 && !has_pc_) {
+            std::memset(dst, 0, N_ * sizeof(double));
+            continue;
+        }
+        convert<double, T>(rho64_.as<double>(), (w == 0 ? vnuc_ : pc_).template as<T>(), N_, stream_);
+        LIRA_HIP(hipMemcpyAsync(dst, rho64_.ptr, N_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        LIRA_HIP(hipStreamSynchronize(stream_));
+    }
+}
+
+template <typename T>
+void Solver<T>::nonlocal_apply(const double *x, int nb, double *out, double *c, int c_cap, int *rows) {
+    if (nb < 1 || !x || !out) throw InvalidError("lira_nonlocal_apply: bad arguments");
+    if (rows) *rows = nl_rows_;
+    DevBuf x64((size_t)N_ * nb * sizeof(double)), xt((size_t)N_ * nb * sizeof(T)), ot((size_t)N_ * nb * sizeof(T));
+    LIRA_HIP(hipMemcpyAsync(x64.ptr, x, (size_t)N_ * nb * sizeof(double), hipMemcpyHostToDevice, stream_));
+    convert<T, double>(xt.template as<T>(), x64.as<double>(), N_ * nb, stream_);
+    fill<T>(ot.template as<T>(), N_ * nb, 0.0, stream_);
+    if (nl_rows_ > 0) nl_apply_block(xt.template as<T>(), ot.template as<T>(), nb);
+    convert<double, T>(x64.as<double>(), ot.template as<T>(), N_ * nb, stream_);
+    LIRA_HIP(hipMemcpyAsync(out, x64.ptr, (size_t)N_ * nb * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    LIRA_HIP(hipStreamSynchronize(stream_));
+    if (c && nl_rows_ > 0) {
+        const int cnt = std::min(c_cap, nl_rows_ * nb);
+        LIRA_HIP(hipMemcpy(c, nl_C_.ptr, cnt * sizeof(double), hipMemcpyDeviceToHost));
+    }
+}
+
+// The cold density: per centre a free atom (all-electron centres, free_atoms) rescaled so that its grid sum is its
+// charge, else a sigma = 1 blob of its charge; rho_in_[s] = scale_s * that sum (qm/src/nuclear.rs:153-231)
+template <typename T>
+void Solver<T>::cold_density(bool free_atoms, double sigma, const double *charges, double scale0, double scale1) {
+    const int nat = nat_;
+    std::vector<AtomProfile> atoms;
+    std::vector<double> tab, blobs;
+    for (int a = 0; a < nat; a++) {
+        const bool ae = species_of((int)z_h_[a]) == nullptr;
+        RadialProfile p;
+        if (free_atoms && ae) p = free_atom_profile((int)z_h_[a], sigma);
+        if (p.ok) {
+            AtomProfile A{};
+            for (int d = 0; d < 3; d++) A.pos[d] = pos_h_[3 * a + d];
+            A.x0 = p.x0;
+            A.dx = p.dx;
+            A.w = charges[a];  // replaced by charge / (grid sum) below
+            A.off = (int)tab.size();
+            A.len = (int)p.rho.size();
+            tab.insert(tab.end(), p.rho.begin(), p.rho.end());
+            atoms.push_back(A);
+        } else {
+            for (int d = 0; d < 3; d++) blobs.push_back(pos_h_[3 * a + d]);
+            blobs.push_back(charges[a]);
+        }
+    }
+    LIRA_HIP(hipStreamSynchronize(stream_));
+    if (!tab.empty()) {
+        if (atom_tab_.bytes < tab.size() * sizeof(double)) atom_tab_.alloc(tab.size() * sizeof(double));
+        LIRA_HIP(hipMemcpy(atom_tab_.ptr, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (atoms_dev_.bytes < (atoms.size() + 1) * sizeof(AtomProfile))
+        atoms_dev_.alloc((atoms.size() + 1) * sizeof(AtomProfile));
+    // per-atom grid sums with unit weight, then w = charge / (dv * sum); an atom whose samples sum to nothing
+    // (outside the box) keeps its blob
+    std::vector<AtomProfile> kept;
+    for (AtomProfile A : atoms) {
+        const double q = A.w;
+        A.w = 1.0;
+        LIRA_HIP(hipMemcpy(atoms_dev_.ptr, &A, sizeof(AtomProfile), hipMemcpyHostToDevice));
+        atom_density<T>(rho_t_.template as<T>(), n_, h_, atoms_dev_.as<AtomProfile>(), 1, atom_tab_.as<double>(),
+                        nullptr, 0, 1.0, stream_);
+        RedJobs<T> j{};
+        j.a[0] = j.b[0] = rho_t_.template as<T>();
+        j.kind[0] = RED_SUM;
+        j.count = 1;
+        const double sum = reduce(j, dv_)[0];
+        if (std::isfinite(sum) && sum > 0.0) {
+            A.w = q / sum;
+            kept.push_back(A);
+        } else {
+            for (int d = 0; d < 3; d++) blobs.push_back(A.pos[d]);
+            blobs.push_back(q);
+        }
+    }
+    if (!kept.empty())
+        LIRA_HIP(hipMemcpy(atoms_dev_.ptr, kept.data(), kept.size() * sizeof(AtomProfile), hipMemcpyHostToDevice));
+    if (blobs_dev_.bytes < (blobs.size() + 4) * sizeof(double)) blobs_dev_.alloc((blobs.size() + 4) * sizeof(double));
+    if (!blobs.empty())
+        LIRA_HIP(hipMemcpy(blobs_dev_.ptr, blobs.data(), blobs.size() * sizeof(double), hipMemcpyHostToDevice));
+    for (int s = 0; s < 2; s++)
+        atom_density<T>(rho_in_[s].template as<T>(), n_, h_, atoms_dev_.as<AtomProfile>(), (int)kept.size(),
+                        atom_tab_.as<double>(), blobs_dev_.as<double>(), (int)blobs.size() / 4, s == 0 ? scale0 : scale1,
+                        stream_);
+}
+
 // ----------------------------------------------------------------------------------------------- SCF ----
 template <typename T>
 lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, int warm, lira_abort_fn abort,
@@ -909,7 +1275,13 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
         throw InvalidError("lira_solve: bad system");
     if (!(st->sigma > 0.0) || !(st->tol > 0.0) || st->max_iter < 1 || st->diis_history > 15)
         throw InvalidError("lira_solve: bad settings");
-    // nucleus-nucleus repulsion and the coincidence check (qm/src/nuclear.rs:277-289): the warm state is kept
+    // nucleus-nucleus repulsion and the coincidence check (qm/src/nuclear.rs:277-289; with valence charges under an
+    // ionic setup, nuclear.rs:371-382): the warm state is kept
+    const bool ionic = !species_.empty();
+    auto charge = [&](int z) {
+        const SpeciesHost *sp = species_of(z);
+        return sp ? sp->z_v : (double)z;
+    };
     double e_nn = 0.0;
     for (int i = 0; i < nat; i++)
         for (int j = i + 1; j < nat; j++) {
@@ -917,12 +1289,12 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
             for (int a = 0; a < 3; a++) d2 += std::pow(sys->nuclei[i].pos[a] - sys->nuclei[j].pos[a], 2);
             const double d = std::sqrt(d2);
             if (d < 1e-8) throw std::pair<int, int>(i, j);
-            e_nn += (double)sys->nuclei[i].z * sys->nuclei[j].z / d;
+            e_nn += charge(sys->nuclei[i].z) * charge(sys->nuclei[j].z) / d;
         }
     const bool fd = st->occupations == LIRA_FERMI_DIRAC;
     const double kt = fd ? st->kt : 0.0;
     const bool reuse = warm && valid_ && nel_[0] == sys->n_up && nel_[1] == sys->n_dn && occ_scheme_ == st->occupations &&
-                       (!fd || kt_ == st->kt) && nat_ == nat;
+                       (!fd || kt_ == st->kt) && nat_ == nat && state_ionic_version_ == ionic_version_;
     upload_nuclei(sys->nuclei, nat);
     sigma_ = st->sigma;
     const int nel[2] = {sys->n_up, sys->n_dn};
@@ -937,17 +1309,26 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
         fill<double>(vh_.as<double>(), N_, 0.0, stream_);
     }
     if (rho64_.bytes < N_ * sizeof(double)) rho64_.alloc(N_ * sizeof(double));
-    nuclear_potential<T>(vnuc_.template as<T>(), n_, h_, pos_.as<double>(), zf_.as<double>(), nat, st->sigma, stream_);
+    if (ionic) {
+        ionic_geometry(st->sigma);  // V_loc, rho_pc, projectors (qm/src/scf.rs:733-745)
+    } else {
+        nuclear_potential<T>(vnuc_.template as<T>(), n_, h_, pos_.as<double>(), zf_.as<double>(), nat, st->sigma, stream_);
+        nl_rows_ = 0;
+        has_pc_ = false;
+    }
+    const T *pcp = has_pc_ ? pc_.template as<T>() : nullptr;
     if (!reuse) {
-        // cold start (qm/src/scf.rs:948-965): sigma = 1 blobs scaled to the electron count, seeded orbitals, V_H = 0
+        // cold start (qm/src/scf.rs:948-965): sigma = 1 blobs of each centre's charge scaled to the electron count,
+        // seeded orbitals, V_H = 0
         double ztot = 0.0;
-        for (int i = 0; i < nat; i++) ztot += sys->nuclei[i].z;
+        for (int i = 0; i < nat; i++) ztot += charge(sys->nuclei[i].z);
         const double total = nel[0] + nel[1];
         const double scale = ztot > 0.0 ? total / ztot : 0.0;
         for (int s = 0; s < 2; s++) {
             const double f = total > 0.0 ? nel[s] / total : 0.0;
-            deposit_charge<T>(rho_in_[s].template as<T>(), n_, h_, pos_.as<double>(), zf_.as<double>(), nat, 1.0, scale * f,
-                              stream_);
+            if (!fd)
+                deposit_charge<T>(rho_in_[s].template as<T>(), n_, h_, pos_.as<double>(),
+                                  (ionic ? qf_ : zf_).template as<double>(), nat, 1.0, scale * f, stream_);
             Channel<T> &c = ch_[s];
             const int k = nel[s] + (fd && nel[s] > 0 ? 1 : 0);
             c.k = 0;
@@ -958,6 +1339,14 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
             c.res.assign(k, 0.0);
             c.occ.assign(k, 1.0);
             c.tight = nel[s];
+        }
+        if (fd) {
+            // Fermi-Dirac: all-electron centres start from their sigma-smeared free atoms, pseudised ones (and any
+            // atom that does not solve) from sigma = 1 blobs of their charge (qm/src/nuclear.rs:198-231)
+            std::vector<double> q(nat);
+            for (int i = 0; i < nat; i++) q[i] = charge(sys->nuclei[i].z);
+            cold_density(true, st->sigma, q.data(), scale * (total > 0.0 ? nel[0] / total : 0.0),
+                         scale * (total > 0.0 ? nel[1] / total : 0.0));
         }
         fill<double>(vh_.as<double>(), N_, 0.0, stream_);
     } else {
@@ -997,7 +1386,7 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
         max_pres = std::max(max_pres, po.residual);
         max_pit = std::max(max_pit, po.iterations);
         convert<T, double>(vh_t_.template as<T>(), vh_.as<double>(), N_, stream_);
-        xc_lsda<T>(rho_in_[0].template as<T>(), rho_in_[1].template as<T>(), exc_.template as<T>(), vxc_[0].template as<T>(), vxc_[1].template as<T>(), N_, stream_);
+        xc_lsda<T>(rho_in_[0].template as<T>(), rho_in_[1].template as<T>(), exc_.template as<T>(), vxc_[0].template as<T>(), vxc_[1].template as<T>(), N_, stream_, pcp);
         sum3<T>(veff_[0].template as<T>(), vnuc_.template as<T>(), vh_t_.template as<T>(), vxc_[0].template as<T>(), N_, stream_);
         if (!restricted) sum3<T>(veff_[1].template as<T>(), vnuc_.template as<T>(), vh_t_.template as<T>(), vxc_[1].template as<T>(), N_, stream_);
         // the eigen tolerance follows delta_rho (crates/qm-gpu/src/scf.rs:383-386); a loose first step
@@ -1157,7 +1546,10 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
     t_poisson += po.seconds;
     max_pres = std::max(max_pres, po.residual);
     max_pit = std::max(max_pit, po.iterations);
-    xc_lsda<T>(rho_out_[0].template as<T>(), rho_out_[1].template as<T>(), exc_.template as<T>(), vxc_[0].template as<T>(), vxc_[1].template as<T>(), N_, stream_);
+    xc_lsda<T>(rho_out_[0].template as<T>(), rho_out_[1].template as<T>(), exc_.template as<T>(), vxc_[0].template as<T>(), vxc_[1].template as<T>(), N_, stream_, pcp);
+    double e_nl = 0.0;
+    if (nl_rows_ > 0)  // E_NL per spin channel, f-weighted under Fermi-Dirac (qm/src/scf.rs:1066-1072)
+        e_nl = neumaier(nl_energy_terms(ch_[0], fd)) + neumaier(nl_energy_terms(ch_[1], fd));
     double kinetic = 0.0;
     {
         std::vector<double> terms;
@@ -1208,7 +1600,7 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
     for (int i = 1; i < 5; i++) jt.b[i] = jt.a[i];
     reduce_jobs<T>(jt, n_, dv_, partials_.as<double>(), dres_.as<double>(), stream_);
     const double *ev = readback(dres_.as<double>(), 33);
-    const double external_ = ev[0], xc_e = ev[1], hartree = 0.5 * ev[32];
+    const double external_ = ev[0] + e_nl, xc_e = ev[1], hartree = 0.5 * ev[32];
     out->electrons[0] = ev[2];
     out->electrons[1] = ev[3];
     out->boundary_density = ev[4];
@@ -1230,6 +1622,7 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
     out->seconds_total = secs(t0);
     out->seconds_other = out->seconds_total - t_poisson - t_eigen;
     valid_ = true;
+    state_ionic_version_ = ionic_version_;
     nel_[0] = nel[0];
     nel_[1] = nel[1];
     occ_scheme_ = st->occupations;
@@ -1237,9 +1630,106 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
     return LIRA_OK;
 }
 
+// The five pseudised force terms (qm/src/forces.rs:259-350): [ion-ion, local LR, local SR, nonlocal, partial core]
+template <typename T> void Solver<T>::force_terms(const lira_nucleus *nuc, int nat, double sigma, double *out) {
+    if (!valid_) throw InvalidError("lira_force_terms: no converged state");
+    if (species_.empty() || state_ionic_version_ != ionic_version_)
+        throw InvalidError("lira_force_terms: the state was not converged under the current ionic setup");
+    if (std::fabs(sigma - sigma_) > 1e-15) throw InvalidError("lira_force_terms: sigma differs from the solve's");
+    if (nat < 0 || (nat > 0 && !nuc)) throw InvalidError("lira_force_terms: bad nuclei");
+    upload_nuclei(nuc, nat);
+    ionic_geometry(sigma);
+    std::vector<double> t((size_t)15 * nat, 0.0);
+    // ion-ion with valence charges (forces.rs:335-350)
+    for (int i = 0; i < nat; i++)
+        for (int j = 0; j < nat; j++) {
+            if (i == j) continue;
+            double d[3], d2 = 0.0;
+            for (int a = 0; a < 3; a++) {
+                d[a] = nuc[i].pos[a] - nuc[j].pos[a];
+                d2 += d[a] * d[a];
+            }
+            const double dd = std::sqrt(d2);
+            if (dd < 1e-8) throw std::pair<int, int>(std::min(i, j), std::max(i, j));
+            const double sc = q_h_[i] * q_h_[j] / (dd * dd * dd);
+            for (int a = 0; a < 3; a++) t[3 * i + a] += sc * d[a];
+        }
+    // local LR, local SR and partial core: radial sums on the device
+    add2<T>(rho_t_.template as<T>(), rho_out_[0].template as<T>(), rho_out_[1].template as<T>(), N_, stream_);
+    if (has_pc_)  // v_xc on rho_sigma + rho_pc / 2, the SCF's own XC (forces.rs:320-327)
+        xc_lsda<T>(rho_out_[0].template as<T>(), rho_out_[1].template as<T>(), exc_.template as<T>(),
+                   vxc_[0].template as<T>(), vxc_[1].template as<T>(), N_, stream_, pc_.template as<T>());
+    const size_t np = (size_t)9 * std::max(nat, 1);
+    if (fpart_.bytes < np * RED_BLOCKS * sizeof(double)) fpart_.alloc(np * RED_BLOCKS * sizeof(double));
+    if (fout_.bytes < np * sizeof(double)) fout_.alloc(np * sizeof(double));
+    ionic_forces<T>(rho_t_.template as<T>(), vxc_[0].template as<T>(), vxc_[1].template as<T>(), n_, h_,
+                    centres_.as<IonicCentre>(), tab_.as<double>(), nat, fpart_.as<double>(), fout_.as<double>(),
+                    stream_);
+    if (nat > 0) {
+        const double *r = readback(fout_.as<double>(), (size_t)9 * nat);
+        for (int term = 0; term < 3; term++)
+            for (int i = 0; i < 3 * nat; i++) t[(size_t)(term == 2 ? 4 : term + 1) * 3 * nat + i] = r[(size_t)term * 3 * nat + i];
+    }
+    // nonlocal: 2 sum_i f_i sum_lm E_KB c_lm <grad beta_lm | psi_i> (forces.rs:299-318)
+    if (nl_rows_ > 0) {
+        std::vector<double> cg[2];
+        for (int s = 0; s < 2; s++) {
+            Channel<T> &c = ch_[s];
+            if (c.k == 0) continue;
+            const size_t need = (size_t)nl_rows_ * 4 * c.k;
+            if (need > nl_C_cap_) {
+                nl_C_.alloc(need * sizeof(double));
+                nl_C_cap_ = need;
+            }
+            nl_force<T>(c.X.template as<T>(), c.k, N_, n_, h_, nl_ions_.as<NlIon>(), (int)nl_h_.size(),
+                        tab_.as<double>(), nl_C_.as<double>(), stream_);
+            const double *r = readback(nl_C_.as<double>(), need);
+            cg[s].assign(r, r + need);
+        }
+        const bool fdocc = occ_scheme_ == LIRA_FERMI_DIRAC;
+        std::vector<std::vector<double>> per((size_t)3 * nat);
+        for (size_t ii = 0; ii < nl_h_.size(); ii++) {
+            const NlIon &ion = nl_h_[ii];
+            const int atom = nl_atom_[ii];
+            int row = ion.row_off;
+            for (int j = 0; j < ion.nproj; j++) {  // one sphere per projector
+                const int nm = 2 * ion.l[j] + 1;
+                for (int s = 0; s < 2; s++) {
+                    Channel<T> &c = ch_[s];
+                    for (int b = 0; b < c.k; b++) {
+                        const double f = fdocc ? c.occ[b] : 1.0;
+                        if (f == 0.0) continue;
+                        for (int m = 0; m < nm; m++) {
+                            const long rr = row + m;
+                            const double cv = cg[s][(rr * 4 + 0) * c.k + b];
+                            for (int a = 0; a < 3; a++)
+                                per[(size_t)atom * 3 + a].push_back(2.0 * f * ion.e_kb[j] * cv *
+                                                                    cg[s][(rr * 4 + 1 + a) * c.k + b]);
+                        }
+                    }
+                }
+                row += nm;
+            }
+        }
+        for (int i = 0; i < 3 * nat; i++) t[(size_t)9 * nat + i] = neumaier(per[i]);
+    }
+    std::memcpy(out, t.data(), t.size() * sizeof(double));
+}
+
 template <typename T> void Solver<T>::forces(const lira_nucleus *nuc, int nat, double sigma, double *out) {
     if (!valid_) throw InvalidError("lira_forces: no converged state");
     if (std::fabs(sigma - sigma_) > 1e-15) throw InvalidError("lira_forces: sigma differs from the solve's");
+    if (state_ionic_version_ == ionic_version_ && !species_.empty()) {
+        // pseudised: the per-component compensated sum of the five terms, in order (qm/src/forces.rs:249-257)
+        std::vector<double> t((size_t)15 * nat);
+        force_terms(nuc, nat, sigma, t.data());
+        for (int i = 0; i < 3 * nat; i++) {
+            std::vector<double> five(5);
+            for (int term = 0; term < 5; term++) five[term] = t[(size_t)term * 3 * nat + i];
+            out[i] = neumaier(five);
+        }
+        return;
+    }
     upload_nuclei(nuc, nat);
     add2<T>(rho_t_.template as<T>(), rho_out_[0].template as<T>(), rho_out_[1].template as<T>(), N_, stream_);
     hf_forces<T>(rho_t_.template as<T>(), n_, h_, pos_.as<double>(), zf_.as<double>(), nat, sigma, partials_.as<double>(),
@@ -1264,7 +1754,7 @@ template <typename T> void Solver<T>::forces(const lira_nucleus *nuc, int nat, d
 }
 
 template <typename T> void Solver<T>::get_field(int which, double *out) {
-    if (!valid_ && which != 4) throw InvalidError("lira_get_field: no converged state");
+    if (!valid_ && which != 4 && which != 5) throw InvalidError("lira_get_field: no converged state");
     if (rho64_.bytes < N_ * sizeof(double)) rho64_.alloc(N_ * sizeof(double));
     double *d = rho64_.as<double>();
     switch (which) {
@@ -1278,6 +1768,10 @@ template <typename T> void Solver<T>::get_field(int which, double *out) {
         case 4:
             if (!vnuc_.ptr) throw InvalidError("lira_get_field: no potential yet");
             convert<double, T>(d, vnuc_.template as<T>(), N_, stream_);
+            break;
+        case 5:
+            if (!has_pc_) fill<double>(d, N_, 0.0, stream_);
+            else convert<double, T>(d, pc_.template as<T>(), N_, stream_);
             break;
         default: throw InvalidError("lira_get_field: unknown field");
     }
@@ -1457,6 +1951,38 @@ lira_status lira_solve(lira_solver *s, const lira_system *sys, const lira_settin
 lira_status lira_forces(lira_solver *s, const lira_nucleus *nuclei, int n_nuclei, double sigma, double *out) {
     if (!s || !nuclei || !out || n_nuclei <= 0) return LIRA_INVALID;
     return guarded(s, [&] { s->impl->forces(nuclei, n_nuclei, sigma, out); });
+}
+
+lira_status lira_free_atom_levels(int z, double *eps, double *electrons) {
+    if (z < 1 || z > 36 || !eps || !electrons) return LIRA_INVALID;
+    std::vector<double> e;
+    double el = 0.0;
+    if (!free_atom_levels(z, e, el)) return LIRA_NUMERICAL;
+    for (size_t i = 0; i < e.size() && i < 5; i++) eps[i] = e[i];
+    *electrons = el;
+    return LIRA_OK;
+}
+
+lira_status lira_set_ionic(lira_solver *s, const lira_species *species, int n_species) {
+    if (!s) return LIRA_INVALID;
+    return guarded(s, [&] { s->impl->set_ionic(species, n_species); });
+}
+
+lira_status lira_force_terms(lira_solver *s, const lira_nucleus *nuclei, int n_nuclei, double sigma, double *out) {
+    if (!s || !out) return LIRA_INVALID;
+    return guarded(s, [&] { s->impl->force_terms(nuclei, n_nuclei, sigma, out); });
+}
+
+lira_status lira_ionic_fields(lira_solver *s, const lira_nucleus *nuclei, int n_nuclei, double sigma, double *v_loc,
+                              double *rho_pc) {
+    if (!s) return LIRA_INVALID;
+    return guarded(s, [&] { s->impl->ionic_fields_host(nuclei, n_nuclei, sigma, v_loc, rho_pc); });
+}
+
+lira_status lira_nonlocal_apply(lira_solver *s, const double *x, int nb, double *out, double *c, int c_cap,
+                                int *n_rows) {
+    if (!s) return LIRA_INVALID;
+    return guarded(s, [&] { s->impl->nonlocal_apply(x, nb, out, c, c_cap, n_rows); });
 }
 
 lira_status lira_get_field(lira_solver *s, int which, double *out) {

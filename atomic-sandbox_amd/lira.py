@@ -28,8 +28,22 @@ EXPORTS = [
     "lira_forces", "lira_get_field", "lira_get_levels", "lira_get_orbitals", "lira_effective_eig_tol", "lira_reset",
     "lira_poisson", "lira_lobpcg", "lira_rayleigh_ritz", "lira_hamiltonian", "lira_laplacian", "lira_jacobi_sweep",
     "lira_xc_lsda", "lira_nuclear_potential", "lira_gram", "lira_stencil_variant_count", "lira_stencil_variant_name",
-    "lira_autotune_stencil", "lira_set_stencil_variant", "lira_version",
+    "lira_autotune_stencil", "lira_set_stencil_variant", "lira_version", "lira_set_ionic", "lira_force_terms",
+    "lira_ionic_fields", "lira_nonlocal_apply", "lira_free_atom_levels",
 ]
+
+
+class Projector(C.Structure):
+    """lira_projector (FilteredProjector, crates/qm/src/radial/filter.rs:323)."""
+    _fields_ = [("l", C.c_int), ("e_kb", C.c_double), ("radius", C.c_double), ("t_u", C.c_void_p), ("n_t", C.c_int)]
+
+
+class Species(C.Structure):
+    """lira_species (FilteredTable, crates/qm/src/radial/filter.rs:330-338)."""
+    _fields_ = [("z", C.c_int), ("z_v", C.c_double), ("sigma_lr", C.c_double), ("du", C.c_double),
+                ("v_sr_u", C.c_void_p), ("n_v_sr", C.c_int), ("r_sr", C.c_double), ("rho_pc_u", C.c_void_p),
+                ("n_rho_pc", C.c_int), ("pc_radius", C.c_double), ("projectors", C.c_void_p),
+                ("n_projectors", C.c_int)]
 
 
 class QmError(RuntimeError):
@@ -150,6 +164,11 @@ def lib():
         L.lira_autotune_stencil.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.lira_autotune_stencil.restype = C.c_int
         L.lira_set_stencil_variant.argtypes = [C.c_int, C.c_int]
+        L.lira_set_ionic.argtypes = [vp, vp, C.c_int]
+        L.lira_free_atom_levels.argtypes = [C.c_int, _dp, C.POINTER(C.c_double)]
+        L.lira_force_terms.argtypes = [vp, C.POINTER(Nucleus), C.c_int, C.c_double, _dp]
+        L.lira_ionic_fields.argtypes = [vp, C.POINTER(Nucleus), C.c_int, C.c_double, vp, vp]
+        L.lira_nonlocal_apply.argtypes = [vp, _dp, C.c_int, _dp, vp, C.c_int, C.POINTER(C.c_int)]
         L.lira_set_stencil_variant.restype = None
         L.lira_version.restype = C.c_char_p
         _lib = L
@@ -228,6 +247,55 @@ class GpuSolver:
         _raise(st, self._h, "lira_solve")
         return stats
 
+    def set_ionic(self, species):
+        """lira_set_ionic: species are objects with the attributes of synthetic_tables.Species (z, z_v, sigma_lr,
+        du, v_sr_u, r_sr, rho_pc_u, pc_radius, projectors[l, e_kb, radius, t_u]); an empty list clears the setup."""
+        keep = []
+        arr = (Species * max(1, len(species)))()
+        for i, sp in enumerate(species):
+            projs = (Projector * max(1, len(sp.projectors)))()
+            for j, pr in enumerate(sp.projectors):
+                t = np.ascontiguousarray(pr.t_u, dtype=np.float64)
+                keep.append(t)
+                projs[j] = Projector(pr.l, pr.e_kb, pr.radius, t.ctypes.data, t.size)
+            keep.append(projs)
+            v = np.ascontiguousarray(sp.v_sr_u, dtype=np.float64)
+            keep.append(v)
+            pc = None if sp.rho_pc_u is None else np.ascontiguousarray(sp.rho_pc_u, dtype=np.float64)
+            if pc is not None:
+                keep.append(pc)
+            arr[i] = Species(sp.z, sp.z_v, sp.sigma_lr, sp.du, v.ctypes.data, v.size, sp.r_sr,
+                             None if pc is None else pc.ctypes.data, 0 if pc is None else pc.size,
+                             sp.pc_radius if pc is not None else 0.0, C.cast(projs, C.c_void_p).value,
+                             len(sp.projectors))
+        _raise(lib().lira_set_ionic(self._h, C.cast(arr, C.c_void_p), len(species)), self._h, "lira_set_ionic")
+
+    def force_terms(self, pos, z, sigma) -> np.ndarray:
+        """[ion-ion, local LR, local SR, nonlocal, partial core] x nucleus x 3 (lira_force_terms)."""
+        nuc = nuclei_array(pos, z)
+        out = np.zeros(15 * len(nuc))
+        _raise(lib().lira_force_terms(self._h, nuc, len(nuc), sigma, out), self._h, "lira_force_terms")
+        return out.reshape(5, len(nuc), 3)
+
+    def ionic_fields(self, pos, z, sigma):
+        """(V_loc, rho_pc) of a geometry under the current ionic setup (lira_ionic_fields)."""
+        nuc = nuclei_array(pos, z)
+        v, pc = np.zeros(self.N), np.zeros(self.N)
+        _raise(lib().lira_ionic_fields(self._h, nuc, len(nuc), sigma, v.ctypes.data_as(C.c_void_p),
+                                       pc.ctypes.data_as(C.c_void_p)), self._h, "lira_ionic_fields")
+        return v, pc
+
+    def nonlocal_apply(self, x):
+        """(V_NL x, projections c[rows][band]) for host vectors x (nb x n^3) (lira_nonlocal_apply)."""
+        x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float64)
+        nb = x.shape[0]
+        out = np.zeros_like(x)
+        c = np.zeros(64 * nb)
+        rows = C.c_int()
+        _raise(lib().lira_nonlocal_apply(self._h, x.reshape(-1), nb, out.reshape(-1), c.ctypes.data_as(C.c_void_p),
+                                         c.size, C.byref(rows)), self._h, "lira_nonlocal_apply")
+        return out, c[:rows.value * nb].reshape(rows.value, nb)
+
     def forces(self, pos, z, sigma) -> np.ndarray:
         nuc = nuclei_array(pos, z)
         out = np.zeros(3 * len(nuc))
@@ -277,6 +345,14 @@ class GpuSolver:
         _raise(lib().lira_lobpcg(self._h, np.ascontiguousarray(v, dtype=np.float64), k, x, ev, precond_lambda, tol,
                                  max_iter, C.byref(st)), self._h, "lira_lobpcg")
         return x, ev, st
+
+
+def free_atom_levels(z):
+    """(1s, 2s, 3s, 2p, 3p eigenvalues, electron count) of the free LSDA atom behind the Fermi-Dirac cold guess."""
+    eps = np.zeros(5)
+    el = C.c_double()
+    _raise(lib().lira_free_atom_levels(z, eps, C.byref(el)), None, "lira_free_atom_levels")
+    return eps, el.value
 
 
 def rayleigh_ritz(a, b, nk):

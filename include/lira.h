@@ -159,7 +159,7 @@ typedef struct {
 /* The ionic setup of the next solves (System::ionic, crates/qm/src/scf.rs:329): species listed here are pseudised,
  * every other Z is all-electron at settings.sigma (Ionic::AllElectron). n_species = 0 clears it. The tables are
  * copied. A warm state converged under another setup is not reused (scf.rs:771-777). The caller passes the valence
- * electron counts in lira_system (System::with_ionic, scf.rs:362-365). */
+ * electron counts in the lira_system it solves, as System::with_ionic does (scf.rs:362-365). */
 lira_status lira_set_ionic(lira_solver *s, const lira_species *species, int n_species);
 /* The five pseudised force terms of the converged state, out[(t * n_nuclei + I) * 3 + a] for t in [ion-ion, local LR,
  * local SR, nonlocal, partial core] (GpuSolver::force_terms, crates/qm-gpu/src/forces.rs:276; qm::forces::
@@ -173,6 +173,11 @@ lira_status lira_ionic_fields(lira_solver *s, const lira_nucleus *nuclei, int n_
  * geometry of the last lira_ionic_fields or solve (NonlocalOp::apply_add / project, crates/qm/src/nonlocal.rs:133-145). */
 lira_status lira_nonlocal_apply(lira_solver *s, const double *x, int nb, double *out, double *c, int c_cap,
                                 int *n_rows);
+
+/* The spherical spin-restricted LSDA free atom behind the Fermi-Dirac cold guess (qm::radial::atom::ground_state,
+ * crates/qm/src/radial/atom.rs:474; used by free_atom_density, crates/qm/src/nuclear.rs:153): eps receives the
+ * 1s, 2s, 3s, 2p, 3p eigenvalues (1.0 for an unbound level) and *electrons the density's electron count. Host only. */
+lira_status lira_free_atom_levels(int z, double *eps, double *electrons);
 
 /* state getters (GpuSolver::state / ks_state / density_f32, crates/qm-gpu/src/scf.rs:1172, 1763-1851) as f64 in the
  * qm layout. which: 0 = rho_up, 1 = rho_dn, 2 = rho, 3 = V_H, 4 = V_nuc (V_loc under an ionic setup), 5 = rho_pc. */

@@ -1,0 +1,162 @@
+"""GPU parity of the pseudised-ion path (KB nonlocal operator, ionic local potential, partial core, the five force
+terms, the SCF under an ionic setup) against the oracle, on the closed-form synthetic tables of synthetic_tables.
+
+Tolerances: FP64 kernels to rounding against the oracle (1e-12 relative); MIXED kernels to f32 storage (beta and
+fields in f32, projections accumulated in f64); SCF states as in test_gpu_solvers (tight FP64 fixed point: energies
+1e-10, eigenvalues and forces 1e-8).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "atomic-sandbox_amd"))
+import synthetic_tables as T  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+WATER = ([[0.05, -0.03, 0.02], [1.43, 1.11, 0.0], [-1.43, 1.11, 0.0]], [8, 1, 1])
+
+
+@pytest.fixture(scope="module")
+def lira_mod():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    import lira
+    lira.lib()
+    return lira
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_ionic_fields_match_oracle(lira_mod, precision):
+    lira = lira_mod
+    n, h = 33, 0.25
+    pos, z = WATER
+    sp = [T.oxygen_like(h)]
+    s = lira.GpuSolver(n, h, precision)
+    s.set_ionic(sp)
+    v, pc = s.ionic_fields(pos, z, h)
+    vo = O.ionic_local_potential(n, h, pos, z, sp, h)
+    pco, _ = O.partial_core_field(n, h, pos, z, sp)
+    tol = 1e-12 if precision == 0 else 1e-6
+    assert np.abs(v - vo).max() < tol * np.abs(vo).max()
+    assert np.abs(pc - pco).max() < tol * np.abs(pco).max()
+    assert abs(pc.sum() - pco.sum()) * h ** 3 < 10 * tol
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_nonlocal_apply_and_projections_match_oracle(lira_mod, precision):
+    lira = lira_mod
+    n, h = 33, 0.25
+    pos, z = WATER
+    # a second pseudised O whose box overlaps the first: two colours in the scatter
+    pos2 = pos + [[0.9, -1.4, 0.3]]
+    z2 = z + [8]
+    sp = [T.oxygen_like(h)]
+    s = lira.GpuSolver(n, h, precision)
+    s.set_ionic(sp)
+    s.ionic_fields(pos2, z2, h)
+    x = np.random.default_rng(9).standard_normal((3, n ** 3))
+    out, c = s.nonlocal_apply(x)
+    nl = O.Nonlocal(n, h, pos2, z2, sp)
+    assert c.shape == (nl.rows, 3) == (8, 3)
+    tol = 1e-12 if precision == 0 else 2e-6
+    for b in range(3):
+        want = nl.apply_add(x[b], np.zeros(n ** 3))
+        assert np.abs(out[b] - want).max() < tol * np.abs(want).max(), b
+        cw = nl.project(x[b])
+        assert np.abs(c[:, b] - cw).max() < tol * np.abs(cw).max(), b
+
+
+def _pseudo_water_gpu(lira, precision, n, h, occ=0, tol=1e-6, eig_tol=1e-6):
+    pos, z = WATER
+    s = lira.GpuSolver(n, h, precision)
+    s.set_ionic([T.oxygen_like(h)])
+    st = s.solve(pos, z, 4, 4, lira.default_settings(h, occ, tol=tol, eig_tol=eig_tol))
+    return s, st
+
+
+def test_pseudo_scf_fp64_tight_matches_oracle(lira_mod):
+    # both solvers at delta_rho < 1e-9, eig_tol 1e-9 on the same discrete problem (25^3 x 0.3, O-like pseudo-ion + 2
+    # all-electron H): energies, levels, density and the five force terms agree to the tolerances
+    lira = lira_mod
+    n, h = 25, 0.3
+    pos, z = WATER
+    sp = [T.oxygen_like(h)]
+    s, st = _pseudo_water_gpu(lira, 0, n, h, tol=1e-9, eig_tol=1e-9)
+    so, out = O.scf(n, h, pos, z, 4, 4, settings=O.default_settings(h, "aufbau", tol=1e-9, eig_tol=1e-9),
+                    species=sp, want_orbitals=True)
+    assert so.status == O.OK
+    print(f"pseudo water: GPU E {st.energy.total:.12f} ({st.iterations} steps) oracle {so.total:.12f} "
+          f"({so.iterations} steps)")
+    assert abs(st.energy.total - so.total) < 1e-10
+    for name in ("kinetic", "external", "hartree", "xc", "nuclear"):
+        assert abs(getattr(st.energy, name) - getattr(so, name)) < 1e-8, name
+    state = s.state()
+    assert np.abs(state.evals_up - np.sort(out["evals_up"])).max() < 1e-8
+    rho = state.rho_up + state.rho_dn
+    rho_o = out["rho_up"] + out["rho_dn"]
+    assert np.abs(rho - rho_o).max() < 1e-7 * rho_o.max()
+    terms = s.force_terms(pos, z, h)
+    orbs = np.concatenate([out["orbitals_up"], out["orbitals_dn"]])
+    occ = np.concatenate([out["occ_up"], out["occ_dn"]])
+    _, to = O.pseudo_force_terms(n, h, out["rho_up"], out["rho_dn"], orbs, occ, pos, z, sp, h)
+    for t in range(5):
+        assert np.abs(terms[t] - to[t]).max() < 1e-8, (t, terms[t], to[t])
+    f = s.forces(pos, z, h)
+    assert np.abs(f - out["forces"]).max() < 1e-8
+    assert np.abs(f - terms.sum(0)).max() < 1e-12
+
+
+def test_pseudo_scf_mixed_matches_fp64(lira_mod):
+    lira = lira_mod
+    n, h = 33, 0.25
+    pos, z = WATER
+    s0, st0 = _pseudo_water_gpu(lira, 0, n, h)
+    s1, st1 = _pseudo_water_gpu(lira, 1, n, h)
+    f0, f1 = s0.forces(pos, z, h), s1.forces(pos, z, h)
+    print(f"pseudo water 33^3: FP64 {st0.energy.total:.9f} ({st0.seconds_total * 1e3:.0f} ms), MIXED "
+          f"{st1.energy.total:.9f} ({st1.seconds_total * 1e3:.0f} ms), dF {np.abs(f0 - f1).max():.2e}")
+    assert abs(st0.energy.total - st1.energy.total) < 3e-6
+    assert np.abs(f0 - f1).max() < 2e-5
+    assert st1.electrons[0] == pytest.approx(4.0, abs=1e-6)
+
+
+def test_pseudo_scf_fermi_dirac_runs_and_is_gapped(lira_mod):
+    lira = lira_mod
+    n, h = 33, 0.25
+    s, st = _pseudo_water_gpu(lira, 1, n, h, occ=1)
+    ev, occ = s.levels(0)
+    assert abs(occ.sum() - 4.0) < 1e-9 and st.energy.ts < 1e-9
+
+
+def test_local_only_species_reproduce_the_all_electron_solve(lira_mod):
+    # H2 with both H as local-only pseudo-ions (Z_v = 1, sigma_LR = sigma_AE): the all-electron answer
+    lira = lira_mod
+    n, h = 33, 0.25
+    pos, z = [[0.7, 0.0, 0.0], [-0.7, 0.0, 0.0]], [1, 1]
+    a = lira.GpuSolver(n, h, 0)
+    sa = a.solve(pos, z, 1, 1, lira.default_settings(h, 0))
+    b = lira.GpuSolver(n, h, 0)
+    b.set_ionic([T.local_only(1, 1.0, h, sigma_lr=h)])
+    sb = b.solve(pos, z, 1, 1, lira.default_settings(h, 0))
+    assert abs(sa.energy.total - sb.energy.total) < 1e-11
+    assert np.abs(a.forces(pos, z, h) - b.forces(pos, z, h)).max() < 1e-10
+    # a changed setup is a different problem: the warm state is not reused
+    b.set_ionic([])
+    sc = b.solve(pos, z, 1, 1, lira.default_settings(h, 0), warm=True)
+    assert sc.iterations == sa.iterations
+
+
+def test_set_ionic_rejects_bad_tables(lira_mod):
+    lira = lira_mod
+    s = lira.GpuSolver(17, 0.3, 1)
+    bad = T.oxygen_like(0.3)
+    bad.projectors = bad.projectors * 2  # 4 projectors
+    with pytest.raises(lira.Invalid):
+        s.set_ionic([bad])
+    with pytest.raises(lira.Invalid):
+        s.set_ionic([T.oxygen_like(0.3), T.oxygen_like(0.3)])  # Z listed twice

@@ -101,6 +101,27 @@ def test_lobpcg_oscillator(lira_mod, precision, tol):
     assert np.abs(ev - evo).max() < (1e-10 if precision == 0 else 2e-7)
 
 
+def test_lobpcg_wide_block_anisotropic_oscillator(lira_mod):
+    # k = 24 columns (S = [X, W, P] holds 72 > 64 columns): the lowest 24 levels of an anisotropic oscillator
+    # E = sum_a w_a (n_a + 1/2), w = (1, 1.13, 1.31): distinct levels, each to the discretisation error
+    lira = lira_mod
+    n, h = 41, 0.3
+    x, y, z = O.grid_points(n, h)
+    w = (1.0, 1.13, 1.31)
+    v = 0.5 * (w[0] ** 2 * x * x + w[1] ** 2 * y * y + w[2] ** 2 * z * z)
+    k = 24
+    levels = sorted(w[0] * (a + 0.5) + w[1] * (b + 0.5) + w[2] * (c + 0.5)
+                    for a in range(8) for b in range(8) for c in range(8))[:k]
+    guess = O.initial_orbitals(n, h, [[0.0, 0.0, 0.0]], k, 777)
+    s = lira.GpuSolver(n, h, 0)
+    xs, ev, st = s.lobpcg(v, guess, 1e-6, 800, precond_lambda=1.5)
+    print(f"wide lobpcg: {st.iterations} iterations, residual {st.max_residual:.2e}")
+    assert st.max_residual < 1e-6
+    assert np.abs(np.sort(ev) - np.array(levels)).max() < 2e-3
+    g = xs @ xs.T * h ** 3
+    assert np.abs(g - np.eye(k)).max() < 1e-8
+
+
 # ----------------------------------------------------------------------------------------------- SCF ----
 
 H2 = ([[0.7, 0.0, 0.0], [-0.7, 0.0, 0.0]], [1, 1], 1, 1)
