@@ -936,30 +936,33 @@ __device__ inline const void *col_base(const ColPtrs &c, int col, long N, int es
 // sub-tiles. Columns are staged 32 points at a time in LDS as f64 (f32 inputs convert exactly), so the products are
 // exact and the accumulation is f64 in every precision mode.
 typedef double lira_d4 __attribute__((ext_vector_type(4)));
-constexpr int G2_TI = 32, G2_TJ = 64, G2_P = 32;
+constexpr int G2_TI = 32, G2_TJ = 64;
 
-// Loads: the 96 columns x 32 points of a tile are 12 elements per thread, consecutive threads on consecutive points
-// of one column (coalesced). They are issued for tile t + 1 into registers before the MFMAs of tile t run out of LDS,
-// so their latency hides behind the matrix work; 16 x 16 output sub-tiles wholly outside m x q are skipped.
-constexpr int G2_LOADS = (G2_TI + G2_TJ) * G2_P / 256;
+// Each wave streams its own 16-point stages through a private LDS slice (96 columns x 16 points): no block barrier
+// in the loop, so the four waves of a workgroup never wait on each other. A stage is 24 elements per lane
+// (consecutive lanes on consecutive points of one column: 16 lanes read 16 contiguous values), loaded into registers
+// one stage ahead so their latency hides behind the 32 MFMAs of the current stage. The four waves take interleaved
+// stages of the workgroup's chunk; their 32 x 64 accumulators are summed in a fixed order at the end.
+constexpr int G2_S = 16;                             // points per wave stage
+constexpr int G2_LOADS = (G2_TI + G2_TJ) * G2_S / 64;  // elements per lane per stage
 
 template <typename T>
-__device__ inline void gram_fetch(const T *const *cols, long p0, long p_end, double (&v)[G2_LOADS]) {
-    const int t = threadIdx.x;
+__device__ inline void gram_fetch(const T *const *cols, long p0, long p_end, T (&v)[G2_LOADS]) {
+    const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int u = 0; u < G2_LOADS; u++) {
-        const int e = t + 256 * u;
-        const int col = e / G2_P, pp = e - col * G2_P;
+        const int e = lane + 64 * u;
+        const int col = e >> 4, pp = e & 15;
         const long p = p0 + pp;
         const T *c = cols[col];
-        v[u] = (c && p < p_end) ? (double)c[p] : 0.0;
+        v[u] = (c && p < p_end) ? c[p] : T(0);
     }
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q, long N, long chunk,
                                               double *partials) {
-    __shared__ double sab[G2_TI + G2_TJ][G2_P + 1];  // rows 0..31: S columns, 32..95: U columns
+    __shared__ double slab[4][G2_TI + G2_TJ][G2_S + 1];  // one slice per wave; reused for the final reduction
     __shared__ const T *cols[G2_TI + G2_TJ];
     const int nbj = (q + G2_TJ - 1) / G2_TJ;
     const int bi = blockIdx.y / nbj, bj = blockIdx.y - bi * nbj;
@@ -970,50 +973,71 @@ __global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q
         const int c = t - 64;
         cols[G2_TI + c] = j0 + c < q ? static_cast<const T *>(col_base(U, j0 + c, N, sizeof(T))) : nullptr;
     }
-    const int isub = wave & 1, jsub0 = (wave >> 1) * 2;
-    // wave-uniform: does this wave's pair of 16 x 16 sub-tiles hold any output?
-    const bool live_i = i0 + isub * 16 < m;
-    const bool live0 = live_i && j0 + jsub0 * 16 < q, live1 = live_i && j0 + (jsub0 + 1) * 16 < q;
-    lira_d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+    // which 16 x 16 output blocks hold any output (uniform over the block)
+    const int ni = min(2, (m - i0 + 15) / 16), nj = min(4, (q - j0 + 15) / 16);
+    lira_d4 acc[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = lira_d4{0.0, 0.0, 0.0, 0.0};
     const long p_begin = (long)blockIdx.x * chunk;
     const long p_end = min(N, p_begin + chunk);
     __syncthreads();
-    double v[G2_LOADS];
-    if (p_begin < p_end) gram_fetch<T>(cols, p_begin, p_end, v);
+    double(*sl)[G2_S + 1] = slab[wave];
+    T v[G2_LOADS];
+    long p0 = p_begin + (long)wave * G2_S;
+    if (p0 < p_end) gram_fetch<T>(cols, p0, p_end, v);
     const int r16 = lane & 15, k4 = lane >> 4;
-    for (long p0 = p_begin; p0 < p_end; p0 += G2_P) {
+    for (; p0 < p_end; p0 += 4 * G2_S) {
 #pragma unroll
         for (int u = 0; u < G2_LOADS; u++) {
-            const int e = t + 256 * u;
-            const int col = e / G2_P, pp = e - col * G2_P;
-            sab[col][pp] = v[u];
+            const int e = lane + 64 * u;
+            sl[e >> 4][e & 15] = (double)v[u];
         }
-        __syncthreads();
-        if (p0 + G2_P < p_end) gram_fetch<T>(cols, p0 + G2_P, p_end, v);
-        if (live0) {
+        __builtin_amdgcn_wave_barrier();
+        if (p0 + 4 * G2_S < p_end) gram_fetch<T>(cols, p0 + 4 * G2_S, p_end, v);
 #pragma unroll
-            for (int kk = 0; kk < G2_P; kk += 4) {
-                const double a = sab[isub * 16 + r16][kk + k4];
-                const double b0 = sab[G2_TI + jsub0 * 16 + r16][kk + k4];
-                acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc0, 0, 0, 0);
-                if (live1) {
-                    const double b1 = sab[G2_TI + (jsub0 + 1) * 16 + r16][kk + k4];
-                    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc1, 0, 0, 0);
+        for (int kk = 0; kk < G2_S; kk += 4) {
+            const double a0 = sl[r16][kk + k4];
+            const double a1 = sl[16 + r16][kk + k4];
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                if (b < nj) {
+                    const double bv = sl[G2_TI + 16 * b + r16][kk + k4];
+                    acc[0][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc[0][b], 0, 0, 0);
+                    if (ni > 1) acc[1][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, acc[1][b], 0, 0, 0);
                 }
             }
         }
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();
     }
-    double *dst = partials + (long)blockIdx.x * m * q;
-    const int jl = lane & 15, ib = lane >> 4;
+    // sum the four waves' accumulators in wave order through LDS (32 doubles per lane per wave)
+    __syncthreads();
+    double *red = &slab[0][0][0];
+    if (wave > 0) {
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const int i = i0 + isub * 16 + ib + 4 * r;
-        const int ja = j0 + jsub0 * 16 + jl, jb = ja + 16;
-        if (i < m) {
-            if (ja < q) dst[(long)i * q + ja] = acc0[r];
-            if (jb < q) dst[(long)i * q + jb] = acc1[r];
-        }
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) red[(((wave - 1) * 32) + (a * 16 + b * 4 + r)) * 64 + lane] = acc[a][b][r];
+    }
+    __syncthreads();
+    if (wave == 0) {
+        double *dst = partials + (long)blockIdx.x * m * q;
+        const int jl = lane & 15, ib = lane >> 4;
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int slot = a * 16 + b * 4 + r;
+                    const double sum = ((acc[a][b][r] + red[(0 * 32 + slot) * 64 + lane]) +
+                                        red[(1 * 32 + slot) * 64 + lane]) + red[(2 * 32 + slot) * 64 + lane];
+                    const int i = i0 + a * 16 + ib + 4 * r, j = j0 + b * 16 + jl;
+                    if (i < m && j < q) dst[(long)i * q + j] = sum;
+                }
     }
 }
 
@@ -1034,7 +1058,7 @@ __global__ __launch_bounds__(256) void k_gram_finish(const double *partials, int
 static int gram_chunks(int m, int q, long N) {
     const int blocks = ((m + G2_TI - 1) / G2_TI) * ((q + G2_TJ - 1) / G2_TJ);
     int chunks = (512 + blocks - 1) / blocks;
-    const long max_chunks = (N + G2_P - 1) / G2_P;
+    const long max_chunks = (N + 4 * G2_S - 1) / (4 * G2_S);
     if (chunks > max_chunks) chunks = (int)max_chunks;
     if (chunks < 1) chunks = 1;
     return chunks;
@@ -1063,7 +1087,7 @@ void gram(const Cols &S, const Cols &U, long N, double scale, double *partials, 
     const int chunks = gram_chunks(m, q, N);
     if ((size_t)chunks * m * q > partial_cap) throw InvalidError("gram: partial buffer too small");
     long chunk = (N + chunks - 1) / chunks;
-    chunk = ((chunk + G2_P - 1) / G2_P) * G2_P;
+    chunk = ((chunk + 4 * G2_S - 1) / (4 * G2_S)) * (4 * G2_S);
     const int blocks = ((m + G2_TI - 1) / G2_TI) * ((q + G2_TJ - 1) / G2_TJ);
     hipLaunchKernelGGL(k_gram<T>, dim3(chunks, blocks), dim3(256), 0, s, to_ptrs(S), to_ptrs(U), m, q, N, chunk,
                        partials);
