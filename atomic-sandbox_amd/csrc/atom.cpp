@@ -210,6 +210,22 @@ std::vector<double> smear(const LogGrid &g, const std::vector<double> &rho, doub
     return out;
 }
 
+// the Kohn-Sham potential -Z/r + V_H + v_xc of a restricted density on the grid
+std::vector<double> potential_of(const LogGrid &g, int z, const std::vector<double> &rho) {
+    const int n = g.size();
+    std::vector<double> f(n), Q(n, 0.0), outer(n, 0.0), V(n);
+    for (int i = 0; i < n; i++) f[i] = 4.0 * M_PI * g.r[i] * g.r[i] * rho[i];
+    Q[0] = 4.0 * M_PI / 3.0 * rho[0] * g.r[0] * g.r[0] * g.r[0];
+    for (int i = 1; i < n; i++) Q[i] = Q[i - 1] + 0.5 * (f[i - 1] * g.r[i - 1] + f[i] * g.r[i]) * g.dx;
+    for (int i = n - 2; i >= 0; i--) outer[i] = outer[i + 1] + 0.5 * (f[i] + f[i + 1]) * g.dx;
+    for (int i = 0; i < n; i++) {
+        double e, vu, vd;
+        exc_point(0.5 * rho[i], 0.5 * rho[i], e, vu, vd);
+        V[i] = -(double)z / g.r[i] + Q[i] / g.r[i] + outer[i] + vu;
+    }
+    return V;
+}
+
 }  // namespace
 
 RadialProfile free_atom_profile(int z, double sigma) {
@@ -224,7 +240,32 @@ RadialProfile free_atom_profile(int z, double sigma) {
     p.dx = g.dx;
     std::vector<double> rho = z >= 1 && z <= 36 ? solve_atom(g, z) : std::vector<double>();
     p.ok = !rho.empty();
-    if (p.ok) p.rho = smear(g, rho, sigma);
+    if (p.ok) {
+        p.rho = smear(g, rho, sigma);
+        // the occupied levels of the converged potential, R(r) = u(r) / r on the same grid, ascending in energy
+        const std::vector<double> V = potential_of(g, z, rho);
+        std::vector<Level> lv;
+        for (int l = 0; l <= 2; l++)
+            for (int nn = l + 1; nn <= 4; nn++) {
+                double eps;
+                if (eigen(g, V, nn, l, z, eps)) lv.push_back({nn, l, eps});
+            }
+        std::sort(lv.begin(), lv.end(), [](const Level &a, const Level &b) {
+            return a.eps < b.eps || (a.eps == b.eps && a.l < b.l);
+        });
+        double left = z;
+        for (const Level &L : lv) {
+            if (left <= 1e-12) break;
+            left -= std::min(left, 2.0 * (2 * L.l + 1));
+            const std::vector<double> w = orbital(g, V, L.l, L.eps);
+            RadialProfile::Orbital o;
+            o.l = L.l;
+            o.eps = L.eps;
+            o.R.resize(g.size());
+            for (int i = 0; i < g.size(); i++) o.R[i] = w[i] / std::sqrt(g.r[i]);
+            p.orbitals.push_back(std::move(o));
+        }
+    }
     cache[{z, sigma}] = p;
     return p;
 }

@@ -1102,39 +1102,34 @@ void gram(const Cols &S, const Cols &U, long N, double scale, double *partials, 
 // pointers are staged in LDS once per workgroup; grid.y covers q in groups of QB.
 constexpr int LC_MAX_M = 192;  // input columns of one lincomb (3 x the largest block LOBPCG forms)
 
+// The coefficients are wave-uniform (the row index and the output column are loop constants), so they are read
+// straight from C: the compiler keeps them in scalar registers and every FMA takes one as its scalar operand.
 template <typename T, int QB>
 __global__ __launch_bounds__(256) void k_lincomb(ColPtrs S, int m, const double *__restrict__ C, int q,
                                                  T *__restrict__ Y, int qsplit, T *__restrict__ Y2, long N) {
-    extern __shared__ double cs[];  // mp x QB coefficients (dynamic: m is a launch parameter)
     __shared__ const T *cols[LC_MAX_M];
     const int j0 = blockIdx.y * QB;
     const int qn = min(QB, q - j0);
-    const int mp = (m + 7) & ~7;  // rows m .. mp - 1 are zero: the input loop runs in unguarded groups of 8
-    for (int e = threadIdx.x; e < mp * QB; e += 256) {
-        const int i = e / QB, c = e - i * QB;
-        cs[e] = (i < m && c < qn) ? C[(long)i * q + j0 + c] : 0.0;
-    }
-    for (int i = threadIdx.x; i < mp; i += 256)
-        cols[i] = i < m ? static_cast<const T *>(col_base(S, i, N, sizeof(T))) : nullptr;
+    for (int i = threadIdx.x; i < m; i += 256) cols[i] = static_cast<const T *>(col_base(S, i, N, sizeof(T)));
     __syncthreads();
     const long p = blockIdx.x * 256L + threadIdx.x;
     if (p >= N) return;
     double acc[QB];
 #pragma unroll
     for (int c = 0; c < QB; c++) acc[c] = 0.0;
-    for (int i0 = 0; i0 < mp; i0 += 8) {
+    for (int i0 = 0; i0 < m; i0 += 8) {
         // the 8 loads of a group are independent and issued together
         double sv[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const T *c = cols[i0 + u];
-            sv[u] = c ? (double)c[p] : 0.0;
-        }
+        for (int u = 0; u < 8; u++) sv[u] = i0 + u < m ? (double)cols[i0 + u][p] : 0.0;
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-            const double *row = cs + (i0 + u) * QB;
+            if (i0 + u < m) {
+                const double *row = C + (long)(i0 + u) * q + j0;
 #pragma unroll
-            for (int c = 0; c < QB; c++) acc[c] += sv[u] * row[c];
+                for (int c = 0; c < QB; c++)
+                    if (c < qn) acc[c] += sv[u] * row[c];
+            }
         }
     }
 #pragma unroll
@@ -1157,19 +1152,14 @@ void lincomb(const Cols &S, const double *C, int q, T *Y, long N, hipStream_t s,
     const int groups = (q + 31) / 32;
     const int per = (q + groups - 1) / groups;
     const ColPtrs sp = to_ptrs(S);
-    const size_t mp = (size_t)((m + 7) & ~7);
     if (per <= 8) {
-        hipLaunchKernelGGL((k_lincomb<T, 8>), dim3(gx, (q + 7) / 8), dim3(256), mp * 8 * sizeof(double), s, sp, m, C,
-                           q, Y, qsplit, Y2, N);
+        hipLaunchKernelGGL((k_lincomb<T, 8>), dim3(gx, (q + 7) / 8), dim3(256), 0, s, sp, m, C, q, Y, qsplit, Y2, N);
     } else if (per <= 16) {
-        hipLaunchKernelGGL((k_lincomb<T, 16>), dim3(gx, (q + 15) / 16), dim3(256), mp * 16 * sizeof(double), s, sp, m,
-                           C, q, Y, qsplit, Y2, N);
+        hipLaunchKernelGGL((k_lincomb<T, 16>), dim3(gx, (q + 15) / 16), dim3(256), 0, s, sp, m, C, q, Y, qsplit, Y2, N);
     } else if (per <= 24) {
-        hipLaunchKernelGGL((k_lincomb<T, 24>), dim3(gx, (q + 23) / 24), dim3(256), mp * 24 * sizeof(double), s, sp, m,
-                           C, q, Y, qsplit, Y2, N);
+        hipLaunchKernelGGL((k_lincomb<T, 24>), dim3(gx, (q + 23) / 24), dim3(256), 0, s, sp, m, C, q, Y, qsplit, Y2, N);
     } else {
-        hipLaunchKernelGGL((k_lincomb<T, 32>), dim3(gx, (q + 31) / 32), dim3(256), mp * 32 * sizeof(double), s, sp, m,
-                           C, q, Y, qsplit, Y2, N);
+        hipLaunchKernelGGL((k_lincomb<T, 32>), dim3(gx, (q + 31) / 32), dim3(256), 0, s, sp, m, C, q, Y, qsplit, Y2, N);
     }
     LIRA_HIP(hipGetLastError());
 }
@@ -1296,6 +1286,58 @@ template <typename T> __global__ void k_xc(const T *nu, const T *nd, T *e, T *vu
 }
 template <typename T> void xc_lsda(const T *nu, const T *nd, T *e, T *vu, T *vd, long N, hipStream_t s, const T *pc) {
     hipLaunchKernelGGL(k_xc<T>, LIRA_GRID1(N), 0, s, nu, nd, e, vu, vd, N, pc);
+    LIRA_HIP(hipGetLastError());
+}
+
+// rho_in = max(2 rho_t - rho_{t-1}, 0): the density extrapolation of a warm start (crates/qm-gpu/src/scf.rs:1146-1162)
+template <typename T> __global__ void k_extrapolate(T *out, const T *a, const T *b, long N) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= N) return;
+    const double v = 2.0 * (double)a[i] - (double)b[i];
+    out[i] = (T)(v > 0.0 ? v : 0.0);
+}
+template <typename T> void extrapolate(T *out, const T *a, const T *b, long N, hipStream_t s) {
+    hipLaunchKernelGGL(k_extrapolate<T>, LIRA_GRID1(N), 0, s, out, a, b, N);
+    LIRA_HIP(hipGetLastError());
+}
+
+// (u + d)(p) by trilinear interpolation: the cell and fractions of qm::Grid::trilinear_cell in f64, neighbours beyond an
+// upper face read 0, blended a (1 - t) + b t along x, then y, then z (crates/qm/src/grid.rs:54-86)
+template <typename T>
+__global__ void k_sample(const T *u, const T *d, int n, double h, const double *pts, int count, double *out) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= count) return;
+    const double o = -((double)n - 1.0) * h / 2.0;
+    long cell[3];
+    double t[3];
+    for (int a = 0; a < 3; a++) {
+        const double x = (pts[3 * q + a] - o) / h;
+        if (!(x >= 0.0 && x <= (double)(n - 1))) {
+            out[q] = 0.0;
+            return;
+        }
+        const double i0 = floor(x);
+        cell[a] = (long)i0;
+        t[a] = x - i0;
+    }
+    auto at = [&](int di, int dj, int dk) {
+        const long a = cell[0] + di, b = cell[1] + dj, c = cell[2] + dk;
+        if (a < n && b < n && c < n) {
+            const long id = a + (long)n * (b + (long)n * c);
+            return (double)u[id] + (double)d[id];
+        }
+        return 0.0;
+    };
+    auto lerp = [](double a, double b, double tt) { return a * (1.0 - tt) + b * tt; };
+    auto xl = [&](int dj, int dk) { return lerp(at(0, dj, dk), at(1, dj, dk), t[0]); };
+    auto yl = [&](int dk) { return lerp(xl(0, dk), xl(1, dk), t[1]); };
+    out[q] = lerp(yl(0), yl(1), t[2]);
+}
+template <typename T>
+void sample_density(const T *u, const T *d, int n, double h, const double *pts, int count, double *out,
+                    hipStream_t s) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_sample<T>, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, u, d, n, h, pts, count, out);
     LIRA_HIP(hipGetLastError());
 }
 
@@ -1502,7 +1544,9 @@ void hf_forces(const T *rho, int n, double h, const double *pos, const double *z
     template void diis_mix<T>(T *, const T *const *, const T *const *, const double *, int, double, long,            \
                               hipStream_t);                                                                           \
     template void hf_forces<T>(const T *, int, double, const double *, const double *, int, double, double *,        \
-                               double *, hipStream_t);
+                               double *, hipStream_t);                                                                \
+    template void extrapolate<T>(T *, const T *, const T *, long, hipStream_t);                                      \
+    template void sample_density<T>(const T *, const T *, int, double, const double *, int, double *, hipStream_t);
 LIRA_INST(float)
 LIRA_INST(double)
 template void convert<float, double>(float *, const double *, long, hipStream_t);

@@ -183,6 +183,49 @@ void atom_density(T *out, int n, double h, const AtomProfile *atoms, int natoms,
     LIRA_HIP(hipGetLastError());
 }
 
+// one free-atom orbital sampled on the grid: R(d) (log-linear in ln d, the innermost value inside the grid, 0 beyond
+// it) times the real solid harmonic of (l, m) divided by d^l (an initial LOBPCG column; its norm does not matter)
+template <typename T>
+__global__ void k_atom_orbital(T *out, int n, double h, AtomProfile A, int l, int m, const double *tab) {
+    const long N = (long)n * n * n;
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= N) return;
+    const long nn = (long)n * n;
+    const long k = i / nn, rem = i - k * nn, j = rem / n, ii = rem - j * n;
+    const double o = origin(n, h);
+    const double dx = o + (double)ii * h - A.pos[0], dy = o + (double)j * h - A.pos[1], dz = o + (double)k * h - A.pos[2];
+    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    const double *R = tab + A.off;
+    const double r0 = exp(A.x0), rl = exp(A.x0 + (A.len - 1) * A.dx);
+    double v;
+    if (d <= r0) {
+        v = R[0];
+    } else if (d >= rl) {
+        v = 0.0;
+    } else {
+        const double s = (log(d) - A.x0) / A.dx;
+        int q = (int)floor(s);
+        q = q < 0 ? 0 : (q > A.len - 2 ? A.len - 2 : q);
+        const double t = s - (double)q;
+        v = R[q] + t * (R[q + 1] - R[q]);
+    }
+    double ang = 1.0;
+    if (l > 0) {
+        double Y[5];
+        solid_l(l, dx, dy, dz, Y);
+        const double dl = l == 1 ? d : d * d;
+        ang = d > 1e-12 ? Y[m + l] / dl : 0.0;
+    }
+    out[i] = (T)(v * ang);
+}
+
+template <typename T>
+void atom_orbital(T *out, int n, double h, const AtomProfile &a, int l, int m, const double *tab, hipStream_t s) {
+    const long N = (long)n * n * n;
+    hipLaunchKernelGGL(k_atom_orbital<T>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, out, n, h, a, l, m, tab);
+    LIRA_HIP(hipGetLastError());
+}
+
 // --------------------------------------------------------------------------------------- projectors ----
 // beta_lm = t_j(d) sqrt((2l+1)/4pi) R_lm(d) on the ion's box (qm/src/nonlocal.rs:94-128)
 template <typename T> __global__ void k_nl_beta(T *beta, NlIon ion, int n, double h, const double *tab) {
@@ -445,6 +488,7 @@ void nl_force(const T *psi, int nb, long N, int n, double h, const NlIon *ions, 
 }
 
 #define LIRA_IONIC_INST(T)                                                                                            \
+    template void atom_orbital<T>(T *, int, double, const AtomProfile &, int, int, const double *, hipStream_t);     \
     template void atom_density<T>(T *, int, double, const AtomProfile *, int, const double *, const double *, int,   \
                                   double, hipStream_t);                                                               \
     template void ionic_fields<T>(T *, T *, int, double, const IonicCentre *, int, const double *, hipStream_t);      \

@@ -187,6 +187,11 @@ void ionic_forces(const T *rho, const T *vu, const T *vd, int n, double h, const
 template <typename T>
 void nl_force(const T *psi, int nb, long N, int n, double h, const NlIon *ions, int nions, const double *tab,
               double *CG, hipStream_t s);
+// out = max(2 a - b, 0) (warm-start density extrapolation)
+template <typename T> void extrapolate(T *out, const T *a, const T *b, long N, hipStream_t s);
+// out[q] = (u + d) at pts[3q..3q+2] by trilinear interpolation (0 outside the grid)
+template <typename T>
+void sample_density(const T *u, const T *d, int n, double h, const double *pts, int count, double *out, hipStream_t s);
 // --------------------------------------------------------------- Poisson ----
 // Q_lm = dv * sum_p rho_p R_lm(r_p) for the 49 real solid harmonics l <= 6 (qm/src/poisson.rs:97-162)
 template <typename T> void multipole_moments(const T *rho, int n, double h, double *partials, double *q, hipStream_t s);
@@ -217,6 +222,12 @@ struct RadialProfile {
     double x0 = 0.0, dx = 0.0;
     std::vector<double> rho;
     bool ok = false;
+    struct Orbital {
+        int l;
+        double eps;
+        std::vector<double> R;  // the radial function u / r on the log grid
+    };
+    std::vector<Orbital> orbitals;  // the occupied levels, ascending in energy
 };
 RadialProfile free_atom_profile(int z, double sigma);
 // one free atom on the device: its profile at tab + off (len values), weight w, position
@@ -225,6 +236,9 @@ struct AtomProfile {
     double x0, dx, w;
     int off, len;
 };
+// out = R(|r - R_A|) * S_lm(r - R_A) / |r - R_A|^l (the real solid harmonic of index m in -l..l, Racah order)
+template <typename T>
+void atom_orbital(T *out, int n, double h, const AtomProfile &a, int l, int m, const double *tab, hipStream_t s);
 // out = scale * (sum_A w_A rho_A(|r - R_A|) + sum_B q_B (2 pi)^{-3/2} e^{-|r - R_B|^2 / 2}); blobs = (x, y, z, q) rows
 template <typename T>
 void atom_density(T *out, int n, double h, const AtomProfile *atoms, int natoms, const double *tab,

@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <numeric>
 
@@ -231,6 +232,11 @@ struct SolverIface {
     virtual void force_terms(const lira_nucleus *nuc, int nat, double sigma, double *out) = 0;
     virtual void ionic_fields_host(const lira_nucleus *nuc, int nat, double sigma, double *vloc, double *pc) = 0;
     virtual void nonlocal_apply(const double *x, int nb, double *out, double *c, int c_cap, int *rows) = 0;
+    virtual void set_witness_cap(int cap) = 0;
+    virtual int witness_cap() const = 0;
+    virtual void set_density_extrapolation(bool on) = 0;
+    virtual void get_profile(lira_profile *out, double *drho, double *etol, int cap, int *n) const = 0;
+    virtual void sample(const double *pts, int count, double *out) = 0;
 };
 
 // One pseudised species as the solver keeps it: scalars plus offsets of its tables in the f64 table arena.
@@ -277,6 +283,22 @@ template <typename T> class Solver final : public SolverIface {
     void force_terms(const lira_nucleus *nuc, int nat, double sigma, double *out) override;
     void ionic_fields_host(const lira_nucleus *nuc, int nat, double sigma, double *vloc, double *pc) override;
     void nonlocal_apply(const double *x, int nb, double *out, double *c, int c_cap, int *rows) override;
+    void set_witness_cap(int cap) override { witness_cap_ = std::min(std::max(cap, 1), MAX_WITNESS_GROWTH); }
+    int witness_cap() const override { return witness_cap_; }
+    void set_density_extrapolation(bool on) override {
+        extrapolate_ = on;
+        if (!on) have_prev_ = false;
+    }
+    void get_profile(lira_profile *out, double *drho, double *etol, int cap, int *n) const override {
+        *out = profile_;
+        const int cnt = (int)drho_steps_.size();
+        if (n) *n = cnt;
+        for (int i = 0; i < std::min(cap, cnt); i++) {
+            if (drho) drho[i] = drho_steps_[i];
+            if (etol) etol[i] = etol_steps_[i];
+        }
+    }
+    void sample(const double *pts, int count, double *out) override;
 
   private:
     int n_;
@@ -332,8 +354,15 @@ template <typename T> class Solver final : public SolverIface {
     int nl_rows_ = 0;
     bool has_pc_ = false;
     std::vector<double> q_h_;  // the charge each nucleus carries (Z_v pseudised, Z all-electron)
-    DevBuf atom_tab_, atoms_dev_, blobs_dev_;
+    DevBuf atom_tab_, atoms_dev_, blobs_dev_, ao_tab_;
+    // provider controls (crates/qm-gpu/src/scf.rs:1122-1170) and the last solve's profile
+    int witness_cap_ = MAX_WITNESS_GROWTH;
+    bool extrapolate_ = false, have_prev_ = false;
+    DevBuf prev_rho_[2], save_rho_[2], pts_dev_, samp_dev_;
+    lira_profile profile_{};
+    std::vector<double> drho_steps_, etol_steps_;
     void cold_density(bool free_atoms, double sigma, const double *charges, double scale0, double scale1);
+    void seed_atomic_orbitals(double sigma);
     const SpeciesHost *species_of(int z) const {
         for (const auto &sp : species_)
             if (sp.z == z) return &sp;
@@ -853,7 +882,7 @@ void Solver<T>::solve_channel(int spin, const T *v, int nel, bool fd, double kt,
         return;
     }
     const double guard = std::max(WITNESS_GUARD_TOL, tol);
-    const int cap = MAX_WITNESS_GROWTH;
+    const int cap = witness_cap_;
     auto buffered = [&](int len) { return fd && len >= nel + 2 && len - nel < cap; };
     auto add_column = [&]() {
         ensure_channel(c, c.k + 1);
@@ -868,6 +897,7 @@ void Solver<T>::solve_channel(int spin, const T *v, int nel, bool fd, double kt,
         const double lam = c.evals.empty() ? -1.0 : c.evals[0];
         const int checked = buffered(c.k) ? c.k - 1 : c.k;
         const int head = fd ? std::min(std::max(c.tight, 0), checked) : c.k;
+        profile_.peak_block = std::max(profile_.peak_block, c.k);
         EigenOut e = lobpcg(c, v, c.k, lam, tol, 200, fd ? guard : tol, head, checked);
         if (trace_on())
             std::fprintf(stderr, "lira:   spin %d k %d head %d tested %d tol %.1e: %d iterations, max residual %.2e, fallbacks %d\n",
@@ -897,6 +927,7 @@ void Solver<T>::solve_channel(int spin, const T *v, int nel, bool fd, double kt,
             keep = witness_keep_within(ev, rs, nel, occ.mu, kt, r_w);
             if (keep < 0) {
                 add_column();
+                profile_.witness_growths++;
                 continue;
             }
         } else if (witness) {
@@ -905,6 +936,7 @@ void Solver<T>::solve_channel(int spin, const T *v, int nel, bool fd, double kt,
         if (keep >= 0) {
             // keep the lowest `keep` tested levels (the first columns) plus a buffer above them when one fits
             const int len = std::min(c.k, buffered(keep + 1) ? keep + 1 : keep);
+            if (len < c.k) profile_.witness_trims++;
             c.k = len;
             c.evals.resize(len);
             c.res.resize(len);
@@ -921,6 +953,7 @@ void Solver<T>::solve_channel(int spin, const T *v, int nel, bool fd, double kt,
         const int extras = c.k - nel;
         const int target = std::min(std::max(2 * extras, extras + 1), cap);
         while (c.k - nel < target) add_column();
+        profile_.witness_growths++;
     }
 }
 
@@ -1260,6 +1293,75 @@ void Solver<T>::cold_density(bool free_atoms, double sigma, const double *charge
                         stream_);
 }
 
+// The first columns of each channel's cold block: the occupied orbitals of the all-electron centres' free atoms,
+// lowest levels first (the LCG columns beyond them stay). Only the starting subspace changes: LOBPCG converges to
+// the same levels from any block that is not deficient in them.
+template <typename T> void Solver<T>::seed_atomic_orbitals(double sigma) {
+    struct Ao {
+        int atom, orb, m;
+        double eps;
+    };
+    std::vector<RadialProfile> profs(nat_);
+    std::vector<Ao> list;
+    for (int a = 0; a < nat_; a++) {
+        if (species_of((int)z_h_[a])) continue;
+        profs[a] = free_atom_profile((int)z_h_[a], sigma);
+        if (!profs[a].ok) continue;
+        for (int i = 0; i < (int)profs[a].orbitals.size(); i++) {
+            const auto &o = profs[a].orbitals[i];
+            for (int m = -o.l; m <= o.l; m++) list.push_back({a, i, m, o.eps});
+        }
+    }
+    if (list.empty()) return;
+    std::stable_sort(list.begin(), list.end(), [](const Ao &x, const Ao &y) { return x.eps < y.eps; });
+    std::vector<double> tab;
+    std::vector<int> off(list.size());
+    std::map<std::pair<int, int>, int> placed;
+    for (size_t i = 0; i < list.size(); i++) {
+        auto key = std::make_pair(list[i].atom, list[i].orb);
+        auto it = placed.find(key);
+        if (it == placed.end()) {
+            const auto &R = profs[list[i].atom].orbitals[list[i].orb].R;
+            it = placed.emplace(key, (int)tab.size()).first;
+            tab.insert(tab.end(), R.begin(), R.end());
+        }
+        off[i] = it->second;
+    }
+    LIRA_HIP(hipStreamSynchronize(stream_));
+    if (ao_tab_.bytes < tab.size() * sizeof(double)) ao_tab_.alloc(tab.size() * sizeof(double));
+    LIRA_HIP(hipMemcpy(ao_tab_.ptr, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    for (int s = 0; s < 2; s++) {
+        Channel<T> &c = ch_[s];
+        const int cols = std::min(c.k, (int)list.size());
+        for (int col = 0; col < cols; col++) {
+            const Ao &e = list[col];
+            const RadialProfile &p = profs[e.atom];
+            AtomProfile A{};
+            for (int d = 0; d < 3; d++) A.pos[d] = pos_h_[3 * e.atom + d];
+            A.x0 = p.x0;
+            A.dx = p.dx;
+            A.w = 1.0;
+            A.off = off[col];
+            A.len = (int)p.orbitals[e.orb].R.size();
+            atom_orbital<T>(c.X.template as<T>() + (long)col * N_, n_, h_, A, p.orbitals[e.orb].l, e.m,
+                            ao_tab_.as<double>(), stream_);
+        }
+    }
+}
+
+template <typename T> void Solver<T>::sample(const double *pts, int count, double *out) {
+    if (count < 0 || (count > 0 && (!pts || !out))) throw InvalidError("lira_sample_density: bad arguments");
+    if (!valid_) throw InvalidError("lira_sample_density: no converged state");
+    if (count == 0) return;
+    if (pts_dev_.bytes < (size_t)count * 3 * sizeof(double)) pts_dev_.alloc((size_t)count * 3 * sizeof(double));
+    if (samp_dev_.bytes < (size_t)count * sizeof(double)) samp_dev_.alloc((size_t)count * sizeof(double));
+    LIRA_HIP(hipMemcpyAsync(pts_dev_.ptr, pts, (size_t)count * 3 * sizeof(double), hipMemcpyHostToDevice, stream_));
+    sample_density<T>(rho_out_[0].template as<T>(), rho_out_[1].template as<T>(), n_, h_, pts_dev_.as<double>(), count,
+                      samp_dev_.as<double>(), stream_);
+    LIRA_HIP(hipMemcpyAsync(out, samp_dev_.ptr, (size_t)count * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    LIRA_HIP(hipStreamSynchronize(stream_));
+}
+
 // ----------------------------------------------------------------------------------------------- SCF ----
 template <typename T>
 lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, int warm, lira_abort_fn abort,
@@ -1270,6 +1372,11 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
     cg_total_ = 0;
     lobpcg_total_ = 0;
     double t_poisson = 0.0, t_eigen = 0.0;
+    profile_ = lira_profile{};
+    drho_steps_.clear();
+    etol_steps_.clear();
+    const bool had_prev = have_prev_;
+    have_prev_ = false;  // re-established only by a converged warm solve
     const int nat = sys->n_nuclei;
     if (nat < 0 || (nat > 0 && !sys->nuclei) || sys->n_up < 0 || sys->n_dn < 0)
         throw InvalidError("lira_solve: bad system");
@@ -1347,12 +1454,36 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
             for (int i = 0; i < nat; i++) q[i] = charge(sys->nuclei[i].z);
             cold_density(true, st->sigma, q.data(), scale * (total > 0.0 ? nel[0] / total : 0.0),
                          scale * (total > 0.0 ? nel[1] / total : 0.0));
+            seed_atomic_orbitals(st->sigma);
         }
         fill<double>(vh_.as<double>(), N_, 0.0, stream_);
     } else {
-        for (int s = 0; s < 2; s++) copy<T>(rho_in_[s].template as<T>(), rho_out_[s].template as<T>(), N_, stream_);
+        for (int s = 0; s < 2; s++) {
+            if (extrapolate_) {  // keep rho_t: it becomes rho_{t-1} once this solve converges
+                alloc_field(save_rho_[s]);
+                copy<T>(save_rho_[s].template as<T>(), rho_out_[s].template as<T>(), N_, stream_);
+            }
+            if (extrapolate_ && had_prev)
+                extrapolate<T>(rho_in_[s].template as<T>(), rho_out_[s].template as<T>(), prev_rho_[s].template as<T>(),
+                               N_, stream_);
+            else
+                copy<T>(rho_in_[s].template as<T>(), rho_out_[s].template as<T>(), N_, stream_);
+        }
+        profile_.extrapolated = extrapolate_ && had_prev ? 1 : 0;
     }
     tune_stencil(std::max(ch_[0].k, 1));
+    profile_.start_wall = secs(t0);
+    auto record = [&](int steps, const EigenOut &e) {
+        profile_.steps = steps;
+        profile_.readbacks = readbacks_;
+        profile_.poisson_iterations = cg_total_;
+        profile_.eigen_iterations = lobpcg_total_;
+        profile_.eigen_fallbacks = e.fallbacks;
+        profile_.poisson_wall = t_poisson;
+        profile_.eigen_wall = t_eigen;
+        profile_.eig_tol_used = etol_steps_.empty() ? 0.0 : etol_steps_.back();
+        profile_.wall = secs(t0);
+    };
     const double ptol = std::min(1e-10, 1e-3 * st->tol);  // qm/src/scf.rs:519
     const double lo = effective_eig_tol(st->eig_tol), hi = std::max(EIG_TOL_MAX, lo);
     const int H = std::max(st->diis_history, 1);
@@ -1377,6 +1508,7 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
     while (it < st->max_iter) {
         if (abort && abort(user)) {
             out->iterations = it;
+            record(it, eo);
             return LIRA_ABORTED;
         }
         it++;
@@ -1429,6 +1561,8 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
         jd.count = 2;
         auto dr = reduce(jd, dv_);
         delta_rho = dr[0] + dr[1];
+        drho_steps_.push_back(delta_rho);
+        etol_steps_.push_back(tol_step);
         if (trace_on())
             std::fprintf(stderr, "lira: SCF step %d delta_rho %.3e eig tol %.1e%s, stored %d/%d, %.1f ms\n", it, delta_rho,
                          tol_step, tight ? " (tight)" : "", ch_[0].k, ch_[1].k, 1e3 * secs(t0));
@@ -1532,6 +1666,7 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
     out->lobpcg_iterations = lobpcg_total_;
     out->restricted = restricted ? 1 : 0;
     out->tol_used = st->tol;
+    record(it, eo);
     if (stalled) {
         out->seconds_total = secs(t0);
         return LIRA_STALLED;
@@ -1623,6 +1758,11 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
     out->seconds_other = out->seconds_total - t_poisson - t_eigen;
     valid_ = true;
     state_ionic_version_ = ionic_version_;
+    if (extrapolate_ && reuse) {
+        for (int s = 0; s < 2; s++) std::swap(prev_rho_[s], save_rho_[s]);
+        have_prev_ = true;
+    }
+    record(it, eo);
     nel_[0] = nel[0];
     nel_[1] = nel[1];
     occ_scheme_ = st->occupations;
@@ -1951,6 +2091,29 @@ lira_status lira_solve(lira_solver *s, const lira_system *sys, const lira_settin
 lira_status lira_forces(lira_solver *s, const lira_nucleus *nuclei, int n_nuclei, double sigma, double *out) {
     if (!s || !nuclei || !out || n_nuclei <= 0) return LIRA_INVALID;
     return guarded(s, [&] { s->impl->forces(nuclei, n_nuclei, sigma, out); });
+}
+
+lira_status lira_set_witness_cap(lira_solver *s, int cap) {
+    if (!s) return LIRA_INVALID;
+    return guarded(s, [&] { s->impl->set_witness_cap(cap); });
+}
+
+int lira_witness_cap(const lira_solver *s) { return s ? s->impl->witness_cap() : 0; }
+
+lira_status lira_set_density_extrapolation(lira_solver *s, int on) {
+    if (!s) return LIRA_INVALID;
+    return guarded(s, [&] { s->impl->set_density_extrapolation(on != 0); });
+}
+
+lira_status lira_get_profile(lira_solver *s, lira_profile *out, double *delta_rho_steps, double *eig_tol_steps,
+                             int cap, int *n_steps) {
+    if (!s || !out) return LIRA_INVALID;
+    return guarded(s, [&] { s->impl->get_profile(out, delta_rho_steps, eig_tol_steps, cap, n_steps); });
+}
+
+lira_status lira_sample_density(lira_solver *s, const double *points, int count, double *out) {
+    if (!s) return LIRA_INVALID;
+    return guarded(s, [&] { s->impl->sample(points, count, out); });
 }
 
 lira_status lira_free_atom_levels(int z, double *eps, double *electrons) {

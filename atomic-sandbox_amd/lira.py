@@ -29,8 +29,18 @@ EXPORTS = [
     "lira_poisson", "lira_lobpcg", "lira_rayleigh_ritz", "lira_hamiltonian", "lira_laplacian", "lira_jacobi_sweep",
     "lira_xc_lsda", "lira_nuclear_potential", "lira_gram", "lira_stencil_variant_count", "lira_stencil_variant_name",
     "lira_autotune_stencil", "lira_set_stencil_variant", "lira_version", "lira_set_ionic", "lira_force_terms",
-    "lira_ionic_fields", "lira_nonlocal_apply", "lira_free_atom_levels",
+    "lira_ionic_fields", "lira_nonlocal_apply", "lira_free_atom_levels", "lira_set_witness_cap", "lira_witness_cap",
+    "lira_set_density_extrapolation", "lira_get_profile", "lira_sample_density",
 ]
+
+
+class Profile(C.Structure):
+    """lira_profile (SolveProfile, crates/qm-gpu/src/scf.rs:439-514)."""
+    _fields_ = [("steps", C.c_int), ("readbacks", C.c_int), ("poisson_iterations", C.c_int),
+                ("eigen_iterations", C.c_int), ("eigen_fallbacks", C.c_int), ("witness_growths", C.c_int),
+                ("witness_trims", C.c_int), ("peak_block", C.c_int), ("extrapolated", C.c_int),
+                ("wall", C.c_double), ("start_wall", C.c_double), ("poisson_wall", C.c_double),
+                ("eigen_wall", C.c_double), ("eig_tol_used", C.c_double)]
 
 
 class Projector(C.Structure):
@@ -166,6 +176,12 @@ def lib():
         L.lira_set_stencil_variant.argtypes = [C.c_int, C.c_int]
         L.lira_set_ionic.argtypes = [vp, vp, C.c_int]
         L.lira_free_atom_levels.argtypes = [C.c_int, _dp, C.POINTER(C.c_double)]
+        L.lira_set_witness_cap.argtypes = [vp, C.c_int]
+        L.lira_witness_cap.restype = C.c_int
+        L.lira_witness_cap.argtypes = [vp]
+        L.lira_set_density_extrapolation.argtypes = [vp, C.c_int]
+        L.lira_get_profile.argtypes = [vp, C.POINTER(Profile), _dp, _dp, C.c_int, C.POINTER(C.c_int)]
+        L.lira_sample_density.argtypes = [vp, _dp, C.c_int, _dp]
         L.lira_force_terms.argtypes = [vp, C.POINTER(Nucleus), C.c_int, C.c_double, _dp]
         L.lira_ionic_fields.argtypes = [vp, C.POINTER(Nucleus), C.c_int, C.c_double, vp, vp]
         L.lira_nonlocal_apply.argtypes = [vp, _dp, C.c_int, _dp, vp, C.c_int, C.POINTER(C.c_int)]
@@ -295,6 +311,31 @@ class GpuSolver:
         _raise(lib().lira_nonlocal_apply(self._h, x.reshape(-1), nb, out.reshape(-1), c.ctypes.data_as(C.c_void_p),
                                          c.size, C.byref(rows)), self._h, "lira_nonlocal_apply")
         return out, c[:rows.value * nb].reshape(rows.value, nb)
+
+    def set_witness_cap(self, cap: int):
+        _raise(lib().lira_set_witness_cap(self._h, cap), self._h, "lira_set_witness_cap")
+
+    def witness_cap(self) -> int:
+        return lib().lira_witness_cap(self._h)
+
+    def set_density_extrapolation(self, on: bool):
+        _raise(lib().lira_set_density_extrapolation(self._h, int(bool(on))), self._h, "lira_set_density_extrapolation")
+
+    def profile(self):
+        """(Profile, delta_rho per step, eigen tolerance per step) of the last solve (lira_get_profile)."""
+        p = Profile()
+        dr, et = np.zeros(1024), np.zeros(1024)
+        cnt = C.c_int()
+        _raise(lib().lira_get_profile(self._h, C.byref(p), dr, et, 1024, C.byref(cnt)), self._h, "lira_get_profile")
+        k = min(cnt.value, 1024)
+        return p, dr[:k].copy(), et[:k].copy()
+
+    def sample_density(self, points) -> np.ndarray:
+        """rho_up + rho_dn of the converged state at points (count x 3, Bohr) (lira_sample_density)."""
+        pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
+        out = np.zeros(len(pts))
+        _raise(lib().lira_sample_density(self._h, pts.reshape(-1), len(pts), out), self._h, "lira_sample_density")
+        return out
 
     def forces(self, pos, z, sigma) -> np.ndarray:
         nuc = nuclei_array(pos, z)

@@ -174,6 +174,28 @@ lira_status lira_ionic_fields(lira_solver *s, const lira_nucleus *nuclei, int n_
 lira_status lira_nonlocal_apply(lira_solver *s, const double *x, int nb, double *out, double *c, int c_cap,
                                 int *n_rows);
 
+/* ---- provider-facing controls and products of GpuSolver (crates/qm-gpu/src/scf.rs, sample.rs) ---- */
+/* GpuSolver::set_witness_cap (scf.rs:1126): at most cap extra vectors per spin under Fermi-Dirac, clamped to 1..32
+ * (the provider derives it from its memory budget). lira_witness_cap returns the cap in force (scf.rs:1132). */
+lira_status lira_set_witness_cap(lira_solver *s, int cap);
+int lira_witness_cap(const lira_solver *s);
+/* GpuSolver::set_density_extrapolation (scf.rs:1160): a warm solve that follows two converged solves starts from
+ * rho_in = max(2 rho_t - rho_{t-1}, 0); off drops rho_{t-1}; a cold or failed solve drops it too. */
+lira_status lira_set_density_extrapolation(lira_solver *s, int on);
+/* SolveProfile of the last solve, kept after a failed one (scf.rs:439-514). */
+typedef struct {
+    int steps, readbacks, poisson_iterations, eigen_iterations, eigen_fallbacks;
+    int witness_growths, witness_trims, peak_block, extrapolated;
+    double wall, start_wall, poisson_wall, eigen_wall, eig_tol_used;
+} lira_profile;
+/* the profile, plus delta_rho and the eigen tolerance of every step (up to cap each; *n_steps the count) */
+lira_status lira_get_profile(lira_solver *s, lira_profile *out, double *delta_rho_steps, double *eig_tol_steps,
+                             int cap, int *n_steps);
+/* GpuSolver::sample_density (crates/qm-gpu/src/sample.rs:84): rho_up + rho_dn of the converged state at count points
+ * (3 doubles each, Bohr) by trilinear interpolation with qm::Grid::trilinear's cell and blend order (x, then y, then
+ * z; crates/qm/src/grid.rs:54-86); 0 outside [origin, origin + (n-1)h]^3. */
+lira_status lira_sample_density(lira_solver *s, const double *points, int count, double *out);
+
 /* The spherical spin-restricted LSDA free atom behind the Fermi-Dirac cold guess (qm::radial::atom::ground_state,
  * crates/qm/src/radial/atom.rs:474; used by free_atom_density, crates/qm/src/nuclear.rs:153): eps receives the
  * 1s, 2s, 3s, 2p, 3p eigenvalues (1.0 for an unbound level) and *electrons the density's electron count. Host only. */

@@ -190,6 +190,8 @@ def lib():
         L.or_scf_ionic.argtypes = [C.c_int, C.c_double, C.c_int, _dp, _ip, C.c_int, C.c_int, C.POINTER(Settings),
                                    C.c_void_p, C.POINTER(ScfStats), C.c_int, _dp, _dp, _dp, _dp, C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_trilinear.restype = C.c_double
+        L.or_trilinear.argtypes = [C.c_int, C.c_double, _dp, _dp]
         L.or_mg_apply_a2.restype = None
         L.or_mg_apply_a2.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp]
         for f in (L.or_mg_restrict, L.or_mg_prolong):
@@ -518,3 +520,10 @@ def pseudo_force_terms(n, h, rho_up, rho_dn, orbitals, occ, pos, z, species, sig
     st = lib().or_pseudo_force_terms(n, h, _f64(rho_up), _f64(rho_dn), k, orbs, _f64(occ), len(zz), p, zz, io.ref,
                                      terms)
     return st, terms.reshape(5, len(zz), 3)
+
+
+def trilinear(n, h, f, points):
+    """qm::Grid::trilinear of f at each point (count x 3)."""
+    f = _f64(f)
+    pts = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+    return np.array([lib().or_trilinear(n, h, f, _f64(p)) for p in pts])

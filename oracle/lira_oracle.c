@@ -1724,6 +1724,35 @@ int or_pseudo_force_terms(int n, double h, const double *rho_up, const double *r
     return OR_OK;
 }
 
+/* ---------------------------------------------------------- trilinear sampling ---- */
+
+/* qm/src/grid.rs:54-86: the cell and fractions, neighbours beyond an upper face read 0, blended along x, y, z */
+double or_trilinear(int n, double h, const double *f, const double p[3]) {
+    double o = origin_of(n, h), t[3];
+    long c[3];
+    for (int a = 0; a < 3; a++) {
+        double x = (p[a] - o) / h;
+        if (!(x >= 0.0 && x <= (double)(n - 1))) return 0.0;
+        double i0 = floor(x);
+        c[a] = (long)i0;
+        t[a] = x - i0;
+    }
+    double v[2][2][2];
+    for (int dk = 0; dk < 2; dk++)
+        for (int dj = 0; dj < 2; dj++)
+            for (int di = 0; di < 2; di++) {
+                long a = c[0] + di, b = c[1] + dj, cc = c[2] + dk;
+                v[dk][dj][di] = (a < n && b < n && cc < n) ? f[a + (long)n * (b + (long)n * cc)] : 0.0;
+            }
+    double y[2];
+    for (int dk = 0; dk < 2; dk++) {
+        double x0 = v[dk][0][0] * (1.0 - t[0]) + v[dk][0][1] * t[0];
+        double x1 = v[dk][1][0] * (1.0 - t[0]) + v[dk][1][1] * t[0];
+        y[dk] = x0 * (1.0 - t[1]) + x1 * t[1];
+    }
+    return y[0] * (1.0 - t[2]) + y[1] * t[2];
+}
+
 /* ---------------------------------------------------------- multigrid model ---- */
 
 /* qm-gpu/src/multigrid.rs:210-223 */

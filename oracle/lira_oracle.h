@@ -184,6 +184,9 @@ int or_scf_ionic(int n, double h, int nat, const double *pos, const int *z, int 
                  double *occ_dn, double *rho_up_out, double *rho_dn_out, double *forces_out, double *orbitals_up_out,
                  double *orbitals_dn_out);
 
+/* qm::Grid::trilinear (qm/src/grid.rs:54-86): f at p by trilinear interpolation, 0 outside the grid */
+double or_trilinear(int n, double h, const double *f, const double p[3]);
+
 /* multigrid f64 helpers (qm-gpu/src/multigrid.rs:207-258): the CPU model of the GPU V-cycle */
 void or_mg_apply_a2(int n, double h, double d_low, double d_high, const double *x, double *out);
 void or_mg_restrict(int n, const double *f, double *out);

@@ -233,3 +233,57 @@ def test_scf_abort_returns_aborted(lira_mod):
 
     with pytest.raises(lira.Aborted):
         s.solve(H2[0], H2[1], 1, 1, lira.default_settings(0.3, 0), abort=abort)
+
+
+# ----------------------------------------------------------------- provider-facing controls and products ----
+
+def test_profile_density_sampling_and_witness_cap(lira_mod):
+    lira = lira_mod
+    n, h = 33, 0.25
+    s = lira.GpuSolver(n, h, 0)
+    st = s.solve(H2[0], H2[1], 1, 1, lira.default_settings(h, 1))
+    p, dr, et = s.profile()
+    # SolveProfile (crates/qm-gpu/src/scf.rs:439-514): one delta_rho and eigen tolerance per step
+    assert p.steps == st.iterations == len(dr) == len(et)
+    assert dr[-1] < 1e-6 and np.all(et > 0)
+    assert p.eigen_iterations == st.lobpcg_iterations and p.readbacks == st.readbacks
+    assert p.peak_block >= 2 and p.wall >= p.eigen_wall > 0 and p.extrapolated == 0
+    # sample_density (crates/qm-gpu/src/sample.rs:84) = qm::Grid::trilinear of the state's density: random points,
+    # nodes (read bitwise) and points outside the grid (0)
+    rho = s.field(2)
+    rng = np.random.default_rng(21)
+    x, y, z = O.grid_points(n, h)
+    nodes = np.stack([x[[0, 5000, 17000, n ** 3 - 1]], y[[0, 5000, 17000, n ** 3 - 1]],
+                      z[[0, 5000, 17000, n ** 3 - 1]]], axis=1)
+    pts = np.concatenate([rng.uniform(-4.0, 4.0, (300, 3)), nodes, [[4.1, 0, 0], [0, -4.01, 0]]])
+    got = s.sample_density(pts)
+    want = O.trilinear(n, h, rho, pts)
+    assert np.abs(got - want).max() <= 1e-14 * rho.max()
+    assert np.array_equal(got[300:304], rho[[0, 5000, 17000, n ** 3 - 1]]) and np.all(got[304:] == 0.0)
+    # GpuSolver::set_witness_cap clamps to 1..32 (scf.rs:1126)
+    s.set_witness_cap(0)
+    assert s.witness_cap() == 1
+    s.set_witness_cap(99)
+    assert s.witness_cap() == 32
+
+
+def test_density_extrapolation_along_a_trajectory(lira_mod):
+    # GpuSolver::set_density_extrapolation (crates/qm-gpu/src/scf.rs:1146-1162): the third and later warm solves of a
+    # trajectory start from 2 rho_t - rho_{t-1}; they land on the cold answer; off drops it
+    lira = lira_mod
+    n, h = 33, 0.25
+    st = lira.default_settings(h, 0)
+    s = lira.GpuSolver(n, h, 1)
+    s.set_density_extrapolation(True)
+    used, steps = [], []
+    for i, xa in enumerate([0.70, 0.71, 0.72, 0.73]):
+        r = s.solve([[xa, 0.0, 0.0], [-0.7, 0.0, 0.0]], [1, 1], 1, 1, st, warm=i > 0)
+        used.append(s.profile()[0].extrapolated)
+        steps.append(r.iterations)
+    print(f"extrapolated trajectory: steps {steps}")
+    assert used == [0, 0, 1, 1]
+    cold = lira.GpuSolver(n, h, 1).solve([[0.73, 0.0, 0.0], [-0.7, 0.0, 0.0]], [1, 1], 1, 1, st)
+    assert abs(r.energy.total - cold.energy.total) < 1e-5
+    s.set_density_extrapolation(False)
+    s.solve([[0.74, 0.0, 0.0], [-0.7, 0.0, 0.0]], [1, 1], 1, 1, st, warm=True)
+    assert s.profile()[0].extrapolated == 0

@@ -347,3 +347,23 @@ def test_scf_h2_fermi_dirac_matches_reference():
 def test_scf_coincident_nuclei_is_an_error():
     st, _ = O.scf(17, 0.3, [[0, 0, 0], [0, 0, 0]], [1, 1], 1, 1)
     assert st.status == O.COINCIDENT
+
+
+# ------------------------------------------------------------------ trilinear sampling (qm/src/grid.rs:54-86) ----
+
+def test_trilinear_reproduces_linear_fields_reads_nodes_and_is_zero_outside():
+    # qm/src/grid.rs:143-170: a linear field is reproduced exactly inside [origin, origin + (n-1)h]^3 (off the upper
+    # faces, whose +1 neighbours read as 0), a node returns its stored value bitwise, and points outside read 0
+    n, h = 9, 0.5
+    x, y, z = O.grid_points(n, h)
+    f = 1.0 + 2.0 * x - y + 3.0 * z
+    rng = np.random.default_rng(5)
+    pts = rng.uniform(-2.0, 1.99, (50, 3))
+    got = O.trilinear(n, h, f, pts)
+    want = 1.0 + 2.0 * pts[:, 0] - pts[:, 1] + 3.0 * pts[:, 2]
+    assert np.abs(got - want).max() < 1e-12
+    g = rng.standard_normal(n ** 3)
+    idx = [0, 17, 300, n ** 3 - 1]
+    nodes = np.stack([x[idx], y[idx], z[idx]], axis=1)
+    assert np.array_equal(O.trilinear(n, h, g, nodes), g[idx])
+    assert np.all(O.trilinear(n, h, f, [[2.01, 0, 0], [0, -2.0001, 0], [0, 0, 9.0]]) == 0.0)
