@@ -242,7 +242,7 @@ RadialProfile free_atom_profile(int z, double sigma) {
     p.ok = !rho.empty();
     if (p.ok) {
         p.rho = smear(g, rho, sigma);
-        // the occupied levels of the converged potential, R(r) = u(r) / r on the same grid, ascending in energy
+        // the bound levels of the converged potential, R(r) = u(r) / r on the same grid, ascending in energy
         const std::vector<double> V = potential_of(g, z, rho);
         std::vector<Level> lv;
         for (int l = 0; l <= 2; l++)
@@ -253,10 +253,8 @@ RadialProfile free_atom_profile(int z, double sigma) {
         std::sort(lv.begin(), lv.end(), [](const Level &a, const Level &b) {
             return a.eps < b.eps || (a.eps == b.eps && a.l < b.l);
         });
-        double left = z;
+        // every bound level of the converged potential (the occupied ones and the bound unoccupied ones above them)
         for (const Level &L : lv) {
-            if (left <= 1e-12) break;
-            left -= std::min(left, 2.0 * (2 * L.l + 1));
             const std::vector<double> w = orbital(g, V, L.l, L.eps);
             RadialProfile::Orbital o;
             o.l = L.l;

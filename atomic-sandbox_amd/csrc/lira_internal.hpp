@@ -227,7 +227,7 @@ struct RadialProfile {
         double eps;
         std::vector<double> R;  // the radial function u / r on the log grid
     };
-    std::vector<Orbital> orbitals;  // the occupied levels, ascending in energy
+    std::vector<Orbital> orbitals;  // the bound levels (n <= 4, l <= 2), ascending in energy
 };
 RadialProfile free_atom_profile(int z, double sigma);
 // one free atom on the device: its profile at tab + off (len values), weight w, position

@@ -363,6 +363,11 @@ template <typename T> class Solver final : public SolverIface {
     std::vector<double> drho_steps_, etol_steps_;
     void cold_density(bool free_atoms, double sigma, const double *charges, double scale0, double scale1);
     void seed_atomic_orbitals(double sigma);
+    // the free-atom orbitals of the current cold start, lowest first: column c of a channel's block starts from
+    // entry c (cold columns and Fermi-Dirac growth alike) while entries last
+    std::vector<AtomProfile> ao_cols_;
+    std::vector<std::pair<int, int>> ao_lm_;
+    void seed_column(T *dst, int col, int spin);
     const SpeciesHost *species_of(int z) const {
         for (const auto &sp : species_)
             if (sp.z == z) return &sp;
@@ -886,8 +891,7 @@ void Solver<T>::solve_channel(int spin, const T *v, int nel, bool fd, double kt,
     auto buffered = [&](int len) { return fd && len >= nel + 2 && len - nel < cap; };
     auto add_column = [&]() {
         ensure_channel(c, c.k + 1);
-        initial_orbitals<T>(c.X.template as<T>() + (long)c.k * N_, n_, h_, pos_.as<double>(), nat_, 1,
-                            1000ULL + 2ULL * (uint64_t)c.k + (uint64_t)spin, stream_);
+        seed_column(c.X.template as<T>() + (long)c.k * N_, c.k, spin);
         c.evals.push_back(c.evals.back());
         c.res.push_back(0.0);
         c.k++;
@@ -1293,60 +1297,75 @@ void Solver<T>::cold_density(bool free_atoms, double sigma, const double *charge
                         stream_);
 }
 
-// The first columns of each channel's cold block: the occupied orbitals of the all-electron centres' free atoms,
-// lowest levels first (the LCG columns beyond them stay). Only the starting subspace changes: LOBPCG converges to
-// the same levels from any block that is not deficient in them.
+// The first columns of each channel's cold block: the bound free-atom levels of every centre (a pseudised centre's
+// above its core), lowest first (the LCG columns beyond them stay). Only the starting subspace changes: LOBPCG
+// converges to the same levels from any block that is not deficient in them.
 template <typename T> void Solver<T>::seed_atomic_orbitals(double sigma) {
     struct Ao {
         int atom, orb, m;
         double eps;
     };
+    ao_cols_.clear();
+    ao_lm_.clear();
     std::vector<RadialProfile> profs(nat_);
     std::vector<Ao> list;
     for (int a = 0; a < nat_; a++) {
-        if (species_of((int)z_h_[a])) continue;
+        // a pseudised centre starts from its all-electron atom's levels above the Z - Z_v core electrons
+        const SpeciesHost *sp = species_of((int)z_h_[a]);
         profs[a] = free_atom_profile((int)z_h_[a], sigma);
         if (!profs[a].ok) continue;
+        double core = sp ? z_h_[a] - sp->z_v : 0.0;
         for (int i = 0; i < (int)profs[a].orbitals.size(); i++) {
             const auto &o = profs[a].orbitals[i];
+            const double shell = 2.0 * (2 * o.l + 1);
+            if (core > 0.5) {
+                core -= shell;
+                continue;
+            }
             for (int m = -o.l; m <= o.l; m++) list.push_back({a, i, m, o.eps});
         }
     }
     if (list.empty()) return;
     std::stable_sort(list.begin(), list.end(), [](const Ao &x, const Ao &y) { return x.eps < y.eps; });
     std::vector<double> tab;
-    std::vector<int> off(list.size());
     std::map<std::pair<int, int>, int> placed;
-    for (size_t i = 0; i < list.size(); i++) {
-        auto key = std::make_pair(list[i].atom, list[i].orb);
+    for (const Ao &e : list) {
+        auto key = std::make_pair(e.atom, e.orb);
         auto it = placed.find(key);
         if (it == placed.end()) {
-            const auto &R = profs[list[i].atom].orbitals[list[i].orb].R;
+            const auto &R = profs[e.atom].orbitals[e.orb].R;
             it = placed.emplace(key, (int)tab.size()).first;
             tab.insert(tab.end(), R.begin(), R.end());
         }
-        off[i] = it->second;
+        const RadialProfile &p = profs[e.atom];
+        AtomProfile A{};
+        for (int d = 0; d < 3; d++) A.pos[d] = pos_h_[3 * e.atom + d];
+        A.x0 = p.x0;
+        A.dx = p.dx;
+        A.w = 1.0;
+        A.off = it->second;
+        A.len = (int)p.orbitals[e.orb].R.size();
+        ao_cols_.push_back(A);
+        ao_lm_.push_back({p.orbitals[e.orb].l, e.m});
     }
     LIRA_HIP(hipStreamSynchronize(stream_));
     if (ao_tab_.bytes < tab.size() * sizeof(double)) ao_tab_.alloc(tab.size() * sizeof(double));
     LIRA_HIP(hipMemcpy(ao_tab_.ptr, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
     for (int s = 0; s < 2; s++) {
         Channel<T> &c = ch_[s];
-        const int cols = std::min(c.k, (int)list.size());
-        for (int col = 0; col < cols; col++) {
-            const Ao &e = list[col];
-            const RadialProfile &p = profs[e.atom];
-            AtomProfile A{};
-            for (int d = 0; d < 3; d++) A.pos[d] = pos_h_[3 * e.atom + d];
-            A.x0 = p.x0;
-            A.dx = p.dx;
-            A.w = 1.0;
-            A.off = off[col];
-            A.len = (int)p.orbitals[e.orb].R.size();
-            atom_orbital<T>(c.X.template as<T>() + (long)col * N_, n_, h_, A, p.orbitals[e.orb].l, e.m,
-                            ao_tab_.as<double>(), stream_);
-        }
+        for (int col = 0; col < std::min(c.k, (int)ao_cols_.size()); col++)
+            seed_column(c.X.template as<T>() + (long)col * N_, col, s);
     }
+}
+
+// column col of a block: the col-th free-atom orbital while there is one, else the reference's seeded LCG vector
+template <typename T> void Solver<T>::seed_column(T *dst, int col, int spin) {
+    if (col < (int)ao_cols_.size())
+        atom_orbital<T>(dst, n_, h_, ao_cols_[col], ao_lm_[col].first, ao_lm_[col].second, ao_tab_.as<double>(),
+                        stream_);
+    else
+        initial_orbitals<T>(dst, n_, h_, pos_.as<double>(), nat_, 1, 1000ULL + 2ULL * (uint64_t)col + (uint64_t)spin,
+                            stream_);
 }
 
 template <typename T> void Solver<T>::sample(const double *pts, int count, double *out) {
@@ -1424,6 +1443,8 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
         has_pc_ = false;
     }
     const T *pcp = has_pc_ ? pc_.template as<T>() : nullptr;
+    ao_cols_.clear();
+    ao_lm_.clear();
     if (!reuse) {
         // cold start (qm/src/scf.rs:948-965): sigma = 1 blobs of each centre's charge scaled to the electron count,
         // seeded orbitals, V_H = 0
@@ -1950,6 +1971,7 @@ template <typename T> PoissonOut Solver<T>::poisson_host(const double *rho, doub
     LIRA_HIP(hipMemcpyAsync(rho64_.ptr, rho, N_ * sizeof(double), hipMemcpyHostToDevice, stream_));
     convert<T, double>(rho_t_.template as<T>(), rho64_.as<double>(), N_, stream_);
     LIRA_HIP(hipMemcpyAsync(vh_.ptr, v, N_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+    LIRA_HIP(hipStreamSynchronize(stream_));  // the reported solve time excludes the host transfers
     cg_total_ = 0;
     PoissonOut po = poisson(rho_t_.template as<T>(), vh_.as<double>(), tol, max_iter);
     LIRA_HIP(hipMemcpyAsync(v, vh_.ptr, N_ * sizeof(double), hipMemcpyDeviceToHost, stream_));

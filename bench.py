@@ -71,6 +71,57 @@ def hx_roofline(torch, lira, n=192, nb=64, reps=20, dtype="float32", variant=Non
             "variant_index": tuned, "frac": achieved / HBM_PEAK_GBS}
 
 
+def poisson_256(lira, n=256, h=0.15, seed=3):
+    """Poisson MG-PCG on 256^3: a neutral density of 32 seeded normalised Gaussians (centres in the inner 60 % of the
+    box, sigma U[0.3, 1.0] A, charges +1 x 16 and -1 x 16), solved to a relative residual of 1e-8 in MIXED precision.
+    Time of the device solve (the density upload excluded), after one untimed solve that builds the levels."""
+    rng = np.random.default_rng(seed)
+    L = (n - 1) * h
+    cen = rng.uniform(-0.3 * L, 0.3 * L, (32, 3))
+    sig = rng.uniform(0.3, 1.0, 32) * BOHR_PER_ANGSTROM
+    q = np.array([1.0] * 16 + [-1.0] * 16)
+    ax = -(n - 1) * h / 2.0 + h * np.arange(n)
+    rho = np.zeros((n, n, n))
+    for c, s_, qq in zip(cen, sig, q):
+        gx = np.exp(-(ax - c[0]) ** 2 / (2 * s_ * s_))
+        gy = np.exp(-(ax - c[1]) ** 2 / (2 * s_ * s_))
+        gz = np.exp(-(ax - c[2]) ** 2 / (2 * s_ * s_))
+        rho += qq / (2 * math.pi * s_ * s_) ** 1.5 * gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
+    rho = rho.reshape(-1)
+    s = lira.GpuSolver(n, h, lira.MIXED)
+    s.poisson(rho, tol=1e-8)
+    _, st = s.poisson(rho, tol=1e-8)
+    it = max(st.iterations, 1)
+    # 162 algorithmic bytes per point per MG-PCG iteration in f32 (stencil, V-cycle, dots, axpys)
+    gbs = 162.0 * n ** 3 * it / st.seconds / 1e9 if st.seconds > 0 else None
+    return {"seconds": st.seconds, "pcg_iterations": st.iterations, "corrections": st.outer, "residual": st.residual,
+            "grid": n, "h_bohr": h, "gbs_algorithmic": gbs}
+
+
+def pseudo_water(lira, n=95, h=0.15):
+    """Water with the O pseudised by the closed-form O-like tables (Z_v = 6, s + p projectors, partial core) and
+    all-electron H: 8 valence electrons, 4 + 4 bands plus witnesses; a cold MIXED and a cold FP64 solve (each after an
+    untimed one), Fermi-Dirac kT 5 mHa, delta_rho < 1e-6."""
+    import synthetic_tables as T
+    pos, z, _, _ = water()
+    settings = lira.default_settings(h, lira.FERMI_DIRAC)
+    out = {}
+    for name, prec in (("mixed", lira.MIXED), ("fp64", lira.FP64)):
+        s = lira.GpuSolver(n, h, prec)
+        s.set_ionic([T.oxygen_like(h)])
+        s.solve(pos, z, 4, 4, settings)
+        t0 = time.perf_counter()
+        st = s.solve(pos, z, 4, 4, settings)
+        out[name] = {"seconds": time.perf_counter() - t0, "steps": st.iterations, "energy": st.energy.total,
+                     "lobpcg_iterations": st.lobpcg_iterations}
+        out[name + "_forces"] = s.forces(pos, z, h)
+        del s
+    out["dE"] = out["mixed"]["energy"] - out["fp64"]["energy"]
+    out["max_dF"] = float(np.abs(out.pop("mixed_forces") - out.pop("fp64_forces")).max())
+    out["speedup"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+    return out
+
+
 def traffic_probe(esize, variant, n=192, nb=64):
     """Child of hbm_traffic, run under rocprofv3 --pmc: two copies of a known size (the counter calibration), then
     two H.X launches of the given stencil variant on the roofline problem."""
@@ -176,6 +227,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the pseudised-water and Poisson 256^3 measurements")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc passes")
     ap.add_argument("--roofline-only", action="store_true", help="only the H.X roofline measurement (for profiling)")
     ap.add_argument("--variant", type=int, default=None, help="stencil tile variant of the roofline (default: tuned)")
@@ -263,6 +315,15 @@ def main():
         result["roofline_fp64"] = {"achieved": r64["gbs"], "frac": r64["frac"], "ms_per_launch": r64["ms"]}
         print(f"# H.X 192^3 x 64 f32: {r['ms']:.3f} ms, {r['gbs']:.0f} GB/s ({100 * r['frac']:.1f} % of peak); "
               f"f64 {r64['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
+    if not args.no_extra:
+        result["pseudo_water_synthetic"] = pseudo_water(lira)
+        print(f"# pseudised water (synthetic O table): mixed {result['pseudo_water_synthetic']['mixed']['seconds']:.3f} s,"
+              f" fp64 {result['pseudo_water_synthetic']['fp64']['seconds']:.3f} s", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+        result["poisson_256"] = poisson_256(lira)
+        print(f"# Poisson 256^3 to 1e-8: {result['poisson_256']['seconds'] * 1e3:.1f} ms, "
+              f"{result['poisson_256']['pcg_iterations']} PCG iterations", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
     if not args.no_cpu and gpu_fp64 is not None:
         threads = min(16, os.cpu_count() or 1)
         result["cpu_baseline"] = cpu_baseline_sample(state, gpu_fp64, n, h, pos, z, threads)
