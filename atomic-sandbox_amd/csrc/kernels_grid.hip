@@ -53,9 +53,13 @@ struct StWork {
 };
 // order 0: tiles fastest (halo sharing between tiles of a band); order 1: bands fastest (the bands of one tile share
 // V and their x planes stream in step).
+// order 0: tiles fastest, then z chunks, then bands; 1: bands fastest; G >= 2: bands in groups of G fastest, then
+// tiles, then z chunks, then groups — the G bands of a tile share V in L2 while neighbouring tiles (the halos) run
+// side by side. Bands padded to a multiple of G are skipped.
 __device__ __forceinline__ StWork st_decode(int ntx, int nty, int nzc, int nb, int order) {
     const int tiles = ntx * nty;
-    const int total = tiles * nzc * nb;
+    const int nbp = order >= 2 ? ((nb + order - 1) / order) * order : nb;
+    const int total = tiles * nzc * nbp;
     const int per = (total + 7) / 8;
     const int lin = blockIdx.x;
     const int L = (lin & 7) * per + (lin >> 3);
@@ -67,18 +71,27 @@ __device__ __forceinline__ StWork st_decode(int ntx, int nty, int nzc, int nb, i
         rest = L / tiles;
         w.zchunk = rest % nzc;
         w.band = rest / nzc;
-    } else {
+    } else if (order == 1) {
         w.band = L % nb;
         rest = L / nb;
         tile = rest % tiles;
         w.zchunk = rest / tiles;
+    } else {
+        const int gi = L % order;
+        rest = L / order;
+        tile = rest % tiles;
+        rest = rest / tiles;
+        w.zchunk = rest % nzc;
+        w.band = (rest / nzc) * order + gi;
+        w.valid = w.valid && w.band < nb;
     }
     w.tile_x = tile % ntx;
     w.tile_y = tile / ntx;
     return w;
 }
-static inline unsigned st_grid(int ntx, int nty, int nzc, int nb) {
-    const long total = (long)ntx * nty * nzc * nb;
+static inline unsigned st_grid(int ntx, int nty, int nzc, int nb, int order) {
+    const long nbp = order >= 2 ? ((nb + order - 1) / order) * order : nb;
+    const long total = (long)ntx * nty * nzc * nbp;
     return (unsigned)(8 * ((total + 7) / 8));
 }
 
@@ -268,6 +281,18 @@ template <typename T, int RPT> struct SvRegs {
     V rq[3][RPT];
 };
 
+// 16-byte nontemporal stores (the stencil output streams out; it should not evict the inputs still in use)
+typedef float lira_nf4 __attribute__((ext_vector_type(4)));
+typedef double lira_nd2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void nt_store(float4 *p, const float4 &v) {
+    const lira_nf4 t = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(t, reinterpret_cast<lira_nf4 *>(p));
+}
+__device__ __forceinline__ void nt_store(double2 *p, const double2 &v) {
+    const lira_nd2 t = {v.x, v.y};
+    __builtin_nontemporal_store(t, reinterpret_cast<lira_nd2 *>(p));
+}
+
 template <typename T> __device__ __forceinline__ T vget(const typename Vec16<T>::type &v, int i);
 template <> __device__ __forceinline__ float vget<float>(const float4 &v, int i) {
     return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
@@ -381,7 +406,8 @@ __device__ __forceinline__ void sv_plane(SvRegs<T, RPT> &R, const SvCtx<T, RPT> 
             }
             vset<T>(outv, i, o);
         }
-        if (c.xin && gy < c.n) *reinterpret_cast<V *>(c.ob + c.gx + (long)c.n * gy + c.nn * z) = outv;
+        // streaming output: a nontemporal store keeps the block's results from evicting the inputs still in use
+        if (c.xin && gy < c.n) nt_store(reinterpret_cast<V *>(c.ob + c.gx + (long)c.n * gy + c.nn * z), outv);
     }
 }
 
@@ -584,7 +610,7 @@ __device__ __forceinline__ void vb_plane(VbRegs<T, NB> &R, const VbCtx<T, NB> &c
             }
             vset<T>(outv, i, o);
         }
-        if (c.bok[b] && c.xin && c.gy < c.n) *reinterpret_cast<V *>(c.ob[b] + c.gx + (long)c.n * c.gy + c.nn * z) = outv;
+        if (c.bok[b] && c.xin && c.gy < c.n) nt_store(reinterpret_cast<V *>(c.ob[b] + c.gx + (long)c.n * c.gy + c.nn * z), outv);
     }
 }
 
@@ -683,7 +709,7 @@ struct StVariant {
     int rpt;
     int zc;     // 0 = whole column
     int vec;    // 1: 16-byte vector lanes (needs n divisible by the vector width)
-    int order;  // XCD work order: 0 tiles fastest, 1 bands fastest
+    int order;  // XCD work order: 0 tiles fastest, 1 bands fastest, G >= 2 bands in groups of G fastest
     const char *name;
 };
 static const StVariant ST_VARIANTS[] = {
@@ -707,6 +733,13 @@ static const StVariant ST_VARIANTS[] = {
     {2, 32, 2, 0, "vector lanes, 2 bands per lane, z chunks of 32"},
     {2, 0, 2, 1, "vector lanes, 2 bands per lane, full z column, bands fastest"},
     {4, 0, 2, 0, "vector lanes, 4 bands per lane, full z column"},
+    {1, 0, 1, 8, "vector lanes, 16 rows, full z column, bands in groups of 8"},
+    {1, 0, 1, 4, "vector lanes, 16 rows, full z column, bands in groups of 4"},
+    {2, 0, 1, 8, "vector lanes, 32 rows, full z column, bands in groups of 8"},
+    {1, 32, 1, 8, "vector lanes, 16 rows, z chunks of 32, bands in groups of 8"},
+    {4, 0, 0, 8, "64x16 tile, full z column, bands in groups of 8"},
+    {2, 16, 0, 8, "64x8 tile, z chunks of 16, bands in groups of 8"},
+    {2, 16, 0, 4, "64x8 tile, z chunks of 16, bands in groups of 4"},
 };
 static int g_default_variant[2] = {-1, -1};
 
@@ -730,7 +763,7 @@ static void launch_stencil(const StencilParams &p, int zc, int order, const T *x
     const long N = (long)n * n * n;
     if (zc <= 0 || zc > n) zc = n;
     const int nzc = (n + zc - 1) / zc;
-    const unsigned grid = st_grid((n + ST_TX - 1) / ST_TX, (n + 4 * RPT - 1) / (4 * RPT), nzc, p.nb);
+    const unsigned grid = st_grid((n + ST_TX - 1) / ST_TX, (n + 4 * RPT - 1) / (4 * RPT), nzc, p.nb, order);
     dim3 block(64, 4, 1);
     hipLaunchKernelGGL((k_stencil<T, MODE, RPT>), dim3(grid), block, 0, s, x, out, v, r, lam, n, N, zc, p.nb, order,
                        (T)(1.0 / (p.h * p.h)), (T)p.scale, (T)p.omega, (T)p.diag_t);
@@ -745,7 +778,7 @@ static void launch_stencil_vec(const StencilParams &p, int zc, int order, const 
     const long N = (long)n * n * n;
     if (zc <= 0 || zc > n) zc = n;
     const int nzc = (n + zc - 1) / zc;
-    const unsigned grid = st_grid((n + 16 * VX - 1) / (16 * VX), (n + 16 * RPT - 1) / (16 * RPT), nzc, p.nb);
+    const unsigned grid = st_grid((n + 16 * VX - 1) / (16 * VX), (n + 16 * RPT - 1) / (16 * RPT), nzc, p.nb, order);
     hipLaunchKernelGGL((k_stencil_vec<T, MODE, RPT>), dim3(grid), dim3(16, 16, 1), 0, s, x, out, v, r, lam, n, N, zc,
                        p.nb, order, (T)(1.0 / (p.h * p.h)), (T)p.scale, (T)p.omega, (T)p.diag_t);
     LIRA_HIP(hipGetLastError());
@@ -759,7 +792,7 @@ static void launch_stencil_vb(const StencilParams &p, int zc, int order, const T
     const long N = (long)n * n * n;
     if (zc <= 0 || zc > n) zc = n;
     const int nzc = (n + zc - 1) / zc;
-    const unsigned grid = st_grid((n + 16 * VX - 1) / (16 * VX), (n + 15) / 16, nzc, (p.nb + NB - 1) / NB);
+    const unsigned grid = st_grid((n + 16 * VX - 1) / (16 * VX), (n + 15) / 16, nzc, (p.nb + NB - 1) / NB, order);
     hipLaunchKernelGGL((k_stencil_vb<T, MODE, NB>), dim3(grid), dim3(16, 16, 1), 0, s, x, out, v, r, lam, n, N, zc,
                        p.nb, order, (T)(1.0 / (p.h * p.h)), (T)p.scale, (T)p.omega, (T)p.diag_t);
     LIRA_HIP(hipGetLastError());
