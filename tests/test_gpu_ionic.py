@@ -160,3 +160,28 @@ def test_set_ionic_rejects_bad_tables(lira_mod):
         s.set_ionic([bad])
     with pytest.raises(lira.Invalid):
         s.set_ionic([T.oxygen_like(0.3), T.oxygen_like(0.3)])  # Z listed twice
+
+
+def test_pseudo_forces_are_minus_the_energy_gradient(lira_mod):
+    # the five-term pseudised force on the O against a Richardson central difference of the converged FP64 energy
+    # (qm/src/forces.rs:151-166: |HF - FD| well below 1e-4 Ha/Bohr), 25^3 x 0.3, O moved along y
+    lira = lira_mod
+    n, h = 25, 0.3
+    st = lira.default_settings(h, 0, tol=1e-9, eig_tol=1e-9)
+    base = [[0.0, 0.0, 0.0], [1.43, 1.11, 0.0], [-1.43, 1.11, 0.0]]
+
+    def run(dy):
+        p = [list(q) for q in base]
+        p[0][1] += dy
+        s = lira.GpuSolver(n, h, 0)
+        s.set_ionic([T.oxygen_like(h)])
+        r = s.solve(p, [8, 1, 1], 4, 4, st)
+        return r.energy.total, s.forces(p, [8, 1, 1], h)
+
+    _, f0 = run(0.0)
+    d = 0.01
+    fd1 = -(run(d)[0] - run(-d)[0]) / (2 * d)
+    fd2 = -(run(2 * d)[0] - run(-2 * d)[0]) / (4 * d)
+    rich = (4 * fd1 - fd2) / 3
+    print(f"pseudised O: HF {f0[0, 1]:+.8f} vs Richardson {rich:+.8f}")
+    assert abs(f0[0, 1] - rich) < 1e-4

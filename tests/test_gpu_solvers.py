@@ -287,3 +287,24 @@ def test_density_extrapolation_along_a_trajectory(lira_mod):
     s.set_density_extrapolation(False)
     s.solve([[0.74, 0.0, 0.0], [-0.7, 0.0, 0.0]], [1, 1], 1, 1, st, warm=True)
     assert s.profile()[0].extrapolated == 0
+
+
+def test_forces_are_minus_the_energy_gradient(lira_mod):
+    # dE/dR of the converged FP64 SCF by a Richardson central difference (forces.rs:151-166) against the GPU's
+    # Hellmann-Feynman force: H2 at 33^3 x 0.25, tight tolerances, nucleus 0 moved along x
+    lira = lira_mod
+    n, h = 33, 0.25
+    st = lira.default_settings(h, 0, tol=1e-9, eig_tol=1e-9)
+
+    def run(dx):
+        s = lira.GpuSolver(n, h, 0)
+        r = s.solve([[0.7 + dx, 0.0, 0.0], [-0.7, 0.0, 0.0]], [1, 1], 1, 1, st)
+        return r.energy.total, s.forces([[0.7 + dx, 0.0, 0.0], [-0.7, 0.0, 0.0]], [1, 1], h)
+
+    _, f0 = run(0.0)
+    d = 0.01
+    fd1 = -(run(d)[0] - run(-d)[0]) / (2 * d)
+    fd2 = -(run(2 * d)[0] - run(-2 * d)[0]) / (4 * d)
+    rich = (4 * fd1 - fd2) / 3
+    print(f"HF {f0[0, 0]:+.8f} vs Richardson {rich:+.8f}")
+    assert abs(f0[0, 0] - rich) < 1e-4
