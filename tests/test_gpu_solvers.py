@@ -308,3 +308,23 @@ def test_forces_are_minus_the_energy_gradient(lira_mod):
     rich = (4 * fd1 - fd2) / 3
     print(f"HF {f0[0, 0]:+.8f} vs Richardson {rich:+.8f}")
     assert abs(f0[0, 0] - rich) < 1e-4
+
+
+def test_config1_h2_48_fermi_dirac_fp64_matches_oracle(lira_mod):
+    # the small plain-method case: H2 at z = +-0.37 A (0.6992 Bohr) on 48^3 x 0.25 Bohr, LSDA singlet, Fermi-Dirac
+    # kT 5 mHa, FP64; both solvers driven to delta_rho < 1e-9: energy, occupied level and forces against the oracle
+    lira = lira_mod
+    n, h = 48, 0.25
+    pos, z = [[0.0, 0.0, 0.6992], [0.0, 0.0, -0.6992]], [1, 1]
+    s = lira.GpuSolver(n, h, 0)
+    st = s.solve(pos, z, 1, 1, lira.default_settings(h, 1, tol=1e-9, eig_tol=1e-9))
+    f = s.forces(pos, z, h)
+    so, out = O.scf(n, h, pos, z, 1, 1, settings=O.default_settings(h, "fd", tol=1e-9, eig_tol=1e-9))
+    assert so.status == O.OK
+    ev, occ = s.levels(0)
+    print(f"config 1: GPU E {st.energy.total:.12f} ({st.iterations} steps, {st.n_stored[0]} stored) oracle "
+          f"{so.total:.12f} ({so.iterations} steps)")
+    assert abs(st.energy.total - so.total) < 1e-10
+    assert abs(ev[0] - np.sort(out["evals_up"])[0]) < 1e-8
+    assert np.abs(f - out["forces"]).max() < 1e-8
+    assert abs(occ.sum() - 1.0) < 1e-12
