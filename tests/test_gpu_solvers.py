@@ -322,7 +322,7 @@ def test_config1_h2_48_fermi_dirac_fp64_matches_oracle(lira_mod):
     so, out = O.scf(n, h, pos, z, 1, 1, settings=O.default_settings(h, "fd", tol=1e-9, eig_tol=1e-9))
     assert so.status == O.OK
     ev, occ = s.levels(0)
-    print(f"config 1: GPU E {st.energy.total:.12f} ({st.iterations} steps, {st.n_stored[0]} stored) oracle "
+    print(f"H2 48^3 FD: GPU E {st.energy.total:.12f} ({st.iterations} steps, {st.n_stored[0]} stored) oracle "
           f"{so.total:.12f} ({so.iterations} steps)")
     assert abs(st.energy.total - so.total) < 1e-10
     assert abs(ev[0] - np.sort(out["evals_up"])[0]) < 1e-8
