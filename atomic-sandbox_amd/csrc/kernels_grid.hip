@@ -28,6 +28,8 @@ void DevBuf::release() {
 
 // 8th-order second-derivative coefficients c1..c4 (qm/src/stencil.rs:24; c0 = -205/72 is folded into the
 // differences: sum_m c_m[(f+ - f) + (f- - f)] per axis equals c0 f + sum_m c_m (f+ + f-) since c0 = -2 sum c_m).
+// Kernels use them as T(Cm): an f32 stencil computes in f32 throughout (a bare double constant would promote every
+// tap product to f64).
 #define C1 (8.0 / 5.0)
 #define C2 (-1.0 / 5.0)
 #define C3 (8.0 / 315.0)
@@ -156,14 +158,14 @@ __device__ __forceinline__ void st_plane(StRegs<T, RPT> &R, const StCtx<T, RPT> 
     for (int rr = 0; rr < RPT; rr++) {
         const int ly = c.ty + 4 * rr + 4, lx = c.tx + 4;
         const T ce = R.q[C0][rr];
-        T ax = C1 * ((tl[ly][lx + 1] - ce) + (tl[ly][lx - 1] - ce)) + C2 * ((tl[ly][lx + 2] - ce) + (tl[ly][lx - 2] - ce)) +
-               C3 * ((tl[ly][lx + 3] - ce) + (tl[ly][lx - 3] - ce)) + C4 * ((tl[ly][lx + 4] - ce) + (tl[ly][lx - 4] - ce));
-        T ay = C1 * ((tl[ly + 1][lx] - ce) + (tl[ly - 1][lx] - ce)) + C2 * ((tl[ly + 2][lx] - ce) + (tl[ly - 2][lx] - ce)) +
-               C3 * ((tl[ly + 3][lx] - ce) + (tl[ly - 3][lx] - ce)) + C4 * ((tl[ly + 4][lx] - ce) + (tl[ly - 4][lx] - ce));
-        T az = C1 * ((R.q[(S + 5) % 9][rr] - ce) + (R.q[(S + 3) % 9][rr] - ce)) +
-               C2 * ((R.q[(S + 6) % 9][rr] - ce) + (R.q[(S + 2) % 9][rr] - ce)) +
-               C3 * ((R.q[(S + 7) % 9][rr] - ce) + (R.q[(S + 1) % 9][rr] - ce)) +
-               C4 * ((R.q[(S + 8) % 9][rr] - ce) + (R.q[S % 9][rr] - ce));
+        T ax = T(C1) * ((tl[ly][lx + 1] - ce) + (tl[ly][lx - 1] - ce)) + T(C2) * ((tl[ly][lx + 2] - ce) + (tl[ly][lx - 2] - ce)) +
+               T(C3) * ((tl[ly][lx + 3] - ce) + (tl[ly][lx - 3] - ce)) + T(C4) * ((tl[ly][lx + 4] - ce) + (tl[ly][lx - 4] - ce));
+        T ay = T(C1) * ((tl[ly + 1][lx] - ce) + (tl[ly - 1][lx] - ce)) + T(C2) * ((tl[ly + 2][lx] - ce) + (tl[ly - 2][lx] - ce)) +
+               T(C3) * ((tl[ly + 3][lx] - ce) + (tl[ly - 3][lx] - ce)) + T(C4) * ((tl[ly + 4][lx] - ce) + (tl[ly - 4][lx] - ce));
+        T az = T(C1) * ((R.q[(S + 5) % 9][rr] - ce) + (R.q[(S + 3) % 9][rr] - ce)) +
+               T(C2) * ((R.q[(S + 6) % 9][rr] - ce) + (R.q[(S + 2) % 9][rr] - ce)) +
+               T(C3) * ((R.q[(S + 7) % 9][rr] - ce) + (R.q[(S + 1) % 9][rr] - ce)) +
+               T(C4) * ((R.q[(S + 8) % 9][rr] - ce) + (R.q[S % 9][rr] - ce));
         const T lap = ((ax + ay) + az) * c.inv_h2;
         const int gy = c.y0 + c.ty + 4 * rr;
         if (c.xin && gy < c.n) {
@@ -381,16 +383,16 @@ __device__ __forceinline__ void sv_plane(SvRegs<T, RPT> &R, const SvCtx<T, RPT> 
 #pragma unroll
         for (int i = 0; i < VX; i++) {
             const T ce = w[4 + i];
-            const T ax = C1 * ((w[5 + i] - ce) + (w[3 + i] - ce)) + C2 * ((w[6 + i] - ce) + (w[2 + i] - ce)) +
-                         C3 * ((w[7 + i] - ce) + (w[1 + i] - ce)) + C4 * ((w[8 + i] - ce) + (w[0 + i] - ce));
-            const T ay = C1 * ((vget<T>(yp[0], i) - ce) + (vget<T>(ym[0], i) - ce)) +
-                         C2 * ((vget<T>(yp[1], i) - ce) + (vget<T>(ym[1], i) - ce)) +
-                         C3 * ((vget<T>(yp[2], i) - ce) + (vget<T>(ym[2], i) - ce)) +
-                         C4 * ((vget<T>(yp[3], i) - ce) + (vget<T>(ym[3], i) - ce));
-            const T az = C1 * ((vget<T>(R.q[(S + 5) % 9][rr], i) - ce) + (vget<T>(R.q[(S + 3) % 9][rr], i) - ce)) +
-                         C2 * ((vget<T>(R.q[(S + 6) % 9][rr], i) - ce) + (vget<T>(R.q[(S + 2) % 9][rr], i) - ce)) +
-                         C3 * ((vget<T>(R.q[(S + 7) % 9][rr], i) - ce) + (vget<T>(R.q[(S + 1) % 9][rr], i) - ce)) +
-                         C4 * ((vget<T>(R.q[(S + 8) % 9][rr], i) - ce) + (vget<T>(R.q[S % 9][rr], i) - ce));
+            const T ax = T(C1) * ((w[5 + i] - ce) + (w[3 + i] - ce)) + T(C2) * ((w[6 + i] - ce) + (w[2 + i] - ce)) +
+                         T(C3) * ((w[7 + i] - ce) + (w[1 + i] - ce)) + T(C4) * ((w[8 + i] - ce) + (w[0 + i] - ce));
+            const T ay = T(C1) * ((vget<T>(yp[0], i) - ce) + (vget<T>(ym[0], i) - ce)) +
+                         T(C2) * ((vget<T>(yp[1], i) - ce) + (vget<T>(ym[1], i) - ce)) +
+                         T(C3) * ((vget<T>(yp[2], i) - ce) + (vget<T>(ym[2], i) - ce)) +
+                         T(C4) * ((vget<T>(yp[3], i) - ce) + (vget<T>(ym[3], i) - ce));
+            const T az = T(C1) * ((vget<T>(R.q[(S + 5) % 9][rr], i) - ce) + (vget<T>(R.q[(S + 3) % 9][rr], i) - ce)) +
+                         T(C2) * ((vget<T>(R.q[(S + 6) % 9][rr], i) - ce) + (vget<T>(R.q[(S + 2) % 9][rr], i) - ce)) +
+                         T(C3) * ((vget<T>(R.q[(S + 7) % 9][rr], i) - ce) + (vget<T>(R.q[(S + 1) % 9][rr], i) - ce)) +
+                         T(C4) * ((vget<T>(R.q[(S + 8) % 9][rr], i) - ce) + (vget<T>(R.q[S % 9][rr], i) - ce));
             const T lap = ((ax + ay) + az) * c.inv_h2;
             T o;
             if (MODE == ST_LAP) {
@@ -585,16 +587,16 @@ __device__ __forceinline__ void vb_plane(VbRegs<T, NB> &R, const VbCtx<T, NB> &c
 #pragma unroll
         for (int i = 0; i < VX; i++) {
             const T ce = w[4 + i];
-            const T ax = C1 * ((w[5 + i] - ce) + (w[3 + i] - ce)) + C2 * ((w[6 + i] - ce) + (w[2 + i] - ce)) +
-                         C3 * ((w[7 + i] - ce) + (w[1 + i] - ce)) + C4 * ((w[8 + i] - ce) + (w[0 + i] - ce));
-            const T ay = C1 * ((vget<T>(yp[0], i) - ce) + (vget<T>(ym[0], i) - ce)) +
-                         C2 * ((vget<T>(yp[1], i) - ce) + (vget<T>(ym[1], i) - ce)) +
-                         C3 * ((vget<T>(yp[2], i) - ce) + (vget<T>(ym[2], i) - ce)) +
-                         C4 * ((vget<T>(yp[3], i) - ce) + (vget<T>(ym[3], i) - ce));
-            const T az = C1 * ((vget<T>(R.q[(S + 5) % 9][b], i) - ce) + (vget<T>(R.q[(S + 3) % 9][b], i) - ce)) +
-                         C2 * ((vget<T>(R.q[(S + 6) % 9][b], i) - ce) + (vget<T>(R.q[(S + 2) % 9][b], i) - ce)) +
-                         C3 * ((vget<T>(R.q[(S + 7) % 9][b], i) - ce) + (vget<T>(R.q[(S + 1) % 9][b], i) - ce)) +
-                         C4 * ((vget<T>(R.q[(S + 8) % 9][b], i) - ce) + (vget<T>(R.q[S % 9][b], i) - ce));
+            const T ax = T(C1) * ((w[5 + i] - ce) + (w[3 + i] - ce)) + T(C2) * ((w[6 + i] - ce) + (w[2 + i] - ce)) +
+                         T(C3) * ((w[7 + i] - ce) + (w[1 + i] - ce)) + T(C4) * ((w[8 + i] - ce) + (w[0 + i] - ce));
+            const T ay = T(C1) * ((vget<T>(yp[0], i) - ce) + (vget<T>(ym[0], i) - ce)) +
+                         T(C2) * ((vget<T>(yp[1], i) - ce) + (vget<T>(ym[1], i) - ce)) +
+                         T(C3) * ((vget<T>(yp[2], i) - ce) + (vget<T>(ym[2], i) - ce)) +
+                         T(C4) * ((vget<T>(yp[3], i) - ce) + (vget<T>(ym[3], i) - ce));
+            const T az = T(C1) * ((vget<T>(R.q[(S + 5) % 9][b], i) - ce) + (vget<T>(R.q[(S + 3) % 9][b], i) - ce)) +
+                         T(C2) * ((vget<T>(R.q[(S + 6) % 9][b], i) - ce) + (vget<T>(R.q[(S + 2) % 9][b], i) - ce)) +
+                         T(C3) * ((vget<T>(R.q[(S + 7) % 9][b], i) - ce) + (vget<T>(R.q[(S + 1) % 9][b], i) - ce)) +
+                         T(C4) * ((vget<T>(R.q[(S + 8) % 9][b], i) - ce) + (vget<T>(R.q[S % 9][b], i) - ce));
             const T lap = ((ax + ay) + az) * c.inv_h2;
             T o;
             if (MODE == ST_LAP) {
