@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <type_traits>
 
 namespace lira {
 
@@ -35,6 +36,40 @@ void DevBuf::release() {
 #define C3 (8.0 / 315.0)
 #define C4 (-1.0 / 560.0)
 
+// ---- buffer addressing of the stencil fields ----
+// The stencils read and write whole fields through raw buffer descriptors whose range is one field. A load at a byte
+// offset outside [0, range) returns 0 -- the Dirichlet zero of every out-of-box neighbour -- and a store there is
+// dropped, so no load or store sits in a branch (a bounds branch around each load forces a wait for every load in
+// flight at the join, which would serialise the look-ahead). A lane's (x, y) position gets the byte offset of its row in
+// a plane, or BUF_OOB when (x, y) lies outside the box; plane z adds z * plane_bytes in 32-bit unsigned arithmetic, so
+// z < 0 wraps to >= 2^31 and z >= n lands at or past the end. Both stay out of range while
+// (N + 4 n^2) * sizeof(T) < 2^31, which the host launchers check.
+constexpr unsigned BUF_OOB = 0x80000000u;
+typedef __amdgpu_buffer_rsrc_t BufRes;
+__device__ __forceinline__ BufRes field_res(const void *base, long bytes) {
+    // word 3 = 0x00020000: 32-bit data format, no swizzle, raw addressing (the gfx9 family's value)
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ unsigned row_off(int xx, int yy, int n, int esize) {
+    return (xx >= 0 && xx < n && yy >= 0 && yy < n) ? (unsigned)(xx + n * yy) * (unsigned)esize : BUF_OOB;
+}
+template <typename T> __device__ __forceinline__ T bload(BufRes r, unsigned off);
+template <> __device__ __forceinline__ float bload<float>(BufRes r, unsigned off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <> __device__ __forceinline__ double bload<double>(BufRes r, unsigned off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+// aux 2 = nontemporal: the stencil output streams out and should not evict the inputs still in use
+template <typename T> __device__ __forceinline__ void bstore_nt(T v, BufRes r, unsigned off);
+template <> __device__ __forceinline__ void bstore_nt<float>(float v, BufRes r, unsigned off) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 2);
+}
+template <> __device__ __forceinline__ void bstore_nt<double>(double v, BufRes r, unsigned off) {
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, off, 0, 2);
+}
+
 // ------------------------------------------------------------------------------------------ stencil ----
 // 2.5-D z-streaming: a workgroup owns a 64 x TY tile of (x, y) columns of one band and a chunk of z planes.
 // Each thread keeps its RPT columns' 9-plane z-window in registers; the x/y taps of the current plane come from an
@@ -53,8 +88,6 @@ struct StWork {
     int tile_x, tile_y, band, zchunk;
     bool valid;
 };
-// order 0: tiles fastest (halo sharing between tiles of a band); order 1: bands fastest (the bands of one tile share
-// V and their x planes stream in step).
 // order 0: tiles fastest, then z chunks, then bands; 1: bands fastest; G >= 2: bands in groups of G fastest, then
 // tiles, then z chunks, then groups — the G bands of a tile share V in L2 while neighbouring tiles (the halos) run
 // side by side. Bands padded to a multiple of G are skipped.
@@ -97,56 +130,63 @@ static inline unsigned st_grid(int ntx, int nty, int nzc, int nb, int order) {
     return (unsigned)(8 * ((total + 7) / 8));
 }
 
-template <typename T, int RPT> struct StRegs {
-    T q[9][RPT];
+// Compile-time plane unrolling: f(integral_constant<int, S>) for S = 0 .. W-1 (every register slot index a constant)
+template <int S, int W> struct PlaneUnroll {
+    template <typename F> __device__ __forceinline__ static void run(F &f) {
+        f(std::integral_constant<int, S>{});
+        PlaneUnroll<S + 1, W>::run(f);
+    }
+};
+template <int W> struct PlaneUnroll<W, W> {
+    template <typename F> __device__ __forceinline__ static void run(F &) {}
+};
+
+// The z window holds W >= 9 planes: the 9 the 8th-order z taps of the current plane z read (z - 4 .. z + 4) and
+// W - 9 planes of look-ahead, so the plane a step loads (z + W - 5, into the slot plane z - 5 frees) is first used
+// W - 9 steps later. W is a multiple of 3, the length of the halo / V / R queues (loaded two planes ahead).
+template <typename T, int RPT, int W> struct StRegs {
+    T q[W][RPT];
     T hq[3][3];
     T vq[3][RPT];
     T rq[3][RPT];
 };
 
 template <typename T, int RPT> struct StCtx {
-    const T *xb;
-    T *ob;
-    const T *v;
-    const T *rb;
+    BufRes bx, bo, bv, br;  // x and out of this band, V, R of this band
+    unsigned pb;            // bytes per plane
+    unsigned co[RPT];       // this lane's centre rows (gx, y0 + ty + 4 rr): byte offsets in a plane, or BUF_OOB
+    unsigned ho0, ho1, hox; // its y-halo cells (gx, y0 - 4 + hly0 / hly1) and x-halo cell, or BUF_OOB
     int n;
-    long nn;
     int gx, x0, y0, tx, ty;
-    bool xin;
     int hly0, hly1, hxr, hlx;
     bool has_xh;
     T inv_h2, scale, omega, diag_t, lamb;
-    __device__ bool inb(int xx, int yy, int zz) const {
-        return xx >= 0 && xx < n && yy >= 0 && yy < n && zz >= 0 && zz < n;
-    }
-    __device__ T ld(const T *base, int xx, int yy, int zz) const {
-        return inb(xx, yy, zz) ? base[xx + (long)n * yy + nn * zz] : T(0);
-    }
+    __device__ T ld(BufRes r, unsigned off, int zz) const { return bload<T>(r, off + (unsigned)zz * pb); }
 };
 
-template <typename T, int MODE, int RPT, int SLOT>
-__device__ __forceinline__ void st_side(StRegs<T, RPT> &R, const StCtx<T, RPT> &c, int zz) {
+template <typename T, int MODE, int RPT, int W, int SLOT>
+__device__ __forceinline__ void st_side(StRegs<T, RPT, W> &R, const StCtx<T, RPT> &c, int zz) {
     constexpr int QS = SLOT % 3;
-    R.hq[QS][0] = c.ld(c.xb, c.gx, c.y0 - 4 + c.hly0, zz);
-    R.hq[QS][1] = c.ld(c.xb, c.gx, c.y0 - 4 + c.hly1, zz);
-    R.hq[QS][2] = c.has_xh ? c.ld(c.xb, c.x0 - 4 + c.hlx, c.y0 - 4 + c.hxr, zz) : T(0);
+    R.hq[QS][0] = c.ld(c.bx, c.ho0, zz);
+    R.hq[QS][1] = c.ld(c.bx, c.ho1, zz);
+    R.hq[QS][2] = c.ld(c.bx, c.hox, zz);
 #pragma unroll
     for (int rr = 0; rr < RPT; rr++) {
-        if (MODE != ST_LAP) R.vq[QS][rr] = c.ld(c.v, c.gx, c.y0 + c.ty + 4 * rr, zz);
-        if (MODE == ST_JAC) R.rq[QS][rr] = c.ld(c.rb, c.gx, c.y0 + c.ty + 4 * rr, zz);
+        if (MODE != ST_LAP) R.vq[QS][rr] = c.ld(c.bv, c.co[rr], zz);
+        if (MODE == ST_JAC) R.rq[QS][rr] = c.ld(c.br, c.co[rr], zz);
     }
 }
 
-// one plane z of the stream; S = (z - z0) mod 9
-template <typename T, int MODE, int RPT, int S>
-__device__ __forceinline__ void st_plane(StRegs<T, RPT> &R, const StCtx<T, RPT> &c, T (*tile)[4 * RPT + 8][ST_TX + 8],
-                                         int z, int buf) {
-    constexpr int C0 = (S + 4) % 9;
-    constexpr int NEW = (S + 8) % 9;
+// one plane z of the stream; S = (z - z0) mod W
+template <typename T, int MODE, int RPT, int W, int S>
+__device__ __forceinline__ void st_plane(StRegs<T, RPT, W> &R, const StCtx<T, RPT> &c,
+                                         T (*tile)[4 * RPT + 8][ST_TX + 8], int z, int buf) {
+    constexpr int C0 = (S + 4) % W;
+    constexpr int NEW = (S + W - 1) % W;
     constexpr int HS = S % 3;
 #pragma unroll
-    for (int rr = 0; rr < RPT; rr++) R.q[NEW][rr] = c.ld(c.xb, c.gx, c.y0 + c.ty + 4 * rr, z + 4);
-    st_side<T, MODE, RPT, S + 2>(R, c, z + 2);
+    for (int rr = 0; rr < RPT; rr++) R.q[NEW][rr] = c.ld(c.bx, c.co[rr], z + W - 5);
+    st_side<T, MODE, RPT, W, S + 2>(R, c, z + 2);
     T(*tl)[ST_TX + 8] = tile[buf];
 #pragma unroll
     for (int rr = 0; rr < RPT; rr++) tl[c.ty + 4 * rr + 4][c.tx + 4] = R.q[C0][rr];
@@ -162,30 +202,29 @@ __device__ __forceinline__ void st_plane(StRegs<T, RPT> &R, const StCtx<T, RPT> 
                T(C3) * ((tl[ly][lx + 3] - ce) + (tl[ly][lx - 3] - ce)) + T(C4) * ((tl[ly][lx + 4] - ce) + (tl[ly][lx - 4] - ce));
         T ay = T(C1) * ((tl[ly + 1][lx] - ce) + (tl[ly - 1][lx] - ce)) + T(C2) * ((tl[ly + 2][lx] - ce) + (tl[ly - 2][lx] - ce)) +
                T(C3) * ((tl[ly + 3][lx] - ce) + (tl[ly - 3][lx] - ce)) + T(C4) * ((tl[ly + 4][lx] - ce) + (tl[ly - 4][lx] - ce));
-        T az = T(C1) * ((R.q[(S + 5) % 9][rr] - ce) + (R.q[(S + 3) % 9][rr] - ce)) +
-               T(C2) * ((R.q[(S + 6) % 9][rr] - ce) + (R.q[(S + 2) % 9][rr] - ce)) +
-               T(C3) * ((R.q[(S + 7) % 9][rr] - ce) + (R.q[(S + 1) % 9][rr] - ce)) +
-               T(C4) * ((R.q[(S + 8) % 9][rr] - ce) + (R.q[S % 9][rr] - ce));
+        T az = T(C1) * ((R.q[(S + 5) % W][rr] - ce) + (R.q[(S + 3) % W][rr] - ce)) +
+               T(C2) * ((R.q[(S + 6) % W][rr] - ce) + (R.q[(S + 2) % W][rr] - ce)) +
+               T(C3) * ((R.q[(S + 7) % W][rr] - ce) + (R.q[(S + 1) % W][rr] - ce)) +
+               T(C4) * ((R.q[(S + 8) % W][rr] - ce) + (R.q[S % W][rr] - ce));
         const T lap = ((ax + ay) + az) * c.inv_h2;
-        const int gy = c.y0 + c.ty + 4 * rr;
-        if (c.xin && gy < c.n) {
-            const long id = c.gx + (long)c.n * gy + c.nn * z;
-            if (MODE == ST_LAP) {
-                c.ob[id] = c.scale * lap;
-            } else if (MODE == ST_HAM) {
-                c.ob[id] = T(-0.5) * lap + R.vq[HS][rr] * ce;
-            } else {
-                const T vv = R.vq[HS][rr];
-                const T hw = T(-0.5) * lap + vv * ce - c.lamb * ce;
-                T d = c.diag_t + vv - c.lamb;
-                d = d > T(0.1) ? d : T(0.1);
-                c.ob[id] = ce + c.omega / d * (R.rq[HS][rr] - hw);
-            }
+        T o;
+        if (MODE == ST_LAP) {
+            o = c.scale * lap;
+        } else if (MODE == ST_HAM) {
+            o = T(-0.5) * lap + R.vq[HS][rr] * ce;
+        } else {
+            const T vv = R.vq[HS][rr];
+            const T hw = T(-0.5) * lap + vv * ce - c.lamb * ce;
+            T d = c.diag_t + vv - c.lamb;
+            d = d > T(0.1) ? d : T(0.1);
+            o = ce + c.omega / d * (R.rq[HS][rr] - hw);
         }
+        // a row outside the box has offset BUF_OOB: its store is dropped
+        bstore_nt<T>(o, c.bo, c.co[rr] + (unsigned)z * c.pb);
     }
 }
 
-template <typename T, int MODE, int RPT>
+template <typename T, int MODE, int RPT, int W>
 __global__ __launch_bounds__(256) void k_stencil(const T *__restrict__ x, T *__restrict__ out, const T *__restrict__ v,
                                                  const T *__restrict__ r, const double *__restrict__ lam, int n,
                                                  long N, int zc, int nb, int order, T inv_h2, T scale, T omega,
@@ -204,15 +243,17 @@ __global__ __launch_bounds__(256) void k_stencil(const T *__restrict__ x, T *__r
     const int band = wk.band;
     const int z0 = wk.zchunk * zc;
     const int z1 = min(n, z0 + zc);
-    c.xb = x + (long)band * N;
-    c.ob = out + (long)band * N;
-    c.v = v;
-    c.rb = MODE == ST_JAC ? r + (long)band * N : nullptr;
+    const long fb = N * (long)sizeof(T);
+    c.bx = field_res(x + (long)band * N, fb);
+    c.bo = field_res(out + (long)band * N, fb);
+    c.bv = field_res(v, MODE == ST_LAP ? 0 : fb);
+    c.br = field_res(MODE == ST_JAC ? r + (long)band * N : x, MODE == ST_JAC ? fb : 0);
     c.lamb = MODE == ST_JAC ? (T)lam[band] : T(0);
     c.n = n;
-    c.nn = (long)n * n;
+    c.pb = (unsigned)((long)n * n * (long)sizeof(T));
     c.gx = c.x0 + c.tx;
-    c.xin = c.gx < n;
+#pragma unroll
+    for (int rr = 0; rr < RPT; rr++) c.co[rr] = row_off(c.gx, c.y0 + c.ty + 4 * rr, n, (int)sizeof(T));
     const int hy0 = t >> 6, hy1 = (t + 256) >> 6;
     c.hly0 = hy0 < 4 ? hy0 : TY + hy0;
     c.hly1 = hy1 < 4 ? hy1 : TY + hy1;
@@ -220,44 +261,40 @@ __global__ __launch_bounds__(256) void k_stencil(const T *__restrict__ x, T *__r
     c.hxr = t >> 3;
     const int hxc = t & 7;
     c.hlx = hxc < 4 ? hxc : ST_TX + hxc;
+    c.ho0 = row_off(c.gx, c.y0 - 4 + c.hly0, n, (int)sizeof(T));
+    c.ho1 = row_off(c.gx, c.y0 - 4 + c.hly1, n, (int)sizeof(T));
+    c.hox = c.has_xh ? row_off(c.x0 - 4 + c.hlx, c.y0 - 4 + c.hxr, n, (int)sizeof(T)) : BUF_OOB;
     c.inv_h2 = inv_h2;
     c.scale = scale;
     c.omega = omega;
     c.diag_t = diag_t;
 
-    StRegs<T, RPT> R;
+    StRegs<T, RPT, W> R;
 #pragma unroll
-    for (int d = 0; d < 8; d++)
+    for (int d = 0; d < W - 1; d++)
 #pragma unroll
-        for (int rr = 0; rr < RPT; rr++) R.q[d][rr] = c.ld(c.xb, c.gx, c.y0 + c.ty + 4 * rr, z0 - 4 + d);
-    st_side<T, MODE, RPT, 0>(R, c, z0);
-    st_side<T, MODE, RPT, 1>(R, c, z0 + 1);
+        for (int rr = 0; rr < RPT; rr++) R.q[d][rr] = c.ld(c.bx, c.co[rr], z0 - 4 + d);
+    st_side<T, MODE, RPT, W, 0>(R, c, z0);
+    st_side<T, MODE, RPT, W, 1>(R, c, z0 + 1);
 
     // consecutive planes alternate the two LDS buffers (plane p uses buffer (p - z0) & 1): one barrier per plane then
     // separates every write of a buffer from the previous reads of it
     int z = z0;
-    for (; z + 9 <= z1; z += 9) {
+    for (; z + W <= z1; z += W) {
         const int b = (z - z0) & 1;
-        st_plane<T, MODE, RPT, 0>(R, c, tile, z + 0, b);
-        st_plane<T, MODE, RPT, 1>(R, c, tile, z + 1, b ^ 1);
-        st_plane<T, MODE, RPT, 2>(R, c, tile, z + 2, b);
-        st_plane<T, MODE, RPT, 3>(R, c, tile, z + 3, b ^ 1);
-        st_plane<T, MODE, RPT, 4>(R, c, tile, z + 4, b);
-        st_plane<T, MODE, RPT, 5>(R, c, tile, z + 5, b ^ 1);
-        st_plane<T, MODE, RPT, 6>(R, c, tile, z + 6, b);
-        st_plane<T, MODE, RPT, 7>(R, c, tile, z + 7, b ^ 1);
-        st_plane<T, MODE, RPT, 8>(R, c, tile, z + 8, b);
+        auto step = [&](auto sc) {
+            constexpr int S = decltype(sc)::value;
+            st_plane<T, MODE, RPT, W, S>(R, c, tile, z + S, b ^ (S & 1));
+        };
+        PlaneUnroll<0, W>::run(step);
     }
-    // remainder (< 9 planes) with the same slot sequence
+    // remainder (< W planes) with the same slot sequence
     const int b = (z - z0) & 1;
-    if (z < z1) st_plane<T, MODE, RPT, 0>(R, c, tile, z + 0, b);
-    if (z + 1 < z1) st_plane<T, MODE, RPT, 1>(R, c, tile, z + 1, b ^ 1);
-    if (z + 2 < z1) st_plane<T, MODE, RPT, 2>(R, c, tile, z + 2, b);
-    if (z + 3 < z1) st_plane<T, MODE, RPT, 3>(R, c, tile, z + 3, b ^ 1);
-    if (z + 4 < z1) st_plane<T, MODE, RPT, 4>(R, c, tile, z + 4, b);
-    if (z + 5 < z1) st_plane<T, MODE, RPT, 5>(R, c, tile, z + 5, b ^ 1);
-    if (z + 6 < z1) st_plane<T, MODE, RPT, 6>(R, c, tile, z + 6, b);
-    if (z + 7 < z1) st_plane<T, MODE, RPT, 7>(R, c, tile, z + 7, b ^ 1);
+    auto tail = [&](auto sc) {
+        constexpr int S = decltype(sc)::value;
+        if (z + S < z1) st_plane<T, MODE, RPT, W, S>(R, c, tile, z + S, b ^ (S & 1));
+    };
+    PlaneUnroll<0, W - 1>::run(tail);
 }
 
 // ---- vectorised stencil: each lane owns VX consecutive x points (16-byte loads and stores: float4 / double2) ----
@@ -274,26 +311,18 @@ template <> struct Vec16<double> {
     static constexpr int N = 2;
 };
 
-template <typename T, int RPT> struct SvRegs {
+template <typename T, int RPT, int W> struct SvRegs {
     typedef typename Vec16<T>::type V;
-    V q[9][RPT];
+    V q[W][RPT];
     V hq[3];
     V hq2[3];  // second x-halo vector when a vector holds fewer than the 4 halo points (double2)
     V vq[3][RPT];
     V rq[3][RPT];
 };
 
-// 16-byte nontemporal stores (the stencil output streams out; it should not evict the inputs still in use)
+// native 16-byte vectors for the buffer intrinsics
 typedef float lira_nf4 __attribute__((ext_vector_type(4)));
 typedef double lira_nd2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ void nt_store(float4 *p, const float4 &v) {
-    const lira_nf4 t = {v.x, v.y, v.z, v.w};
-    __builtin_nontemporal_store(t, reinterpret_cast<lira_nf4 *>(p));
-}
-__device__ __forceinline__ void nt_store(double2 *p, const double2 &v) {
-    const lira_nd2 t = {v.x, v.y};
-    __builtin_nontemporal_store(t, reinterpret_cast<lira_nd2 *>(p));
-}
 
 template <typename T> __device__ __forceinline__ T vget(const typename Vec16<T>::type &v, int i);
 template <> __device__ __forceinline__ float vget<float>(const float4 &v, int i) {
@@ -307,53 +336,65 @@ template <> __device__ __forceinline__ void vset<float>(float4 &v, int i, float 
 template <> __device__ __forceinline__ void vset<double>(double2 &v, int i, double x) {
     if (i == 0) v.x = x; else v.y = x;
 }
+// 16-byte buffer load / nontemporal store of one vector (float4 or double2)
+template <typename T> __device__ __forceinline__ typename Vec16<T>::type bloadv(BufRes r, unsigned off);
+template <> __device__ __forceinline__ float4 bloadv<float>(BufRes r, unsigned off) {
+    const lira_nf4 t = __builtin_bit_cast(lira_nf4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    return make_float4(t.x, t.y, t.z, t.w);
+}
+template <> __device__ __forceinline__ double2 bloadv<double>(BufRes r, unsigned off) {
+    const lira_nd2 t = __builtin_bit_cast(lira_nd2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    return make_double2(t.x, t.y);
+}
+__device__ __forceinline__ void bstorev_nt(const float4 &v, BufRes r, unsigned off) {
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    const lira_nf4 t = {v.x, v.y, v.z, v.w};
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, t), r, off, 0, 2);
+}
+__device__ __forceinline__ void bstorev_nt(const double2 &v, BufRes r, unsigned off) {
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    const lira_nd2 t = {v.x, v.y};
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, t), r, off, 0, 2);
+}
 
 template <typename T, int RPT> struct SvCtx {
     typedef typename Vec16<T>::type V;
     static constexpr int VX = Vec16<T>::N;
-    const T *xb;
-    T *ob;
-    const T *v;
-    const T *rb;
+    BufRes bx, bo, bv, br;
+    unsigned pb;                // bytes per plane
+    unsigned co[RPT];           // centre rows (gx, y0 + ty + 16 rr): byte offsets in a plane, or BUF_OOB
+    unsigned ho, ho2;           // this lane's halo vector(s), or BUF_OOB
     int n;
-    long nn;
     int x0, y0, tx, ty, gx;
-    bool xin;
-    int h_ly, h_lx, h_gx, h_gy;  // this lane's halo vector (LDS row/col, global x/y), if any
-    bool has_h, has_h2;          // has_h2: an x-halo lane that also loads the vector at h_gx + VX
+    int h_ly, h_lx;             // this lane's halo vector (LDS row/col), if any
+    bool has_h, has_h2;         // has_h2: an x-halo lane that also loads the vector at h_gx + VX
     T inv_h2, scale, omega, diag_t, lamb;
-    __device__ V ldv(const T *base, int xx, int yy, int zz) const {
-        if (xx >= 0 && xx < n && yy >= 0 && yy < n && zz >= 0 && zz < n)
-            return *reinterpret_cast<const V *>(base + xx + (long)n * yy + nn * zz);
-        V z0;
-        for (int i = 0; i < VX; i++) vset<T>(z0, i, T(0));
-        return z0;
-    }
+    __device__ V ldv(BufRes r, unsigned off, int zz) const { return bloadv<T>(r, off + (unsigned)zz * pb); }
 };
 
-template <typename T, int MODE, int RPT, int SLOT>
-__device__ __forceinline__ void sv_side(SvRegs<T, RPT> &R, const SvCtx<T, RPT> &c, int zz) {
+template <typename T, int MODE, int RPT, int W, int SLOT>
+__device__ __forceinline__ void sv_side(SvRegs<T, RPT, W> &R, const SvCtx<T, RPT> &c, int zz) {
     constexpr int QS = SLOT % 3;
-    if (c.has_h) R.hq[QS] = c.ldv(c.xb, c.h_gx, c.h_gy, zz);
-    if (SvCtx<T, RPT>::VX < 4 && c.has_h2) R.hq2[QS] = c.ldv(c.xb, c.h_gx + SvCtx<T, RPT>::VX, c.h_gy, zz);
+    R.hq[QS] = c.ldv(c.bx, c.ho, zz);
+    if (SvCtx<T, RPT>::VX < 4) R.hq2[QS] = c.ldv(c.bx, c.ho2, zz);
 #pragma unroll
     for (int rr = 0; rr < RPT; rr++) {
-        if (MODE != ST_LAP) R.vq[QS][rr] = c.ldv(c.v, c.gx, c.y0 + c.ty + 16 * rr, zz);
-        if (MODE == ST_JAC) R.rq[QS][rr] = c.ldv(c.rb, c.gx, c.y0 + c.ty + 16 * rr, zz);
+        if (MODE != ST_LAP) R.vq[QS][rr] = c.ldv(c.bv, c.co[rr], zz);
+        if (MODE == ST_JAC) R.rq[QS][rr] = c.ldv(c.br, c.co[rr], zz);
     }
 }
 
-template <typename T, int MODE, int RPT, int S, int LXP>
-__device__ __forceinline__ void sv_plane(SvRegs<T, RPT> &R, const SvCtx<T, RPT> &c, T (*tile)[16 * RPT + 8][LXP],
+template <typename T, int MODE, int RPT, int W, int S, int LXP>
+__device__ __forceinline__ void sv_plane(SvRegs<T, RPT, W> &R, const SvCtx<T, RPT> &c, T (*tile)[16 * RPT + 8][LXP],
                                          int z, int buf) {
     typedef typename Vec16<T>::type V;
     constexpr int VX = Vec16<T>::N;
-    constexpr int C0 = (S + 4) % 9;
-    constexpr int NEW = (S + 8) % 9;
+    constexpr int C0 = (S + 4) % W;
+    constexpr int NEW = (S + W - 1) % W;
     constexpr int HS = S % 3;
 #pragma unroll
-    for (int rr = 0; rr < RPT; rr++) R.q[NEW][rr] = c.ldv(c.xb, c.gx, c.y0 + c.ty + 16 * rr, z + 4);
-    sv_side<T, MODE, RPT, S + 2>(R, c, z + 2);
+    for (int rr = 0; rr < RPT; rr++) R.q[NEW][rr] = c.ldv(c.bx, c.co[rr], z + W - 5);
+    sv_side<T, MODE, RPT, W, S + 2>(R, c, z + 2);
     T(*tl)[LXP] = tile[buf];
     const int lx = 4 + VX * c.tx;  // LDS column of this lane's first point
 #pragma unroll
@@ -379,7 +420,6 @@ __device__ __forceinline__ void sv_plane(SvRegs<T, RPT> &R, const SvCtx<T, RPT> 
             yp[m - 1] = *reinterpret_cast<const V *>(&tl[ly + m][lx]);
         }
         V outv;
-        const int gy = c.y0 + c.ty + 16 * rr;
 #pragma unroll
         for (int i = 0; i < VX; i++) {
             const T ce = w[4 + i];
@@ -389,10 +429,10 @@ __device__ __forceinline__ void sv_plane(SvRegs<T, RPT> &R, const SvCtx<T, RPT> 
                          T(C2) * ((vget<T>(yp[1], i) - ce) + (vget<T>(ym[1], i) - ce)) +
                          T(C3) * ((vget<T>(yp[2], i) - ce) + (vget<T>(ym[2], i) - ce)) +
                          T(C4) * ((vget<T>(yp[3], i) - ce) + (vget<T>(ym[3], i) - ce));
-            const T az = T(C1) * ((vget<T>(R.q[(S + 5) % 9][rr], i) - ce) + (vget<T>(R.q[(S + 3) % 9][rr], i) - ce)) +
-                         T(C2) * ((vget<T>(R.q[(S + 6) % 9][rr], i) - ce) + (vget<T>(R.q[(S + 2) % 9][rr], i) - ce)) +
-                         T(C3) * ((vget<T>(R.q[(S + 7) % 9][rr], i) - ce) + (vget<T>(R.q[(S + 1) % 9][rr], i) - ce)) +
-                         T(C4) * ((vget<T>(R.q[(S + 8) % 9][rr], i) - ce) + (vget<T>(R.q[S % 9][rr], i) - ce));
+            const T az = T(C1) * ((vget<T>(R.q[(S + 5) % W][rr], i) - ce) + (vget<T>(R.q[(S + 3) % W][rr], i) - ce)) +
+                         T(C2) * ((vget<T>(R.q[(S + 6) % W][rr], i) - ce) + (vget<T>(R.q[(S + 2) % W][rr], i) - ce)) +
+                         T(C3) * ((vget<T>(R.q[(S + 7) % W][rr], i) - ce) + (vget<T>(R.q[(S + 1) % W][rr], i) - ce)) +
+                         T(C4) * ((vget<T>(R.q[(S + 8) % W][rr], i) - ce) + (vget<T>(R.q[S % W][rr], i) - ce));
             const T lap = ((ax + ay) + az) * c.inv_h2;
             T o;
             if (MODE == ST_LAP) {
@@ -408,17 +448,16 @@ __device__ __forceinline__ void sv_plane(SvRegs<T, RPT> &R, const SvCtx<T, RPT> 
             }
             vset<T>(outv, i, o);
         }
-        // streaming output: a nontemporal store keeps the block's results from evicting the inputs still in use
-        if (c.xin && gy < c.n) nt_store(reinterpret_cast<V *>(c.ob + c.gx + (long)c.n * gy + c.nn * z), outv);
+        // a row or vector outside the box has offset BUF_OOB: its store is dropped
+        bstorev_nt(outv, c.bo, c.co[rr] + (unsigned)z * c.pb);
     }
 }
 
-template <typename T, int MODE, int RPT>
+template <typename T, int MODE, int RPT, int W>
 __global__ __launch_bounds__(256) void k_stencil_vec(const T *__restrict__ x, T *__restrict__ out,
                                                      const T *__restrict__ v, const T *__restrict__ r,
                                                      const double *__restrict__ lam, int n, long N, int zc, int nb,
                                                      int order, T inv_h2, T scale, T omega, T diag_t) {
-    typedef typename Vec16<T>::type V;
     constexpr int VX = Vec16<T>::N;
     constexpr int TX = 16 * VX, TY = 16 * RPT;
     constexpr int LXP = TX + 8;
@@ -435,76 +474,74 @@ __global__ __launch_bounds__(256) void k_stencil_vec(const T *__restrict__ x, T 
     const int band = wk.band;
     const int z0 = wk.zchunk * zc;
     const int z1 = min(n, z0 + zc);
-    c.xb = x + (long)band * N;
-    c.ob = out + (long)band * N;
-    c.v = v;
-    c.rb = MODE == ST_JAC ? r + (long)band * N : nullptr;
+    const long fb = N * (long)sizeof(T);
+    c.bx = field_res(x + (long)band * N, fb);
+    c.bo = field_res(out + (long)band * N, fb);
+    c.bv = field_res(v, MODE == ST_LAP ? 0 : fb);
+    c.br = field_res(MODE == ST_JAC ? r + (long)band * N : x, MODE == ST_JAC ? fb : 0);
     c.lamb = MODE == ST_JAC ? (T)lam[band] : T(0);
     c.n = n;
-    c.nn = (long)n * n;
+    c.pb = (unsigned)((long)n * n * (long)sizeof(T));
     c.gx = c.x0 + VX * c.tx;
-    c.xin = c.gx < n;
+#pragma unroll
+    for (int rr = 0; rr < RPT; rr++) c.co[rr] = row_off(c.gx, c.y0 + c.ty + 16 * rr, n, (int)sizeof(T));
+    int h_gx = 0, h_gy = 0;
     // halo vectors: 8 y-halo rows x 16 vectors (lanes 0..127), then (TY + 8) rows x 2 sides (lanes 128..); a side
     // holds 4 points, i.e. 4 / VX vectors per x-halo lane
     if (t < 128) {
         const int row = t >> 4, col = t & 15;
         c.h_ly = row < 4 ? row : TY + row;
         c.h_lx = 4 + VX * col;
-        c.h_gx = c.x0 + VX * col;
-        c.h_gy = c.y0 - 4 + c.h_ly;
+        h_gx = c.x0 + VX * col;
+        h_gy = c.y0 - 4 + c.h_ly;
         c.has_h = true;
     } else if (t < 128 + 2 * (TY + 8)) {
         const int e = t - 128, row = e >> 1, side = e & 1;
         c.h_ly = row;
         c.h_lx = side ? 4 + TX : 0;
-        c.h_gx = side ? c.x0 + TX : c.x0 - 4;
-        c.h_gy = c.y0 - 4 + row;
+        h_gx = side ? c.x0 + TX : c.x0 - 4;
+        h_gy = c.y0 - 4 + row;
         c.has_h = true;
     } else {
-        c.h_ly = c.h_lx = c.h_gx = c.h_gy = 0;
+        c.h_ly = c.h_lx = 0;
         c.has_h = false;
     }
     c.has_h2 = c.has_h && t >= 128;
+    c.ho = c.has_h ? row_off(h_gx, h_gy, n, (int)sizeof(T)) : BUF_OOB;
+    c.ho2 = c.has_h2 ? row_off(h_gx + VX, h_gy, n, (int)sizeof(T)) : BUF_OOB;
     c.inv_h2 = inv_h2;
     c.scale = scale;
     c.omega = omega;
     c.diag_t = diag_t;
-    SvRegs<T, RPT> R;
+    SvRegs<T, RPT, W> R;
 #pragma unroll
-    for (int d = 0; d < 8; d++)
+    for (int d = 0; d < W - 1; d++)
 #pragma unroll
-        for (int rr = 0; rr < RPT; rr++) R.q[d][rr] = c.ldv(c.xb, c.gx, c.y0 + c.ty + 16 * rr, z0 - 4 + d);
-    sv_side<T, MODE, RPT, 0>(R, c, z0);
-    sv_side<T, MODE, RPT, 1>(R, c, z0 + 1);
+        for (int rr = 0; rr < RPT; rr++) R.q[d][rr] = c.ldv(c.bx, c.co[rr], z0 - 4 + d);
+    sv_side<T, MODE, RPT, W, 0>(R, c, z0);
+    sv_side<T, MODE, RPT, W, 1>(R, c, z0 + 1);
     int z = z0;
-    for (; z + 9 <= z1; z += 9) {
+    for (; z + W <= z1; z += W) {
         const int b = (z - z0) & 1;
-        sv_plane<T, MODE, RPT, 0, LXP>(R, c, tile, z + 0, b);
-        sv_plane<T, MODE, RPT, 1, LXP>(R, c, tile, z + 1, b ^ 1);
-        sv_plane<T, MODE, RPT, 2, LXP>(R, c, tile, z + 2, b);
-        sv_plane<T, MODE, RPT, 3, LXP>(R, c, tile, z + 3, b ^ 1);
-        sv_plane<T, MODE, RPT, 4, LXP>(R, c, tile, z + 4, b);
-        sv_plane<T, MODE, RPT, 5, LXP>(R, c, tile, z + 5, b ^ 1);
-        sv_plane<T, MODE, RPT, 6, LXP>(R, c, tile, z + 6, b);
-        sv_plane<T, MODE, RPT, 7, LXP>(R, c, tile, z + 7, b ^ 1);
-        sv_plane<T, MODE, RPT, 8, LXP>(R, c, tile, z + 8, b);
+        auto step = [&](auto sc) {
+            constexpr int S = decltype(sc)::value;
+            sv_plane<T, MODE, RPT, W, S, LXP>(R, c, tile, z + S, b ^ (S & 1));
+        };
+        PlaneUnroll<0, W>::run(step);
     }
     const int b = (z - z0) & 1;
-    if (z < z1) sv_plane<T, MODE, RPT, 0, LXP>(R, c, tile, z + 0, b);
-    if (z + 1 < z1) sv_plane<T, MODE, RPT, 1, LXP>(R, c, tile, z + 1, b ^ 1);
-    if (z + 2 < z1) sv_plane<T, MODE, RPT, 2, LXP>(R, c, tile, z + 2, b);
-    if (z + 3 < z1) sv_plane<T, MODE, RPT, 3, LXP>(R, c, tile, z + 3, b ^ 1);
-    if (z + 4 < z1) sv_plane<T, MODE, RPT, 4, LXP>(R, c, tile, z + 4, b);
-    if (z + 5 < z1) sv_plane<T, MODE, RPT, 5, LXP>(R, c, tile, z + 5, b ^ 1);
-    if (z + 6 < z1) sv_plane<T, MODE, RPT, 6, LXP>(R, c, tile, z + 6, b);
-    if (z + 7 < z1) sv_plane<T, MODE, RPT, 7, LXP>(R, c, tile, z + 7, b ^ 1);
+    auto tail = [&](auto sc) {
+        constexpr int S = decltype(sc)::value;
+        if (z + S < z1) sv_plane<T, MODE, RPT, W, S, LXP>(R, c, tile, z + S, b ^ (S & 1));
+    };
+    PlaneUnroll<0, W - 1>::run(tail);
 }
 
 // ---- vector lanes carrying NB bands: lane (tx, ty) owns VX x points of row y0 + ty for bands b0 .. b0 + NB - 1 ----
 // V (and R in ST_JAC per band) is loaded once per point for the NB bands; each band has its own LDS plane.
-template <typename T, int NB> struct VbRegs {
+template <typename T, int NB, int W> struct VbRegs {
     typedef typename Vec16<T>::type V;
-    V q[9][NB];
+    V q[W][NB];
     V hq[3][NB];
     V hq2[3][NB];
     V vq[3];
@@ -514,51 +551,43 @@ template <typename T, int NB> struct VbRegs {
 template <typename T, int NB> struct VbCtx {
     typedef typename Vec16<T>::type V;
     static constexpr int VX = Vec16<T>::N;
-    const T *xb[NB];
-    T *ob[NB];
-    const T *rb[NB];
+    BufRes bx[NB], bo[NB], br[NB], bv;
     T lamb[NB];
-    bool bok[NB];
-    const T *v;
+    unsigned so[NB];   // this lane's store offset per band: co, or BUF_OOB for a band past the block
+    unsigned pb;       // bytes per plane
+    unsigned co;       // centre row (gx, gy): byte offset in a plane, or BUF_OOB
+    unsigned ho, ho2;  // halo vector(s), or BUF_OOB
     int n;
-    long nn;
     int x0, y0, tx, ty, gx, gy;
-    bool xin;
-    int h_ly, h_lx, h_gx, h_gy;
+    int h_ly, h_lx;
     bool has_h, has_h2;
     T inv_h2, scale, omega, diag_t;
-    __device__ V ldv(const T *base, int xx, int yy, int zz) const {
-        if (xx >= 0 && xx < n && yy >= 0 && yy < n && zz >= 0 && zz < n)
-            return *reinterpret_cast<const V *>(base + xx + (long)n * yy + nn * zz);
-        V z0;
-        for (int i = 0; i < VX; i++) vset<T>(z0, i, T(0));
-        return z0;
-    }
+    __device__ V ldv(BufRes r, unsigned off, int zz) const { return bloadv<T>(r, off + (unsigned)zz * pb); }
 };
 
-template <typename T, int MODE, int NB, int SLOT>
-__device__ __forceinline__ void vb_side(VbRegs<T, NB> &R, const VbCtx<T, NB> &c, int zz) {
+template <typename T, int MODE, int NB, int W, int SLOT>
+__device__ __forceinline__ void vb_side(VbRegs<T, NB, W> &R, const VbCtx<T, NB> &c, int zz) {
     constexpr int QS = SLOT % 3;
 #pragma unroll
     for (int b = 0; b < NB; b++) {
-        if (c.has_h) R.hq[QS][b] = c.ldv(c.xb[b], c.h_gx, c.h_gy, zz);
-        if (VbCtx<T, NB>::VX < 4 && c.has_h2) R.hq2[QS][b] = c.ldv(c.xb[b], c.h_gx + VbCtx<T, NB>::VX, c.h_gy, zz);
-        if (MODE == ST_JAC) R.rq[QS][b] = c.ldv(c.rb[b], c.gx, c.gy, zz);
+        R.hq[QS][b] = c.ldv(c.bx[b], c.ho, zz);
+        if (VbCtx<T, NB>::VX < 4) R.hq2[QS][b] = c.ldv(c.bx[b], c.ho2, zz);
+        if (MODE == ST_JAC) R.rq[QS][b] = c.ldv(c.br[b], c.co, zz);
     }
-    if (MODE != ST_LAP) R.vq[QS] = c.ldv(c.v, c.gx, c.gy, zz);
+    if (MODE != ST_LAP) R.vq[QS] = c.ldv(c.bv, c.co, zz);
 }
 
-template <typename T, int MODE, int NB, int S, int TYP, int LXP>
-__device__ __forceinline__ void vb_plane(VbRegs<T, NB> &R, const VbCtx<T, NB> &c, T (*tile)[NB][TYP][LXP], int z,
+template <typename T, int MODE, int NB, int W, int S, int TYP, int LXP>
+__device__ __forceinline__ void vb_plane(VbRegs<T, NB, W> &R, const VbCtx<T, NB> &c, T (*tile)[NB][TYP][LXP], int z,
                                          int buf) {
     typedef typename Vec16<T>::type V;
     constexpr int VX = Vec16<T>::N;
-    constexpr int C0 = (S + 4) % 9;
-    constexpr int NEW = (S + 8) % 9;
+    constexpr int C0 = (S + 4) % W;
+    constexpr int NEW = (S + W - 1) % W;
     constexpr int HS = S % 3;
 #pragma unroll
-    for (int b = 0; b < NB; b++) R.q[NEW][b] = c.ldv(c.xb[b], c.gx, c.gy, z + 4);
-    vb_side<T, MODE, NB, S + 2>(R, c, z + 2);
+    for (int b = 0; b < NB; b++) R.q[NEW][b] = c.ldv(c.bx[b], c.co, z + W - 5);
+    vb_side<T, MODE, NB, W, S + 2>(R, c, z + 2);
     const int lx = 4 + VX * c.tx, ly = c.ty + 4;
 #pragma unroll
     for (int b = 0; b < NB; b++) {
@@ -593,10 +622,10 @@ __device__ __forceinline__ void vb_plane(VbRegs<T, NB> &R, const VbCtx<T, NB> &c
                          T(C2) * ((vget<T>(yp[1], i) - ce) + (vget<T>(ym[1], i) - ce)) +
                          T(C3) * ((vget<T>(yp[2], i) - ce) + (vget<T>(ym[2], i) - ce)) +
                          T(C4) * ((vget<T>(yp[3], i) - ce) + (vget<T>(ym[3], i) - ce));
-            const T az = T(C1) * ((vget<T>(R.q[(S + 5) % 9][b], i) - ce) + (vget<T>(R.q[(S + 3) % 9][b], i) - ce)) +
-                         T(C2) * ((vget<T>(R.q[(S + 6) % 9][b], i) - ce) + (vget<T>(R.q[(S + 2) % 9][b], i) - ce)) +
-                         T(C3) * ((vget<T>(R.q[(S + 7) % 9][b], i) - ce) + (vget<T>(R.q[(S + 1) % 9][b], i) - ce)) +
-                         T(C4) * ((vget<T>(R.q[(S + 8) % 9][b], i) - ce) + (vget<T>(R.q[S % 9][b], i) - ce));
+            const T az = T(C1) * ((vget<T>(R.q[(S + 5) % W][b], i) - ce) + (vget<T>(R.q[(S + 3) % W][b], i) - ce)) +
+                         T(C2) * ((vget<T>(R.q[(S + 6) % W][b], i) - ce) + (vget<T>(R.q[(S + 2) % W][b], i) - ce)) +
+                         T(C3) * ((vget<T>(R.q[(S + 7) % W][b], i) - ce) + (vget<T>(R.q[(S + 1) % W][b], i) - ce)) +
+                         T(C4) * ((vget<T>(R.q[(S + 8) % W][b], i) - ce) + (vget<T>(R.q[S % W][b], i) - ce));
             const T lap = ((ax + ay) + az) * c.inv_h2;
             T o;
             if (MODE == ST_LAP) {
@@ -612,11 +641,11 @@ __device__ __forceinline__ void vb_plane(VbRegs<T, NB> &R, const VbCtx<T, NB> &c
             }
             vset<T>(outv, i, o);
         }
-        if (c.bok[b] && c.xin && c.gy < c.n) nt_store(reinterpret_cast<V *>(c.ob[b] + c.gx + (long)c.n * c.gy + c.nn * z), outv);
+        bstorev_nt(outv, c.bo[b], c.so[b] + (unsigned)z * c.pb);
     }
 }
 
-template <typename T, int MODE, int NB>
+template <typename T, int MODE, int NB, int W>
 __global__ __launch_bounds__(256) void k_stencil_vb(const T *__restrict__ x, T *__restrict__ out,
                                                     const T *__restrict__ v, const T *__restrict__ r,
                                                     const double *__restrict__ lam, int n, long N, int zc, int nb,
@@ -637,111 +666,120 @@ __global__ __launch_bounds__(256) void k_stencil_vb(const T *__restrict__ x, T *
     c.y0 = wk.tile_y * TY;
     const int z0 = wk.zchunk * zc;
     const int z1 = min(n, z0 + zc);
+    const long fb = N * (long)sizeof(T);
+    c.n = n;
+    c.pb = (unsigned)((long)n * n * (long)sizeof(T));
+    c.gx = c.x0 + VX * c.tx;
+    c.gy = c.y0 + c.ty;
+    c.co = row_off(c.gx, c.gy, n, (int)sizeof(T));
 #pragma unroll
     for (int b = 0; b < NB; b++) {
         const int band = wk.band * NB + b;
-        c.bok[b] = band < nb;
-        const int bb = c.bok[b] ? band : wk.band * NB;  // a missing band reads a valid one and stores nothing
-        c.xb[b] = x + (long)bb * N;
-        c.ob[b] = out + (long)bb * N;
-        c.rb[b] = MODE == ST_JAC ? r + (long)bb * N : nullptr;
+        const bool bok = band < nb;
+        const int bb = bok ? band : wk.band * NB;  // a missing band reads a valid one and stores nothing
+        c.bx[b] = field_res(x + (long)bb * N, fb);
+        c.bo[b] = field_res(out + (long)bb * N, fb);
+        c.br[b] = field_res(MODE == ST_JAC ? r + (long)bb * N : x, MODE == ST_JAC ? fb : 0);
         c.lamb[b] = MODE == ST_JAC ? (T)lam[bb] : T(0);
+        c.so[b] = bok ? c.co : BUF_OOB;
     }
-    c.v = v;
-    c.n = n;
-    c.nn = (long)n * n;
-    c.gx = c.x0 + VX * c.tx;
-    c.gy = c.y0 + c.ty;
-    c.xin = c.gx < n;
+    c.bv = field_res(v, MODE == ST_LAP ? 0 : fb);
+    int h_gx = 0, h_gy = 0;
     if (t < 128) {
         const int row = t >> 4, col = t & 15;
         c.h_ly = row < 4 ? row : TY + row;
         c.h_lx = 4 + VX * col;
-        c.h_gx = c.x0 + VX * col;
-        c.h_gy = c.y0 - 4 + c.h_ly;
+        h_gx = c.x0 + VX * col;
+        h_gy = c.y0 - 4 + c.h_ly;
         c.has_h = true;
     } else if (t < 128 + 2 * TYP) {
         const int e = t - 128, row = e >> 1, side = e & 1;
         c.h_ly = row;
         c.h_lx = side ? 4 + TX : 0;
-        c.h_gx = side ? c.x0 + TX : c.x0 - 4;
-        c.h_gy = c.y0 - 4 + row;
+        h_gx = side ? c.x0 + TX : c.x0 - 4;
+        h_gy = c.y0 - 4 + row;
         c.has_h = true;
     } else {
-        c.h_ly = c.h_lx = c.h_gx = c.h_gy = 0;
+        c.h_ly = c.h_lx = 0;
         c.has_h = false;
     }
     c.has_h2 = c.has_h && t >= 128;
+    c.ho = c.has_h ? row_off(h_gx, h_gy, n, (int)sizeof(T)) : BUF_OOB;
+    c.ho2 = c.has_h2 ? row_off(h_gx + VX, h_gy, n, (int)sizeof(T)) : BUF_OOB;
     c.inv_h2 = inv_h2;
     c.scale = scale;
     c.omega = omega;
     c.diag_t = diag_t;
-    VbRegs<T, NB> R;
+    VbRegs<T, NB, W> R;
 #pragma unroll
-    for (int d = 0; d < 8; d++)
+    for (int d = 0; d < W - 1; d++)
 #pragma unroll
-        for (int b = 0; b < NB; b++) R.q[d][b] = c.ldv(c.xb[b], c.gx, c.gy, z0 - 4 + d);
-    vb_side<T, MODE, NB, 0>(R, c, z0);
-    vb_side<T, MODE, NB, 1>(R, c, z0 + 1);
+        for (int b = 0; b < NB; b++) R.q[d][b] = c.ldv(c.bx[b], c.co, z0 - 4 + d);
+    vb_side<T, MODE, NB, W, 0>(R, c, z0);
+    vb_side<T, MODE, NB, W, 1>(R, c, z0 + 1);
     int z = z0;
-    for (; z + 9 <= z1; z += 9) {
+    for (; z + W <= z1; z += W) {
         const int b = (z - z0) & 1;
-        vb_plane<T, MODE, NB, 0, TYP, LXP>(R, c, tile, z + 0, b);
-        vb_plane<T, MODE, NB, 1, TYP, LXP>(R, c, tile, z + 1, b ^ 1);
-        vb_plane<T, MODE, NB, 2, TYP, LXP>(R, c, tile, z + 2, b);
-        vb_plane<T, MODE, NB, 3, TYP, LXP>(R, c, tile, z + 3, b ^ 1);
-        vb_plane<T, MODE, NB, 4, TYP, LXP>(R, c, tile, z + 4, b);
-        vb_plane<T, MODE, NB, 5, TYP, LXP>(R, c, tile, z + 5, b ^ 1);
-        vb_plane<T, MODE, NB, 6, TYP, LXP>(R, c, tile, z + 6, b);
-        vb_plane<T, MODE, NB, 7, TYP, LXP>(R, c, tile, z + 7, b ^ 1);
-        vb_plane<T, MODE, NB, 8, TYP, LXP>(R, c, tile, z + 8, b);
+        auto step = [&](auto sc) {
+            constexpr int S = decltype(sc)::value;
+            vb_plane<T, MODE, NB, W, S, TYP, LXP>(R, c, tile, z + S, b ^ (S & 1));
+        };
+        PlaneUnroll<0, W>::run(step);
     }
     const int b = (z - z0) & 1;
-    if (z < z1) vb_plane<T, MODE, NB, 0, TYP, LXP>(R, c, tile, z + 0, b);
-    if (z + 1 < z1) vb_plane<T, MODE, NB, 1, TYP, LXP>(R, c, tile, z + 1, b ^ 1);
-    if (z + 2 < z1) vb_plane<T, MODE, NB, 2, TYP, LXP>(R, c, tile, z + 2, b);
-    if (z + 3 < z1) vb_plane<T, MODE, NB, 3, TYP, LXP>(R, c, tile, z + 3, b ^ 1);
-    if (z + 4 < z1) vb_plane<T, MODE, NB, 4, TYP, LXP>(R, c, tile, z + 4, b);
-    if (z + 5 < z1) vb_plane<T, MODE, NB, 5, TYP, LXP>(R, c, tile, z + 5, b ^ 1);
-    if (z + 6 < z1) vb_plane<T, MODE, NB, 6, TYP, LXP>(R, c, tile, z + 6, b);
-    if (z + 7 < z1) vb_plane<T, MODE, NB, 7, TYP, LXP>(R, c, tile, z + 7, b ^ 1);
+    auto tail = [&](auto sc) {
+        constexpr int S = decltype(sc)::value;
+        if (z + S < z1) vb_plane<T, MODE, NB, W, S, TYP, LXP>(R, c, tile, z + S, b ^ (S & 1));
+    };
+    PlaneUnroll<0, W - 1>::run(tail);
 }
 
 struct StVariant {
     int rpt;
     int zc;     // 0 = whole column
-    int vec;    // 1: 16-byte vector lanes (needs n divisible by the vector width)
+    int vec;    // 1: 16-byte vector lanes (needs n divisible by the vector width); 2: lanes carrying rpt bands
     int order;  // XCD work order: 0 tiles fastest, 1 bands fastest, G >= 2 bands in groups of G fastest
+    int win;    // z window: 9 (no look-ahead beyond the taps) or 12 (three planes of look-ahead)
     const char *name;
 };
 static const StVariant ST_VARIANTS[] = {
-    {4, 0, 0, 0, "64x16 tile, full z column"},
-    {4, 32, 0, 0, "64x16 tile, z chunks of 32"},
-    {4, 16, 0, 0, "64x16 tile, z chunks of 16"},
-    {2, 0, 0, 0, "64x8 tile, full z column"},
-    {2, 16, 0, 0, "64x8 tile, z chunks of 16"},
-    {2, 8, 0, 0, "64x8 tile, z chunks of 8"},
-    {1, 0, 1, 0, "vector lanes, 16 rows, full z column"},
-    {1, 32, 1, 0, "vector lanes, 16 rows, z chunks of 32"},
-    {1, 16, 1, 0, "vector lanes, 16 rows, z chunks of 16"},
-    {2, 0, 1, 0, "vector lanes, 32 rows, full z column"},
-    {2, 32, 1, 0, "vector lanes, 32 rows, z chunks of 32"},
-    {1, 0, 1, 1, "vector lanes, 16 rows, full z column, bands fastest"},
-    {1, 32, 1, 1, "vector lanes, 16 rows, z chunks of 32, bands fastest"},
-    {2, 0, 1, 1, "vector lanes, 32 rows, full z column, bands fastest"},
-    {2, 0, 0, 1, "64x8 tile, full z column, bands fastest"},
-    {2, 16, 0, 1, "64x8 tile, z chunks of 16, bands fastest"},
-    {2, 0, 2, 0, "vector lanes, 2 bands per lane, full z column"},
-    {2, 32, 2, 0, "vector lanes, 2 bands per lane, z chunks of 32"},
-    {2, 0, 2, 1, "vector lanes, 2 bands per lane, full z column, bands fastest"},
-    {4, 0, 2, 0, "vector lanes, 4 bands per lane, full z column"},
-    {1, 0, 1, 8, "vector lanes, 16 rows, full z column, bands in groups of 8"},
-    {1, 0, 1, 4, "vector lanes, 16 rows, full z column, bands in groups of 4"},
-    {2, 0, 1, 8, "vector lanes, 32 rows, full z column, bands in groups of 8"},
-    {1, 32, 1, 8, "vector lanes, 16 rows, z chunks of 32, bands in groups of 8"},
-    {4, 0, 0, 8, "64x16 tile, full z column, bands in groups of 8"},
-    {2, 16, 0, 8, "64x8 tile, z chunks of 16, bands in groups of 8"},
-    {2, 16, 0, 4, "64x8 tile, z chunks of 16, bands in groups of 4"},
+    {4, 0, 0, 0, 9, "64x16 tile, full z column"},
+    {4, 32, 0, 0, 9, "64x16 tile, z chunks of 32"},
+    {4, 16, 0, 0, 9, "64x16 tile, z chunks of 16"},
+    {2, 0, 0, 0, 9, "64x8 tile, full z column"},
+    {2, 16, 0, 0, 9, "64x8 tile, z chunks of 16"},
+    {2, 8, 0, 0, 9, "64x8 tile, z chunks of 8"},
+    {1, 0, 1, 0, 9, "vector lanes, 16 rows, full z column"},
+    {1, 32, 1, 0, 9, "vector lanes, 16 rows, z chunks of 32"},
+    {1, 16, 1, 0, 9, "vector lanes, 16 rows, z chunks of 16"},
+    {2, 0, 1, 0, 9, "vector lanes, 32 rows, full z column"},
+    {2, 32, 1, 0, 9, "vector lanes, 32 rows, z chunks of 32"},
+    {1, 0, 1, 1, 9, "vector lanes, 16 rows, full z column, bands fastest"},
+    {1, 32, 1, 1, 9, "vector lanes, 16 rows, z chunks of 32, bands fastest"},
+    {2, 0, 1, 1, 9, "vector lanes, 32 rows, full z column, bands fastest"},
+    {2, 0, 0, 1, 9, "64x8 tile, full z column, bands fastest"},
+    {2, 16, 0, 1, 9, "64x8 tile, z chunks of 16, bands fastest"},
+    {2, 0, 2, 0, 9, "vector lanes, 2 bands per lane, full z column"},
+    {2, 32, 2, 0, 9, "vector lanes, 2 bands per lane, z chunks of 32"},
+    {2, 0, 2, 1, 9, "vector lanes, 2 bands per lane, full z column, bands fastest"},
+    {4, 0, 2, 0, 9, "vector lanes, 4 bands per lane, full z column"},
+    {1, 0, 1, 8, 9, "vector lanes, 16 rows, full z column, bands in groups of 8"},
+    {1, 0, 1, 4, 9, "vector lanes, 16 rows, full z column, bands in groups of 4"},
+    {2, 0, 1, 8, 9, "vector lanes, 32 rows, full z column, bands in groups of 8"},
+    {1, 32, 1, 8, 9, "vector lanes, 16 rows, z chunks of 32, bands in groups of 8"},
+    {4, 0, 0, 8, 9, "64x16 tile, full z column, bands in groups of 8"},
+    {2, 16, 0, 8, 9, "64x8 tile, z chunks of 16, bands in groups of 8"},
+    {2, 16, 0, 4, 9, "64x8 tile, z chunks of 16, bands in groups of 4"},
+    {4, 0, 0, 0, 12, "64x16 tile, full z column, 12-plane window"},
+    {4, 32, 0, 0, 12, "64x16 tile, z chunks of 32, 12-plane window"},
+    {2, 0, 0, 0, 12, "64x8 tile, full z column, 12-plane window"},
+    {2, 16, 0, 0, 12, "64x8 tile, z chunks of 16, 12-plane window"},
+    {2, 16, 0, 4, 12, "64x8 tile, z chunks of 16, bands in groups of 4, 12-plane window"},
+    {1, 0, 1, 0, 12, "vector lanes, 16 rows, full z column, 12-plane window"},
+    {1, 32, 1, 0, 12, "vector lanes, 16 rows, z chunks of 32, 12-plane window"},
+    {1, 0, 1, 4, 12, "vector lanes, 16 rows, full z column, bands in groups of 4, 12-plane window"},
+    {2, 0, 1, 0, 12, "vector lanes, 32 rows, full z column, 12-plane window"},
+    {2, 0, 2, 0, 12, "vector lanes, 2 bands per lane, full z column, 12-plane window"},
 };
 static int g_default_variant[2] = {-1, -1};
 
@@ -758,7 +796,7 @@ static int pick_variant(int n, int nb) {
     return 2;
 }
 
-template <typename T, int MODE, int RPT>
+template <typename T, int MODE, int RPT, int W>
 static void launch_stencil(const StencilParams &p, int zc, int order, const T *x, T *out, const T *v, const T *r,
                            const double *lam, hipStream_t s) {
     const int n = p.n;
@@ -767,12 +805,12 @@ static void launch_stencil(const StencilParams &p, int zc, int order, const T *x
     const int nzc = (n + zc - 1) / zc;
     const unsigned grid = st_grid((n + ST_TX - 1) / ST_TX, (n + 4 * RPT - 1) / (4 * RPT), nzc, p.nb, order);
     dim3 block(64, 4, 1);
-    hipLaunchKernelGGL((k_stencil<T, MODE, RPT>), dim3(grid), block, 0, s, x, out, v, r, lam, n, N, zc, p.nb, order,
+    hipLaunchKernelGGL((k_stencil<T, MODE, RPT, W>), dim3(grid), block, 0, s, x, out, v, r, lam, n, N, zc, p.nb, order,
                        (T)(1.0 / (p.h * p.h)), (T)p.scale, (T)p.omega, (T)p.diag_t);
     LIRA_HIP(hipGetLastError());
 }
 
-template <typename T, int MODE, int RPT>
+template <typename T, int MODE, int RPT, int W>
 static void launch_stencil_vec(const StencilParams &p, int zc, int order, const T *x, T *out, const T *v, const T *r,
                                const double *lam, hipStream_t s) {
     constexpr int VX = Vec16<T>::N;
@@ -781,12 +819,12 @@ static void launch_stencil_vec(const StencilParams &p, int zc, int order, const 
     if (zc <= 0 || zc > n) zc = n;
     const int nzc = (n + zc - 1) / zc;
     const unsigned grid = st_grid((n + 16 * VX - 1) / (16 * VX), (n + 16 * RPT - 1) / (16 * RPT), nzc, p.nb, order);
-    hipLaunchKernelGGL((k_stencil_vec<T, MODE, RPT>), dim3(grid), dim3(16, 16, 1), 0, s, x, out, v, r, lam, n, N, zc,
+    hipLaunchKernelGGL((k_stencil_vec<T, MODE, RPT, W>), dim3(grid), dim3(16, 16, 1), 0, s, x, out, v, r, lam, n, N, zc,
                        p.nb, order, (T)(1.0 / (p.h * p.h)), (T)p.scale, (T)p.omega, (T)p.diag_t);
     LIRA_HIP(hipGetLastError());
 }
 
-template <typename T, int MODE, int NB>
+template <typename T, int MODE, int NB, int W>
 static void launch_stencil_vb(const StencilParams &p, int zc, int order, const T *x, T *out, const T *v, const T *r,
                               const double *lam, hipStream_t s) {
     constexpr int VX = Vec16<T>::N;
@@ -795,9 +833,55 @@ static void launch_stencil_vb(const StencilParams &p, int zc, int order, const T
     if (zc <= 0 || zc > n) zc = n;
     const int nzc = (n + zc - 1) / zc;
     const unsigned grid = st_grid((n + 16 * VX - 1) / (16 * VX), (n + 15) / 16, nzc, (p.nb + NB - 1) / NB, order);
-    hipLaunchKernelGGL((k_stencil_vb<T, MODE, NB>), dim3(grid), dim3(16, 16, 1), 0, s, x, out, v, r, lam, n, N, zc,
+    hipLaunchKernelGGL((k_stencil_vb<T, MODE, NB, W>), dim3(grid), dim3(16, 16, 1), 0, s, x, out, v, r, lam, n, N, zc,
                        p.nb, order, (T)(1.0 / (p.h * p.h)), (T)p.scale, (T)p.omega, (T)p.diag_t);
     LIRA_HIP(hipGetLastError());
+}
+
+// launch the variant's kernel for window W (4 bands per lane exist only at W = 9: their registers)
+template <typename T, int W>
+static void stencil_dispatch(int mode, const StVariant &sv, const StencilParams &p, const T *x, T *out, const T *v,
+                             const T *r, const double *lam, hipStream_t s) {
+#define LIRA_VB(MODE_, NB_) launch_stencil_vb<T, MODE_, NB_, W>(p, sv.zc, sv.order, x, out, v, r, lam, s)
+#define LIRA_SV(MODE_, RPT_) launch_stencil_vec<T, MODE_, RPT_, W>(p, sv.zc, sv.order, x, out, v, r, lam, s)
+#define LIRA_ST(MODE_, RPT_) launch_stencil<T, MODE_, RPT_, W>(p, sv.zc, sv.order, x, out, v, r, lam, s)
+    if (sv.vec == 2) {
+        bool four = false;
+        if constexpr (W == 9) {
+            if (sv.rpt == 4) {
+                four = true;
+                if (mode == ST_LAP) LIRA_VB(ST_LAP, 4);
+                else if (mode == ST_HAM) LIRA_VB(ST_HAM, 4);
+                else LIRA_VB(ST_JAC, 4);
+            }
+        }
+        if (!four) {
+            if (mode == ST_LAP) LIRA_VB(ST_LAP, 2);
+            else if (mode == ST_HAM) LIRA_VB(ST_HAM, 2);
+            else LIRA_VB(ST_JAC, 2);
+        }
+    } else if (sv.vec) {
+        if (sv.rpt == 1) {
+            if (mode == ST_LAP) LIRA_SV(ST_LAP, 1);
+            else if (mode == ST_HAM) LIRA_SV(ST_HAM, 1);
+            else LIRA_SV(ST_JAC, 1);
+        } else {
+            if (mode == ST_LAP) LIRA_SV(ST_LAP, 2);
+            else if (mode == ST_HAM) LIRA_SV(ST_HAM, 2);
+            else LIRA_SV(ST_JAC, 2);
+        }
+    } else if (sv.rpt == 4) {
+        if (mode == ST_LAP) LIRA_ST(ST_LAP, 4);
+        else if (mode == ST_HAM) LIRA_ST(ST_HAM, 4);
+        else LIRA_ST(ST_JAC, 4);
+    } else {
+        if (mode == ST_LAP) LIRA_ST(ST_LAP, 2);
+        else if (mode == ST_HAM) LIRA_ST(ST_HAM, 2);
+        else LIRA_ST(ST_JAC, 2);
+    }
+#undef LIRA_VB
+#undef LIRA_SV
+#undef LIRA_ST
 }
 
 bool stencil_variant_usable(int variant, int n, int dtype_size) {
@@ -811,6 +895,9 @@ void stencil(int mode, const StencilParams &p, const T *x, T *out, const T *v, c
              hipStream_t s, int variant) {
     if (p.nb <= 0) return;
     if (x == out) throw InvalidError("stencil: output aliases the input");
+    if (!stencil_buffer_ok(p.n, (int)sizeof(T)))
+        throw InvalidError("stencil: grid too large for 32-bit field offsets ((n^3 + 4 n^2) values must stay below "
+                           "2^31 bytes)");
     if (variant < 0) variant = g_default_variant[sizeof(T) == 8 ? 1 : 0];
     if (!stencil_variant_usable(variant, p.n, (int)sizeof(T))) variant = pick_variant(p.n, p.nb);
     if (ST_VARIANTS[variant].vec) {
@@ -819,46 +906,10 @@ void stencil(int mode, const StencilParams &p, const T *x, T *out, const T *v, c
         if (misaligned(x) || misaligned(out) || misaligned(v) || misaligned(r)) variant = pick_variant(p.n, p.nb);
     }
     const StVariant &sv = ST_VARIANTS[variant];
-    if (sv.vec == 2) {
-#define LIRA_VB(MODE_, NB_) launch_stencil_vb<T, MODE_, NB_>(p, sv.zc, sv.order, x, out, v, r, lam, s)
-        if (sv.rpt == 2) {
-            if (mode == ST_LAP) LIRA_VB(ST_LAP, 2);
-            else if (mode == ST_HAM) LIRA_VB(ST_HAM, 2);
-            else LIRA_VB(ST_JAC, 2);
-        } else {
-            if (mode == ST_LAP) LIRA_VB(ST_LAP, 4);
-            else if (mode == ST_HAM) LIRA_VB(ST_HAM, 4);
-            else LIRA_VB(ST_JAC, 4);
-        }
-#undef LIRA_VB
-        return;
-    }
-    if (sv.vec) {
-#define LIRA_SV(MODE_, RPT_) launch_stencil_vec<T, MODE_, RPT_>(p, sv.zc, sv.order, x, out, v, r, lam, s)
-        if (sv.rpt == 1) {
-            if (mode == ST_LAP) LIRA_SV(ST_LAP, 1);
-            else if (mode == ST_HAM) LIRA_SV(ST_HAM, 1);
-            else LIRA_SV(ST_JAC, 1);
-        } else {
-            if (mode == ST_LAP) LIRA_SV(ST_LAP, 2);
-            else if (mode == ST_HAM) LIRA_SV(ST_HAM, 2);
-            else LIRA_SV(ST_JAC, 2);
-        }
-#undef LIRA_SV
-        return;
-    }
-#define LIRA_ST(MODE_, RPT_) launch_stencil<T, MODE_, RPT_>(p, sv.zc, sv.order, x, out, v, r, lam, s)
-    if (sv.rpt == 4) {
-        if (mode == ST_LAP) LIRA_ST(ST_LAP, 4);
-        else if (mode == ST_HAM) LIRA_ST(ST_HAM, 4);
-        else LIRA_ST(ST_JAC, 4);
-    } else {
-        if (mode == ST_LAP) LIRA_ST(ST_LAP, 2);
-        else if (mode == ST_HAM) LIRA_ST(ST_HAM, 2);
-        else LIRA_ST(ST_JAC, 2);
-    }
-#undef LIRA_ST
+    if (sv.win == 12) stencil_dispatch<T, 12>(mode, sv, p, x, out, v, r, lam, s);
+    else stencil_dispatch<T, 9>(mode, sv, p, x, out, v, r, lam, s);
 }
+
 
 // ------------------------------------------------------------------------------------------ reductions ----
 __device__ inline double block_sum256(double v, double *sh) {

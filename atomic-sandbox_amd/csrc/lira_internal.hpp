@@ -103,6 +103,12 @@ void stencil(int mode, const StencilParams &p, const T *x, T *out, const T *v, c
              hipStream_t s, int variant = -1);
 int stencil_variants();
 bool stencil_variant_usable(int variant, int n, int dtype_size);
+// true when an n^3 field of esize-byte values (plus 4 planes either side) fits the stencils' 32-bit buffer offsets:
+// (n^3 + 4 n^2) esize < 2^31 (n <= 644 in f64, 812 in f32)
+inline bool stencil_buffer_ok(int n, int esize) {
+    const long nn = (long)n * n;
+    return ((long)n * nn + 4 * nn) * esize < 0x7fffffffL;
+}
 const char *stencil_variant_name(int v);
 void stencil_set_default_variant(int dtype_size, int variant);
 

@@ -5,6 +5,8 @@
 // (stencil applies, V-cycles, block products) while V_H, the Poisson right-hand side and residual corrections, every
 // reduction, the Rayleigh–Ritz and Cholesky steps, DIIS coefficients and the energy sums are f64 (LIRA_MIXED).
 #include <algorithm>
+#include <array>
+#include <mutex>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -24,6 +26,10 @@ static const double COEFF0 = -205.0 / 72.0;
 static const double COEFF[5] = {-205.0 / 72.0, 8.0 / 5.0, -1.0 / 5.0, 8.0 / 315.0, -1.0 / 560.0};
 
 // qm/src/eigen.rs:218-222: omega = 4 / (3 rho), rho = symbol(pi) / c0
+// stencil variants tuned per (element size, n, bands, mode) in this process
+static std::mutex g_tune_mu;
+static std::map<std::array<int, 4>, int> g_tuned;
+
 static double jacobi_omega() {
     const double symbol_pi = COEFF[0] + 2.0 * (-COEFF[1] + COEFF[2] - COEFF[3] + COEFF[4]);
     return 4.0 / (3.0 * (symbol_pi / COEFF[0]));
@@ -432,29 +438,47 @@ template <typename T> class Solver final : public SolverIface {
         }
     }
     void ensure_channel(Channel<T> &c, int kc);
-    int variant_ = -1;  // the stencil tile variant the autotuner picked for this grid
+    int variant_ham_ = -1, variant_jac_ = -1;  // the stencil tile variants the tuner picked for H.X and Jacobi
     void hamiltonian(const T *v, const T *x, T *out, int nb) {
         StencilParams sp{n_, nb, h_, 1.0, 0.0, 0.0};
-        stencil<T>(ST_HAM, sp, x, out, v, nullptr, nullptr, stream_, variant_);
+        stencil<T>(ST_HAM, sp, x, out, v, nullptr, nullptr, stream_, variant_ham_);
         if (nl_rows_ > 0) nl_apply_block(x, out, nb);  // + V_NL (qm/src/nonlocal.rs:165-174)
     }
-    // times every stencil tile variant on this grid with nb bands (the block size of the solve) and keeps the fastest
     void tune_stencil(int nb) {
-        if (variant_ >= 0 || nb <= 0) return;
+        if (variant_ham_ >= 0 || nb <= 0) return;
+        variant_ham_ = tune_stencil_mode(ST_HAM, nb);
+        variant_jac_ = tune_stencil_mode(ST_JAC, nb);
+    }
+    // times every usable stencil tile variant in one mode on this grid with nb bands (the block size of the solve)
+    // and returns the fastest; the choice is kept per (precision, n, nb, mode) for later solvers in the process
+    int tune_stencil_mode(int mode, int nb) {
+        const std::array<int, 4> key{(int)sizeof(T), n_, nb, mode};
+        {
+            std::lock_guard<std::mutex> g(g_tune_mu);
+            auto it = g_tuned.find(key);
+            if (it != g_tuned.end()) return it->second;
+        }
         DevBuf x((size_t)N_ * nb * sizeof(T)), o((size_t)N_ * nb * sizeof(T)), v((size_t)N_ * sizeof(T));
+        DevBuf lam((size_t)nb * sizeof(double));
         fill<T>(x.template as<T>(), N_ * nb, 0.0, stream_);
         fill<T>(v.template as<T>(), N_, 0.0, stream_);
+        fill<double>(lam.as<double>(), nb, 0.0, stream_);
         hipEvent_t a, b;
         LIRA_HIP(hipEventCreate(&a));
         LIRA_HIP(hipEventCreate(&b));
-        StencilParams sp{n_, nb, h_, 1.0, 0.0, 0.0};
+        StencilParams sp{n_, nb, h_, 1.0, 1.0, 1.0};
+        auto run = [&](int var) {
+            stencil<T>(mode, sp, x.template as<T>(), o.template as<T>(), v.template as<T>(),
+                       mode == ST_JAC ? x.template as<T>() : nullptr, mode == ST_JAC ? lam.as<double>() : nullptr,
+                       stream_, var);
+        };
         float best = 1e30f;
         int pick = 0;
         for (int var = 0; var < stencil_variants(); var++) {
-            stencil<T>(ST_HAM, sp, x.template as<T>(), o.template as<T>(), v.template as<T>(), nullptr, nullptr, stream_, var);
+            if (!stencil_variant_usable(var, n_, (int)sizeof(T))) continue;
+            run(var);
             LIRA_HIP(hipEventRecord(a, stream_));
-            for (int r = 0; r < 3; r++)
-                stencil<T>(ST_HAM, sp, x.template as<T>(), o.template as<T>(), v.template as<T>(), nullptr, nullptr, stream_, var);
+            for (int r = 0; r < 8; r++) run(var);
             LIRA_HIP(hipEventRecord(b, stream_));
             LIRA_HIP(hipEventSynchronize(b));
             float ms = 0.f;
@@ -466,7 +490,9 @@ template <typename T> class Solver final : public SolverIface {
         }
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
-        variant_ = pick;
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tuned[key] = pick;
+        return pick;
     }
     std::vector<double> reduce(RedJobs<T> &jobs, double scale) {
         reduce_jobs<T>(jobs, n_, scale, partials_.as<double>(), dres_.as<double>(), stream_);
@@ -759,7 +785,8 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
         residual<T>(c.R.template as<T>(), c.HX.template as<T>(), X(), dth, k, N_, stream_);
         precond_init<T>(c.W.template as<T>(), c.R.template as<T>(), v, dlam, diag_t, k, N_, stream_);
         for (int s = 0; s < 3; s++) {
-            stencil<T>(ST_JAC, jp, c.W.template as<T>(), c.TMP.template as<T>(), v, c.R.template as<T>(), dlam, stream_, variant_);
+            stencil<T>(ST_JAC, jp, c.W.template as<T>(), c.TMP.template as<T>(), v, c.R.template as<T>(), dlam, stream_,
+                       variant_jac_);
             std::swap(c.W, c.TMP);
         }
         hamiltonian(v, c.W.template as<T>(), c.HW.template as<T>(), k);
@@ -2072,7 +2099,8 @@ void lira_default_settings(double h, lira_settings *out) {
 lira_status lira_solver_create(int n, double h, int precision, lira_solver **out) {
     if (!out) return LIRA_INVALID;
     *out = nullptr;
-    if (n < 3 || n > 2048 || !(h > 0.0) || (precision != LIRA_FP64 && precision != LIRA_MIXED)) {
+    // the f64 Poisson fields bound the grid in both precisions
+    if (n < 3 || !stencil_buffer_ok(n, 8) || !(h > 0.0) || (precision != LIRA_FP64 && precision != LIRA_MIXED)) {
         g_create_error = "lira_solver_create: bad grid or precision";
         return LIRA_INVALID;
     }
@@ -2335,6 +2363,7 @@ int lira_autotune_stencil(int dtype, int n, int nbands, double *ms_out) {
         LIRA_HIP(hipEventCreate(&b));
         StencilParams sp{n, nbands, 0.2, 1.0, 0.0, 0.0};
         for (int var = 0; var < stencil_variants(); var++) {
+            if (!stencil_variant_usable(var, n, dtype)) continue;
             auto run = [&] {
                 if (dtype == 4) stencil<float>(ST_HAM, sp, x.as<float>(), o.as<float>(), v.as<float>(), nullptr, nullptr, nullptr, var);
                 else stencil<double>(ST_HAM, sp, x.as<double>(), o.as<double>(), v.as<double>(), nullptr, nullptr, nullptr, var);

@@ -116,7 +116,8 @@ typedef int (*lira_abort_fn)(void *user);
 void lira_default_settings(double h, lira_settings *out);
 
 /* GpuSolver::new (crates/qm-gpu/src/scf.rs:1083-1101): device buffers for an n^3 grid at spacing h on the current
- * HIP device. precision: lira_precision. */
+ * HIP device. precision: lira_precision. 3 <= n <= 644 (a field plus 4 ghost planes must stay below 2 GiB in f64, the
+ * range of the kernels' 32-bit buffer offsets); LIRA_INVALID otherwise. */
 lira_status lira_solver_create(int n, double h, int precision, lira_solver **out);
 void lira_solver_destroy(lira_solver *s);
 const char *lira_last_error(const lira_solver *s);

@@ -66,6 +66,7 @@ def test_bad_arguments_are_invalid_not_crashes():
     assert lira.lib().lira_solver_create(1, 0.25, 0, C.byref(h)) == lira.INVALID
     assert lira.lib().lira_solver_create(33, -1.0, 0, C.byref(h)) == lira.INVALID
     assert lira.lib().lira_solver_create(33, 0.25, 7, C.byref(h)) == lira.INVALID
+    assert lira.lib().lira_solver_create(645, 0.25, 1, C.byref(h)) == lira.INVALID  # beyond 32-bit field offsets
 
 
 # NIST Standard Reference Database 141 (atomic reference data for electronic structure calculations), spherical
