@@ -60,58 +60,83 @@ static void jacobi_eigen(int m, std::vector<double> &a, std::vector<double> &eva
     }
 }
 
+// sqrt(f^2 + g^2) for the QL rotations: the subspace matrices here are scaled to O(1) (unit-diagonal B, energies in
+// A), far from the overflow std::hypot guards against, and hypot's careful path is the bulk of a rotation's latency
+static inline double norm2(double f, double g) {
+    const double af = std::fabs(f), ag = std::fabs(g);
+    if (af > 1e150 || ag > 1e150 || (af < 1e-150 && ag < 1e-150)) return std::hypot(f, g);
+    return std::sqrt(f * f + g * g);
+}
+
+// dot product of n contiguous values with four interleaved partial sums (a fixed order, so deterministic; the
+// compiler maps the four sums onto vector lanes)
+static inline double dot4(const double *__restrict__ x, const double *__restrict__ y, int n) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int j = 0;
+    for (; j + 4 <= n; j += 4) {
+        s0 += x[j] * y[j];
+        s1 += x[j + 1] * y[j + 1];
+        s2 += x[j + 2] * y[j + 2];
+        s3 += x[j + 3] * y[j + 3];
+    }
+    for (; j < n; j++) s0 += x[j] * y[j];
+    return (s0 + s1) + (s2 + s3);
+}
+
 // Householder reduction to tridiagonal form followed by the implicit-shift QL iteration (Golub & Van Loan §8.3),
-// accumulating the orthogonal transformations. Returns false if the QL iteration does not converge.
+// accumulating the orthogonal transformations. Q is kept column-major (qc[c m + r] = Q[r][c]), so the Householder
+// update and every QL rotation run over contiguous memory. Returns false if the QL iteration does not converge.
 static bool tridiagonal_ql_eigen(int m, std::vector<double> a, std::vector<double> &evals, std::vector<double> &evecs) {
-    std::vector<double> q((size_t)m * m, 0.0), v(m), p(m), w(m), qv(m);
-    for (int i = 0; i < m; i++) q[(size_t)i * m + i] = 1.0;
+    std::vector<double> qcv((size_t)m * m, 0.0), vv_(m), pv(m), wv(m), qvv(m);
+    double *__restrict__ qc = qcv.data();
+    double *__restrict__ A = a.data();
+    double *__restrict__ v = vv_.data();
+    double *__restrict__ p = pv.data();
+    double *__restrict__ w = wv.data();
+    double *__restrict__ qv = qvv.data();
+    for (int i = 0; i < m; i++) qc[(size_t)i * m + i] = 1.0;
     for (int k = 0; k + 2 < m; k++) {
         const int len = m - k - 1;
         double xn = 0.0;
         for (int i = 0; i < len; i++) {
-            v[i] = a[(size_t)(k + 1 + i) * m + k];
+            v[i] = A[(size_t)(k + 1 + i) * m + k];
             xn += v[i] * v[i];
         }
         if (xn == 0.0) continue;
         double alpha = std::sqrt(xn);
         if (v[0] > 0) alpha = -alpha;
         v[0] -= alpha;  // v = x - alpha e1, H x = alpha e1
-        double vv = 0.0;
-        for (int i = 0; i < len; i++) vv += v[i] * v[i];
+        const double vv = dot4(v, v, len);
         if (vv == 0.0) continue;
         const double beta = 2.0 / vv;
         // trailing block B <- H B H: p = beta B v, K = beta/2 v.p, w = p - K v, B -= v w^T + w v^T
-        for (int i = 0; i < len; i++) {
-            double sum = 0.0;
-            const double *row = &a[(size_t)(k + 1 + i) * m + k + 1];
-            for (int j = 0; j < len; j++) sum += row[j] * v[j];
-            p[i] = beta * sum;
-        }
-        double kk = 0.0;
-        for (int i = 0; i < len; i++) kk += v[i] * p[i];
-        kk *= 0.5 * beta;
+        for (int i = 0; i < len; i++) p[i] = beta * dot4(&A[(size_t)(k + 1 + i) * m + k + 1], v, len);
+        const double kk = 0.5 * beta * dot4(v, p, len);
         for (int i = 0; i < len; i++) w[i] = p[i] - kk * v[i];
         for (int i = 0; i < len; i++) {
-            double *row = &a[(size_t)(k + 1 + i) * m + k + 1];
-            for (int j = 0; j < len; j++) row[j] -= v[i] * w[j] + w[i] * v[j];
+            double *__restrict__ row = &A[(size_t)(k + 1 + i) * m + k + 1];
+            const double vi = v[i], wi = w[i];
+            for (int j = 0; j < len; j++) row[j] -= vi * w[j] + wi * v[j];
         }
-        a[(size_t)(k + 1) * m + k] = a[(size_t)k * m + k + 1] = alpha;
-        for (int i = 1; i < len; i++) a[(size_t)(k + 1 + i) * m + k] = a[(size_t)k * m + k + 1 + i] = 0.0;
-        // Q <- Q H on columns k+1..
-        for (int r = 0; r < m; r++) {
-            const double *qr = &q[(size_t)r * m + k + 1];
-            double sum = 0.0;
-            for (int j = 0; j < len; j++) sum += qr[j] * v[j];
-            qv[r] = beta * sum;
+        A[(size_t)(k + 1) * m + k] = A[(size_t)k * m + k + 1] = alpha;
+        for (int i = 1; i < len; i++) A[(size_t)(k + 1 + i) * m + k] = A[(size_t)k * m + k + 1 + i] = 0.0;
+        // Q <- Q H on columns k+1..: qv = beta Q[:, k+1:] v, then Q[:, k+1+j] -= qv v_j
+        for (int r = 0; r < m; r++) qv[r] = 0.0;
+        for (int j = 0; j < len; j++) {
+            const double *__restrict__ col = &qc[(size_t)(k + 1 + j) * m];
+            const double vj = v[j];
+            for (int r = 0; r < m; r++) qv[r] += col[r] * vj;
         }
-        for (int r = 0; r < m; r++) {
-            double *qr = &q[(size_t)r * m + k + 1];
-            for (int j = 0; j < len; j++) qr[j] -= qv[r] * v[j];
+        for (int r = 0; r < m; r++) qv[r] *= beta;
+        for (int j = 0; j < len; j++) {
+            double *__restrict__ col = &qc[(size_t)(k + 1 + j) * m];
+            const double vj = v[j];
+            for (int r = 0; r < m; r++) col[r] -= qv[r] * vj;
         }
     }
     std::vector<double> d(m), e(m, 0.0);
-    for (int i = 0; i < m; i++) d[i] = a[(size_t)i * m + i];
-    for (int i = 0; i + 1 < m; i++) e[i] = a[(size_t)(i + 1) * m + i];
+    for (int i = 0; i < m; i++) d[i] = A[(size_t)i * m + i];
+    for (int i = 0; i + 1 < m; i++) e[i] = A[(size_t)(i + 1) * m + i];
     // implicit QL on (d, e): e[i] couples rows i and i + 1
     const double eps = std::numeric_limits<double>::epsilon();
     for (int l = 0; l < m; l++) {
@@ -124,31 +149,35 @@ static bool tridiagonal_ql_eigen(int m, std::vector<double> a, std::vector<doubl
             if (mm != l) {
                 if (iter++ == 60) return false;
                 double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-                double r = std::hypot(g, 1.0);
+                double r = norm2(g, 1.0);
                 g = d[mm] - d[l] + e[l] / (g + std::copysign(r, g));
                 double s = 1.0, c = 1.0, pp = 0.0;
                 int i;
                 for (i = mm - 1; i >= l; i--) {
                     double f = s * e[i];
                     const double b = c * e[i];
-                    r = std::hypot(f, g);
+                    r = norm2(f, g);
                     e[i + 1] = r;
                     if (r == 0.0) {
                         d[i + 1] -= pp;
                         e[mm] = 0.0;
                         break;
                     }
-                    s = f / r;
-                    c = g / r;
+                    const double inv = 1.0 / r;
+                    s = f * inv;
+                    c = g * inv;
                     g = d[i + 1] - pp;
                     r = (d[i] - g) * s + 2.0 * c * b;
                     pp = s * r;
                     d[i + 1] = g + pp;
                     g = c * r - b;
+                    // rotate columns i and i + 1 of Q
+                    double *__restrict__ q0 = &qc[(size_t)i * m];
+                    double *__restrict__ q1 = &qc[(size_t)(i + 1) * m];
                     for (int k = 0; k < m; k++) {
-                        f = q[(size_t)k * m + i + 1];
-                        q[(size_t)k * m + i + 1] = s * q[(size_t)k * m + i] + c * f;
-                        q[(size_t)k * m + i] = c * q[(size_t)k * m + i] - s * f;
+                        const double fk = q1[k];
+                        q1[k] = s * q0[k] + c * fk;
+                        q0[k] = c * q0[k] - s * fk;
                     }
                 }
                 if (r == 0.0 && i >= l) continue;
@@ -165,7 +194,8 @@ static bool tridiagonal_ql_eigen(int m, std::vector<double> a, std::vector<doubl
     evecs.assign((size_t)m * m, 0.0);
     for (int c = 0; c < m; c++) {
         evals[c] = d[order[c]];
-        for (int r = 0; r < m; r++) evecs[(size_t)r * m + c] = q[(size_t)r * m + order[c]];
+        const double *col = &qc[(size_t)order[c] * m];
+        for (int r = 0; r < m; r++) evecs[(size_t)r * m + c] = col[r];
     }
     return true;
 }
@@ -182,16 +212,21 @@ void sym_eigen(int m, std::vector<double> &a, std::vector<double> &evals, std::v
     if (tridiagonal_ql_eigen(m, a, evals, evecs)) {
         double scale = 1.0;
         for (double x : evals) scale = std::max(scale, 1.0 + std::fabs(x));
-        bool ok = true;
-        for (int c = 0; c < m && ok; c++) {
-            double r2 = 0.0;
-            for (int i = 0; i < m; i++) {
-                double s = -evals[c] * evecs[(size_t)i * m + c];
-                for (int j = 0; j < m; j++) s += a[(size_t)i * m + j] * evecs[(size_t)j * m + c];
-                r2 += s * s;
+        // R = A V - V diag(lambda), row by row with contiguous inner loops; every column's norm against the bound
+        std::vector<double> res((size_t)m * m), col2(m, 0.0);
+        for (int i = 0; i < m; i++) {
+            double *__restrict__ ri = &res[(size_t)i * m];
+            const double *__restrict__ vi = &evecs[(size_t)i * m];
+            for (int c = 0; c < m; c++) ri[c] = -evals[c] * vi[c];
+            for (int j = 0; j < m; j++) {
+                const double aij = a[(size_t)i * m + j];
+                const double *__restrict__ vj = &evecs[(size_t)j * m];
+                for (int c = 0; c < m; c++) ri[c] += aij * vj[c];
             }
-            ok = std::sqrt(r2) <= 1e-11 * scale;
+            for (int c = 0; c < m; c++) col2[c] += ri[c] * ri[c];
         }
+        bool ok = true;
+        for (int c = 0; c < m && ok; c++) ok = std::sqrt(col2[c]) <= 1e-11 * scale;
         if (ok) return;
     }
     jacobi_eigen(m, a, evals, evecs);
@@ -298,11 +333,12 @@ double rayleigh_ritz(int m, const std::vector<double> &A, const std::vector<doub
             for (int c = 0; c < nkeep; c++) at[(size_t)i * nkeep + c] += aik * t[(size_t)k * nkeep + c];
         }
     std::vector<double> ah((size_t)nkeep * nkeep, 0.0);
-    for (int r = 0; r < nkeep; r++)
-        for (int c = 0; c < nkeep; c++) {
-            double s = 0.0;
-            for (int i = 0; i < m; i++) s += t[(size_t)i * nkeep + r] * at[(size_t)i * nkeep + c];
-            ah[(size_t)r * nkeep + c] = s;
+    for (int i = 0; i < m; i++)
+        for (int r = 0; r < nkeep; r++) {
+            const double tir = t[(size_t)i * nkeep + r];
+            const double *__restrict__ ati = &at[(size_t)i * nkeep];
+            double *__restrict__ ahr = &ah[(size_t)r * nkeep];
+            for (int c = 0; c < nkeep; c++) ahr[c] += tir * ati[c];
         }
     for (int r = 0; r < nkeep; r++)
         for (int c = r + 1; c < nkeep; c++) {
@@ -319,12 +355,15 @@ double rayleigh_ritz(int m, const std::vector<double> &A, const std::vector<doub
         alpha = std::max(alpha, std::sqrt(s));
     }
     C.assign((size_t)m * nk, 0.0);
-    for (int i = 0; i < m; i++)
-        for (int c = 0; c < nk; c++) {
-            double s = 0.0;
-            for (int k = 0; k < nkeep; k++) s += t[(size_t)i * nkeep + k] * ya[(size_t)k * nkeep + c];
-            C[(size_t)i * nk + c] = d[i] * s;
+    for (int i = 0; i < m; i++) {
+        double *__restrict__ ci = &C[(size_t)i * nk];
+        for (int k = 0; k < nkeep; k++) {
+            const double tik = t[(size_t)i * nkeep + k];
+            const double *__restrict__ yk = &ya[(size_t)k * nkeep];
+            for (int c = 0; c < nk; c++) ci[c] += tik * yk[c];
         }
+        for (int c = 0; c < nk; c++) ci[c] *= d[i];
+    }
     return alpha;
 }
 

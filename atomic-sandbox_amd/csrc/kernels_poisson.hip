@@ -178,33 +178,25 @@ __device__ inline void a2_parts(const T *x, int i, int j, int k, int n, T g_low,
     diag = T(6) - g;
 }
 
+// the level operators per point (shared by the per-level kernels and the fused coarse V-cycle, so both compute
+// bitwise the same values)
 template <typename T>
-__global__ void k_mg_jacobi(T *dst, const T *x, const T *b, int n, T h2, T omega, T g_low, T g_high) {
-    const long N = (long)n * n * n, nn = (long)n * n;
-    const long p = blockIdx.x * 256L + threadIdx.x;
-    if (p >= N) return;
-    const int k = (int)(p / nn), rem = (int)(p - (long)k * nn), j = rem / n, i = rem - j * n;
+__device__ inline T mg_jacobi_pt(const T *x, const T *b, long p, int i, int j, int k, int n, T h2, T omega, T g_low,
+                                 T g_high) {
     T nb, diag;
     a2_parts(x, i, j, k, n, g_low, g_high, nb, diag);
     const T xq = x[p];
-    dst[p] = xq + (omega / diag) * (h2 * b[p] + nb - diag * xq);
+    return xq + (omega / diag) * (h2 * b[p] + nb - diag * xq);
 }
-
-template <typename T> __global__ void k_mg_residual(T *dst, const T *x, const T *b, int n, T h2, T g_low, T g_high) {
-    const long N = (long)n * n * n, nn = (long)n * n;
-    const long p = blockIdx.x * 256L + threadIdx.x;
-    if (p >= N) return;
-    const int k = (int)(p / nn), rem = (int)(p - (long)k * nn), j = rem / n, i = rem - j * n;
+template <typename T>
+__device__ inline T mg_residual_pt(const T *x, const T *b, long p, int i, int j, int k, int n, T h2, T g_low,
+                                   T g_high) {
     T nb, diag;
     a2_parts(x, i, j, k, n, g_low, g_high, nb, diag);
-    dst[p] = b[p] + (nb - diag * x[p]) / h2;
+    return b[p] + (nb - diag * x[p]) / h2;
 }
-
-template <typename T> __global__ void k_mg_restrict(T *dst, const T *f, int nc, int nf) {
-    const long Nc = (long)nc * nc * nc, ncc = (long)nc * nc;
-    const long p = blockIdx.x * 256L + threadIdx.x;
-    if (p >= Nc) return;
-    const int k = (int)(p / ncc), rem = (int)(p - (long)k * ncc), j = rem / nc, i = rem - j * nc;
+// full weighting onto coarse point (i, j, k) from the fine point (2i + 1, 2j + 1, 2k + 1) and its 26 neighbours
+template <typename T> __device__ inline T mg_restrict_pt(const T *f, int i, int j, int k, int nf) {
     const long nff = (long)nf * nf;
     T s = T(0);
     for (int dz = -1; dz <= 1; dz++) {
@@ -223,14 +215,11 @@ template <typename T> __global__ void k_mg_restrict(T *dst, const T *f, int nc, 
             }
         }
     }
-    dst[p] = s;
+    return s;
 }
-
-template <typename T> __global__ void k_mg_prolong_add(T *fine, const T *c, int nf, int nc) {
-    const long Nf = (long)nf * nf * nf, nff = (long)nf * nf;
-    const long p = blockIdx.x * 256L + threadIdx.x;
-    if (p >= Nf) return;
-    const int k = (int)(p / nff), rem = (int)(p - (long)k * nff), j = rem / nf, i = rem - j * nf;
+// trilinear interpolation of the coarse correction at fine point (i, j, k): odd fine indices sit on a coarse point,
+// even ones halfway between two
+template <typename T> __device__ inline T mg_prolong_pt(const T *c, int i, int j, int k, int nc) {
     const long ncc = (long)nc * nc;
     const int t[3] = {i, j, k};
     int ci[3][2];
@@ -261,7 +250,155 @@ template <typename T> __global__ void k_mg_prolong_add(T *fine, const T *c, int 
                 if (x < 0 || y < 0 || z < 0 || x >= nc || y >= nc || z >= nc) continue;
                 s += wi[0][a] * wi[1][b] * wi[2][d] * c[x + (long)nc * y + ncc * z];
             }
-    fine[p] += s;
+    return s;
+}
+
+template <typename T>
+__global__ void k_mg_jacobi(T *dst, const T *x, const T *b, int n, T h2, T omega, T g_low, T g_high) {
+    const long N = (long)n * n * n, nn = (long)n * n;
+    const long p = blockIdx.x * 256L + threadIdx.x;
+    if (p >= N) return;
+    const int k = (int)(p / nn), rem = (int)(p - (long)k * nn), j = rem / n, i = rem - j * n;
+    dst[p] = mg_jacobi_pt(x, b, p, i, j, k, n, h2, omega, g_low, g_high);
+}
+
+// the first sweep of a level from x = 0 (the zero fill folded in): the Jacobi formula with x and its neighbours 0
+template <typename T> __global__ void k_mg_jacobi0(T *dst, const T *b, int n, T h2, T omega, T g_low, T g_high) {
+    const long N = (long)n * n * n, nn = (long)n * n;
+    const long p = blockIdx.x * 256L + threadIdx.x;
+    if (p >= N) return;
+    const int k = (int)(p / nn), rem = (int)(p - (long)k * nn), j = rem / n, i = rem - j * n;
+    T g = T(0);
+    if (i == 0) g += g_low;
+    if (i == n - 1) g += g_high;
+    if (j == 0) g += g_low;
+    if (j == n - 1) g += g_high;
+    if (k == 0) g += g_low;
+    if (k == n - 1) g += g_high;
+    const T diag = T(6) - g;
+    dst[p] = T(0) + (omega / diag) * (h2 * b[p] + T(0) - diag * T(0));
+}
+
+template <typename T> __global__ void k_mg_residual(T *dst, const T *x, const T *b, int n, T h2, T g_low, T g_high) {
+    const long N = (long)n * n * n, nn = (long)n * n;
+    const long p = blockIdx.x * 256L + threadIdx.x;
+    if (p >= N) return;
+    const int k = (int)(p / nn), rem = (int)(p - (long)k * nn), j = rem / n, i = rem - j * n;
+    dst[p] = mg_residual_pt(x, b, p, i, j, k, n, h2, g_low, g_high);
+}
+
+template <typename T> __global__ void k_mg_restrict(T *dst, const T *f, int nc, int nf) {
+    const long Nc = (long)nc * nc * nc, ncc = (long)nc * nc;
+    const long p = blockIdx.x * 256L + threadIdx.x;
+    if (p >= Nc) return;
+    const int k = (int)(p / ncc), rem = (int)(p - (long)k * ncc), j = rem / nc, i = rem - j * nc;
+    dst[p] = mg_restrict_pt(f, i, j, k, nf);
+}
+
+template <typename T> __global__ void k_mg_prolong_add(T *fine, const T *c, int nf, int nc) {
+    const long Nf = (long)nf * nf * nf, nff = (long)nf * nf;
+    const long p = blockIdx.x * 256L + threadIdx.x;
+    if (p >= Nf) return;
+    const int k = (int)(p / nff), rem = (int)(p - (long)k * nff), j = rem / nf, i = rem - j * nf;
+    fine[p] += mg_prolong_pt(c, i, j, k, nc);
+}
+
+// The V-cycle from a small level down, in one workgroup with every level's x, b and scratch in LDS: the same
+// sequence as the per-level kernels (zero fill, pre-smoothing, residual, restriction, ..., the coarsest sweeps,
+// prolongation, post-smoothing), each pass a loop over the level's points followed by a barrier. Replaces some 25
+// launches per V-cycle whose fixed dispatch cost dominates these tiny grids.
+template <typename T> __global__ __launch_bounds__(1024) void k_mg_coarse(T *x_out, const T *b_in, MgCoarse a) {
+    extern __shared__ __align__(16) unsigned char mg_smem[];
+    T *lds = reinterpret_cast<T *>(mg_smem);
+    const int tid = threadIdx.x;
+    const T omega = (T)a.omega;
+    auto level = [&](int l, T *&x, T *&b, T *&tmp) {
+        const int N = a.n[l] * a.n[l] * a.n[l];
+        x = lds + a.off[l];
+        b = x + N;
+        tmp = b + N;
+    };
+    auto decode = [](int p, int n, int &i, int &j, int &k) {
+        k = p / (n * n);
+        const int rem = p - k * n * n;
+        j = rem / n;
+        i = rem - j * n;
+    };
+    auto smooth = [&](int l, int sweeps) {
+        T *x, *b, *tmp;
+        level(l, x, b, tmp);
+        const int n = a.n[l], N = n * n * n;
+        const T h2 = (T)a.h2[l], gl = (T)a.gl[l], gh = (T)a.gh[l];
+        for (int sw = 0; sw < sweeps; sw += 2) {
+            for (int p = tid; p < N; p += 1024) {
+                int i, j, k;
+                decode(p, n, i, j, k);
+                tmp[p] = mg_jacobi_pt<T>(x, b, p, i, j, k, n, h2, omega, gl, gh);
+            }
+            __syncthreads();
+            for (int p = tid; p < N; p += 1024) {
+                int i, j, k;
+                decode(p, n, i, j, k);
+                x[p] = mg_jacobi_pt<T>(tmp, b, p, i, j, k, n, h2, omega, gl, gh);
+            }
+            __syncthreads();
+        }
+    };
+    {
+        T *x, *b, *tmp;
+        level(0, x, b, tmp);
+        const int N = a.n[0] * a.n[0] * a.n[0];
+        for (int p = tid; p < N; p += 1024) b[p] = b_in[p];
+    }
+    __syncthreads();
+    const int L = a.levels;
+    for (int l = 0; l < L; l++) {
+        T *x, *b, *tmp;
+        level(l, x, b, tmp);
+        const int n = a.n[l], N = n * n * n;
+        for (int p = tid; p < N; p += 1024) x[p] = T(0);
+        __syncthreads();
+        if (l == L - 1) {
+            smooth(l, a.coarse);
+            break;
+        }
+        smooth(l, a.pre);
+        const T h2 = (T)a.h2[l], gl = (T)a.gl[l], gh = (T)a.gh[l];
+        for (int p = tid; p < N; p += 1024) {
+            int i, j, k;
+            decode(p, n, i, j, k);
+            tmp[p] = mg_residual_pt<T>(x, b, p, i, j, k, n, h2, gl, gh);
+        }
+        __syncthreads();
+        T *xc, *bc, *tc;
+        level(l + 1, xc, bc, tc);
+        const int nc = a.n[l + 1], Nc = nc * nc * nc;
+        for (int p = tid; p < Nc; p += 1024) {
+            int i, j, k;
+            decode(p, nc, i, j, k);
+            bc[p] = mg_restrict_pt<T>(tmp, i, j, k, n);
+        }
+        __syncthreads();
+    }
+    for (int l = L - 2; l >= 0; l--) {
+        T *x, *b, *tmp, *xc, *bc, *tc;
+        level(l, x, b, tmp);
+        level(l + 1, xc, bc, tc);
+        const int n = a.n[l], N = n * n * n;
+        for (int p = tid; p < N; p += 1024) {
+            int i, j, k;
+            decode(p, n, i, j, k);
+            x[p] += mg_prolong_pt<T>(xc, i, j, k, a.n[l + 1]);
+        }
+        __syncthreads();
+        smooth(l, a.post);
+    }
+    {
+        T *x, *b, *tmp;
+        level(0, x, b, tmp);
+        const int N = a.n[0] * a.n[0] * a.n[0];
+        for (int p = tid; p < N; p += 1024) x_out[p] = x[p];
+    }
 }
 
 #define LIRA_G1(N) dim3((unsigned)(((N) + 255) / 256)), dim3(256)
@@ -287,6 +424,22 @@ template <typename T> void mg_restrict(T *dst, const T *fine, int nc, int nf, hi
 template <typename T> void mg_prolong_add(T *fine, const T *coarse, int nf, int nc, hipStream_t s) {
     const long N = (long)nf * nf * nf;
     hipLaunchKernelGGL(k_mg_prolong_add<T>, LIRA_G1(N), 0, s, fine, coarse, nf, nc);
+    LIRA_HIP(hipGetLastError());
+}
+template <typename T>
+void mg_jacobi0(T *dst, const T *b, int n, double h2, double omega, double g_low, double g_high, hipStream_t s) {
+    const long N = (long)n * n * n;
+    hipLaunchKernelGGL(k_mg_jacobi0<T>, LIRA_G1(N), 0, s, dst, b, n, (T)h2, (T)omega, (T)g_low, (T)g_high);
+    LIRA_HIP(hipGetLastError());
+}
+// LDS bytes of the fused V-cycle over the levels of a (x, b and scratch per level)
+size_t mg_coarse_bytes(const MgCoarse &a, int esize) {
+    size_t e = 0;
+    for (int l = 0; l < a.levels; l++) e += 3 * (size_t)a.n[l] * a.n[l] * a.n[l];
+    return e * esize;
+}
+template <typename T> void mg_coarse(T *x, const T *b, const MgCoarse &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(1024), mg_coarse_bytes(a, (int)sizeof(T)), s, x, b, a);
     LIRA_HIP(hipGetLastError());
 }
 
@@ -332,6 +485,8 @@ template <typename T> void add_into_f64(double *x, const T *e, long N, hipStream
     template void mg_residual<T>(T *, const T *, const T *, int, double, double, double, hipStream_t);        \
     template void mg_restrict<T>(T *, const T *, int, int, hipStream_t);                                      \
     template void mg_prolong_add<T>(T *, const T *, int, int, hipStream_t);                                   \
+    template void mg_jacobi0<T>(T *, const T *, int, double, double, double, double, hipStream_t);            \
+    template void mg_coarse<T>(T *, const T *, const MgCoarse &, hipStream_t);                                \
     template void cg_direction<T>(T *, const T *, const double *, int, int, long, hipStream_t);               \
     template void add_into_f64<T>(double *, const T *, long, hipStream_t);
 LIRA_PINST(float)

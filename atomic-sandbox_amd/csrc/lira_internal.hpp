@@ -211,6 +211,21 @@ template <typename T>
 void mg_residual(T *dst, const T *x, const T *b, int n, double h2, double g_low, double g_high, hipStream_t s);
 template <typename T> void mg_restrict(T *dst, const T *fine, int nc, int nf, hipStream_t s);
 template <typename T> void mg_prolong_add(T *fine, const T *coarse, int nf, int nc, hipStream_t s);
+template <typename T>
+void mg_jacobi0(T *dst, const T *b, int n, double h2, double omega, double g_low, double g_high, hipStream_t s);
+// the V-cycle from a small level down in one workgroup with every level in LDS (level 0 = the entry level)
+constexpr int MG_MAX_LEVELS = 8;
+struct MgCoarse {
+    int levels;
+    int n[MG_MAX_LEVELS];
+    double h2[MG_MAX_LEVELS], gl[MG_MAX_LEVELS], gh[MG_MAX_LEVELS];
+    int off[MG_MAX_LEVELS];  // element offset of each level's x, b, scratch block in LDS
+    int pre, post, coarse;
+    double omega;
+};
+size_t mg_coarse_bytes(const MgCoarse &a, int esize);
+constexpr size_t MG_COARSE_LDS = 65536;  // the fused levels' LDS budget (the default dynamic LDS limit)
+template <typename T> void mg_coarse(T *x, const T *b, const MgCoarse &a, hipStream_t s);
 // device-side CG scalar updates
 // x += (num/den) p ; r -= (num/den) q    (alpha = sc[ia]/sc[ib])
 template <typename Tv, typename Tr>

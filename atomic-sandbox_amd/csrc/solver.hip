@@ -243,6 +243,7 @@ struct SolverIface {
     virtual void set_density_extrapolation(bool on) = 0;
     virtual void get_profile(lira_profile *out, double *drho, double *etol, int cap, int *n) const = 0;
     virtual void sample(const double *pts, int count, double *out) = 0;
+    virtual void set_graphs(bool on) = 0;
 };
 
 // One pseudised species as the solver keeps it: scalars plus offsets of its tables in the f64 table arena.
@@ -267,6 +268,7 @@ template <typename T> class Solver final : public SolverIface {
     Solver(int n, double h);
     ~Solver() override {
         if (stream_) (void)hipStreamSynchronize(stream_);
+        drop_graphs();
         if (hbuf_) (void)hipHostFree(hbuf_);
         if (up_) (void)hipHostFree(up_);
         if (stream_) (void)hipStreamDestroy(stream_);
@@ -305,8 +307,57 @@ template <typename T> class Solver final : public SolverIface {
         }
     }
     void sample(const double *pts, int count, double *out) override;
+    void set_graphs(bool on) override {
+        if (!on) drop_graphs();
+        graphs_ = on;
+    }
 
   private:
+    // ---- HIP graphs: a fixed kernel sequence between two host syncs is captured once and replayed ----
+    // A slot holds one captured sequence and the key it was captured under: the device pointers and sizes its
+    // kernels were given. A different key (another buffer, a grown block) captures again. Captured bodies hold
+    // kernel launches and device-to-device copies only: no host wait, no host copy, no memset.
+    struct GraphSlot {
+        std::vector<uintptr_t> key;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    bool graphs_ = true;
+    GraphSlot pcg_graph_[2], pcg_start_graph_;
+    static void drop_graph(GraphSlot &g) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+        g.exec = nullptr;
+        g.graph = nullptr;
+        g.key.clear();
+    }
+    void drop_graphs() {
+        for (GraphSlot *g : {&pcg_graph_[0], &pcg_graph_[1], &pcg_start_graph_}) drop_graph(*g);
+    }
+    // runs body's launches on stream_: replayed from g when g was captured under key, else captured into g first
+    template <typename F> void run_graph(GraphSlot &g, const std::vector<uintptr_t> &key, F &&body) {
+        if (!graphs_) {
+            body();
+            return;
+        }
+        if (g.exec == nullptr || g.key != key) {
+            drop_graph(g);
+            LIRA_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            try {
+                body();
+            } catch (...) {
+                hipGraph_t partial = nullptr;
+                (void)hipStreamEndCapture(stream_, &partial);
+                if (partial) (void)hipGraphDestroy(partial);
+                throw;
+            }
+            LIRA_HIP(hipStreamEndCapture(stream_, &g.graph));
+            LIRA_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+            g.key = key;
+        }
+        LIRA_HIP(hipGraphLaunch(g.exec, stream_));
+    }
+
     int n_;
     double h_, dv_;
     long N_;
@@ -325,6 +376,8 @@ template <typename T> class Solver final : public SolverIface {
         DevBuf x, b, res, tmp;
     };
     std::vector<Level> levels_;
+    int mg_entry_ = 0;  // the level from which the V-cycle runs fused in one workgroup (mgc_)
+    MgCoarse mgc_{};
     DevBuf pb_, pr64_, ptmp64_, pr_, pz_, pp_, pq_, pe_, psc_;
     bool poisson_ready_ = false;
     int cg_total_ = 0;
@@ -558,33 +611,63 @@ template <typename T> void Solver<T>::poisson_setup() {
         c.tmp.alloc(bytes);
         levels_.push_back(std::move(c));
     }
+    // the finest level from which every coarser level fits the fused kernel's LDS budget
+    mg_entry_ = (int)levels_.size();
+    for (int l = (int)levels_.size() - 1; l >= 0 && (int)levels_.size() - l <= MG_MAX_LEVELS; l--) {
+        MgCoarse a{};
+        a.levels = (int)levels_.size() - l;
+        int off = 0;
+        for (int q = 0; q < a.levels; q++) {
+            const Level &L = levels_[l + q];
+            a.n[q] = L.n;
+            a.h2[q] = L.h * L.h;
+            a.gl[q] = 1.0 - 1.0 / L.d_low;
+            a.gh[q] = 1.0 - 1.0 / L.d_high;
+            a.off[q] = off;
+            off += 3 * L.n * L.n * L.n;
+        }
+        a.pre = MG_PRE;
+        a.post = MG_POST;
+        a.coarse = MG_COARSE;
+        a.omega = MG_OMEGA;
+        if (mg_coarse_bytes(a, (int)sizeof(T)) > MG_COARSE_LDS) break;
+        mg_entry_ = l;
+        mgc_ = a;
+    }
     poisson_ready_ = true;
 }
 
 // one V-cycle from level l: x = M_l^{-1} rhs (crates/qm-gpu/src/multigrid.rs:165-181)
 template <typename T> void Solver<T>::vcycle(int l, T *x, const T *rhs) {
+    if (l == mg_entry_) {
+        mg_coarse<T>(x, rhs, mgc_, stream_);  // this level and every coarser one in one workgroup
+        return;
+    }
     Level &L = levels_[l];
     const long Nl = (long)L.n * L.n * L.n;
     const double h2 = L.h * L.h, gl = 1.0 - 1.0 / L.d_low, gh = 1.0 - 1.0 / L.d_high;
-    fill<T>(x, Nl, 0.0, stream_);
     T *tmp = L.tmp.template as<T>();
-    auto smooth = [&](int sweeps) {
+    // x = 0, then sweeps in pairs; the first sweep of the first pair reads no x (the zero fill folded into it)
+    auto smooth = [&](int sweeps, bool from_zero) {
         for (int s = 0; s < sweeps; s += 2) {
-            mg_jacobi<T>(tmp, x, rhs, L.n, h2, MG_OMEGA, gl, gh, stream_);
+            if (from_zero && s == 0) mg_jacobi0<T>(tmp, rhs, L.n, h2, MG_OMEGA, gl, gh, stream_);
+            else mg_jacobi<T>(tmp, x, rhs, L.n, h2, MG_OMEGA, gl, gh, stream_);
             mg_jacobi<T>(x, tmp, rhs, L.n, h2, MG_OMEGA, gl, gh, stream_);
         }
     };
     if (l + 1 == (int)levels_.size()) {
-        smooth(MG_COARSE);
+        if (MG_COARSE == 0) fill<T>(x, Nl, 0.0, stream_);
+        smooth(MG_COARSE, true);
         return;
     }
-    smooth(MG_PRE);
+    if (MG_PRE == 0) fill<T>(x, Nl, 0.0, stream_);
+    smooth(MG_PRE, true);
     mg_residual<T>(L.res.template as<T>(), x, rhs, L.n, h2, gl, gh, stream_);
     Level &C = levels_[l + 1];
     mg_restrict<T>(C.b.template as<T>(), L.res.template as<T>(), C.n, L.n, stream_);
     vcycle(l + 1, C.x.template as<T>(), C.b.template as<T>());
     mg_prolong_add<T>(x, C.x.template as<T>(), L.n, C.n, stream_);
-    smooth(MG_POST);
+    smooth(MG_POST, false);
 }
 
 // Preconditioned CG in precision T on A e = r0 (A = -lap, 8th order, Dirichlet) from e = 0, with the V-cycle as M.
@@ -593,23 +676,31 @@ template <typename T> void Solver<T>::vcycle(int l, T *x, const T *rhs) {
 template <typename T> int Solver<T>::pcg(const T *r0, T *e, double rel_tol, int max_iter, double *rel_out) {
     T *r = pr_.template as<T>(), *z = pz_.template as<T>(), *p = pp_.template as<T>(), *q = pq_.template as<T>();
     double *sc = psc_.as<double>();
-    copy<T>(r, r0, N_, stream_);
-    fill<T>(e, N_, 0.0, stream_);
+    // every launch below reads and writes buffers fixed for the solver's life (the MG levels, the CG vectors, the
+    // scalar slots, the reduction partials); the key adds the two that vary between callers
+    auto key = [&](uintptr_t a, uintptr_t b) {
+        return std::vector<uintptr_t>{a, b, (uintptr_t)r, (uintptr_t)z, (uintptr_t)p, (uintptr_t)q, (uintptr_t)sc,
+                                      (uintptr_t)partials_.ptr, (uintptr_t)dres_.ptr, (uintptr_t)levels_[0].tmp.ptr};
+    };
     // sc: [0],[1] rz ping-pong, [2] pq, [3 + i % 2] rr of iteration i, [8] r0.r0
-    RedJobs<T> j0{};
-    j0.a[0] = r;
-    j0.b[0] = r;
-    j0.kind[0] = RED_DOT;
-    j0.count = 1;
-    reduce_jobs<T>(j0, n_, 1.0, partials_.as<double>(), sc + 8, stream_);
-    vcycle(0, z, r);
-    copy<T>(p, z, N_, stream_);
-    RedJobs<T> jrz{};
-    jrz.a[0] = r;
-    jrz.b[0] = z;
-    jrz.kind[0] = RED_DOT;
-    jrz.count = 1;
-    reduce_jobs<T>(jrz, n_, 1.0, partials_.as<double>(), sc + 0, stream_);
+    run_graph(pcg_start_graph_, key((uintptr_t)r0, (uintptr_t)e), [&] {
+        copy<T>(r, r0, N_, stream_);
+        fill<T>(e, N_, 0.0, stream_);
+        RedJobs<T> j0{};
+        j0.a[0] = r;
+        j0.b[0] = r;
+        j0.kind[0] = RED_DOT;
+        j0.count = 1;
+        reduce_jobs<T>(j0, n_, 1.0, partials_.as<double>(), sc + 8, stream_);
+        vcycle(0, z, r);
+        copy<T>(p, z, N_, stream_);
+        RedJobs<T> jrz{};
+        jrz.a[0] = r;
+        jrz.b[0] = z;
+        jrz.kind[0] = RED_DOT;
+        jrz.count = 1;
+        reduce_jobs<T>(jrz, n_, 1.0, partials_.as<double>(), sc + 0, stream_);
+    });
     const double r00 = readback(sc + 8, 1)[0];
     if (!(r00 > 0.0)) {
         *rel_out = 0.0;
@@ -626,27 +717,31 @@ template <typename T> int Solver<T>::pcg(const T *r0, T *e, double rel_tol, int 
     bool pending[2] = {false, false};
     bool done = false;
     while (!done && it < max_iter) {
-        stencil<T>(ST_LAP, sp, p, q, nullptr, nullptr, nullptr, stream_);  // q = A p = -lap p
-        RedJobs<T> jpq{};
-        jpq.a[0] = p;
-        jpq.b[0] = q;
-        jpq.kind[0] = RED_DOT;
-        jpq.count = 1;
-        reduce_jobs<T>(jpq, n_, 1.0, partials_.as<double>(), sc + 2, stream_);
-        cg_update<T, T>(e, p, r, q, sc, cur, 2, N_, stream_);
-        vcycle(0, z, r);
-        RedJobs<T> jn{};
-        jn.a[0] = r;
-        jn.b[0] = z;
-        jn.kind[0] = RED_DOT;
-        jn.a[1] = r;
-        jn.b[1] = r;
-        jn.kind[1] = RED_DOT;
-        jn.count = 2;
-        // rz_new into sc[1 - cur], rr into sc[3 + it % 2]: reduce writes consecutive slots, so route via dres
-        reduce_jobs<T>(jn, n_, 1.0, partials_.as<double>(), dres_.as<double>() + 16, stream_);
-        LIRA_HIP(hipMemcpyAsync(sc + (1 - cur), dres_.as<double>() + 16, sizeof(double), hipMemcpyDeviceToDevice, stream_));
-        cg_direction<T>(p, z, sc, 1 - cur, cur, N_, stream_);
+        // one iteration; cur (which rz slot is current) alternates, so two captured forms
+        run_graph(pcg_graph_[cur], key((uintptr_t)e, (uintptr_t)cur), [&] {
+            stencil<T>(ST_LAP, sp, p, q, nullptr, nullptr, nullptr, stream_);  // q = A p = -lap p
+            RedJobs<T> jpq{};
+            jpq.a[0] = p;
+            jpq.b[0] = q;
+            jpq.kind[0] = RED_DOT;
+            jpq.count = 1;
+            reduce_jobs<T>(jpq, n_, 1.0, partials_.as<double>(), sc + 2, stream_);
+            cg_update<T, T>(e, p, r, q, sc, cur, 2, N_, stream_);
+            vcycle(0, z, r);
+            RedJobs<T> jn{};
+            jn.a[0] = r;
+            jn.b[0] = z;
+            jn.kind[0] = RED_DOT;
+            jn.a[1] = r;
+            jn.b[1] = r;
+            jn.kind[1] = RED_DOT;
+            jn.count = 2;
+            // rz_new into sc[1 - cur], rr into sc[3 + it % 2]: reduce writes consecutive slots, so route via dres
+            reduce_jobs<T>(jn, n_, 1.0, partials_.as<double>(), dres_.as<double>() + 16, stream_);
+            LIRA_HIP(hipMemcpyAsync(sc + (1 - cur), dres_.as<double>() + 16, sizeof(double), hipMemcpyDeviceToDevice,
+                                    stream_));
+            cg_direction<T>(p, z, sc, 1 - cur, cur, N_, stream_);
+        });
         const int slot = it & 1;
         LIRA_HIP(hipMemcpyAsync(hrr + slot, dres_.as<double>() + 17, sizeof(double), hipMemcpyDeviceToHost, stream_));
         LIRA_HIP(hipEventRecord(ev[slot], stream_));
@@ -730,7 +825,7 @@ template <typename T> PoissonOut Solver<T>::poisson(const T *rho, double *v, dou
 
 // ------------------------------------------------------------------------------------------- LOBPCG ----
 // Block LOBPCG with one readback per iteration (crates/qm-gpu/src/eigen.rs:685-943, restated): R = HX - theta X;
-// W = 4-sweep damped Jacobi on (H - lam) w = r; one Gram of S = [X, W, P] against [S, R, HW, HP] gives
+// W = 4-sweep damped Jacobi on (H - lam_i) w = r (lam_i the column's Ritz value, or one fixed shift); one Gram of S = [X, W, P] against [S, R, HW, HP] gives
 // B = S^T S and A = S^T H S (A's X columns restored from R); host f64 generalised Rayleigh–Ritz with the
 // conditioning guard; X, P updated by one block lincomb on the device; HX recomputed by the stencil.
 template <typename T>
@@ -742,10 +837,14 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
     const double omega = jacobi_omega();
     const double floor = sizeof(T) == 4 ? residual_floor_f32(h_) : 0.0;
     double *dth = dsmall_.as<double>() + 256, *dlam = dsmall_.as<double>() + 512;
-    {
+    // the Jacobi preconditioner's shift: per column its current Ritz value (lam NaN: the reference's GPU solver,
+    // crates/qm-gpu/src/eigen.rs:740-757), or one fixed shift for every column (crates/qm/src/eigen.rs:227)
+    const bool per_column = std::isnan(lam);
+    if (!per_column) {
         std::vector<double> l(k, lam);
         upload(dlam, l.data(), k);
     }
+    const double *dshift = per_column ? dth : dlam;
     auto X = [&]() { return c.X.template as<T>(); };
     // initial Rayleigh–Ritz on X
     hamiltonian(v, X(), c.HX.template as<T>(), k);
@@ -783,9 +882,9 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
     for (;;) {
         upload(dth, theta.data(), k);
         residual<T>(c.R.template as<T>(), c.HX.template as<T>(), X(), dth, k, N_, stream_);
-        precond_init<T>(c.W.template as<T>(), c.R.template as<T>(), v, dlam, diag_t, k, N_, stream_);
+        precond_init<T>(c.W.template as<T>(), c.R.template as<T>(), v, dshift, diag_t, k, N_, stream_);
         for (int s = 0; s < 3; s++) {
-            stencil<T>(ST_JAC, jp, c.W.template as<T>(), c.TMP.template as<T>(), v, c.R.template as<T>(), dlam, stream_,
+            stencil<T>(ST_JAC, jp, c.W.template as<T>(), c.TMP.template as<T>(), v, c.R.template as<T>(), dshift, stream_,
                        variant_jac_);
             std::swap(c.W, c.TMP);
         }
@@ -845,13 +944,15 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
         double alpha;
         const double alpha_budget =
             sizeof(T) == 4 ? std::max(ALPHA_MAX, ALPHA_RESIDUAL_FRACTION * max_res / std::max(floor, 1e-300)) : 1e300;
+        bool rr_failed = false;  // the subspace with P lost rank: retry without P (in every precision)
         try {
             alpha = rayleigh_ritz(m, Am, Bm, k, RR_DROP, th, Cc);
         } catch (const NumericalError &) {
             if (np == 0) throw;
+            rr_failed = true;
             alpha = 1e300;
         }
-        if (alpha > alpha_budget && np > 0) {
+        if ((rr_failed || alpha > alpha_budget) && np > 0) {
             out.fallbacks++;
             mm = 2 * k;
             std::vector<double> As((size_t)mm * mm), Bs((size_t)mm * mm);
@@ -864,6 +965,8 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
             Am.swap(As);
             Bm.swap(Bs);
         }
+        if (Cc.size() != (size_t)mm * k || th.size() != (size_t)k)
+            throw NumericalError("lobpcg: Rayleigh-Ritz returned no coefficients for the step");
         // P = the Ritz vector minus its X component: C_p = C - [Bxx^{-1} Bxs C ; 0]
         std::vector<double> M((size_t)mm * 2 * k, 0.0);
         for (int i = 0; i < mm; i++)
@@ -925,11 +1028,10 @@ void Solver<T>::solve_channel(int spin, const T *v, int nel, bool fd, double kt,
     };
     if (fd && c.k == nel + 1 && cap >= 2) add_column();
     for (;;) {
-        const double lam = c.evals.empty() ? -1.0 : c.evals[0];
         const int checked = buffered(c.k) ? c.k - 1 : c.k;
         const int head = fd ? std::min(std::max(c.tight, 0), checked) : c.k;
         profile_.peak_block = std::max(profile_.peak_block, c.k);
-        EigenOut e = lobpcg(c, v, c.k, lam, tol, 200, fd ? guard : tol, head, checked);
+        EigenOut e = lobpcg(c, v, c.k, NAN, tol, 200, fd ? guard : tol, head, checked);
         if (trace_on())
             std::fprintf(stderr, "lira:   spin %d k %d head %d tested %d tol %.1e: %d iterations, max residual %.2e, fallbacks %d\n",
                          spin, c.k, head, checked, tol, e.iterations, e.max_residual, e.fallbacks);
@@ -2164,6 +2266,11 @@ lira_status lira_get_profile(lira_solver *s, lira_profile *out, double *delta_rh
 lira_status lira_sample_density(lira_solver *s, const double *points, int count, double *out) {
     if (!s) return LIRA_INVALID;
     return guarded(s, [&] { s->impl->sample(points, count, out); });
+}
+
+lira_status lira_set_graphs(lira_solver *s, int on) {
+    if (!s) return LIRA_INVALID;
+    return guarded(s, [&] { s->impl->set_graphs(on != 0); });
 }
 
 lira_status lira_free_atom_levels(int z, double *eps, double *electrons) {

@@ -30,7 +30,7 @@ EXPORTS = [
     "lira_xc_lsda", "lira_nuclear_potential", "lira_gram", "lira_stencil_variant_count", "lira_stencil_variant_name",
     "lira_autotune_stencil", "lira_set_stencil_variant", "lira_version", "lira_set_ionic", "lira_force_terms",
     "lira_ionic_fields", "lira_nonlocal_apply", "lira_free_atom_levels", "lira_set_witness_cap", "lira_witness_cap",
-    "lira_set_density_extrapolation", "lira_get_profile", "lira_sample_density",
+    "lira_set_density_extrapolation", "lira_get_profile", "lira_sample_density", "lira_set_graphs",
 ]
 
 
@@ -182,6 +182,7 @@ def lib():
         L.lira_set_density_extrapolation.argtypes = [vp, C.c_int]
         L.lira_get_profile.argtypes = [vp, C.POINTER(Profile), _dp, _dp, C.c_int, C.POINTER(C.c_int)]
         L.lira_sample_density.argtypes = [vp, _dp, C.c_int, _dp]
+        L.lira_set_graphs.argtypes = [vp, C.c_int]
         L.lira_force_terms.argtypes = [vp, C.POINTER(Nucleus), C.c_int, C.c_double, _dp]
         L.lira_ionic_fields.argtypes = [vp, C.POINTER(Nucleus), C.c_int, C.c_double, vp, vp]
         L.lira_nonlocal_apply.argtypes = [vp, _dp, C.c_int, _dp, vp, C.c_int, C.POINTER(C.c_int)]
@@ -336,6 +337,11 @@ class GpuSolver:
         out = np.zeros(len(pts))
         _raise(lib().lira_sample_density(self._h, pts.reshape(-1), len(pts), out), self._h, "lira_sample_density")
         return out
+
+    def set_graphs(self, on):
+        """Replay the fixed kernel sequences as HIP graphs (on, the default) or launch kernels one by one
+        (lira_set_graphs)."""
+        _raise(lib().lira_set_graphs(self._h, int(bool(on))), self._h, "lira_set_graphs")
 
     def forces(self, pos, z, sigma) -> np.ndarray:
         nuc = nuclei_array(pos, z)

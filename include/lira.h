@@ -197,6 +197,12 @@ lira_status lira_get_profile(lira_solver *s, lira_profile *out, double *delta_rh
  * z; crates/qm/src/grid.rs:54-86); 0 outside [origin, origin + (n-1)h]^3. */
 lira_status lira_sample_density(lira_solver *s, const double *points, int count, double *out);
 
+/* Dispatch control, no counterpart in the reference (its recorder batches dispatches instead,
+ * crates/qm-gpu/src/batch.rs:55-233): on (the default), the fixed kernel sequences between two host syncs (the
+ * Poisson MG-PCG iteration and its start, the LOBPCG step) are captured once as HIP graphs and replayed; off, every
+ * kernel is launched one by one. Results are bitwise identical either way. */
+lira_status lira_set_graphs(lira_solver *s, int on);
+
 /* The spherical spin-restricted LSDA free atom behind the Fermi-Dirac cold guess (qm::radial::atom::ground_state,
  * crates/qm/src/radial/atom.rs:474; used by free_atom_density, crates/qm/src/nuclear.rs:153): eps receives the
  * 1s, 2s, 3s, 2p, 3p eigenvalues (1.0 for an unbound level) and *electrons the density's electron count. Host only. */

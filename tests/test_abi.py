@@ -49,6 +49,24 @@ def test_rayleigh_ritz_matches_a_dense_generalised_solve():
     assert np.allclose(c.T @ b @ c, np.eye(4), atol=1e-10)  # B-orthonormal coefficients
 
 
+@pytest.mark.parametrize("cond", [1e2, 1e5])
+def test_rayleigh_ritz_conditioned_subspaces_match_a_dense_generalised_solve(cond):
+    # B~ conditioned at 1e2 and 1e5 (nothing dropped at 1e-6): scipy's generalised eigenpairs, B-orthonormal
+    import scipy.linalg as sl
+    rng = np.random.default_rng(11)
+    m = 14
+    q, _ = np.linalg.qr(rng.standard_normal((m, m)))
+    b = q @ np.diag(np.logspace(0.0, -np.log10(cond), m)) @ q.T
+    a = rng.standard_normal((m, m))
+    a = a + a.T
+    th, c = lira.rayleigh_ritz(a, b, 5)
+    w = sl.eigh(a, b, eigvals_only=True)[:5]
+    assert np.allclose(th, w, rtol=1e-9, atol=1e-9)
+    assert np.allclose(c.T @ b @ c, np.eye(5), atol=1e-8)
+    # residual relative to the problem's scale (|theta| up to 3e5, |c| up to 2e2 at 1e5)
+    assert np.abs(a @ c - b @ c @ np.diag(th)).max() <= 1e-12 * np.abs(th).max() * np.abs(c).max()
+
+
 def test_rank_loss_is_a_numerical_error():
     # two identical columns: B~ has a zero direction, so only 2 of 3 survive and 3 Ritz pairs cannot be formed
     a = np.diag([1.0, 2.0, 3.0])

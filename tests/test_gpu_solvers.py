@@ -136,7 +136,11 @@ def _scf(lira, precision, system, occ=0, n=33, h=0.25):
     return s, st, f
 
 
-@pytest.mark.parametrize("precision,e_tol,f_tol", [(0, 1e-9, 1e-7), (1, 2e-6, 5e-6)])
+# the pins are the reference CPU solver's stopping point (delta_rho < 1e-6, one preconditioner shift for every column);
+# this solver shifts each column by its own Ritz value as the reference's GPU solver does, so it stops at another
+# point within the same tolerance: energies agree to second order (1e-9), forces to first order (the reference's own
+# Aufbau and FD H2 forces differ by 6.3e-7, qm/src/scf.rs:1277, 1291)
+@pytest.mark.parametrize("precision,e_tol,f_tol", [(0, 1e-9, 1e-6), (1, 2e-6, 5e-6)])
 def test_scf_h2_matches_reference_pins(lira_mod, precision, e_tol, f_tol):
     # qm/src/scf.rs:1276-1277: E = -0.952449176828, F_x = +0.128755968665 (33^3 x 0.25, Aufbau, 10 CPU steps)
     lira = lira_mod
