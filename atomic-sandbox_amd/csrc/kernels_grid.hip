@@ -1184,69 +1184,77 @@ void gram(const Cols &S, const Cols &U, long N, double scale, double *partials, 
 }
 
 // ------------------------------------------------------------------------------------------ lincomb ----
-// Y_j = sum_i S_i C_ij: one thread per point holds QB outputs in f64 registers; C (m <= 64 rows) and the column
-// pointers are staged in LDS once per workgroup; grid.y covers q in groups of QB.
+// Y_j = sum_i S_i C_ij: one thread per point holds QB outputs in f64 registers; grid.y covers q in groups of QB.
+// C arrives padded (lincomb_rows(m) rows, lincomb_ldc(q) columns, zeros beyond m x q), so every loop below has a
+// compile-time trip count and no bounds branch: each row's QB coefficients are wave-uniform and come in a few wide
+// scalar loads, and every FMA takes one as its scalar operand. Padded rows read column 0 (their coefficients are 0).
 constexpr int LC_MAX_M = 192;  // input columns of one lincomb (3 x the largest block LOBPCG forms)
+constexpr int LC_GROUP = 8;    // input columns loaded together
 
-// The coefficients are wave-uniform (the row index and the output column are loop constants), so they are read
-// straight from C: the compiler keeps them in scalar registers and every FMA takes one as its scalar operand.
+static int lincomb_qb(int q) {
+    // the output group QB: the smallest of 8, 16, 24, 32 that splits q into equal-count launches
+    const int groups = (q + 31) / 32;
+    const int per = (q + groups - 1) / groups;
+    return per <= 8 ? 8 : per <= 16 ? 16 : per <= 24 ? 24 : 32;
+}
+int lincomb_rows(int m) { return (m + LC_GROUP - 1) / LC_GROUP * LC_GROUP; }
+int lincomb_ldc(int q) {
+    const int qb = lincomb_qb(q);
+    return (q + qb - 1) / qb * qb;
+}
+
 template <typename T, int QB>
-__global__ __launch_bounds__(256) void k_lincomb(ColPtrs S, int m, const double *__restrict__ C, int q,
-                                                 T *__restrict__ Y, int qsplit, T *__restrict__ Y2, long N) {
+__global__ __launch_bounds__(256) void k_lincomb(ColPtrs S, int m, int mp, const double *__restrict__ C, int ldc,
+                                                 int q, T *__restrict__ Y, int qsplit, T *__restrict__ Y2, long N) {
     __shared__ const T *cols[LC_MAX_M];
     const int j0 = blockIdx.y * QB;
-    const int qn = min(QB, q - j0);
-    for (int i = threadIdx.x; i < m; i += 256) cols[i] = static_cast<const T *>(col_base(S, i, N, sizeof(T)));
+    for (int i = threadIdx.x; i < mp; i += 256)
+        cols[i] = static_cast<const T *>(col_base(S, i < m ? i : 0, N, sizeof(T)));
     __syncthreads();
     const long p = blockIdx.x * 256L + threadIdx.x;
     if (p >= N) return;
     double acc[QB];
 #pragma unroll
     for (int c = 0; c < QB; c++) acc[c] = 0.0;
-    for (int i0 = 0; i0 < m; i0 += 8) {
-        // the 8 loads of a group are independent and issued together
-        double sv[8];
+    for (int i0 = 0; i0 < mp; i0 += LC_GROUP) {
+        // the group's loads are independent and issued together
+        double sv[LC_GROUP];
 #pragma unroll
-        for (int u = 0; u < 8; u++) sv[u] = i0 + u < m ? (double)cols[i0 + u][p] : 0.0;
+        for (int u = 0; u < LC_GROUP; u++) sv[u] = (double)cols[i0 + u][p];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            if (i0 + u < m) {
-                const double *row = C + (long)(i0 + u) * q + j0;
+        for (int u = 0; u < LC_GROUP; u++) {
+            const double *__restrict__ row = C + (long)(i0 + u) * ldc + j0;
 #pragma unroll
-                for (int c = 0; c < QB; c++)
-                    if (c < qn) acc[c] += sv[u] * row[c];
-            }
+            for (int c = 0; c < QB; c++) acc[c] += sv[u] * row[c];
         }
     }
 #pragma unroll
-    for (int c = 0; c < QB; c++)
-        if (c < qn) {
-            const int j = j0 + c;
+    for (int c = 0; c < QB; c++) {
+        const int j = j0 + c;
+        if (j < q) {
             T *dst = j < qsplit ? Y + (long)j * N : Y2 + (long)(j - qsplit) * N;
             dst[p] = (T)acc[c];
         }
+    }
 }
 
 template <typename T>
-void lincomb(const Cols &S, const double *C, int q, T *Y, long N, hipStream_t s, int qsplit, T *Y2) {
+void lincomb(const Cols &S, const double *C, int ldc, int q, T *Y, long N, hipStream_t s, int qsplit, T *Y2) {
     const int m = S.total();
     if (q <= 0) return;
     if (m > LC_MAX_M) throw InvalidError("lincomb: more than 192 input columns");
+    if (ldc != lincomb_ldc(q)) throw InvalidError("lincomb: coefficients not padded to lincomb_ldc(q) columns");
     if (qsplit < 0 || qsplit > q) qsplit = q;
     const unsigned gx = (unsigned)((N + 255) / 256);
-    // the output group QB: the smallest of 8, 16, 24, 32 that splits q into equal-count launches
-    const int groups = (q + 31) / 32;
-    const int per = (q + groups - 1) / groups;
+    const int qb = lincomb_qb(q), mp = lincomb_rows(m);
     const ColPtrs sp = to_ptrs(S);
-    if (per <= 8) {
-        hipLaunchKernelGGL((k_lincomb<T, 8>), dim3(gx, (q + 7) / 8), dim3(256), 0, s, sp, m, C, q, Y, qsplit, Y2, N);
-    } else if (per <= 16) {
-        hipLaunchKernelGGL((k_lincomb<T, 16>), dim3(gx, (q + 15) / 16), dim3(256), 0, s, sp, m, C, q, Y, qsplit, Y2, N);
-    } else if (per <= 24) {
-        hipLaunchKernelGGL((k_lincomb<T, 24>), dim3(gx, (q + 23) / 24), dim3(256), 0, s, sp, m, C, q, Y, qsplit, Y2, N);
-    } else {
-        hipLaunchKernelGGL((k_lincomb<T, 32>), dim3(gx, (q + 31) / 32), dim3(256), 0, s, sp, m, C, q, Y, qsplit, Y2, N);
-    }
+    const dim3 grid(gx, (unsigned)(ldc / qb));
+#define LIRA_LC(QB_) hipLaunchKernelGGL((k_lincomb<T, QB_>), grid, dim3(256), 0, s, sp, m, mp, C, ldc, q, Y, qsplit, Y2, N)
+    if (qb == 8) LIRA_LC(8);
+    else if (qb == 16) LIRA_LC(16);
+    else if (qb == 24) LIRA_LC(24);
+    else LIRA_LC(32);
+#undef LIRA_LC
     LIRA_HIP(hipGetLastError());
 }
 
@@ -1612,7 +1620,7 @@ void hf_forces(const T *rho, int n, double h, const double *pos, const double *z
                              hipStream_t, int);                                                                       \
     template void reduce_jobs<T>(const RedJobs<T> &, int, double, double *, double *, hipStream_t);                 \
     template void gram<T>(const Cols &, const Cols &, long, double, double *, size_t, double *, hipStream_t);       \
-    template void lincomb<T>(const Cols &, const double *, int, T *, long, hipStream_t, int, T *);                             \
+    template void lincomb<T>(const Cols &, const double *, int, int, T *, long, hipStream_t, int, T *);                       \
     template void fill<T>(T *, long, double, hipStream_t);                                                           \
     template void axpy<T>(T *, const T *, long, double, hipStream_t);                                                \
     template void residual<T>(T *, const T *, const T *, const double *, int, long, hipStream_t);                    \

@@ -86,7 +86,11 @@ size_t gram_partials_needed(int m, int q, long N);
 // Y_j = sum_i S_i * C[i * q + j] (C f64 on the device, m x q), accumulated in f64. Outputs j < qsplit go to
 // Y + j N, the others to Y2 + (j - qsplit) N.
 template <typename T>
-void lincomb(const Cols &S, const double *C, int q, T *Y, long N, hipStream_t s, int qsplit = -1, T *Y2 = nullptr);
+void lincomb(const Cols &S, const double *C, int ldc, int q, T *Y, long N, hipStream_t s, int qsplit = -1,
+             T *Y2 = nullptr);
+// the padded shape lincomb's coefficients must have: lincomb_rows(m) rows of lincomb_ldc(q) doubles, zero beyond m x q
+int lincomb_rows(int m);
+int lincomb_ldc(int q);
 
 // ------------------------------------------------------------------- stencil ----
 enum StencilMode : int { ST_LAP = 0, ST_HAM = 1, ST_JAC = 2 };

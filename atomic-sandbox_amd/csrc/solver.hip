@@ -490,6 +490,16 @@ template <typename T> class Solver final : public SolverIface {
             dC_cap_ = count;
         }
     }
+    // uploads the m x q row-major coefficients of a lincomb into dC_, zero-padded to lincomb's shape; returns ldc
+    int upload_coeffs(const std::vector<double> &C, int m, int q) {
+        const int mp = lincomb_rows(m), ldc = lincomb_ldc(q);
+        std::vector<double> pad((size_t)mp * ldc, 0.0);
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < q; j++) pad[(size_t)i * ldc + j] = C[(size_t)i * q + j];
+        ensure_coeff(pad.size() + 64);
+        upload(dC_.as<double>(), pad.data(), pad.size());
+        return ldc;
+    }
     void ensure_channel(Channel<T> &c, int kc);
     int variant_ham_ = -1, variant_jac_ = -1;  // the stencil tile variants the tuner picked for H.X and Jacobi
     void hamiltonian(const T *v, const T *x, T *out, int nb) {
@@ -868,11 +878,10 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
         if (!std::isfinite(x)) throw NumericalError("lobpcg: non-finite Gram of the guess (NaN in V or X?)");
     std::vector<double> theta, C;
     rayleigh_ritz(k, A, B, k, 1e-10, theta, C);
-    ensure_coeff((size_t)9 * k * k + 64);
-    upload(dC_.as<double>(), C.data(), (size_t)k * k);
     {
+        const int ldc = upload_coeffs(C, k, k);
         Cols S{{{X(), k}}, 1};
-        lincomb<T>(S, dC_.as<double>(), k, c.XN.template as<T>(), N_, stream_);
+        lincomb<T>(S, dC_.as<double>(), ldc, k, c.XN.template as<T>(), N_, stream_);
     }
     std::swap(c.X, c.XN);
     hamiltonian(v, X(), c.HX.template as<T>(), k);
@@ -987,10 +996,9 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
                 for (int j = 0; j < k; j++)
                     M[(size_t)i * 2 * k + k + j] = Cc[(size_t)i * k + j] - (i < k ? rhs[(size_t)i * k + j] : 0.0);
         }
-        ensure_coeff(M.size() + 64);
-        upload(dC_.as<double>(), M.data(), M.size());
+        const int ldc = upload_coeffs(M, mm, 2 * k);
         Cols Sm{{{X(), k}, {c.W.ptr, k}, {c.P.ptr, mm > 2 * k ? np : 0}}, mm > 2 * k ? 3 : 2};
-        lincomb<T>(Sm, dC_.as<double>(), 2 * k, c.XN.template as<T>(), N_, stream_, k, c.PN.template as<T>());
+        lincomb<T>(Sm, dC_.as<double>(), ldc, 2 * k, c.XN.template as<T>(), N_, stream_, k, c.PN.template as<T>());
         std::swap(c.X, c.XN);
         std::swap(c.P, c.PN);
         np = k;
