@@ -1002,6 +1002,13 @@ void reduce_jobs(const RedJobs<T> &jobs, int n, double scale, double *partials, 
 // G = scale S^T U over N points. A workgroup computes a 32 (S) x 64 (U) output block over one chunk of points,
 // staging 64 points of its 96 columns in LDS at a time; each thread owns a 2 x 4 register tile accumulated in f64.
 
+// A load through a pointer the compiler cannot place (one read from an LDS table of column bases) becomes a flat
+// load, which also counts against the LDS wait counter: every wait for an LDS read would then wait for the loads in
+// flight as well. Column data is always global memory, so these loads name that address space.
+template <typename T> __device__ __forceinline__ T gload(const T *p) {
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+
 struct ColPtrs {
     const void *base[MAX_SEGS];
     int start[MAX_SEGS + 1];
@@ -1032,16 +1039,28 @@ constexpr int G2_TI = 32, G2_TJ = 64;
 constexpr int G2_S = 16;                             // points per wave stage
 constexpr int G2_LOADS = (G2_TI + G2_TJ) * G2_S / 64;  // elements per lane per stage
 
+// Every load is issued unconditionally and its value is not touched until the stage is staged: a missing column or a
+// point past the chunk reads a zero constant instead. (A load under a branch, or a select on the loaded value, makes
+// the compiler wait for the load at once, and the stage-ahead prefetch would never be in flight.)
+template <typename T> struct GramZero;
+template <> struct GramZero<float> { static __device__ const float *ptr(); };
+template <> struct GramZero<double> { static __device__ const double *ptr(); };
+__device__ const float lira_gram_zero_f = 0.0f;
+__device__ const double lira_gram_zero_d = 0.0;
+__device__ const float *GramZero<float>::ptr() { return &lira_gram_zero_f; }
+__device__ const double *GramZero<double>::ptr() { return &lira_gram_zero_d; }
+
 template <typename T>
 __device__ inline void gram_fetch(const T *const *cols, long p0, long p_end, T (&v)[G2_LOADS]) {
     const int lane = threadIdx.x & 63;
+    const T *zero = GramZero<T>::ptr();
 #pragma unroll
     for (int u = 0; u < G2_LOADS; u++) {
         const int e = lane + 64 * u;
         const int col = e >> 4, pp = e & 15;
         const long p = p0 + pp;
         const T *c = cols[col];
-        v[u] = (c && p < p_end) ? c[p] : T(0);
+        v[u] = gload(c != nullptr && p < p_end ? c + p : zero);
     }
 }
 
@@ -1050,6 +1069,7 @@ __global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q
                                               double *partials) {
     __shared__ double slab[4][G2_TI + G2_TJ][G2_S + 1];  // one slice per wave; reused for the final reduction
     __shared__ const T *cols[G2_TI + G2_TJ];
+    __shared__ int urow[G2_TJ];  // the slab row each U column is read from: its own, or the S row of the same column
     const int nbj = (q + G2_TJ - 1) / G2_TJ;
     const int bi = blockIdx.y / nbj, bj = blockIdx.y - bi * nbj;
     const int i0 = bi * G2_TI, j0 = bj * G2_TJ;
@@ -1059,6 +1079,23 @@ __global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q
         const int c = t - 64;
         cols[G2_TI + c] = j0 + c < q ? static_cast<const T *>(col_base(U, j0 + c, N, sizeof(T))) : nullptr;
     }
+    __syncthreads();
+    // a U column that is also one of this tile's S columns (U = [S, R, HW, HP] in LOBPCG) is not loaded again: the
+    // MFMA reads it from the S row
+    if (t >= 64 && t < 64 + G2_TJ) {
+        const int c = t - 64;
+        const T *uc = cols[G2_TI + c];
+        int row = G2_TI + c;
+        if (uc)
+            for (int i = 0; i < G2_TI; i++)
+                if (cols[i] == uc) {
+                    row = i;
+                    break;
+                }
+        urow[c] = row;
+    }
+    __syncthreads();
+    if (t >= 64 && t < 64 + G2_TJ && urow[t - 64] < G2_TI) cols[G2_TI + t - 64] = nullptr;
     // which 16 x 16 output blocks hold any output (uniform over the block)
     const int ni = min(2, (m - i0 + 15) / 16), nj = min(4, (q - j0 + 15) / 16);
     lira_d4 acc[2][4];
@@ -1070,18 +1107,25 @@ __global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q
     const long p_end = min(N, p_begin + chunk);
     __syncthreads();
     double(*sl)[G2_S + 1] = slab[wave];
-    T v[G2_LOADS];
+    // two stage buffers in registers: the loads of the stage after next are in flight while a stage is consumed
+    T va[G2_LOADS], vb[G2_LOADS];
     long p0 = p_begin + (long)wave * G2_S;
-    if (p0 < p_end) gram_fetch<T>(cols, p0, p_end, v);
+    gram_fetch<T>(cols, p0, p_end, va);
+    gram_fetch<T>(cols, p0 + 4 * G2_S, p_end, vb);
     const int r16 = lane & 15, k4 = lane >> 4;
-    for (; p0 < p_end; p0 += 4 * G2_S) {
+    int brow[4];
+#pragma unroll
+    for (int b = 0; b < 4; b++) brow[b] = urow[16 * b + r16];
+    // one stage: stage v into the wave's slice, refill v with the stage two ahead (past the chunk: zeros), then the
+    // MFMAs of the staged points
+    auto stage = [&](T(&v)[G2_LOADS], long pn) {
 #pragma unroll
         for (int u = 0; u < G2_LOADS; u++) {
             const int e = lane + 64 * u;
             sl[e >> 4][e & 15] = (double)v[u];
         }
         __builtin_amdgcn_wave_barrier();
-        if (p0 + 4 * G2_S < p_end) gram_fetch<T>(cols, p0 + 4 * G2_S, p_end, v);
+        gram_fetch<T>(cols, pn, p_end, v);
 #pragma unroll
         for (int kk = 0; kk < G2_S; kk += 4) {
             const double a0 = sl[r16][kk + k4];
@@ -1089,13 +1133,18 @@ __global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q
 #pragma unroll
             for (int b = 0; b < 4; b++) {
                 if (b < nj) {
-                    const double bv = sl[G2_TI + 16 * b + r16][kk + k4];
+                    const double bv = sl[brow[b]][kk + k4];
                     acc[0][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc[0][b], 0, 0, 0);
                     if (ni > 1) acc[1][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, acc[1][b], 0, 0, 0);
                 }
             }
         }
         __builtin_amdgcn_wave_barrier();
+    };
+    // stages in pairs (a stage past the chunk holds zeros and adds nothing)
+    for (; p0 < p_end; p0 += 8 * G2_S) {
+        stage(va, p0 + 8 * G2_S);
+        stage(vb, p0 + 12 * G2_S);
     }
     // sum the four waves' accumulators in wave order through LDS (32 doubles per lane per wave)
     __syncthreads();
@@ -1220,7 +1269,7 @@ __global__ __launch_bounds__(256) void k_lincomb(ColPtrs S, int m, int mp, const
         // the group's loads are independent and issued together
         double sv[LC_GROUP];
 #pragma unroll
-        for (int u = 0; u < LC_GROUP; u++) sv[u] = (double)cols[i0 + u][p];
+        for (int u = 0; u < LC_GROUP; u++) sv[u] = (double)gload(cols[i0 + u] + p);
 #pragma unroll
         for (int u = 0; u < LC_GROUP; u++) {
             const double *__restrict__ row = C + (long)(i0 + u) * ldc + j0;

@@ -199,3 +199,20 @@ def test_gram_matches_numpy(dev, m, q):
     g32 = lira.gram(n, h, _t(torch, s, torch.float32), m, _t(torch, u, torch.float32), q)
     want32 = s.astype(np.float32).astype(np.float64) @ u.astype(np.float32).astype(np.float64).T * h ** 3
     assert np.abs(g32 - want32).max() <= 1e-12 * np.abs(want32).max()  # f32 inputs, f64 accumulation
+
+
+
+@pytest.mark.parametrize("m,q", [(27, 54), (40, 70), (9, 9)])
+def test_gram_with_u_beginning_with_s(dev, m, q):
+    # U's first m columns are S's own memory (the LOBPCG shape U = [S, R, HW, HP]): those columns are read once
+    import lira
+    torch = dev
+    n, h = 21, 0.3
+    rng = np.random.default_rng(100 + m)
+    x = rng.standard_normal((q, n ** 3))
+    for dt in (torch.float64, torch.float32):
+        xt = _t(torch, x, dt)
+        g = lira.gram(n, h, xt, m, xt, q)
+        xd = xt.double().cpu().numpy()
+        want = xd[:m] @ xd.T * h ** 3
+        assert np.abs(g - want).max() <= 1e-12 * np.abs(want).max(), dt
