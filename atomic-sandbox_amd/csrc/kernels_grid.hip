@@ -228,7 +228,7 @@ template <typename T, int MODE, int RPT, int W>
 __global__ __launch_bounds__(256) void k_stencil(const T *__restrict__ x, T *__restrict__ out, const T *__restrict__ v,
                                                  const T *__restrict__ r, const double *__restrict__ lam, int n,
                                                  long N, int zc, int nb, int order, T inv_h2, T scale, T omega,
-                                                 T diag_t) {
+                                                 T diag_t, const T *__restrict__ x2, T *__restrict__ out2, int nb1) {
     constexpr int TY = 4 * RPT;
     __shared__ T tile[2][TY + 8][ST_TX + 8];
     StCtx<T, RPT> c;
@@ -244,8 +244,8 @@ __global__ __launch_bounds__(256) void k_stencil(const T *__restrict__ x, T *__r
     const int z0 = wk.zchunk * zc;
     const int z1 = min(n, z0 + zc);
     const long fb = N * (long)sizeof(T);
-    c.bx = field_res(x + (long)band * N, fb);
-    c.bo = field_res(out + (long)band * N, fb);
+    c.bx = field_res(band < nb1 ? x + (long)band * N : x2 + (long)(band - nb1) * N, fb);
+    c.bo = field_res(band < nb1 ? out + (long)band * N : out2 + (long)(band - nb1) * N, fb);
     c.bv = field_res(v, MODE == ST_LAP ? 0 : fb);
     c.br = field_res(MODE == ST_JAC ? r + (long)band * N : x, MODE == ST_JAC ? fb : 0);
     c.lamb = MODE == ST_JAC ? (T)lam[band] : T(0);
@@ -457,7 +457,8 @@ template <typename T, int MODE, int RPT, int W>
 __global__ __launch_bounds__(256) void k_stencil_vec(const T *__restrict__ x, T *__restrict__ out,
                                                      const T *__restrict__ v, const T *__restrict__ r,
                                                      const double *__restrict__ lam, int n, long N, int zc, int nb,
-                                                     int order, T inv_h2, T scale, T omega, T diag_t) {
+                                                     int order, T inv_h2, T scale, T omega, T diag_t,
+                                                     const T *__restrict__ x2, T *__restrict__ out2, int nb1) {
     constexpr int VX = Vec16<T>::N;
     constexpr int TX = 16 * VX, TY = 16 * RPT;
     constexpr int LXP = TX + 8;
@@ -475,8 +476,8 @@ __global__ __launch_bounds__(256) void k_stencil_vec(const T *__restrict__ x, T 
     const int z0 = wk.zchunk * zc;
     const int z1 = min(n, z0 + zc);
     const long fb = N * (long)sizeof(T);
-    c.bx = field_res(x + (long)band * N, fb);
-    c.bo = field_res(out + (long)band * N, fb);
+    c.bx = field_res(band < nb1 ? x + (long)band * N : x2 + (long)(band - nb1) * N, fb);
+    c.bo = field_res(band < nb1 ? out + (long)band * N : out2 + (long)(band - nb1) * N, fb);
     c.bv = field_res(v, MODE == ST_LAP ? 0 : fb);
     c.br = field_res(MODE == ST_JAC ? r + (long)band * N : x, MODE == ST_JAC ? fb : 0);
     c.lamb = MODE == ST_JAC ? (T)lam[band] : T(0);
@@ -649,7 +650,8 @@ template <typename T, int MODE, int NB, int W>
 __global__ __launch_bounds__(256) void k_stencil_vb(const T *__restrict__ x, T *__restrict__ out,
                                                     const T *__restrict__ v, const T *__restrict__ r,
                                                     const double *__restrict__ lam, int n, long N, int zc, int nb,
-                                                    int order, T inv_h2, T scale, T omega, T diag_t) {
+                                                    int order, T inv_h2, T scale, T omega, T diag_t,
+                                                    const T *__restrict__ x2, T *__restrict__ out2, int nb1) {
     constexpr int VX = Vec16<T>::N;
     constexpr int TX = 16 * VX, TY = 16;
     constexpr int LXP = TX + 8, TYP = TY + 8;
@@ -677,8 +679,8 @@ __global__ __launch_bounds__(256) void k_stencil_vb(const T *__restrict__ x, T *
         const int band = wk.band * NB + b;
         const bool bok = band < nb;
         const int bb = bok ? band : wk.band * NB;  // a missing band reads a valid one and stores nothing
-        c.bx[b] = field_res(x + (long)bb * N, fb);
-        c.bo[b] = field_res(out + (long)bb * N, fb);
+        c.bx[b] = field_res(bb < nb1 ? x + (long)bb * N : x2 + (long)(bb - nb1) * N, fb);
+        c.bo[b] = field_res(bb < nb1 ? out + (long)bb * N : out2 + (long)(bb - nb1) * N, fb);
         c.br[b] = field_res(MODE == ST_JAC ? r + (long)bb * N : x, MODE == ST_JAC ? fb : 0);
         c.lamb[b] = MODE == ST_JAC ? (T)lam[bb] : T(0);
         c.so[b] = bok ? c.co : BUF_OOB;
@@ -806,7 +808,8 @@ static void launch_stencil(const StencilParams &p, int zc, int order, const T *x
     const unsigned grid = st_grid((n + ST_TX - 1) / ST_TX, (n + 4 * RPT - 1) / (4 * RPT), nzc, p.nb, order);
     dim3 block(64, 4, 1);
     hipLaunchKernelGGL((k_stencil<T, MODE, RPT, W>), dim3(grid), block, 0, s, x, out, v, r, lam, n, N, zc, p.nb, order,
-                       (T)(1.0 / (p.h * p.h)), (T)p.scale, (T)p.omega, (T)p.diag_t);
+                       (T)(1.0 / (p.h * p.h)), (T)p.scale, (T)p.omega, (T)p.diag_t, static_cast<const T *>(p.x2),
+                       static_cast<T *>(p.out2), p.nb1 < 0 ? p.nb : p.nb1);
     LIRA_HIP(hipGetLastError());
 }
 
@@ -820,7 +823,8 @@ static void launch_stencil_vec(const StencilParams &p, int zc, int order, const 
     const int nzc = (n + zc - 1) / zc;
     const unsigned grid = st_grid((n + 16 * VX - 1) / (16 * VX), (n + 16 * RPT - 1) / (16 * RPT), nzc, p.nb, order);
     hipLaunchKernelGGL((k_stencil_vec<T, MODE, RPT, W>), dim3(grid), dim3(16, 16, 1), 0, s, x, out, v, r, lam, n, N, zc,
-                       p.nb, order, (T)(1.0 / (p.h * p.h)), (T)p.scale, (T)p.omega, (T)p.diag_t);
+                       p.nb, order, (T)(1.0 / (p.h * p.h)), (T)p.scale, (T)p.omega, (T)p.diag_t,
+                       static_cast<const T *>(p.x2), static_cast<T *>(p.out2), p.nb1 < 0 ? p.nb : p.nb1);
     LIRA_HIP(hipGetLastError());
 }
 
@@ -834,7 +838,8 @@ static void launch_stencil_vb(const StencilParams &p, int zc, int order, const T
     const int nzc = (n + zc - 1) / zc;
     const unsigned grid = st_grid((n + 16 * VX - 1) / (16 * VX), (n + 15) / 16, nzc, (p.nb + NB - 1) / NB, order);
     hipLaunchKernelGGL((k_stencil_vb<T, MODE, NB, W>), dim3(grid), dim3(16, 16, 1), 0, s, x, out, v, r, lam, n, N, zc,
-                       p.nb, order, (T)(1.0 / (p.h * p.h)), (T)p.scale, (T)p.omega, (T)p.diag_t);
+                       p.nb, order, (T)(1.0 / (p.h * p.h)), (T)p.scale, (T)p.omega, (T)p.diag_t,
+                       static_cast<const T *>(p.x2), static_cast<T *>(p.out2), p.nb1 < 0 ? p.nb : p.nb1);
     LIRA_HIP(hipGetLastError());
 }
 
@@ -895,6 +900,10 @@ void stencil(int mode, const StencilParams &p, const T *x, T *out, const T *v, c
              hipStream_t s, int variant) {
     if (p.nb <= 0) return;
     if (x == out) throw InvalidError("stencil: output aliases the input");
+    if (p.nb1 >= 0 && p.nb1 < p.nb) {
+        if (!p.x2 || !p.out2 || p.x2 == p.out2 || mode == ST_JAC)
+            throw InvalidError("stencil: bad second band segment");
+    }
     if (!stencil_buffer_ok(p.n, (int)sizeof(T)))
         throw InvalidError("stencil: grid too large for 32-bit field offsets ((n^3 + 4 n^2) values must stay below "
                            "2^31 bytes)");
@@ -903,7 +912,9 @@ void stencil(int mode, const StencilParams &p, const T *x, T *out, const T *v, c
     if (ST_VARIANTS[variant].vec) {
         // vector lanes need 16-byte aligned bases (every band then stays aligned since n is a multiple of the width)
         auto misaligned = [](const void *q) { return q && (reinterpret_cast<uintptr_t>(q) & 15) != 0; };
-        if (misaligned(x) || misaligned(out) || misaligned(v) || misaligned(r)) variant = pick_variant(p.n, p.nb);
+        if (misaligned(x) || misaligned(out) || misaligned(v) || misaligned(r) || misaligned(p.x2) ||
+            misaligned(p.out2))
+            variant = pick_variant(p.n, p.nb);
     }
     const StVariant &sv = ST_VARIANTS[variant];
     if (sv.win == 12) stencil_dispatch<T, 12>(mode, sv, p, x, out, v, r, lam, s);
@@ -1343,6 +1354,30 @@ template <typename T> void residual(T *r, const T *hx, const T *x, const double 
     LIRA_HIP(hipGetLastError());
 }
 
+// R = HX - theta X and the preconditioner's first pass W = R / max(diag_t + V - lam, 0.1) in one pass (the same
+// per-point formulas as k_residual and k_precond_init)
+template <typename T>
+__global__ void k_residual_precond(T *r, T *w, const T *hx, const T *x, const double *theta, const T *v,
+                                   const double *lam, T diag_t, int nb, long N) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    const int b = blockIdx.y;
+    if (i >= N) return;
+    const long id = (long)b * N + i;
+    const T rv = (T)((double)hx[id] - theta[b] * (double)x[id]);
+    T d = diag_t + v[i] - (T)lam[b];
+    d = d > T(0.1) ? d : T(0.1);
+    r[id] = rv;
+    w[id] = rv / d;
+}
+template <typename T>
+void residual_precond(T *r, T *w, const T *hx, const T *x, const double *theta, const T *v, const double *lam,
+                      double diag_t, int nb, long N, hipStream_t s) {
+    if (nb <= 0) return;
+    hipLaunchKernelGGL(k_residual_precond<T>, dim3((unsigned)((N + 255) / 256), nb), dim3(256), 0, s, r, w, hx, x,
+                       theta, v, lam, (T)diag_t, nb, N);
+    LIRA_HIP(hipGetLastError());
+}
+
 template <typename T>
 __global__ void k_precond_init(T *w, const T *r, const T *v, const double *lam, T diag_t, int nb, long N) {
     const long i = blockIdx.x * 256L + threadIdx.x;
@@ -1669,6 +1704,8 @@ void hf_forces(const T *rho, int n, double h, const double *pos, const double *z
                              hipStream_t, int);                                                                       \
     template void reduce_jobs<T>(const RedJobs<T> &, int, double, double *, double *, hipStream_t);                 \
     template void gram<T>(const Cols &, const Cols &, long, double, double *, size_t, double *, hipStream_t);       \
+    template void residual_precond<T>(T *, T *, const T *, const T *, const double *, const T *, const double *,     \
+                                      double, int, long, hipStream_t);                                         \
     template void lincomb<T>(const Cols &, const double *, int, int, T *, long, hipStream_t, int, T *);                       \
     template void fill<T>(T *, long, double, hipStream_t);                                                           \
     template void axpy<T>(T *, const T *, long, double, hipStream_t);                                                \

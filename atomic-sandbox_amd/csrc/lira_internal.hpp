@@ -88,6 +88,10 @@ size_t gram_partials_needed(int m, int q, long N);
 template <typename T>
 void lincomb(const Cols &S, const double *C, int ldc, int q, T *Y, long N, hipStream_t s, int qsplit = -1,
              T *Y2 = nullptr);
+// R = HX - theta X and W = R / max(diag_t + V - lam, 0.1) for nb bands in one pass
+template <typename T>
+void residual_precond(T *r, T *w, const T *hx, const T *x, const double *theta, const T *v, const double *lam,
+                      double diag_t, int nb, long N, hipStream_t s);
 // the padded shape lincomb's coefficients must have: lincomb_rows(m) rows of lincomb_ldc(q) doubles, zero beyond m x q
 int lincomb_rows(int m);
 int lincomb_ldc(int q);
@@ -100,6 +104,11 @@ struct StencilParams {
     double scale;   // ST_LAP: out = scale * lap x
     double omega;   // ST_JAC damping
     double diag_t;  // ST_JAC: diagonal of -1/2 lap, -1.5 c0 / h^2
+    // a second band segment (ST_HAM, ST_LAP): bands nb1 .. nb - 1 read x2 and write out2 (band b at (b - nb1) N);
+    // nb1 < 0: every band in the first segment
+    int nb1 = -1;
+    const void *x2 = nullptr;
+    void *out2 = nullptr;
 };
 // ST_HAM: out_b = -1/2 lap x_b + v x_b. ST_JAC: out_b = x_b + omega/max(diag_t + v - lam_b, 0.1) (r_b - (H - lam_b) x_b).
 template <typename T>

@@ -507,6 +507,22 @@ template <typename T> class Solver final : public SolverIface {
         stencil<T>(ST_HAM, sp, x, out, v, nullptr, nullptr, stream_, variant_ham_);
         if (nl_rows_ > 0) nl_apply_block(x, out, nb);  // + V_NL (qm/src/nonlocal.rs:165-174)
     }
+    // H applied to two blocks in one stencil launch (x1: n1 bands, x2: n2 bands); V_NL per block
+    void hamiltonian2(const T *v, const T *x1, T *out1, int n1, const T *x2, T *out2, int n2) {
+        if (n2 <= 0) {
+            hamiltonian(v, x1, out1, n1);
+            return;
+        }
+        StencilParams sp{n_, n1 + n2, h_, 1.0, 0.0, 0.0};
+        sp.nb1 = n1;
+        sp.x2 = x2;
+        sp.out2 = out2;
+        stencil<T>(ST_HAM, sp, x1, out1, v, nullptr, nullptr, stream_, variant_ham_);
+        if (nl_rows_ > 0) {
+            nl_apply_block(x1, out1, n1);
+            nl_apply_block(x2, out2, n2);
+        }
+    }
     void tune_stencil(int nb) {
         if (variant_ham_ >= 0 || nb <= 0) return;
         variant_ham_ = tune_stencil_mode(ST_HAM, nb);
@@ -890,15 +906,14 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
     StencilParams jp{n_, k, h_, 1.0, omega, diag_t};
     for (;;) {
         upload(dth, theta.data(), k);
-        residual<T>(c.R.template as<T>(), c.HX.template as<T>(), X(), dth, k, N_, stream_);
-        precond_init<T>(c.W.template as<T>(), c.R.template as<T>(), v, dshift, diag_t, k, N_, stream_);
+        residual_precond<T>(c.R.template as<T>(), c.W.template as<T>(), c.HX.template as<T>(), X(), dth, v, dshift,
+                            diag_t, k, N_, stream_);
         for (int s = 0; s < 3; s++) {
             stencil<T>(ST_JAC, jp, c.W.template as<T>(), c.TMP.template as<T>(), v, c.R.template as<T>(), dshift, stream_,
                        variant_jac_);
             std::swap(c.W, c.TMP);
         }
-        hamiltonian(v, c.W.template as<T>(), c.HW.template as<T>(), k);
-        if (np > 0) hamiltonian(v, c.P.template as<T>(), c.HP.template as<T>(), np);
+        hamiltonian2(v, c.W.template as<T>(), c.HW.template as<T>(), k, c.P.template as<T>(), c.HP.template as<T>(), np);
         const int m = 2 * k + np, q = 2 * m;
         ensure_gram(m, q);
         Cols S{{{X(), k}, {c.W.ptr, k}, {c.P.ptr, np}}, np > 0 ? 3 : 2};
