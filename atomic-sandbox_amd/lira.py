@@ -209,6 +209,22 @@ def default_settings(h: float, occupations: int = FERMI_DIRAC, **kw) -> Settings
     return s
 
 
+def sigma_ladder(h: float):
+    """The four nuclear smearing widths of a sigma -> 0 series: 1, 1.2, 1.4, 1.6 h (crates/validate/src/lib.rs:87)."""
+    return [f * h for f in (1.0, 1.2, 1.4, 1.6)]
+
+
+def extrapolate_sigma(sigmas, values) -> float:
+    """Point-nucleus limit of a smeared-nucleus series (crates/qm/src/scf.rs:1122-1127): solves
+    v_i = a + b s_i^2 + c s_i^3 + d s_i^4 for the four (s_i, v_i) and returns a. Host arithmetic only."""
+    s = np.asarray(sigmas, dtype=np.float64)
+    v = np.asarray(values, dtype=np.float64)
+    if s.shape != (4,) or v.shape != (4,) or len(set(s.tolist())) != 4:
+        raise Invalid("extrapolate_sigma: four distinct sigmas and four values")
+    a = np.stack([np.ones(4), s ** 2, s ** 3, s ** 4], axis=1)
+    return float(np.linalg.solve(a, v)[0])
+
+
 def nuclei_array(pos, z):
     pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
     arr = (Nucleus * len(pos))()

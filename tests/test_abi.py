@@ -107,3 +107,16 @@ def test_free_atom_levels_match_lda_reference_data(z):
     got = dict(zip(["1s", "2s", "3s", "2p", "3p"], eps))
     for level, want in NIST_LDA[z].items():
         assert abs(got[level] - want) < 5e-4, (z, level, got[level], want)
+
+
+def test_extrapolate_sigma_recovers_the_intercept_of_the_series():
+    # the model a + b s^2 + c s^3 + d s^4 (crates/qm/src/scf.rs:1122-1127) is recovered exactly on the ladder
+    s = lira.sigma_ladder(0.15)
+    assert s == pytest.approx([0.15, 0.18, 0.21, 0.24])
+    a, b, c, d = -0.478671, 0.31, -1.7, 2.9
+    v = [a + b * x ** 2 + c * x ** 3 + d * x ** 4 for x in s]
+    assert abs(lira.extrapolate_sigma(s, v) - a) < 1e-11
+    # a first-order (linear in s) term is not in the model: the intercept then carries its bias, not 0
+    assert abs(lira.extrapolate_sigma(s, [a + 0.1 * x for x in s]) - a) > 1e-3
+    with pytest.raises(lira.Invalid):
+        lira.extrapolate_sigma([0.1, 0.1, 0.2, 0.3], [1.0, 2.0, 3.0, 4.0])
