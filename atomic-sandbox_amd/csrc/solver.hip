@@ -244,6 +244,7 @@ struct SolverIface {
     virtual void get_profile(lira_profile *out, double *drho, double *etol, int cap, int *n) const = 0;
     virtual void sample(const double *pts, int count, double *out) = 0;
     virtual void set_graphs(bool on) = 0;
+    virtual void state_summary(double *spin_abs, double *rho_max) = 0;
 };
 
 // One pseudised species as the solver keeps it: scalars plus offsets of its tables in the f64 table arena.
@@ -307,6 +308,22 @@ template <typename T> class Solver final : public SolverIface {
         }
     }
     void sample(const double *pts, int count, double *out) override;
+    // GpuSolver::spin_abs_integral and density_max (crates/qm-gpu/src/scf.rs:1834-1845): one reduction pass over the
+    // converged state, one readback
+    void state_summary(double *spin_abs, double *rho_max) override {
+        if (!valid_) throw InvalidError("lira_state_summary: no converged state");
+        add2<T>(rho_t_.template as<T>(), rho_out_[0].template as<T>(), rho_out_[1].template as<T>(), N_, stream_);
+        RedJobs<T> j{};
+        j.a[0] = rho_out_[0].template as<T>();
+        j.b[0] = rho_out_[1].template as<T>();
+        j.kind[0] = RED_ABSDIFF;
+        j.a[1] = j.b[1] = rho_t_.template as<T>();
+        j.kind[1] = RED_MAX_ABS;
+        j.count = 2;
+        const std::vector<double> r = reduce(j, dv_);
+        *spin_abs = r[0];
+        *rho_max = r[1];
+    }
     void set_graphs(bool on) override {
         if (!on) drop_graphs();
         graphs_ = on;
@@ -2289,6 +2306,11 @@ lira_status lira_get_profile(lira_solver *s, lira_profile *out, double *delta_rh
 lira_status lira_sample_density(lira_solver *s, const double *points, int count, double *out) {
     if (!s) return LIRA_INVALID;
     return guarded(s, [&] { s->impl->sample(points, count, out); });
+}
+
+lira_status lira_state_summary(lira_solver *s, double *spin_abs_integral, double *density_max) {
+    if (!s || !spin_abs_integral || !density_max) return LIRA_INVALID;
+    return guarded(s, [&] { s->impl->state_summary(spin_abs_integral, density_max); });
 }
 
 lira_status lira_set_graphs(lira_solver *s, int on) {

@@ -31,6 +31,7 @@ EXPORTS = [
     "lira_autotune_stencil", "lira_set_stencil_variant", "lira_version", "lira_set_ionic", "lira_force_terms",
     "lira_ionic_fields", "lira_nonlocal_apply", "lira_free_atom_levels", "lira_set_witness_cap", "lira_witness_cap",
     "lira_set_density_extrapolation", "lira_get_profile", "lira_sample_density", "lira_set_graphs",
+    "lira_state_summary",
 ]
 
 
@@ -183,6 +184,7 @@ def lib():
         L.lira_get_profile.argtypes = [vp, C.POINTER(Profile), _dp, _dp, C.c_int, C.POINTER(C.c_int)]
         L.lira_sample_density.argtypes = [vp, _dp, C.c_int, _dp]
         L.lira_set_graphs.argtypes = [vp, C.c_int]
+        L.lira_state_summary.argtypes = [vp, _dp, _dp]
         L.lira_force_terms.argtypes = [vp, C.POINTER(Nucleus), C.c_int, C.c_double, _dp]
         L.lira_ionic_fields.argtypes = [vp, C.POINTER(Nucleus), C.c_int, C.c_double, vp, vp]
         L.lira_nonlocal_apply.argtypes = [vp, _dp, C.c_int, _dp, vp, C.c_int, C.POINTER(C.c_int)]
@@ -353,6 +355,12 @@ class GpuSolver:
         out = np.zeros(len(pts))
         _raise(lib().lira_sample_density(self._h, pts.reshape(-1), len(pts), out), self._h, "lira_sample_density")
         return out
+
+    def state_summary(self):
+        """(integral of |rho_up - rho_dn|, max of rho) of the converged state (lira_state_summary)."""
+        a, m = np.zeros(1), np.zeros(1)
+        _raise(lib().lira_state_summary(self._h, a, m), self._h, "lira_state_summary")
+        return float(a[0]), float(m[0])
 
     def set_graphs(self, on):
         """Replay the fixed kernel sequences as HIP graphs (on, the default) or launch kernels one by one

@@ -197,6 +197,10 @@ lira_status lira_get_profile(lira_solver *s, lira_profile *out, double *delta_rh
  * z; crates/qm/src/grid.rs:54-86); 0 outside [origin, origin + (n-1)h]^3. */
 lira_status lira_sample_density(lira_solver *s, const double *points, int count, double *out);
 
+/* GpuSolver::spin_abs_integral and density_max (crates/qm-gpu/src/scf.rs:1834-1845) of the converged state: the
+ * integral of |rho_up - rho_dn| (electrons; the provider's spin guard) and the maximum of rho_up + rho_dn (1/Bohr^3). */
+lira_status lira_state_summary(lira_solver *s, double *spin_abs_integral, double *density_max);
+
 /* Dispatch control, no counterpart in the reference (its recorder batches dispatches instead,
  * crates/qm-gpu/src/batch.rs:55-233): on (the default), the fixed kernel sequences between two host syncs (the
  * Poisson MG-PCG iteration and its start, the LOBPCG step) are captured once as HIP graphs and replayed; off, every

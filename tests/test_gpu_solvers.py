@@ -332,3 +332,21 @@ def test_config1_h2_48_fermi_dirac_fp64_matches_oracle(lira_mod):
     assert abs(ev[0] - np.sort(out["evals_up"])[0]) < 1e-8
     assert np.abs(f - out["forces"]).max() < 1e-8
     assert abs(occ.sum() - 1.0) < 1e-12
+
+
+def test_state_summary_spin_integral_and_density_max(lira_mod):
+    # GpuSolver::spin_abs_integral / density_max (crates/qm-gpu/src/scf.rs:1834-1845): a singlet's channels are the
+    # same field (0 exactly); a lone H electron's spin integral is its electron count; the maximum is the field's
+    lira = lira_mod
+    n, h = 33, 0.25
+    for prec in (0, 1):
+        s = lira.GpuSolver(n, h, prec)
+        s.solve(H2[0], H2[1], 1, 1, lira.default_settings(h, 1))
+        a, m = s.state_summary()
+        assert a == 0.0 and m == s.field(2).max()
+        s = lira.GpuSolver(n, h, prec)
+        st = s.solve([[0.1, 0.0, -0.2]], [1], 1, 0, lira.default_settings(h, 1))
+        a, m = s.state_summary()
+        rho = s.field(2)
+        assert abs(a - st.electrons[0]) < 1e-6 * (1 if prec == 0 else 10) and m == rho.max()
+        assert abs(a - np.abs(s.field(0) - s.field(1)).sum() * h ** 3) < 1e-9
