@@ -19,6 +19,7 @@
 
 #include "lira.h"
 #include "lira_internal.hpp"
+#include "provider_policy.hpp"
 
 namespace lira {
 
@@ -245,6 +246,8 @@ struct SolverIface {
     virtual void sample(const double *pts, int count, double *out) = 0;
     virtual void set_graphs(bool on) = 0;
     virtual void state_summary(double *spin_abs, double *rho_max) = 0;
+    virtual bool ionic() const = 0;
+    virtual bool density_extrapolation() const = 0;
 };
 
 // One pseudised species as the solver keeps it: scalars plus offsets of its tables in the f64 table arena.
@@ -324,6 +327,8 @@ template <typename T> class Solver final : public SolverIface {
         *spin_abs = r[0];
         *rho_max = r[1];
     }
+    bool ionic() const override { return !species_.empty(); }
+    bool density_extrapolation() const override { return extrapolate_; }
     void set_graphs(bool on) override {
         if (!on) drop_graphs();
         graphs_ = on;
@@ -2283,6 +2288,54 @@ lira_status lira_solve(lira_solver *s, const lira_system *sys, const lira_settin
 lira_status lira_forces(lira_solver *s, const lira_nucleus *nuclei, int n_nuclei, double sigma, double *out) {
     if (!s || !nuclei || !out || n_nuclei <= 0) return LIRA_INVALID;
     return guarded(s, [&] { s->impl->forces(nuclei, n_nuclei, sigma, out); });
+}
+
+double lira_effective_gpu_tol(double tol, int n_valence) { return effective_gpu_tol(tol, n_valence); }
+
+lira_status lira_evaluate(lira_solver *s, const lira_system *sys, const lira_settings *settings, int warm,
+                          lira_abort_fn abort, void *user, lira_scf_stats *stats, double *forces, int *flags,
+                          double *tol_used) {
+    if (!s || !sys || !settings || !stats || !forces || !flags || !tol_used || sys->n_nuclei <= 0) return LIRA_INVALID;
+    lira_settings set = *settings;
+    int fl = 0;
+    // the floor-aware stop: under an ionic setup the tolerance is raised to what the f32 density can reach
+    if (s->impl->ionic()) {
+        const double te = effective_gpu_tol(set.tol, (long)sys->n_up + sys->n_dn);
+        if (te > set.tol) {
+            set.tol = te;
+            fl |= LIRA_EVAL_TOL_RAISED;
+        }
+    }
+    // R15: a warm attempt that did not converge or stalled is retried once cold, density extrapolation off for that
+    // solve and back as it was afterwards; the failed attempt's steps and readbacks count in the result
+    const bool extrapolate = s->impl->density_extrapolation();
+    lira_scf_stats failed{};
+    bool retried = false;
+    lira_status first = LIRA_OK;
+    const lira_status r = retry_cold_on_stall(
+        warm != 0, LIRA_NOT_CONVERGED, LIRA_STALLED,
+        [&](bool w) {
+            if (warm && !w) {
+                failed = *stats;
+                s->impl->set_density_extrapolation(false);
+            }
+            return lira_solve(s, sys, &set, w ? 1 : 0, abort, user, stats);
+        },
+        &retried, &first);
+    if (retried) {
+        s->impl->set_density_extrapolation(extrapolate);
+        fl |= LIRA_EVAL_RETRIED_COLD | (first == LIRA_STALLED ? LIRA_EVAL_WARM_STALLED : 0);
+        stats->iterations += failed.iterations;
+        stats->readbacks += failed.readbacks;
+    }
+    *flags = fl;
+    *tol_used = set.tol;
+    if (r != LIRA_OK) return r;
+    lira_profile prof{};
+    const lira_status pg = guarded(s, [&] { s->impl->get_profile(&prof, nullptr, nullptr, 0, nullptr); });
+    if (pg != LIRA_OK) return pg;
+    if (prof.extrapolated) *flags |= LIRA_EVAL_EXTRAPOLATED;
+    return lira_forces(s, sys->nuclei, sys->n_nuclei, set.sigma, forces);
 }
 
 lira_status lira_set_witness_cap(lira_solver *s, int cap) {

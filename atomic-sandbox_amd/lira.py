@@ -31,7 +31,7 @@ EXPORTS = [
     "lira_autotune_stencil", "lira_set_stencil_variant", "lira_version", "lira_set_ionic", "lira_force_terms",
     "lira_ionic_fields", "lira_nonlocal_apply", "lira_free_atom_levels", "lira_set_witness_cap", "lira_witness_cap",
     "lira_set_density_extrapolation", "lira_get_profile", "lira_sample_density", "lira_set_graphs",
-    "lira_state_summary",
+    "lira_state_summary", "lira_effective_gpu_tol", "lira_evaluate",
 ]
 
 
@@ -185,6 +185,10 @@ def lib():
         L.lira_sample_density.argtypes = [vp, _dp, C.c_int, _dp]
         L.lira_set_graphs.argtypes = [vp, C.c_int]
         L.lira_state_summary.argtypes = [vp, _dp, _dp]
+        L.lira_effective_gpu_tol.argtypes = [C.c_double, C.c_int]
+        L.lira_effective_gpu_tol.restype = C.c_double
+        L.lira_evaluate.argtypes = [vp, C.POINTER(System), C.POINTER(Settings), C.c_int, ABORT_FN, vp,
+                                    C.POINTER(ScfStats), _dp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.lira_force_terms.argtypes = [vp, C.POINTER(Nucleus), C.c_int, C.c_double, _dp]
         L.lira_ionic_fields.argtypes = [vp, C.POINTER(Nucleus), C.c_int, C.c_double, vp, vp]
         L.lira_nonlocal_apply.argtypes = [vp, _dp, C.c_int, _dp, vp, C.c_int, C.POINTER(C.c_int)]
@@ -209,6 +213,14 @@ def default_settings(h: float, occupations: int = FERMI_DIRAC, **kw) -> Settings
     for k, v in kw.items():
         setattr(s, k, v)
     return s
+
+
+EVAL_TOL_RAISED, EVAL_RETRIED_COLD, EVAL_WARM_STALLED, EVAL_EXTRAPOLATED = 1, 2, 4, 8
+
+
+def effective_gpu_tol(tol: float, n_valence: int) -> float:
+    """max(tol, 2 * 5e-8 * n_valence), the SCF tolerance the GPU reaches (lira_effective_gpu_tol)."""
+    return lib().lira_effective_gpu_tol(tol, n_valence)
 
 
 def sigma_ladder(h: float):
@@ -281,6 +293,21 @@ class GpuSolver:
         st = lib().lira_solve(self._h, C.byref(sysd), C.byref(s), int(warm), cb, None, C.byref(stats))
         _raise(st, self._h, "lira_solve")
         return stats
+
+    def evaluate(self, pos, z, n_up, n_dn, settings: Settings | None = None, warm=False, abort=None):
+        """lira_evaluate, one provider evaluation (floor-aware tolerance, the cold retry of a failed warm solve,
+        forces): returns (status, stats, forces (nuclei x 3), flags, tol_used); a failed solve is a status, not an
+        exception, so its flags stay readable."""
+        nuc = nuclei_array(pos, z)
+        sysd = System(C.cast(nuc, C.POINTER(Nucleus)), len(nuc), n_up, n_dn)
+        s = settings if settings is not None else default_settings(self.h)
+        stats = ScfStats()
+        f = np.zeros(3 * len(nuc))
+        flags, tol = C.c_int(), C.c_double()
+        cb = ABORT_FN(lambda _u: int(bool(abort()))) if abort else ABORT_FN(0)
+        st = lib().lira_evaluate(self._h, C.byref(sysd), C.byref(s), int(warm), cb, None, C.byref(stats), f,
+                                 C.byref(flags), C.byref(tol))
+        return st, stats, f.reshape(-1, 3), flags.value, tol.value
 
     def set_ionic(self, species):
         """lira_set_ionic: species are objects with the attributes of synthetic_tables.Species (z, z_v, sigma_lr,

@@ -201,6 +201,25 @@ lira_status lira_sample_density(lira_solver *s, const double *points, int count,
  * integral of |rho_up - rho_dn| (electrons; the provider's spin guard) and the maximum of rho_up + rho_dn (1/Bohr^3). */
 lira_status lira_state_summary(lira_solver *s, double *spin_abs_integral, double *density_max);
 
+/* ---- the provider's evaluation around GpuSolver (providers::ks::GpuKs::evaluate_once) ---- */
+/* max(tol, 2 * 5e-8 * n_valence): the SCF tolerance the GPU's f32 density can reach (effective_gpu_tol,
+ * crates/providers/src/pseudo.rs:423-440). Host only. */
+double lira_effective_gpu_tol(double tol, int n_valence);
+typedef enum {
+    LIRA_EVAL_TOL_RAISED = 1,    /* ionic setup: tol raised to lira_effective_gpu_tol */
+    LIRA_EVAL_RETRIED_COLD = 2,  /* the warm attempt failed and was retried cold (R15) */
+    LIRA_EVAL_WARM_STALLED = 4,  /* ... and that warm attempt had stalled (else it ran out of steps) */
+    LIRA_EVAL_EXTRAPOLATED = 8   /* the converged solve started from an extrapolated density */
+} lira_eval_flag;
+/* One evaluation as GpuKs::evaluate_once drives the solver (crates/providers/src/ks.rs:1573-1642): under an ionic
+ * setup the tolerance is raised to the GPU floor (raise_to_gpu_floor, ks.rs:543-558); a warm solve that ends
+ * LIRA_NOT_CONVERGED or LIRA_STALLED is retried once cold with density extrapolation off for that solve
+ * (retry_cold_on_stall, ks.rs:585-601), the failed attempt's steps and readbacks added to *stats; then the forces
+ * (3 n doubles). Returns the last attempt's status; *flags (lira_eval_flag) and *tol_used are set in every case. */
+lira_status lira_evaluate(lira_solver *s, const lira_system *sys, const lira_settings *settings, int warm,
+                          lira_abort_fn abort, void *user, lira_scf_stats *stats, double *forces, int *flags,
+                          double *tol_used);
+
 /* Dispatch control, no counterpart in the reference (its recorder batches dispatches instead,
  * crates/qm-gpu/src/batch.rs:55-233): on (the default), the fixed kernel sequences between two host syncs (the
  * Poisson MG-PCG iteration and its start, the LOBPCG step) are captured once as HIP graphs and replayed; off, every

@@ -350,3 +350,40 @@ def test_state_summary_spin_integral_and_density_max(lira_mod):
         rho = s.field(2)
         assert abs(a - st.electrons[0]) < 1e-6 * (1 if prec == 0 else 10) and m == rho.max()
         assert abs(a - np.abs(s.field(0) - s.field(1)).sum() * h ** 3) < 1e-9
+
+
+def test_provider_evaluation_policies(lira_mod):
+    # GpuKs::evaluate_once's drive of the solver (crates/providers/src/ks.rs:1573-1642): forces of the converged
+    # state; R15 (a failed warm attempt retried once cold, its steps counted); the extrapolated-start flag
+    lira = lira_mod
+    n, h = 33, 0.25
+    s = lira.GpuSolver(n, h, 1)
+    s.set_density_extrapolation(True)
+    st = lira.default_settings(h, 1)
+    traj = [0.70, 0.71, 0.72]
+    for i, x in enumerate(traj):
+        pos = [[x, 0.0, 0.0], [-0.7, 0.0, 0.0]]
+        code, stats, f, flags, tol = s.evaluate(pos, [1, 1], 1, 1, st, warm=i > 0)
+        assert code == lira.OK and tol == st.tol and not flags & lira.EVAL_TOL_RAISED  # all-electron: untouched
+        assert np.array_equal(f, s.forces(pos, [1, 1], h))
+        assert bool(flags & lira.EVAL_EXTRAPOLATED) == (i == 2)
+    # a warm attempt that runs out of steps is retried cold; with the same cap the cold one fails too, and its
+    # status is the result; both attempts' steps are counted
+    one = lira.default_settings(h, 1, max_iter=1)
+    code, stats, _, flags, _ = s.evaluate([[0.74, 0.0, 0.0], [-0.7, 0.0, 0.0]], [1, 1], 1, 1, one, warm=True)
+    assert code == lira.NOT_CONVERGED and flags & lira.EVAL_RETRIED_COLD and not flags & lira.EVAL_WARM_STALLED
+    assert stats.iterations == 2
+
+
+def test_provider_evaluation_raises_the_tolerance_to_the_gpu_floor(lira_mod):
+    # under an ionic setup tol is raised to max(tol, 2 * 5e-8 * N_valence) (crates/providers/src/pseudo.rs:437-440):
+    # pseudised water has 8 valence electrons, so 1e-9 becomes 8e-7
+    import synthetic_tables as T
+    lira = lira_mod
+    n, h = 25, 0.3
+    s = lira.GpuSolver(n, h, 1)
+    s.set_ionic([T.oxygen_like(h)])
+    pos, z = [[0.05, -0.03, 0.02], [1.43, 1.11, 0.0], [-1.43, 1.11, 0.0]], [8, 1, 1]
+    code, stats, f, flags, tol = s.evaluate(pos, z, 4, 4, lira.default_settings(h, 1, tol=1e-9))
+    assert code == lira.OK and flags & lira.EVAL_TOL_RAISED and tol == pytest.approx(8e-7)
+    assert stats.delta_rho < 8e-7 and np.array_equal(f, s.forces(pos, z, h))
