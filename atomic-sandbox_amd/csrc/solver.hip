@@ -473,6 +473,11 @@ template <typename T> class Solver final : public SolverIface {
         readbacks_++;
         return hbuf_;
     }
+    // a blocking copy ordered on this solver's stream (which does not synchronise with the null stream)
+    void copy_now(void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+        LIRA_HIP(hipMemcpyAsync(dst, src, bytes, kind, stream_));
+        LIRA_HIP(hipStreamSynchronize(stream_));
+    }
     // queues a host->device copy of count doubles through the pinned arena (no wait)
     void upload(double *d, const double *h, size_t count) {
         if (count == 0) return;
@@ -554,11 +559,10 @@ template <typename T> class Solver final : public SolverIface {
     // and returns the fastest; the choice is kept per (precision, n, nb, mode) for later solvers in the process
     int tune_stencil_mode(int mode, int nb) {
         const std::array<int, 4> key{(int)sizeof(T), n_, nb, mode};
-        {
-            std::lock_guard<std::mutex> g(g_tune_mu);
-            auto it = g_tuned.find(key);
-            if (it != g_tuned.end()) return it->second;
-        }
+        // one tuning at a time in the process: solvers on other threads that want the same key wait for its result
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) return it->second;
         DevBuf x((size_t)N_ * nb * sizeof(T)), o((size_t)N_ * nb * sizeof(T)), v((size_t)N_ * sizeof(T));
         DevBuf lam((size_t)nb * sizeof(double));
         fill<T>(x.template as<T>(), N_ * nb, 0.0, stream_);
@@ -591,7 +595,6 @@ template <typename T> class Solver final : public SolverIface {
         }
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
-        std::lock_guard<std::mutex> g(g_tune_mu);
         g_tuned[key] = pick;
         return pick;
     }
@@ -614,7 +617,8 @@ template <typename T> class Solver final : public SolverIface {
 };
 
 template <typename T> Solver<T>::Solver(int n, double h) : n_(n), h_(h), dv_(h * h * h), N_((long)n * n * n) {
-    LIRA_HIP(hipStreamCreate(&stream_));
+    // non-blocking: solvers driven from several host threads must not serialise on null-stream work
+    LIRA_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     up_cap_ = (size_t)1 << 18;
     LIRA_HIP(hipHostMalloc((void **)&up_, up_cap_ * sizeof(double), hipHostMallocDefault));
     partials_.alloc((size_t)RED_BLOCKS * std::max(MAX_JOBS, 49) * sizeof(double) * 4);
@@ -1205,7 +1209,7 @@ template <typename T> void Solver<T>::set_ionic(const lira_species *sp, int nsp)
     tab_h_ = std::move(tab);
     if (!tab_h_.empty()) {
         if (tab_.bytes < tab_h_.size() * sizeof(double)) tab_.alloc(tab_h_.size() * sizeof(double));
-        LIRA_HIP(hipMemcpy(tab_.ptr, tab_h_.data(), tab_h_.size() * sizeof(double), hipMemcpyHostToDevice));
+        copy_now(tab_.ptr, tab_h_.data(), tab_h_.size() * sizeof(double), hipMemcpyHostToDevice);
     }
     ionic_version_++;
     nl_rows_ = 0;
@@ -1282,9 +1286,9 @@ template <typename T> void Solver<T>::ionic_geometry(double sigma) {
     }
     LIRA_HIP(hipStreamSynchronize(stream_));
     if (centres_.bytes < cs.size() * sizeof(IonicCentre)) centres_.alloc(cs.size() * sizeof(IonicCentre));
-    LIRA_HIP(hipMemcpy(centres_.ptr, cs.data(), cs.size() * sizeof(IonicCentre), hipMemcpyHostToDevice));
+    copy_now(centres_.ptr, cs.data(), cs.size() * sizeof(IonicCentre), hipMemcpyHostToDevice);
     if (qf_.bytes < (size_t)(nat + 1) * sizeof(double)) qf_.alloc((size_t)(nat + 1) * sizeof(double));
-    if (nat > 0) LIRA_HIP(hipMemcpy(qf_.ptr, q_h_.data(), nat * sizeof(double), hipMemcpyHostToDevice));
+    if (nat > 0) copy_now(qf_.ptr, q_h_.data(), nat * sizeof(double), hipMemcpyHostToDevice);
     alloc_field(vnuc_);
     if (has_pc_) alloc_field(pc_);
     ionic_fields<T>(vnuc_.template as<T>(), has_pc_ ? pc_.template as<T>() : nullptr, n_, h_,
@@ -1320,11 +1324,11 @@ template <typename T> void Solver<T>::ionic_geometry(double sigma) {
         colours_.push_back(c);
     }
     if (nl_ions_.bytes < nl_h_.size() * sizeof(NlIon)) nl_ions_.alloc(nl_h_.size() * sizeof(NlIon));
-    LIRA_HIP(hipMemcpy(nl_ions_.ptr, nl_h_.data(), nl_h_.size() * sizeof(NlIon), hipMemcpyHostToDevice));
+    copy_now(nl_ions_.ptr, nl_h_.data(), nl_h_.size() * sizeof(NlIon), hipMemcpyHostToDevice);
     if (nl_members_.bytes < members.size() * sizeof(int)) nl_members_.alloc(members.size() * sizeof(int));
-    LIRA_HIP(hipMemcpy(nl_members_.ptr, members.data(), members.size() * sizeof(int), hipMemcpyHostToDevice));
+    copy_now(nl_members_.ptr, members.data(), members.size() * sizeof(int), hipMemcpyHostToDevice);
     if (nl_erow_.bytes < nl_e_h_.size() * sizeof(double)) nl_erow_.alloc(nl_e_h_.size() * sizeof(double));
-    LIRA_HIP(hipMemcpy(nl_erow_.ptr, nl_e_h_.data(), nl_e_h_.size() * sizeof(double), hipMemcpyHostToDevice));
+    copy_now(nl_erow_.ptr, nl_e_h_.data(), nl_e_h_.size() * sizeof(double), hipMemcpyHostToDevice);
     if (beta_.bytes < (size_t)beta_total * sizeof(T)) beta_.alloc((size_t)beta_total * sizeof(T));
     for (const NlIon &ion : nl_h_) nl_beta<T>(beta_.template as<T>(), ion, n_, h_, tab_.as<double>(), stream_);
 }
@@ -1401,7 +1405,7 @@ void Solver<T>::nonlocal_apply(const double *x, int nb, double *out, double *c, 
     LIRA_HIP(hipStreamSynchronize(stream_));
     if (c && nl_rows_ > 0) {
         const int cnt = std::min(c_cap, nl_rows_ * nb);
-        LIRA_HIP(hipMemcpy(c, nl_C_.ptr, cnt * sizeof(double), hipMemcpyDeviceToHost));
+        copy_now(c, nl_C_.ptr, cnt * sizeof(double), hipMemcpyDeviceToHost);
     }
 }
 
@@ -1434,7 +1438,7 @@ void Solver<T>::cold_density(bool free_atoms, double sigma, const double *charge
     LIRA_HIP(hipStreamSynchronize(stream_));
     if (!tab.empty()) {
         if (atom_tab_.bytes < tab.size() * sizeof(double)) atom_tab_.alloc(tab.size() * sizeof(double));
-        LIRA_HIP(hipMemcpy(atom_tab_.ptr, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+        copy_now(atom_tab_.ptr, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
     }
     if (atoms_dev_.bytes < (atoms.size() + 1) * sizeof(AtomProfile))
         atoms_dev_.alloc((atoms.size() + 1) * sizeof(AtomProfile));
@@ -1444,7 +1448,7 @@ void Solver<T>::cold_density(bool free_atoms, double sigma, const double *charge
     for (AtomProfile A : atoms) {
         const double q = A.w;
         A.w = 1.0;
-        LIRA_HIP(hipMemcpy(atoms_dev_.ptr, &A, sizeof(AtomProfile), hipMemcpyHostToDevice));
+        copy_now(atoms_dev_.ptr, &A, sizeof(AtomProfile), hipMemcpyHostToDevice);
         atom_density<T>(rho_t_.template as<T>(), n_, h_, atoms_dev_.as<AtomProfile>(), 1, atom_tab_.as<double>(),
                         nullptr, 0, 1.0, stream_);
         RedJobs<T> j{};
@@ -1461,10 +1465,10 @@ void Solver<T>::cold_density(bool free_atoms, double sigma, const double *charge
         }
     }
     if (!kept.empty())
-        LIRA_HIP(hipMemcpy(atoms_dev_.ptr, kept.data(), kept.size() * sizeof(AtomProfile), hipMemcpyHostToDevice));
+        copy_now(atoms_dev_.ptr, kept.data(), kept.size() * sizeof(AtomProfile), hipMemcpyHostToDevice);
     if (blobs_dev_.bytes < (blobs.size() + 4) * sizeof(double)) blobs_dev_.alloc((blobs.size() + 4) * sizeof(double));
     if (!blobs.empty())
-        LIRA_HIP(hipMemcpy(blobs_dev_.ptr, blobs.data(), blobs.size() * sizeof(double), hipMemcpyHostToDevice));
+        copy_now(blobs_dev_.ptr, blobs.data(), blobs.size() * sizeof(double), hipMemcpyHostToDevice);
     for (int s = 0; s < 2; s++)
         atom_density<T>(rho_in_[s].template as<T>(), n_, h_, atoms_dev_.as<AtomProfile>(), (int)kept.size(),
                         atom_tab_.as<double>(), blobs_dev_.as<double>(), (int)blobs.size() / 4, s == 0 ? scale0 : scale1,
@@ -1524,7 +1528,7 @@ template <typename T> void Solver<T>::seed_atomic_orbitals(double sigma) {
     }
     LIRA_HIP(hipStreamSynchronize(stream_));
     if (ao_tab_.bytes < tab.size() * sizeof(double)) ao_tab_.alloc(tab.size() * sizeof(double));
-    LIRA_HIP(hipMemcpy(ao_tab_.ptr, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    copy_now(ao_tab_.ptr, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
     for (int s = 0; s < 2; s++) {
         Channel<T> &c = ch_[s];
         for (int col = 0; col < std::min(c.k, (int)ao_cols_.size()); col++)

@@ -31,7 +31,7 @@ EXPORTS = [
     "lira_autotune_stencil", "lira_set_stencil_variant", "lira_version", "lira_set_ionic", "lira_force_terms",
     "lira_ionic_fields", "lira_nonlocal_apply", "lira_free_atom_levels", "lira_set_witness_cap", "lira_witness_cap",
     "lira_set_density_extrapolation", "lira_get_profile", "lira_sample_density", "lira_set_graphs",
-    "lira_state_summary", "lira_effective_gpu_tol", "lira_evaluate",
+    "lira_state_summary", "lira_effective_gpu_tol", "lira_evaluate", "lira_solve_batch",
 ]
 
 
@@ -121,6 +121,18 @@ class ScfStats(C.Structure):
                 ("seconds_poisson", C.c_double), ("seconds_eigen", C.c_double), ("seconds_other", C.c_double)]
 
 
+class BatchJob(C.Structure):
+    _fields_ = [("n", C.c_int), ("h", C.c_double), ("system", System)]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("stats", ScfStats), ("worker", C.c_int), ("start", C.c_double),
+                ("end", C.c_double), ("message", C.c_char * 128)]
+
+
+MAX_GRID, BATCH_MAX_WORKERS = 644, 64
+
+
 class PoissonStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("outer", C.c_int), ("residual", C.c_double), ("seconds", C.c_double)]
 
@@ -192,6 +204,8 @@ def lib():
         L.lira_force_terms.argtypes = [vp, C.POINTER(Nucleus), C.c_int, C.c_double, _dp]
         L.lira_ionic_fields.argtypes = [vp, C.POINTER(Nucleus), C.c_int, C.c_double, vp, vp]
         L.lira_nonlocal_apply.argtypes = [vp, _dp, C.c_int, _dp, vp, C.c_int, C.POINTER(C.c_int)]
+        L.lira_solve_batch.argtypes = [C.POINTER(BatchJob), C.c_int, C.c_int, C.POINTER(Settings), C.c_int,
+                                       C.POINTER(BatchResult), vp]
         L.lira_set_stencil_variant.restype = None
         L.lira_version.restype = C.c_char_p
         _lib = L
@@ -246,6 +260,61 @@ def nuclei_array(pos, z):
         arr[i].pos[:] = [float(p[0]), float(p[1]), float(p[2])]
         arr[i].z = int(zz)
     return arr
+
+
+@dataclass
+class Molecule:
+    """One all-electron molecule of a batch on its own n^3 grid at spacing h (Bohr)."""
+    n: int
+    h: float
+    pos: np.ndarray
+    z: list
+    n_up: int
+    n_dn: int
+    name: str = ""
+
+
+@dataclass
+class BatchOutcome:
+    status: int
+    stats: ScfStats
+    worker: int
+    start: float
+    end: float
+    message: str
+    forces: np.ndarray
+
+
+def solve_batch(molecules, precision: int = MIXED, workers: int = 8, settings=None, forces: bool = True):
+    """lira_solve_batch: a cold SCF and the forces of every molecule on the current HIP device, over `workers` host
+    threads that each own a solver. settings: None (for_grid(h) per molecule) or one Settings per molecule. Returns one
+    BatchOutcome per molecule in input order; a failed molecule is a status, not an exception."""
+    mols = list(molecules)
+    nj = len(mols)
+    keep = []
+    jobs = (BatchJob * max(nj, 1))()
+    for i, m in enumerate(mols):
+        nuc = nuclei_array(m.pos, m.z)
+        keep.append(nuc)
+        jobs[i].n, jobs[i].h = int(m.n), float(m.h)
+        jobs[i].system = System(C.cast(nuc, C.POINTER(Nucleus)), len(nuc), int(m.n_up), int(m.n_dn))
+    sets = None
+    if settings is not None:
+        if len(settings) != nj:
+            raise Invalid("solve_batch: one Settings per molecule")
+        sets = (Settings * nj)(*settings)
+    res = (BatchResult * max(nj, 1))()
+    off = np.concatenate([[0], np.cumsum([3 * len(np.asarray(m.z)) for m in mols])]).astype(int)
+    fout = np.zeros(max(int(off[-1]), 1))
+    st = lib().lira_solve_batch(jobs, nj, int(precision), C.cast(sets, C.POINTER(Settings)) if sets else None,
+                                int(workers), res, fout.ctypes.data if forces else None)
+    _raise(st, None, "lira_solve_batch")
+    out = []
+    for i in range(nj):
+        r = res[i]
+        out.append(BatchOutcome(r.status, r.stats, r.worker, r.start, r.end, r.message.decode(errors="replace"),
+                                fout[off[i]:off[i + 1]].reshape(-1, 3).copy() if forces else None))
+    return out
 
 
 @dataclass

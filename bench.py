@@ -131,6 +131,45 @@ def pseudo_water(lira, n=95, h=0.15):
     return out
 
 
+def batch_scf(lira, count=256, workers=8, seed=0, seq_count=32):
+    """Batched SCF: `count` all-electron molecules drawn from H2, LiH, H2O, NH3, CH4 (equilibrium geometry plus a seeded
+    uniform +-0.1 A per nucleus and coordinate), each on an n^3 grid with n drawn from 48, 56, ..., 96 at h = 0.25 Bohr,
+    MIXED precision, cold SCF (delta_rho < 1e-6, Fermi-Dirac kT 5 mHa) + forces, over `workers` host threads that
+    each own a solver. An untimed pass over one molecule of every (species, grid) pair first runs the one-time stencil
+    tuning. Reports converged SCFs/s of the whole batch, p50/p99 of the per-molecule time (its solve and forces on its
+    worker, plus a solver rebuild when the worker's grid changed), and the same first `seq_count` molecules run one at
+    a time (one worker) for comparison."""
+    import molecules as M
+    mols = M.sample(count, seed=seed)
+    seen, warm = set(), []
+    for m in mols:
+        if (m.name, m.n) not in seen:
+            seen.add((m.name, m.n))
+            warm.append(m)
+    lira.solve_batch(warm, precision=lira.MIXED, workers=workers)
+    t0 = time.perf_counter()
+    out = lira.solve_batch(mols, precision=lira.MIXED, workers=workers)
+    wall = time.perf_counter() - t0
+    ok = [o for o in out if o.status == lira.OK]
+    per = np.array([o.end - o.start for o in ok])
+    t1 = time.perf_counter()
+    seq = lira.solve_batch(mols[:seq_count], precision=lira.MIXED, workers=1)
+    seq_wall = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    lira.solve_batch(mols[:seq_count], precision=lira.MIXED, workers=workers)
+    sub_wall = time.perf_counter() - t2
+    names = sorted({m.name for m in mols})
+    return {"molecules": count, "workers": workers, "converged": len(ok), "seconds": wall,
+            "scf_per_s": len(ok) / wall, "p50_s": float(np.percentile(per, 50)) if len(per) else None,
+            "p99_s": float(np.percentile(per, 99)) if len(per) else None,
+            "steps_mean": float(np.mean([o.stats.iterations for o in ok])) if ok else None,
+            "mix": {k: sum(m.name == k for m in mols) for k in names},
+            "grids": sorted({m.n for m in mols}), "h_bohr": mols[0].h,
+            "one_at_a_time": {"molecules": seq_count, "seconds": seq_wall,
+                              "scf_per_s": sum(o.status == lira.OK for o in seq) / seq_wall,
+                              "batched_seconds_same_set": sub_wall, "speedup": seq_wall / sub_wall}}
+
+
 def traffic_probe(esize, variant, n=192, nb=64):
     """Child of hbm_traffic, run under rocprofv3 --pmc: two copies of a known size (the counter calibration), then
     two H.X launches of the given stencil variant on the roofline problem."""
@@ -238,6 +277,9 @@ def main():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the pseudised-water and Poisson 256^3 measurements")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc passes")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched small-molecule SCF measurement")
+    ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--batch-workers", type=int, default=8)
     ap.add_argument("--roofline-only", action="store_true", help="only the H.X roofline measurement (for profiling)")
     ap.add_argument("--variant", type=int, default=None, help="stencil tile variant of the roofline (default: tuned)")
     ap.add_argument("--traffic-probe", nargs=2, type=int, metavar=("ESIZE", "VARIANT"), help=argparse.SUPPRESS)
@@ -337,6 +379,14 @@ def main():
         print(f"# Poisson 256^3 to 1e-8: {result['poisson_256']['seconds'] * 1e3:.1f} ms, "
               f"{result['poisson_256']['pcg_iterations']} PCG iterations", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
+    if not args.no_batch:
+        b = batch_scf(lira, count=args.batch_count, workers=args.batch_workers)
+        result["batch"] = b
+        print(f"# batch of {b['molecules']} molecules, {b['workers']} workers: {b['seconds']:.2f} s, "
+              f"{b['scf_per_s']:.1f} SCF/s, p50 {b['p50_s'] * 1e3:.0f} ms, p99 {b['p99_s'] * 1e3:.0f} ms; "
+              f"one at a time {b['one_at_a_time']['scf_per_s']:.1f} SCF/s "
+              f"(x{b['one_at_a_time']['speedup']:.2f} batched on the same {b['one_at_a_time']['molecules']})",
+              file=sys.stderr, flush=True)
     if not args.no_cpu and gpu_fp64 is not None:
         threads = min(16, os.cpu_count() or 1)
         result["cpu_baseline"] = cpu_baseline_sample(state, gpu_fp64, n, h, pos, z, threads)

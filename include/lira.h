@@ -220,6 +220,32 @@ lira_status lira_evaluate(lira_solver *s, const lira_system *sys, const lira_set
                           lira_abort_fn abort, void *user, lira_scf_stats *stats, double *forces, int *flags,
                           double *tol_used);
 
+/* ---- batched SCF (no counterpart in the reference, which evaluates a set of molecules one at a time:
+ * crates/providers/src/sampling/mod.rs:1635) ---- */
+#define LIRA_MAX_GRID 644
+#define LIRA_BATCH_MAX_WORKERS 64
+/* one all-electron molecule on its own grid */
+typedef struct {
+    int n;              /* grid points per edge, 3 .. LIRA_MAX_GRID */
+    double h;           /* spacing (Bohr) */
+    lira_system system; /* at least one nucleus */
+} lira_batch_job;
+typedef struct {
+    lira_status status; /* of this molecule (LIRA_ABORTED: no worker took it) */
+    lira_scf_stats stats;
+    int worker;         /* the worker thread that ran it */
+    double start, end;  /* seconds from the start of the batch: its wait in the queue ends at start */
+    char message[128];  /* lira_last_error of a failed molecule */
+} lira_batch_result;
+/* Runs a cold SCF and the forces for each of n_jobs molecules on the current HIP device. workers host threads
+ * (1 .. LIRA_BATCH_MAX_WORKERS) each own one solver (and so one HIP stream) and take molecules from a shared queue,
+ * largest grids first; the solver is rebuilt when a worker's grid changes. Each molecule's result is the one a lone
+ * lira_solve + lira_forces on a fresh solver gives, bit for bit. settings: n_jobs entries, or NULL for
+ * lira_default_settings(h) per molecule. forces: 3 * n_nuclei doubles per molecule, concatenated in job order, or NULL.
+ * Returns LIRA_INVALID for bad arguments (before any work), else LIRA_OK with each molecule's status in results. */
+lira_status lira_solve_batch(const lira_batch_job *jobs, int n_jobs, int precision, const lira_settings *settings,
+                             int workers, lira_batch_result *results, double *forces);
+
 /* Dispatch control, no counterpart in the reference (its recorder batches dispatches instead,
  * crates/qm-gpu/src/batch.rs:55-233): on (the default), the fixed kernel sequences between two host syncs (the
  * Poisson MG-PCG iteration and its start, the LOBPCG step) are captured once as HIP graphs and replayed; off, every

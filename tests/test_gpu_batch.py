@@ -1,0 +1,52 @@
+"""Batched SCF on the GPU (lira_solve_batch): molecules spread over host worker threads, each with its own solver and
+stream, give exactly the results of lone solves on fresh solvers, and one molecule's failure stays its own."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lira_mod():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    import lira
+    lira.lib()
+    return lira
+
+
+def _lone(lira, m, precision):
+    s = lira.GpuSolver(m.n, m.h, precision)
+    st = s.solve(m.pos, m.z, m.n_up, m.n_dn, lira.default_settings(m.h))
+    return st, s.forces(m.pos, m.z, m.h)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_equals_lone_solves_bit_for_bit(lira_mod, precision):
+    lira = lira_mod
+    import molecules as M
+    mols = M.sample(6, seed=11, h=0.3, sizes=(33, 37, 41))
+    print("batch:", [(m.name, m.n) for m in mols], flush=True)
+    out = lira.solve_batch(mols, precision=precision, workers=3)
+    assert len({o.worker for o in out}) >= 2
+    for m, o in zip(mols, out):
+        assert o.status == lira.OK, (m.name, o.message)
+        assert o.end > o.start >= 0.0
+        st, f = _lone(lira, m, precision)
+        assert o.stats.iterations == st.iterations, m.name
+        assert o.stats.energy.total == st.energy.total, (m.name, o.stats.energy.total - st.energy.total)
+        assert o.stats.delta_rho == st.delta_rho
+        assert np.array_equal(o.forces, f), m.name
+        assert o.forces.shape == (len(m.z), 3)
+
+
+def test_a_failing_molecule_does_not_stop_the_batch(lira_mod):
+    lira = lira_mod
+    import molecules as M
+    mols = M.sample(3, seed=5, h=0.3, sizes=(33,))
+    bad = lira.Molecule(n=33, h=0.3, pos=np.zeros((2, 3)), z=[1, 1], n_up=1, n_dn=1, name="coincident")
+    out = lira.solve_batch(mols[:1] + [bad] + mols[1:], precision=1, workers=2)
+    assert out[1].status == lira.COINCIDENT_NUCLEI and out[1].message
+    assert [o.status for o in out[:1] + out[2:]] == [lira.OK] * 3
+    st, f = _lone(lira, mols[1], 1)
+    assert out[2].stats.energy.total == st.energy.total and np.array_equal(out[2].forces, f)
