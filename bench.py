@@ -152,22 +152,24 @@ def batch_scf(lira, count=256, workers=8, seed=0, seq_count=32):
     wall = time.perf_counter() - t0
     ok = [o for o in out if o.status == lira.OK]
     per = np.array([o.end - o.start for o in ok])
-    t1 = time.perf_counter()
-    seq = lira.solve_batch(mols[:seq_count], precision=lira.MIXED, workers=1)
-    seq_wall = time.perf_counter() - t1
-    t2 = time.perf_counter()
-    lira.solve_batch(mols[:seq_count], precision=lira.MIXED, workers=workers)
-    sub_wall = time.perf_counter() - t2
     names = sorted({m.name for m in mols})
-    return {"molecules": count, "workers": workers, "converged": len(ok), "seconds": wall,
+    res = {"molecules": count, "workers": workers, "converged": len(ok), "seconds": wall,
             "scf_per_s": len(ok) / wall, "p50_s": float(np.percentile(per, 50)) if len(per) else None,
             "p99_s": float(np.percentile(per, 99)) if len(per) else None,
             "steps_mean": float(np.mean([o.stats.iterations for o in ok])) if ok else None,
             "mix": {k: sum(m.name == k for m in mols) for k in names},
-            "grids": sorted({m.n for m in mols}), "h_bohr": mols[0].h,
-            "one_at_a_time": {"molecules": seq_count, "seconds": seq_wall,
-                              "scf_per_s": sum(o.status == lira.OK for o in seq) / seq_wall,
-                              "batched_seconds_same_set": sub_wall, "speedup": seq_wall / sub_wall}}
+            "grids": sorted({m.n for m in mols}), "h_bohr": mols[0].h}
+    if seq_count > 0:
+        t1 = time.perf_counter()
+        seq = lira.solve_batch(mols[:seq_count], precision=lira.MIXED, workers=1)
+        seq_wall = time.perf_counter() - t1
+        t2 = time.perf_counter()
+        lira.solve_batch(mols[:seq_count], precision=lira.MIXED, workers=workers)
+        sub_wall = time.perf_counter() - t2
+        res["one_at_a_time"] = {"molecules": seq_count, "seconds": seq_wall,
+                                "scf_per_s": sum(o.status == lira.OK for o in seq) / seq_wall,
+                                "batched_seconds_same_set": sub_wall, "speedup": seq_wall / sub_wall}
+    return res
 
 
 def traffic_probe(esize, variant, n=192, nb=64):
@@ -280,6 +282,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched small-molecule SCF measurement")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-workers", type=int, default=8)
+    ap.add_argument("--batch-only", action="store_true", help="only the batched SCF, without the one-at-a-time run")
     ap.add_argument("--roofline-only", action="store_true", help="only the H.X roofline measurement (for profiling)")
     ap.add_argument("--variant", type=int, default=None, help="stencil tile variant of the roofline (default: tuned)")
     ap.add_argument("--traffic-probe", nargs=2, type=int, metavar=("ESIZE", "VARIANT"), help=argparse.SUPPRESS)
@@ -296,6 +299,10 @@ def main():
     torch.cuda.set_device(0)
     import lira
 
+    if args.batch_only:
+        b = batch_scf(lira, count=args.batch_count, workers=args.batch_workers, seq_count=0)
+        print(json.dumps({"batch_only": True, **b}))
+        return
     if args.roofline_only:
         r = hx_roofline(torch, lira, variant=args.variant)
         print(json.dumps({"roofline_only": True, "ms_per_launch": r["ms"], "achieved_gbs": r["gbs"],
