@@ -71,10 +71,11 @@ template <> __device__ __forceinline__ void bstore_nt<double>(double v, BufRes r
 }
 
 // ------------------------------------------------------------------------------------------ stencil ----
-// 2.5-D z-streaming: a workgroup owns a 64 x TY tile of (x, y) columns of one band and a chunk of z planes.
-// Each thread keeps its RPT columns' 9-plane z-window in registers; the x/y taps of the current plane come from an
-// LDS tile with a 4-point halo (double-buffered: one barrier per plane). The potential V is read once per point.
-constexpr int ST_TX = 64;
+// 2.5-D z-streaming: a workgroup owns a TX x TY tile of (x, y) columns of one band and a chunk of z planes, TX = 64
+// (one wave per row) or 32 (a wave covers two rows: for n just above a multiple of 32, e.g. 95 = 3 x 32 - 1, the 64-wide
+// tile would leave a quarter of its lanes outside the box). Each thread keeps its RPT columns' 9-plane z-window in
+// registers; the x/y taps of the current plane come from an LDS tile with a 4-point halo (double-buffered: one barrier
+// per plane). The potential V is read once per point.
 
 // Register state of one thread: the 9-plane z window of its RPT columns (plane p in slot (p - z0 + 4) mod 9) and the
 // look-ahead queues of its halo cells, V and R (plane p in slot (p - z0) mod 3). The z loop is unrolled by 9 so every
@@ -154,8 +155,8 @@ template <typename T, int RPT, int W> struct StRegs {
 template <typename T, int RPT> struct StCtx {
     BufRes bx, bo, bv, br;  // x and out of this band, V, R of this band
     unsigned pb;            // bytes per plane
-    unsigned co[RPT];       // this lane's centre rows (gx, y0 + ty + 4 rr): byte offsets in a plane, or BUF_OOB
-    unsigned ho0, ho1, hox; // its y-halo cells (gx, y0 - 4 + hly0 / hly1) and x-halo cell, or BUF_OOB
+    unsigned co[RPT];       // this lane's centre rows (gx, y0 + ty + (256 / TX) rr): byte offsets in a plane, or BUF_OOB
+    unsigned ho0, ho1, hox; // its y-halo cells (gx, y0 - 4 + hly0 / hly1; hly1 only at TX = 64) and x-halo cell
     int n;
     int gx, x0, y0, tx, ty;
     int hly0, hly1, hxr, hlx;
@@ -164,11 +165,11 @@ template <typename T, int RPT> struct StCtx {
     __device__ T ld(BufRes r, unsigned off, int zz) const { return bload<T>(r, off + (unsigned)zz * pb); }
 };
 
-template <typename T, int MODE, int RPT, int W, int SLOT>
+template <typename T, int MODE, int RPT, int W, int TX, int SLOT>
 __device__ __forceinline__ void st_side(StRegs<T, RPT, W> &R, const StCtx<T, RPT> &c, int zz) {
     constexpr int QS = SLOT % 3;
     R.hq[QS][0] = c.ld(c.bx, c.ho0, zz);
-    R.hq[QS][1] = c.ld(c.bx, c.ho1, zz);
+    if (TX == 64) R.hq[QS][1] = c.ld(c.bx, c.ho1, zz);
     R.hq[QS][2] = c.ld(c.bx, c.hox, zz);
 #pragma unroll
     for (int rr = 0; rr < RPT; rr++) {
@@ -178,25 +179,26 @@ __device__ __forceinline__ void st_side(StRegs<T, RPT, W> &R, const StCtx<T, RPT
 }
 
 // one plane z of the stream; S = (z - z0) mod W
-template <typename T, int MODE, int RPT, int W, int S>
+template <typename T, int MODE, int RPT, int W, int TX, int S>
 __device__ __forceinline__ void st_plane(StRegs<T, RPT, W> &R, const StCtx<T, RPT> &c,
-                                         T (*tile)[4 * RPT + 8][ST_TX + 8], int z, int buf) {
+                                         T (*tile)[(256 / TX) * RPT + 8][TX + 8], int z, int buf) {
+    constexpr int TYB = 256 / TX;  // rows per pass of the workgroup
     constexpr int C0 = (S + 4) % W;
     constexpr int NEW = (S + W - 1) % W;
     constexpr int HS = S % 3;
 #pragma unroll
     for (int rr = 0; rr < RPT; rr++) R.q[NEW][rr] = c.ld(c.bx, c.co[rr], z + W - 5);
-    st_side<T, MODE, RPT, W, S + 2>(R, c, z + 2);
-    T(*tl)[ST_TX + 8] = tile[buf];
+    st_side<T, MODE, RPT, W, TX, S + 2>(R, c, z + 2);
+    T(*tl)[TX + 8] = tile[buf];
 #pragma unroll
-    for (int rr = 0; rr < RPT; rr++) tl[c.ty + 4 * rr + 4][c.tx + 4] = R.q[C0][rr];
+    for (int rr = 0; rr < RPT; rr++) tl[c.ty + TYB * rr + 4][c.tx + 4] = R.q[C0][rr];
     tl[c.hly0][c.tx + 4] = R.hq[HS][0];
-    tl[c.hly1][c.tx + 4] = R.hq[HS][1];
+    if (TX == 64) tl[c.hly1][c.tx + 4] = R.hq[HS][1];
     if (c.has_xh) tl[c.hxr][c.hlx] = R.hq[HS][2];
     __syncthreads();
 #pragma unroll
     for (int rr = 0; rr < RPT; rr++) {
-        const int ly = c.ty + 4 * rr + 4, lx = c.tx + 4;
+        const int ly = c.ty + TYB * rr + 4, lx = c.tx + 4;
         const T ce = R.q[C0][rr];
         T ax = T(C1) * ((tl[ly][lx + 1] - ce) + (tl[ly][lx - 1] - ce)) + T(C2) * ((tl[ly][lx + 2] - ce) + (tl[ly][lx - 2] - ce)) +
                T(C3) * ((tl[ly][lx + 3] - ce) + (tl[ly][lx - 3] - ce)) + T(C4) * ((tl[ly][lx + 4] - ce) + (tl[ly][lx - 4] - ce));
@@ -224,21 +226,25 @@ __device__ __forceinline__ void st_plane(StRegs<T, RPT, W> &R, const StCtx<T, RP
     }
 }
 
-template <typename T, int MODE, int RPT, int W>
+template <typename T, int MODE, int RPT, int W, int TX>
 __global__ __launch_bounds__(256) void k_stencil(const T *__restrict__ x, T *__restrict__ out, const T *__restrict__ v,
                                                  const T *__restrict__ r, const double *__restrict__ lam, int n,
                                                  long N, int zc, int nb, int order, T inv_h2, T scale, T omega,
                                                  T diag_t, const T *__restrict__ x2, T *__restrict__ out2, int nb1) {
-    constexpr int TY = 4 * RPT;
-    __shared__ T tile[2][TY + 8][ST_TX + 8];
+    static_assert(TX == 64 || TX == 32, "tile width");
+    constexpr int TYB = 256 / TX;  // workgroup (TX, TYB)
+    constexpr int TY = TYB * RPT;
+    // the y halo (8 rows of TX cells) takes 512 / TX cells per lane, the x halo (TY + 8 rows of 8 cells) one
+    static_assert(8 * (TY + 8) <= 256, "x halo larger than the workgroup");
+    __shared__ T tile[2][TY + 8][TX + 8];
     StCtx<T, RPT> c;
     c.tx = threadIdx.x;
     c.ty = threadIdx.y;
-    const int t = c.ty * 64 + c.tx;
+    const int t = c.ty * TX + c.tx;
     const int nzc = (n + zc - 1) / zc;
-    const StWork wk = st_decode((n + ST_TX - 1) / ST_TX, (n + TY - 1) / TY, nzc, nb, order);
+    const StWork wk = st_decode((n + TX - 1) / TX, (n + TY - 1) / TY, nzc, nb, order);
     if (!wk.valid) return;
-    c.x0 = wk.tile_x * ST_TX;
+    c.x0 = wk.tile_x * TX;
     c.y0 = wk.tile_y * TY;
     const int band = wk.band;
     const int z0 = wk.zchunk * zc;
@@ -253,16 +259,17 @@ __global__ __launch_bounds__(256) void k_stencil(const T *__restrict__ x, T *__r
     c.pb = (unsigned)((long)n * n * (long)sizeof(T));
     c.gx = c.x0 + c.tx;
 #pragma unroll
-    for (int rr = 0; rr < RPT; rr++) c.co[rr] = row_off(c.gx, c.y0 + c.ty + 4 * rr, n, (int)sizeof(T));
-    const int hy0 = t >> 6, hy1 = (t + 256) >> 6;
+    for (int rr = 0; rr < RPT; rr++) c.co[rr] = row_off(c.gx, c.y0 + c.ty + TYB * rr, n, (int)sizeof(T));
+    // y-halo rows 0..3 above the tile, 4..7 below it (LDS rows TY + 4 ..): row ty, and at TX = 64 also row ty + 4
+    const int hy0 = c.ty, hy1 = c.ty + 4;
     c.hly0 = hy0 < 4 ? hy0 : TY + hy0;
-    c.hly1 = hy1 < 4 ? hy1 : TY + hy1;
+    c.hly1 = TY + hy1;
     c.has_xh = t < 8 * (TY + 8);
     c.hxr = t >> 3;
     const int hxc = t & 7;
-    c.hlx = hxc < 4 ? hxc : ST_TX + hxc;
+    c.hlx = hxc < 4 ? hxc : TX + hxc;
     c.ho0 = row_off(c.gx, c.y0 - 4 + c.hly0, n, (int)sizeof(T));
-    c.ho1 = row_off(c.gx, c.y0 - 4 + c.hly1, n, (int)sizeof(T));
+    c.ho1 = TX == 64 ? row_off(c.gx, c.y0 - 4 + c.hly1, n, (int)sizeof(T)) : BUF_OOB;
     c.hox = c.has_xh ? row_off(c.x0 - 4 + c.hlx, c.y0 - 4 + c.hxr, n, (int)sizeof(T)) : BUF_OOB;
     c.inv_h2 = inv_h2;
     c.scale = scale;
@@ -274,8 +281,8 @@ __global__ __launch_bounds__(256) void k_stencil(const T *__restrict__ x, T *__r
     for (int d = 0; d < W - 1; d++)
 #pragma unroll
         for (int rr = 0; rr < RPT; rr++) R.q[d][rr] = c.ld(c.bx, c.co[rr], z0 - 4 + d);
-    st_side<T, MODE, RPT, W, 0>(R, c, z0);
-    st_side<T, MODE, RPT, W, 1>(R, c, z0 + 1);
+    st_side<T, MODE, RPT, W, TX, 0>(R, c, z0);
+    st_side<T, MODE, RPT, W, TX, 1>(R, c, z0 + 1);
 
     // consecutive planes alternate the two LDS buffers (plane p uses buffer (p - z0) & 1): one barrier per plane then
     // separates every write of a buffer from the previous reads of it
@@ -284,7 +291,7 @@ __global__ __launch_bounds__(256) void k_stencil(const T *__restrict__ x, T *__r
         const int b = (z - z0) & 1;
         auto step = [&](auto sc) {
             constexpr int S = decltype(sc)::value;
-            st_plane<T, MODE, RPT, W, S>(R, c, tile, z + S, b ^ (S & 1));
+            st_plane<T, MODE, RPT, W, TX, S>(R, c, tile, z + S, b ^ (S & 1));
         };
         PlaneUnroll<0, W>::run(step);
     }
@@ -292,7 +299,7 @@ __global__ __launch_bounds__(256) void k_stencil(const T *__restrict__ x, T *__r
     const int b = (z - z0) & 1;
     auto tail = [&](auto sc) {
         constexpr int S = decltype(sc)::value;
-        if (z + S < z1) st_plane<T, MODE, RPT, W, S>(R, c, tile, z + S, b ^ (S & 1));
+        if (z + S < z1) st_plane<T, MODE, RPT, W, TX, S>(R, c, tile, z + S, b ^ (S & 1));
     };
     PlaneUnroll<0, W - 1>::run(tail);
 }
@@ -739,7 +746,8 @@ __global__ __launch_bounds__(256) void k_stencil_vb(const T *__restrict__ x, T *
 struct StVariant {
     int rpt;
     int zc;     // 0 = whole column
-    int vec;    // 1: 16-byte vector lanes (needs n divisible by the vector width); 2: lanes carrying rpt bands
+    int vec;    // 1: 16-byte vector lanes (needs n divisible by the vector width); 2: lanes carrying rpt bands;
+                // 3: scalar lanes on a 32-wide tile (rpt 1 or 2)
     int order;  // XCD work order: 0 tiles fastest, 1 bands fastest, G >= 2 bands in groups of G fastest
     int win;    // z window: 9 (no look-ahead beyond the taps) or 12 (three planes of look-ahead)
     const char *name;
@@ -782,6 +790,12 @@ static const StVariant ST_VARIANTS[] = {
     {1, 0, 1, 4, 12, "vector lanes, 16 rows, full z column, bands in groups of 4, 12-plane window"},
     {2, 0, 1, 0, 12, "vector lanes, 32 rows, full z column, 12-plane window"},
     {2, 0, 2, 0, 12, "vector lanes, 2 bands per lane, full z column, 12-plane window"},
+    {2, 0, 3, 0, 9, "32x16 tile, full z column"},
+    {2, 16, 3, 0, 9, "32x16 tile, z chunks of 16"},
+    {1, 0, 3, 0, 9, "32x8 tile, full z column"},
+    {1, 16, 3, 0, 9, "32x8 tile, z chunks of 16"},
+    {2, 16, 3, 1, 9, "32x16 tile, z chunks of 16, bands fastest"},
+    {2, 16, 3, 4, 9, "32x16 tile, z chunks of 16, bands in groups of 4"},
 };
 static int g_default_variant[2] = {-1, -1};
 
@@ -798,16 +812,17 @@ static int pick_variant(int n, int nb) {
     return 2;
 }
 
-template <typename T, int MODE, int RPT, int W>
+template <typename T, int MODE, int RPT, int W, int TX>
 static void launch_stencil(const StencilParams &p, int zc, int order, const T *x, T *out, const T *v, const T *r,
                            const double *lam, hipStream_t s) {
+    constexpr int TYB = 256 / TX, TY = TYB * RPT;
     const int n = p.n;
     const long N = (long)n * n * n;
     if (zc <= 0 || zc > n) zc = n;
     const int nzc = (n + zc - 1) / zc;
-    const unsigned grid = st_grid((n + ST_TX - 1) / ST_TX, (n + 4 * RPT - 1) / (4 * RPT), nzc, p.nb, order);
-    dim3 block(64, 4, 1);
-    hipLaunchKernelGGL((k_stencil<T, MODE, RPT, W>), dim3(grid), block, 0, s, x, out, v, r, lam, n, N, zc, p.nb, order,
+    const unsigned grid = st_grid((n + TX - 1) / TX, (n + TY - 1) / TY, nzc, p.nb, order);
+    dim3 block(TX, TYB, 1);
+    hipLaunchKernelGGL((k_stencil<T, MODE, RPT, W, TX>), dim3(grid), block, 0, s, x, out, v, r, lam, n, N, zc, p.nb, order,
                        (T)(1.0 / (p.h * p.h)), (T)p.scale, (T)p.omega, (T)p.diag_t, static_cast<const T *>(p.x2),
                        static_cast<T *>(p.out2), p.nb1 < 0 ? p.nb : p.nb1);
     LIRA_HIP(hipGetLastError());
@@ -849,7 +864,8 @@ static void stencil_dispatch(int mode, const StVariant &sv, const StencilParams 
                              const T *r, const double *lam, hipStream_t s) {
 #define LIRA_VB(MODE_, NB_) launch_stencil_vb<T, MODE_, NB_, W>(p, sv.zc, sv.order, x, out, v, r, lam, s)
 #define LIRA_SV(MODE_, RPT_) launch_stencil_vec<T, MODE_, RPT_, W>(p, sv.zc, sv.order, x, out, v, r, lam, s)
-#define LIRA_ST(MODE_, RPT_) launch_stencil<T, MODE_, RPT_, W>(p, sv.zc, sv.order, x, out, v, r, lam, s)
+#define LIRA_ST(MODE_, RPT_) launch_stencil<T, MODE_, RPT_, W, 64>(p, sv.zc, sv.order, x, out, v, r, lam, s)
+#define LIRA_ST32(MODE_, RPT_) launch_stencil<T, MODE_, RPT_, W, 32>(p, sv.zc, sv.order, x, out, v, r, lam, s)
     if (sv.vec == 2) {
         bool four = false;
         if constexpr (W == 9) {
@@ -865,7 +881,7 @@ static void stencil_dispatch(int mode, const StVariant &sv, const StencilParams 
             else if (mode == ST_HAM) LIRA_VB(ST_HAM, 2);
             else LIRA_VB(ST_JAC, 2);
         }
-    } else if (sv.vec) {
+    } else if (sv.vec == 1) {
         if (sv.rpt == 1) {
             if (mode == ST_LAP) LIRA_SV(ST_LAP, 1);
             else if (mode == ST_HAM) LIRA_SV(ST_HAM, 1);
@@ -875,6 +891,21 @@ static void stencil_dispatch(int mode, const StVariant &sv, const StencilParams 
             else if (mode == ST_HAM) LIRA_SV(ST_HAM, 2);
             else LIRA_SV(ST_JAC, 2);
         }
+    } else if (sv.vec == 3) {
+        bool done = false;
+        if constexpr (W == 9) {
+            done = true;
+            if (sv.rpt == 1) {
+                if (mode == ST_LAP) LIRA_ST32(ST_LAP, 1);
+                else if (mode == ST_HAM) LIRA_ST32(ST_HAM, 1);
+                else LIRA_ST32(ST_JAC, 1);
+            } else {
+                if (mode == ST_LAP) LIRA_ST32(ST_LAP, 2);
+                else if (mode == ST_HAM) LIRA_ST32(ST_HAM, 2);
+                else LIRA_ST32(ST_JAC, 2);
+            }
+        }
+        if (!done) throw InvalidError("stencil: 32-wide tiles exist only with the 9-plane window");
     } else if (sv.rpt == 4) {
         if (mode == ST_LAP) LIRA_ST(ST_LAP, 4);
         else if (mode == ST_HAM) LIRA_ST(ST_HAM, 4);
@@ -887,11 +918,13 @@ static void stencil_dispatch(int mode, const StVariant &sv, const StencilParams 
 #undef LIRA_VB
 #undef LIRA_SV
 #undef LIRA_ST
+#undef LIRA_ST32
 }
 
 bool stencil_variant_usable(int variant, int n, int dtype_size) {
     if (variant < 0 || variant >= stencil_variants()) return false;
-    if (!ST_VARIANTS[variant].vec) return true;
+    const int vec = ST_VARIANTS[variant].vec;
+    if (vec == 0 || vec == 3) return true;
     return n % (16 / dtype_size) == 0;
 }
 
@@ -909,7 +942,7 @@ void stencil(int mode, const StencilParams &p, const T *x, T *out, const T *v, c
                            "2^31 bytes)");
     if (variant < 0) variant = g_default_variant[sizeof(T) == 8 ? 1 : 0];
     if (!stencil_variant_usable(variant, p.n, (int)sizeof(T))) variant = pick_variant(p.n, p.nb);
-    if (ST_VARIANTS[variant].vec) {
+    if (ST_VARIANTS[variant].vec == 1 || ST_VARIANTS[variant].vec == 2) {
         // vector lanes need 16-byte aligned bases (every band then stays aligned since n is a multiple of the width)
         auto misaligned = [](const void *q) { return q && (reinterpret_cast<uintptr_t>(q) & 15) != 0; };
         if (misaligned(x) || misaligned(out) || misaligned(v) || misaligned(r) || misaligned(p.x2) ||
