@@ -248,7 +248,7 @@ lira_status lira_solve_batch(const lira_batch_job *jobs, int n_jobs, int precisi
 
 /* Dispatch control, no counterpart in the reference (its recorder batches dispatches instead,
  * crates/qm-gpu/src/batch.rs:55-233): on (the default), the fixed kernel sequences between two host syncs (the
- * Poisson MG-PCG iteration and its start, the LOBPCG step) are captured once as HIP graphs and replayed; off, every
+ * Poisson MG-PCG iteration, in its two forms, and its start) are captured once as HIP graphs and replayed; off, every
  * kernel is launched one by one. Results are bitwise identical either way. */
 lira_status lira_set_graphs(lira_solver *s, int on);
 

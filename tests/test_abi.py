@@ -67,6 +67,27 @@ def test_rayleigh_ritz_conditioned_subspaces_match_a_dense_generalised_solve(con
     assert np.abs(a @ c - b @ c @ np.diag(th)).max() <= 1e-12 * np.abs(th).max() * np.abs(c).max()
 
 
+@pytest.mark.parametrize("m,nk", [(12, 4), (27, 9), (36, 12)])
+def test_rayleigh_ritz_degenerate_and_clustered_levels(m, nk):
+    # lowest levels with an exact triple degeneracy (a p shell), a pair 1e-9 apart and a cluster within 1e-4: the
+    # Ritz values are scipy's, the coefficients B-orthonormal and each pair's residual at rounding level
+    import scipy.linalg as sl
+    rng = np.random.default_rng(m)
+    lam = np.sort(rng.uniform(1.0, 40.0, m))
+    lam[:nk] = np.sort(np.concatenate([[-20.0, -1.5, -1.5, -1.5, -0.7, -0.7 + 1e-9, 0.2, 0.2 + 1e-4, 0.2 + 2e-4],
+                                       rng.uniform(0.5, 0.9, max(nk - 9, 0))]))[:nk]
+    q, _ = np.linalg.qr(rng.standard_normal((m, m)))
+    a = q @ np.diag(lam) @ q.T
+    a = 0.5 * (a + a.T)
+    g = rng.standard_normal((m, m))
+    b = g @ g.T / m + np.eye(m)
+    th, c = lira.rayleigh_ritz(a, b, nk)
+    w = sl.eigh(a, b, eigvals_only=True)[:nk]
+    assert np.allclose(th, w, rtol=1e-10, atol=1e-10)
+    assert np.allclose(c.T @ b @ c, np.eye(nk), atol=1e-9)
+    assert np.abs(a @ c - b @ c @ np.diag(th)).max() <= 1e-10 * max(1.0, np.abs(th).max()) * np.abs(c).max()
+
+
 def test_rank_loss_is_a_numerical_error():
     # two identical columns: B~ has a zero direction, so only 2 of 3 survive and 3 Ritz pairs cannot be formed
     a = np.diag([1.0, 2.0, 3.0])
