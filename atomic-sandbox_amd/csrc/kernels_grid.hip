@@ -1281,7 +1281,7 @@ void gram(const Cols &S, const Cols &U, long N, double scale, double *partials, 
 // C arrives padded (lincomb_rows(m) rows, lincomb_ldc(q) columns, zeros beyond m x q), so every loop below has a
 // compile-time trip count and no bounds branch: each row's QB coefficients are wave-uniform and come in a few wide
 // scalar loads, and every FMA takes one as its scalar operand. Padded rows read column 0 (their coefficients are 0).
-constexpr int LC_MAX_M = 192;  // input columns of one lincomb (3 x the largest block LOBPCG forms)
+constexpr int LC_MAX_M = 384;  // input columns of one lincomb: [X, W, P] of blocks up to 128 columns
 constexpr int LC_GROUP = 8;    // input columns loaded together
 
 static int lincomb_qb(int q) {
@@ -1335,7 +1335,7 @@ template <typename T>
 void lincomb(const Cols &S, const double *C, int ldc, int q, T *Y, long N, hipStream_t s, int qsplit, T *Y2) {
     const int m = S.total();
     if (q <= 0) return;
-    if (m > LC_MAX_M) throw InvalidError("lincomb: more than 192 input columns");
+    if (m > LC_MAX_M) throw InvalidError("lincomb: more than " + std::to_string(LC_MAX_M) + " input columns");
     if (ldc != lincomb_ldc(q)) throw InvalidError("lincomb: coefficients not padded to lincomb_ldc(q) columns");
     if (qsplit < 0 || qsplit > q) qsplit = q;
     const unsigned gx = (unsigned)((N + 255) / 256);

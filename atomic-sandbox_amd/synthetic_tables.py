@@ -71,6 +71,26 @@ def oxygen_like(h, partial_core=True, projectors=True):
                    pc, O_PC[2] if partial_core else 0.0, projs)
 
 
+# the C-like ion: Z = 6, Z_v = 4 (same profile family, softer and wider)
+C_V_SR = (3.0, 0.50, 1.7)
+C_PC = (1.0, 0.40, 1.3)
+C_S = (1.0, 0.50, 1.5, 1.2)
+C_P = (1.0, 0.55, 1.6, 0.5)
+
+
+def carbon_like(h, partial_core=True, projectors=True):
+    """A C-like pseudo-ion (Z = 6, Z_v = 4) filtered for spacing h: a repulsive V_SR, an s and a p projector with
+    E_KB > 0, and a partial core."""
+    du = h / TABLE_POINTS_PER_H
+    projs = []
+    if projectors:
+        for l, (a, w, r, e) in ((0, C_S), (1, C_P)):
+            projs.append(Projector(l, e, r, uniform_table(du, r, a, w)))
+    pc = uniform_table(du, C_PC[2], C_PC[0], C_PC[1]) if partial_core else None
+    return Species(6, 4.0, SIGMA_LR_OVER_H * h, du, uniform_table(du, C_V_SR[2], C_V_SR[0], C_V_SR[1]), C_V_SR[2],
+                   pc, C_PC[2] if partial_core else 0.0, projs)
+
+
 def local_only(z, z_v, h, sigma_lr=None):
     """A pseudo-ion with V_SR = 0, no projectors and no core: the smeared charge Z_v at sigma_LR alone."""
     du = h / TABLE_POINTS_PER_H

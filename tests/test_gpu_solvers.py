@@ -122,6 +122,28 @@ def test_lobpcg_wide_block_anisotropic_oscillator(lira_mod):
     assert np.abs(g - np.eye(k)).max() < 1e-8
 
 
+def test_lobpcg_block_beyond_64_columns(lira_mod):
+    # k = 66 (S = [X, W, P] holds 198 > 192 columns, the size a 34-electron Fermi-Dirac solve reaches with its 32
+    # witness columns): the lowest 66 levels of the anisotropic oscillator, in mixed precision
+    lira = lira_mod
+    n, h = 33, 0.35
+    x, y, z = O.grid_points(n, h)
+    w = (1.0, 1.13, 1.31)
+    v = 0.5 * (w[0] ** 2 * x * x + w[1] ** 2 * y * y + w[2] ** 2 * z * z)
+    k = 66
+    levels = sorted(w[0] * (a + 0.5) + w[1] * (b + 0.5) + w[2] * (c + 0.5)
+                    for a in range(12) for b in range(12) for c in range(12))[:k]
+    guess = O.initial_orbitals(n, h, [[0.0, 0.0, 0.0]], k, 778)
+    s = lira.GpuSolver(n, h, 1)
+    xs, ev, st = s.lobpcg(v, guess, 1e-4, 600, precond_lambda=1.5)
+    print(f"k = 66 lobpcg: {st.iterations} iterations, residual {st.max_residual:.2e}, "
+          f"max level error {np.abs(np.sort(ev) - np.array(levels)).max():.2e}")
+    assert st.max_residual < 1e-4
+    assert np.abs(np.sort(ev) - np.array(levels)).max() < 3e-2
+    g = xs @ xs.T * h ** 3
+    assert np.abs(g - np.eye(k)).max() < 1e-5
+
+
 # ----------------------------------------------------------------------------------------------- SCF ----
 
 H2 = ([[0.7, 0.0, 0.0], [-0.7, 0.0, 0.0]], [1, 1], 1, 1)
