@@ -1059,6 +1059,14 @@ struct ColPtrs {
     int nseg;
 };
 
+// column col's base address on the host (the same arithmetic as col_base)
+static const void *col_host(const ColPtrs &c, int col, long N, int esize) {
+    int s = 0;
+    for (int i = 1; i < c.nseg; i++)
+        if (col >= c.start[i]) s = i;
+    return static_cast<const char *>(c.base[s]) + (size_t)(col - c.start[s]) * (size_t)N * (size_t)esize;
+}
+
 __device__ inline const void *col_base(const ColPtrs &c, int col, long N, int esize) {
     int s = 0;
 #pragma unroll
@@ -1108,9 +1116,16 @@ __device__ inline void gram_fetch(const T *const *cols, long p0, long p_end, T (
     }
 }
 
+// With sym (U's first m columns are S's), the 16 x 16 blocks of S^T S strictly below its diagonal are skipped
+// (gram_mirrored): k_gram_finish reads their values from the transposed block, the same sum of the same exact products.
+__host__ __device__ inline bool gram_mirrored(int sym, int m, int i, int j) {
+    const int rb = i >> 4, cb = j >> 4;
+    return sym && cb < rb && 16 * (cb + 1) <= m;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q, long N, long chunk,
-                                              double *partials) {
+                                              double *partials, int sym) {
     __shared__ double slab[4][G2_TI + G2_TJ][G2_S + 1];  // one slice per wave; reused for the final reduction
     __shared__ const T *cols[G2_TI + G2_TJ];
     __shared__ int urow[G2_TJ];  // the slab row each U column is read from: its own, or the S row of the same column
@@ -1118,6 +1133,19 @@ __global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q
     const int bi = blockIdx.y / nbj, bj = blockIdx.y - bi * nbj;
     const int i0 = bi * G2_TI, j0 = bj * G2_TJ;
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    // which 16 x 16 output blocks this tile computes (uniform over the workgroup); a tile with none returns here,
+    // before any barrier
+    const int ni = min(2, (m - i0 + 15) / 16), nj = min(4, (q - j0 + 15) / 16);
+    bool live[2][4];
+    bool any = false;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            live[a][b] = a < ni && b < nj && !gram_mirrored(sym, m, i0 + 16 * a, j0 + 16 * b);
+            any = any || live[a][b];
+        }
+    if (!any) return;
     if (t < G2_TI) cols[t] = i0 + t < m ? static_cast<const T *>(col_base(S, i0 + t, N, sizeof(T))) : nullptr;
     if (t >= 64 && t < 64 + G2_TJ) {
         const int c = t - 64;
@@ -1140,8 +1168,6 @@ __global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q
     }
     __syncthreads();
     if (t >= 64 && t < 64 + G2_TJ && urow[t - 64] < G2_TI) cols[G2_TI + t - 64] = nullptr;
-    // which 16 x 16 output blocks hold any output (uniform over the block)
-    const int ni = min(2, (m - i0 + 15) / 16), nj = min(4, (q - j0 + 15) / 16);
     lira_d4 acc[2][4];
 #pragma unroll
     for (int a = 0; a < 2; a++)
@@ -1176,10 +1202,10 @@ __global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q
             const double a1 = sl[16 + r16][kk + k4];
 #pragma unroll
             for (int b = 0; b < 4; b++) {
-                if (b < nj) {
+                if (live[0][b] || live[1][b]) {
                     const double bv = sl[brow[b]][kk + k4];
-                    acc[0][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc[0][b], 0, 0, 0);
-                    if (ni > 1) acc[1][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, acc[1][b], 0, 0, 0);
+                    if (live[0][b]) acc[0][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc[0][b], 0, 0, 0);
+                    if (live[1][b]) acc[1][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, acc[1][b], 0, 0, 0);
                 }
             }
         }
@@ -1222,13 +1248,20 @@ __global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q
 
 // sums the per-chunk partials of every output in a fixed order: 4 outputs per workgroup, 64 lanes per output each
 // summing a strided subset of the chunks, then a fixed shuffle tree
-__global__ __launch_bounds__(256) void k_gram_finish(const double *partials, int chunks, long mq, double scale,
-                                                      double *G) {
+__global__ __launch_bounds__(256) void k_gram_finish(const double *partials, int chunks, int m, int q, double scale,
+                                                      double *G, int sym) {
     const int lane = threadIdx.x & 63;
+    const long mq = (long)m * q;
     const long out = blockIdx.x * 4L + (threadIdx.x >> 6);
+    // a skipped block's output is summed from its transposed entry (i, j) -> (j, i)
+    long src = out;
+    if (out < mq) {
+        const int i = (int)(out / q), j = (int)(out - (long)i * q);
+        if (gram_mirrored(sym, m, i, j)) src = (long)j * q + i;
+    }
     double acc = 0.0;
     if (out < mq)
-        for (int c = lane; c < chunks; c += 64) acc += partials[(long)c * mq + out];
+        for (int c = lane; c < chunks; c += 64) acc += partials[(long)c * mq + src];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
     if (lane == 0 && out < mq) G[out] = acc * scale;
@@ -1268,11 +1301,16 @@ void gram(const Cols &S, const Cols &U, long N, double scale, double *partials, 
     long chunk = (N + chunks - 1) / chunks;
     chunk = ((chunk + 4 * G2_S - 1) / (4 * G2_S)) * (4 * G2_S);
     const int blocks = ((m + G2_TI - 1) / G2_TI) * ((q + G2_TJ - 1) / G2_TJ);
-    hipLaunchKernelGGL(k_gram<T>, dim3(chunks, blocks), dim3(256), 0, s, to_ptrs(S), to_ptrs(U), m, q, N, chunk,
-                       partials);
+    // U begins with S's m columns (LOBPCG's U = [S, R, HW, HP]): S^T S is symmetric and its lower blocks are mirrored
+    const ColPtrs sp = to_ptrs(S), up = to_ptrs(U);
+    int sym = q >= m ? 1 : 0;
+    for (int j = 0; j < m && sym; j++)
+        if (col_host(sp, j, N, sizeof(T)) != col_host(up, j, N, sizeof(T))) sym = 0;
+    hipLaunchKernelGGL(k_gram<T>, dim3(chunks, blocks), dim3(256), 0, s, sp, up, m, q, N, chunk, partials, sym);
     LIRA_HIP(hipGetLastError());
     const long mq = (long)m * q;
-    hipLaunchKernelGGL(k_gram_finish, dim3((unsigned)((mq + 3) / 4)), dim3(256), 0, s, partials, chunks, mq, scale, G);
+    hipLaunchKernelGGL(k_gram_finish, dim3((unsigned)((mq + 3) / 4)), dim3(256), 0, s, partials, chunks, m, q, scale, G,
+                       sym);
     LIRA_HIP(hipGetLastError());
 }
 
