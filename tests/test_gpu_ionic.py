@@ -111,6 +111,26 @@ def test_pseudo_scf_fp64_tight_matches_oracle(lira_mod):
     assert np.abs(f - terms.sum(0)).max() < 1e-12
 
 
+def test_two_pseudised_species_scf_matches_oracle(lira_mod):
+    # CO with both nuclei pseudised by different tables (C-like Z_v = 4, O-like Z_v = 6: 10 valence electrons), tight
+    # FP64 on 25^3 x 0.3 against the oracle: the species lookup by Z, two projector sets and two partial cores
+    lira = lira_mod
+    n, h = 25, 0.3
+    r = 1.128 * 1.8897261
+    pos, z = [[-0.43 * r, 0.05, -0.02], [0.57 * r, 0.0, 0.03]], [6, 8]
+    sp = [T.carbon_like(h), T.oxygen_like(h)]
+    s = lira.GpuSolver(n, h, 0)
+    s.set_ionic(sp)
+    st = s.solve(pos, z, 5, 5, lira.default_settings(h, 0, tol=1e-9, eig_tol=1e-9))
+    so, out = O.scf(n, h, pos, z, 5, 5, settings=O.default_settings(h, "aufbau", tol=1e-9, eig_tol=1e-9),
+                    species=sp, want_orbitals=True)
+    assert so.status == O.OK
+    print(f"pseudised CO: GPU E {st.energy.total:.12f} ({st.iterations} steps) oracle {so.total:.12f}")
+    assert abs(st.energy.total - so.total) < 1e-10
+    assert np.abs(s.state().evals_up - np.sort(out["evals_up"])).max() < 1e-8
+    assert np.abs(s.forces(pos, z, h) - out["forces"]).max() < 1e-8
+
+
 def test_pseudo_scf_mixed_matches_fp64(lira_mod):
     lira = lira_mod
     n, h = 33, 0.25
