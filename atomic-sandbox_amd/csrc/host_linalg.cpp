@@ -7,7 +7,14 @@
 
 #include "lira_internal.hpp"
 
+// This file is compiled twice: as lira::linalg_base for any x86-64 host, and as lira::linalg_avx2 with -mavx2 -mfma;
+// host_linalg_dispatch.cpp defines the lira:: entry points and picks the build the host CPU runs.
+#ifndef LIRA_LINALG_NS
+#define LIRA_LINALG_NS linalg_base
+#endif
+
 namespace lira {
+namespace LIRA_LINALG_NS {
 
 // Cyclic Jacobi (Golub & Van Loan §8.5; the reference's fallback eigensolver, crates/qm/src/eigen.rs:175-211):
 // rotations until the off-diagonal norm is below 1e-15 ||A||; pairs returned ascending. a is m x m row-major.
@@ -367,4 +374,5 @@ double rayleigh_ritz(int m, const std::vector<double> &A, const std::vector<doub
     return alpha;
 }
 
+}  // namespace LIRA_LINALG_NS
 }  // namespace lira

@@ -88,6 +88,35 @@ def test_rayleigh_ritz_degenerate_and_clustered_levels(m, nk):
     assert np.abs(a @ c - b @ c @ np.diag(th)).max() <= 1e-10 * max(1.0, np.abs(th).max()) * np.abs(c).max()
 
 
+def test_rayleigh_ritz_host_builds_agree():
+    # the baseline and the AVX2 + FMA builds of the host linear algebra (LIRA_HOST_ISA=base forces the baseline, in a
+    # child process: the choice is made once per process) give the same Ritz pairs to rounding: on the reference's
+    # ill-conditioned fixture the Ritz values, on a well-conditioned m = 27 problem also the coefficients
+    import json
+    import subprocess
+    import sys
+    code = ("import sys, json, numpy as np; sys.path.insert(0, %r); import lira; "
+            "a = np.loadtxt(%r); b = np.loadtxt(%r); th0, _ = lira.rayleigh_ritz(a, b, 8); "
+            "r = np.random.default_rng(4); g = r.standard_normal((27, 27)); b = g @ g.T / 27 + np.eye(27); "
+            "a = r.standard_normal((27, 27)); a = a + a.T; th, c = lira.rayleigh_ritz(a, b, 9); "
+            "print(json.dumps([th0.tolist(), th.tolist(), c.tolist()]))") % (
+        os.path.join(ROOT, "atomic-sandbox_amd"), os.path.join(GOLD, "rr_i1_a.txt"), os.path.join(GOLD, "rr_i1_b.txt"))
+    out = {}
+    for isa in ("base", "auto"):
+        env = dict(os.environ)
+        env.pop("LIRA_HOST_ISA", None)
+        if isa == "base":
+            env["LIRA_HOST_ISA"] = "base"
+        r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        out[isa] = [np.array(x) for x in json.loads(r.stdout.strip().splitlines()[-1])]
+    assert np.allclose(out["base"][0], out["auto"][0], rtol=1e-10, atol=1e-10)
+    assert np.allclose(out["base"][1], out["auto"][1], rtol=1e-12, atol=1e-12)
+    ca, cb = out["base"][2], out["auto"][2]
+    sgn = np.sign(np.sum(ca * cb, axis=0))  # eigenvectors up to sign
+    assert np.allclose(ca, cb * sgn, rtol=1e-9, atol=1e-10)
+
+
 def test_rank_loss_is_a_numerical_error():
     # two identical columns: B~ has a zero direction, so only 2 of 3 survive and 3 Ritz pairs cannot be formed
     a = np.diag([1.0, 2.0, 3.0])
