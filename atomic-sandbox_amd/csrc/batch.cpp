@@ -115,7 +115,17 @@ extern "C" lira_status lira_solve_batch(const lira_batch_job *jobs, int n_jobs, 
     const int nw = std::min(workers, n_jobs);
     std::vector<std::thread> pool;
     pool.reserve(nw);
-    for (int w = 0; w < nw; w++) pool.emplace_back(work, w);
+    for (int w = 0; w < nw; w++) {
+        try {
+            pool.emplace_back(work, w);
+        } catch (...) {  // no more threads: the ones started share the queue
+            break;
+        }
+    }
+    if (pool.empty()) work(0);  // not even one: the calling thread works through the queue
     for (auto &t : pool) t.join();
+    // a molecule no worker took (every worker failed to select the device) is a device failure of the batch
+    for (int i = 0; i < n_jobs; i++)
+        if (results[i].worker < 0) return LIRA_DEVICE;
     return LIRA_OK;
 }

@@ -242,7 +242,8 @@ typedef struct {
  * largest grids first; the solver is rebuilt when a worker's grid changes. Each molecule's result is the one a lone
  * lira_solve + lira_forces on a fresh solver gives, bit for bit. settings: n_jobs entries, or NULL for
  * lira_default_settings(h) per molecule. forces: 3 * n_nuclei doubles per molecule, concatenated in job order, or NULL.
- * Returns LIRA_INVALID for bad arguments (before any work), else LIRA_OK with each molecule's status in results. */
+ * Returns LIRA_INVALID for bad arguments (before any work), LIRA_DEVICE if some molecule found no worker able to select
+ * the device, else LIRA_OK with each molecule's status in results. */
 lira_status lira_solve_batch(const lira_batch_job *jobs, int n_jobs, int precision, const lira_settings *settings,
                              int workers, lira_batch_result *results, double *forces);
 

@@ -50,3 +50,19 @@ def test_a_failing_molecule_does_not_stop_the_batch(lira_mod):
     assert [o.status for o in out[:1] + out[2:]] == [lira.OK] * 3
     st, f = _lone(lira, mols[1], 1)
     assert out[2].stats.energy.total == st.energy.total and np.array_equal(out[2].forces, f)
+
+
+def test_batch_takes_per_molecule_settings(lira_mod):
+    # explicit settings per molecule (Aufbau occupations, a tighter tolerance on one) are the ones each solve uses
+    lira = lira_mod
+    import molecules as M
+    mols = M.sample(3, seed=8, h=0.3, sizes=(33,))
+    sets = [lira.default_settings(m.h, lira.AUFBAU) for m in mols]
+    sets[1].tol = 1e-8
+    out = lira.solve_batch(mols, precision=0, workers=2, settings=sets)
+    for m, st_, o in zip(mols, sets, out):
+        assert o.status == lira.OK, o.message
+        s = lira.GpuSolver(m.n, m.h, 0)
+        lone = s.solve(m.pos, m.z, m.n_up, m.n_dn, st_)
+        assert o.stats.energy.total == lone.energy.total and o.stats.iterations == lone.iterations
+        assert o.stats.delta_rho < st_.tol and o.stats.energy.ts == 0.0
