@@ -528,10 +528,20 @@ template <typename T> class Solver final : public SolverIface {
         return ldc;
     }
     void ensure_channel(Channel<T> &c, int kc);
-    int variant_ham_ = -1, variant_jac_ = -1;  // the stencil tile variants the tuner picked for H.X and Jacobi
+    // the stencil tile variant the tuner picked for H.X / Jacobi on this grid with nb bands (tuned on first use of a
+    // block size, then kept; block sizes change with the witness columns)
+    std::map<std::pair<int, int>, int> variants_;
+    int variant(int mode, int nb) {
+        const std::pair<int, int> key{mode, nb};
+        auto it = variants_.find(key);
+        if (it != variants_.end()) return it->second;
+        const int v = nb > 0 ? tune_stencil_mode(mode, nb) : -1;
+        variants_[key] = v;
+        return v;
+    }
     void hamiltonian(const T *v, const T *x, T *out, int nb) {
         StencilParams sp{n_, nb, h_, 1.0, 0.0, 0.0};
-        stencil<T>(ST_HAM, sp, x, out, v, nullptr, nullptr, stream_, variant_ham_);
+        stencil<T>(ST_HAM, sp, x, out, v, nullptr, nullptr, stream_, variant(ST_HAM, nb));
         if (nl_rows_ > 0) nl_apply_block(x, out, nb);  // + V_NL (qm/src/nonlocal.rs:165-174)
     }
     // H applied to two blocks in one stencil launch (x1: n1 bands, x2: n2 bands); V_NL per block
@@ -544,16 +554,16 @@ template <typename T> class Solver final : public SolverIface {
         sp.nb1 = n1;
         sp.x2 = x2;
         sp.out2 = out2;
-        stencil<T>(ST_HAM, sp, x1, out1, v, nullptr, nullptr, stream_, variant_ham_);
+        stencil<T>(ST_HAM, sp, x1, out1, v, nullptr, nullptr, stream_, variant(ST_HAM, n1 + n2));
         if (nl_rows_ > 0) {
             nl_apply_block(x1, out1, n1);
             nl_apply_block(x2, out2, n2);
         }
     }
     void tune_stencil(int nb) {
-        if (variant_ham_ >= 0 || nb <= 0) return;
-        variant_ham_ = tune_stencil_mode(ST_HAM, nb);
-        variant_jac_ = tune_stencil_mode(ST_JAC, nb);
+        if (nb <= 0) return;
+        variant(ST_HAM, nb);
+        variant(ST_JAC, nb);
     }
     // times every usable stencil tile variant in one mode on this grid with nb bands (the block size of the solve)
     // and returns the fastest; the choice is kept per (precision, n, nb, mode) for later solvers in the process
@@ -936,7 +946,7 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
                             diag_t, k, N_, stream_);
         for (int s = 0; s < 3; s++) {
             stencil<T>(ST_JAC, jp, c.W.template as<T>(), c.TMP.template as<T>(), v, c.R.template as<T>(), dshift, stream_,
-                       variant_jac_);
+                       variant(ST_JAC, k));
             std::swap(c.W, c.TMP);
         }
         hamiltonian2(v, c.W.template as<T>(), c.HW.template as<T>(), k, c.P.template as<T>(), c.HP.template as<T>(), np);
