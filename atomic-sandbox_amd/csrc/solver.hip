@@ -518,13 +518,19 @@ template <typename T> class Solver final : public SolverIface {
         }
     }
     // uploads the m x q row-major coefficients of a lincomb into dC_, zero-padded to lincomb's shape; returns ldc
-    int upload_coeffs(const std::vector<double> &C, int m, int q) {
+    // tail (optional) rides along in the same copy, right after the padded coefficients; *tail_dev receives its
+    // device address (valid until the next upload_coeffs)
+    int upload_coeffs(const std::vector<double> &C, int m, int q, const std::vector<double> *tail = nullptr,
+                      const double **tail_dev = nullptr) {
         const int mp = lincomb_rows(m), ldc = lincomb_ldc(q);
         std::vector<double> pad((size_t)mp * ldc, 0.0);
         for (int i = 0; i < m; i++)
             for (int j = 0; j < q; j++) pad[(size_t)i * ldc + j] = C[(size_t)i * q + j];
+        const size_t off = pad.size();
+        if (tail) pad.insert(pad.end(), tail->begin(), tail->end());
         ensure_coeff(pad.size() + 64);
         upload(dC_.as<double>(), pad.data(), pad.size());
+        if (tail_dev) *tail_dev = dC_.as<double>() + off;
         return ldc;
     }
     void ensure_channel(Channel<T> &c, int kc);
@@ -906,7 +912,6 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
         std::vector<double> l(k, lam);
         upload(dlam, l.data(), k);
     }
-    const double *dshift = per_column ? dth : dlam;
     auto X = [&]() { return c.X.template as<T>(); };
     // initial Rayleigh–Ritz on X
     hamiltonian(v, X(), c.HX.template as<T>(), k);
@@ -930,8 +935,10 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
         if (!std::isfinite(x)) throw NumericalError("lobpcg: non-finite Gram of the guess (NaN in V or X?)");
     std::vector<double> theta, C;
     rayleigh_ritz(k, A, B, k, 1e-10, theta, C);
+    // theta on the device: uploaded with the coefficients it belongs to (one copy), else on its own
+    const double *dth_cur = nullptr;
     {
-        const int ldc = upload_coeffs(C, k, k);
+        const int ldc = upload_coeffs(C, k, k, &theta, &dth_cur);
         Cols S{{{X(), k}}, 1};
         lincomb<T>(S, dC_.as<double>(), ldc, k, c.XN.template as<T>(), N_, stream_);
     }
@@ -941,8 +948,12 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
     std::vector<double> lambda(k), res(k);
     StencilParams jp{n_, k, h_, 1.0, omega, diag_t};
     for (;;) {
-        upload(dth, theta.data(), k);
-        residual_precond<T>(c.R.template as<T>(), c.W.template as<T>(), c.HX.template as<T>(), X(), dth, v, dshift,
+        if (!dth_cur) {
+            upload(dth, theta.data(), k);
+            dth_cur = dth;
+        }
+        const double *dshift = per_column ? dth_cur : dlam;
+        residual_precond<T>(c.R.template as<T>(), c.W.template as<T>(), c.HX.template as<T>(), X(), dth_cur, v, dshift,
                             diag_t, k, N_, stream_);
         for (int s = 0; s < 3; s++) {
             stencil<T>(ST_JAC, jp, c.W.template as<T>(), c.TMP.template as<T>(), v, c.R.template as<T>(), dshift, stream_,
@@ -1047,7 +1058,7 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
                 for (int j = 0; j < k; j++)
                     M[(size_t)i * 2 * k + k + j] = Cc[(size_t)i * k + j] - (i < k ? rhs[(size_t)i * k + j] : 0.0);
         }
-        const int ldc = upload_coeffs(M, mm, 2 * k);
+        const int ldc = upload_coeffs(M, mm, 2 * k, &th, &dth_cur);
         Cols Sm{{{X(), k}, {c.W.ptr, k}, {c.P.ptr, mm > 2 * k ? np : 0}}, mm > 2 * k ? 3 : 2};
         lincomb<T>(Sm, dC_.as<double>(), ldc, 2 * k, c.XN.template as<T>(), N_, stream_, k, c.PN.template as<T>());
         std::swap(c.X, c.XN);
