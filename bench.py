@@ -131,7 +131,7 @@ def pseudo_water(lira, n=95, h=0.15):
     return out
 
 
-def batch_scf(lira, count=256, workers=8, seed=0, seq_count=32):
+def batch_scf(lira, count=256, workers=4, seed=0, seq_count=32):
     """Batched SCF: `count` all-electron molecules drawn from H2, LiH, H2O, NH3, CH4 (equilibrium geometry plus a seeded
     uniform +-0.1 A per nucleus and coordinate), each on an n^3 grid with n drawn from 48, 56, ..., 96 at h = 0.25 Bohr,
     MIXED precision, cold SCF (delta_rho < 1e-6, Fermi-Dirac kT 5 mHa) + forces, over `workers` host threads that
@@ -332,7 +332,7 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc passes")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched small-molecule SCF measurement")
     ap.add_argument("--batch-count", type=int, default=256)
-    ap.add_argument("--batch-workers", type=int, default=8)
+    ap.add_argument("--batch-workers", type=int, default=4)
     ap.add_argument("--batch-only", action="store_true", help="only the batched SCF, without the one-at-a-time run")
     ap.add_argument("--aspirin", action="store_true", help="add the 160^3 aspirin SCF and its force check")
     ap.add_argument("--aspirin-only", action="store_true", help=argparse.SUPPRESS)
