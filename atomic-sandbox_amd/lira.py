@@ -285,7 +285,7 @@ class BatchOutcome:
     forces: np.ndarray
 
 
-def solve_batch(molecules, precision: int = MIXED, workers: int = 8, settings=None, forces: bool = True):
+def solve_batch(molecules, precision: int = MIXED, workers: int = 4, settings=None, forces: bool = True):
     """lira_solve_batch: a cold SCF and the forces of every molecule on the current HIP device, over `workers` host
     threads that each own a solver. settings: None (for_grid(h) per molecule) or one Settings per molecule. Returns one
     BatchOutcome per molecule in input order; a failed molecule is a status, not an exception."""
