@@ -7,14 +7,20 @@
 
 #include "lira_internal.hpp"
 
-// This file is compiled twice: as lira::linalg_base for any x86-64 host, and as lira::linalg_avx2 with -mavx2 -mfma;
-// host_linalg_dispatch.cpp defines the lira:: entry points and picks the build the host CPU runs.
+// This file is compiled twice: as lira::linalg_base for any x86-64 host, and as lira::linalg_avx2 with
+// LIRA_LINALG_AVX2 defined; host_linalg_dispatch.cpp defines the lira:: entry points and picks the build the host CPU
+// runs. The AVX2 + FMA target is given to the functions defined below only (a pragma, not -mavx2): the standard-library
+// code the two objects instantiate (std::vector members and the like) is emitted as baseline code in both, so whichever
+// copy the linker keeps of such a shared inline function runs on any x86-64 host, whatever the link order.
 #ifndef LIRA_LINALG_NS
 #define LIRA_LINALG_NS linalg_base
 #endif
 
 namespace lira {
 namespace LIRA_LINALG_NS {
+#ifdef LIRA_LINALG_AVX2
+#pragma clang attribute push(__attribute__((target("avx2,fma"))), apply_to = function)
+#endif
 
 // Cyclic Jacobi (Golub & Van Loan §8.5; the reference's fallback eigensolver, crates/qm/src/eigen.rs:175-211):
 // rotations until the off-diagonal norm is below 1e-15 ||A||; pairs returned ascending. a is m x m row-major.
@@ -374,5 +380,8 @@ double rayleigh_ritz(int m, const std::vector<double> &A, const std::vector<doub
     return alpha;
 }
 
+#ifdef LIRA_LINALG_AVX2
+#pragma clang attribute pop
+#endif
 }  // namespace LIRA_LINALG_NS
 }  // namespace lira

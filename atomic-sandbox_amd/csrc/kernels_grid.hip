@@ -1550,6 +1550,20 @@ template <typename T> void extrapolate(T *out, const T *a, const T *b, long N, h
     LIRA_HIP(hipGetLastError());
 }
 
+// y *= num / den when both are > 0 (the per-channel charge renormalisation of the mixed density to the output's
+// integral, crates/qm-gpu/src/scf.rs:1574-1582): the ratio in f64, rounded to T, applied in T
+template <typename T> __global__ void k_scale_ratio(T *y, long N, const double *num, const double *den) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= N) return;
+    const double a = *num, b = *den;
+    if (!(a > 0.0 && b > 0.0)) return;
+    y[i] = y[i] * (T)(a / b);
+}
+template <typename T> void scale_ratio(T *y, long N, const double *num, const double *den, hipStream_t s) {
+    hipLaunchKernelGGL(k_scale_ratio<T>, LIRA_GRID1(N), 0, s, y, N, num, den);
+    LIRA_HIP(hipGetLastError());
+}
+
 // (u + d)(p) by trilinear interpolation: the cell and fractions of qm::Grid::trilinear_cell in f64, neighbours beyond an
 // upper face read 0, blended a (1 - t) + b t along x, then y, then z (crates/qm/src/grid.rs:54-86)
 template <typename T>
@@ -1797,6 +1811,7 @@ void hf_forces(const T *rho, int n, double h, const double *pos, const double *z
     template void hf_forces<T>(const T *, int, double, const double *, const double *, int, double, double *,        \
                                double *, hipStream_t);                                                                \
     template void extrapolate<T>(T *, const T *, const T *, long, hipStream_t);                                      \
+    template void scale_ratio<T>(T *, long, const double *, const double *, hipStream_t);                             \
     template void sample_density<T>(const T *, const T *, int, double, const double *, int, double *, hipStream_t);
 LIRA_INST(float)
 LIRA_INST(double)

@@ -208,6 +208,8 @@ void nl_force(const T *psi, int nb, long N, int n, double h, const NlIon *ions, 
               double *CG, hipStream_t s);
 // out = max(2 a - b, 0) (warm-start density extrapolation)
 template <typename T> void extrapolate(T *out, const T *a, const T *b, long N, hipStream_t s);
+// y *= (*num / *den) when both are > 0 (device scalars)
+template <typename T> void scale_ratio(T *y, long N, const double *num, const double *den, hipStream_t s);
 // out[q] = (u + d) at pts[3q..3q+2] by trilinear interpolation (0 outside the grid)
 template <typename T>
 void sample_density(const T *u, const T *d, int n, double h, const double *pts, int count, double *out, hipStream_t s);

@@ -74,6 +74,20 @@ constexpr double DIIS_REG = 1e-12;
 constexpr int MAX_WITNESS_GROWTH = 32;
 constexpr double WITNESS_GAP = 0.1, WITNESS_TAIL_KT = 36.0, WITNESS_GUARD_TOL = 1e-3, WITNESS_RESIDUAL_FACTOR = 1.5;
 constexpr double STALL_LEVEL_PER_ELECTRON = 1e-3;
+// a warm guess within this factor of the step's eigen tolerance is kept without an iteration
+// (EIG_GUESS_FACTOR, crates/qm-gpu/src/scf.rs; crates/qm-gpu/src/eigen.rs:323-326)
+constexpr double EIG_GUESS_FACTOR = 1.5;
+// a guess column within sqrt(GUESS_DEP) of the span of the earlier ones is re-seeded (crates/qm-gpu/src/eigen.rs:258)
+constexpr double GUESS_DEP = 1e-4;
+constexpr int MAX_RESEED_ROUNDS = 8;
+// the step-accuracy witness tail on a loosened step (crates/qm/src/occupations.rs:174-177, 319-326)
+constexpr double STEP_TAIL_C = 1e-3, STEP_TAIL_MIN_KT = 6.0;
+// a level with min(f, 1 - f) at or above this is partly occupied (crates/qm/src/occupations.rs:173)
+constexpr double PARTIAL_OCCUPATION_MIN = 1e-6;
+// the f32 solve's stop on a step with a partly occupied level (GPU_PARTIAL_TOL, crates/qm/src/occupations.rs:180)
+constexpr double GPU_PARTIAL_TOL = 1e-5;
+// density on a box face above this (1/Bohr^3) warns that the box is too small (crates/qm/src/scf.rs:577)
+constexpr double BOUNDARY_DENSITY_WARN = 1e-5;
 
 // ------------------------------------------------------------------------------------- occupations ----
 static double occupation(double x) {
@@ -102,7 +116,7 @@ struct Fd {
     std::vector<double> f;
     double mu = NAN, entropy = 0.0;
 };
-// qm/src/occupations.rs:166-213: mu by two tie-directed bisections, reported at their midpoint
+// qm/src/occupations.rs:244-270: mu by two tie-directed bisections, reported at their midpoint
 static Fd fermi_dirac(const std::vector<double> &ev, int n, double kt) {
     Fd r;
     const int m = (int)ev.size();
@@ -147,9 +161,10 @@ static std::vector<int> ascending(const std::vector<double> &ev) {
     std::stable_sort(o.begin(), o.end(), [&](int a, int b) { return ev[a] < ev[b]; });
     return o;
 }
-// qm/src/occupations.rs:244-257
+// qm/src/occupations.rs:299-317: the top level's lower bound theta - r clears both witness arms (WITNESS_GAP above
+// eps_N and tail_kt kT above mu)
 static bool witness_bound_reached(const std::vector<double> &ev, const std::vector<double> &res, int n, double mu,
-                                  double kt) {
+                                  double kt, double tail_kt = WITNESS_TAIL_KT) {
     const int m = (int)ev.size();
     if (n == 0) return true;
     if (m <= n) return false;
@@ -157,27 +172,29 @@ static bool witness_bound_reached(const std::vector<double> &ev, const std::vect
     const double eps_n = ev[o[n - 1]];
     const int top = o[m - 1];
     const double lower = ev[top] - res[top];
-    return lower >= eps_n + WITNESS_GAP && lower - mu >= WITNESS_TAIL_KT * kt;
+    return lower >= eps_n + WITNESS_GAP && lower - mu >= tail_kt * kt;
 }
-// qm/src/occupations.rs:322-340
-static int witness_keep(const std::vector<double> &ev, const std::vector<double> &res, int n, double mu, double kt) {
+// qm/src/occupations.rs:372-399: t + 1 for the lowest rank t >= n whose lower bound clears both arms, t + 2 with a
+// spare above it that clears them too
+static int witness_keep(const std::vector<double> &ev, const std::vector<double> &res, int n, double mu, double kt,
+                        double tail_kt = WITNESS_TAIL_KT) {
     const int m = (int)ev.size();
     if (n == 0 || m <= n) return m;
     auto o = ascending(ev);
     const double eps_n = ev[o[n - 1]];
     auto passes = [&](int rank) {
         const double lower = ev[o[rank]] - res[o[rank]];
-        return lower >= eps_n + WITNESS_GAP && lower - mu >= WITNESS_TAIL_KT * kt;
+        return lower >= eps_n + WITNESS_GAP && lower - mu >= tail_kt * kt;
     };
     for (int t = n; t < m; t++)
         if (passes(t)) return (t + 1 < m && passes(t + 1)) ? t + 2 : t + 1;
     return m;
 }
 
-// qm/src/occupations.rs:352-375: the lowest rank t >= n whose lower bound clears both witness arms with r_t <= r_max;
+// qm/src/occupations.rs:401-421: the lowest rank t >= n whose lower bound clears both witness arms with r_t <= r_max;
 // returns t + 1 (t + 2 with a spare above), or -1 when none qualifies
 static int witness_keep_within(const std::vector<double> &ev, const std::vector<double> &res, int n, double mu,
-                               double kt, double r_max) {
+                               double kt, double r_max, double tail_kt = WITNESS_TAIL_KT) {
     const int m = (int)ev.size();
     if (n == 0) return m;
     if (m <= n) return -1;
@@ -185,14 +202,112 @@ static int witness_keep_within(const std::vector<double> &ev, const std::vector<
     const double eps_n = ev[o[n - 1]];
     auto passes = [&](int rank) {
         const double lower = ev[o[rank]] - res[o[rank]];
-        return lower >= eps_n + WITNESS_GAP && lower - mu >= WITNESS_TAIL_KT * kt;
+        return lower >= eps_n + WITNESS_GAP && lower - mu >= tail_kt * kt;
     };
     for (int t = n; t < m; t++)
         if (passes(t) && res[o[t]] <= r_max) return (t + 1 < m && passes(t + 1)) ? t + 2 : t + 1;
     return -1;
 }
 
-// qm/src/occupations.rs:271-291: the columns whose upper-bound occupation f(theta - r; mu) the tail sum needs to stay
+// qm/src/occupations.rs:319-326: ln(n_w / (STEP_TAIL_C * delta_rho_prev)) clamped to [STEP_TAIL_MIN_KT, WITNESS_TAIL_KT]
+static double step_tail_kt(int n_w, double delta_rho_prev) {
+    if (!(std::isfinite(delta_rho_prev) && delta_rho_prev > 0.0)) return WITNESS_TAIL_KT;
+    const double x = std::log((double)std::max(n_w, 1) / (STEP_TAIL_C * delta_rho_prev));
+    return std::min(std::max(x, STEP_TAIL_MIN_KT), WITNESS_TAIL_KT);
+}
+
+// qm/src/occupations.rs:352-366: the tight count a warm solve starts from, at least n
+static int tight_start(const std::vector<double> &f, int n, double tol) {
+    const double limit = 1e-2 * tol;
+    double tail = 0.0;
+    int loose_from = (int)f.size();
+    for (int i = (int)f.size() - 1; i >= 0; i--) {
+        tail += f[i];
+        if (!(tail < limit)) break;
+        loose_from = i;
+    }
+    return std::max(loose_from, n);
+}
+
+// Sequential elimination on a Gram matrix B (m x m, row-major): the columns whose Schur complement against the
+// earlier kept columns is at or below rel x their norm^2 (crates/qm-gpu/src/eigen.rs:491-509)
+static std::vector<int> dependent_columns(int m, const std::vector<double> &B, double rel) {
+    std::vector<double> L((size_t)m * m, 0.0);
+    std::vector<int> kept, dep;
+    for (int j = 0; j < m; j++) {
+        const double bjj = B[(size_t)j * m + j];
+        std::vector<double> lj(kept.size());
+        double s = bjj;
+        for (size_t a = 0; a < kept.size(); a++) {
+            const int kk = kept[a];
+            double v = B[(size_t)j * m + kk];
+            for (size_t b = 0; b < a; b++) v -= lj[b] * L[(size_t)kk * m + b];
+            lj[a] = v / L[(size_t)kk * m + a];
+            s -= lj[a] * lj[a];
+        }
+        if (!(bjj > 0.0) || !(s > rel * bjj)) {
+            dep.push_back(j);
+            continue;
+        }
+        for (size_t a = 0; a < kept.size(); a++) L[(size_t)j * m + a] = lj[a];
+        L[(size_t)j * m + kept.size()] = std::sqrt(s);
+        kept.push_back(j);
+    }
+    return dep;
+}
+
+// The Rayleigh-Ritz step with the first `locked` columns of S held fixed (crates/qm-gpu/src/eigen.rs:669-681): the
+// pencil restricted to the B-orthogonal complement of the locked columns, Q = [-Byy^-1 Byz; I], and C = Q C'
+// (m x na, row-major). Returns the amplification of the reduced step.
+static double rayleigh_ritz_locked(int m, const std::vector<double> &A, const std::vector<double> &B, int locked,
+                                   int na, double drop, std::vector<double> &th, std::vector<double> &C) {
+    if (locked == 0) return rayleigh_ritz(m, A, B, na, drop, th, C);
+    const int r = m - locked;
+    std::vector<double> byy((size_t)locked * locked), K((size_t)locked * r);
+    for (int i = 0; i < locked; i++) {
+        for (int j = 0; j < locked; j++) byy[(size_t)i * locked + j] = B[(size_t)i * m + j];
+        for (int j = 0; j < r; j++) K[(size_t)i * r + j] = B[(size_t)i * m + locked + j];
+    }
+    if (!cholesky(locked, byy)) throw NumericalError("lobpcg: Cholesky of the locked columns' Gram failed");
+    chol_solve(locked, byy, K, r);
+    auto reduce = [&](const std::vector<double> &M) {
+        // M Q (m x r), then Q^T (M Q), symmetrised
+        std::vector<double> MQ((size_t)m * r), out((size_t)r * r);
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < r; j++) {
+                double v = M[(size_t)i * m + locked + j];
+                for (int l = 0; l < locked; l++) v -= M[(size_t)i * m + l] * K[(size_t)l * r + j];
+                MQ[(size_t)i * r + j] = v;
+            }
+        for (int a = 0; a < r; a++)
+            for (int b = 0; b < r; b++) {
+                double v = MQ[(size_t)(locked + a) * r + b];
+                for (int l = 0; l < locked; l++) v -= K[(size_t)l * r + a] * MQ[(size_t)l * r + b];
+                out[(size_t)a * r + b] = v;
+            }
+        for (int a = 0; a < r; a++)
+            for (int b = 0; b < a; b++) {
+                const double v = 0.5 * (out[(size_t)a * r + b] + out[(size_t)b * r + a]);
+                out[(size_t)a * r + b] = out[(size_t)b * r + a] = v;
+            }
+        return out;
+    };
+    const std::vector<double> Ap = reduce(A), Bp = reduce(B);
+    std::vector<double> Cp;
+    const double alpha = rayleigh_ritz(r, Ap, Bp, na, drop, th, Cp);
+    C.assign((size_t)m * na, 0.0);
+    for (int j = 0; j < na; j++) {
+        for (int i = 0; i < locked; i++) {
+            double v = 0.0;
+            for (int b = 0; b < r; b++) v -= K[(size_t)i * r + b] * Cp[(size_t)b * na + j];
+            C[(size_t)i * na + j] = v;
+        }
+        for (int a = 0; a < r; a++) C[(size_t)(locked + a) * na + j] = Cp[(size_t)a * na + j];
+    }
+    return alpha;
+}
+
+// qm/src/occupations.rs:334-350: the columns whose upper-bound occupation f(theta - r; mu) the tail sum needs to stay
 // below LOOSE_F_FACTOR * tol are tight; returns 1 + the largest column position among them (0 if none)
 static int tight_columns(const std::vector<double> &ev, const std::vector<double> &res, double mu, double kt,
                          double tol) {
@@ -218,13 +333,17 @@ struct PoissonOut {
 };
 struct EigenOut {
     int iterations = 0, fallbacks = 0;
-    double max_residual = 0.0;
+    double max_residual = 0.0;   // the tight (head) columns' worst exit residual
+    double guard_residual = 0.0; // the loose tested columns' worst exit residual
+    bool head_kept = true;       // the head came back bit for bit (a 0-iteration accept or a locked head)
+    bool aborted = false;        // the inner abort stopped the solve
+    int locked_calls = 0;
 };
 
 struct SolverIface {
     virtual ~SolverIface() = default;
     virtual lira_status solve(const lira_system *sys, const lira_settings *st, int warm, lira_abort_fn abort,
-                              void *user, lira_scf_stats *out) = 0;
+                              lira_abort_fn inner_abort, lira_observe_fn observe, void *user, lira_scf_stats *out) = 0;
     virtual void forces(const lira_nucleus *nuc, int nat, double sigma, double *out) = 0;
     virtual void get_field(int which, double *out) = 0;
     virtual void get_levels(int spin, double *evals, double *occ, int cap, int *count) = 0;
@@ -234,6 +353,7 @@ struct SolverIface {
                                  int max_iter) = 0;
     virtual double effective_eig_tol(double requested) const = 0;
     virtual void reset() = 0;
+    virtual void inject_readback_fault(int after) = 0;
     virtual int precision() const = 0;
     virtual void set_ionic(const lira_species *sp, int nsp) = 0;
     virtual void force_terms(const lira_nucleus *nuc, int nat, double sigma, double *out) = 0;
@@ -253,7 +373,7 @@ struct SolverIface {
 // One pseudised species as the solver keeps it: scalars plus offsets of its tables in the f64 table arena.
 struct SpeciesHost {
     int z;
-    double z_v, sigma_lr, du, r_sr, pc_r;
+    double z_v, sigma_lr, du, r_sr, pc_r, r_core;
     int sr_off, sr_len, pc_off, pc_len;
     int nproj;
     int l[NL_MAX_PROJ], t_off[NL_MAX_PROJ], t_len[NL_MAX_PROJ];
@@ -277,8 +397,8 @@ template <typename T> class Solver final : public SolverIface {
         if (up_) (void)hipHostFree(up_);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
-    lira_status solve(const lira_system *sys, const lira_settings *st, int warm, lira_abort_fn abort, void *user,
-                      lira_scf_stats *out) override;
+    lira_status solve(const lira_system *sys, const lira_settings *st, int warm, lira_abort_fn abort,
+                      lira_abort_fn inner_abort, lira_observe_fn observe, void *user, lira_scf_stats *out) override;
     void forces(const lira_nucleus *nuc, int nat, double sigma, double *out) override;
     void get_field(int which, double *out) override;
     void get_levels(int spin, double *evals, double *occ, int cap, int *count) override;
@@ -289,7 +409,11 @@ template <typename T> class Solver final : public SolverIface {
     double effective_eig_tol(double requested) const override {
         return sizeof(T) == 4 ? std::max(requested, EIG_TOL_FLOOR_FACTOR * residual_floor_f32(h_)) : requested;
     }
-    void reset() override { valid_ = false; }
+    void reset() override {
+        valid_ = false;
+        parked_ = false;
+    }
+    void inject_readback_fault(int after) override { fault_after_ = after < 0 ? -1 : after; }
     int precision() const override { return sizeof(T) == 8 ? LIRA_FP64 : LIRA_MIXED; }
     void set_ionic(const lira_species *sp, int nsp) override;
     void force_terms(const lira_nucleus *nuc, int nat, double sigma, double *out) override;
@@ -344,7 +468,7 @@ template <typename T> class Solver final : public SolverIface {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
     };
-    bool graphs_ = true;
+    bool graphs_ = false;  // opt-in (lira_set_graphs): one solver per thread may capture
     GraphSlot pcg_graph_[2], pcg_start_graph_;
     static void drop_graph(GraphSlot &g) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -411,6 +535,8 @@ template <typename T> class Solver final : public SolverIface {
     std::vector<Slot> slots_;
     Channel<T> ch_[2];
     bool valid_ = false;
+    bool parked_ = false;  // an aborted solve's iterate, usable as the next warm start (not a converged state)
+    int fault_after_ = -1; // the readback that is reported failed (0 = the next one); -1 = none
     int nat_ = 0;
     std::vector<double> pos_h_, z_h_;
     double sigma_ = 0.0;
@@ -465,12 +591,15 @@ template <typename T> class Solver final : public SolverIface {
         LIRA_HIP(hipHostMalloc((void **)&hbuf_, hcap_ * sizeof(double), hipHostMallocDefault));
     }
     // copies count doubles from the device and waits (the solve's only sync points)
+    // A readback marked failed by lira_inject_readback_fault completes its copy and wait like any other, then reports a
+    // device failure (the reference's readback-fault latch, crates/qm-gpu/src/grid.rs:113-157): no kernel is disturbed.
     const double *readback(const double *d, size_t count) {
         ensure_host(count);
         LIRA_HIP(hipMemcpyAsync(hbuf_, d, count * sizeof(double), hipMemcpyDeviceToHost, stream_));
         LIRA_HIP(hipStreamSynchronize(stream_));
         up_pos_ = 0;  // the stream is idle: every queued upload has been consumed
         readbacks_++;
+        if (fault_after_ >= 0 && fault_after_-- == 0) throw DeviceError("a readback failed (injected)");
         return hbuf_;
     }
     // a blocking copy ordered on this solver's stream (which does not synchronise with the null stream)
@@ -619,6 +748,28 @@ template <typename T> class Solver final : public SolverIface {
         const double *r = readback(dres_.as<double>(), jobs.count);
         return std::vector<double>(r, r + jobs.count);
     }
+    // any number of reductions (up to RES_SLOTS) in launches of MAX_JOBS, results in consecutive slots, one readback
+    struct Job {
+        const T *a, *b;
+        int kind;
+    };
+    static constexpr int RES_SLOTS = 960;  // dres_ holds 1024 doubles; the slots above serve single reductions
+    std::vector<double> reduce_list(const std::vector<Job> &jobs, double scale) {
+        if (jobs.empty()) return {};
+        if ((int)jobs.size() > RES_SLOTS) throw InvalidError("too many reductions in one readback");
+        for (size_t c0 = 0; c0 < jobs.size(); c0 += MAX_JOBS) {
+            RedJobs<T> rj{};
+            for (size_t j = c0; j < std::min(jobs.size(), c0 + MAX_JOBS); j++) {
+                rj.a[j - c0] = jobs[j].a;
+                rj.b[j - c0] = jobs[j].b;
+                rj.kind[j - c0] = jobs[j].kind;
+                rj.count = (int)(j - c0 + 1);
+            }
+            reduce_jobs<T>(rj, n_, scale, partials_.as<double>(), dres_.as<double>() + c0, stream_);
+        }
+        const double *r = readback(dres_.as<double>(), jobs.size());
+        return std::vector<double>(r, r + jobs.size());
+    }
     // Poisson
     void poisson_setup();
     void vcycle(int l, T *x, const T *rhs);
@@ -626,9 +777,16 @@ template <typename T> class Solver final : public SolverIface {
     int pcg(const T *r0, T *e, double rel_tol, int max_iter, double *rel_out);
     // LOBPCG on a channel's X (k columns) for H = -1/2 lap + v
     EigenOut lobpcg(Channel<T> &c, const T *v, int k, double lam, double tol, int max_iter, double tail_tol, int head,
-                    int checked);
-    void solve_channel(int spin, const T *v, int nel, bool fd, double kt, double tol, double scf_tol, EigenOut &eo,
-                       Fd &occ);
+                    int checked, double guess_tol, bool lock);
+    bool solve_channel(int spin, const T *v, int nel, bool fd, double kt, double tol, double scf_tol, bool guess_band,
+                       double loosened_delta, EigenOut &eo, Fd &occ);
+    void permute_columns(Channel<T> &c, const std::vector<int> &order);
+    int buffer_last(Channel<T> &c);
+    // the SCF's inner abort, polled once per LOBPCG iteration under Fermi-Dirac (solve_polled's inner_abort)
+    lira_abort_fn inner_abort_ = nullptr;
+    void *abort_user_ = nullptr;
+    double step_tail_ = 0.0;  // the widest witness tail of the current SCF step (kT)
+    int doublings_ = 0;       // witness growths that doubled the extra block (the widened-window rule reads them)
     void upload_nuclei(const lira_nucleus *nuc, int nat);
 };
 
@@ -896,14 +1054,21 @@ template <typename T> PoissonOut Solver<T>::poisson(const T *rho, double *v, dou
 // W = 4-sweep damped Jacobi on (H - lam_i) w = r (lam_i the column's Ritz value, or one fixed shift); one Gram of S = [X, W, P] against [S, R, HW, HP] gives
 // B = S^T S and A = S^T H S (A's X columns restored from R); host f64 generalised Rayleigh–Ritz with the
 // conditioning guard; X, P updated by one block lincomb on the device; HX recomputed by the stencil.
+// LOBPCG on the channel's first k columns for H = -1/2 lap + v (+ V_NL) (crates/qm-gpu/src/eigen.rs:685-943). The
+// guess is measured (X^T X and <x_i, H x_i>, one readback) and neither rotated nor rescaled, so a guess that already
+// meets guess_tol (>= tol) at the first test comes back bit for bit; a guess column within sqrt(GUESS_DEP) of the span
+// of the earlier ones is re-seeded. Split stop: the first `head` columns at tol, the rest below `checked` at tail_tol,
+// the columns from `checked` on untested. lock: when the head meets guess_tol at the first test and the tested tail
+// does not, the head is locked, keeps its columns bitwise, and only the others move (eigen.rs:815-818).
 template <typename T>
 EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double tol, int max_iter, double tail_tol,
-                           int head, int checked) {
+                           int head, int checked, double guess_tol, bool lock) {
     EigenOut out;
     if (k <= 0) return out;
     const double diag_t = -1.5 * COEFF0 / (h_ * h_);
     const double omega = jacobi_omega();
     const double floor = sizeof(T) == 4 ? residual_floor_f32(h_) : 0.0;
+    guess_tol = std::max(guess_tol, tol);
     double *dth = dsmall_.as<double>() + 256, *dlam = dsmall_.as<double>() + 512;
     // the Jacobi preconditioner's shift: per column its current Ritz value (lam NaN: the reference's GPU solver,
     // crates/qm-gpu/src/eigen.rs:740-757), or one fixed shift for every column (crates/qm/src/eigen.rs:227)
@@ -913,65 +1078,73 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
         upload(dlam, l.data(), k);
     }
     auto X = [&]() { return c.X.template as<T>(); };
-    // initial Rayleigh–Ritz on X
-    hamiltonian(v, X(), c.HX.template as<T>(), k);
+    auto col = [&](DevBuf &b, int i) { return b.template as<T>() + (long)i * N_; };
     ensure_gram(3 * k, 6 * k);
-    {
+    // the guess: B = X^T X and the Rayleigh quotients theta_i = <x_i, H x_i> / <x_i, x_i>
+    std::vector<double> theta(k);
+    for (int round = 0;; round++) {
+        hamiltonian(v, X(), c.HX.template as<T>(), k);
         Cols S{{{X(), k}}, 1}, U{{{X(), k}, {c.HX.ptr, k}}, 2};
         gram<T>(S, U, N_, dv_, gpart_.as<double>(), gpart_cap_, dG_.as<double>(), stream_);
-    }
-    std::vector<double> G;
-    {
         const double *g = readback(dG_.as<double>(), (size_t)k * 2 * k);
-        G.assign(g, g + (size_t)k * 2 * k);
-    }
-    std::vector<double> A((size_t)k * k), B((size_t)k * k);
-    for (int i = 0; i < k; i++)
-        for (int j = 0; j <= i; j++) {
-            B[(size_t)i * k + j] = B[(size_t)j * k + i] = G[(size_t)i * 2 * k + j];
-            A[(size_t)i * k + j] = A[(size_t)j * k + i] = G[(size_t)i * 2 * k + k + j];
+        for (size_t i = 0; i < (size_t)k * 2 * k; i++)
+            if (!std::isfinite(g[i])) throw NumericalError("lobpcg: non-finite Gram of the guess (NaN in V or X?)");
+        std::vector<double> B((size_t)k * k);
+        for (int i = 0; i < k; i++) {
+            for (int j = 0; j <= i; j++) B[(size_t)i * k + j] = B[(size_t)j * k + i] = g[(size_t)i * 2 * k + j];
+            theta[i] = g[(size_t)i * 2 * k + k + i] / B[(size_t)i * k + i];
         }
-    for (double x : G)
-        if (!std::isfinite(x)) throw NumericalError("lobpcg: non-finite Gram of the guess (NaN in V or X?)");
-    std::vector<double> theta, C;
-    rayleigh_ritz(k, A, B, k, 1e-10, theta, C);
-    // theta on the device: uploaded with the coefficients it belongs to (one copy), else on its own
-    const double *dth_cur = nullptr;
-    {
-        const int ldc = upload_coeffs(C, k, k, &theta, &dth_cur);
-        Cols S{{{X(), k}}, 1};
-        lincomb<T>(S, dC_.as<double>(), ldc, k, c.XN.template as<T>(), N_, stream_);
+        const std::vector<int> dep = dependent_columns(k, B, GUESS_DEP);
+        if (dep.empty()) break;
+        if (round + 1 >= MAX_RESEED_ROUNDS)
+            throw NumericalError("lobpcg: could not generate " + std::to_string(k) + " independent guess vectors");
+        // deterministic noise under a Gaussian envelope about the nuclei (about the box centre without nuclei)
+        const double *cpos = pos_.as<double>();
+        int cnat = nat_;
+        if (cnat == 0 || !cpos) {
+            const double zero[3] = {0.0, 0.0, 0.0};
+            upload(dsmall_.as<double>() + 3072, zero, 3);
+            cpos = dsmall_.as<double>() + 3072;
+            cnat = 1;
+        }
+        for (int i : dep)
+            initial_orbitals<T>(col(c.X, i), n_, h_, cpos, cnat, 1, 7919ULL * (uint64_t)(round + 1) + 2ULL * (uint64_t)i,
+                                stream_);
     }
-    std::swap(c.X, c.XN);
-    hamiltonian(v, X(), c.HX.template as<T>(), k);
-    int np = 0;
-    std::vector<double> lambda(k), res(k);
-    StencilParams jp{n_, k, h_, 1.0, omega, diag_t};
+    std::vector<double> lambda(theta), res(k, 0.0);
+    int np = 0;      // valid P columns: P[locked .. locked + np)
+    int locked = 0;  // the locked head; 0 = none
+    const double *dth_cur = nullptr;
     for (;;) {
+        const int lk0 = locked;     // the lock this iteration's S was built under
+        const int na = k - lk0;     // its W columns
         if (!dth_cur) {
             upload(dth, theta.data(), k);
             dth_cur = dth;
         }
         const double *dshift = per_column ? dth_cur : dlam;
+        // 1. R = HX - theta X for every column (the head's residuals are measured too), W = T R for the moving ones
         residual_precond<T>(c.R.template as<T>(), c.W.template as<T>(), c.HX.template as<T>(), X(), dth_cur, v, dshift,
                             diag_t, k, N_, stream_);
+        StencilParams jp{n_, na, h_, 1.0, omega, diag_t};
         for (int s = 0; s < 3; s++) {
-            stencil<T>(ST_JAC, jp, c.W.template as<T>(), c.TMP.template as<T>(), v, c.R.template as<T>(), dshift, stream_,
-                       variant(ST_JAC, k));
+            stencil<T>(ST_JAC, jp, col(c.W, locked), col(c.TMP, locked), v, col(c.R, locked), dshift + locked, stream_,
+                       variant(ST_JAC, na));
             std::swap(c.W, c.TMP);
         }
-        hamiltonian2(v, c.W.template as<T>(), c.HW.template as<T>(), k, c.P.template as<T>(), c.HP.template as<T>(), np);
-        const int m = 2 * k + np, q = 2 * m;
+        hamiltonian2(v, col(c.W, locked), col(c.HW, locked), na, col(c.P, locked), col(c.HP, locked), np);
+        // 2. the readback: B = S^T S and S^T T with S = [X, W, P], T = [R, HW, HP] in one Gram product, and ||r_i||^2
+        const int m = k + na + np, q = 2 * m;
         ensure_gram(m, q);
-        Cols S{{{X(), k}, {c.W.ptr, k}, {c.P.ptr, np}}, np > 0 ? 3 : 2};
-        // one Gram product: G = S^T [S, R, HW, HP] (m x 2m); its left half is B = S^T S, its right half S^T T
-        Cols U{{{X(), k}, {c.W.ptr, k}, {c.P.ptr, np}, {c.R.ptr, k}, {c.HW.ptr, k}, {c.HP.ptr, np}}, 6};
-        if (np == 0) U = Cols{{{X(), k}, {c.W.ptr, k}, {c.R.ptr, k}, {c.HW.ptr, k}}, 4};
+        Cols S{{{X(), k}, {col(c.W, locked), na}, {col(c.P, locked), np}}, np > 0 ? 3 : 2};
+        Cols U{{{X(), k}, {col(c.W, locked), na}, {col(c.P, locked), np}, {c.R.ptr, k}, {col(c.HW, locked), na},
+                {col(c.HP, locked), np}}, 6};
+        if (np == 0) U = Cols{{{X(), k}, {col(c.W, locked), na}, {c.R.ptr, k}, {col(c.HW, locked), na}}, 4};
         double *dG = dG_.as<double>();
         gram<T>(S, U, N_, dv_, gpart_.as<double>(), gpart_cap_, dG, stream_);
         RedJobs<T> jr{};
         for (int i = 0; i < k; i++) {
-            jr.a[i % MAX_JOBS] = c.R.template as<T>() + (long)i * N_;
+            jr.a[i % MAX_JOBS] = col(c.R, i);
             jr.b[i % MAX_JOBS] = jr.a[i % MAX_JOBS];
             jr.kind[i % MAX_JOBS] = RED_DOT;
             jr.count = i % MAX_JOBS + 1;
@@ -982,6 +1155,7 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
             }
         }
         const double *g = readback(dG, (size_t)m * q + k);
+        // A's X columns come through R: A_ij = S_i^T R_j + theta_j B_ij
         std::vector<double> Bm((size_t)m * m), Am((size_t)m * m);
         for (int i = 0; i < m; i++)
             for (int j = 0; j <= i; j++) {
@@ -991,33 +1165,50 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
                 Bm[(size_t)i * m + j] = Bm[(size_t)j * m + i] = bij;
                 Am[(size_t)i * m + j] = Am[(size_t)j * m + i] = aij;
             }
-        double max_res = 0.0, head_res = 0.0, tail_res = 0.0;
+        double head_res = 0.0, tail_res = 0.0, lock_res = 0.0;
         bool finite = true;
+        const double tail_scale = tol / tail_tol;
         for (int i = 0; i < k; i++) {
             const double xx = Bm[(size_t)i * m + i];
-            lambda[i] = Am[(size_t)i * m + i] / xx;
+            lambda[i] = Am[(size_t)i * m + i] / xx;  // the Rayleigh quotient of the stored x_i
             res[i] = std::sqrt(std::max(g[(size_t)m * q + i], 0.0) / xx);
             if (!std::isfinite(lambda[i]) || !std::isfinite(res[i])) finite = false;
             if (i >= checked) continue;  // an untested buffer column (it only speeds up the columns below it)
-            max_res = std::max(max_res, res[i]);
             if (i < head) head_res = std::max(head_res, res[i]);
-            else tail_res = std::max(tail_res, res[i]);
+            else {
+                tail_res = std::max(tail_res, res[i]);
+                lock_res = std::max(lock_res, res[i] * tail_scale);
+            }
         }
         for (double x : Am)
             if (!std::isfinite(x)) finite = false;
         if (!finite) throw NumericalError("lobpcg: non-finite Gram entry or residual (NaN in a vector or in V)");
-        out.max_residual = max_res;
-        if ((head_res < tol && tail_res < tail_tol) || out.iterations >= max_iter) break;
+        out.max_residual = head_res;
+        out.guard_residual = tail_res;
+        const double split_res = std::max(head_res, tail_res * tail_scale);
+        // at the guess, a head under its guess tolerance whose tested tail is not is locked
+        if (lock && out.iterations == 0 && locked == 0 && head > 0 && head < checked && head_res < guess_tol &&
+            !(split_res < guess_tol))
+            locked = head;
+        const double stop_res = locked > 0 ? lock_res : split_res;
+        if (stop_res < tol || (out.iterations == 0 && stop_res < guess_tol) || out.iterations >= max_iter) break;
+        if (inner_abort_ && inner_abort_(abort_user_)) {
+            out.aborted = true;
+            break;
+        }
         out.iterations++;
-        // Rayleigh–Ritz with the guard: a step whose coefficient amplification exceeds the budget drops P
+        const int nm = k - locked;  // the columns this step moves (the lock may have been set above)
+        // 3. Rayleigh–Ritz with the conditioning guard: a step whose coefficient amplification exceeds the budget drops P
         int mm = m;
         std::vector<double> th, Cc;
         double alpha;
-        const double alpha_budget =
-            sizeof(T) == 4 ? std::max(ALPHA_MAX, ALPHA_RESIDUAL_FRACTION * max_res / std::max(floor, 1e-300)) : 1e300;
+        const double budget_res = locked > 0 ? tail_res : stop_res;
+        const double alpha_budget = sizeof(T) == 4
+                                        ? std::max(ALPHA_MAX, ALPHA_RESIDUAL_FRACTION * budget_res / std::max(floor, 1e-300))
+                                        : 1e300;
         bool rr_failed = false;  // the subspace with P lost rank: retry without P (in every precision)
         try {
-            alpha = rayleigh_ritz(m, Am, Bm, k, RR_DROP, th, Cc);
+            alpha = rayleigh_ritz_locked(m, Am, Bm, locked, nm, RR_DROP, th, Cc);
         } catch (const NumericalError &) {
             if (np == 0) throw;
             rr_failed = true;
@@ -1025,70 +1216,137 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
         }
         if ((rr_failed || alpha > alpha_budget) && np > 0) {
             out.fallbacks++;
-            mm = 2 * k;
+            mm = k + na;
             std::vector<double> As((size_t)mm * mm), Bs((size_t)mm * mm);
             for (int i = 0; i < mm; i++)
                 for (int j = 0; j < mm; j++) {
                     As[(size_t)i * mm + j] = Am[(size_t)i * m + j];
                     Bs[(size_t)i * mm + j] = Bm[(size_t)i * m + j];
                 }
-            rayleigh_ritz(mm, As, Bs, k, RR_DROP, th, Cc);
+            rayleigh_ritz_locked(mm, As, Bs, locked, nm, RR_DROP, th, Cc);
             Am.swap(As);
             Bm.swap(Bs);
         }
-        if (Cc.size() != (size_t)mm * k || th.size() != (size_t)k)
+        if (Cc.size() != (size_t)mm * nm || th.size() != (size_t)nm)
             throw NumericalError("lobpcg: Rayleigh-Ritz returned no coefficients for the step");
-        // P = the Ritz vector minus its X component: C_p = C - [Bxx^{-1} Bxs C ; 0]
-        std::vector<double> M((size_t)mm * 2 * k, 0.0);
+        // 4. X_new = S C and P_new = X_new - X (X^T X)^-1 X^T X_new = S C_p (C_p = C - [Bxx^-1 Bxs C ; 0])
+        std::vector<double> M((size_t)mm * 2 * nm, 0.0);
         for (int i = 0; i < mm; i++)
-            for (int j = 0; j < k; j++) M[(size_t)i * 2 * k + j] = Cc[(size_t)i * k + j];
+            for (int j = 0; j < nm; j++) M[(size_t)i * 2 * nm + j] = Cc[(size_t)i * nm + j];
         {
-            std::vector<double> bxx((size_t)k * k), rhs((size_t)k * k, 0.0);
+            std::vector<double> bxx((size_t)k * k), rhs((size_t)k * nm, 0.0);
             for (int i = 0; i < k; i++)
                 for (int j = 0; j < k; j++) bxx[(size_t)i * k + j] = Bm[(size_t)i * mm + j];
             for (int i = 0; i < k; i++)
-                for (int j = 0; j < k; j++) {
-                    double s = 0.0;
-                    for (int a = 0; a < mm; a++) s += Bm[(size_t)i * mm + a] * Cc[(size_t)a * k + j];
-                    rhs[(size_t)i * k + j] = s;
+                for (int j = 0; j < nm; j++) {
+                    double sacc = 0.0;
+                    for (int a = 0; a < mm; a++) sacc += Bm[(size_t)i * mm + a] * Cc[(size_t)a * nm + j];
+                    rhs[(size_t)i * nm + j] = sacc;
                 }
             if (!cholesky(k, bxx)) throw NumericalError("lobpcg: Cholesky of X^T X failed (X lost rank)");
-            chol_solve(k, bxx, rhs, k);
+            chol_solve(k, bxx, rhs, nm);
             for (int i = 0; i < mm; i++)
-                for (int j = 0; j < k; j++)
-                    M[(size_t)i * 2 * k + k + j] = Cc[(size_t)i * k + j] - (i < k ? rhs[(size_t)i * k + j] : 0.0);
+                for (int j = 0; j < nm; j++)
+                    M[(size_t)i * 2 * nm + nm + j] = Cc[(size_t)i * nm + j] - (i < k ? rhs[(size_t)i * nm + j] : 0.0);
         }
-        const int ldc = upload_coeffs(M, mm, 2 * k, &th, &dth_cur);
-        Cols Sm{{{X(), k}, {c.W.ptr, k}, {c.P.ptr, mm > 2 * k ? np : 0}}, mm > 2 * k ? 3 : 2};
-        lincomb<T>(Sm, dC_.as<double>(), ldc, 2 * k, c.XN.template as<T>(), N_, stream_, k, c.PN.template as<T>());
+        for (int a = 0; a < nm; a++) theta[locked + a] = th[a];
+        const int ldc = upload_coeffs(M, mm, 2 * nm, &theta, &dth_cur);
+        const int np_used = mm > k + na ? np : 0;
+        Cols Sm{{{X(), k}, {col(c.W, lk0), na}, {col(c.P, lk0), np_used}}, np_used > 0 ? 3 : 2};
+        lincomb<T>(Sm, dC_.as<double>(), ldc, 2 * nm, col(c.XN, locked), N_, stream_, nm, col(c.PN, locked));
+        if (locked > 0)  // the locked head is not written: it moves to the new block as it is
+            LIRA_HIP(hipMemcpyAsync(c.XN.ptr, c.X.ptr, (size_t)N_ * locked * sizeof(T), hipMemcpyDeviceToDevice, stream_));
         std::swap(c.X, c.XN);
         std::swap(c.P, c.PN);
-        np = k;
-        theta = th;
-        hamiltonian(v, X(), c.HX.template as<T>(), k);
+        np = nm;
+        // 5. HX for the new columns (the locked head's is unchanged)
+        hamiltonian(v, col(c.X, locked), col(c.HX, locked), nm);
     }
+    out.head_kept = out.iterations == 0 || locked > 0;
+    if (out.head_kept && out.iterations > 0) out.locked_calls = 1;
     c.evals.assign(lambda.begin(), lambda.end());
     c.res.assign(res.begin(), res.end());
     lobpcg_total_ += out.iterations;
     return out;
 }
 
-// one spin channel under Aufbau or Fermi–Dirac (qm/src/scf.rs:774-877; crates/qm-gpu/src/scf.rs:874-1033). Under
+// Rearranges the first k columns of a channel's block (X, evals, res) so that new column j is old column order[j]
+// (order.size() <= k columns are kept); device copies through the TMP block.
+template <typename T> void Solver<T>::permute_columns(Channel<T> &c, const std::vector<int> &order) {
+    const size_t bytes = (size_t)N_ * sizeof(T);
+    for (size_t j = 0; j < order.size(); j++)
+        LIRA_HIP(hipMemcpyAsync(c.TMP.template as<T>() + (long)j * N_, c.X.template as<T>() + (long)order[j] * N_, bytes,
+                                hipMemcpyDeviceToDevice, stream_));
+    std::swap(c.X, c.TMP);
+    std::vector<double> ev(order.size()), rs(order.size());
+    for (size_t j = 0; j < order.size(); j++) {
+        ev[j] = c.evals[order[j]];
+        rs[j] = c.res[order[j]];
+    }
+    c.evals = ev;
+    c.res = rs;
+    c.k = (int)order.size();
+}
+
+// Moves the block's highest Ritz value to the last position, the buffer's, when a lock or a 0-iteration accept left
+// it elsewhere; returns the position the former last column moved to, or -1 (crates/qm-gpu/src/scf.rs:1031-1042).
+template <typename T> int Solver<T>::buffer_last(Channel<T> &c) {
+    const int last = c.k - 1;
+    int top = last;
+    for (int i = 0; i < c.k; i++)
+        if (c.evals[i] > c.evals[top]) top = i;
+    if (top == last) return -1;
+    std::vector<int> order(c.k);
+    std::iota(order.begin(), order.end(), 0);
+    std::swap(order[top], order[last]);
+    permute_columns(c, order);
+    return top;
+}
+
+// One spin channel under Aufbau or Fermi–Dirac (qm/src/scf.rs:774-877; crates/qm-gpu/src/scf.rs:872-1033). Under
 // Fermi–Dirac the block carries, above the tested levels, one untested buffer column (it speeds up the column below
 // it, f = 0); the witness columns whose occupation is negligible against the SCF tolerance are held to the guard
-// tolerance; a top tested witness with r above 1.5 guard yields to the lowest tested level that is a witness at
-// r <= 1.5 guard, else the block grows by one.
+// tolerance; on a loosened step the witness tail shrinks with the previous step's delta_rho (the step-accuracy tail);
+// on a tight step a top tested witness with r above 1.5 guard yields to the lowest tested level that is a witness at
+// r <= 1.5 guard, else the block grows by one; a buffer that clears both witness arms decides at r <= guard (one fresh
+// buffer then joins), else the block grows by one. guess_band: a warm guess within 1.5 tol is kept (else only within
+// tol). loosened_delta: the previous step's delta_rho on a loosened step, NaN on a tight one. Returns whether every
+// call kept the tight columns bit for bit (the step's "frozen"); eo.aborted when the inner abort stopped a call.
 template <typename T>
-void Solver<T>::solve_channel(int spin, const T *v, int nel, bool fd, double kt, double tol, double scf_tol,
-                              EigenOut &eo, Fd &occ) {
+bool Solver<T>::solve_channel(int spin, const T *v, int nel, bool fd, double kt, double tol, double scf_tol,
+                              bool guess_band, double loosened_delta, EigenOut &eo, Fd &occ) {
     Channel<T> &c = ch_[spin];
     if (c.k == 0) {
         occ = Fd{};
-        return;
+        return true;
+    }
+    auto account = [&](const EigenOut &e, int head, int checked) {
+        eo.iterations += e.iterations;
+        eo.fallbacks += e.fallbacks;
+        eo.locked_calls += e.locked_calls;
+        for (int i = 0; i < std::min(head, c.k); i++) eo.max_residual = std::max(eo.max_residual, c.res[i]);
+        for (int i = head; i < std::min(checked, c.k); i++) eo.guard_residual = std::max(eo.guard_residual, c.res[i]);
+    };
+    if (!fd) {
+        profile_.peak_block = std::max(profile_.peak_block, c.k);
+        EigenOut e = lobpcg(c, v, c.k, NAN, tol, 200, tol, c.k, c.k, EIG_GUESS_FACTOR * tol, false);
+        if (trace_on())
+            std::fprintf(stderr, "lira:   spin %d k %d tol %.1e: %d iterations, max residual %.2e, fallbacks %d%s\n", spin,
+                         c.k, tol, e.iterations, e.max_residual, e.fallbacks, e.head_kept ? " (kept)" : "");
+        account(e, c.k, c.k);
+        if (e.aborted) eo.aborted = true;
+        occ.f.assign(c.k, 1.0);
+        occ.mu = NAN;
+        occ.entropy = 0.0;
+        c.occ = occ.f;
+        return e.head_kept;
     }
     const double guard = std::max(WITNESS_GUARD_TOL, tol);
+    const double guess_tol = (guess_band ? EIG_GUESS_FACTOR : 1.0) * tol;
+    const bool loosened = !std::isnan(loosened_delta);
     const int cap = witness_cap_;
-    auto buffered = [&](int len) { return fd && len >= nel + 2 && len - nel < cap; };
+    bool kept = true;
+    auto buffered = [&](int len) { return len >= nel + 2 && len - nel < cap; };
     auto add_column = [&]() {
         ensure_channel(c, c.k + 1);
         seed_column(c.X.template as<T>() + (long)c.k * N_, c.k, spin);
@@ -1096,59 +1354,97 @@ void Solver<T>::solve_channel(int spin, const T *v, int nel, bool fd, double kt,
         c.res.push_back(0.0);
         c.k++;
     };
-    if (fd && c.k == nel + 1 && cap >= 2) add_column();
+    if (c.k == nel + 1 && cap >= 2) add_column();
+    int swap_retries = 0;
     for (;;) {
         const int checked = buffered(c.k) ? c.k - 1 : c.k;
-        const int head = fd ? std::min(std::max(c.tight, 0), checked) : c.k;
+        const int head = std::min(std::max(c.tight, 0), checked);
+        const double tail_kt = loosened ? step_tail_kt(c.k, loosened_delta) : WITNESS_TAIL_KT;
         profile_.peak_block = std::max(profile_.peak_block, c.k);
-        EigenOut e = lobpcg(c, v, c.k, NAN, tol, 200, fd ? guard : tol, head, checked);
+        EigenOut e = lobpcg(c, v, c.k, NAN, tol, 200, guard, head, checked, guess_tol, true);
         if (trace_on())
-            std::fprintf(stderr, "lira:   spin %d k %d head %d tested %d tol %.1e: %d iterations, max residual %.2e, fallbacks %d\n",
-                         spin, c.k, head, checked, tol, e.iterations, e.max_residual, e.fallbacks);
-        eo.iterations += e.iterations;
-        eo.fallbacks += e.fallbacks;
-        for (int i = 0; i < std::min(head, c.k); i++) eo.max_residual = std::max(eo.max_residual, c.res[i]);
-        if (!fd) {
-            occ.f.assign(c.k, 1.0);
-            occ.mu = NAN;
-            occ.entropy = 0.0;
-            c.occ = occ.f;
-            return;
+            std::fprintf(stderr,
+                         "lira:   spin %d k %d head %d tested %d tol %.1e: %d iterations, residual %.2e / %.2e, "
+                         "fallbacks %d%s\n",
+                         spin, c.k, head, checked, tol, e.iterations, e.max_residual, e.guard_residual, e.fallbacks,
+                         e.head_kept ? " (head kept)" : "");
+        account(e, head, checked);
+        if (e.aborted) {
+            eo.aborted = true;
+            return false;
         }
-        // LOBPCG returns its columns in ascending Ritz order: the tested levels are the first `checked`
+        if (checked < c.k) {
+            // an untested column moved into the tested range is no witness until it meets the guard tolerance:
+            // re-solve (it is tested now), at most twice per step
+            const int moved = buffer_last(c);
+            if (moved >= 0 && c.res[moved] > guard && swap_retries < 2) {
+                swap_retries++;
+                kept &= e.head_kept;
+                continue;
+            }
+        }
+        kept &= e.head_kept;
+        // the tested levels are the first `checked` columns (the buffer, when there is one, is last)
         std::vector<double> ev(c.evals.begin(), c.evals.begin() + checked), rs(c.res.begin(), c.res.begin() + checked);
         occ = fermi_dirac(ev, nel, kt);
         const int need = tight_columns(ev, rs, occ.mu, kt, scf_tol);
         c.tight = need;
         if (need > head) continue;  // a loose column needs the eigen tolerance: re-solve with it tight
-        const bool witness = witness_bound_reached(ev, rs, nel, occ.mu, kt);
+        step_tail_ = std::max(step_tail_, tail_kt);
+        const bool witness = witness_bound_reached(ev, rs, nel, occ.mu, kt, tail_kt);
         auto o = ascending(ev);
         const double top_r = rs[o[checked - 1]];
         const double r_w = WITNESS_RESIDUAL_FACTOR * WITNESS_GUARD_TOL;
         int keep = -1;
-        if (witness && top_r > r_w && c.k - nel < cap) {
-            keep = witness_keep_within(ev, rs, nel, occ.mu, kt, r_w);
+        if (witness && !loosened && top_r > r_w && c.k - nel < cap) {
+            keep = witness_keep_within(ev, rs, nel, occ.mu, kt, r_w, tail_kt);
             if (keep < 0) {
                 add_column();
                 profile_.witness_growths++;
                 continue;
             }
         } else if (witness) {
-            keep = witness_keep(ev, rs, nel, occ.mu, kt);
+            keep = witness_keep(ev, rs, nel, occ.mu, kt, tail_kt);
         }
         if (keep >= 0) {
-            // keep the lowest `keep` tested levels (the first columns) plus a buffer above them when one fits
+            // the trim: keep the lowest `keep` tested levels (by value) plus the next one up as the buffer when one fits
             const int len = std::min(c.k, buffered(keep + 1) ? keep + 1 : keep);
-            if (len < c.k) profile_.witness_trims++;
-            c.k = len;
-            c.evals.resize(len);
-            c.res.resize(len);
-            const int chk = buffered(len) ? len - 1 : len;
-            std::vector<double> ev2(c.evals.begin(), c.evals.begin() + chk);
-            occ = fermi_dirac(ev2, nel, kt);
-            occ.f.resize(len, 0.0);  // the buffer holds no charge
+            int chk = checked;
+            if (len < c.k) {
+                profile_.witness_trims++;
+                const std::vector<int> all = ascending(c.evals);
+                std::vector<bool> drop(c.k, false);
+                for (int r = len; r < c.k; r++) drop[all[r]] = true;
+                std::vector<int> order;
+                for (int i = 0; i < c.k; i++)
+                    if (!drop[i]) order.push_back(i);
+                permute_columns(c, order);
+                if (buffered(len)) buffer_last(c);
+                chk = buffered(len) ? len - 1 : len;
+                std::vector<double> ev2(c.evals.begin(), c.evals.begin() + chk);
+                occ = fermi_dirac(ev2, nel, kt);
+            }
+            occ.f.resize(c.k, 0.0);  // the buffer holds no charge
             c.occ = occ.f;
-            return;
+            return kept;
+        }
+        // the buffer as the witness: it decides the step only at r <= guard; a buffer that clears both arms at a
+        // larger residual grows the block by one (it is tested on the re-solve)
+        if (checked < c.k && witness_bound_reached(c.evals, c.res, nel, occ.mu, kt, tail_kt)) {
+            if (c.res[c.k - 1] <= WITNESS_GUARD_TOL) {
+                occ.f.resize(c.k, 0.0);
+                if (c.k - nel < cap) {
+                    add_column();
+                    occ.f.push_back(0.0);
+                }
+                c.occ = occ.f;
+                return kept;
+            }
+            if (c.k - nel < cap) {
+                add_column();
+                profile_.witness_growths++;
+                continue;
+            }
         }
         if (c.k - nel >= cap)
             throw NumericalError("no gap witness within " + std::to_string(cap) + " extra levels of spin " +
@@ -1157,6 +1453,7 @@ void Solver<T>::solve_channel(int spin, const T *v, int nel, bool fd, double kt,
         const int target = std::min(std::max(2 * extras, extras + 1), cap);
         while (c.k - nel < target) add_column();
         profile_.witness_growths++;
+        doublings_++;
     }
 }
 
@@ -1203,6 +1500,8 @@ template <typename T> void Solver<T>::set_ionic(const lira_species *sp, int nsp)
         h.sigma_lr = s.sigma_lr;
         h.du = s.du;
         h.r_sr = s.r_sr;
+        if (!(s.r_core >= 0.0)) throw InvalidError("lira_set_ionic: r_core must be >= 0");
+        h.r_core = s.r_core;
         h.sr_len = s.n_v_sr;
         h.sr_off = put(s.v_sr_u, s.n_v_sr, "v_sr");
         const bool core = s.rho_pc_u && s.n_rho_pc > 0 && s.pc_radius > 0.0;
@@ -1583,7 +1882,7 @@ template <typename T> void Solver<T>::sample(const double *pts, int count, doubl
 // ----------------------------------------------------------------------------------------------- SCF ----
 template <typename T>
 lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, int warm, lira_abort_fn abort,
-                             void *user, lira_scf_stats *out) {
+                             lira_abort_fn inner_abort, lira_observe_fn observe, void *user, lira_scf_stats *out) {
     auto t0 = Clock::now();
     std::memset(out, 0, sizeof *out);
     readbacks_ = 0;
@@ -1598,7 +1897,7 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
     const int nat = sys->n_nuclei;
     if (nat < 0 || (nat > 0 && !sys->nuclei) || sys->n_up < 0 || sys->n_dn < 0)
         throw InvalidError("lira_solve: bad system");
-    if (!(st->sigma > 0.0) || !(st->tol > 0.0) || st->max_iter < 1 || st->diis_history > 15)
+    if (!(st->sigma > 0.0) || !(st->tol > 0.0) || st->max_iter < 1 || st->diis_history > LIRA_MAX_DIIS_HISTORY)
         throw InvalidError("lira_solve: bad settings");
     // nucleus-nucleus repulsion and the coincidence check (qm/src/nuclear.rs:277-289; with valence charges under an
     // ionic setup, nuclear.rs:371-382): the warm state is kept
@@ -1616,15 +1915,30 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
             if (d < 1e-8) throw std::pair<int, int>(i, j);
             e_nn += charge(sys->nuclei[i].z) * charge(sys->nuclei[j].z) / d;
         }
+    // cores that overlap (qm/src/ionic.rs:175-185): reported, the solve goes on
+    std::vector<lira_core_overlap> overlaps;
+    if (ionic)
+        for (int i = 0; i < nat; i++)
+            for (int j = i + 1; j < nat; j++) {
+                const SpeciesHost *a = species_of(sys->nuclei[i].z), *b = species_of(sys->nuclei[j].z);
+                const double rc = (a ? a->r_core : 0.0) + (b ? b->r_core : 0.0);
+                double d2 = 0.0;
+                for (int x = 0; x < 3; x++) d2 += std::pow(sys->nuclei[i].pos[x] - sys->nuclei[j].pos[x], 2);
+                if (std::sqrt(d2) < rc) overlaps.push_back(lira_core_overlap{i, j, std::sqrt(d2)});
+            }
     const bool fd = st->occupations == LIRA_FERMI_DIRAC;
     const double kt = fd ? st->kt : 0.0;
-    const bool reuse = warm && valid_ && nel_[0] == sys->n_up && nel_[1] == sys->n_dn && occ_scheme_ == st->occupations &&
-                       (!fd || kt_ == st->kt) && nat_ == nat && state_ionic_version_ == ionic_version_;
+    // the warm guard: a converged state, or the one an aborted solve parked, with the same counts and scheme
+    const bool reuse = warm && (valid_ || parked_) && nel_[0] == sys->n_up && nel_[1] == sys->n_dn &&
+                       occ_scheme_ == st->occupations && (!fd || kt_ == st->kt) && nat_ == nat &&
+                       state_ionic_version_ == ionic_version_;
+    const bool prev_converged = valid_;
     upload_nuclei(sys->nuclei, nat);
     sigma_ = st->sigma;
     const int nel[2] = {sys->n_up, sys->n_dn};
     const bool restricted = sys->n_up == sys->n_dn;
     valid_ = false;
+    parked_ = false;
     // fields
     for (DevBuf *b : {&vnuc_, &vh_t_, &rho_in_[0], &rho_in_[1], &rho_out_[0], &rho_out_[1], &vxc_[0], &vxc_[1], &exc_,
                       &veff_[0], &veff_[1], &rho_t_})
@@ -1679,17 +1993,19 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
         fill<double>(vh_.as<double>(), N_, 0.0, stream_);
     } else {
         for (int s = 0; s < 2; s++) {
+            // the tight count starts from the warm state's occupations (qm/src/occupations.rs:352-366)
+            if (fd) ch_[s].tight = tight_start(ch_[s].occ, nel[s], st->tol);
             if (extrapolate_) {  // keep rho_t: it becomes rho_{t-1} once this solve converges
                 alloc_field(save_rho_[s]);
                 copy<T>(save_rho_[s].template as<T>(), rho_out_[s].template as<T>(), N_, stream_);
             }
-            if (extrapolate_ && had_prev)
+            if (extrapolate_ && had_prev && prev_converged)
                 extrapolate<T>(rho_in_[s].template as<T>(), rho_out_[s].template as<T>(), prev_rho_[s].template as<T>(),
                                N_, stream_);
             else
                 copy<T>(rho_in_[s].template as<T>(), rho_out_[s].template as<T>(), N_, stream_);
         }
-        profile_.extrapolated = extrapolate_ && had_prev ? 1 : 0;
+        profile_.extrapolated = extrapolate_ && had_prev && prev_converged ? 1 : 0;
     }
     tune_stencil(std::max(ch_[0].k, 1));
     profile_.start_wall = secs(t0);
@@ -1699,6 +2015,7 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
         profile_.poisson_iterations = cg_total_;
         profile_.eigen_iterations = lobpcg_total_;
         profile_.eigen_fallbacks = e.fallbacks;
+        profile_.locked_calls = e.locked_calls;
         profile_.poisson_wall = t_poisson;
         profile_.eigen_wall = t_eigen;
         profile_.eig_tol_used = etol_steps_.empty() ? 0.0 : etol_steps_.back();
@@ -1706,16 +2023,21 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
     };
     const double ptol = std::min(1e-10, 1e-3 * st->tol);  // qm/src/scf.rs:519
     const double lo = effective_eig_tol(st->eig_tol), hi = std::max(EIG_TOL_MAX, lo);
-    const int H = std::max(st->diis_history, 1);
-    if ((int)slots_.size() < H) slots_.resize(H);
+    // the partial-level stop of the f32 solve (crates/qm-gpu/src/scf.rs:1519-1521); the f64 solve has no
+    // occupation-noise floor and keeps settings.tol, as the CPU solver does
+    const double partial_tol = sizeof(T) == 4 ? GPU_PARTIAL_TOL : 0.0;
+    // DIIS (crates/qm-gpu/src/scf.rs:563-646): H entries, one more slot for the entry being pushed
+    const int H = std::max(st->diis_history, 1), H1 = H + 1;
+    if ((int)slots_.size() < H1) slots_.resize(H1);
     for (auto &sl : slots_)
         for (int s = 0; s < 2; s++) {
             alloc_field(sl.rho[s]);
             alloc_field(sl.res[s]);
         }
-    std::deque<int> order;       // DIIS history, oldest first (slot indices)
-    std::vector<double> gram((size_t)H * H, 0.0);  // <r_i, r_j> by slot
-    double delta_rho = INFINITY, delta_prev = INFINITY;
+    std::deque<int> order;                           // DIIS history, oldest first (slot indices)
+    std::vector<double> gram((size_t)H1 * H1, 0.0);  // <r_i, r_j> by slot
+    double delta_rho = INFINITY, test_tol = st->tol;
+    // the stall watch (qm/src/scf.rs:442-480)
     double anchor = INFINITY, best = INFINITY;
     int anchor_at = 0;
     const double level = STALL_LEVEL_PER_ELECTRON * (nel[0] + nel[1]);
@@ -1724,14 +2046,33 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
     double max_pres = 0.0;
     int max_pit = 0;
     int it = 0;
-    bool converged = false, stalled = false;
+    bool converged = false, stalled = false, abandoned = false, aborted_in_step = false;
+    bool frozen = false;  // the last step's eigensolves kept their tight columns bit for bit
+    int consistent = 0;   // DIIS entries measured on the current frozen orbitals' output
+    double prev_tail = INFINITY;
+    std::vector<lira_partial_level> partial;
+    double q_out[2] = {0.0, 0.0};
+    inner_abort_ = fd ? inner_abort : nullptr;
+    abort_user_ = user;
+    struct InnerReset {
+        Solver *s;
+        ~InnerReset() { s->inner_abort_ = nullptr; }
+    } inner_reset{this};
     while (it < st->max_iter) {
         if (abort && abort(user)) {
-            out->iterations = it;
-            record(it, eo);
-            return LIRA_ABORTED;
+            abandoned = true;
+            break;
         }
         it++;
+        // the step's eigen tolerance from the previous step's delta_rho (crates/qm-gpu/src/scf.rs:383-386, 1417-1419): a
+        // delta_rho already below the tolerance it was tested against buys a tight step
+        const double step_tol = delta_rho < test_tol ? lo
+                                : std::isfinite(delta_rho) ? std::min(std::max(EIG_TOL_FRACTION * delta_rho, lo), hi)
+                                                           : hi;
+        const double delta_prev = delta_rho, test_prev = test_tol;
+        const bool tight = step_tol <= lo;
+        const int growths_before = doublings_;
+        step_tail_ = 0.0;
         add2<T>(rho_t_.template as<T>(), rho_in_[0].template as<T>(), rho_in_[1].template as<T>(), N_, stream_);
         PoissonOut po = poisson(rho_t_.template as<T>(), vh_.as<double>(), ptol, 5000);
         t_poisson += po.seconds;
@@ -1741,26 +2082,46 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
         xc_lsda<T>(rho_in_[0].template as<T>(), rho_in_[1].template as<T>(), exc_.template as<T>(), vxc_[0].template as<T>(), vxc_[1].template as<T>(), N_, stream_, pcp);
         sum3<T>(veff_[0].template as<T>(), vnuc_.template as<T>(), vh_t_.template as<T>(), vxc_[0].template as<T>(), N_, stream_);
         if (!restricted) sum3<T>(veff_[1].template as<T>(), vnuc_.template as<T>(), vh_t_.template as<T>(), vxc_[1].template as<T>(), N_, stream_);
-        // the eigen tolerance follows delta_rho (crates/qm-gpu/src/scf.rs:383-386); a loose first step
-        const double tol_step = std::isfinite(delta_prev) ? std::min(std::max(EIG_TOL_FRACTION * delta_prev, lo), hi) : hi;
-        const bool tight = tol_step <= lo;
         auto te = Clock::now();
-        solve_channel(0, veff_[0].template as<T>(), nel[0], fd, kt, tol_step, st->tol, eo, occ[0]);
-        if (restricted) {
-            Channel<T> &u = ch_[0], &d = ch_[1];
-            ensure_channel(d, std::max(u.k, 1));
-            if (u.k > 0)
-                LIRA_HIP(hipMemcpyAsync(d.X.ptr, u.X.ptr, (size_t)N_ * u.k * sizeof(T), hipMemcpyDeviceToDevice, stream_));
-            d.k = u.k;
-            d.evals = u.evals;
-            d.res = u.res;
-            d.occ = u.occ;
-            occ[1] = occ[0];
-        } else {
-            solve_channel(1, veff_[1].template as<T>(), nel[1], fd, kt, tol_step, st->tol, eo, occ[1]);
+        // Fermi-Dirac: a warm guess within 1.5 tol is kept only when the previous step's delta_rho was below its stop;
+        // on a loosened step the witness tail follows the previous delta_rho (crates/qm-gpu/src/scf.rs:1433-1437)
+        const bool guess_band = delta_prev < test_prev;
+        const double loosened = step_tol > lo ? delta_prev : NAN;
+        bool step_frozen = solve_channel(0, veff_[0].template as<T>(), nel[0], fd, kt, step_tol, st->tol, guess_band,
+                                         loosened, eo, occ[0]);
+        if (!eo.aborted) {
+            if (restricted) {
+                Channel<T> &u = ch_[0], &d = ch_[1];
+                ensure_channel(d, std::max(u.k, 1));
+                if (u.k > 0)
+                    LIRA_HIP(hipMemcpyAsync(d.X.ptr, u.X.ptr, (size_t)N_ * u.k * sizeof(T), hipMemcpyDeviceToDevice, stream_));
+                d.k = u.k;
+                d.evals = u.evals;
+                d.res = u.res;
+                d.occ = u.occ;
+                d.tight = u.tight;
+                occ[1] = occ[0];
+            } else {
+                step_frozen &= solve_channel(1, veff_[1].template as<T>(), nel[1], fd, kt, step_tol, st->tol, guess_band,
+                                             loosened, eo, occ[1]);
+            }
         }
         t_eigen += secs(te);
-        // output densities and delta_rho
+        if (eo.aborted) {
+            // stopped inside an eigensolve: the iterate is parked with occupations 1 on each channel's lowest n
+            // levels, and the step does not count (crates/qm-gpu/src/scf.rs:1445-1456)
+            for (int s = 0; s < 2; s++) {
+                Channel<T> &c = ch_[s];
+                const std::vector<int> o = ascending(c.evals);
+                c.occ.assign(c.evals.size(), 0.0);
+                for (int i = 0; i < std::min((int)o.size(), nel[s]); i++) c.occ[o[i]] = 1.0;
+            }
+            it--;
+            abandoned = true;
+            aborted_in_step = true;
+            break;
+        }
+        // output densities
         double *df = dsmall_.as<double>() + 1024;
         for (int s = 0; s < 2; s++) {
             Channel<T> &c = ch_[s];
@@ -1771,73 +2132,115 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
                 fill<T>(rho_out_[s].template as<T>(), N_, 0.0, stream_);
             }
         }
-        RedJobs<T> jd{};
-        jd.a[0] = rho_out_[0].template as<T>();
-        jd.b[0] = rho_in_[0].template as<T>();
-        jd.kind[0] = RED_ABSDIFF;
-        jd.a[1] = rho_out_[1].template as<T>();
-        jd.b[1] = rho_in_[1].template as<T>();
-        jd.kind[1] = RED_ABSDIFF;
-        jd.count = 2;
-        auto dr = reduce(jd, dv_);
-        delta_rho = dr[0] + dr[1];
+        // the residual of the entry this step would push goes to a free slot (one always is: H + 1 slots)
+        int slot = 0;
+        {
+            std::vector<bool> used(H1, false);
+            for (int o : order) used[o] = true;
+            while (used[slot]) slot++;
+        }
+        for (int s = 0; s < 2; s++)
+            sub2<T>(slots_[slot].res[s].template as<T>(), rho_out_[s].template as<T>(), rho_in_[s].template as<T>(), N_, stream_);
+        // delta_rho, the electron counts, the new DIIS Gram row and (observed solves) the Harris-Foulkes double
+        // counting in one readback (crates/qm-gpu/src/scf.rs:1488-1510)
+        std::vector<Job> jobs;
+        jobs.push_back({rho_out_[0].template as<T>(), rho_in_[0].template as<T>(), RED_ABSDIFF});
+        jobs.push_back({rho_out_[1].template as<T>(), rho_in_[1].template as<T>(), RED_ABSDIFF});
+        jobs.push_back({rho_out_[0].template as<T>(), rho_out_[0].template as<T>(), RED_SUM});
+        jobs.push_back({rho_out_[1].template as<T>(), rho_out_[1].template as<T>(), RED_SUM});
+        for (int o : order)
+            for (int s = 0; s < 2; s++)
+                jobs.push_back({slots_[slot].res[s].template as<T>(), slots_[o].res[s].template as<T>(), RED_DOT});
+        for (int s = 0; s < 2; s++)
+            jobs.push_back({slots_[slot].res[s].template as<T>(), slots_[slot].res[s].template as<T>(), RED_DOT});
+        const size_t row_end = jobs.size();
+        if (observe) {
+            jobs.push_back({rho_t_.template as<T>(), vh_t_.template as<T>(), RED_DOT});
+            jobs.push_back({rho_in_[0].template as<T>(), vxc_[0].template as<T>(), RED_DOT});
+            jobs.push_back({rho_in_[1].template as<T>(), vxc_[1].template as<T>(), RED_DOT});
+            jobs.push_back({exc_.template as<T>(), exc_.template as<T>(), RED_SUM});
+        }
+        const std::vector<double> vals = reduce_list(jobs, dv_);
+        delta_rho = vals[0] + vals[1];
+        q_out[0] = vals[2];
+        q_out[1] = vals[3];
+        std::vector<double> row;
+        for (size_t i = 4; i < row_end; i += 2) row.push_back(vals[i] + vals[i + 1]);
         drho_steps_.push_back(delta_rho);
-        etol_steps_.push_back(tol_step);
+        etol_steps_.push_back(step_tol);
+        // the tolerance this step is tested against: max(tol, partial_tol) when its occupations have a partly occupied
+        // level (crates/qm/src/occupations.rs:427-446)
+        partial.clear();
+        if (fd)
+            for (int s = 0; s < 2; s++) {
+                const Channel<T> &c = ch_[s];
+                const std::vector<int> o = ascending(c.evals);
+                for (int r = 0; r < (int)o.size(); r++) {
+                    const double f = c.occ[o[r]];
+                    if (std::min(f, 1.0 - f) >= PARTIAL_OCCUPATION_MIN)
+                        partial.push_back(lira_partial_level{s, r, c.evals[o[r]], f});
+                }
+            }
+        test_tol = fd && !partial.empty() ? std::max(st->tol, partial_tol) : st->tol;
         if (trace_on())
-            std::fprintf(stderr, "lira: SCF step %d delta_rho %.3e eig tol %.1e%s, stored %d/%d, %.1f ms\n", it, delta_rho,
-                         tol_step, tight ? " (tight)" : "", ch_[0].k, ch_[1].k, 1e3 * secs(t0));
-        // stall watch (qm/src/scf.rs:433-469)
+            std::fprintf(stderr, "lira: SCF step %d delta_rho %.3e (test %.0e) eig tol %.1e%s, stored %d/%d, DIIS %d, %.1f ms\n",
+                         it, delta_rho, test_tol, step_tol, tight ? " (tight)" : "", ch_[0].k, ch_[1].k,
+                         (int)order.size(), 1e3 * secs(t0));
+        // the stall watch (qm/src/scf.rs:455-466)
         if (delta_rho < best) best = delta_rho;
         if (std::isfinite(delta_rho) && (!std::isfinite(anchor) || delta_rho <= 0.5 * anchor)) {
             anchor = delta_rho;
             anchor_at = it;
         }
         const int since = it - anchor_at;
+        if (observe) {
+            // Harris-Foulkes energy of the step's input density: sum f eps + double counting - kT S
+            std::vector<double> band;
+            for (int s = 0; s < 2; s++)
+                for (int i = 0; i < ch_[s].k; i++) band.push_back(ch_[s].evals[i] * ch_[s].occ[i]);
+            const double dc = neumaier({-0.5 * vals[row_end], -vals[row_end + 1], -vals[row_end + 2], vals[row_end + 3], e_nn});
+            const double ent = fd ? neumaier({occ[0].entropy, occ[1].entropy}) : 0.0;
+            const lira_progress pr{it, st->max_iter, delta_rho, st->tol, neumaier(band) + dc - kt * ent, best, since,
+                                   st->stall_window, level};
+            observe(&pr, user);
+        }
         if (!std::isfinite(delta_rho)) break;
-        if (delta_rho < st->tol && tight) {
+        if (delta_rho < test_tol && tight) {
             converged = true;
             break;
         }
-        if (delta_rho >= st->tol && st->stall_window > 0 && since >= st->stall_window && best >= level) {
+        if (!(delta_rho < test_tol) && st->stall_window > 0 && since >= st->stall_window && best >= level) {
             stalled = true;
             break;
         }
-        delta_prev = delta_rho;
-        // DIIS (qm/src/scf.rs:681-718): push (rho_in, r = rho_out - rho_in), drop the oldest beyond the history
-        int slot;
-        if ((int)order.size() < H) {
-            std::vector<bool> used(H, false);
-            for (int o : order) used[o] = true;
-            slot = 0;
-            while (used[slot]) slot++;
-        } else {
-            slot = order.front();
-            order.pop_front();
-        }
-        for (int s = 0; s < 2; s++) {
-            copy<T>(slots_[slot].rho[s].template as<T>(), rho_in_[s].template as<T>(), N_, stream_);
-            sub2<T>(slots_[slot].res[s].template as<T>(), rho_out_[s].template as<T>(), rho_in_[s].template as<T>(), N_, stream_);
-        }
+        // a frozen orbital set re-solved changes the map rho_in -> rho_out, and so does a witness window that widened
+        // and grew the block: the history restarts (crates/qm-gpu/src/scf.rs:1538-1552)
+        const bool widened = step_tail_ > prev_tail && doublings_ > growths_before;
+        prev_tail = step_tail_;
+        const bool map_changed = ((frozen && !step_frozen) || widened) && !order.empty();
+        if (map_changed) profile_.diis_resets++;
+        // a frozen phase that does not contract the residual by (1 - alpha) per step in the Gram norm drops the
+        // entries older than the freeze (crates/qm-gpu/src/scf.rs:1553-1566)
+        consistent = step_frozen && fd ? (frozen ? consistent + 1 : 2) : 0;
+        const bool not_contracting =
+            !order.empty() && row.back() > (1.0 - st->mixing) * (1.0 - st->mixing) * gram[(size_t)order.back() * H1 + order.back()];
+        const bool freeze_trim = fd && frozen && step_frozen && !map_changed && not_contracting &&
+                                 (int)order.size() + 1 > consistent;
+        if (freeze_trim) profile_.diis_freeze_trims++;
+        if (step_frozen && fd) profile_.frozen_steps++;
+        frozen = step_frozen;
+        // push (rho_in, r): the Gram row against every entry, then the restart, the trim or the oldest dropped
+        for (size_t i = 0; i < order.size(); i++)
+            gram[(size_t)slot * H1 + order[i]] = gram[(size_t)order[i] * H1 + slot] = row[i];
+        gram[(size_t)slot * H1 + slot] = row.back();
+        if (map_changed) order.clear();
+        else if (freeze_trim)
+            while ((int)order.size() > consistent - 1) order.pop_front();
+        for (int s = 0; s < 2; s++) std::swap(slots_[slot].rho[s], rho_in_[s]);  // the entry keeps this rho_in
         order.push_back(slot);
-        {
-            RedJobs<T> jg{};
-            int cnt = 0;
-            for (int o : order)
-                for (int s = 0; s < 2; s++) {
-                    jg.a[cnt] = slots_[slot].res[s].template as<T>();
-                    jg.b[cnt] = slots_[o].res[s].template as<T>();
-                    jg.kind[cnt] = RED_DOT;
-                    cnt++;
-                }
-            jg.count = cnt;
-            auto gv = reduce(jg, dv_);
-            int idx = 0;
-            for (int o : order) {
-                const double val = gv[idx] + gv[idx + 1];
-                idx += 2;
-                gram[(size_t)slot * H + o] = gram[(size_t)o * H + slot] = val;
-            }
-        }
+        while ((int)order.size() > H) order.pop_front();
+        // Pulay coefficients: the bordered system on the Gram scaled by its largest diagonal, +1e-12 on the diagonal,
+        // LU; the oldest entries dropped while the solve fails (crates/qm-gpu/src/scf.rs:606-646)
         std::vector<double> cvec;
         for (;;) {
             const int m = (int)order.size();
@@ -1848,11 +2251,11 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
             const int mm = m + 1;
             std::vector<double> Bm((size_t)mm * mm, 0.0), rhs(mm, 0.0);
             double scale = 0.0;
-            for (int i = 0; i < m; i++) scale = std::max(scale, gram[(size_t)order[i] * H + order[i]]);
+            for (int i = 0; i < m; i++) scale = std::max(scale, gram[(size_t)order[i] * H1 + order[i]]);
             bool ok = false;
             if (scale > 0.0) {
                 for (int i = 0; i < m; i++) {
-                    for (int j = 0; j < m; j++) Bm[(size_t)i * mm + j] = gram[(size_t)order[i] * H + order[j]] / scale;
+                    for (int j = 0; j < m; j++) Bm[(size_t)i * mm + j] = gram[(size_t)order[i] * H1 + order[j]] / scale;
                     Bm[(size_t)i * mm + i] += DIIS_REG;
                     Bm[(size_t)i * mm + m] = Bm[(size_t)m * mm + i] = 1.0;
                 }
@@ -1879,14 +2282,46 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
             }
             diis_mix<T>(rho_in_[s].template as<T>(), rp.data(), rr.data(), dcv, (int)order.size(), st->mixing, N_, stream_);
         }
+        // each channel's charge renormalised after the clamp to its output's integral, on the device (no readback;
+        // crates/qm-gpu/src/scf.rs:1574-1582)
+        {
+            RedJobs<T> jq{};
+            for (int s = 0; s < 2; s++) {
+                jq.a[s] = jq.b[s] = rho_in_[s].template as<T>();
+                jq.kind[s] = RED_SUM;
+            }
+            jq.count = 2;
+            reduce_jobs<T>(jq, n_, dv_, partials_.as<double>(), dres_.as<double>() + 1000, stream_);
+            for (int s = 0; s < 2; s++)
+                if (nel[s] > 0)
+                    scale_ratio<T>(rho_in_[s].template as<T>(), N_, dres_.as<double>() + 2 + s, dres_.as<double>() + 1000 + s,
+                                   stream_);
+        }
     }
     out->iterations = it;
     out->delta_rho = delta_rho;
     out->max_eigen_residual = eo.max_residual;
+    out->max_guard_residual = fd ? eo.guard_residual : 0.0;
     out->lobpcg_iterations = lobpcg_total_;
     out->restricted = restricted ? 1 : 0;
-    out->tol_used = st->tol;
+    out->tol_used = test_tol;
     record(it, eo);
+    if (abandoned) {
+        // the in-progress state is parked as the next warm start: its input densities stand in for the output ones
+        // (crates/qm-gpu/src/scf.rs:1588-1606)
+        for (int s = 0; s < 2; s++) copy<T>(rho_out_[s].template as<T>(), rho_in_[s].template as<T>(), N_, stream_);
+        LIRA_HIP(hipStreamSynchronize(stream_));
+        parked_ = true;
+        state_ionic_version_ = ionic_version_;
+        nel_[0] = nel[0];
+        nel_[1] = nel[1];
+        occ_scheme_ = st->occupations;
+        kt_ = st->kt;
+        profile_.abandoned_in_step = aborted_in_step ? it + 1 : 0;
+        profile_.abandoned_before_step = aborted_in_step ? 0 : it + 1;
+        out->seconds_total = secs(t0);
+        return LIRA_ABORTED;
+    }
     if (stalled) {
         out->seconds_total = secs(t0);
         return LIRA_STALLED;
@@ -1972,13 +2407,36 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
     out->n_stored[1] = ch_[1].k;
     out->cg_iterations = cg_total_;
     out->readbacks = readbacks_;
+    // the partly occupied levels, the core overlaps and the warnings (qm/src/scf.rs:1083-1090)
+    out->n_partial = std::min((int)partial.size(), LIRA_MAX_PARTIAL);
+    for (int i = 0; i < out->n_partial; i++) out->partial[i] = partial[i];
+    out->n_core_overlaps = std::min((int)overlaps.size(), LIRA_MAX_OVERLAPS);
+    for (int i = 0; i < out->n_core_overlaps; i++) out->core_overlaps[i] = overlaps[i];
+    {
+        std::string w;
+        int nw = 0;
+        char line[160];
+        if (out->boundary_density > BOUNDARY_DENSITY_WARN) {
+            std::snprintf(line, sizeof line, "density on box faces is %.2e: box too small\n", out->boundary_density);
+            w += line;
+            nw++;
+        }
+        for (const auto &ov : overlaps) {
+            std::snprintf(line, sizeof line, "cores overlap: %d-%d at R = %.3f Bohr\n", ov.i, ov.j, ov.r);
+            w += line;
+            nw++;
+        }
+        if (w.size() >= sizeof out->warnings) w.resize(sizeof out->warnings - 1);
+        std::memcpy(out->warnings, w.c_str(), w.size() + 1);
+        out->n_warnings = nw;
+    }
     out->seconds_poisson = t_poisson;
     out->seconds_eigen = t_eigen;
     out->seconds_total = secs(t0);
     out->seconds_other = out->seconds_total - t_poisson - t_eigen;
     valid_ = true;
     state_ionic_version_ = ionic_version_;
-    if (extrapolate_ && reuse) {
+    if (extrapolate_ && reuse && prev_converged) {
         for (int s = 0; s < 2; s++) std::swap(prev_rho_[s], save_rho_[s]);
         have_prev_ = true;
     }
@@ -2092,9 +2550,13 @@ template <typename T> void Solver<T>::forces(const lira_nucleus *nuc, int nat, d
     }
     upload_nuclei(nuc, nat);
     add2<T>(rho_t_.template as<T>(), rho_out_[0].template as<T>(), rho_out_[1].template as<T>(), N_, stream_);
-    hf_forces<T>(rho_t_.template as<T>(), n_, h_, pos_.as<double>(), zf_.as<double>(), nat, sigma, partials_.as<double>(),
-                 dres_.as<double>(), stream_);
-    const double *f = readback(dres_.as<double>(), (size_t)3 * nat);
+    // 3 RED_BLOCKS partials and 3 sums per nucleus: sized by nat, like the ionic terms' buffers
+    const size_t np = (size_t)3 * std::max(nat, 1);
+    if (fpart_.bytes < np * RED_BLOCKS * sizeof(double)) fpart_.alloc(np * RED_BLOCKS * sizeof(double));
+    if (fout_.bytes < np * sizeof(double)) fout_.alloc(np * sizeof(double));
+    hf_forces<T>(rho_t_.template as<T>(), n_, h_, pos_.as<double>(), zf_.as<double>(), nat, sigma, fpart_.as<double>(),
+                 fout_.as<double>(), stream_);
+    const double *f = readback(fout_.as<double>(), (size_t)3 * nat);
     std::vector<double> fo(f, f + 3 * nat);
     // nucleus-nucleus part (qm/src/forces.rs:224-234)
     for (int i = 0; i < nat; i++)
@@ -2175,7 +2637,8 @@ template <typename T> PoissonOut Solver<T>::poisson_host(const double *rho, doub
     PoissonOut po = poisson(rho_t_.template as<T>(), vh_.as<double>(), tol, max_iter);
     LIRA_HIP(hipMemcpyAsync(v, vh_.ptr, N_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
     LIRA_HIP(hipStreamSynchronize(stream_));
-    valid_ = false;
+    valid_ = false;  // V_H is no longer the state's
+    parked_ = false;
     return po;
 }
 
@@ -2195,8 +2658,9 @@ EigenOut Solver<T>::lobpcg_host(const double *v, int k, double *x, double *evals
         convert<T, double>(c.X.template as<T>() + (long)b * N_, rho64_.as<double>(), N_, stream_);
     }
     c.k = k;
-    valid_ = false;
-    EigenOut eo = lobpcg(c, veff_[0].template as<T>(), k, lam, tol, max_iter, tol, k, k);
+    valid_ = false;  // the block is no longer the state's
+    parked_ = false;
+    EigenOut eo = lobpcg(c, veff_[0].template as<T>(), k, lam, tol, max_iter, tol, k, k, tol, false);
     for (int b = 0; b < k; b++) {
         convert<double, T>(rho64_.as<double>(), c.X.template as<T>() + (long)b * N_, N_, stream_);
         LIRA_HIP(hipMemcpyAsync(x + (long)b * N_, rho64_.ptr, N_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
@@ -2249,6 +2713,12 @@ template <typename F> static lira_status guarded(lira_solver *s, F &&f) {
             if (s->impl) s->impl->reset();
         }
         return LIRA_DEVICE;
+    } catch (...) {  // nothing crosses the C ABI
+        if (s) {
+            s->err = "unknown failure";
+            if (s->impl) s->impl->reset();
+        }
+        return LIRA_DEVICE;
     }
 }
 
@@ -2276,7 +2746,13 @@ lira_status lira_solver_create(int n, double h, int precision, lira_solver **out
         g_create_error = "lira_solver_create: bad grid or precision";
         return LIRA_INVALID;
     }
-    auto s = std::make_unique<lira_solver>();
+    std::unique_ptr<lira_solver> s;
+    try {
+        s = std::make_unique<lira_solver>();
+    } catch (const std::exception &e) {
+        g_create_error = std::string("lira_solver_create: ") + e.what();
+        return LIRA_DEVICE;
+    }
     s->n = n;
     s->h = h;
     lira_status st = guarded(s.get(), [&] {
@@ -2297,9 +2773,16 @@ const char *lira_last_error(const lira_solver *s) { return s ? s->err.c_str() : 
 
 lira_status lira_solve(lira_solver *s, const lira_system *sys, const lira_settings *settings, int warm,
                        lira_abort_fn abort, void *user, lira_scf_stats *stats) {
+    return lira_solve_polled(s, sys, settings, warm, abort, nullptr, nullptr, user, stats);
+}
+
+lira_status lira_solve_polled(lira_solver *s, const lira_system *sys, const lira_settings *settings, int warm,
+                              lira_abort_fn abort, lira_abort_fn inner_abort, lira_observe_fn observe, void *user,
+                              lira_scf_stats *stats) {
     if (!s || !sys || !settings || !stats) return LIRA_INVALID;
     lira_status result = LIRA_OK;
-    lira_status g = guarded(s, [&] { result = s->impl->solve(sys, settings, warm, abort, user, stats); });
+    lira_status g =
+        guarded(s, [&] { result = s->impl->solve(sys, settings, warm, abort, inner_abort, observe, user, stats); });
     if (g != LIRA_OK) return g;
     if (result == LIRA_NOT_CONVERGED)
         s->err = "SCF did not converge after " + std::to_string(stats->iterations) + " iterations";
@@ -2451,6 +2934,12 @@ void lira_reset(lira_solver *s) {
     if (s) s->impl->reset();
 }
 
+lira_status lira_inject_readback_fault(lira_solver *s, int after) {
+    if (!s) return LIRA_INVALID;
+    s->impl->inject_readback_fault(after);
+    return LIRA_OK;
+}
+
 lira_status lira_poisson(lira_solver *s, const double *rho, double *v, double tol, int max_iter,
                          lira_poisson_stats *stats) {
     if (!s || !rho || !v || !(tol > 0.0) || max_iter < 1) return LIRA_INVALID;
@@ -2478,6 +2967,10 @@ lira_status lira_rayleigh_ritz(int m, const double *a, const double *b, int nk, 
         std::memcpy(c, C.data(), (size_t)m * nk * sizeof(double));
         return LIRA_OK;
     } catch (const NumericalError &) {
+        return LIRA_NUMERICAL;
+    } catch (const std::bad_alloc &) {
+        return LIRA_INVALID;  // m too large for the host
+    } catch (...) {
         return LIRA_NUMERICAL;
     }
 }

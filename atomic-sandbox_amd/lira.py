@@ -31,7 +31,8 @@ EXPORTS = [
     "lira_autotune_stencil", "lira_set_stencil_variant", "lira_version", "lira_set_ionic", "lira_force_terms",
     "lira_ionic_fields", "lira_nonlocal_apply", "lira_free_atom_levels", "lira_set_witness_cap", "lira_witness_cap",
     "lira_set_density_extrapolation", "lira_get_profile", "lira_sample_density", "lira_set_graphs",
-    "lira_state_summary", "lira_effective_gpu_tol", "lira_evaluate", "lira_solve_batch",
+    "lira_state_summary", "lira_effective_gpu_tol", "lira_evaluate", "lira_solve_batch", "lira_solve_polled",
+    "lira_inject_readback_fault",
 ]
 
 
@@ -41,7 +42,9 @@ class Profile(C.Structure):
                 ("eigen_iterations", C.c_int), ("eigen_fallbacks", C.c_int), ("witness_growths", C.c_int),
                 ("witness_trims", C.c_int), ("peak_block", C.c_int), ("extrapolated", C.c_int),
                 ("wall", C.c_double), ("start_wall", C.c_double), ("poisson_wall", C.c_double),
-                ("eigen_wall", C.c_double), ("eig_tol_used", C.c_double)]
+                ("eigen_wall", C.c_double), ("eig_tol_used", C.c_double), ("diis_resets", C.c_int),
+                ("diis_freeze_trims", C.c_int), ("frozen_steps", C.c_int), ("locked_calls", C.c_int),
+                ("abandoned_in_step", C.c_int), ("abandoned_before_step", C.c_int)]
 
 
 class Projector(C.Structure):
@@ -54,7 +57,7 @@ class Species(C.Structure):
     _fields_ = [("z", C.c_int), ("z_v", C.c_double), ("sigma_lr", C.c_double), ("du", C.c_double),
                 ("v_sr_u", C.c_void_p), ("n_v_sr", C.c_int), ("r_sr", C.c_double), ("rho_pc_u", C.c_void_p),
                 ("n_rho_pc", C.c_int), ("pc_radius", C.c_double), ("projectors", C.c_void_p),
-                ("n_projectors", C.c_int)]
+                ("n_projectors", C.c_int), ("r_core", C.c_double)]
 
 
 class QmError(RuntimeError):
@@ -111,6 +114,18 @@ class Energy(C.Structure):
                                            "total")]
 
 
+MAX_PARTIAL, MAX_OVERLAPS, WARNINGS_BYTES, MAX_DIIS_HISTORY = 16, 16, 512, 15
+
+
+class PartialLevel(C.Structure):
+    """lira_partial_level (qm::PartialLevel, crates/qm/src/occupations.rs:425-437)."""
+    _fields_ = [("spin", C.c_int), ("index", C.c_int), ("eps", C.c_double), ("f", C.c_double)]
+
+
+class CoreOverlap(C.Structure):
+    _fields_ = [("i", C.c_int), ("j", C.c_int), ("r", C.c_double)]
+
+
 class ScfStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("delta_rho", C.c_double), ("energy", Energy),
                 ("boundary_density", C.c_double), ("max_eigen_residual", C.c_double),
@@ -118,7 +133,27 @@ class ScfStats(C.Structure):
                 ("mu", C.c_double * 2), ("electrons", C.c_double * 2), ("tol_used", C.c_double),
                 ("n_stored", C.c_int * 2), ("restricted", C.c_int), ("lobpcg_iterations", C.c_int),
                 ("cg_iterations", C.c_int), ("readbacks", C.c_int), ("seconds_total", C.c_double),
-                ("seconds_poisson", C.c_double), ("seconds_eigen", C.c_double), ("seconds_other", C.c_double)]
+                ("seconds_poisson", C.c_double), ("seconds_eigen", C.c_double), ("seconds_other", C.c_double),
+                ("max_guard_residual", C.c_double), ("n_partial", C.c_int), ("partial", PartialLevel * MAX_PARTIAL),
+                ("n_core_overlaps", C.c_int), ("core_overlaps", CoreOverlap * MAX_OVERLAPS),
+                ("n_warnings", C.c_int), ("warnings", C.c_char * WARNINGS_BYTES)]
+
+    def partial_levels(self):
+        """[(spin, index, eps, f)] of the final step's partly occupied levels."""
+        return [(p.spin, p.index, p.eps, p.f) for p in self.partial[:self.n_partial]]
+
+    def overlaps(self):
+        return [(o.i, o.j, o.r) for o in self.core_overlaps[:self.n_core_overlaps]]
+
+    def warning_lines(self):
+        return [w for w in self.warnings.decode(errors="replace").split("\n") if w]
+
+
+class Progress(C.Structure):
+    """lira_progress (qm::ScfProgress, crates/qm/src/scf.rs:482-498)."""
+    _fields_ = [("iteration", C.c_int), ("max_iter", C.c_int), ("delta_rho", C.c_double), ("tol", C.c_double),
+                ("energy", C.c_double), ("best_delta_rho", C.c_double), ("since_gain", C.c_int), ("window", C.c_int),
+                ("stall_level", C.c_double)]
 
 
 class BatchJob(C.Structure):
@@ -142,6 +177,7 @@ class EigenStats(C.Structure):
 
 
 ABORT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
+OBSERVE_FN = C.CFUNCTYPE(None, C.POINTER(Progress), C.c_void_p)
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 
 
@@ -163,6 +199,9 @@ def lib():
         L.lira_last_error.restype = C.c_char_p
         L.lira_solve.argtypes = [vp, C.POINTER(System), C.POINTER(Settings), C.c_int, ABORT_FN, vp,
                                  C.POINTER(ScfStats)]
+        L.lira_solve_polled.argtypes = [vp, C.POINTER(System), C.POINTER(Settings), C.c_int, ABORT_FN, ABORT_FN,
+                                        OBSERVE_FN, vp, C.POINTER(ScfStats)]
+        L.lira_inject_readback_fault.argtypes = [vp, C.c_int]
         L.lira_forces.argtypes = [vp, C.POINTER(Nucleus), C.c_int, C.c_double, _dp]
         L.lira_get_field.argtypes = [vp, C.c_int, _dp]
         L.lira_get_levels.argtypes = [vp, C.c_int, _dp, _dp, C.c_int, C.POINTER(C.c_int)]
@@ -363,6 +402,32 @@ class GpuSolver:
         _raise(st, self._h, "lira_solve")
         return stats
 
+    def solve_polled(self, pos, z, n_up, n_dn, settings: Settings | None = None, warm=False, abort=None,
+                     inner_abort=None, observe=None):
+        """lira_solve_polled (GpuSolver::solve_polled): abort() before each step, inner_abort() once per Fermi-Dirac
+        LOBPCG iteration, observe(dict) once per step. Returns (status, stats); an abandoned solve is ABORTED, not an
+        exception, and parks its iterate as the next warm start."""
+        nuc = nuclei_array(pos, z)
+        sysd = System(C.cast(nuc, C.POINTER(Nucleus)), len(nuc), n_up, n_dn)
+        s = settings if settings is not None else default_settings(self.h)
+        stats = ScfStats()
+        cb = ABORT_FN(lambda _u: int(bool(abort()))) if abort else ABORT_FN(0)
+        icb = ABORT_FN(lambda _u: int(bool(inner_abort()))) if inner_abort else ABORT_FN(0)
+
+        def _obs(p, _u):
+            q = p.contents
+            observe({k: getattr(q, k) for k, _ in Progress._fields_})
+        ocb = OBSERVE_FN(_obs) if observe else OBSERVE_FN(0)
+        st = lib().lira_solve_polled(self._h, C.byref(sysd), C.byref(s), int(warm), cb, icb, ocb, None,
+                                     C.byref(stats))
+        if st not in (OK, ABORTED, NOT_CONVERGED, STALLED):
+            _raise(st, self._h, "lira_solve_polled")
+        return st, stats
+
+    def inject_readback_fault(self, after: int):
+        """lira_inject_readback_fault: the readback `after` readbacks from now reports a device failure."""
+        _raise(lib().lira_inject_readback_fault(self._h, int(after)), self._h, "lira_inject_readback_fault")
+
     def evaluate(self, pos, z, n_up, n_dn, settings: Settings | None = None, warm=False, abort=None):
         """lira_evaluate, one provider evaluation (floor-aware tolerance, the cold retry of a failed warm solve,
         forces): returns (status, stats, forces (nuclei x 3), flags, tol_used); a failed solve is a status, not an
@@ -398,7 +463,7 @@ class GpuSolver:
             arr[i] = Species(sp.z, sp.z_v, sp.sigma_lr, sp.du, v.ctypes.data, v.size, sp.r_sr,
                              None if pc is None else pc.ctypes.data, 0 if pc is None else pc.size,
                              sp.pc_radius if pc is not None else 0.0, C.cast(projs, C.c_void_p).value,
-                             len(sp.projectors))
+                             len(sp.projectors), float(getattr(sp, "r_core", 0.0)))
         _raise(lib().lira_set_ionic(self._h, C.cast(arr, C.c_void_p), len(species)), self._h, "lira_set_ionic")
 
     def force_terms(self, pos, z, sigma) -> np.ndarray:
@@ -459,7 +524,7 @@ class GpuSolver:
         return float(a[0]), float(m[0])
 
     def set_graphs(self, on):
-        """Replay the fixed kernel sequences as HIP graphs (on, the default) or launch kernels one by one
+        """Replay the fixed kernel sequences as HIP graphs (on) or launch kernels one by one (off, the default)
         (lira_set_graphs)."""
         _raise(lib().lira_set_graphs(self._h, int(bool(on))), self._h, "lira_set_graphs")
 

@@ -40,15 +40,17 @@ class Projector:
 
 class Species:
     """The fields of FilteredTable (filter.rs:330-338) the grid reads: Z, Z_v, sigma_LR, du, the V_SR table and its
-    radius, the partial-core table and its radius (<= 0: no core), and the projectors."""
+    radius, the partial-core table and its radius (<= 0: no core), the projectors, and the derived core radius r_core
+    (two cores closer than the sum of theirs warn)."""
 
-    def __init__(self, z, z_v, sigma_lr, du, v_sr_u, r_sr, rho_pc_u=None, pc_radius=0.0, projectors=()):
+    def __init__(self, z, z_v, sigma_lr, du, v_sr_u, r_sr, rho_pc_u=None, pc_radius=0.0, projectors=(), r_core=0.0):
         self.z, self.z_v, self.sigma_lr, self.du = int(z), float(z_v), float(sigma_lr), float(du)
         self.v_sr_u = np.ascontiguousarray(v_sr_u, dtype=np.float64)
         self.r_sr = float(r_sr)
         self.rho_pc_u = None if rho_pc_u is None else np.ascontiguousarray(rho_pc_u, dtype=np.float64)
         self.pc_radius = float(pc_radius) if rho_pc_u is not None else 0.0
         self.projectors = list(projectors)
+        self.r_core = float(r_core)
 
 
 # profile parameters (amplitude, width, radius) of the O-like ion: Z = 8, Z_v = 6

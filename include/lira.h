@@ -71,6 +71,22 @@ typedef struct {
     double kinetic, external_, hartree, xc, nuclear, internal, ts, total;
 } lira_energy;
 
+/* qm::PartialLevel (crates/qm/src/occupations.rs:425-437): a level with min(f, 1 - f) >= 1e-6 */
+typedef struct {
+    int spin;   /* 0 up, 1 down */
+    int index;  /* rank in ascending order of eps within its channel */
+    double eps, f;
+} lira_partial_level;
+/* one entry of ScfStats::core_overlaps (crates/qm/src/ionic.rs:175-185): nuclei i < j whose core radii overlap */
+typedef struct {
+    int i, j;
+    double r;
+} lira_core_overlap;
+#define LIRA_MAX_PARTIAL 16
+#define LIRA_MAX_OVERLAPS 16
+#define LIRA_WARNINGS_BYTES 512
+#define LIRA_MAX_DIIS_HISTORY 15
+
 /* qm::ScfStats (crates/qm/src/scf.rs:558-574), plus the solve's profile (SolveProfile, crates/qm-gpu/src/scf.rs:420-514) */
 typedef struct {
     int iterations;
@@ -83,7 +99,8 @@ typedef struct {
     double entropy;
     double mu[2];
     double electrons[2];
-    double tol_used;
+    double tol_used;  /* the delta_rho tolerance the converged step was tested against: settings.tol, or in LIRA_MIXED
+                         under Fermi-Dirac max(settings.tol, 1e-5) on a step with a partly occupied level */
     int n_stored[2];  /* orbitals stored per spin */
     int restricted;   /* 1 when the spin-down channel mirrored the spin-up one (n_up == n_dn) */
     /* profile */
@@ -91,7 +108,28 @@ typedef struct {
     int cg_iterations;
     int readbacks;
     double seconds_total, seconds_poisson, seconds_eigen, seconds_other;
+    /* Fermi-Dirac: the worst exit residual of the loose (guard) witness columns; 0 under Aufbau */
+    double max_guard_residual;
+    /* the final step's partly occupied levels, ascending eps per spin, up first (at most LIRA_MAX_PARTIAL) */
+    int n_partial;
+    lira_partial_level partial[LIRA_MAX_PARTIAL];
+    /* pseudised nuclei whose cores overlap (at most LIRA_MAX_OVERLAPS); each is also a warning */
+    int n_core_overlaps;
+    lira_core_overlap core_overlaps[LIRA_MAX_OVERLAPS];
+    /* ScfStats::warnings: n_warnings lines, each ending in '\n', NUL-terminated (cut at LIRA_WARNINGS_BYTES) */
+    int n_warnings;
+    char warnings[LIRA_WARNINGS_BYTES];
 } lira_scf_stats;
+
+/* qm::ScfProgress (crates/qm/src/scf.rs:482-498): one SCF step as an observer sees it. energy is the Harris-Foulkes
+ * energy of the step's input density (F_HF = sum f eps + double counting - kT S under Fermi-Dirac). */
+typedef struct {
+    int iteration, max_iter;
+    double delta_rho, tol, energy, best_delta_rho;
+    int since_gain, window;
+    double stall_level;
+} lira_progress;
+typedef void (*lira_observe_fn)(const lira_progress *progress, void *user);
 
 /* Poisson solve statistics (PoissonStats, crates/qm/src/poisson.rs:136) */
 typedef struct {
@@ -122,10 +160,21 @@ lira_status lira_solver_create(int n, double h, int precision, lira_solver **out
 void lira_solver_destroy(lira_solver *s);
 const char *lira_last_error(const lira_solver *s);
 
-/* GpuSolver::solve_polled (crates/qm-gpu/src/scf.rs:1195-1235): runs the SCF. warm != 0 reuses the previous converged
- * state when it fits (same grid, counts and scheme). abort (may be NULL) is polled before each SCF step. */
+/* GpuSolver::solve_with (crates/qm-gpu/src/scf.rs:1195-1212): runs the SCF. warm != 0 reuses the previous converged
+ * state (or the one an aborted solve parked) when it fits (same grid, counts and scheme). abort (may be NULL) is polled
+ * before each SCF step. lira_solve_polled with no inner abort and no observer. */
 lira_status lira_solve(lira_solver *s, const lira_system *sys, const lira_settings *settings, int warm,
                        lira_abort_fn abort, void *user, lira_scf_stats *stats);
+
+/* GpuSolver::solve_polled (crates/qm-gpu/src/scf.rs:1213-1235; the caller providers::ks::GpuKs::evaluate_once,
+ * crates/providers/src/ks.rs:1617-1621). abort is polled before each SCF step; inner_abort (may be NULL) also once per
+ * LOBPCG iteration of the Fermi-Dirac eigensolves. When either returns nonzero the solve is abandoned: LIRA_ABORTED, no
+ * converged state, and the in-progress iterate (orbitals, levels, the mixed input densities, V_H) is parked as the next
+ * warm start. observe (may be NULL) gets one lira_progress per SCF step after its delta_rho readback; an observer
+ * changes no number of the solve. user is passed to all three. */
+lira_status lira_solve_polled(lira_solver *s, const lira_system *sys, const lira_settings *settings, int warm,
+                              lira_abort_fn abort, lira_abort_fn inner_abort, lira_observe_fn observe, void *user,
+                              lira_scf_stats *stats);
 
 /* GpuSolver::forces (crates/qm-gpu/src/forces.rs:239-266): Hellmann–Feynman forces of the last converged state,
  * out[3 * n_nuclei]. sigma must be the solve's. */
@@ -155,6 +204,7 @@ typedef struct {
     double pc_radius;
     const lira_projector *projectors; /* at most 3, one per non-local channel */
     int n_projectors;
+    double r_core; /* the derived core radius (FilteredTable::r_core): two cores closer than the sum of theirs warn */
 } lira_species;
 
 /* The ionic setup of the next solves (System::ionic, crates/qm/src/scf.rs:329): species listed here are pseudised,
@@ -188,6 +238,11 @@ typedef struct {
     int steps, readbacks, poisson_iterations, eigen_iterations, eigen_fallbacks;
     int witness_growths, witness_trims, peak_block, extrapolated;
     double wall, start_wall, poisson_wall, eigen_wall, eig_tol_used;
+    /* DIIS restarts (a frozen orbital set re-solved, or a widened witness window that grew the block), freeze trims,
+     * frozen Fermi-Dirac steps and eigensolves with a locked head (SolveProfile, crates/qm-gpu/src/scf.rs:420-514) */
+    int diis_resets, diis_freeze_trims, frozen_steps, locked_calls;
+    /* an abandoned solve: the step whose eigensolve inner_abort stopped, or the step abort came before (else 0) */
+    int abandoned_in_step, abandoned_before_step;
 } lira_profile;
 /* the profile, plus delta_rho and the eigen tolerance of every step (up to cap each; *n_steps the count) */
 lira_status lira_get_profile(lira_solver *s, lira_profile *out, double *delta_rho_steps, double *eig_tol_steps,
@@ -248,9 +303,11 @@ lira_status lira_solve_batch(const lira_batch_job *jobs, int n_jobs, int precisi
                              int workers, lira_batch_result *results, double *forces);
 
 /* Dispatch control, no counterpart in the reference (its recorder batches dispatches instead,
- * crates/qm-gpu/src/batch.rs:55-233): on (the default), the fixed kernel sequences between two host syncs (the
- * Poisson MG-PCG iteration, in its two forms, and its start) are captured once as HIP graphs and replayed; off, every
- * kernel is launched one by one. Results are bitwise identical either way. */
+ * crates/qm-gpu/src/batch.rs:55-233): on, the fixed kernel sequences between two host syncs (the Poisson MG-PCG
+ * iteration, in its two forms, and its start) are captured once as HIP graphs and replayed; off (the default), every
+ * kernel is launched one by one. Results are bitwise identical either way. Turn it on only where no other host thread
+ * allocates device memory or launches work while this solver captures (a capture is open during the first call of
+ * each sequence); with solvers on several threads keep it off. */
 lira_status lira_set_graphs(lira_solver *s, int on);
 
 /* The spherical spin-restricted LSDA free atom behind the Fermi-Dirac cold guess (qm::radial::atom::ground_state,
@@ -269,6 +326,11 @@ lira_status lira_get_orbitals(lira_solver *s, int spin, double *out, int cap, in
 double lira_effective_eig_tol(const lira_solver *s, double requested);
 /* drops the warm state (the next solve is cold) */
 void lira_reset(lira_solver *s);
+/* Fault injection for the device-failure path (the reference's readback-fault latch, crates/qm-gpu/src/grid.rs:113-157,
+ * and its test hook, crates/gpu/src/readback.rs:37-69): the readback `after` readbacks from now (0 = the next one)
+ * completes its copy, then reports a failure, which the call in progress returns as LIRA_DEVICE with no state kept.
+ * Host side only: every kernel runs and exits normally. after < 0 disarms it. */
+lira_status lira_inject_readback_fault(lira_solver *s, int after);
 
 /* --------------------------------------------------------------------------------- sub-solvers ---- */
 

