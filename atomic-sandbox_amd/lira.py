@@ -32,7 +32,7 @@ EXPORTS = [
     "lira_ionic_fields", "lira_nonlocal_apply", "lira_free_atom_levels", "lira_set_witness_cap", "lira_witness_cap",
     "lira_set_density_extrapolation", "lira_get_profile", "lira_sample_density", "lira_set_graphs",
     "lira_state_summary", "lira_effective_gpu_tol", "lira_evaluate", "lira_solve_batch", "lira_solve_polled",
-    "lira_inject_readback_fault",
+    "lira_inject_readback_fault", "lira_gram_noalloc",
 ]
 
 
@@ -220,6 +220,7 @@ def lib():
         L.lira_nuclear_potential.argtypes = [C.c_int, C.c_int, C.c_double, C.POINTER(Nucleus), C.c_int, C.c_double,
                                              vp, vp]
         L.lira_gram.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, vp, C.c_int, vp, _dp, vp]
+        L.lira_gram_noalloc.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, vp, C.c_int, vp, vp, vp]
         L.lira_stencil_variant_count.restype = C.c_int
         L.lira_stencil_variant_name.argtypes = [C.c_int]
         L.lira_stencil_variant_name.restype = C.c_char_p
@@ -633,6 +634,12 @@ def gram(n, h, s_block, m, u_block, q, stream=None):
     g = np.zeros(m * q)
     _raise(lib().lira_gram(_dtype(s_block), n, h, m, _ptr(s_block), q, _ptr(u_block), g, stream), None, "gram")
     return g.reshape(m, q)
+
+
+def gram_noalloc(n, h, s_block, m, u_block, q, g_dev, stream=None):
+    """lira_gram_noalloc: G = h^3 S^T U into the device tensor g_dev (m x q f64) with no device allocation; returns the
+    status (LIRA_INVALID when the shape's partials exceed the fixed scratch array)."""
+    return lib().lira_gram_noalloc(_dtype(s_block), n, h, m, _ptr(s_block), q, _ptr(u_block), _ptr(g_dev), stream)
 
 
 def autotune_stencil(dtype_size, n, nbands):

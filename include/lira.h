@@ -372,6 +372,17 @@ lira_status lira_nuclear_potential(int dtype, int n, double h, const lira_nucleu
 lira_status lira_gram(int dtype, int n, double h, int m, const void *s_block, int q, const void *u_block, double *g,
                       void *stream);
 
+/* No counterpart in the reference: lira_gram for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). G = h^3 S^T U goes to g_dev, a device buffer of
+ * m x q f64 (row-major), on `stream`, without waiting; the per-chunk partial sums live in a fixed device array of
+ * LIRA_NOALLOC_SCRATCH_BYTES on each GPU the library runs on (held until the process ends), so the call allocates
+ * nothing. The result is bitwise lira_gram's. A shape whose partials do not fit the array returns LIRA_INVALID before
+ * any device work. Calls on one GPU share its array: calls that overlap in time on one GPU race (order them on one
+ * stream, or with events). */
+#define LIRA_NOALLOC_SCRATCH_BYTES (16u << 20)
+lira_status lira_gram_noalloc(int dtype, int n, double h, int m, const void *s_block, int q, const void *u_block,
+                              double *g_dev, void *stream);
+
 /* stencil tile variants and the autotuner (the tile shape is picked per grid by timing) */
 int lira_stencil_variant_count(void);
 const char *lira_stencil_variant_name(int v);
