@@ -32,4 +32,30 @@ Status retry_cold_on_stall(bool warm, Status not_converged, Status stalled, Solv
     return s;
 }
 
+// The "GPU busy" policy (crates/providers/src/ks.rs:317-350): attempt() runs; a device failure (a readback the shared
+// device did not deliver) is retried after a pause, at most DEVICE_RETRIES times, pause(ms) returning false when the
+// evaluation was cancelled meanwhile (then `cancelled` is returned). Every other outcome is the attempt's own.
+// *failed receives the number of failed attempts before the returned one.
+constexpr int DEVICE_RETRIES = 2;
+constexpr int DEVICE_BACKOFF_MS[DEVICE_RETRIES] = {100, 400};
+template <typename Status, typename Attempt, typename Pause>
+Status retry_on_device_fault(Status device, Status cancelled, Attempt &&attempt, Pause &&pause, int *failed) {
+    *failed = 0;
+    for (;;) {
+        const Status s = attempt();
+        if (s != device || *failed == DEVICE_RETRIES) return s;
+        if (!pause(DEVICE_BACKOFF_MS[*failed])) return cancelled;
+        ++*failed;
+    }
+}
+
+// witness_cap_for (crates/providers/src/ks.rs:400-428): the largest w in (counted, max_cap] whose footprint(w) fits
+// budget, never below `counted` (the start the solve allocates). footprint must grow with w.
+template <typename Footprint>
+int witness_cap_for_budget(Footprint &&footprint, unsigned long long budget, int counted, int max_cap) {
+    int cap = counted;
+    for (int w = counted + 1; w <= max_cap && footprint(w) <= budget; w++) cap = w;
+    return cap;
+}
+
 }  // namespace lira

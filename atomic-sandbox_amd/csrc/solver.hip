@@ -16,6 +16,8 @@
 #include <map>
 #include <memory>
 #include <numeric>
+#include <thread>
+#include <type_traits>
 
 #include "lira.h"
 #include "lira_internal.hpp"
@@ -88,6 +90,31 @@ constexpr double PARTIAL_OCCUPATION_MIN = 1e-6;
 constexpr double GPU_PARTIAL_TOL = 1e-5;
 // density on a box face above this (1/Bohr^3) warns that the box is too small (crates/qm/src/scf.rs:577)
 constexpr double BOUNDARY_DENSITY_WARN = 1e-5;
+// display solves: guard levels above the shown ones and their tolerance (WITNESS_GAP / 100), and the level-matching
+// tolerance tau = max(100 tol, 1e-4) Ha (crates/qm/src/orbitals.rs:368-415; crates/qm-gpu/src/harris.rs)
+constexpr int UNOCC_GUARD = 2;
+constexpr double GUARD_TOL = 1e-3;
+constexpr double TAU_PER_TOL = 100.0, TAU_MIN = 1e-4;
+// a display solve whose occupied levels disagree with the SCF's: reported, the converged state kept
+struct DisplayRejected : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+// the witness vectors per spin a solve allocates at its start (WITNESS_GROWTH_COUNTED, crates/providers/src/ks.rs:375)
+constexpr int WITNESS_GROWTH_COUNTED = 3;
+
+// Device bytes one solve holds on an n^3 grid (the provider's footprint model, crates/providers/src/ks.rs:360-400,
+// restated for this library's buffers): per spin 10 block fields (X, HX, R, W, HW, P, HP, X', P', scratch) per column
+// for n_sigma + witness + 1 columns (the buffer), 12 SCF fields, 4 per DIIS slot (history + 1), 4 for density
+// extrapolation, 7 of the Poisson solve and about 4/7 of a field for its coarse levels, all in precision esize; 5 f64
+// fields (V_H, the f64 density copy, the Poisson right-hand side, residual and scratch); 1.1 for the small buffers.
+static unsigned long long solve_footprint(int n, int esize, int n_up, int n_dn, int witness, int diis_history) {
+    const double N = (double)n * n * n;
+    const int h1 = std::max(diis_history, 1) + 1;
+    const double cols = (n_up > 0 ? n_up + witness + 1 : 1) + (n_dn > 0 ? n_dn + witness + 1 : 1);
+    const double t_fields = 10.0 * cols + 12.0 + 4.0 * h1 + 4.0 + 7.0 + 4.0 / 7.0;
+    return (unsigned long long)(1.1 * N * (esize * t_fields + 8.0 * 5.0));
+}
 
 // ------------------------------------------------------------------------------------- occupations ----
 static double occupation(double x) {
@@ -349,6 +376,11 @@ struct SolverIface {
     virtual void get_levels(int spin, double *evals, double *occ, int cap, int *count) = 0;
     virtual void get_orbitals(int spin, double *out, int cap, int *count) = 0;
     virtual PoissonOut poisson_host(const double *rho, double *v, double tol, int max_iter) = 0;
+    virtual void mg_vcycle_host(const double *r, double *z) = 0;
+    virtual void unoccupied(int m, lira_unoccupied_result *out, double *orbitals) = 0;
+    virtual void harris(const lira_system *sys, const lira_settings *st, const double *rho_in_up,
+                        const double *rho_in_dn, int m, lira_harris_result *out, double *rho_out_up,
+                        double *rho_out_dn) = 0;
     virtual EigenOut lobpcg_host(const double *v, int k, double *x, double *evals, double lam, double tol,
                                  int max_iter) = 0;
     virtual double effective_eig_tol(double requested) const = 0;
@@ -382,6 +414,7 @@ struct SpeciesHost {
 
 template <typename T> struct Channel {
     int kcap = 0, k = 0;
+    int limit = 0;  // the most columns the channel may need (electrons + witness cap + buffer); 0 = unbounded
     int tight = 0;  // Fermi-Dirac: the leading columns held to the eigen tolerance (the rest to the guard tolerance)
     DevBuf X, HX, R, W, HW, P, HP, XN, PN, TMP;
     std::vector<double> evals, occ, res;
@@ -404,6 +437,10 @@ template <typename T> class Solver final : public SolverIface {
     void get_levels(int spin, double *evals, double *occ, int cap, int *count) override;
     void get_orbitals(int spin, double *out, int cap, int *count) override;
     PoissonOut poisson_host(const double *rho, double *v, double tol, int max_iter) override;
+    void mg_vcycle_host(const double *r, double *z) override;
+    void unoccupied(int m, lira_unoccupied_result *out, double *orbitals) override;
+    void harris(const lira_system *sys, const lira_settings *st, const double *rho_in_up, const double *rho_in_dn, int m,
+                lira_harris_result *out, double *rho_out_up, double *rho_out_dn) override;
     EigenOut lobpcg_host(const double *v, int k, double *x, double *evals, double lam, double tol,
                          int max_iter) override;
     double effective_eig_tol(double requested) const override {
@@ -782,6 +819,13 @@ template <typename T> class Solver final : public SolverIface {
                        double loosened_delta, EigenOut &eo, Fd &occ);
     void permute_columns(Channel<T> &c, const std::vector<int> &order);
     int buffer_last(Channel<T> &c);
+    void sort_block(Channel<T> &c);
+    // display solves: the unoccupied view's block and kept vectors, the Harris solve's channels and fields
+    Channel<T> disp_, hch_[2];
+    DevBuf unocc_[2];
+    int unocc_k_[2] = {0, 0};
+    DevBuf hr_[2], hrho_, hvh_, hvht_, hexc_, hxc_[2], hv_[2], hvnuc_, hvnuc_save_;
+    double last_tol_ = 1e-6, last_eig_tol_ = 1e-6;  // the converged solve's settings (display tolerances)
     // the SCF's inner abort, polled once per LOBPCG iteration under Fermi-Dirac (solve_polled's inner_abort)
     lira_abort_fn inner_abort_ = nullptr;
     void *abort_user_ = nullptr;
@@ -803,7 +847,8 @@ template <typename T> Solver<T>::Solver(int n, double h) : n_(n), h_(h), dv_(h *
 
 template <typename T> void Solver<T>::ensure_channel(Channel<T> &c, int kc) {
     if (kc <= c.kcap) return;
-    if (c.kcap > 0) kc = std::max(kc, std::min(2 * c.kcap, kc + 8));  // headroom for witness growth
+    // headroom for witness growth, never beyond what the witness cap lets the channel hold
+    if (c.kcap > 0) kc = std::max(kc, std::min({2 * c.kcap, kc + 8, std::max(kc, c.limit)}));
     const size_t bytes = (size_t)N_ * kc * sizeof(T);
     DevBuf nx(bytes);
     if (c.k > 0) LIRA_HIP(hipMemcpyAsync(nx.ptr, c.X.ptr, (size_t)N_ * c.k * sizeof(T), hipMemcpyDeviceToDevice, stream_));
@@ -1967,6 +2012,7 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
         const double scale = ztot > 0.0 ? total / ztot : 0.0;
         for (int s = 0; s < 2; s++) {
             const double f = total > 0.0 ? nel[s] / total : 0.0;
+            ch_[s].limit = nel[s] + witness_cap_ + 1;
             if (!fd)
                 deposit_charge<T>(rho_in_[s].template as<T>(), n_, h_, pos_.as<double>(),
                                   (ionic ? qf_ : zf_).template as<double>(), nat, 1.0, scale * f, stream_);
@@ -1993,6 +2039,7 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
         fill<double>(vh_.as<double>(), N_, 0.0, stream_);
     } else {
         for (int s = 0; s < 2; s++) {
+            ch_[s].limit = nel[s] + witness_cap_ + 1;
             // the tight count starts from the warm state's occupations (qm/src/occupations.rs:352-366)
             if (fd) ch_[s].tight = tight_start(ch_[s].occ, nel[s], st->tol);
             if (extrapolate_) {  // keep rho_t: it becomes rho_{t-1} once this solve converges
@@ -2436,6 +2483,8 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
     out->seconds_other = out->seconds_total - t_poisson - t_eigen;
     valid_ = true;
     state_ionic_version_ = ionic_version_;
+    last_tol_ = st->tol;
+    last_eig_tol_ = st->eig_tol;
     if (extrapolate_ && reuse && prev_converged) {
         for (int s = 0; s < 2; s++) std::swap(prev_rho_[s], save_rho_[s]);
         have_prev_ = true;
@@ -2642,6 +2691,20 @@ template <typename T> PoissonOut Solver<T>::poisson_host(const double *rho, doub
     return po;
 }
 
+// z = M r: one V-cycle of the Poisson preconditioner on host f64 arrays (Multigrid::apply_in,
+// crates/qm-gpu/src/multigrid.rs:107-197), through the solve's own levels and kernels in precision T
+template <typename T> void Solver<T>::mg_vcycle_host(const double *r, double *z) {
+    if (!r || !z) throw InvalidError("lira_mg_vcycle: null array");
+    poisson_setup();
+    if (rho64_.bytes < N_ * sizeof(double)) rho64_.alloc(N_ * sizeof(double));
+    LIRA_HIP(hipMemcpyAsync(rho64_.ptr, r, N_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+    convert<T, double>(pq_.template as<T>(), rho64_.as<double>(), N_, stream_);
+    vcycle(0, pe_.template as<T>(), pq_.template as<T>());
+    convert<double, T>(rho64_.as<double>(), pe_.template as<T>(), N_, stream_);
+    LIRA_HIP(hipMemcpyAsync(z, rho64_.ptr, N_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    LIRA_HIP(hipStreamSynchronize(stream_));
+}
+
 template <typename T>
 EigenOut Solver<T>::lobpcg_host(const double *v, int k, double *x, double *evals, double lam, double tol,
                                 int max_iter) {
@@ -2668,6 +2731,336 @@ EigenOut Solver<T>::lobpcg_host(const double *v, int k, double *x, double *evals
     }
     for (int i = 0; i < k; i++) evals[i] = c.evals[i];
     return eo;
+}
+
+// ------------------------------------------------------------------------------- display products ----
+// One spin channel's display eigensolve on a given potential: the block's columns ascending after the solve.
+template <typename T> void Solver<T>::sort_block(Channel<T> &c) {
+    const std::vector<int> o = ascending(c.evals);
+    bool sorted = true;
+    for (int i = 0; i < (int)o.size(); i++)
+        if (o[i] != i) sorted = false;
+    if (!sorted) permute_columns(c, o);
+}
+
+// The unoccupied levels of the converged state (OrbitalView::unoccupied, crates/qm-gpu/src/orbitals.rs:997-1148): in
+// the state's own output potentials V_nuc + V_H[rho_out] + v_xc[rho_out] (V_NL inside H under an ionic setup), one
+// LOBPCG of the shown occupied columns plus m + UNOCC_GUARD more at the SCF's effective eigen tolerance, guessed from
+// copies of the occupied orbitals, the previous call's unoccupied vectors, the witness columns and fresh LCG ones (seed
+// 3 up / 4 down). The occupied pairs are checked against the SCF's (within tau) and dropped; the others are projected
+// onto the complement of the SCF's occupied orbitals, re-measured (Rayleigh quotient and residual) and sorted, and kept
+// as the next call's guess.
+template <typename T>
+void Solver<T>::unoccupied(int m, lira_unoccupied_result *out, double *orbitals) {
+    if (!valid_) throw InvalidError("lira_unoccupied: no converged state");
+    if (m < 0 || m + UNOCC_GUARD > LIRA_UNOCC_MAX) throw InvalidError("lira_unoccupied: m out of range");
+    std::memset(out, 0, sizeof *out);
+    const int ku = m + UNOCC_GUARD;
+    const double tol = effective_eig_tol(last_eig_tol_);
+    const double tau = last_tol_ <= TAU_MIN / TAU_PER_TOL ? TAU_MIN : std::max(TAU_PER_TOL * last_tol_, TAU_MIN);
+    const bool fd = occ_scheme_ == LIRA_FERMI_DIRAC;
+    const T *pcp = has_pc_ ? pc_.template as<T>() : nullptr;
+    // the output potentials of the state (the SCF's kernels and term order)
+    xc_lsda<T>(rho_out_[0].template as<T>(), rho_out_[1].template as<T>(), exc_.template as<T>(),
+               vxc_[0].template as<T>(), vxc_[1].template as<T>(), N_, stream_, pcp);
+    convert<T, double>(vh_t_.template as<T>(), vh_.as<double>(), N_, stream_);
+    for (int s = 0; s < 2; s++)
+        sum3<T>(veff_[s].template as<T>(), vnuc_.template as<T>(), vh_t_.template as<T>(), vxc_[s].template as<T>(), N_,
+                stream_);
+    const bool mirror = nel_[0] == nel_[1];
+    const size_t fb = (size_t)N_ * sizeof(T);
+    std::string failure;
+    for (int s = 0; s < 2; s++) {
+        if (mirror && s == 1) {  // a singlet's down channel is the up channel
+            out->count[1] = out->count[0];
+            out->iterations[1] = out->iterations[0];
+            out->occupied_error[1] = out->occupied_error[0];
+            for (int i = 0; i < out->count[0]; i++) {
+                out->evals[1][i] = out->evals[0][i];
+                out->residuals[1][i] = out->residuals[0][i];
+            }
+            if (orbitals)
+                std::memcpy(orbitals + (size_t)ku * N_, orbitals, (size_t)ku * N_ * sizeof(double));
+            continue;
+        }
+        Channel<T> &c = ch_[s];
+        // the shown columns: every stored one, or under Fermi-Dirac the f-bearing ones (qm/src/orbitals.rs:1518-1526)
+        std::vector<int> shown, witness;
+        for (int i = 0; i < c.k; i++)
+            ((!fd || c.occ[i] >= PARTIAL_OCCUPATION_MIN) ? shown : witness).push_back(i);
+        const int nc = (int)shown.size();
+        std::vector<double> ev_scf;
+        for (int i : shown) ev_scf.push_back(c.evals[i]);
+        std::sort(ev_scf.begin(), ev_scf.end());
+        Channel<T> &d = disp_;
+        d.k = 0;
+        ensure_channel(d, nc + ku);
+        int col = 0;
+        auto copy_col = [&](const T *src) {
+            LIRA_HIP(hipMemcpyAsync(d.X.template as<T>() + (long)col * N_, src, fb, hipMemcpyDeviceToDevice, stream_));
+            col++;
+        };
+        for (int i : shown) copy_col(c.X.template as<T>() + (long)i * N_);
+        const int prev = std::min(unocc_k_[s], ku);
+        for (int i = 0; i < prev; i++) copy_col(unocc_[s].template as<T>() + (long)i * N_);
+        for (int i = 0; i < (int)witness.size() && col < nc + ku; i++) copy_col(c.X.template as<T>() + (long)witness[i] * N_);
+        if (col < nc + ku)
+            initial_orbitals<T>(d.X.template as<T>() + (long)col * N_, n_, h_, pos_.as<double>(), nat_, nc + ku - col,
+                                3ULL + (uint64_t)s, stream_);
+        d.k = nc + ku;
+        d.evals.assign(d.k, 0.0);
+        d.res.assign(d.k, 0.0);
+        const T *v = veff_[s].template as<T>();
+        EigenOut e = lobpcg(d, v, d.k, NAN, tol, 200, tol, d.k, d.k, tol, false);
+        out->iterations[s] = e.iterations;
+        sort_block(d);
+        double err = 0.0;
+        for (int i = 0; i < nc; i++) err = std::max(err, std::fabs(d.evals[i] - ev_scf[i]));
+        out->occupied_error[s] = err;
+        if (!(err <= tau) && failure.empty()) {
+            char msg[200];
+            std::snprintf(msg, sizeof msg,
+                          "the display eigensolve's occupied eigenvalues disagree with the SCF's by %.2e Ha (spin %d) > "
+                          "tau = %.0e",
+                          err, s, tau);
+            failure = msg;
+        }
+        // the unoccupied vectors, projected onto the complement of the SCF's occupied orbitals
+        T *phi = d.X.template as<T>() + (long)nc * N_;
+        if (nc > 0) {
+            for (int i = 0; i < nc; i++)
+                LIRA_HIP(hipMemcpyAsync(d.TMP.template as<T>() + (long)i * N_, c.X.template as<T>() + (long)shown[i] * N_,
+                                        fb, hipMemcpyDeviceToDevice, stream_));
+            ensure_gram(ku, nc);
+            Cols S{{{phi, ku}}, 1}, U{{{d.TMP.ptr, nc}}, 1};
+            gram<T>(S, U, N_, dv_, gpart_.as<double>(), gpart_cap_, dG_.as<double>(), stream_);
+            const double *g = readback(dG_.as<double>(), (size_t)ku * nc);
+            std::vector<double> C((size_t)(ku + nc) * ku, 0.0);
+            for (int i = 0; i < ku; i++) {
+                C[(size_t)i * ku + i] = 1.0;
+                for (int j = 0; j < nc; j++) C[(size_t)(ku + j) * ku + i] = -g[(size_t)i * nc + j];
+            }
+            const int ldc = upload_coeffs(C, ku + nc, ku);
+            Cols Sm{{{phi, ku}, {d.TMP.ptr, nc}}, 2};
+            lincomb<T>(Sm, dC_.as<double>(), ldc, ku, d.XN.template as<T>(), N_, stream_);
+            LIRA_HIP(hipMemcpyAsync(phi, d.XN.ptr, (size_t)ku * fb, hipMemcpyDeviceToDevice, stream_));
+        }
+        // re-measured: the Rayleigh quotient and residual of each projected vector
+        hamiltonian(v, phi, d.HX.template as<T>(), ku);
+        ensure_gram(ku, 2 * ku);
+        {
+            Cols S{{{phi, ku}}, 1}, U{{{phi, ku}, {d.HX.ptr, ku}}, 2};
+            gram<T>(S, U, N_, dv_, gpart_.as<double>(), gpart_cap_, dG_.as<double>(), stream_);
+        }
+        const double *g = readback(dG_.as<double>(), (size_t)ku * 2 * ku);
+        std::vector<double> theta(ku), xx(ku);
+        for (int i = 0; i < ku; i++) {
+            xx[i] = g[(size_t)i * 2 * ku + i];
+            theta[i] = g[(size_t)i * 2 * ku + ku + i] / xx[i];
+        }
+        upload(dsmall_.as<double>() + 256, theta.data(), ku);
+        residual<T>(d.R.template as<T>(), d.HX.template as<T>(), phi, dsmall_.as<double>() + 256, ku, N_, stream_);
+        std::vector<Job> jobs;
+        for (int i = 0; i < ku; i++)
+            jobs.push_back({d.R.template as<T>() + (long)i * N_, d.R.template as<T>() + (long)i * N_, RED_DOT});
+        const std::vector<double> rr = reduce_list(jobs, dv_);
+        std::vector<int> o(ku);
+        std::iota(o.begin(), o.end(), 0);
+        std::stable_sort(o.begin(), o.end(), [&](int a, int b) { return theta[a] < theta[b]; });
+        // kept in order as the next call's guess
+        if ((size_t)unocc_[s].bytes < (size_t)ku * fb) unocc_[s].alloc((size_t)ku * fb);
+        for (int i = 0; i < ku; i++)
+            LIRA_HIP(hipMemcpyAsync(unocc_[s].template as<T>() + (long)i * N_, phi + (long)o[i] * N_, fb,
+                                    hipMemcpyDeviceToDevice, stream_));
+        unocc_k_[s] = ku;
+        out->count[s] = ku;
+        for (int i = 0; i < ku; i++) {
+            out->evals[s][i] = theta[o[i]];
+            out->residuals[s][i] = std::sqrt(std::max(rr[o[i]], 0.0) / xx[o[i]]);
+        }
+        if (orbitals)
+            for (int i = 0; i < ku; i++) {
+                convert<double, T>(rho64_.as<double>(), unocc_[s].template as<T>() + (long)i * N_, N_, stream_);
+                copy_now(orbitals + ((size_t)s * ku + i) * N_, rho64_.ptr, (size_t)N_ * sizeof(double),
+                         hipMemcpyDeviceToHost);
+            }
+    }
+    LIRA_HIP(hipStreamSynchronize(stream_));
+    if (!failure.empty()) {
+        unocc_k_[0] = unocc_k_[1] = 0;
+        throw DisplayRejected(failure);
+    }
+}
+
+// The Harris display solve (crates/qm-gpu/src/harris.rs:477-): one non-self-consistent diagonalisation in the potential
+// of a given input density, V_sigma = v_nuc + V_H[rho_in] + v_xc,sigma[rho_in (+ rho_pc / 2)], under the solver's ionic
+// setup at the given geometry. Per channel a block of n_sigma + m + UNOCC_GUARD levels plus one untested buffer column:
+// the lowest n_sigma + m held to the SCF's effective eigen tolerance, the guards to GUARD_TOL; the occupations applied
+// once (Aufbau: the n lowest; Fermi-Dirac: fermi_dirac over the tested levels, the block grown while no tested level is
+// a witness, as the SCF grows its block); rho_out,sigma = sum f psi^2. The solver's converged state is left as it was.
+template <typename T>
+void Solver<T>::harris(const lira_system *sys, const lira_settings *st, const double *rho_in_up, const double *rho_in_dn,
+                       int m, lira_harris_result *out, double *rho_out_up, double *rho_out_dn) {
+    auto t0 = Clock::now();
+    const int nat = sys->n_nuclei;
+    if (nat < 0 || (nat > 0 && !sys->nuclei) || sys->n_up < 0 || sys->n_dn < 0 || !rho_in_up || !rho_in_dn || m < 0)
+        throw InvalidError("lira_harris: bad arguments");
+    if (!(st->sigma > 0.0) || !(st->tol > 0.0)) throw InvalidError("lira_harris: bad settings");
+    const int nel[2] = {sys->n_up, sys->n_dn};
+    for (int s = 0; s < 2; s++)
+        if (nel[s] + m + UNOCC_GUARD + witness_cap_ + 1 > LIRA_HARRIS_MAX_LEVELS)
+            throw InvalidError("lira_harris: more levels than a result holds");
+    std::memset(out, 0, sizeof *out);
+    const bool fd = st->occupations == LIRA_FERMI_DIRAC;
+    const double kt = st->kt;
+    const double tol = effective_eig_tol(st->eig_tol);
+    // the state's geometry is restored afterwards: Harris has its own
+    const std::vector<double> keep_pos = pos_h_, keep_z = z_h_;
+    const int keep_nat = nat_;
+    const double keep_sigma = sigma_;
+    auto restore = [&]() {
+        std::vector<lira_nucleus> nk(keep_nat);
+        for (int i = 0; i < keep_nat; i++) {
+            for (int a = 0; a < 3; a++) nk[i].pos[a] = keep_pos[3 * i + a];
+            nk[i].z = (int)keep_z[i];
+        }
+        upload_nuclei(nk.data(), keep_nat);
+        if (!species_.empty() && (valid_ || parked_)) {
+            ionic_geometry(keep_sigma);
+            LIRA_HIP(hipMemcpyAsync(vnuc_.ptr, hvnuc_save_.ptr, (size_t)N_ * sizeof(T), hipMemcpyDeviceToDevice, stream_));
+        }
+    };
+    for (DevBuf *b : {&hr_[0], &hr_[1], &hrho_, &hvht_, &hexc_, &hxc_[0], &hxc_[1], &hv_[0], &hv_[1], &hvnuc_})
+        alloc_field(*b);
+    if (hvh_.bytes < N_ * sizeof(double)) {
+        hvh_.alloc(N_ * sizeof(double));
+        fill<double>(hvh_.as<double>(), N_, 0.0, stream_);  // the first call's Poisson starts cold
+    }
+    if (rho64_.bytes < N_ * sizeof(double)) rho64_.alloc(N_ * sizeof(double));
+    const double *rin[2] = {rho_in_up, rho_in_dn};
+    for (int s = 0; s < 2; s++) {
+        LIRA_HIP(hipMemcpyAsync(rho64_.ptr, rin[s], N_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+        convert<T, double>(hr_[s].template as<T>(), rho64_.as<double>(), N_, stream_);
+    }
+    const bool ionic = !species_.empty();
+    if (ionic && (valid_ || parked_)) {
+        alloc_field(hvnuc_save_);
+        LIRA_HIP(hipMemcpyAsync(hvnuc_save_.ptr, vnuc_.ptr, (size_t)N_ * sizeof(T), hipMemcpyDeviceToDevice, stream_));
+    }
+    upload_nuclei(sys->nuclei, nat);
+    try {
+        if (ionic) {
+            ionic_geometry(st->sigma);
+            LIRA_HIP(hipMemcpyAsync(hvnuc_.ptr, vnuc_.ptr, (size_t)N_ * sizeof(T), hipMemcpyDeviceToDevice, stream_));
+        } else {
+            nuclear_potential<T>(hvnuc_.template as<T>(), n_, h_, pos_.as<double>(), zf_.as<double>(), nat, st->sigma,
+                                 stream_);
+            nl_rows_ = 0;
+            has_pc_ = false;
+        }
+        const T *pcp = has_pc_ ? pc_.template as<T>() : nullptr;
+        add2<T>(hrho_.template as<T>(), hr_[0].template as<T>(), hr_[1].template as<T>(), N_, stream_);
+        const double ptol = std::min(1e-10, 1e-3 * st->tol);
+        PoissonOut po = poisson(hrho_.template as<T>(), hvh_.as<double>(), ptol, 5000);
+        (void)po;
+        convert<T, double>(hvht_.template as<T>(), hvh_.as<double>(), N_, stream_);
+        xc_lsda<T>(hr_[0].template as<T>(), hr_[1].template as<T>(), hexc_.template as<T>(), hxc_[0].template as<T>(),
+                   hxc_[1].template as<T>(), N_, stream_, pcp);
+        for (int s = 0; s < 2; s++)
+            sum3<T>(hv_[s].template as<T>(), hvnuc_.template as<T>(), hvht_.template as<T>(), hxc_[s].template as<T>(), N_,
+                    stream_);
+        for (int s = 0; s < 2; s++) {
+            Channel<T> &c = hch_[s];
+            const int base = nel[s] + m + UNOCC_GUARD;
+            // the guess: the previous call's vectors (trimmed or topped up with fresh LCG columns, seed 5 up / 6 down)
+            const int keep = std::min(c.k, base + 1);
+            ensure_channel(c, base + 1);
+            if (keep < base + 1)
+                initial_orbitals<T>(c.X.template as<T>() + (long)keep * N_, n_, h_, pos_.as<double>(), nat, base + 1 - keep,
+                                    5ULL + (uint64_t)s + 2ULL * (uint64_t)keep, stream_);
+            if (c.k == 0) out->cold = 1;
+            c.k = base + 1;
+            c.evals.resize(c.k, 0.0);
+            c.res.resize(c.k, 0.0);
+            for (;;) {
+                const int checked = c.k - 1;  // the top column is the untested buffer
+                const int head = std::min(nel[s] + m, checked);
+                EigenOut e = lobpcg(c, hv_[s].template as<T>(), c.k, NAN, tol, 200, std::max(GUARD_TOL, tol), head,
+                                    checked, tol, false);
+                out->iterations[s] += e.iterations;
+                sort_block(c);
+                if (!fd || nel[s] == 0) break;
+                std::vector<double> ev(c.evals.begin(), c.evals.begin() + checked);
+                const Fd o = fermi_dirac(ev, nel[s], kt);
+                // a short window (no witness among the tested levels, qm::witness_reached on theta): the block grows
+                // as the SCF's does, up to the witness cap
+                auto ordr = ascending(ev);
+                const double top = ev[ordr.back()], eps_n = ev[ordr[nel[s] - 1]];
+                const bool reached = top >= eps_n + WITNESS_GAP && top - o.mu >= WITNESS_TAIL_KT * kt;
+                if (reached || c.k - nel[s] >= witness_cap_ + 1) {
+                    out->window_short[s] = reached ? 0 : 1;
+                    break;
+                }
+                const int extras = c.k - nel[s];
+                const int target = std::min(std::max(2 * extras, extras + 1), witness_cap_ + 1);
+                ensure_channel(c, nel[s] + target);
+                initial_orbitals<T>(c.X.template as<T>() + (long)c.k * N_, n_, h_, pos_.as<double>(), nat,
+                                    nel[s] + target - c.k, 5ULL + (uint64_t)s + 2ULL * (uint64_t)c.k, stream_);
+                c.evals.resize(nel[s] + target, c.evals.back());
+                c.res.resize(nel[s] + target, 0.0);
+                c.k = nel[s] + target;
+            }
+            // the occupations, applied once (qm::one_shot_occupations, crates/qm/src/scf.rs:304-325), over the tested
+            // levels; the buffer holds no charge
+            const int checked = c.k - 1;
+            std::vector<double> f(c.k, 0.0);
+            if (nel[s] > 0) {
+                if (fd) {
+                    std::vector<double> ev(c.evals.begin(), c.evals.begin() + checked);
+                    const Fd o = fermi_dirac(ev, nel[s], kt);
+                    for (int i = 0; i < checked; i++) f[i] = o.f[i];
+                } else {
+                    for (int i = 0; i < nel[s]; i++) f[i] = 1.0;
+                }
+            }
+            int nocc = 0;
+            for (int i = 0; i < c.k; i++)
+                if (f[i] >= (fd ? PARTIAL_OCCUPATION_MIN : 0.5)) nocc++;
+            out->n_occ[s] = nocc;
+            out->n_levels[s] = checked;
+            for (int i = 0; i < checked; i++) {
+                out->evals[s][i] = c.evals[i];
+                out->occ[s][i] = f[i];
+            }
+            c.occ = f;
+            // rho_out = sum f psi^2
+            double *df = dsmall_.as<double>() + 1024 + 256 * s;
+            upload(df, f.data(), c.k);
+            density<T>(hr_[s].template as<T>(), c.X.template as<T>(), df, c.k, N_, stream_);
+        }
+        RedJobs<T> jq{};
+        for (int s = 0; s < 2; s++) {
+            jq.a[s] = jq.b[s] = hr_[s].template as<T>();
+            jq.kind[s] = RED_SUM;
+        }
+        jq.count = 2;
+        const std::vector<double> q = reduce(jq, dv_);
+        out->electrons[0] = q[0];
+        out->electrons[1] = q[1];
+        double *ro[2] = {rho_out_up, rho_out_dn};
+        for (int s = 0; s < 2; s++)
+            if (ro[s]) {
+                convert<double, T>(rho64_.as<double>(), hr_[s].template as<T>(), N_, stream_);
+                copy_now(ro[s], rho64_.ptr, (size_t)N_ * sizeof(double), hipMemcpyDeviceToHost);
+            }
+    } catch (...) {
+        for (auto &c : hch_) c.k = 0;  // the next call is cold
+        restore();
+        throw;
+    }
+    restore();
+    LIRA_HIP(hipStreamSynchronize(stream_));
+    out->seconds = secs(t0);
 }
 
 }  // namespace lira
@@ -2721,6 +3114,8 @@ template <typename F> static lira_status guarded(lira_solver *s, F &&f) {
         return LIRA_DEVICE;
     }
 }
+
+static hipStream_t as_stream(void *s) { return static_cast<hipStream_t>(s); }
 
 extern "C" {
 
@@ -2800,10 +3195,25 @@ lira_status lira_forces(lira_solver *s, const lira_nucleus *nuclei, int n_nuclei
 
 double lira_effective_gpu_tol(double tol, int n_valence) { return effective_gpu_tol(tol, n_valence); }
 
-lira_status lira_evaluate(lira_solver *s, const lira_system *sys, const lira_settings *settings, int warm,
-                          lira_abort_fn abort, void *user, lira_scf_stats *stats, double *forces, int *flags,
-                          double *tol_used) {
-    if (!s || !sys || !settings || !stats || !forces || !flags || !tol_used || sys->n_nuclei <= 0) return LIRA_INVALID;
+unsigned long long lira_solve_footprint_bytes(int n, int precision, int n_up, int n_dn, int witness,
+                                              int diis_history) {
+    if (n < 3 || n_up < 0 || n_dn < 0 || witness < 0 || (precision != LIRA_FP64 && precision != LIRA_MIXED)) return 0;
+    return solve_footprint(n, precision == LIRA_FP64 ? 8 : 4, n_up, n_dn, witness, diis_history);
+}
+
+int lira_witness_cap_for_budget(int n, int precision, int n_up, int n_dn, int diis_history,
+                                unsigned long long budget) {
+    const int esize = precision == LIRA_FP64 ? 8 : 4;
+    return witness_cap_for_budget([&](int w) { return solve_footprint(n, esize, n_up, n_dn, w, diis_history); },
+                                  budget, WITNESS_GROWTH_COUNTED, MAX_WITNESS_GROWTH);
+}
+
+// One attempt of the provider's evaluation (GpuKs::evaluate_once, crates/providers/src/ks.rs:1573-1686): the
+// floor-aware tolerance, solve_polled with both polls and the observer, the R15 cold retry of a failed warm solve, then
+// the forces.
+static lira_status evaluate_once(lira_solver *s, const lira_system *sys, const lira_settings *settings, int warm,
+                                 lira_abort_fn abort, lira_abort_fn inner_abort, lira_observe_fn observe, void *user,
+                                 lira_scf_stats *stats, double *forces, int *flags, double *tol_used) {
     lira_settings set = *settings;
     int fl = 0;
     // the floor-aware stop: under an ionic setup the tolerance is raised to what the f32 density can reach
@@ -2827,7 +3237,7 @@ lira_status lira_evaluate(lira_solver *s, const lira_system *sys, const lira_set
                 failed = *stats;
                 s->impl->set_density_extrapolation(false);
             }
-            return lira_solve(s, sys, &set, w ? 1 : 0, abort, user, stats);
+            return lira_solve_polled(s, sys, &set, w ? 1 : 0, abort, inner_abort, observe, user, stats);
         },
         &retried, &first);
     if (retried) {
@@ -2844,6 +3254,44 @@ lira_status lira_evaluate(lira_solver *s, const lira_system *sys, const lira_set
     if (pg != LIRA_OK) return pg;
     if (prof.extrapolated) *flags |= LIRA_EVAL_EXTRAPOLATED;
     return lira_forces(s, sys->nuclei, sys->n_nuclei, set.sigma, forces);
+}
+
+lira_status lira_evaluate(lira_solver *s, const lira_system *sys, const lira_settings *settings, int warm,
+                          lira_abort_fn abort, lira_abort_fn inner_abort, lira_observe_fn observe, void *user,
+                          unsigned long long vram_budget, lira_scf_stats *stats, double *forces, int *flags,
+                          double *tol_used) {
+    if (!s || !sys || !settings || !stats || !forces || !flags || !tol_used || sys->n_nuclei <= 0) return LIRA_INVALID;
+    // the witness cap a memory budget leaves this solve (GpuKs::apply_witness_cap, crates/providers/src/ks.rs:400-428)
+    int cap = s->impl->witness_cap();
+    const bool budgeted = vram_budget > 0;
+    if (budgeted) {
+        cap = lira_witness_cap_for_budget(s->n, s->impl->precision(), sys->n_up, sys->n_dn, settings->diis_history,
+                                          vram_budget);
+        s->impl->set_witness_cap(cap);
+    }
+    // the "GPU busy" policy: a device failure is retried after a pause (ks.rs:317-350, 1688-1706)
+    int fl = 0, device_failed = 0;
+    const lira_status r = retry_on_device_fault(
+        LIRA_DEVICE, LIRA_ABORTED,
+        [&] { return evaluate_once(s, sys, settings, warm, abort, inner_abort, observe, user, stats, forces, &fl,
+                                   tol_used); },
+        [&](int ms) {  // sleeps in 20 ms slices, false once the caller asks to abort
+            const auto end = std::chrono::steady_clock::now() + std::chrono::milliseconds(ms);
+            while (std::chrono::steady_clock::now() < end) {
+                if (abort && abort(user)) return false;
+                std::this_thread::sleep_for(std::chrono::milliseconds(20));
+            }
+            return !(abort && abort(user));
+        },
+        &device_failed);
+    if (device_failed > 0) fl |= LIRA_EVAL_DEVICE_RETRIED;
+    if (budgeted) fl |= LIRA_EVAL_BUDGET_CAP;
+    *flags = fl;
+    // a witness search stopped by the budget's cap names the way out (with_witness_hint, ks.rs:289-297)
+    const std::string witness_msg = "numerical failure in the solver: no gap witness within";
+    if (r == LIRA_NUMERICAL && budgeted && cap < MAX_WITNESS_GROWTH && s->err.compare(0, witness_msg.size(), witness_msg) == 0)
+        s->err += " - the device memory budget sets this cap: raise the budget or use a coarser grid";
+    return r;
 }
 
 lira_status lira_set_witness_cap(lira_solver *s, int cap) {
@@ -2949,6 +3397,67 @@ lira_status lira_poisson(lira_solver *s, const double *rho, double *v, double to
     });
 }
 
+lira_status lira_unoccupied(lira_solver *s, int m, lira_unoccupied_result *out, double *orbitals) {
+    if (!s || !out) return LIRA_INVALID;
+    lira_status result = LIRA_OK;
+    const lira_status g = guarded(s, [&] {
+        try {
+            s->impl->unoccupied(m, out, orbitals);
+        } catch (const DisplayRejected &e) {  // the converged state stays
+            s->err = e.what();
+            result = LIRA_NUMERICAL;
+        }
+    });
+    return g != LIRA_OK ? g : result;
+}
+
+lira_status lira_harris(lira_solver *s, const lira_system *sys, const lira_settings *settings, const double *rho_in_up,
+                        const double *rho_in_dn, int m, lira_harris_result *out, double *rho_out_up,
+                        double *rho_out_dn) {
+    if (!s || !sys || !settings || !out || !rho_in_up || !rho_in_dn) return LIRA_INVALID;
+    lira_status result = LIRA_OK;
+    const lira_status g = guarded(s, [&] {
+        try {
+            s->impl->harris(sys, settings, rho_in_up, rho_in_dn, m, out, rho_out_up, rho_out_dn);
+        } catch (const NumericalError &e) {  // a display solve's numerical failure leaves the SCF state alone
+            s->err = std::string("the Harris display solve failed: ") + e.what();
+            result = LIRA_NUMERICAL;
+        }
+    });
+    return g != LIRA_OK ? g : result;
+}
+
+lira_status lira_mg_vcycle(lira_solver *s, const double *r, double *z) {
+    if (!s || !r || !z) return LIRA_INVALID;
+    return guarded(s, [&] { s->impl->mg_vcycle_host(r, z); });
+}
+
+lira_status lira_mg_level_op(int dtype, int op, int n, double h, double d_low, double d_high, const void *x,
+                             const void *b, void *out, void *stream) {
+    if ((dtype != 4 && dtype != 8) || n < 1 || !(h > 0.0) || !(d_low >= 1.0) || !(d_high >= 1.0) || !x || !out ||
+        x == out)
+        return LIRA_INVALID;
+    if ((op == LIRA_MG_JACOBI || op == LIRA_MG_RESIDUAL) && (!b || b == out)) return LIRA_INVALID;
+    if ((op == LIRA_MG_RESTRICT || op == LIRA_MG_PROLONG_ADD) && n < 2) return LIRA_INVALID;
+    if (op < LIRA_MG_JACOBI || op > LIRA_MG_PROLONG_ADD) return LIRA_INVALID;
+    return guarded(nullptr, [&] {
+        const double h2 = h * h, gl = 1.0 - 1.0 / d_low, gh = 1.0 - 1.0 / d_high;
+        hipStream_t st = as_stream(stream);
+        auto run = [&](auto *tag) {
+            using U = std::remove_pointer_t<decltype(tag)>;
+            const U *xx = static_cast<const U *>(x), *bb = static_cast<const U *>(b);
+            U *o = static_cast<U *>(out);
+            if (op == LIRA_MG_JACOBI) mg_jacobi<U>(o, xx, bb, n, h2, 6.0 / 7.0, gl, gh, st);
+            else if (op == LIRA_MG_RESIDUAL) mg_residual<U>(o, xx, bb, n, h2, gl, gh, st);
+            else if (op == LIRA_MG_RESTRICT) mg_restrict<U>(o, xx, n / 2, n, st);
+            else mg_prolong_add<U>(o, xx, n, n / 2, st);
+        };
+        if (dtype == 4) run((float *)nullptr);
+        else run((double *)nullptr);
+        LIRA_HIP(hipStreamSynchronize(st));
+    });
+}
+
 lira_status lira_lobpcg(lira_solver *s, const double *v, int k, double *x, double *evals, double precond_lambda,
                         double tol, int max_iter, lira_eigen_stats *stats) {
     if (!s || !v || !x || !evals || k < 1 || !(tol > 0.0)) return LIRA_INVALID;
@@ -2974,8 +3483,6 @@ lira_status lira_rayleigh_ritz(int m, const double *a, const double *b, int nk, 
         return LIRA_NUMERICAL;
     }
 }
-
-static hipStream_t as_stream(void *s) { return static_cast<hipStream_t>(s); }
 
 lira_status lira_hamiltonian(int dtype, int n, double h, int nbands, const void *v, const void *x, void *hx,
                              void *stream) {

@@ -32,7 +32,8 @@ EXPORTS = [
     "lira_ionic_fields", "lira_nonlocal_apply", "lira_free_atom_levels", "lira_set_witness_cap", "lira_witness_cap",
     "lira_set_density_extrapolation", "lira_get_profile", "lira_sample_density", "lira_set_graphs",
     "lira_state_summary", "lira_effective_gpu_tol", "lira_evaluate", "lira_solve_batch", "lira_solve_polled",
-    "lira_inject_readback_fault", "lira_gram_noalloc",
+    "lira_inject_readback_fault", "lira_gram_noalloc", "lira_mg_vcycle", "lira_mg_level_op",
+    "lira_solve_footprint_bytes", "lira_witness_cap_for_budget", "lira_unoccupied", "lira_harris",
 ]
 
 
@@ -149,6 +150,24 @@ class ScfStats(C.Structure):
         return [w for w in self.warnings.decode(errors="replace").split("\n") if w]
 
 
+UNOCC_MAX, HARRIS_MAX_LEVELS = 34, 96
+
+
+class UnoccupiedResult(C.Structure):
+    """lira_unoccupied_result (OrbitalView::unoccupied's record, crates/qm-gpu/src/orbitals.rs:997)."""
+    _fields_ = [("count", C.c_int * 2), ("evals", (C.c_double * UNOCC_MAX) * 2),
+                ("residuals", (C.c_double * UNOCC_MAX) * 2), ("occupied_error", C.c_double * 2),
+                ("iterations", C.c_int * 2)]
+
+
+class HarrisResult(C.Structure):
+    """lira_harris_result (HarrisSolve, crates/qm-gpu/src/harris.rs:210-238)."""
+    _fields_ = [("n_levels", C.c_int * 2), ("n_occ", C.c_int * 2), ("evals", (C.c_double * HARRIS_MAX_LEVELS) * 2),
+                ("occ", (C.c_double * HARRIS_MAX_LEVELS) * 2), ("electrons", C.c_double * 2),
+                ("iterations", C.c_int * 2), ("window_short", C.c_int * 2), ("cold", C.c_int),
+                ("seconds", C.c_double)]
+
+
 class Progress(C.Structure):
     """lira_progress (qm::ScfProgress, crates/qm/src/scf.rs:482-498)."""
     _fields_ = [("iteration", C.c_int), ("max_iter", C.c_int), ("delta_rho", C.c_double), ("tol", C.c_double),
@@ -202,6 +221,9 @@ def lib():
         L.lira_solve_polled.argtypes = [vp, C.POINTER(System), C.POINTER(Settings), C.c_int, ABORT_FN, ABORT_FN,
                                         OBSERVE_FN, vp, C.POINTER(ScfStats)]
         L.lira_inject_readback_fault.argtypes = [vp, C.c_int]
+        L.lira_unoccupied.argtypes = [vp, C.c_int, C.POINTER(UnoccupiedResult), vp]
+        L.lira_harris.argtypes = [vp, C.POINTER(System), C.POINTER(Settings), _dp, _dp, C.c_int,
+                                  C.POINTER(HarrisResult), vp, vp]
         L.lira_forces.argtypes = [vp, C.POINTER(Nucleus), C.c_int, C.c_double, _dp]
         L.lira_get_field.argtypes = [vp, C.c_int, _dp]
         L.lira_get_levels.argtypes = [vp, C.c_int, _dp, _dp, C.c_int, C.POINTER(C.c_int)]
@@ -221,6 +243,8 @@ def lib():
                                              vp, vp]
         L.lira_gram.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, vp, C.c_int, vp, _dp, vp]
         L.lira_gram_noalloc.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, vp, C.c_int, vp, vp, vp]
+        L.lira_mg_vcycle.argtypes = [vp, _dp, _dp]
+        L.lira_mg_level_op.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, vp, vp, vp, vp]
         L.lira_stencil_variant_count.restype = C.c_int
         L.lira_stencil_variant_name.argtypes = [C.c_int]
         L.lira_stencil_variant_name.restype = C.c_char_p
@@ -239,8 +263,13 @@ def lib():
         L.lira_state_summary.argtypes = [vp, _dp, _dp]
         L.lira_effective_gpu_tol.argtypes = [C.c_double, C.c_int]
         L.lira_effective_gpu_tol.restype = C.c_double
-        L.lira_evaluate.argtypes = [vp, C.POINTER(System), C.POINTER(Settings), C.c_int, ABORT_FN, vp,
-                                    C.POINTER(ScfStats), _dp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.lira_evaluate.argtypes = [vp, C.POINTER(System), C.POINTER(Settings), C.c_int, ABORT_FN, ABORT_FN,
+                                    OBSERVE_FN, vp, C.c_ulonglong, C.POINTER(ScfStats), _dp, C.POINTER(C.c_int),
+                                    C.POINTER(C.c_double)]
+        L.lira_solve_footprint_bytes.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.lira_solve_footprint_bytes.restype = C.c_ulonglong
+        L.lira_witness_cap_for_budget.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong]
+        L.lira_witness_cap_for_budget.restype = C.c_int
         L.lira_force_terms.argtypes = [vp, C.POINTER(Nucleus), C.c_int, C.c_double, _dp]
         L.lira_ionic_fields.argtypes = [vp, C.POINTER(Nucleus), C.c_int, C.c_double, vp, vp]
         L.lira_nonlocal_apply.argtypes = [vp, _dp, C.c_int, _dp, vp, C.c_int, C.POINTER(C.c_int)]
@@ -270,6 +299,17 @@ def default_settings(h: float, occupations: int = FERMI_DIRAC, **kw) -> Settings
 
 
 EVAL_TOL_RAISED, EVAL_RETRIED_COLD, EVAL_WARM_STALLED, EVAL_EXTRAPOLATED = 1, 2, 4, 8
+EVAL_DEVICE_RETRIED, EVAL_BUDGET_CAP = 16, 32
+
+
+def solve_footprint_bytes(n, precision, n_up, n_dn, witness, diis_history=6):
+    """Device bytes one solve holds (lira_solve_footprint_bytes)."""
+    return lib().lira_solve_footprint_bytes(n, precision, n_up, n_dn, witness, diis_history)
+
+
+def witness_cap_for_budget(n, precision, n_up, n_dn, budget, diis_history=6):
+    """The witness cap a device memory budget leaves a solve (lira_witness_cap_for_budget)."""
+    return lib().lira_witness_cap_for_budget(n, precision, n_up, n_dn, diis_history, int(budget))
 
 
 def effective_gpu_tol(tol: float, n_valence: int) -> float:
@@ -425,14 +465,42 @@ class GpuSolver:
             _raise(st, self._h, "lira_solve_polled")
         return st, stats
 
+    def unoccupied(self, m: int, want_orbitals=False):
+        """lira_unoccupied: per spin the m + 2 lowest unoccupied levels of the converged state, their residuals and
+        (optionally) orbitals ((2, m + 2, n^3)). Returns (evals [2][m + 2], residuals, occupied_error, orbitals)."""
+        r = UnoccupiedResult()
+        ku = m + 2
+        orbs = np.zeros(2 * ku * self.N) if want_orbitals else None
+        _raise(lib().lira_unoccupied(self._h, int(m), C.byref(r), None if orbs is None else orbs.ctypes.data),
+               self._h, "lira_unoccupied")
+        ev = [np.array(r.evals[s][:r.count[s]]) for s in range(2)]
+        res = [np.array(r.residuals[s][:r.count[s]]) for s in range(2)]
+        return ev, res, list(r.occupied_error), None if orbs is None else orbs.reshape(2, ku, self.N)
+
+    def harris(self, pos, z, n_up, n_dn, rho_up, rho_dn, m=0, settings: Settings | None = None, want_rho=True):
+        """lira_harris: the non-self-consistent display solve in the potential of (rho_up, rho_dn). Returns
+        (HarrisResult, rho_out_up, rho_out_dn)."""
+        nuc = nuclei_array(pos, z)
+        sysd = System(C.cast(nuc, C.POINTER(Nucleus)), len(nuc), n_up, n_dn)
+        s = settings if settings is not None else default_settings(self.h)
+        r = HarrisResult()
+        ou = np.zeros(self.N) if want_rho else None
+        od = np.zeros(self.N) if want_rho else None
+        _raise(lib().lira_harris(self._h, C.byref(sysd), C.byref(s), np.ascontiguousarray(rho_up, dtype=np.float64),
+                                 np.ascontiguousarray(rho_dn, dtype=np.float64), int(m), C.byref(r),
+                                 None if ou is None else ou.ctypes.data, None if od is None else od.ctypes.data),
+               self._h, "lira_harris")
+        return r, ou, od
+
     def inject_readback_fault(self, after: int):
         """lira_inject_readback_fault: the readback `after` readbacks from now reports a device failure."""
         _raise(lib().lira_inject_readback_fault(self._h, int(after)), self._h, "lira_inject_readback_fault")
 
-    def evaluate(self, pos, z, n_up, n_dn, settings: Settings | None = None, warm=False, abort=None):
-        """lira_evaluate, one provider evaluation (floor-aware tolerance, the cold retry of a failed warm solve,
-        forces): returns (status, stats, forces (nuclei x 3), flags, tol_used); a failed solve is a status, not an
-        exception, so its flags stay readable."""
+    def evaluate(self, pos, z, n_up, n_dn, settings: Settings | None = None, warm=False, abort=None,
+                 inner_abort=None, observe=None, vram_budget=0):
+        """lira_evaluate, one provider evaluation (the budget's witness cap, the floor-aware tolerance, the cold retry of
+        a failed warm solve, the retry of a device failure, forces): returns (status, stats, forces (nuclei x 3),
+        flags, tol_used); a failed solve is a status, not an exception, so its flags stay readable."""
         nuc = nuclei_array(pos, z)
         sysd = System(C.cast(nuc, C.POINTER(Nucleus)), len(nuc), n_up, n_dn)
         s = settings if settings is not None else default_settings(self.h)
@@ -440,8 +508,14 @@ class GpuSolver:
         f = np.zeros(3 * len(nuc))
         flags, tol = C.c_int(), C.c_double()
         cb = ABORT_FN(lambda _u: int(bool(abort()))) if abort else ABORT_FN(0)
-        st = lib().lira_evaluate(self._h, C.byref(sysd), C.byref(s), int(warm), cb, None, C.byref(stats), f,
-                                 C.byref(flags), C.byref(tol))
+        icb = ABORT_FN(lambda _u: int(bool(inner_abort()))) if inner_abort else ABORT_FN(0)
+
+        def _obs(p, _u):
+            q = p.contents
+            observe({k: getattr(q, k) for k, _ in Progress._fields_})
+        ocb = OBSERVE_FN(_obs) if observe else OBSERVE_FN(0)
+        st = lib().lira_evaluate(self._h, C.byref(sysd), C.byref(s), int(warm), cb, icb, ocb, None, int(vram_budget),
+                                 C.byref(stats), f, C.byref(flags), C.byref(tol))
         return st, stats, f.reshape(-1, 3), flags.value, tol.value
 
     def set_ionic(self, species):
@@ -570,6 +644,12 @@ class GpuSolver:
                                   C.byref(st)), self._h, "lira_poisson")
         return v, st
 
+    def mg_vcycle(self, r):
+        """z = M r, one V-cycle of the Poisson preconditioner (lira_mg_vcycle)."""
+        z = np.zeros(self.N)
+        _raise(lib().lira_mg_vcycle(self._h, np.ascontiguousarray(r, dtype=np.float64), z), self._h, "lira_mg_vcycle")
+        return z
+
     def lobpcg(self, v, x0, tol, max_iter=500, precond_lambda=-1.0):
         x = np.ascontiguousarray(x0, dtype=np.float64).copy()
         k = x.shape[0]
@@ -640,6 +720,16 @@ def gram_noalloc(n, h, s_block, m, u_block, q, g_dev, stream=None):
     """lira_gram_noalloc: G = h^3 S^T U into the device tensor g_dev (m x q f64) with no device allocation; returns the
     status (LIRA_INVALID when the shape's partials exceed the fixed scratch array)."""
     return lib().lira_gram_noalloc(_dtype(s_block), n, h, m, _ptr(s_block), q, _ptr(u_block), _ptr(g_dev), stream)
+
+
+MG_JACOBI, MG_RESIDUAL, MG_RESTRICT, MG_PROLONG_ADD = range(4)
+
+
+def mg_level_op(op, n, h, d_low, d_high, x, b, out, stream=None):
+    """lira_mg_level_op on torch tensors: one V-cycle level operator (Jacobi sweep, residual, restriction,
+    prolongation-add)."""
+    _raise(lib().lira_mg_level_op(_dtype(x), op, n, h, d_low, d_high, _ptr(x), None if b is None else _ptr(b),
+                                  _ptr(out), stream), None, "mg_level_op")
 
 
 def autotune_stencil(dtype_size, n, nbands):

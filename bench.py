@@ -84,19 +84,8 @@ def poisson_256(lira, n=256, h=0.15, seed=3):
     """Poisson MG-PCG on 256^3: a neutral density of 32 seeded normalised Gaussians (centres in the inner 60 % of the
     box, sigma U[0.3, 1.0] A, charges +1 x 16 and -1 x 16), solved to a relative residual of 1e-8 in MIXED precision.
     Time of the device solve (the density upload excluded), after one untimed solve that builds the levels."""
-    rng = np.random.default_rng(seed)
-    L = (n - 1) * h
-    cen = rng.uniform(-0.3 * L, 0.3 * L, (32, 3))
-    sig = rng.uniform(0.3, 1.0, 32) * BOHR_PER_ANGSTROM
-    q = np.array([1.0] * 16 + [-1.0] * 16)
-    ax = -(n - 1) * h / 2.0 + h * np.arange(n)
-    rho = np.zeros((n, n, n))
-    for c, s_, qq in zip(cen, sig, q):
-        gx = np.exp(-(ax - c[0]) ** 2 / (2 * s_ * s_))
-        gy = np.exp(-(ax - c[1]) ** 2 / (2 * s_ * s_))
-        gz = np.exp(-(ax - c[2]) ** 2 / (2 * s_ * s_))
-        rho += qq / (2 * math.pi * s_ * s_) ** 1.5 * gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
-    rho = rho.reshape(-1)
+    import workloads
+    rho = workloads.neutral_blobs(n, h, seed)
     s = lira.GpuSolver(n, h, lira.MIXED)
     s.poisson(rho, tol=1e-8)
     _, st = s.poisson(rho, tol=1e-8)
@@ -288,36 +277,62 @@ def hbm_traffic(esize, variant, n=192, nb=64):
             "write": per["WRITE_SIZE"]}
 
 
-def cpu_baseline_sample(state, gpu_fp64, n, h, pos, z, threads):
-    """Bounded oracle sample on the same water problem: one f64 Poisson solve of the converged density and two
-    LOBPCG iterations on the converged potential, extrapolated with the SCF's own step and iteration counts."""
-    import ctypes
+def cpu_share():
+    """The host threads this process may use: the OpenMP setting the GPU box exports (its CPU share per GPU), else the
+    CPU affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
 
+
+def cpu_baseline(state, n, h, pos):
+    """The oracle (the C f64 restatement of the reference's single-threaded CPU solver, crates/qm) on this host: a
+    bounded sample of the same water problem, one f64 Poisson solve of the converged density to the SCF's Poisson
+    tolerance and LOBPCG at the oracle's own block size on the converged potential (setup alone, then two iterations),
+    timed on 1 thread (the reference's CPU solver) and on the host's CPU share, and scaled to one whole SCF with the
+    oracle's own counts on this workload (tests/golden/oracle_water95_counts.json: its steps, LOBPCG iterations and
+    Poisson CG iterations, from one full oracle solve)."""
     import oracle as O
-    os.environ["OMP_NUM_THREADS"] = str(threads)
+    with open(os.path.join(ROOT, "tests", "golden", "oracle_water95_counts.json")) as fh:
+        counts = json.load(fh)
     rho, vh, vnuc = state
-    t0 = time.perf_counter()
-    _, it_p, _ = O.poisson_solve(n, h, rho, v0=vh * 0.0, tol=1e-10, max_iter=5000)
-    t_poisson = time.perf_counter() - t0
     v = vnuc + vh
-    k = gpu_fp64["k"]
+    k = int(counts["k_up"])
     guess = O.initial_orbitals(n, h, pos, k, 1)
-    t0 = time.perf_counter()
-    O.lobpcg(n, h, v, guess, 1e-30, 0, precond_lambda=-1.0)
-    t_setup = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    O.lobpcg(n, h, v, guess, 1e-30, 2, precond_lambda=-1.0)
-    t_two = time.perf_counter() - t0
-    t_iter = max(t_two - t_setup, 0.0) / 2
-    steps = gpu_fp64["steps"]
-    iters = gpu_fp64["lobpcg_iterations"] * 2  # the CPU solver solves both spin channels
-    est = steps * (t_poisson + 2 * t_setup) + iters * t_iter
-    return {"value": est, "unit": "s", "cores": threads, "kind": "port",
-            "sample": f"oracle (C f64 restatement of crates/qm) on the same water 95^3 problem: one Poisson solve "
-                      f"({it_p} CG it, {t_poisson:.2f} s) and 2 LOBPCG iterations of k={k} ({t_iter:.2f} s/it, "
-                      f"{t_setup:.2f} s setup), extrapolated to {steps} SCF steps x 2 spins and {iters} LOBPCG "
-                      f"iterations (the FP64 GPU run's counts)",
-            "measured_seconds": t_poisson + t_setup + t_two}
+    runs = {}
+    share = cpu_share()
+    for label, threads in (("threads_1", 1), ("threads_all", share)):
+        O.set_threads(threads)
+        t0 = time.perf_counter()
+        _, it_p, _ = O.poisson_solve(n, h, rho, tol=1e-10, max_iter=5000)
+        t_poisson = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        O.lobpcg(n, h, v, guess, 1e-30, 0, precond_lambda=-1.0)
+        t_setup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        O.lobpcg(n, h, v, guess, 1e-30, 2, precond_lambda=-1.0)
+        t_iter = max(time.perf_counter() - t0 - t_setup, 0.0) / 2
+        per_cg = t_poisson / max(it_p, 1)
+        calls = 2 * counts["steps"]  # one eigensolve per spin channel and step
+        est = per_cg * counts["cg_iterations"] + t_iter * counts["lobpcg_iterations"] + t_setup * calls
+        runs[label] = {"seconds_per_scf": est, "threads": threads, "poisson_seconds_per_cg_iteration": per_cg,
+                       "lobpcg_seconds_per_iteration": t_iter, "lobpcg_setup_seconds": t_setup,
+                       "sample_seconds": t_poisson + t_setup * 2 + t_iter * 2}
+        print(f"# cpu baseline {label} ({threads} threads): {est:.1f} s per SCF (Poisson {per_cg * 1e3:.1f} ms/CG it, "
+              f"LOBPCG {t_iter:.2f} s/it)", file=sys.stderr, flush=True)
+    O.set_threads(share)
+    top = runs["threads_all"]
+    return {"value": top["seconds_per_scf"], "unit": "s", "cores": share, "kind": "port",
+            "sample": f"oracle (C f64 restatement of crates/qm) on the same water 95^3 problem: one Poisson solve and "
+                      f"LOBPCG setup + 2 iterations at k = {k}, timed on 1 and {share} threads, scaled to the oracle's "
+                      f"own full solve ({counts['steps']} SCF steps, {counts['lobpcg_iterations']} LOBPCG iterations, "
+                      f"{counts['cg_iterations']} CG iterations)",
+            "threads_1": runs["threads_1"], "threads_all": runs["threads_all"], "cpu_share": share,
+            "nproc": os.cpu_count(), "oracle_counts": counts}
 
 
 def main():
@@ -454,8 +469,7 @@ def main():
         torch.cuda.empty_cache()
         result["aspirin_synthetic"] = aspirin_leg(lira)
     if not args.no_cpu and gpu_fp64 is not None:
-        threads = min(16, os.cpu_count() or 1)
-        result["cpu_baseline"] = cpu_baseline_sample(state, gpu_fp64, n, h, pos, z, threads)
+        result["cpu_baseline"] = cpu_baseline(state, n, h, pos)
     print(json.dumps(result))
 
 

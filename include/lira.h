@@ -256,23 +256,79 @@ lira_status lira_sample_density(lira_solver *s, const double *points, int count,
  * integral of |rho_up - rho_dn| (electrons; the provider's spin guard) and the maximum of rho_up + rho_dn (1/Bohr^3). */
 lira_status lira_state_summary(lira_solver *s, double *spin_abs_integral, double *density_max);
 
+/* ---- display products of the converged state (crates/qm-gpu/src/orbitals.rs, harris.rs) ---- */
+#define LIRA_UNOCC_MAX 34
+typedef struct {
+    int count[2];                               /* m + 2 guard levels per spin (ascending) */
+    double evals[2][LIRA_UNOCC_MAX];            /* re-measured Rayleigh quotients (Ha) */
+    double residuals[2][LIRA_UNOCC_MAX];        /* ||H phi - eps phi|| / ||phi|| */
+    double occupied_error[2];                   /* max |display - SCF| over the occupied levels */
+    int iterations[2];                          /* LOBPCG iterations per spin */
+} lira_unoccupied_result;
+/* OrbitalView::unoccupied (crates/qm-gpu/src/orbitals.rs:997-1148): the m lowest unoccupied levels (plus 2 guard
+ * levels) of each spin channel of the converged state, in the state's own output potentials, from one LOBPCG of the
+ * shown occupied columns + m + 2 at the SCF's effective eigen tolerance; the unoccupied vectors are projected onto the
+ * complement of the SCF's occupied orbitals, re-measured and sorted, and kept as the next call's guess. orbitals (may
+ * be NULL): 2 (m + 2) n^3 f64, spin-major, band-major. LIRA_NUMERICAL when the display's occupied levels disagree
+ * with the SCF's by more than tau = max(100 tol, 1e-4) Ha (the state is kept); 0 <= m <= 32. */
+lira_status lira_unoccupied(lira_solver *s, int m, lira_unoccupied_result *out, double *orbitals);
+
+#define LIRA_HARRIS_MAX_LEVELS 96
+typedef struct {
+    int n_levels[2];    /* tested levels kept per spin (ascending): occupied, then unoccupied */
+    int n_occ[2];       /* levels with f > 0 (Fermi-Dirac: f >= 1e-6) */
+    double evals[2][LIRA_HARRIS_MAX_LEVELS], occ[2][LIRA_HARRIS_MAX_LEVELS];
+    double electrons[2]; /* integral of rho_out per spin */
+    int iterations[2];  /* LOBPCG iterations per spin */
+    int window_short[2]; /* Fermi-Dirac: no witness level within the block at the witness cap */
+    int cold;           /* some channel started from fresh vectors */
+    double seconds;
+} lira_harris_result;
+/* GpuSolver::harris (crates/qm-gpu/src/harris.rs:477): one non-self-consistent Kohn-Sham diagonalisation in the
+ * potential of the input density rho_in (host f64, n^3 per spin) at the geometry of sys under the solver's ionic setup
+ * (V = v_nuc + V_H[rho_in] + v_xc[rho_in (+ rho_pc/2)]), for display: per spin the lowest n_sigma + m + 2 levels at the
+ * SCF's effective eigen tolerance (an untested buffer column above them), the occupations of settings applied once
+ * (Fermi-Dirac grows the block while no witness level is within it), rho_out = sum f psi^2 (host f64, may be NULL). No
+ * energy, no forces, no SCF loop. The previous call's vectors and V_H start the next; the converged SCF state is left
+ * as it was. */
+lira_status lira_harris(lira_solver *s, const lira_system *sys, const lira_settings *settings, const double *rho_in_up,
+                        const double *rho_in_dn, int m, lira_harris_result *out, double *rho_out_up,
+                        double *rho_out_dn);
+
 /* ---- the provider's evaluation around GpuSolver (providers::ks::GpuKs::evaluate_once) ---- */
 /* max(tol, 2 * 5e-8 * n_valence): the SCF tolerance the GPU's f32 density can reach (effective_gpu_tol,
  * crates/providers/src/pseudo.rs:423-440). Host only. */
 double lira_effective_gpu_tol(double tol, int n_valence);
 typedef enum {
-    LIRA_EVAL_TOL_RAISED = 1,    /* ionic setup: tol raised to lira_effective_gpu_tol */
-    LIRA_EVAL_RETRIED_COLD = 2,  /* the warm attempt failed and was retried cold (R15) */
-    LIRA_EVAL_WARM_STALLED = 4,  /* ... and that warm attempt had stalled (else it ran out of steps) */
-    LIRA_EVAL_EXTRAPOLATED = 8   /* the converged solve started from an extrapolated density */
+    LIRA_EVAL_TOL_RAISED = 1,      /* ionic setup: tol raised to lira_effective_gpu_tol */
+    LIRA_EVAL_RETRIED_COLD = 2,    /* the warm attempt failed and was retried cold (R15) */
+    LIRA_EVAL_WARM_STALLED = 4,    /* ... and that warm attempt had stalled (else it ran out of steps) */
+    LIRA_EVAL_EXTRAPOLATED = 8,    /* the converged solve started from an extrapolated density */
+    LIRA_EVAL_DEVICE_RETRIED = 16, /* a device failure was retried after a pause ("GPU busy") */
+    LIRA_EVAL_BUDGET_CAP = 32      /* the witness cap came from the device memory budget */
 } lira_eval_flag;
-/* One evaluation as GpuKs::evaluate_once drives the solver (crates/providers/src/ks.rs:1573-1642): under an ionic
- * setup the tolerance is raised to the GPU floor (raise_to_gpu_floor, ks.rs:543-558); a warm solve that ends
- * LIRA_NOT_CONVERGED or LIRA_STALLED is retried once cold with density extrapolation off for that solve
- * (retry_cold_on_stall, ks.rs:585-601), the failed attempt's steps and readbacks added to *stats; then the forces
- * (3 n doubles). Returns the last attempt's status; *flags (lira_eval_flag) and *tol_used are set in every case. */
+/* The device memory (bytes) one solve holds on an n^3 grid with n_up / n_dn electrons, at most `witness` witness vectors
+ * per spin and a DIIS history of diis_history (the provider's footprint model, crates/providers/src/ks.rs:360-400,
+ * for this library's buffers). 0 for bad arguments. Host only. */
+unsigned long long lira_solve_footprint_bytes(int n, int precision, int n_up, int n_dn, int witness,
+                                              int diis_history);
+/* witness_cap_for (crates/providers/src/ks.rs:400-428): the largest witness cap w <= 32 whose solve footprint fits
+ * budget bytes, never below 3 (the witness vectors a solve starts with). Host only. */
+int lira_witness_cap_for_budget(int n, int precision, int n_up, int n_dn, int diis_history,
+                                unsigned long long budget);
+/* One evaluation as the reference's provider drives the solver (GpuKs::evaluate / evaluate_once,
+ * crates/providers/src/ks.rs:1573-1706): with vram_budget > 0 the witness cap is set from the budget
+ * (lira_witness_cap_for_budget); under an ionic setup the tolerance is raised to the GPU floor (raise_to_gpu_floor,
+ * ks.rs:543-558); the solve runs through lira_solve_polled with abort before each step, inner_abort per Fermi-Dirac
+ * LOBPCG iteration and observe per step; a warm solve that ends LIRA_NOT_CONVERGED or LIRA_STALLED is retried once cold
+ * with density extrapolation off for that solve (retry_cold_on_stall, ks.rs:585-601), the failed attempt's steps and
+ * readbacks added to *stats; then the forces (3 n doubles). A LIRA_DEVICE failure anywhere in the attempt is retried
+ * after 100 ms, then 400 ms (retry_on_device_fault, ks.rs:317-350), abort polled every 20 ms meanwhile; a witness
+ * search stopped at a budget's cap says so in lira_last_error. Returns the last attempt's status; *flags
+ * (lira_eval_flag) and *tol_used are set in every case. */
 lira_status lira_evaluate(lira_solver *s, const lira_system *sys, const lira_settings *settings, int warm,
-                          lira_abort_fn abort, void *user, lira_scf_stats *stats, double *forces, int *flags,
+                          lira_abort_fn abort, lira_abort_fn inner_abort, lira_observe_fn observe, void *user,
+                          unsigned long long vram_budget, lira_scf_stats *stats, double *forces, int *flags,
                           double *tol_used);
 
 /* ---- batched SCF (no counterpart in the reference, which evaluates a set of molecules one at a time:
@@ -338,6 +394,20 @@ lira_status lira_inject_readback_fault(lira_solver *s, int after);
  * (l <= 6); rho and v are host f64 arrays (v: initial guess in, solution out). */
 lira_status lira_poisson(lira_solver *s, const double *rho, double *v, double tol, int max_iter,
                          lira_poisson_stats *stats);
+
+/* Multigrid::apply_in (crates/qm-gpu/src/multigrid.rs:107-197): z = M r, one V-cycle of the Poisson solve's
+ * preconditioner (V(2,2), omega = 6/7, odd-index coarsening with boundary-distance ghosts, 12 coarsest sweeps) on the
+ * solver's grid and in its precision; r and z are host f64 arrays (n^3). */
+lira_status lira_mg_vcycle(lira_solver *s, const double *r, double *z);
+
+/* One level operator of the V-cycle on device pointers (crates/qm-gpu/src/shaders/multigrid.wgsl:30-126). A2 is the
+ * 7-point -lap at spacing h whose missing neighbours are ghosts (1 - 1/d) x0 at boundary distance d_low / d_high (1 on
+ * the finest level); D its diagonal. LIRA_MG_JACOBI: out = x + (6/7) D^-1 (b - A2 x); LIRA_MG_RESIDUAL: out = b - A2 x;
+ * LIRA_MG_RESTRICT: out ((n/2)^3) = full weighting of x (n^3) about the odd fine points; LIRA_MG_PROLONG_ADD: out
+ * (n^3) += trilinear interpolation of x ((n/2)^3). Waits for the stream before returning. */
+enum { LIRA_MG_JACOBI = 0, LIRA_MG_RESIDUAL = 1, LIRA_MG_RESTRICT = 2, LIRA_MG_PROLONG_ADD = 3 };
+lira_status lira_mg_level_op(int dtype, int op, int n, double h, double d_low, double d_high, const void *x,
+                             const void *b, void *out, void *stream);
 
 /* lobpcg (crates/qm-gpu/src/eigen.rs:595-653) for H = -1/2 lap + V with the damped-Jacobi preconditioner shifted by
  * precond_lambda: V (n^3), x (k * n^3, guess in, orthonormal eigenvectors out), evals (k). Host f64 arrays. */

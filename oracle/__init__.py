@@ -35,7 +35,8 @@ class ScfStats(C.Structure):
                 ("boundary_density", C.c_double), ("max_eigen_residual", C.c_double),
                 ("poisson_residual", C.c_double), ("poisson_iterations", C.c_int), ("entropy", C.c_double),
                 ("mu", C.c_double * 2), ("electrons", C.c_double * 2), ("k_up", C.c_int), ("k_dn", C.c_int),
-                ("seconds_poisson", C.c_double), ("seconds_eigen", C.c_double), ("seconds_total", C.c_double)]
+                ("seconds_poisson", C.c_double), ("seconds_eigen", C.c_double), ("seconds_total", C.c_double),
+                ("lobpcg_iterations", C.c_int), ("cg_iterations", C.c_int)]
 
 
 class _Projector(C.Structure):
@@ -115,6 +116,9 @@ def lib():
         L.or_multipole_moments.argtypes = [C.c_int, C.c_double, _dp, _dp]
         L.or_multipole_potential.restype = C.c_double
         L.or_multipole_potential.argtypes = [_dp, _dp]
+        L.or_set_threads.restype = None
+        L.or_set_threads.argtypes = [C.c_int]
+        L.or_threads.restype = C.c_int
         L.or_poisson_rhs.restype = None
         L.or_poisson_rhs.argtypes = [C.c_int, C.c_double, _dp, _dp]
         L.or_poisson_solve.restype = C.c_int
@@ -264,6 +268,15 @@ def poisson_rhs(n, h, rho):
     out = np.empty(n ** 3)
     lib().or_poisson_rhs(n, h, _f64(rho), out)
     return out
+
+
+def set_threads(threads: int):
+    """OpenMP threads of the oracle's grid loops (1 = the reference's single-threaded solver)."""
+    lib().or_set_threads(int(threads))
+
+
+def threads() -> int:
+    return lib().or_threads()
 
 
 def poisson_solve(n, h, rho, v0=None, tol=1e-10, max_iter=5000):

@@ -18,6 +18,25 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* the OpenMP thread count of the grid loops (the reference's CPU solver is single-threaded: 1 is the faithful run) */
+void or_set_threads(int threads) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+}
+int or_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 static const double COEFF[5] = {-205.0 / 72.0, 8.0 / 5.0, -1.0 / 5.0, 8.0 / 315.0, -1.0 / 560.0};
 
@@ -549,6 +568,9 @@ static int orthonormalise(int n, double h, double *s, int cnt) {
     return kept;
 }
 
+/* LOBPCG iterations run by this process (the SCF reports its own share) */
+static long or_lobpcg_iterations_total = 0;
+
 int or_lobpcg(int n, double h, const double *v, int k, double *x, double *evals, double precond_lambda, double tol,
               int max_iter, double *max_residual, double *residuals) {
     return or_lobpcg_nl(n, h, v, NULL, k, x, evals, precond_lambda, tol, max_iter, max_residual, residuals);
@@ -627,6 +649,7 @@ int or_lobpcg_nl(int n, double h, const double *v, const or_nl *nl, int k, doubl
     memcpy(x, xs, (size_t)(k * len) * sizeof(double));
     if (max_residual) *max_residual = max_res;
     free(xs); free(p); free(hx); free(w); free(r); free(s); free(hs); free(nx); free(np);
+    or_lobpcg_iterations_total += iterations;
     return iterations;
 }
 
@@ -989,6 +1012,7 @@ int or_scf_ionic(int n, double h, int nat, const double *pos, const int *z, int 
                  double *orbitals_dn_out) {
     double t_start = now_s();
     memset(st, 0, sizeof *st);
+    const long lobpcg_before = or_lobpcg_iterations_total;
     long len = (long)n * n * n;
     double e_nn;
     int ci, cj;
@@ -1086,6 +1110,7 @@ int or_scf_ionic(int n, double h, int nat, const double *pos, const int *z, int 
         t_poisson += now_s() - tp;
         if (pres > st->poisson_residual) st->poisson_residual = pres;
         if (pit > st->poisson_iterations) st->poisson_iterations = pit;
+        st->cg_iterations += pit > 0 ? pit : 0;
         xc_with_core(len, rin_up, rin_dn, pc, xe, xu, xd);
         for (long i = 0; i < len; i++) {
             vu[i] = v_nuc[i] + v_h[i] + xu[i];
@@ -1267,6 +1292,7 @@ int or_scf_ionic(int n, double h, int nat, const double *pos, const int *z, int 
     st->seconds_poisson = t_poisson;
     st->seconds_eigen = t_eigen;
     st->seconds_total = now_s() - t_start;
+    st->lobpcg_iterations = (int)(or_lobpcg_iterations_total - lobpcg_before);
     for (int q = 0; q < hlen; q++) free_entry(&hist[q]);
     free(hist);
     free(c);

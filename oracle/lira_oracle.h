@@ -47,6 +47,9 @@ double or_multipole_potential(const double q[OR_N_MOMENTS], const double r[3]);
 void or_poisson_rhs(int n, double h, const double *rho, double *b);
 /* returns iterations; *residual = ||b - A v|| / ||b|| at exit */
 int or_poisson_solve(int n, double h, const double *rho, double *v, double tol, int max_iter, double *residual);
+/* OpenMP threads of the grid loops (set / query) */
+void or_set_threads(int threads);
+int or_threads(void);
 
 /* nuclei: qm/src/nuclear.rs:106-289 */
 void or_deposit_charge(int n, double h, int nat, const double *pos, const int *z, double sigma, double *out);
@@ -92,6 +95,7 @@ typedef struct {
     double entropy, mu[2], electrons[2];
     int k_up, k_dn; /* stored orbitals per spin */
     double seconds_poisson, seconds_eigen, seconds_total;
+    int lobpcg_iterations, cg_iterations; /* totals over the solve (both spins; every Poisson solve of the loop) */
 } or_scf_stats;
 
 /* Runs the SCF. evals_up/evals_dn/occ_up/occ_dn have room for `cap` entries; rho_up_out/rho_dn_out (n^3, may be

@@ -66,3 +66,27 @@ def test_batch_takes_per_molecule_settings(lira_mod):
         lone = s.solve(m.pos, m.z, m.n_up, m.n_dn, st_)
         assert o.stats.energy.total == lone.energy.total and o.stats.iterations == lone.iterations
         assert o.stats.delta_rho < st_.tol and o.stats.energy.ts == 0.0
+
+
+def test_batch_workload_grids_at_h_025(lira_mod):
+    """The batched workload's own molecule set at its own spacing and grids (H2, LiH, H2O, NH3, CH4 with +-0.1 A
+    seeded displacements, n from 48 to 96 at h = 0.25 Bohr, MIXED, 4 workers): every molecule converges, each result is
+    its lone solve's bit for bit, each channel holds its electron count, and the net force (zero for the exact
+    functional, broken only by the grid's egg-box effect) is small against the largest component."""
+    lira = lira_mod
+    import molecules as M
+    mols = M.sample(10, seed=21, h=0.25)
+    names = {m.name for m in mols}
+    assert names == set(M.SPECIES) and min(m.n for m in mols) >= 48 and max(m.n for m in mols) <= 96
+    print("batch:", [(m.name, m.n) for m in mols], flush=True)
+    out = lira.solve_batch(mols, precision=lira.MIXED, workers=4)
+    for m, o in zip(mols, out):
+        assert o.status == lira.OK, (m.name, o.message)
+        st, f = _lone(lira, m, lira.MIXED)
+        assert o.stats.energy.total == st.energy.total and o.stats.iterations == st.iterations, m.name
+        assert np.array_equal(o.forces, f), m.name
+        assert abs(o.stats.electrons[0] - m.n_up) < 1e-5 and abs(o.stats.electrons[1] - m.n_dn) < 1e-5
+        net = np.abs(o.forces.sum(axis=0)).max()
+        print(f"{m.name} {m.n}^3: E {o.stats.energy.total:.8f}, {o.stats.iterations} steps, net force {net:.1e}, "
+              f"max {np.abs(o.forces).max():.2e}")
+        assert net < 5e-2 * np.abs(o.forces).max() + 1e-3

@@ -207,3 +207,29 @@ def test_pseudo_forces_are_minus_the_energy_gradient(lira_mod):
     rich = (4 * fd1 - fd2) / 3
     print(f"pseudised O: HF {f0[0, 1]:+.8f} vs Richardson {rich:+.8f}")
     assert abs(f0[0, 1] - rich) < 1e-4
+
+
+def test_pseudised_methanol_matches_the_oracle_fixture(lira_mod):
+    """The reduced pseudised organic of the adjoint-gradient workload: methanol with C and O pseudised (C-like and
+    O-like tables) and four all-electron H, 7 + 7 valence electrons, tight FP64 on 33^3 x 0.3 against the oracle's
+    solve of the same problem (tests/golden/pseudo_methanol_33.json, written by make_pseudo_methanol.py): energy and
+    every term, the levels, the five force terms and the forces."""
+    import json
+    import os
+    lira = lira_mod
+    with open(os.path.join(os.path.dirname(__file__), "golden", "pseudo_methanol_33.json")) as fh:
+        ref = json.load(fh)
+    n, h = ref["n"], ref["h"]
+    pos, z = np.array(ref["pos"]), ref["z"]
+    s = lira.GpuSolver(n, h, 0)
+    s.set_ionic([T.carbon_like(h), T.oxygen_like(h)])
+    st = s.solve(pos, z, ref["n_up"], ref["n_dn"], lira.default_settings(h, 0, tol=1e-9, eig_tol=1e-9))
+    print(f"pseudised methanol: GPU E {st.energy.total:.12f} ({st.iterations} steps) oracle "
+          f"{ref['energy']['total']:.12f} ({ref['steps']} steps)")
+    assert abs(st.energy.total - ref["energy"]["total"]) < 1e-10
+    for name in ("kinetic", "external", "hartree", "xc", "nuclear"):
+        assert abs(getattr(st.energy, name) - ref["energy"][name]) < 1e-8, name
+    assert np.abs(s.state().evals_up - np.array(ref["evals_up"])).max() < 1e-8
+    terms = s.force_terms(pos, z, h)
+    assert np.abs(terms - np.array(ref["force_terms"])).max() < 1e-8
+    assert np.abs(s.forces(pos, z, h) - np.array(ref["forces"])).max() < 1e-8

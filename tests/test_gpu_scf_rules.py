@@ -234,3 +234,30 @@ def test_solvers_on_two_threads_with_default_settings(lira_mod):
     for t in th:
         t.join()
     assert got == lone
+
+
+def test_evaluation_repeats_a_device_failure(lira_mod):
+    """The provider's "GPU busy" policy: an evaluation whose readback fails is repeated after a pause and returns the
+    clean evaluation's answer, flagged (crates/providers/src/ks.rs:317-350, 1688-1706)."""
+    lira = lira_mod
+    n, h = 25, 0.3
+    st = lira.default_settings(h, lira.FERMI_DIRAC)
+    clean = lira.GpuSolver(n, h, lira.MIXED).evaluate(H2[0], H2[1], 1, 1, st)
+    s = lira.GpuSolver(n, h, lira.MIXED)
+    s.inject_readback_fault(4)
+    status, stats, f, flags, _ = s.evaluate(H2[0], H2[1], 1, 1, st)
+    assert status == lira.OK and flags & lira.EVAL_DEVICE_RETRIED
+    assert stats.energy.total == clean[1].energy.total and np.array_equal(f, clean[2])
+
+
+def test_evaluation_takes_its_witness_cap_from_the_budget(lira_mod):
+    """A device memory budget sets the witness cap to the largest that fits (never below 3): the solve's block stays
+    within it and the evaluation says where its cap came from."""
+    lira = lira_mod
+    n, h = 33, 0.25
+    budget = lira.solve_footprint_bytes(n, lira.MIXED, 1, 1, 5) + 1
+    s = lira.GpuSolver(n, h, lira.MIXED)
+    status, stats, _, flags, _ = s.evaluate(H2[0], H2[1], 1, 1, lira.default_settings(h, lira.FERMI_DIRAC),
+                                            vram_budget=budget)
+    assert status == lira.OK and flags & lira.EVAL_BUDGET_CAP
+    assert s.witness_cap() == 5 and max(stats.n_stored) <= 1 + 5
