@@ -56,57 +56,221 @@ __device__ inline void solid_harmonics(double x, double y, double z, double out[
     }
 }
 
-__device__ inline double block_sum256p(double v, double *sh) {
-    sh[threadIdx.x] = v;
-    __syncthreads();
+// ------------------------------------------------------------------------------------ multipole moments ----
+// Q_lm = h^3 sum_p rho_p R_lm(r_p) (qm/src/poisson.rs:155-162). R_lm is a homogeneous polynomial of degree l in
+// (x, y, z), so Q_lm = h^3 sum_{a+b+c=l} C_lm,abc M_abc with the raw moments M_abc = sum_p rho_p x^a y^b z^c
+// (a + b + c <= 6, 84 of them), and those separate over the grid's axes: rows S_a(j,k) = sum_i x_i^a rho(i,j,k),
+// planes T_ab(k) = sum_j y_j^b S_a(j,k), then M_abc = sum_k z_k^c T_ab(k). One streaming pass over rho with seven
+// f64 accumulations per point, two small contractions, and the C table built on the host by running the recursion
+// of solid_harmonics on polynomials (moment_coefficients below). Monomials are enumerated a outer, b, c inner.
+constexpr int N_PAIRS = 28;      // (a, b) with a + b <= 6
+constexpr int N_MONOMIALS = 84;  // (a, b, c) with a + b + c <= 6
+
+__device__ inline double wave_sum(double v) {
 #pragma unroll
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
-    const double r = sh[0];
-    __syncthreads();
-    return r;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
 }
+
+// one wave per row (j, k): rows[a * n^2 + j + n k] = S_a(j, k)
+template <typename T> __global__ __launch_bounds__(256) void k_moment_rows(const T *rho, int n, double h, double *rows) {
+    const int lane = threadIdx.x & 63;
+    const long nn = (long)n * n;
+    const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= nn) return;  // a whole wave
+    const double o = -((double)n - 1.0) * h / 2.0;
+    const T *src = rho + row * n;
+    double s[LMAX + 1];
+#pragma unroll
+    for (int a = 0; a <= LMAX; a++) s[a] = 0.0;
+#pragma unroll 4
+    for (int i = lane; i < n; i += 64) {
+        const double x = o + (double)i * h;
+        double t = (double)src[i];
+#pragma unroll
+        for (int a = 0; a <= LMAX; a++) {
+            s[a] += t;
+            t *= x;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a <= LMAX; a++) {
+        const double v = wave_sum(s[a]);
+        if (lane == 0) rows[a * nn + row] = v;
+    }
+}
+
+// one block per plane k: planes[pair * n + k] = T_ab(k)
+__global__ __launch_bounds__(256) void k_moment_planes(const double *rows, int n, double h, double *planes) {
+    __shared__ double sh[4][N_PAIRS];
+    const long nn = (long)n * n;
+    const int k = blockIdx.x;
+    const double o = -((double)n - 1.0) * h / 2.0;
+    double t[N_PAIRS];
+#pragma unroll
+    for (int q = 0; q < N_PAIRS; q++) t[q] = 0.0;
+    for (int j = threadIdx.x; j < n; j += 256) {
+        const double y = o + (double)j * h;
+        double yp[LMAX + 1];
+        yp[0] = 1.0;
+#pragma unroll
+        for (int b = 1; b <= LMAX; b++) yp[b] = yp[b - 1] * y;
+        int q = 0;
+#pragma unroll
+        for (int a = 0; a <= LMAX; a++) {
+            const double sa = rows[a * nn + j + (long)n * k];
+#pragma unroll
+            for (int b = 0; b <= LMAX - a; b++) t[q++] += sa * yp[b];
+        }
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < N_PAIRS; q++) {
+        const double v = wave_sum(t[q]);
+        if (lane == 0) sh[w][q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < N_PAIRS) {
+        const int q = threadIdx.x;
+        planes[(long)q * n + k] = (sh[0][q] + sh[1][q]) + (sh[2][q] + sh[3][q]);
+    }
+}
+
+// one block: M_abc = sum_k z_k^c T_ab(k) (monomial t on wave t % 4, its k spread over the lanes), then
+// q[lm] = h^3 sum_t coef[lm * 84 + t] M_t
+__global__ __launch_bounds__(256) void k_moment_finish(const double *planes, int n, double h, const double *coef,
+                                                       double *q) {
+    __shared__ double mono[N_MONOMIALS];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const double o = -((double)n - 1.0) * h / 2.0;
+    int t = 0, pair = 0;
+    for (int a = 0; a <= LMAX; a++)
+        for (int b = 0; b <= LMAX - a; b++, pair++)
+            for (int c = 0; c <= LMAX - a - b; c++, t++) {
+                if ((t & 3) != w) continue;
+                double m = 0.0;
+                for (int k = lane; k < n; k += 64) {
+                    const double z = o + (double)k * h;
+                    double zc = 1.0;
+                    for (int e = 0; e < c; e++) zc *= z;
+                    m += planes[(long)pair * n + k] * zc;
+                }
+                m = wave_sum(m);
+                if (lane == 0) mono[t] = m;
+            }
+    __syncthreads();
+    if (threadIdx.x < NMOM) {
+        const int lm = threadIdx.x;
+        double acc = 0.0;
+        for (int u = 0; u < N_MONOMIALS; u++) acc += coef[lm * N_MONOMIALS + u] * mono[u];
+        q[lm] = acc * (h * h * h);
+    }
+}
+
+// The monomial coefficients of R_lm: solid_harmonics' recursion run on polynomials in (x, y, z) (dense over
+// exponents 0..6 per axis), then packed as coef[lm * 84 + t] in the monomial order above.
+void moment_coefficients(double *coef) {
+    constexpr int E = LMAX + 1;
+    struct Poly {
+        double c[E][E][E];
+    };
+    auto zero = [](Poly &p) {
+        for (auto &a : p.c)
+            for (auto &b : a)
+                for (double &v : b) v = 0.0;
+    };
+    // dst = s1 * (axis-monomial u times p1) + s2 * (axis-monomial w times p2), u/w one of x, y, z, r^2 (0..3) or none
+    auto times = [](const Poly &p, int axis, Poly &out) {  // out = m * p, m = x, y, z (axis 0..2) or r^2 (axis 3)
+        for (auto &a : out.c)
+            for (auto &b : a)
+                for (double &v : b) v = 0.0;
+        for (int a = 0; a < E; a++)
+            for (int b = 0; b < E; b++)
+                for (int c = 0; c < E; c++) {
+                    const double v = p.c[a][b][c];
+                    if (v == 0.0) continue;
+                    if (axis == 0 && a + 1 < E) out.c[a + 1][b][c] += v;
+                    if (axis == 1 && b + 1 < E) out.c[a][b + 1][c] += v;
+                    if (axis == 2 && c + 1 < E) out.c[a][b][c + 1] += v;
+                    if (axis == 3) {
+                        if (a + 2 < E) out.c[a + 2][b][c] += v;
+                        if (b + 2 < E) out.c[a][b + 2][c] += v;
+                        if (c + 2 < E) out.c[a][b][c + 2] += v;
+                    }
+                }
+    };
+    static Poly cr[E][E], ci[E][E];
+    for (int l = 0; l < E; l++)
+        for (int m = 0; m < E; m++) {
+            zero(cr[l][m]);
+            zero(ci[l][m]);
+        }
+    cr[0][0].c[0][0][0] = 1.0;
+    Poly t1, t2;
+    for (int l = 1; l <= LMAX; l++) {
+        const double lf = (double)l;
+        const double f = std::sqrt((2.0 * lf - 1.0) / (2.0 * lf));
+        // C_l^l = f (x + i y) C_{l-1}^{l-1}
+        times(cr[l - 1][l - 1], 0, t1);
+        times(ci[l - 1][l - 1], 1, t2);
+        for (int a = 0; a < E; a++)
+            for (int b = 0; b < E; b++)
+                for (int c = 0; c < E; c++) cr[l][l].c[a][b][c] = f * (t1.c[a][b][c] - t2.c[a][b][c]);
+        times(ci[l - 1][l - 1], 0, t1);
+        times(cr[l - 1][l - 1], 1, t2);
+        for (int a = 0; a < E; a++)
+            for (int b = 0; b < E; b++)
+                for (int c = 0; c < E; c++) ci[l][l].c[a][b][c] = f * (t1.c[a][b][c] + t2.c[a][b][c]);
+        // C_l^m = ((2l - 1) z C_{l-1}^m - sqrt((l-1)^2 - m^2) r^2 C_{l-2}^m) / sqrt(l^2 - m^2)
+        for (int m = 0; m < l; m++) {
+            const double mf = (double)m;
+            const double tt = (lf - 1.0) * (lf - 1.0) - mf * mf;
+            const double bq = std::sqrt(tt > 0.0 ? tt : 0.0);
+            const double d = std::sqrt(lf * lf - mf * mf);
+            for (int part = 0; part < 2; part++) {
+                Poly(&C)[E][E] = part == 0 ? cr : ci;
+                times(C[l - 1][m], 2, t1);
+                zero(t2);
+                if (l >= 2 && m <= l - 2) times(C[l - 2][m], 3, t2);
+                for (int a = 0; a < E; a++)
+                    for (int b = 0; b < E; b++)
+                        for (int c = 0; c < E; c++)
+                            C[l][m].c[a][b][c] = ((2.0 * lf - 1.0) * t1.c[a][b][c] - bq * t2.c[a][b][c]) / d;
+            }
+        }
+    }
+    const double s2 = std::sqrt(2.0);
+    for (int i = 0; i < NMOM * N_MONOMIALS; i++) coef[i] = 0.0;
+    auto pack = [&](int lm, const Poly &p, double scale) {
+        int idx = 0;
+        for (int a = 0; a <= LMAX; a++)
+            for (int b = 0; b <= LMAX - a; b++)
+                for (int c = 0; c <= LMAX - a - b; c++, idx++) coef[lm * N_MONOMIALS + idx] = scale * p.c[a][b][c];
+    };
+    for (int l = 0; l <= LMAX; l++) {
+        pack(l * l + l, cr[l][0], 1.0);
+        for (int m = 1; m <= l; m++) {
+            pack(l * l + l + m, cr[l][m], s2);
+            pack(l * l + l - m, ci[l][m], s2);
+        }
+    }
+}
+
+size_t moment_work_doubles(int n) { return (size_t)(LMAX + 1) * n * n + (size_t)N_PAIRS * n; }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_moments(const T *rho, int n, double h, double *partials) {
-    __shared__ double sh[256];
-    const long N = (long)n * n * n, nn = (long)n * n;
-    const double o = -((double)n - 1.0) * h / 2.0;
-    double acc[NMOM];
-#pragma unroll
-    for (int q = 0; q < NMOM; q++) acc[q] = 0.0;
-    for (long p = blockIdx.x * 256L + threadIdx.x; p < N; p += (long)gridDim.x * 256) {
-        const double rv = (double)rho[p];
-        if (rv == 0.0) continue;
-        const long k = p / nn, rem = p - k * nn, j = rem / n, i = rem - j * n;
-        double hv[NMOM];
-        solid_harmonics(o + (double)i * h, o + (double)j * h, o + (double)k * h, hv);
-#pragma unroll
-        for (int q = 0; q < NMOM; q++) acc[q] += rv * hv[q];
-    }
-    for (int q = 0; q < NMOM; q++) {
-        const double r = block_sum256p(acc[q], sh);
-        if (threadIdx.x == 0) partials[(long)q * RED_BLOCKS + blockIdx.x] = r;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_moments_finish(const double *partials, double *q, double scale) {
-    __shared__ double sh[256];
-    double acc = 0.0;
-    for (int p = threadIdx.x; p < RED_BLOCKS; p += 256) acc += partials[(long)blockIdx.x * RED_BLOCKS + p];
-    const double r = block_sum256p(acc, sh);
-    if (threadIdx.x == 0) q[blockIdx.x] = r * scale;
-}
-
-template <typename T> void multipole_moments(const T *rho, int n, double h, double *partials, double *q, hipStream_t s) {
-    hipLaunchKernelGGL(k_moments<T>, dim3(RED_BLOCKS), dim3(256), 0, s, rho, n, h, partials);
+void multipole_moments(const T *rho, int n, double h, double *work, const double *coef, double *q, hipStream_t s) {
+    const long nn = (long)n * n;
+    double *rows = work, *planes = work + (LMAX + 1) * nn;
+    hipLaunchKernelGGL(k_moment_rows<T>, dim3((unsigned)((nn + 3) / 4)), dim3(256), 0, s, rho, n, h, rows);
     LIRA_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_moments_finish, dim3(NMOM), dim3(256), 0, s, partials, q, h * h * h);
+    hipLaunchKernelGGL(k_moment_planes, dim3((unsigned)n), dim3(256), 0, s, rows, n, h, planes);
+    LIRA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_moment_finish, dim3(1), dim3(256), 0, s, planes, n, h, coef, q);
     LIRA_HIP(hipGetLastError());
 }
 
+// ----------------------------------------------------------------------------------- boundary terms ----
 // V_bc(r) = sum_l (sum_m Q_lm R_lm(r)) / |r|^{2l+1}  (qm/src/poisson.rs:175-190)
 __device__ inline double multipole_potential(const double *q, double x, double y, double z) {
     const double d2 = x * x + y * y + z * z;
@@ -124,39 +288,65 @@ __device__ inline double multipole_potential(const double *q, double x, double y
     return acc;
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_poisson_rhs(double *b, const T *rho, const double *qg, int n, double h) {
+// The 8th-order Laplacian of an edge point reaches up to 4 points past a face; those ghost points carry V_bc. Each is
+// evaluated once, one thread per ghost point: ghost[(slab * n^2 + u + n v)], slab = (axis * 2 + side) * 4 + (m - 1)
+// for the point m steps past the low (side 0) or high (side 1) face of that axis, (u, v) the indices along the other
+// two axes in increasing axis order.
+__global__ __launch_bounds__(256) void k_boundary_ghosts(double *ghost, const double *qg, int n, double h) {
     __shared__ double q[NMOM];
     if (threadIdx.x < NMOM) q[threadIdx.x] = qg[threadIdx.x];
     __syncthreads();
-    const long N = (long)n * n * n, nn = (long)n * n;
-    const long p = blockIdx.x * 256L + threadIdx.x;
-    if (p >= N) return;
+    const long nn = (long)n * n;
+    const long g = blockIdx.x * 256L + threadIdx.x;
+    if (g >= 24 * nn) return;
+    const int slab = (int)(g / nn);
+    const int uv = (int)(g - (long)slab * nn);
+    const int u = uv % n, v = uv / n;
+    const int axis = slab >> 3, side = (slab >> 2) & 1, m = (slab & 3) + 1;
+    const double o = -((double)n - 1.0) * h / 2.0;
+    int idx[3];
+    idx[axis] = side ? n - 1 + m : -m;
+    idx[axis == 0 ? 1 : 0] = u;
+    idx[axis == 2 ? 1 : 2] = v;
+    ghost[g] = multipole_potential(q, o + (double)idx[0] * h, o + (double)idx[1] * h, o + (double)idx[2] * h);
+}
+
+// b = 4 pi rho + (sum over an edge point's ghost neighbours of c_m V_bc) / h^2 (qm/src/poisson.rs:174-197,
+// 204-214): the terms in the order axis, step m = 1..4, below then above, as the oracle sums them.
+template <typename T>
+__global__ __launch_bounds__(256) void k_poisson_rhs(double *b, const T *rho, const double *ghost, int n, double h) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= n || j >= n) return;
+    const long nn = (long)n * n;
+    const long p = i + (long)n * j + nn * k;
     double val = 4.0 * M_PI * (double)rho[p];
-    const int k = (int)(p / nn), rem = (int)(p - (long)k * nn), j = rem / n, i = rem - j * n;
-    const int pos[3] = {i, j, k};
     const bool near = i < 4 || j < 4 || k < 4 || i >= n - 4 || j >= n - 4 || k >= n - 4;
     if (near) {
         const double c[5] = {0.0, 8.0 / 5.0, -1.0 / 5.0, 8.0 / 315.0, -1.0 / 560.0};
-        const double o = -((double)n - 1.0) * h / 2.0;
+        const int pos[3] = {i, j, k};
         double acc = 0.0;
-        for (int a = 0; a < 3; a++)
+        for (int a = 0; a < 3; a++) {
+            const int u = a == 0 ? j : i, v = a == 2 ? j : k;
             for (int m = 1; m <= 4; m++)
                 for (int sg = -1; sg <= 1; sg += 2) {
                     const int qa = pos[a] + sg * m;
                     if (qa >= 0 && qa < n) continue;
-                    double r[3] = {o + (double)i * h, o + (double)j * h, o + (double)k * h};
-                    r[a] += (double)(sg * m) * h;
-                    acc += c[m] * multipole_potential(q, r[0], r[1], r[2]);
+                    const int side = sg > 0, layer = side ? qa - (n - 1) : -qa;
+                    acc += c[m] * ghost[(long)((a * 2 + side) * 4 + layer - 1) * nn + u + (long)n * v];
                 }
+        }
         val += acc / (h * h);
     }
     b[p] = val;
 }
 
-template <typename T> void poisson_rhs(double *b, const T *rho, const double *q, int n, double h, hipStream_t s) {
-    const long N = (long)n * n * n;
-    hipLaunchKernelGGL(k_poisson_rhs<T>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, b, rho, q, n, h);
+template <typename T>
+void poisson_rhs(double *b, const T *rho, const double *q, double *ghost, int n, double h, hipStream_t s) {
+    const long nn = (long)n * n;
+    hipLaunchKernelGGL(k_boundary_ghosts, dim3((unsigned)((24 * nn + 255) / 256)), dim3(256), 0, s, ghost, q, n, h);
+    LIRA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_poisson_rhs<T>, dim3((unsigned)((n + 63) / 64), (unsigned)((n + 3) / 4), (unsigned)n),
+                       dim3(64, 4), 0, s, b, rho, ghost, n, h);
     LIRA_HIP(hipGetLastError());
 }
 
@@ -253,21 +443,57 @@ template <typename T> __device__ inline T mg_prolong_pt(const T *c, int i, int j
     return s;
 }
 
+// The per-level kernels of the large levels. One thread per point on a 3-D launch (64 x 4 threads: x fastest, a
+// wave on 64 consecutive points of one row; no index divisions). Every neighbour load is issued unconditionally from a
+// clamped, always valid address and a missing neighbour's term is replaced by an exact zero afterwards, so the loads of
+// a thread are in flight together (a load under a bounds branch made the compiler wait for each in turn). Each sum runs
+// over the same terms in the same order as the per-point functions above (an added exact zero leaves a float sum
+// unchanged), so these kernels produce the values of the fused coarse kernel bit for bit.
+constexpr int MG_BX = 64, MG_BY = 4;
+inline dim3 mg_grid(int n) { return dim3((unsigned)((n + MG_BX - 1) / MG_BX), (unsigned)((n + MG_BY - 1) / MG_BY), (unsigned)n); }
+
 template <typename T>
-__global__ void k_mg_jacobi(T *dst, const T *x, const T *b, int n, T h2, T omega, T g_low, T g_high) {
-    const long N = (long)n * n * n, nn = (long)n * n;
-    const long p = blockIdx.x * 256L + threadIdx.x;
-    if (p >= N) return;
-    const int k = (int)(p / nn), rem = (int)(p - (long)k * nn), j = rem / n, i = rem - j * n;
-    dst[p] = mg_jacobi_pt(x, b, p, i, j, k, n, h2, omega, g_low, g_high);
+__device__ inline void a2_parts_bf(const T *x, int i, int j, int k, int n, T g_low, T g_high, T &nb, T &diag, T &xq) {
+    const long nn = (long)n * n;
+    const long id = i + (long)n * j + nn * k;
+    const bool il = i > 0, ih = i < n - 1, jl = j > 0, jh = j < n - 1, kl = k > 0, kh = k < n - 1;
+    const T a0 = x[il ? id - 1 : id], a1 = x[ih ? id + 1 : id], a2 = x[jl ? id - n : id], a3 = x[jh ? id + n : id];
+    const T a4 = x[kl ? id - nn : id], a5 = x[kh ? id + nn : id];
+    xq = x[id];
+    T s = T(0), g = T(0);
+    s += il ? a0 : T(0);
+    g += il ? T(0) : g_low;
+    s += ih ? a1 : T(0);
+    g += ih ? T(0) : g_high;
+    s += jl ? a2 : T(0);
+    g += jl ? T(0) : g_low;
+    s += jh ? a3 : T(0);
+    g += jh ? T(0) : g_high;
+    s += kl ? a4 : T(0);
+    g += kl ? T(0) : g_low;
+    s += kh ? a5 : T(0);
+    g += kh ? T(0) : g_high;
+    nb = s;
+    diag = T(6) - g;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_jacobi(T *dst, const T *x, const T *b, int n, T h2, T omega, T g_low,
+                                                   T g_high) {
+    const int i = blockIdx.x * MG_BX + threadIdx.x, j = blockIdx.y * MG_BY + threadIdx.y, k = blockIdx.z;
+    if (i >= n || j >= n) return;
+    const long p = i + (long)n * j + (long)n * n * k;
+    T nb, diag, xq;
+    a2_parts_bf(x, i, j, k, n, g_low, g_high, nb, diag, xq);
+    dst[p] = xq + (omega / diag) * (h2 * b[p] + nb - diag * xq);
 }
 
 // the first sweep of a level from x = 0 (the zero fill folded in): the Jacobi formula with x and its neighbours 0
-template <typename T> __global__ void k_mg_jacobi0(T *dst, const T *b, int n, T h2, T omega, T g_low, T g_high) {
-    const long N = (long)n * n * n, nn = (long)n * n;
-    const long p = blockIdx.x * 256L + threadIdx.x;
-    if (p >= N) return;
-    const int k = (int)(p / nn), rem = (int)(p - (long)k * nn), j = rem / n, i = rem - j * n;
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_jacobi0(T *dst, const T *b, int n, T h2, T omega, T g_low, T g_high) {
+    const int i = blockIdx.x * MG_BX + threadIdx.x, j = blockIdx.y * MG_BY + threadIdx.y, k = blockIdx.z;
+    if (i >= n || j >= n) return;
+    const long p = i + (long)n * j + (long)n * n * k;
     T g = T(0);
     if (i == 0) g += g_low;
     if (i == n - 1) g += g_high;
@@ -279,28 +505,81 @@ template <typename T> __global__ void k_mg_jacobi0(T *dst, const T *b, int n, T 
     dst[p] = T(0) + (omega / diag) * (h2 * b[p] + T(0) - diag * T(0));
 }
 
-template <typename T> __global__ void k_mg_residual(T *dst, const T *x, const T *b, int n, T h2, T g_low, T g_high) {
-    const long N = (long)n * n * n, nn = (long)n * n;
-    const long p = blockIdx.x * 256L + threadIdx.x;
-    if (p >= N) return;
-    const int k = (int)(p / nn), rem = (int)(p - (long)k * nn), j = rem / n, i = rem - j * n;
-    dst[p] = mg_residual_pt(x, b, p, i, j, k, n, h2, g_low, g_high);
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_residual(T *dst, const T *x, const T *b, int n, T h2, T g_low, T g_high) {
+    const int i = blockIdx.x * MG_BX + threadIdx.x, j = blockIdx.y * MG_BY + threadIdx.y, k = blockIdx.z;
+    if (i >= n || j >= n) return;
+    const long p = i + (long)n * j + (long)n * n * k;
+    T nb, diag, xq;
+    a2_parts_bf(x, i, j, k, n, g_low, g_high, nb, diag, xq);
+    dst[p] = b[p] + (nb - diag * xq) / h2;
 }
 
-template <typename T> __global__ void k_mg_restrict(T *dst, const T *f, int nc, int nf) {
-    const long Nc = (long)nc * nc * nc, ncc = (long)nc * nc;
-    const long p = blockIdx.x * 256L + threadIdx.x;
-    if (p >= Nc) return;
-    const int k = (int)(p / ncc), rem = (int)(p - (long)k * ncc), j = rem / nc, i = rem - j * nc;
-    dst[p] = mg_restrict_pt(f, i, j, k, nf);
+// full weighting onto coarse point (i, j, k) from the 27 fine points about (2i + 1, 2j + 1, 2k + 1); a fine index past
+// the grid (the top of an even level) reads a clamped address and contributes an exact zero
+template <typename T> __global__ __launch_bounds__(256) void k_mg_restrict(T *dst, const T *f, int nc, int nf) {
+    const int i = blockIdx.x * MG_BX + threadIdx.x, j = blockIdx.y * MG_BY + threadIdx.y, k = blockIdx.z;
+    if (i >= nc || j >= nc) return;
+    const long nff = (long)nf * nf;
+    T s = T(0);
+#pragma unroll
+    for (int dz = -1; dz <= 1; dz++) {
+        const int c = 2 * k + 1 + dz;
+        const bool cz = c < nf;  // c >= 0 always
+        const T wz = dz == 0 ? T(0.5) : T(0.25);
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++) {
+            const int bb = 2 * j + 1 + dy;
+            const bool cy = bb < nf;
+            const T wy = dy == 0 ? T(0.5) : T(0.25);
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++) {
+                const int a = 2 * i + 1 + dx;
+                const bool cx = a < nf;
+                const T wx = dx == 0 ? T(0.5) : T(0.25);
+                const T v = f[(cx ? a : nf - 1) + (long)nf * (cy ? bb : nf - 1) + nff * (cz ? c : nf - 1)];
+                s += (cx && cy && cz) ? wx * wy * wz * v : T(0);
+            }
+        }
+    }
+    dst[i + (long)nc * j + (long)nc * nc * k] = s;
 }
 
-template <typename T> __global__ void k_mg_prolong_add(T *fine, const T *c, int nf, int nc) {
-    const long Nf = (long)nf * nf * nf, nff = (long)nf * nf;
-    const long p = blockIdx.x * 256L + threadIdx.x;
-    if (p >= Nf) return;
-    const int k = (int)(p / nff), rem = (int)(p - (long)k * nff), j = rem / nf, i = rem - j * nf;
-    fine[p] += mg_prolong_pt(c, i, j, k, nc);
+// the trilinear interpolation of the coarse correction at fine point (i, j, k) added in place: per axis an odd fine
+// index sits on coarse point t >> 1 (weight 1), an even one between (t >> 1) - 1 and t >> 1 (1/2 each); the eight
+// terms in the order of mg_prolong_pt, a missing one an exact zero
+template <typename T> __global__ __launch_bounds__(256) void k_mg_prolong_add(T *fine, const T *c, int nf, int nc) {
+    const int i = blockIdx.x * MG_BX + threadIdx.x, j = blockIdx.y * MG_BY + threadIdx.y, k = blockIdx.z;
+    if (i >= nf || j >= nf) return;
+    const long ncc = (long)nc * nc;
+    const int t[3] = {i, j, k};
+    int ci[3][2];
+    T wi[3][2];
+    bool ok[3][2];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const bool odd = t[a] & 1;
+        const int cc = t[a] >> 1;
+        ci[a][0] = odd ? cc : cc - 1;
+        ci[a][1] = cc;
+        wi[a][0] = odd ? T(1) : T(0.5);
+        wi[a][1] = T(0.5);
+        ok[a][0] = ci[a][0] >= 0 && ci[a][0] < nc;
+        ok[a][1] = !odd && cc < nc;  // an odd index has one term
+    }
+    T s = T(0);
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int d = 0; d < 2; d++) {
+                const bool in = ok[0][a] && ok[1][b] && ok[2][d];
+                const int x = in ? ci[0][a] : 0, y = in ? ci[1][b] : 0, z = in ? ci[2][d] : 0;
+                const T v = c[x + (long)nc * y + ncc * z];
+                s += in ? wi[0][a] * wi[1][b] * wi[2][d] * v : T(0);
+            }
+    fine[i + (long)nf * j + (long)nf * nf * k] += s;
 }
 
 // The V-cycle from a small level down, in one workgroup with every level's x, b and scratch in LDS: the same
@@ -406,30 +685,27 @@ template <typename T> __global__ __launch_bounds__(1024) void k_mg_coarse(T *x_o
 template <typename T>
 void mg_jacobi(T *dst, const T *x, const T *b, int n, double h2, double omega, double g_low, double g_high,
                hipStream_t s) {
-    const long N = (long)n * n * n;
-    hipLaunchKernelGGL(k_mg_jacobi<T>, LIRA_G1(N), 0, s, dst, x, b, n, (T)h2, (T)omega, (T)g_low, (T)g_high);
+    hipLaunchKernelGGL(k_mg_jacobi<T>, mg_grid(n), dim3(MG_BX, MG_BY), 0, s, dst, x, b, n, (T)h2, (T)omega, (T)g_low,
+                       (T)g_high);
     LIRA_HIP(hipGetLastError());
 }
 template <typename T>
 void mg_residual(T *dst, const T *x, const T *b, int n, double h2, double g_low, double g_high, hipStream_t s) {
-    const long N = (long)n * n * n;
-    hipLaunchKernelGGL(k_mg_residual<T>, LIRA_G1(N), 0, s, dst, x, b, n, (T)h2, (T)g_low, (T)g_high);
+    hipLaunchKernelGGL(k_mg_residual<T>, mg_grid(n), dim3(MG_BX, MG_BY), 0, s, dst, x, b, n, (T)h2, (T)g_low, (T)g_high);
     LIRA_HIP(hipGetLastError());
 }
 template <typename T> void mg_restrict(T *dst, const T *fine, int nc, int nf, hipStream_t s) {
-    const long N = (long)nc * nc * nc;
-    hipLaunchKernelGGL(k_mg_restrict<T>, LIRA_G1(N), 0, s, dst, fine, nc, nf);
+    hipLaunchKernelGGL(k_mg_restrict<T>, mg_grid(nc), dim3(MG_BX, MG_BY), 0, s, dst, fine, nc, nf);
     LIRA_HIP(hipGetLastError());
 }
 template <typename T> void mg_prolong_add(T *fine, const T *coarse, int nf, int nc, hipStream_t s) {
-    const long N = (long)nf * nf * nf;
-    hipLaunchKernelGGL(k_mg_prolong_add<T>, LIRA_G1(N), 0, s, fine, coarse, nf, nc);
+    hipLaunchKernelGGL(k_mg_prolong_add<T>, mg_grid(nf), dim3(MG_BX, MG_BY), 0, s, fine, coarse, nf, nc);
     LIRA_HIP(hipGetLastError());
 }
 template <typename T>
 void mg_jacobi0(T *dst, const T *b, int n, double h2, double omega, double g_low, double g_high, hipStream_t s) {
-    const long N = (long)n * n * n;
-    hipLaunchKernelGGL(k_mg_jacobi0<T>, LIRA_G1(N), 0, s, dst, b, n, (T)h2, (T)omega, (T)g_low, (T)g_high);
+    hipLaunchKernelGGL(k_mg_jacobi0<T>, mg_grid(n), dim3(MG_BX, MG_BY), 0, s, dst, b, n, (T)h2, (T)omega, (T)g_low,
+                       (T)g_high);
     LIRA_HIP(hipGetLastError());
 }
 // LDS bytes of the fused V-cycle over the levels of a (x, b and scratch per level)
@@ -479,8 +755,8 @@ template <typename T> void add_into_f64(double *x, const T *e, long N, hipStream
 }
 
 #define LIRA_PINST(T)                                                                                         \
-    template void multipole_moments<T>(const T *, int, double, double *, double *, hipStream_t);              \
-    template void poisson_rhs<T>(double *, const T *, const double *, int, double, hipStream_t);              \
+    template void multipole_moments<T>(const T *, int, double, double *, const double *, double *, hipStream_t); \
+    template void poisson_rhs<T>(double *, const T *, const double *, double *, int, double, hipStream_t);     \
     template void mg_jacobi<T>(T *, const T *, const T *, int, double, double, double, double, hipStream_t);  \
     template void mg_residual<T>(T *, const T *, const T *, int, double, double, double, hipStream_t);        \
     template void mg_restrict<T>(T *, const T *, int, int, hipStream_t);                                      \

@@ -214,10 +214,18 @@ template <typename T> void scale_ratio(T *y, long N, const double *num, const do
 template <typename T>
 void sample_density(const T *u, const T *d, int n, double h, const double *pts, int count, double *out, hipStream_t s);
 // --------------------------------------------------------------- Poisson ----
-// Q_lm = dv * sum_p rho_p R_lm(r_p) for the 49 real solid harmonics l <= 6 (qm/src/poisson.rs:97-162)
-template <typename T> void multipole_moments(const T *rho, int n, double h, double *partials, double *q, hipStream_t s);
-// b = 4 pi rho + L_bc (qm/src/poisson.rs:175-214) with the moments q on the device
-template <typename T> void poisson_rhs(double *b, const T *rho, const double *q, int n, double h, hipStream_t s);
+// Q_lm = dv * sum_p rho_p R_lm(r_p) for the 49 real solid harmonics l <= 6 (qm/src/poisson.rs:97-162), from the raw
+// moments x^a y^b z^c (a + b + c <= 6) by separable sums; work holds moment_work_doubles(n), coef the 49 x 84 table
+// moment_coefficients() writes (on the device)
+constexpr int MOMENT_COEFS = 49 * 84;
+void moment_coefficients(double *coef);
+size_t moment_work_doubles(int n);
+template <typename T>
+void multipole_moments(const T *rho, int n, double h, double *work, const double *coef, double *q, hipStream_t s);
+// b = 4 pi rho + L_bc (qm/src/poisson.rs:175-214) with the moments q on the device; ghost holds 24 n^2 doubles (the
+// boundary potential on the 4 layers past each face)
+template <typename T>
+void poisson_rhs(double *b, const T *rho, const double *q, double *ghost, int n, double h, hipStream_t s);
 // multigrid level operators (7-point A2 with boundary-distance ghost factors)
 template <typename T>
 void mg_jacobi(T *dst, const T *x, const T *b, int n, double h2, double omega, double g_low, double g_high,

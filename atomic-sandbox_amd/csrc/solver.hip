@@ -113,7 +113,8 @@ static unsigned long long solve_footprint(int n, int esize, int n_up, int n_dn, 
     const int h1 = std::max(diis_history, 1) + 1;
     const double cols = (n_up > 0 ? n_up + witness + 1 : 1) + (n_dn > 0 ? n_dn + witness + 1 : 1);
     const double t_fields = 10.0 * cols + 12.0 + 4.0 * h1 + 4.0 + 7.0 + 4.0 / 7.0;
-    return (unsigned long long)(1.1 * N * (esize * t_fields + 8.0 * 5.0));
+    const double boundary = 8.0 * (31.0 * n * n + 28.0 * n);  // the moments' row sums and the ghost layers
+    return (unsigned long long)(1.1 * (N * (esize * t_fields + 8.0 * 5.0) + boundary));
 }
 
 // ------------------------------------------------------------------------------------- occupations ----
@@ -562,6 +563,7 @@ template <typename T> class Solver final : public SolverIface {
     int mg_entry_ = 0;  // the level from which the V-cycle runs fused in one workgroup (mgc_)
     MgCoarse mgc_{};
     DevBuf pb_, pr64_, ptmp64_, pr_, pz_, pp_, pq_, pe_, psc_;
+    DevBuf pmom_, pghost_, pcoef_;  // the moments' row/plane sums, the boundary ghost layers, R_lm's monomial table
     bool poisson_ready_ = false;
     int cg_total_ = 0;
     // SCF
@@ -863,6 +865,14 @@ template <typename T> void Solver<T>::poisson_setup() {
     pb_.alloc(N_ * sizeof(double));
     pr64_.alloc(N_ * sizeof(double));
     ptmp64_.alloc(N_ * sizeof(double));
+    pmom_.alloc(moment_work_doubles(n_) * sizeof(double));
+    pghost_.alloc((size_t)24 * n_ * n_ * sizeof(double));
+    {
+        std::vector<double> coef(MOMENT_COEFS);
+        moment_coefficients(coef.data());
+        pcoef_.alloc(coef.size() * sizeof(double));
+        LIRA_HIP(hipMemcpy(pcoef_.ptr, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     for (DevBuf *b : {&pr_, &pz_, &pp_, &pq_, &pe_}) alloc_field(*b);
     psc_.alloc(64 * sizeof(double));
     // level 0 works in pz_ (x) with rhs pr_; coarse levels own x and b
@@ -953,9 +963,9 @@ template <typename T> int Solver<T>::pcg(const T *r0, T *e, double rel_tol, int 
         return std::vector<uintptr_t>{a, b, (uintptr_t)r, (uintptr_t)z, (uintptr_t)p, (uintptr_t)q, (uintptr_t)sc,
                                       (uintptr_t)partials_.ptr, (uintptr_t)dres_.ptr, (uintptr_t)levels_[0].tmp.ptr};
     };
-    // sc: [0],[1] rz ping-pong, [2] pq, [3 + i % 2] rr of iteration i, [8] r0.r0
+    // sc: [4 c] r.z and [4 c + 1] r.r of the current slot pair c (c alternates between 0 and 1), [2] p.q, [8] r0.r0
     run_graph(pcg_start_graph_, key((uintptr_t)r0, (uintptr_t)e), [&] {
-        copy<T>(r, r0, N_, stream_);
+        if (r0 != r) copy<T>(r, r0, N_, stream_);
         fill<T>(e, N_, 0.0, stream_);
         RedJobs<T> j0{};
         j0.a[0] = r;
@@ -963,11 +973,10 @@ template <typename T> int Solver<T>::pcg(const T *r0, T *e, double rel_tol, int 
         j0.kind[0] = RED_DOT;
         j0.count = 1;
         reduce_jobs<T>(j0, n_, 1.0, partials_.as<double>(), sc + 8, stream_);
-        vcycle(0, z, r);
-        copy<T>(p, z, N_, stream_);
+        vcycle(0, p, r);  // the first direction is the preconditioned residual itself
         RedJobs<T> jrz{};
         jrz.a[0] = r;
-        jrz.b[0] = z;
+        jrz.b[0] = p;
         jrz.kind[0] = RED_DOT;
         jrz.count = 1;
         reduce_jobs<T>(jrz, n_, 1.0, partials_.as<double>(), sc + 0, stream_);
@@ -997,7 +1006,7 @@ template <typename T> int Solver<T>::pcg(const T *r0, T *e, double rel_tol, int 
             jpq.kind[0] = RED_DOT;
             jpq.count = 1;
             reduce_jobs<T>(jpq, n_, 1.0, partials_.as<double>(), sc + 2, stream_);
-            cg_update<T, T>(e, p, r, q, sc, cur, 2, N_, stream_);
+            cg_update<T, T>(e, p, r, q, sc, 4 * cur, 2, N_, stream_);
             vcycle(0, z, r);
             RedJobs<T> jn{};
             jn.a[0] = r;
@@ -1007,14 +1016,12 @@ template <typename T> int Solver<T>::pcg(const T *r0, T *e, double rel_tol, int 
             jn.b[1] = r;
             jn.kind[1] = RED_DOT;
             jn.count = 2;
-            // rz_new into sc[1 - cur], rr into sc[3 + it % 2]: reduce writes consecutive slots, so route via dres
-            reduce_jobs<T>(jn, n_, 1.0, partials_.as<double>(), dres_.as<double>() + 16, stream_);
-            LIRA_HIP(hipMemcpyAsync(sc + (1 - cur), dres_.as<double>() + 16, sizeof(double), hipMemcpyDeviceToDevice,
-                                    stream_));
-            cg_direction<T>(p, z, sc, 1 - cur, cur, N_, stream_);
+            // the new r.z and r.r into the other slot pair (consecutive slots, as the reduction writes them)
+            reduce_jobs<T>(jn, n_, 1.0, partials_.as<double>(), sc + 4 * (1 - cur), stream_);
+            cg_direction<T>(p, z, sc, 4 * (1 - cur), 4 * cur, N_, stream_);
         });
         const int slot = it & 1;
-        LIRA_HIP(hipMemcpyAsync(hrr + slot, dres_.as<double>() + 17, sizeof(double), hipMemcpyDeviceToHost, stream_));
+        LIRA_HIP(hipMemcpyAsync(hrr + slot, sc + 4 * (1 - cur) + 1, sizeof(double), hipMemcpyDeviceToHost, stream_));
         LIRA_HIP(hipEventRecord(ev[slot], stream_));
         pending[slot] = true;
         cur = 1 - cur;
@@ -1047,9 +1054,9 @@ template <typename T> PoissonOut Solver<T>::poisson(const T *rho, double *v, dou
     poisson_setup();
     PoissonOut out;
     double *q = dsmall_.as<double>();
-    multipole_moments<T>(rho, n_, h_, partials_.as<double>(), q, stream_);
+    multipole_moments<T>(rho, n_, h_, pmom_.as<double>(), pcoef_.as<double>(), q, stream_);
     double *b = pb_.as<double>(), *r64 = pr64_.as<double>(), *t64 = ptmp64_.as<double>();
-    poisson_rhs<T>(b, rho, q, n_, h_, stream_);
+    poisson_rhs<T>(b, rho, q, pghost_.as<double>(), n_, h_, stream_);
     RedJobs<double> jb{};
     jb.a[0] = b;
     jb.b[0] = b;
@@ -1080,12 +1087,10 @@ template <typename T> PoissonOut Solver<T>::poisson(const T *rho, double *v, dou
         if (sizeof(T) == 8) {
             r0 = reinterpret_cast<const T *>(r64);
         } else {
-            convert<T, double>(pe_.template as<T>(), r64, N_, stream_);
-            copy<T>(pq_.template as<T>(), pe_.template as<T>(), N_, stream_);
-            r0 = pq_.template as<T>();
+            convert<T, double>(pr_.template as<T>(), r64, N_, stream_);  // straight into the CG's residual
+            r0 = pr_.template as<T>();
         }
         double rel_out = 0.0;
-        // r0 may alias pq_ (the CG's q); pcg copies it into r before q is written
         out.iterations += pcg(r0, e, rel, max_iter - out.iterations, &rel_out);
         add_into_f64<T>(v, e, N_, stream_);
     }
@@ -2262,16 +2267,19 @@ lira_status Solver<T>::solve(const lira_system *sys, const lira_settings *st, in
         }
         // a frozen orbital set re-solved changes the map rho_in -> rho_out, and so does a witness window that widened
         // and grew the block: the history restarts (crates/qm-gpu/src/scf.rs:1538-1552)
+        // (the f32 solve's rules: its eigen tolerance sits at the f32 residual floor, where a converged head comes back
+        // frozen and re-solved by turns; the f64 solve keeps its history through such steps, as the CPU solver does)
+        const bool f32_rules = sizeof(T) == 4;
         const bool widened = step_tail_ > prev_tail && doublings_ > growths_before;
         prev_tail = step_tail_;
-        const bool map_changed = ((frozen && !step_frozen) || widened) && !order.empty();
+        const bool map_changed = f32_rules && ((frozen && !step_frozen) || widened) && !order.empty();
         if (map_changed) profile_.diis_resets++;
         // a frozen phase that does not contract the residual by (1 - alpha) per step in the Gram norm drops the
         // entries older than the freeze (crates/qm-gpu/src/scf.rs:1553-1566)
         consistent = step_frozen && fd ? (frozen ? consistent + 1 : 2) : 0;
         const bool not_contracting =
             !order.empty() && row.back() > (1.0 - st->mixing) * (1.0 - st->mixing) * gram[(size_t)order.back() * H1 + order.back()];
-        const bool freeze_trim = fd && frozen && step_frozen && !map_changed && not_contracting &&
+        const bool freeze_trim = f32_rules && fd && frozen && step_frozen && !map_changed && not_contracting &&
                                  (int)order.size() + 1 > consistent;
         if (freeze_trim) profile_.diis_freeze_trims++;
         if (step_frozen && fd) profile_.frozen_steps++;
