@@ -34,6 +34,7 @@ EXPORTS = [
     "lira_state_summary", "lira_effective_gpu_tol", "lira_evaluate", "lira_solve_batch", "lira_solve_polled",
     "lira_inject_readback_fault", "lira_gram_noalloc", "lira_mg_vcycle", "lira_mg_level_op",
     "lira_solve_footprint_bytes", "lira_witness_cap_for_budget", "lira_unoccupied", "lira_harris",
+    "lira_stream_bandwidth",
 ]
 
 
@@ -244,6 +245,8 @@ def lib():
         L.lira_gram.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, vp, C.c_int, vp, _dp, vp]
         L.lira_gram_noalloc.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, vp, C.c_int, vp, vp, vp]
         L.lira_mg_vcycle.argtypes = [vp, _dp, _dp]
+        L.lira_stream_bandwidth.argtypes = [C.c_int, C.c_int, vp, vp, C.c_ulonglong, C.c_int, vp,
+                                            C.POINTER(C.c_double)]
         L.lira_mg_level_op.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, vp, vp, vp, vp]
         L.lira_stencil_variant_count.restype = C.c_int
         L.lira_stencil_variant_name.argtypes = [C.c_int]
@@ -720,6 +723,20 @@ def gram_noalloc(n, h, s_block, m, u_block, q, g_dev, stream=None):
     """lira_gram_noalloc: G = h^3 S^T U into the device tensor g_dev (m x q f64) with no device allocation; returns the
     status (LIRA_INVALID when the shape's partials exceed the fixed scratch array)."""
     return lib().lira_gram_noalloc(_dtype(s_block), n, h, m, _ptr(s_block), q, _ptr(u_block), _ptr(g_dev), stream)
+
+
+STREAM_COPY, STREAM_READ, STREAM_WRITE = range(3)
+
+
+def stream_bandwidth(kind, src, dst, nbytes, reps=20, nontemporal=False, stream=None):
+    """lira_stream_bandwidth: GB/s of the hand-written streaming kernels (copy: 2 x nbytes moved per launch; read,
+    write: nbytes) over device tensors src / dst, timed with HIP events on `stream`."""
+    out = C.c_double(0.0)
+    st = lib().lira_stream_bandwidth(kind, int(nontemporal), _ptr(src) if src is not None else None,
+                                     _ptr(dst) if dst is not None else None, nbytes, reps, stream, C.byref(out))
+    if st != OK:
+        raise RuntimeError(f"lira_stream_bandwidth: status {st}")
+    return out.value
 
 
 MG_JACOBI, MG_RESIDUAL, MG_RESTRICT, MG_PROLONG_ADD = range(4)

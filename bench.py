@@ -70,14 +70,25 @@ def hx_roofline(torch, lira, n=192, nb=64, reps=20, dtype="float32", variant=Non
         out.copy_(x)
     stop.record(stream)
     stop.synchronize()
-    copy_gbs = 2 * x.numel() * esize / (start.elapsed_time(stop) / reps * 1e-3) / 1e9
+    torch_copy_gbs = 2 * x.numel() * esize / (start.elapsed_time(stop) / reps * 1e-3) / 1e9
+    # the hand-written streaming kernels on the same buffers (16-byte vector loads and stores, plain and nontemporal
+    # stores): the device's read, write and copy ceilings on this box
+    nbytes = x.numel() * esize
+    stream_gbs = {"copy": lira.stream_bandwidth(lira.STREAM_COPY, x, out, nbytes, reps, stream=sp),
+                  "copy_nontemporal": lira.stream_bandwidth(lira.STREAM_COPY, x, out, nbytes, reps, True, stream=sp),
+                  "read": lira.stream_bandwidth(lira.STREAM_READ, x, None, nbytes, reps, stream=sp),
+                  "write": lira.stream_bandwidth(lira.STREAM_WRITE, None, out, nbytes, reps, stream=sp),
+                  "write_nontemporal": lira.stream_bandwidth(lira.STREAM_WRITE, None, out, nbytes, reps, True,
+                                                             stream=sp),
+                  "torch_copy": torch_copy_gbs}
+    copy_gbs = max(stream_gbs["copy"], stream_gbs["copy_nontemporal"], torch_copy_gbs)
     N = n ** 3
     algo_bytes = N * (2 * esize * nb + esize)  # x in, HX out per band, V once per block
     achieved = algo_bytes / (ms * 1e-3) / 1e9
     del x, v, out
     torch.cuda.empty_cache()
     return {"ms": ms, "bytes": algo_bytes, "gbs": achieved, "variant": lira.stencil_variants()[tuned],
-            "variant_index": tuned, "frac": achieved / HBM_PEAK_GBS, "copy_gbs": copy_gbs}
+            "variant_index": tuned, "frac": achieved / HBM_PEAK_GBS, "copy_gbs": copy_gbs, "stream_gbs": stream_gbs}
 
 
 def poisson_256(lira, n=256, h=0.15, seed=3):
@@ -381,7 +392,8 @@ def main():
     if args.roofline_only:
         r = hx_roofline(torch, lira, variant=args.variant)
         print(json.dumps({"roofline_only": True, "ms_per_launch": r["ms"], "achieved_gbs": r["gbs"],
-                          "variant": r["variant"], "variant_index": r["variant_index"], "copy_gbs": r["copy_gbs"]}))
+                          "variant": r["variant"], "variant_index": r["variant_index"], "copy_gbs": r["copy_gbs"],
+                          "stream_gbs": r["stream_gbs"]}))
         return
     n, h = 95, 0.15
     pos, z, nu, nd = water()
@@ -452,6 +464,7 @@ def main():
                               "kernel": "lira::k_stencil* (ST_HAM, f32) H.X on 192^3 x 64 bands",
                               "ms_per_launch": r["ms"], "algorithmic_bytes": r["bytes"], "variant": r["variant"],
                               "copy_gbs": r["copy_gbs"], "frac_of_copy": r["gbs"] / r["copy_gbs"],
+                              "stream_gbs": r["stream_gbs"],
                               "traffic_detail": traffic}
         r64 = hx_roofline(torch, lira, dtype="float64")
         result["roofline_fp64"] = {"achieved": r64["gbs"], "frac": r64["frac"], "ms_per_launch": r64["ms"],

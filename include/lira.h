@@ -453,6 +453,13 @@ lira_status lira_gram(int dtype, int n, double h, int m, const void *s_block, in
 lira_status lira_gram_noalloc(int dtype, int n, double h, int m, const void *s_block, int q, const void *u_block,
                               double *g_dev, void *stream);
 
+/* No counterpart in the reference: the device's streaming ceiling, from hand-written kernels of 16-byte vector loads
+ * and stores. kind 0 copies src to dst (2 x bytes moved), 1 reads src (bytes moved), 2 writes dst (bytes moved);
+ * nontemporal != 0 uses nontemporal stores. bytes a multiple of 16, pointers 16-byte aligned, device memory. Runs two
+ * untimed launches, then reps launches timed with HIP events on `stream`; *gbs = bytes moved / mean launch time. */
+lira_status lira_stream_bandwidth(int kind, int nontemporal, const void *src, void *dst, unsigned long long bytes,
+                                  int reps, void *stream, double *gbs);
+
 /* stencil tile variants and the autotuner (the tile shape is picked per grid by timing) */
 int lira_stencil_variant_count(void);
 const char *lira_stencil_variant_name(int v);

@@ -216,3 +216,26 @@ def test_gram_with_u_beginning_with_s(dev, m, q):
         xd = xt.double().cpu().numpy()
         want = xd[:m] @ xd.T * h ** 3
         assert np.abs(g - want).max() <= 1e-12 * np.abs(want).max(), dt
+
+
+def test_stream_kernels_copy_and_fill(dev):
+    """The streaming-ceiling kernels move exactly the bytes they are timed on: the copy (plain and nontemporal
+    stores) leaves dst equal to src over a length that is not a multiple of a workgroup's span, the write fills every
+    vector with 1.0, and each reports a positive rate."""
+    import lira
+    torch = dev
+    count = 4 * (3 * 256 * 4 + 7)  # floats: 3 full workgroup spans plus 7 vectors
+    src = torch.randn(count, device="cuda", dtype=torch.float32)
+    for nt in (False, True):
+        dst = torch.zeros(count + 8, device="cuda", dtype=torch.float32)
+        gbs = lira.stream_bandwidth(lira.STREAM_COPY, src, dst, count * 4, reps=2, nontemporal=nt)
+        torch.cuda.synchronize()
+        assert gbs > 0.0
+        assert torch.equal(dst[:count], src)
+        assert torch.count_nonzero(dst[count:]) == 0  # nothing past the span
+        gbs = lira.stream_bandwidth(lira.STREAM_WRITE, None, dst, count * 4, reps=2, nontemporal=nt)
+        torch.cuda.synchronize()
+        assert gbs > 0.0 and bool((dst[:count] == 1.0).all()) and torch.count_nonzero(dst[count:]) == 0
+    assert lira.stream_bandwidth(lira.STREAM_READ, src, None, count * 4, reps=2) > 0.0
+    with pytest.raises(RuntimeError):
+        lira.stream_bandwidth(lira.STREAM_COPY, src, src, count * 4 - 4)  # not a multiple of 16 bytes
