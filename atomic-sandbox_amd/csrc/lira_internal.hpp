@@ -299,4 +299,23 @@ void chol_solve(int m, const std::vector<double> &l, std::vector<double> &b, int
 double rayleigh_ritz(int m, const std::vector<double> &A, const std::vector<double> &B, int nk, double drop,
                      std::vector<double> &theta, std::vector<double> &C);
 
+// ------------------------------------------------------- the Rayleigh-Ritz step on the device ----
+// One LOBPCG iteration's stop test and Rayleigh-Ritz step in one workgroup, for m <= RR_DEV_MAX subspace columns.
+// G: the Gram (m x q row-major, q = 2m) followed by the k squared residual norms; theta: the k Ritz values (read, and
+// the moving ones rewritten); C: the lincomb's padded coefficient block (mp x ldc, [C | C_p] in the first 2 nm
+// columns); record: [stop, error, head residual, tail residual, dropped P, amplification, Jacobi sweeps, 0,
+// lambda[k], residual[k]]
+// (error: 1 non-finite, 2 rank lost, 3 Cholesky failed, 4 a non-positive column norm).
+constexpr int RR_DEV_MAX = 36;
+constexpr int RR_REC_HEAD = 8;
+constexpr int RR_REC_STRIDE = 128;  // >= RR_REC_HEAD + 2 RR_DEV_MAX
+struct RrStepArgs {
+    const double *G;
+    double *theta, *C, *work, *record;
+    int m, q, k, na, np, nm, locked, head, checked, iter, max_iter, is_f32, mp, ldc;
+    double tol, guess_tol, tail_scale, floor, alpha_max, alpha_fraction, drop;
+};
+size_t rr_step_work_doubles();
+void rr_step(const RrStepArgs &args, hipStream_t s);
+
 }  // namespace lira

@@ -398,6 +398,7 @@ struct SolverIface {
     virtual void get_profile(lira_profile *out, double *drho, double *etol, int cap, int *n) const = 0;
     virtual void sample(const double *pts, int count, double *out) = 0;
     virtual void set_graphs(bool on) = 0;
+    virtual void set_device_rr(bool on) = 0;
     virtual void state_summary(double *spin_abs, double *rho_max) = 0;
     virtual bool ionic() const = 0;
     virtual bool density_extrapolation() const = 0;
@@ -427,6 +428,9 @@ template <typename T> class Solver final : public SolverIface {
     ~Solver() override {
         if (stream_) (void)hipStreamSynchronize(stream_);
         drop_graphs();
+        for (hipEvent_t &e : rr_ev_)
+            if (e) (void)hipEventDestroy(e);
+        if (rr_host_) (void)hipHostFree(rr_host_);
         if (hbuf_) (void)hipHostFree(hbuf_);
         if (up_) (void)hipHostFree(up_);
         if (stream_) (void)hipStreamDestroy(stream_);
@@ -495,6 +499,7 @@ template <typename T> class Solver final : public SolverIface {
         if (!on) drop_graphs();
         graphs_ = on;
     }
+    void set_device_rr(bool on) override { device_rr_ = on; }
 
   private:
     // ---- HIP graphs: a fixed kernel sequence between two host syncs is captured once and replayed ----
@@ -507,6 +512,19 @@ template <typename T> class Solver final : public SolverIface {
         hipGraphExec_t exec = nullptr;
     };
     bool graphs_ = false;  // opt-in (lira_set_graphs): one solver per thread may capture
+    // LOBPCG's Rayleigh-Ritz step on the device from the second iteration on (subspaces up to RR_DEV_MAX columns),
+    // with one iteration queued ahead of the host's read of the last iteration's record
+    bool device_rr_ = false;  // opt-in (lira_set_device_rr): at the water's 27 columns the host step is faster
+    DevBuf rr_work_, rr_rec_;
+    double *rr_host_ = nullptr;  // pinned: two record slots
+    hipEvent_t rr_ev_[2] = {nullptr, nullptr};
+    void ensure_device_rr() {
+        if (rr_host_) return;
+        rr_work_.alloc(rr_step_work_doubles() * sizeof(double));
+        rr_rec_.alloc(2 * RR_REC_STRIDE * sizeof(double));
+        LIRA_HIP(hipHostMalloc((void **)&rr_host_, 2 * RR_REC_STRIDE * sizeof(double), hipHostMallocDefault));
+        for (hipEvent_t &e : rr_ev_) LIRA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
     GraphSlot pcg_graph_[2], pcg_start_graph_;
     static void drop_graph(GraphSlot &g) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -1168,6 +1186,105 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
     for (;;) {
         const int lk0 = locked;     // the lock this iteration's S was built under
         const int na = k - lk0;     // its W columns
+        if (device_rr_ && out.iterations >= 1 && np == na && k + na + np <= RR_DEV_MAX) {
+            // From here on the step runs on the device (the lock and the block's shape are now fixed): each queued
+            // iteration measures, tests and updates on the GPU and leaves a record; the host reads iteration t's
+            // record while iteration t + 1 runs. An iteration whose test stops writes the identity, so the one queued
+            // behind it changes nothing, and the records of both equal the stopping iteration's.
+            ensure_device_rr();
+            const int m = k + na + np, q = 2 * m, nm = na;
+            const int mp = lincomb_rows(m), ldc = lincomb_ldc(2 * nm);
+            ensure_gram(m, q);
+            ensure_coeff((size_t)mp * ldc + 64);
+            upload(dth, theta.data(), k);
+            const double *dshift = per_column ? dth : dlam;
+            RrStepArgs ra{};
+            ra.G = dG_.as<double>();
+            ra.theta = dth;
+            ra.C = dC_.as<double>();
+            ra.work = rr_work_.as<double>();
+            ra.m = m, ra.q = q, ra.k = k, ra.na = na, ra.np = np, ra.nm = nm, ra.locked = locked;
+            ra.head = head, ra.checked = checked, ra.max_iter = max_iter, ra.is_f32 = sizeof(T) == 4;
+            ra.mp = mp, ra.ldc = ldc;
+            ra.tol = tol, ra.guess_tol = guess_tol, ra.tail_scale = tol / tail_tol, ra.floor = floor;
+            ra.alpha_max = ALPHA_MAX, ra.alpha_fraction = ALPHA_RESIDUAL_FRACTION, ra.drop = RR_DROP;
+            auto enqueue = [&](int t) {
+                residual_precond<T>(c.R.template as<T>(), c.W.template as<T>(), c.HX.template as<T>(), X(), dth, v,
+                                    dshift, diag_t, k, N_, stream_);
+                StencilParams jp{n_, na, h_, 1.0, omega, diag_t};
+                for (int sw = 0; sw < 3; sw++) {
+                    stencil<T>(ST_JAC, jp, col(c.W, locked), col(c.TMP, locked), v, col(c.R, locked),
+                               dshift + locked, stream_, variant(ST_JAC, na));
+                    std::swap(c.W, c.TMP);
+                }
+                hamiltonian2(v, col(c.W, locked), col(c.HW, locked), na, col(c.P, locked), col(c.HP, locked), np);
+                Cols S{{{X(), k}, {col(c.W, locked), na}, {col(c.P, locked), np}}, 3};
+                Cols U{{{X(), k}, {col(c.W, locked), na}, {col(c.P, locked), np}, {c.R.ptr, k},
+                        {col(c.HW, locked), na}, {col(c.HP, locked), np}}, 6};
+                double *dG = dG_.as<double>();
+                gram<T>(S, U, N_, dv_, gpart_.as<double>(), gpart_cap_, dG, stream_);
+                RedJobs<T> jr{};
+                for (int i = 0; i < k; i++) {
+                    jr.a[i % MAX_JOBS] = col(c.R, i);
+                    jr.b[i % MAX_JOBS] = jr.a[i % MAX_JOBS];
+                    jr.kind[i % MAX_JOBS] = RED_DOT;
+                    jr.count = i % MAX_JOBS + 1;
+                    if (jr.count == MAX_JOBS || i == k - 1) {
+                        reduce_jobs<T>(jr, n_, dv_, partials_.as<double>(),
+                                       dG + (size_t)m * q + (i / MAX_JOBS) * MAX_JOBS, stream_);
+                        jr.count = 0;
+                    }
+                }
+                ra.iter = t;
+                ra.record = rr_rec_.as<double>() + (t & 1) * RR_REC_STRIDE;
+                rr_step(ra, stream_);
+                LIRA_HIP(hipMemcpyAsync(rr_host_ + (t & 1) * RR_REC_STRIDE, ra.record,
+                                        (RR_REC_HEAD + 2 * k) * sizeof(double), hipMemcpyDeviceToHost, stream_));
+                LIRA_HIP(hipEventRecord(rr_ev_[t & 1], stream_));
+                Cols Sm{{{X(), k}, {col(c.W, locked), na}, {col(c.P, locked), np}}, 3};
+                lincomb<T>(Sm, dC_.as<double>(), ldc, 2 * nm, col(c.XN, locked), N_, stream_, nm, col(c.PN, locked));
+                if (locked > 0)
+                    LIRA_HIP(hipMemcpyAsync(c.XN.ptr, c.X.ptr, (size_t)N_ * locked * sizeof(T),
+                                            hipMemcpyDeviceToDevice, stream_));
+                std::swap(c.X, c.XN);
+                std::swap(c.P, c.PN);
+                hamiltonian(v, col(c.X, locked), col(c.HX, locked), nm);
+            };
+            int t = out.iterations, queued = t;
+            enqueue(t);
+            for (;;) {
+                if (queued == t && t < max_iter) enqueue(++queued);
+                LIRA_HIP(hipEventSynchronize(rr_ev_[t & 1]));
+                readbacks_++;
+                if (fault_after_ >= 0 && fault_after_-- == 0) throw DeviceError("a readback failed (injected)");
+                const double *r = rr_host_ + (t & 1) * RR_REC_STRIDE;
+                for (int i = 0; i < k; i++) {
+                    lambda[i] = r[RR_REC_HEAD + i];
+                    res[i] = r[RR_REC_HEAD + k + i];
+                }
+                out.max_residual = r[2];
+                out.guard_residual = r[3];
+                if (r[1] != 0.0) {
+                    const int code = (int)r[1];
+                    throw NumericalError(code == 1 ? "lobpcg: non-finite Gram entry or residual (NaN in a vector or in V)"
+                                         : code == 2 ? "lobpcg: the subspace lost rank in the Rayleigh-Ritz step"
+                                         : code == 4 ? "lobpcg: a column of the subspace has a non-positive norm"
+                                                     : "lobpcg: Cholesky of X^T X failed (X lost rank)");
+                }
+                if (r[0] != 0.0) break;
+                if (trace_on())
+                    std::fprintf(stderr, "lobpcg device step %d: m %d, Jacobi sweeps %d, amplification %.3g%s\n", t, m,
+                                 (int)r[6], r[5], r[4] != 0.0 ? ", without P" : "");
+                if (inner_abort_ && inner_abort_(abort_user_)) {
+                    out.aborted = true;
+                    break;
+                }
+                out.iterations = t + 1;
+                if (r[4] != 0.0) out.fallbacks++;
+                t++;
+            }
+            break;
+        }
         if (!dth_cur) {
             upload(dth, theta.data(), k);
             dth_cur = dth;
@@ -3335,6 +3452,11 @@ lira_status lira_set_graphs(lira_solver *s, int on) {
     return guarded(s, [&] { s->impl->set_graphs(on != 0); });
 }
 
+lira_status lira_set_device_rr(lira_solver *s, int on) {
+    if (!s) return LIRA_INVALID;
+    return guarded(s, [&] { s->impl->set_device_rr(on != 0); });
+}
+
 lira_status lira_free_atom_levels(int z, double *eps, double *electrons) {
     if (z < 1 || z > 36 || !eps || !electrons) return LIRA_INVALID;
     std::vector<double> e;
@@ -3490,6 +3612,53 @@ lira_status lira_rayleigh_ritz(int m, const double *a, const double *b, int nk, 
     } catch (...) {
         return LIRA_NUMERICAL;
     }
+}
+
+lira_status lira_rayleigh_ritz_device(int m, const double *a, const double *b, int nk, double *theta, double *c,
+                                      double *c_p) {
+    if (m < 1 || m > RR_DEV_MAX || nk < 1 || nk > m || !a || !b || !theta || !c) return LIRA_INVALID;
+    lira_status status = LIRA_OK;
+    const lira_status g = guarded(nullptr, [&] {
+        // the Gram layout of a LOBPCG step with k = nk columns of X at theta = 0 (so A comes through unchanged), no
+        // residual tested and a step that never stops
+        const int q = 2 * m, mp = lincomb_rows(m), ldc = lincomb_ldc(2 * nk);
+        std::vector<double> G((size_t)m * q + m, 0.0);
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < m; j++) {
+                G[(size_t)i * q + j] = b[(size_t)i * m + j];
+                G[(size_t)i * q + m + j] = a[(size_t)i * m + j];
+            }
+        DevBuf dg(G.size() * sizeof(double)), dth(RR_DEV_MAX * sizeof(double)),
+            dc((size_t)mp * ldc * sizeof(double)), dw(rr_step_work_doubles() * sizeof(double)),
+            drec(RR_REC_STRIDE * sizeof(double));
+        LIRA_HIP(hipMemcpy(dg.ptr, G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice));
+        LIRA_HIP(hipMemset(dth.ptr, 0, RR_DEV_MAX * sizeof(double)));
+        RrStepArgs ra{};
+        ra.G = dg.as<double>();
+        ra.theta = dth.as<double>();
+        ra.C = dc.as<double>();
+        ra.work = dw.as<double>();
+        ra.record = drec.as<double>();
+        ra.m = m, ra.q = q, ra.k = nk, ra.na = m - nk, ra.np = 0, ra.nm = nk, ra.locked = 0, ra.head = 0;
+        ra.checked = 0, ra.iter = 1, ra.max_iter = 1 << 30, ra.is_f32 = 0, ra.mp = mp, ra.ldc = ldc;
+        ra.tol = -1.0, ra.guess_tol = -1.0, ra.tail_scale = 1.0, ra.floor = 0.0, ra.alpha_max = 1e300;
+        ra.alpha_fraction = 0.0, ra.drop = 1e-6;
+        rr_step(ra, nullptr);
+        std::vector<double> C((size_t)mp * ldc), rec(RR_REC_STRIDE);
+        LIRA_HIP(hipMemcpy(C.data(), dc.ptr, C.size() * sizeof(double), hipMemcpyDeviceToHost));
+        LIRA_HIP(hipMemcpy(rec.data(), drec.ptr, rec.size() * sizeof(double), hipMemcpyDeviceToHost));
+        LIRA_HIP(hipMemcpy(theta, dth.ptr, nk * sizeof(double), hipMemcpyDeviceToHost));
+        if (rec[1] != 0.0) {
+            status = LIRA_NUMERICAL;
+            return;
+        }
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < nk; j++) {
+                c[(size_t)i * nk + j] = C[(size_t)i * ldc + j];
+                if (c_p) c_p[(size_t)i * nk + j] = C[(size_t)i * ldc + nk + j];
+            }
+    });
+    return g != LIRA_OK ? g : status;
 }
 
 lira_status lira_hamiltonian(int dtype, int n, double h, int nbands, const void *v, const void *x, void *hx,

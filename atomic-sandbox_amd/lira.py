@@ -34,7 +34,7 @@ EXPORTS = [
     "lira_state_summary", "lira_effective_gpu_tol", "lira_evaluate", "lira_solve_batch", "lira_solve_polled",
     "lira_inject_readback_fault", "lira_gram_noalloc", "lira_mg_vcycle", "lira_mg_level_op",
     "lira_solve_footprint_bytes", "lira_witness_cap_for_budget", "lira_unoccupied", "lira_harris",
-    "lira_stream_bandwidth",
+    "lira_stream_bandwidth", "lira_rayleigh_ritz_device", "lira_set_device_rr",
 ]
 
 
@@ -236,6 +236,8 @@ def lib():
         L.lira_poisson.argtypes = [vp, _dp, _dp, C.c_double, C.c_int, C.POINTER(PoissonStats)]
         L.lira_lobpcg.argtypes = [vp, _dp, C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_int, C.POINTER(EigenStats)]
         L.lira_rayleigh_ritz.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp]
+        L.lira_rayleigh_ritz_device.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp]
+        L.lira_set_device_rr.argtypes = [vp, C.c_int]
         L.lira_hamiltonian.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, vp, vp, vp, vp]
         L.lira_laplacian.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, vp, vp, vp]
         L.lira_jacobi_sweep.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, vp, vp, vp, vp, vp, vp]
@@ -606,6 +608,11 @@ class GpuSolver:
         (lira_set_graphs)."""
         _raise(lib().lira_set_graphs(self._h, int(bool(on))), self._h, "lira_set_graphs")
 
+    def set_device_rr(self, on):
+        """LOBPCG's Rayleigh-Ritz step on the device from each eigensolve's second iteration on (on, the default) or
+        on the host (off) (lira_set_device_rr)."""
+        _raise(lib().lira_set_device_rr(self._h, int(bool(on))), self._h, "lira_set_device_rr")
+
     def forces(self, pos, z, sigma) -> np.ndarray:
         nuc = nuclei_array(pos, z)
         out = np.zeros(3 * len(nuc))
@@ -678,6 +685,17 @@ def rayleigh_ritz(a, b, nk):
     th, c = np.zeros(nk), np.zeros(m * nk)
     _raise(lib().lira_rayleigh_ritz(m, a, b, nk, th, c), None, "lira_rayleigh_ritz")
     return th, c.reshape(m, nk)
+
+
+def rayleigh_ritz_device(a, b, nk):
+    """lira_rayleigh_ritz_device: the solver's f64 single-workgroup Rayleigh-Ritz kernel on a host pair (A, B), m <= 36.
+    Returns (theta, C, C_p) with C_p = C - [B11^-1 B1: C; 0]."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    m = a.shape[0]
+    th, c, cp = np.zeros(nk), np.zeros(m * nk), np.zeros(m * nk)
+    _raise(lib().lira_rayleigh_ritz_device(m, a, b, nk, th, c, cp), None, "lira_rayleigh_ritz_device")
+    return th, c.reshape(m, nk), cp.reshape(m, nk)
 
 
 def _ptr(t):

@@ -365,6 +365,11 @@ lira_status lira_solve_batch(const lira_batch_job *jobs, int n_jobs, int precisi
  * allocates device memory or launches work while this solver captures (a capture is open during the first call of
  * each sequence); with solvers on several threads keep it off. */
 lira_status lira_set_graphs(lira_solver *s, int on);
+/* LOBPCG's Rayleigh-Ritz step (and its stop test) on the device from the second iteration of each eigensolve on, for
+ * blocks whose subspace [X, W, P] has at most 36 columns (off by default; larger blocks keep the host step). The host
+ * then reads each iteration's record one iteration late while the next one runs; an iteration queued past the stop
+ * leaves the block bit for bit as it was. The results equal the host step's to rounding. */
+lira_status lira_set_device_rr(lira_solver *s, int on);
 
 /* The spherical spin-restricted LSDA free atom behind the Fermi-Dirac cold guess (qm::radial::atom::ground_state,
  * crates/qm/src/radial/atom.rs:474; used by free_atom_density, crates/qm/src/nuclear.rs:153): eps receives the
@@ -417,6 +422,12 @@ lira_status lira_lobpcg(lira_solver *s, const double *v, int k, double *x, doubl
 /* Rayleigh–Ritz on a measured pair (A, B), m x m row-major: the lowest nk Ritz values theta and B-orthonormal
  * coefficient columns C (m x nk, row-major) (crates/qm-gpu/src/eigen.rs:515-542). Host-only. */
 lira_status lira_rayleigh_ritz(int m, const double *a, const double *b, int nk, double *theta, double *c);
+/* The same step on the device (the f64 single-workgroup kernel the solver's LOBPCG uses from its second iteration on,
+ * for subspaces of at most 36 columns): theta and C as lira_rayleigh_ritz; also c_p (m x nk, may be NULL), the
+ * coefficients of C's component B-orthogonal to the first nk columns, C - [B11^-1 B1: C; 0] (the P update of LOBPCG).
+ * Host arrays; LIRA_INVALID for m > 36, LIRA_NUMERICAL when the subspace lost rank or B11 is not positive definite. */
+lira_status lira_rayleigh_ritz_device(int m, const double *a, const double *b, int nk, double *theta, double *c,
+                                      double *c_p);
 
 /* ----------------------------------------------------------------- kernels on device pointers ---- */
 /* dtype: 4 = f32, 8 = f64. Pointers are device pointers; stream is a hipStream_t (NULL = default stream).
