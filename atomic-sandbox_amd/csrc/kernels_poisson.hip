@@ -488,6 +488,36 @@ __global__ __launch_bounds__(256) void k_mg_jacobi(T *dst, const T *x, const T *
     dst[p] = xq + (omega / diag) * (h2 * b[p] + nb - diag * xq);
 }
 
+// a block's sum of one double per thread (256 threads, fixed order): thread 0 returns it
+__device__ inline double block_sum256(double v, double *sh) {
+    v = wave_sum(v);
+    if ((threadIdx.x + threadIdx.y * blockDim.x) % 64 == 0) sh[(threadIdx.x + threadIdx.y * blockDim.x) / 64] = v;
+    __syncthreads();
+    return (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// k_mg_jacobi that also leaves each wave's share of sum_p b[p] dst[p] (the CG's r.z when b is the residual and dst the
+// preconditioned one) in partials[block * 4 + wave]; every lane takes part in its wave's sum
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_jacobi_dot(T *dst, const T *x, const T *b, int n, T h2, T omega, T g_low,
+                                                       T g_high, double *partials) {
+    const int i = blockIdx.x * MG_BX + threadIdx.x, j = blockIdx.y * MG_BY + threadIdx.y, k = blockIdx.z;
+    double d = 0.0;
+    if (i < n && j < n) {
+        const long p = i + (long)n * j + (long)n * n * k;
+        T nb, diag, xq;
+        a2_parts_bf(x, i, j, k, n, g_low, g_high, nb, diag, xq);
+        const T bp = b[p];
+        const T out = xq + (omega / diag) * (h2 * bp + nb - diag * xq);
+        dst[p] = out;
+        d = (double)bp * (double)out;
+    }
+    // one partial per wave (no block barrier in a streaming kernel)
+    d = wave_sum(d);
+    if (threadIdx.x == 0)
+        partials[(blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z)) * MG_BY + threadIdx.y] = d;
+}
+
 // the first sweep of a level from x = 0 (the zero fill folded in): the Jacobi formula with x and its neighbours 0
 template <typename T>
 __global__ __launch_bounds__(256) void k_mg_jacobi0(T *dst, const T *b, int n, T h2, T omega, T g_low, T g_high) {
@@ -580,6 +610,114 @@ template <typename T> __global__ __launch_bounds__(256) void k_mg_prolong_add(T 
                 s += in ? wi[0][a] * wi[1][b] * wi[2][d] * v : T(0);
             }
     fine[i + (long)nf * j + (long)nf * nf * k] += s;
+}
+
+// Restriction with the fine rows read as pairs: coarse point i takes fine 2i, 2i + 1 as one vector load and 2i + 2
+// from its x neighbour's pair (a lane shuffle; the wave's last lane, and the last coarse point, load it). One vector
+// load per fine row instead of three strided ones; the 27 terms in mg_restrict_pt's order. Needs an even fine n (the
+// pairs aligned); every lane stays live for the shuffles and only stores inside the level.
+template <typename T> struct Pair2;
+template <> struct Pair2<float> { typedef float type __attribute__((ext_vector_type(2))); };
+template <> struct Pair2<double> { typedef double type __attribute__((ext_vector_type(2))); };
+
+template <typename T> __global__ __launch_bounds__(256) void k_mg_restrict_pairs(T *dst, const T *f, int nc, int nf) {
+    typedef typename Pair2<T>::type P2;
+    const int i = blockIdx.x * MG_BX + threadIdx.x, j = blockIdx.y * MG_BY + threadIdx.y, k = blockIdx.z;
+    const int ic = i < nc ? i : nc - 1, jc = j < nc ? j : nc - 1;  // loads of lanes past the level stay in it
+    const long nff = (long)nf * nf;
+    const bool last = threadIdx.x == MG_BX - 1 || i + 1 >= nc;
+    T s = T(0);
+#pragma unroll
+    for (int dz = -1; dz <= 1; dz++) {
+        const int c = 2 * k + 1 + dz;
+        const bool cz = c < nf;
+        const T wz = dz == 0 ? T(0.5) : T(0.25);
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++) {
+            const int bb = 2 * jc + 1 + dy;
+            const bool cy = bb < nf;
+            const T wy = dy == 0 ? T(0.5) : T(0.25);
+            const T *row = f + (long)nf * (cy ? bb : nf - 1) + nff * (cz ? c : nf - 1);
+            const P2 pr = *reinterpret_cast<const P2 *>(row + 2 * ic);
+            const int a2 = 2 * ic + 2;
+            T x2 = __shfl_down(pr.x, 1, 64);
+            if (last) x2 = row[a2 < nf ? a2 : nf - 1];
+            const bool c2 = a2 < nf;
+            s += (cy && cz) ? T(0.25) * wy * wz * pr.x : T(0);
+            s += (cy && cz) ? T(0.5) * wy * wz * pr.y : T(0);
+            s += (c2 && cy && cz) ? T(0.25) * wy * wz * x2 : T(0);
+        }
+    }
+    if (i < nc && j < nc) dst[i + (long)nc * j + (long)nc * nc * k] = s;
+}
+
+// Prolongation-add with one thread per coarse cell: the cell's (up to) 8 fine children 2c + (0, 1) per axis share the
+// 8 coarse values at c - 1 and c, loaded once; each child's terms in mg_prolong_pt's order and weights (an odd fine
+// index takes coarse c with weight 1; an even one c - 1 and c with 1/2 each; a missing term an exact zero).
+template <typename T> __global__ __launch_bounds__(256) void k_mg_prolong_cells(T *fine, const T *c, int nf, int nc) {
+    const int hx = (nf + 1) / 2;
+    const int cx = blockIdx.x * MG_BX + threadIdx.x, cy = blockIdx.y * MG_BY + threadIdx.y, cz = blockIdx.z;
+    if (cx >= hx || cy >= hx) return;
+    const long ncc = (long)nc * nc;
+    const int cc[3] = {cx, cy, cz};
+    int idx[3][2];
+    bool ok[3][2];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        idx[a][0] = cc[a] - 1 < 0 ? 0 : (cc[a] - 1 < nc ? cc[a] - 1 : nc - 1);
+        idx[a][1] = cc[a] < nc ? cc[a] : nc - 1;
+        ok[a][0] = cc[a] >= 1;  // c - 1 <= nc - 1 always (c <= nc)
+        ok[a][1] = cc[a] < nc;
+    }
+    T v[2][2][2];  // [x slot][y slot][z slot]
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int d = 0; d < 2; d++) v[a][b][d] = c[idx[0][a] + (long)nc * idx[1][b] + ncc * idx[2][d]];
+#pragma unroll
+    for (int sz = 0; sz < 2; sz++) {
+        const int k = 2 * cz + sz;
+        if (k >= nf) continue;
+#pragma unroll
+        for (int sy = 0; sy < 2; sy++) {
+            const int j = 2 * cy + sy;
+            if (j >= nf) continue;
+#pragma unroll
+            for (int sx = 0; sx < 2; sx++) {
+                const int i = 2 * cx + sx;
+                if (i >= nf) continue;
+                const int sv[3] = {sx, sy, sz};
+                // per axis the two terms of mg_prolong_pt: (slot, weight, present)
+                int sl[3][2];
+                T wt[3][2];
+                bool in[3][2];
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    const bool odd = sv[a] == 1;
+                    sl[a][0] = odd ? 1 : 0;
+                    wt[a][0] = odd ? T(1) : T(0.5);
+                    in[a][0] = odd ? ok[a][1] : ok[a][0];
+                    sl[a][1] = 1;
+                    wt[a][1] = T(0.5);
+                    in[a][1] = !odd && ok[a][1];
+                }
+                T s = T(0);
+#pragma unroll
+                for (int a = 0; a < 2; a++)
+#pragma unroll
+                    for (int b = 0; b < 2; b++)
+#pragma unroll
+                        for (int d = 0; d < 2; d++) {
+                            const bool present = in[0][a] && in[1][b] && in[2][d];
+                            const T val = v[sl[0][a]][sl[1][b]][sl[2][d]];
+                            s += present ? wt[0][a] * wt[1][b] * wt[2][d] * val : T(0);
+                        }
+                fine[i + (long)nf * j + (long)nf * nf * k] += s;
+            }
+        }
+    }
 }
 
 // The V-cycle from a small level down, in one workgroup with every level's x, b and scratch in LDS: the same
@@ -694,12 +832,28 @@ void mg_residual(T *dst, const T *x, const T *b, int n, double h2, double g_low,
     hipLaunchKernelGGL(k_mg_residual<T>, mg_grid(n), dim3(MG_BX, MG_BY), 0, s, dst, x, b, n, (T)h2, (T)g_low, (T)g_high);
     LIRA_HIP(hipGetLastError());
 }
+template <typename T>
+long mg_jacobi_dot(T *dst, const T *x, const T *b, int n, double h2, double omega, double g_low, double g_high,
+                   double *partials, hipStream_t s) {
+    const dim3 g = mg_grid(n);
+    hipLaunchKernelGGL(k_mg_jacobi_dot<T>, g, dim3(MG_BX, MG_BY), 0, s, dst, x, b, n, (T)h2, (T)omega, (T)g_low,
+                       (T)g_high, partials);
+    LIRA_HIP(hipGetLastError());
+    return (long)g.x * g.y * g.z * MG_BY;
+}
+long mg_dot_blocks(int n) {
+    const dim3 g = mg_grid(n);
+    return (long)g.x * g.y * g.z * MG_BY;
+}
 template <typename T> void mg_restrict(T *dst, const T *fine, int nc, int nf, hipStream_t s) {
-    hipLaunchKernelGGL(k_mg_restrict<T>, mg_grid(nc), dim3(MG_BX, MG_BY), 0, s, dst, fine, nc, nf);
+    if (nf % 2 == 0)  // fine rows start at even offsets: the pair loads are aligned
+        hipLaunchKernelGGL(k_mg_restrict_pairs<T>, mg_grid(nc), dim3(MG_BX, MG_BY), 0, s, dst, fine, nc, nf);
+    else
+        hipLaunchKernelGGL(k_mg_restrict<T>, mg_grid(nc), dim3(MG_BX, MG_BY), 0, s, dst, fine, nc, nf);
     LIRA_HIP(hipGetLastError());
 }
 template <typename T> void mg_prolong_add(T *fine, const T *coarse, int nf, int nc, hipStream_t s) {
-    hipLaunchKernelGGL(k_mg_prolong_add<T>, mg_grid(nf), dim3(MG_BX, MG_BY), 0, s, fine, coarse, nf, nc);
+    hipLaunchKernelGGL(k_mg_prolong_cells<T>, mg_grid((nf + 1) / 2), dim3(MG_BX, MG_BY), 0, s, fine, coarse, nf, nc);
     LIRA_HIP(hipGetLastError());
 }
 template <typename T>
@@ -728,10 +882,106 @@ __global__ void k_cg_update(Tv *x, const Tr *p, Tr *r, const Tr *q, const double
     x[i] = (Tv)((double)x[i] + alpha * (double)p[i]);
     r[i] = (Tr)((double)r[i] - alpha * (double)q[i]);
 }
+// k_cg_update that also leaves each wave's share of the new r.r in partials[block * 4 + wave]
+// (each thread updates DOT_PER_THREAD points 256 apart, so the wave sums that end the kernel are few)
+constexpr int DOT_PER_THREAD = 4;
+template <typename Tv, typename Tr>
+__global__ __launch_bounds__(256) void k_cg_update_dot(Tv *x, const Tr *p, Tr *r, const Tr *q, const double *sc, int ia,
+                                                       int ib, long N, double *partials) {
+    const double alpha = sc[ia] / sc[ib];
+    double d = 0.0;
+#pragma unroll
+    for (int u = 0; u < DOT_PER_THREAD; u++) {
+        const long i = (blockIdx.x * (long)DOT_PER_THREAD + u) * 256L + threadIdx.x;
+        if (i < N) {
+            x[i] = (Tv)((double)x[i] + alpha * (double)p[i]);
+            const Tr rn = (Tr)((double)r[i] - alpha * (double)q[i]);
+            r[i] = rn;
+            d += (double)rn * (double)rn;
+        }
+    }
+    d = wave_sum(d);
+    if ((threadIdx.x & 63) == 0) partials[blockIdx.x * 4L + (threadIdx.x >> 6)] = d;
+}
+
+// out[0] = the sum of a's na partials, out[1] = that of b's nb, in a fixed order: DOT_FINISH_BLOCKS blocks per sum each
+// take a contiguous share of the partials into stage[], and the last block to finish (a device counter, reset by it)
+// sums the stages in block order.
+// each wave's share of sum a[i] b[i] into partials[block * 4 + wave] (a dot at full-grid parallelism)
+template <typename T>
+__global__ __launch_bounds__(256) void k_dot_partials(const T *a, const T *b, long N, double *partials) {
+    double d = 0.0;
+#pragma unroll
+    for (int u = 0; u < DOT_PER_THREAD; u++) {
+        const long i = (blockIdx.x * (long)DOT_PER_THREAD + u) * 256L + threadIdx.x;
+        if (i < N) d += (double)a[i] * (double)b[i];
+    }
+    d = wave_sum(d);
+    if ((threadIdx.x & 63) == 0) partials[blockIdx.x * 4L + (threadIdx.x >> 6)] = d;
+}
+
+constexpr int DOT_FINISH_BLOCKS = 32;
+__global__ __launch_bounds__(256) void k_dot_finish(const double *a, long na, const double *b, long nb, double *out,
+                                                    double *stage, unsigned *counter) {
+    __shared__ double sh[4];
+    __shared__ bool last;
+    const int which = blockIdx.x / DOT_FINISH_BLOCKS, part = blockIdx.x % DOT_FINISH_BLOCKS;
+    const double *src = which == 0 ? a : b;
+    const long cnt = which == 0 ? na : nb;
+    const long per = (cnt + DOT_FINISH_BLOCKS - 1) / DOT_FINISH_BLOCKS;
+    const long lo = part * per, hi = lo + per < cnt ? lo + per : cnt;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    long p = lo + threadIdx.x;
+    for (; p + 3 * 256 < hi; p += 4 * 256) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) acc[u] += src[p + u * 256];
+    }
+    for (int u = 0; p < hi; p += 256, u++) acc[u & 3] += src[p];
+    const double t = block_sum256((acc[0] + acc[1]) + (acc[2] + acc[3]), sh);
+    if (threadIdx.x == 0) {
+        stage[blockIdx.x] = t;
+        __threadfence();
+        last = atomicAdd(counter, 1u) == gridDim.x - 1u;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (threadIdx.x < gridDim.x / DOT_FINISH_BLOCKS) {
+        double s = 0.0;
+        for (int k = 0; k < DOT_FINISH_BLOCKS; k++) s += stage[threadIdx.x * DOT_FINISH_BLOCKS + k];
+        out[threadIdx.x] = s;
+    }
+    if (threadIdx.x == 0) *counter = 0u;
+}
+
 template <typename Tv, typename Tr>
 void cg_update(Tv *x, const Tr *p, Tr *r, const Tr *q, const double *sc, int ia, int ib, long N, hipStream_t s) {
     hipLaunchKernelGGL((k_cg_update<Tv, Tr>), LIRA_G1(N), 0, s, x, p, r, q, sc, ia, ib, N);
     LIRA_HIP(hipGetLastError());
+}
+
+template <typename Tv, typename Tr>
+long cg_update_dot(Tv *x, const Tr *p, Tr *r, const Tr *q, const double *sc, int ia, int ib, long N, double *partials,
+                   hipStream_t s) {
+    const long blocks = (N + 256L * DOT_PER_THREAD - 1) / (256L * DOT_PER_THREAD);
+    hipLaunchKernelGGL((k_cg_update_dot<Tv, Tr>), dim3((unsigned)blocks), dim3(256), 0, s, x, p, r, q, sc, ia, ib, N,
+                       partials);
+    LIRA_HIP(hipGetLastError());
+    return blocks * 4;
+}
+size_t dot_finish_scratch_doubles() { return 2 * DOT_FINISH_BLOCKS + 1; }
+void dot_finish(const double *a, long na, const double *b, long nb, double *out, double *scratch, hipStream_t s) {
+    // scratch: the stage sums of the (one or two: b == nullptr for one) sums, then the counter (zero between calls)
+    const int sums = b ? 2 : 1;
+    hipLaunchKernelGGL(k_dot_finish, dim3(sums * DOT_FINISH_BLOCKS), dim3(256), 0, s, a, na, b, nb, out, scratch,
+                       reinterpret_cast<unsigned *>(scratch + 2 * DOT_FINISH_BLOCKS));
+    LIRA_HIP(hipGetLastError());
+}
+template <typename T> long dot_partials(const T *a, const T *b, long N, double *partials, hipStream_t s) {
+    const long blocks = (N + 256L * DOT_PER_THREAD - 1) / (256L * DOT_PER_THREAD);
+    hipLaunchKernelGGL(k_dot_partials<T>, dim3((unsigned)blocks), dim3(256), 0, s, a, b, N, partials);
+    LIRA_HIP(hipGetLastError());
+    return blocks * 4;
 }
 
 template <typename T> __global__ void k_cg_direction(T *p, const T *z, const double *sc, int ia, int ib, long N) {
@@ -762,6 +1012,11 @@ template <typename T> void add_into_f64(double *x, const T *e, long N, hipStream
     template void mg_restrict<T>(T *, const T *, int, int, hipStream_t);                                      \
     template void mg_prolong_add<T>(T *, const T *, int, int, hipStream_t);                                   \
     template void mg_jacobi0<T>(T *, const T *, int, double, double, double, double, hipStream_t);            \
+    template long mg_jacobi_dot<T>(T *, const T *, const T *, int, double, double, double, double, double *,  \
+                                   hipStream_t);                                                              \
+    template long cg_update_dot<T, T>(T *, const T *, T *, const T *, const double *, int, int, long, double *, \
+                                      hipStream_t);                                                           \
+    template long dot_partials<T>(const T *, const T *, long, double *, hipStream_t);                          \
     template void mg_coarse<T>(T *, const T *, const MgCoarse &, hipStream_t);                                \
     template void cg_direction<T>(T *, const T *, const double *, int, int, long, hipStream_t);               \
     template void add_into_f64<T>(double *, const T *, long, hipStream_t);

@@ -227,6 +227,19 @@ void multipole_moments(const T *rho, int n, double h, double *work, const double
 template <typename T>
 void poisson_rhs(double *b, const T *rho, const double *q, double *ghost, int n, double h, hipStream_t s);
 // multigrid level operators (7-point A2 with boundary-distance ghost factors)
+// the last post-smoothing sweep of the finest level with the block partials of sum b dst (returns the partial count);
+// the CG update with the block partials of the new r.r; their two sums into out[0], out[1] in a fixed order
+template <typename T>
+long mg_jacobi_dot(T *dst, const T *x, const T *b, int n, double h2, double omega, double g_low, double g_high,
+                   double *partials, hipStream_t s);
+long mg_dot_blocks(int n);  // the partial count of mg_jacobi_dot on an n^3 level (one per wave)
+template <typename Tv, typename Tr>
+long cg_update_dot(Tv *x, const Tr *p, Tr *r, const Tr *q, const double *sc, int ia, int ib, long N, double *partials,
+                   hipStream_t s);
+void dot_finish(const double *a, long na, const double *b, long nb, double *out, double *scratch, hipStream_t s);
+// the per-wave partials of a . b (returns their count; 4 per 256 points), finished by dot_finish
+template <typename T> long dot_partials(const T *a, const T *b, long N, double *partials, hipStream_t s);
+size_t dot_finish_scratch_doubles();  // dot_finish's scratch (zeroed once before the first call)
 template <typename T>
 void mg_jacobi(T *dst, const T *x, const T *b, int n, double h2, double omega, double g_low, double g_high,
                hipStream_t s);

@@ -582,6 +582,7 @@ template <typename T> class Solver final : public SolverIface {
     MgCoarse mgc_{};
     DevBuf pb_, pr64_, ptmp64_, pr_, pz_, pp_, pq_, pe_, psc_;
     DevBuf pmom_, pghost_, pcoef_;  // the moments' row/plane sums, the boundary ghost layers, R_lm's monomial table
+    DevBuf pdot_;                   // the CG's r.z and r.r block partials (from the kernels that produce r and z)
     bool poisson_ready_ = false;
     int cg_total_ = 0;
     // SCF
@@ -829,7 +830,7 @@ template <typename T> class Solver final : public SolverIface {
     }
     // Poisson
     void poisson_setup();
-    void vcycle(int l, T *x, const T *rhs);
+    void vcycle(int l, T *x, const T *rhs, double *dot_partials = nullptr);
     PoissonOut poisson(const T *rho, double *v, double tol, int max_iter);
     int pcg(const T *r0, T *e, double rel_tol, int max_iter, double *rel_out);
     // LOBPCG on a channel's X (k columns) for H = -1/2 lap + v
@@ -886,6 +887,11 @@ template <typename T> void Solver<T>::poisson_setup() {
     pmom_.alloc(moment_work_doubles(n_) * sizeof(double));
     pghost_.alloc((size_t)24 * n_ * n_ * sizeof(double));
     {
+        const size_t cnt = (size_t)(mg_dot_blocks(n_) + 4 * ((N_ + 255) / 256)) + dot_finish_scratch_doubles();
+        pdot_.alloc(cnt * sizeof(double));
+        LIRA_HIP(hipMemset(pdot_.ptr, 0, cnt * sizeof(double)));  // the finish's counter starts at zero
+    }
+    {
         std::vector<double> coef(MOMENT_COEFS);
         moment_coefficients(coef.data());
         pcoef_.alloc(coef.size() * sizeof(double));
@@ -937,7 +943,9 @@ template <typename T> void Solver<T>::poisson_setup() {
 }
 
 // one V-cycle from level l: x = M_l^{-1} rhs (crates/qm-gpu/src/multigrid.rs:165-181)
-template <typename T> void Solver<T>::vcycle(int l, T *x, const T *rhs) {
+// dot_partials (finest level only, never the fused coarse kernel): the last post-smoothing sweep also leaves the
+// block partials of rhs . x
+template <typename T> void Solver<T>::vcycle(int l, T *x, const T *rhs, double *dot_partials) {
     if (l == mg_entry_) {
         mg_coarse<T>(x, rhs, mgc_, stream_);  // this level and every coarser one in one workgroup
         return;
@@ -947,26 +955,27 @@ template <typename T> void Solver<T>::vcycle(int l, T *x, const T *rhs) {
     const double h2 = L.h * L.h, gl = 1.0 - 1.0 / L.d_low, gh = 1.0 - 1.0 / L.d_high;
     T *tmp = L.tmp.template as<T>();
     // x = 0, then sweeps in pairs; the first sweep of the first pair reads no x (the zero fill folded into it)
-    auto smooth = [&](int sweeps, bool from_zero) {
+    auto smooth = [&](int sweeps, bool from_zero, double *dot) {
         for (int s = 0; s < sweeps; s += 2) {
             if (from_zero && s == 0) mg_jacobi0<T>(tmp, rhs, L.n, h2, MG_OMEGA, gl, gh, stream_);
             else mg_jacobi<T>(tmp, x, rhs, L.n, h2, MG_OMEGA, gl, gh, stream_);
-            mg_jacobi<T>(x, tmp, rhs, L.n, h2, MG_OMEGA, gl, gh, stream_);
+            if (dot && s + 2 >= sweeps) mg_jacobi_dot<T>(x, tmp, rhs, L.n, h2, MG_OMEGA, gl, gh, dot, stream_);
+            else mg_jacobi<T>(x, tmp, rhs, L.n, h2, MG_OMEGA, gl, gh, stream_);
         }
     };
     if (l + 1 == (int)levels_.size()) {
         if (MG_COARSE == 0) fill<T>(x, Nl, 0.0, stream_);
-        smooth(MG_COARSE, true);
+        smooth(MG_COARSE, true, dot_partials);
         return;
     }
     if (MG_PRE == 0) fill<T>(x, Nl, 0.0, stream_);
-    smooth(MG_PRE, true);
+    smooth(MG_PRE, true, nullptr);
     mg_residual<T>(L.res.template as<T>(), x, rhs, L.n, h2, gl, gh, stream_);
     Level &C = levels_[l + 1];
     mg_restrict<T>(C.b.template as<T>(), L.res.template as<T>(), C.n, L.n, stream_);
     vcycle(l + 1, C.x.template as<T>(), C.b.template as<T>());
     mg_prolong_add<T>(x, C.x.template as<T>(), L.n, C.n, stream_);
-    smooth(MG_POST, false);
+    smooth(MG_POST, false, dot_partials);
 }
 
 // Preconditioned CG in precision T on A e = r0 (A = -lap, 8th order, Dirichlet) from e = 0, with the V-cycle as M.
@@ -1018,24 +1027,39 @@ template <typename T> int Solver<T>::pcg(const T *r0, T *e, double rel_tol, int 
         // one iteration; cur (which rz slot is current) alternates, so two captured forms
         run_graph(pcg_graph_[cur], key((uintptr_t)e, (uintptr_t)cur), [&] {
             stencil<T>(ST_LAP, sp, p, q, nullptr, nullptr, nullptr, stream_);  // q = A p = -lap p
-            RedJobs<T> jpq{};
-            jpq.a[0] = p;
-            jpq.b[0] = q;
-            jpq.kind[0] = RED_DOT;
-            jpq.count = 1;
-            reduce_jobs<T>(jpq, n_, 1.0, partials_.as<double>(), sc + 2, stream_);
-            cg_update<T, T>(e, p, r, q, sc, 4 * cur, 2, N_, stream_);
-            vcycle(0, z, r);
-            RedJobs<T> jn{};
-            jn.a[0] = r;
-            jn.b[0] = z;
-            jn.kind[0] = RED_DOT;
-            jn.a[1] = r;
-            jn.b[1] = r;
-            jn.kind[1] = RED_DOT;
-            jn.count = 2;
-            // the new r.z and r.r into the other slot pair (consecutive slots, as the reduction writes them)
-            reduce_jobs<T>(jn, n_, 1.0, partials_.as<double>(), sc + 4 * (1 - cur), stream_);
+            if (mg_entry_ > 0) {
+                double *pr = pdot_.as<double>(), *pz = pr + 4 * ((N_ + 255) / 256);
+                const long np = dot_partials<T>(p, q, N_, pr, stream_);
+                dot_finish(pr, np, nullptr, 0, sc + 2, pz + mg_dot_blocks(n_), stream_);
+            } else {
+                RedJobs<T> jpq{};
+                jpq.a[0] = p;
+                jpq.b[0] = q;
+                jpq.kind[0] = RED_DOT;
+                jpq.count = 1;
+                reduce_jobs<T>(jpq, n_, 1.0, partials_.as<double>(), sc + 2, stream_);
+            }
+            if (mg_entry_ > 0) {
+                // the new r.r and r.z as block partials of the kernels that write r and z, summed in one launch
+                double *pr = pdot_.as<double>(), *pz = pr + 4 * ((N_ + 255) / 256);
+                double *fin = pz + mg_dot_blocks(n_);
+                const long nr = cg_update_dot<T, T>(e, p, r, q, sc, 4 * cur, 2, N_, pr, stream_);
+                vcycle(0, z, r, pz);
+                dot_finish(pz, mg_dot_blocks(n_), pr, nr, sc + 4 * (1 - cur), fin, stream_);
+            } else {
+                cg_update<T, T>(e, p, r, q, sc, 4 * cur, 2, N_, stream_);
+                vcycle(0, z, r);
+                RedJobs<T> jn{};
+                jn.a[0] = r;
+                jn.b[0] = z;
+                jn.kind[0] = RED_DOT;
+                jn.a[1] = r;
+                jn.b[1] = r;
+                jn.kind[1] = RED_DOT;
+                jn.count = 2;
+                // the new r.z and r.r into the other slot pair (consecutive slots, as the reduction writes them)
+                reduce_jobs<T>(jn, n_, 1.0, partials_.as<double>(), sc + 4 * (1 - cur), stream_);
+            }
             cg_direction<T>(p, z, sc, 4 * (1 - cur), 4 * cur, N_, stream_);
         });
         const int slot = it & 1;
