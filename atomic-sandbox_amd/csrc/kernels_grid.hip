@@ -4,9 +4,11 @@
 #include "lira_internal.hpp"
 #include "lsda.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <type_traits>
+#include <vector>
 
 namespace lira {
 
@@ -1088,8 +1090,8 @@ constexpr int G2_TI = 32, G2_TJ = 64;
 // (consecutive lanes on consecutive points of one column: 16 lanes read 16 contiguous values), loaded into registers
 // one stage ahead so their latency hides behind the 32 MFMAs of the current stage. The four waves take interleaved
 // stages of the workgroup's chunk; their 32 x 64 accumulators are summed in a fixed order at the end.
-constexpr int G2_S = 16;                             // points per wave stage
-constexpr int G2_LOADS = (G2_TI + G2_TJ) * G2_S / 64;  // elements per lane per stage
+constexpr int G2_S = 16;  // points per wave stage
+constexpr int G2_ZERO_ROW = G2_TI + G2_TJ;  // a slab row that stays zero (the B operand of a U column past q)
 
 // Every load is issued unconditionally and its value is not touched until the stage is staged: a missing column or a
 // point past the chunk reads a zero constant instead. (A load under a branch, or a select on the loaded value, makes
@@ -1102,12 +1104,12 @@ __device__ const double lira_gram_zero_d = 0.0;
 __device__ const float *GramZero<float>::ptr() { return &lira_gram_zero_f; }
 __device__ const double *GramZero<double>::ptr() { return &lira_gram_zero_d; }
 
-template <typename T>
-__device__ inline void gram_fetch(const T *const *cols, long p0, long p_end, T (&v)[G2_LOADS]) {
+template <typename T, int LOADS>
+__device__ inline void gram_fetch(const T *const *cols, long p0, long p_end, T (&v)[LOADS]) {
     const int lane = threadIdx.x & 63;
     const T *zero = GramZero<T>::ptr();
 #pragma unroll
-    for (int u = 0; u < G2_LOADS; u++) {
+    for (int u = 0; u < LOADS; u++) {
         const int e = lane + 64 * u;
         const int col = e >> 4, pp = e & 15;
         const long p = p0 + pp;
@@ -1123,12 +1125,16 @@ __host__ __device__ inline bool gram_mirrored(int sym, int m, int i, int j) {
     return sym && cb < rb && 16 * (cb + 1) <= m;
 }
 
-template <typename T>
+// The slab holds the tile's 32 S columns in rows 0..31 and, packed after them, only the U columns that are neither past
+// q nor one of the tile's S columns (U = [S, R, HW, HP] in LOBPCG): each U column reads its B operand from its packed
+// row, its S row, or the zero row. LOADS = 8 (the S rows) + a quarter of the packed U columns, rounded up to a
+// multiple of 4: the elements each lane loads per 16-point stage.
+template <typename T, int LOADS>
 __global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q, long N, long chunk,
                                               double *partials, int sym) {
-    __shared__ double slab[4][G2_TI + G2_TJ][G2_S + 1];  // one slice per wave; reused for the final reduction
+    __shared__ double slab[4][G2_TI + G2_TJ + 1][G2_S + 1];  // one slice per wave; reused for the final reduction
     __shared__ const T *cols[G2_TI + G2_TJ];
-    __shared__ int urow[G2_TJ];  // the slab row each U column is read from: its own, or the S row of the same column
+    __shared__ int urow[G2_TJ];  // the slab row each U column's B operand is read from
     const int nbj = (q + G2_TJ - 1) / G2_TJ;
     const int bi = blockIdx.y / nbj, bj = blockIdx.y - bi * nbj;
     const int i0 = bi * G2_TI, j0 = bj * G2_TJ;
@@ -1147,27 +1153,27 @@ __global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q
         }
     if (!any) return;
     if (t < G2_TI) cols[t] = i0 + t < m ? static_cast<const T *>(col_base(S, i0 + t, N, sizeof(T))) : nullptr;
-    if (t >= 64 && t < 64 + G2_TJ) {
-        const int c = t - 64;
-        cols[G2_TI + c] = j0 + c < q ? static_cast<const T *>(col_base(U, j0 + c, N, sizeof(T))) : nullptr;
-    }
+    if (t >= 64 + G2_TI && t < 64 + G2_TI + G2_TJ) cols[t - 64] = nullptr;  // packed U rows default to zero loads
     __syncthreads();
-    // a U column that is also one of this tile's S columns (U = [S, R, HW, HP] in LOBPCG) is not loaded again: the
-    // MFMA reads it from the S row
-    if (t >= 64 && t < 64 + G2_TJ) {
-        const int c = t - 64;
-        const T *uc = cols[G2_TI + c];
-        int row = G2_TI + c;
+    if (wave == 1) {
+        // U column c = lane: an alias of one of the tile's S columns, a real column (packed in ballot order), or none
+        const int c = lane;
+        const T *uc = j0 + c < q ? static_cast<const T *>(col_base(U, j0 + c, N, sizeof(T))) : nullptr;
+        int srow = -1;
         if (uc)
             for (int i = 0; i < G2_TI; i++)
                 if (cols[i] == uc) {
-                    row = i;
+                    srow = i;
                     break;
                 }
-        urow[c] = row;
+        const bool real = uc != nullptr && srow < 0;
+        const unsigned long long mask = __ballot(real);
+        const int rank = __popcll(mask & ((1ull << c) - 1ull));
+        urow[c] = srow >= 0 ? srow : real ? G2_TI + rank : G2_ZERO_ROW;
+        if (real) cols[G2_TI + rank] = uc;
     }
-    __syncthreads();
-    if (t >= 64 && t < 64 + G2_TJ && urow[t - 64] < G2_TI) cols[G2_TI + t - 64] = nullptr;
+    // the zero row of every wave's slice
+    if (lane < G2_S) slab[wave][G2_ZERO_ROW][lane] = 0.0;
     lira_d4 acc[2][4];
 #pragma unroll
     for (int a = 0; a < 2; a++)
@@ -1178,24 +1184,24 @@ __global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q
     __syncthreads();
     double(*sl)[G2_S + 1] = slab[wave];
     // two stage buffers in registers: the loads of the stage after next are in flight while a stage is consumed
-    T va[G2_LOADS], vb[G2_LOADS];
+    T va[LOADS], vb[LOADS];
     long p0 = p_begin + (long)wave * G2_S;
-    gram_fetch<T>(cols, p0, p_end, va);
-    gram_fetch<T>(cols, p0 + 4 * G2_S, p_end, vb);
+    gram_fetch<T, LOADS>(cols, p0, p_end, va);
+    gram_fetch<T, LOADS>(cols, p0 + 4 * G2_S, p_end, vb);
     const int r16 = lane & 15, k4 = lane >> 4;
     int brow[4];
 #pragma unroll
     for (int b = 0; b < 4; b++) brow[b] = urow[16 * b + r16];
     // one stage: stage v into the wave's slice, refill v with the stage two ahead (past the chunk: zeros), then the
     // MFMAs of the staged points
-    auto stage = [&](T(&v)[G2_LOADS], long pn) {
+    auto stage = [&](T(&v)[LOADS], long pn) {
 #pragma unroll
-        for (int u = 0; u < G2_LOADS; u++) {
+        for (int u = 0; u < LOADS; u++) {
             const int e = lane + 64 * u;
             sl[e >> 4][e & 15] = (double)v[u];
         }
         __builtin_amdgcn_wave_barrier();
-        gram_fetch<T>(cols, pn, p_end, v);
+        gram_fetch<T, LOADS>(cols, pn, p_end, v);
 #pragma unroll
         for (int kk = 0; kk < G2_S; kk += 4) {
             const double a0 = sl[r16][kk + k4];
@@ -1306,7 +1312,25 @@ void gram(const Cols &S, const Cols &U, long N, double scale, double *partials, 
     int sym = q >= m ? 1 : 0;
     for (int j = 0; j < m && sym; j++)
         if (col_host(sp, j, N, sizeof(T)) != col_host(up, j, N, sizeof(T))) sym = 0;
-    hipLaunchKernelGGL(k_gram<T>, dim3(chunks, blocks), dim3(256), 0, s, sp, up, m, q, N, chunk, partials, sym);
+    // the packed U columns of the fullest tile: those neither past q nor among the tile's S columns
+    int packed = 0;
+    const int nbi = (m + G2_TI - 1) / G2_TI, nbj = (q + G2_TJ - 1) / G2_TJ;
+    for (int bi = 0; bi < nbi; bi++) {
+        std::vector<const void *> sc;
+        for (int i = bi * G2_TI; i < std::min(m, (bi + 1) * G2_TI); i++) sc.push_back(col_host(sp, i, N, sizeof(T)));
+        for (int bj = 0; bj < nbj; bj++) {
+            int cnt = 0;
+            for (int j = bj * G2_TJ; j < std::min(q, (bj + 1) * G2_TJ); j++)
+                if (std::find(sc.begin(), sc.end(), col_host(up, j, N, sizeof(T))) == sc.end()) cnt++;
+            packed = std::max(packed, cnt);
+        }
+    }
+    const int loads = 8 + (packed + 15) / 16 * 4;  // 12, 16, 20 or 24 elements per lane per stage
+    const dim3 grid(chunks, blocks);
+    if (loads <= 12) hipLaunchKernelGGL((k_gram<T, 12>), grid, dim3(256), 0, s, sp, up, m, q, N, chunk, partials, sym);
+    else if (loads <= 16) hipLaunchKernelGGL((k_gram<T, 16>), grid, dim3(256), 0, s, sp, up, m, q, N, chunk, partials, sym);
+    else if (loads <= 20) hipLaunchKernelGGL((k_gram<T, 20>), grid, dim3(256), 0, s, sp, up, m, q, N, chunk, partials, sym);
+    else hipLaunchKernelGGL((k_gram<T, 24>), grid, dim3(256), 0, s, sp, up, m, q, N, chunk, partials, sym);
     LIRA_HIP(hipGetLastError());
     const long mq = (long)m * q;
     hipLaunchKernelGGL(k_gram_finish, dim3((unsigned)((mq + 3) / 4)), dim3(256), 0, s, partials, chunks, m, q, scale, G,
