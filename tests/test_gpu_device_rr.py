@@ -86,8 +86,9 @@ def test_device_step_drops_dependent_directions(lira_mod):
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_scf_with_the_device_step_equals_the_host_step(lira_mod, precision):
-    """Water-shaped all-electron SCF (Fermi-Dirac, 41^3 x 0.3): the device step and the host step converge to the same
-    state: FP64 energies within 1e-9 Ha and levels within 1e-7; mixed within the f32 path's own noise (2e-6 Ha)."""
+    """Water-shaped all-electron SCF (Fermi-Dirac, 41^3 x 0.3, delta_rho < 1e-6): the device step and the host step
+    converge to the same state: FP64 energies within 1e-9 Ha (second order in the density), levels within 2e-6 Ha and
+    forces within 1e-5 Ha/Bohr (first order, at the SCF tolerance); mixed within the f32 path's own noise."""
     import math
     lira = lira_mod
     n, h = 41, 0.3
@@ -111,6 +112,6 @@ def test_scf_with_the_device_step_equals_the_host_step(lira_mod, precision):
     df = np.abs(out[True][4] - out[False][4]).max()
     print(f"dE {de:.2e}, max d eps {deps:.2e}, max dF {df:.2e}")
     if precision == 0:
-        assert de < 1e-9 and deps < 1e-7 and df < 1e-6
+        assert de < 1e-9 and deps < 2e-6 and df < 1e-5
     else:
         assert de < 2e-6 and deps < 2e-5 and df < 1e-4
