@@ -1015,6 +1015,7 @@ template <typename T> int Solver<T>::pcg(const T *r0, T *e, double rel_tol, int 
     }
     const double r0n = std::sqrt(r00);
     StencilParams sp{n_, 1, h_, -1.0, 0.0, 0.0};
+    const int lap_variant = variant(ST_LAP, 1);  // tuned on first use, outside any graph capture
     int cur = 0, it = 0;
     double rel = 1.0;
     hipEvent_t ev[2];
@@ -1026,7 +1027,7 @@ template <typename T> int Solver<T>::pcg(const T *r0, T *e, double rel_tol, int 
     while (!done && it < max_iter) {
         // one iteration; cur (which rz slot is current) alternates, so two captured forms
         run_graph(pcg_graph_[cur], key((uintptr_t)e, (uintptr_t)cur), [&] {
-            stencil<T>(ST_LAP, sp, p, q, nullptr, nullptr, nullptr, stream_);  // q = A p = -lap p
+            stencil<T>(ST_LAP, sp, p, q, nullptr, nullptr, nullptr, stream_, lap_variant);  // q = A p = -lap p
             if (mg_entry_ > 0) {
                 double *pr = pdot_.as<double>(), *pz = pr + 4 * ((N_ + 255) / 256);
                 const long np = dot_partials<T>(p, q, N_, pr, stream_);
