@@ -289,7 +289,12 @@ def lib():
 def _raise(status: int, handle=None, what: str = ""):
     if status == OK:
         return
-    msg = lib().lira_last_error(handle).decode() if handle is not None else ""
+    if handle is not None:
+        msg = lib().lira_last_error(handle).decode()
+    elif what == "lira_solver_create":  # a failed creation leaves its reason in the library's creation error
+        msg = lib().lira_last_error(None).decode()
+    else:
+        msg = ""
     raise _ERRORS.get(status, QmError)(f"{what}: {msg}" if what else msg)
 
 
