@@ -34,7 +34,8 @@ EXPORTS = [
     "lira_state_summary", "lira_effective_gpu_tol", "lira_evaluate", "lira_solve_batch", "lira_solve_polled",
     "lira_inject_readback_fault", "lira_gram_noalloc", "lira_mg_vcycle", "lira_mg_level_op",
     "lira_solve_footprint_bytes", "lira_witness_cap_for_budget", "lira_unoccupied", "lira_harris",
-    "lira_stream_bandwidth", "lira_rayleigh_ritz_device", "lira_set_device_rr",
+    "lira_stream_bandwidth", "lira_rayleigh_ritz_device", "lira_set_device_rr", "lira_pseudo_generate",
+    "lira_pseudo_error", "lira_pseudo_record_of", "lira_pseudo_species", "lira_pseudo_free",
 ]
 
 
@@ -60,6 +61,15 @@ class Species(C.Structure):
                 ("v_sr_u", C.c_void_p), ("n_v_sr", C.c_int), ("r_sr", C.c_double), ("rho_pc_u", C.c_void_p),
                 ("n_rho_pc", C.c_int), ("pc_radius", C.c_double), ("projectors", C.c_void_p),
                 ("n_projectors", C.c_int), ("r_core", C.c_double)]
+
+
+class PseudoRecord(C.Structure):
+    """lira_pseudo_record (qm::radial::record::GenerationRecord, crates/qm/src/radial/record.rs)."""
+    _fields_ = [("z", C.c_int), ("f", C.c_double), ("local", C.c_int), ("local_l", C.c_int), ("z_v", C.c_double),
+                ("n_core", C.c_int), ("n_channels", C.c_int), ("channel_l", C.c_int * 4), ("r_c", C.c_double * 4),
+                ("eps", C.c_double * 4), ("t5", C.c_double * 4), ("n_projectors", C.c_int), ("proj_l", C.c_int * 3),
+                ("e_kb", C.c_double * 3), ("gss", C.c_double * 3), ("r_beta", C.c_double * 3), ("r_nlc", C.c_double),
+                ("r_core", C.c_double), ("r_cmp", C.c_double), ("r_pc_support", C.c_double), ("h", C.c_double)]
 
 
 class QmError(RuntimeError):
@@ -258,6 +268,13 @@ def lib():
         L.lira_set_stencil_variant.argtypes = [C.c_int, C.c_int]
         L.lira_set_ionic.argtypes = [vp, vp, C.c_int]
         L.lira_free_atom_levels.argtypes = [C.c_int, _dp, C.POINTER(C.c_double)]
+        L.lira_pseudo_generate.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(vp)]
+        L.lira_pseudo_error.argtypes = []
+        L.lira_pseudo_error.restype = C.c_char_p
+        L.lira_pseudo_record_of.argtypes = [vp, C.POINTER(PseudoRecord)]
+        L.lira_pseudo_species.argtypes = [vp, C.POINTER(Species), vp, C.c_int]
+        L.lira_pseudo_free.argtypes = [vp]
+        L.lira_pseudo_free.restype = None
         L.lira_set_witness_cap.argtypes = [vp, C.c_int]
         L.lira_witness_cap.restype = C.c_int
         L.lira_witness_cap.argtypes = [vp]
@@ -681,6 +698,60 @@ def free_atom_levels(z):
     el = C.c_double()
     _raise(lib().lira_free_atom_levels(z, eps, C.byref(el)), None, "lira_free_atom_levels")
     return eps, el.value
+
+
+PSEUDO_LOCAL_HIGHEST, PSEUDO_LOCAL_POLYNOMIAL = 0, 1
+F_LADDER = (1.6, 1.4, 1.2, 1.0)
+
+
+class PseudoProjector:
+    """One filtered KB projector (FilteredProjector, crates/qm/src/radial/filter.rs:323): l, E_KB, radius, t_u."""
+
+    def __init__(self, l, e_kb, radius, t_u):
+        self.l, self.e_kb, self.radius, self.t_u = l, e_kb, radius, t_u
+
+
+class PseudoSpecies:
+    """A generated ion filtered for spacing h (qm::radial::filter::FilteredTable, filter.rs:330-338), in the form
+    GpuSolver.set_ionic takes, with its generation record (`record`, a dict of lira_pseudo_record's fields)."""
+
+    def __init__(self, handle):
+        rec = PseudoRecord()
+        _raise(lib().lira_pseudo_record_of(handle, C.byref(rec)), None, "lira_pseudo_record_of")
+        sp = Species()
+        projs = (Projector * 3)()
+        _raise(lib().lira_pseudo_species(handle, C.byref(sp), C.cast(projs, C.c_void_p), 3), None, "lira_pseudo_species")
+        copy = lambda ptr, n: np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_double)), shape=(n,)).copy()
+        self.z, self.z_v, self.sigma_lr, self.du = sp.z, sp.z_v, sp.sigma_lr, sp.du
+        self.v_sr_u, self.r_sr = copy(sp.v_sr_u, sp.n_v_sr), sp.r_sr
+        self.rho_pc_u = copy(sp.rho_pc_u, sp.n_rho_pc) if sp.rho_pc_u else None
+        self.pc_radius = sp.pc_radius
+        self.projectors = [PseudoProjector(p.l, p.e_kb, p.radius, copy(p.t_u, p.n_t)) for p in projs[:sp.n_projectors]]
+        self.r_core = sp.r_core
+        nc, npj = rec.n_channels, rec.n_projectors
+        self.record = {
+            "z": rec.z, "f": rec.f, "local_kind": rec.local,
+            "local_l": rec.local_l, "z_v": rec.z_v, "n_core": rec.n_core,
+            "channels": [(rec.channel_l[k], rec.r_c[k], rec.eps[k], rec.t5[k]) for k in range(nc)],
+            "projectors": [(rec.proj_l[k], rec.e_kb[k], rec.gss[k], rec.r_beta[k]) for k in range(npj)],
+            "r_nlc": rec.r_nlc if rec.r_nlc > 0 else None, "r_core": rec.r_core, "r_cmp": rec.r_cmp,
+            "r_pc_support": rec.r_pc_support, "h": rec.h,
+        }
+        self.record["local"] = "polynomial" if rec.local == PSEUDO_LOCAL_POLYNOMIAL else "spdf"[rec.local_l] + "-local"
+
+
+def generate_pseudo(z: int, h: float, f_index: int = -1, local: int = -1) -> PseudoSpecies:
+    """qm::radial::generate(z) filtered for spacing h (lira_pseudo_generate). Raises Numerical with the reason when
+    the element is refused."""
+    handle = C.c_void_p()
+    st = lib().lira_pseudo_generate(int(z), float(h), int(f_index), int(local), C.byref(handle))
+    if st != 0:
+        why = lib().lira_pseudo_error().decode()
+        raise {4: Numerical, 6: Invalid}.get(st, QmError)(f"lira_pseudo_generate: {why}")
+    try:
+        return PseudoSpecies(handle)
+    finally:
+        lib().lira_pseudo_free(handle)
 
 
 def rayleigh_ritz(a, b, nk):

@@ -212,6 +212,52 @@ typedef struct {
  * copied. A warm state converged under another setup is not reused (scf.rs:771-777). The caller passes the valence
  * electron counts in the lira_system it solves, as System::with_ionic does (scf.rs:362-365). */
 lira_status lira_set_ionic(lira_solver *s, const lira_species *species, int n_species);
+
+/* ---- the pseudopotential generator (qm::radial, crates/qm/src/radial/recipe.rs:140-176; host code, f64) ----
+ * Z in, a norm-conserving Troullier-Martins / Kleinman-Bylander ion with the LFC partial core out, filtered for a grid
+ * of spacing h (qm::radial::filter::filtered, filter.rs:384). The ladder tries f in {1.6, 1.4, 1.2, 1.0} (softest
+ * first) and, per f, the highest generated channel as the local potential, then the polynomial local
+ * (ledger.rs:181-182); the first entry that passes the gates restated here (G1, kb.rs:611-634; T5, gates.rs:313-325)
+ * is kept. f_index (0..3) and local (0, 1) pin one entry instead (-1: the ladder). Z in 1..18. */
+typedef enum { LIRA_PSEUDO_LOCAL_HIGHEST = 0, LIRA_PSEUDO_LOCAL_POLYNOMIAL = 1 } lira_pseudo_local;
+
+/* The generation record (qm::radial::record::GenerationRecord, crates/qm/src/radial/record.rs; the columns of the
+ * reference's pinned records table): f, the local candidate, r_c / eps / T5 per generated channel, E_KB, GSS and the
+ * support r_beta per projector, r_nlc (0: no core), r_core, r_cmp, the partial core's support, and h. */
+typedef struct {
+    int z;
+    double f;
+    int local;   /* lira_pseudo_local */
+    int local_l; /* the channel used as V_loc (-1: the polynomial local) */
+    double z_v;
+    int n_core;
+    int n_channels;
+    int channel_l[4];
+    double r_c[4];
+    double eps[4];
+    double t5[4]; /* max |delta E_equiv| over eps_l +- 0.5 Ha (Ha) */
+    int n_projectors;
+    int proj_l[3];
+    double e_kb[3];
+    double gss[3];
+    double r_beta[3];
+    double r_nlc;
+    double r_core;
+    double r_cmp;
+    double r_pc_support;
+    double h;
+} lira_pseudo_record;
+
+typedef struct lira_pseudo lira_pseudo;
+/* qm::radial::generate + filtered (recipe.rs:116, filter.rs:384). LIRA_NUMERICAL with lira_pseudo_error()'s reason
+ * when the element is refused (no ladder entry passes, two valence states of one l, a failed radial solve). */
+lira_status lira_pseudo_generate(int z, double h, int f_index, int local, lira_pseudo **out);
+const char *lira_pseudo_error(void);
+lira_status lira_pseudo_record_of(const lira_pseudo *p, lira_pseudo_record *rec);
+/* The filtered tables as a lira_species for lira_set_ionic (IonicSetup::from_tables, crates/qm/src/ionic.rs); sp and
+ * proj[0..n_projectors) point into p, which must outlive them (lira_set_ionic copies them). */
+lira_status lira_pseudo_species(const lira_pseudo *p, lira_species *sp, lira_projector *proj, int proj_cap);
+void lira_pseudo_free(lira_pseudo *p);
 /* The five pseudised force terms of the converged state, out[(t * n_nuclei + I) * 3 + a] for t in [ion-ion, local LR,
  * local SR, nonlocal, partial core] (GpuSolver::force_terms, crates/qm-gpu/src/forces.rs:276; qm::forces::
  * pseudo_force_terms, crates/qm/src/forces.rs:259). lira_forces is their per-component sum. */
