@@ -268,33 +268,53 @@ __device__ inline void block_sum_rows(double *acc, int rows, double (*lds)[RMAX]
     __syncthreads();
 }
 
-// C[row_off + r][b] = dv * sum_{p in box} beta_r(p) x_b(p): one block per (ion, band)
+// Partial projections over one chunk of an ion's box: P[((row_off + r) * nb + b) * chunks + chunk] = sum over the
+// chunk's points of beta_r(p) x_b(p). Grid (chunks, ion, band); a chunk is NL_CHUNK consecutive box points, 8 per
+// thread, so the box is spread over many workgroups; chunks past an ion's box write zeros.
+constexpr int NL_CHUNK = 2048;
 template <typename T>
-__global__ __launch_bounds__(256) void k_nl_project(const T *x, int nb, long N, int n, const T *beta,
-                                                    const NlIon *ions, double dv, double *C) {
+__global__ __launch_bounds__(256) void k_nl_project_partial(const T *x, int nb, long N, int n, const T *beta,
+                                                            const NlIon *ions, int chunks, double *P) {
     __shared__ double lds[4][NL_MAX_ROWS];
-    const NlIon &ion = ions[blockIdx.x];
-    const int b = blockIdx.y;
+    const NlIon &ion = ions[blockIdx.y];
+    const int b = blockIdx.z, chunk = blockIdx.x;
     const int rows = ion.rows;
+    const int pts = (int)ion.pts, dx = ion.dim[0], dy = ion.dim[1];
     const T *xb = x + (long)b * N;
+    const T *bb = beta + ion.beta_off;
     double acc[NL_MAX_ROWS];
 #pragma unroll
     for (int r = 0; r < NL_MAX_ROWS; r++) acc[r] = 0.0;
-    for (long p = threadIdx.x; p < ion.pts; p += 256) {
-        const long px = p % ion.dim[0], t = p / ion.dim[0];
-        const long py = t % ion.dim[1], pz = t / ion.dim[1];
-        const long g = (ion.lo[0] + px) + (long)n * ((ion.lo[1] + py) + (long)n * (ion.lo[2] + pz));
-        const double xv = (double)xb[g];
-        const T *bp = beta + ion.beta_off + p;
+    const int p0 = chunk * NL_CHUNK + (int)threadIdx.x;
+#pragma unroll 2
+    for (int k = 0; k < NL_CHUNK / 256; k++) {
+        const int p = p0 + k * 256;
+        if (p < pts) {
+            const int px = p % dx, t = p / dx;
+            const int py = t % dy, pz = t / dy;
+            const long g = (ion.lo[0] + px) + (long)n * ((ion.lo[1] + py) + (long)n * (ion.lo[2] + pz));
+            const double xv = (double)xb[g];
 #pragma unroll
-        for (int r = 0; r < NL_MAX_ROWS; r++)
-            if (r < rows) acc[r] += (double)bp[(long)r * ion.pts] * xv;
+            for (int r = 0; r < NL_MAX_ROWS; r++)
+                if (r < rows) acc[r] += (double)bb[(long)r * pts + p] * xv;
+        }
     }
     block_sum_rows<NL_MAX_ROWS>(acc, rows, lds);
     if ((int)threadIdx.x < rows) {
         const int r = threadIdx.x;
-        C[(long)(ion.row_off + r) * nb + b] = dv * (((lds[0][r] + lds[1][r]) + lds[2][r]) + lds[3][r]);
+        P[((long)(ion.row_off + r) * nb + b) * chunks + chunk] = ((lds[0][r] + lds[1][r]) + lds[2][r]) + lds[3][r];
     }
+}
+
+// C[row * nb + b] = dv * (the row's chunk partials summed in chunk order), one thread per (row, band)
+__global__ __launch_bounds__(256) void k_nl_project_finish(const double *P, int total, int chunks, double dv,
+                                                           double *C) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= total) return;
+    const double *p = P + (long)i * chunks;
+    double s = 0.0;
+    for (int c = 0; c < chunks; c++) s += p[c];
+    C[i] = dv * s;
 }
 
 // out_b(p) += sum_r E_r C[r][b] beta_r(p) over the boxes of the ions of one colour (disjoint boxes: no two threads
@@ -318,11 +338,18 @@ __global__ __launch_bounds__(256) void k_nl_apply(T *out, int nb, long N, int n,
     ob[g] = (T)((double)ob[g] + acc);
 }
 
+int nl_project_chunks(long max_pts) { return (int)std::max(1L, (max_pts + NL_CHUNK - 1) / NL_CHUNK); }
+
 template <typename T>
-void nl_project(const T *x, int nb, long N, int n, const T *beta, const NlIon *ions, int nions, double dv, double *C,
-                hipStream_t s) {
-    if (nions <= 0 || nb <= 0) return;
-    hipLaunchKernelGGL(k_nl_project<T>, dim3(nions, nb), dim3(256), 0, s, x, nb, N, n, beta, ions, dv, C);
+void nl_project(const T *x, int nb, long N, int n, const T *beta, const NlIon *ions, int nions, int rows,
+                long max_pts, double dv, double *C, double *work, hipStream_t s) {
+    if (nions <= 0 || nb <= 0 || rows <= 0) return;
+    const int chunks = nl_project_chunks(max_pts);
+    hipLaunchKernelGGL(k_nl_project_partial<T>, dim3(chunks, nions, nb), dim3(256), 0, s, x, nb, N, n, beta, ions,
+                       chunks, work);
+    LIRA_HIP(hipGetLastError());
+    const int total = rows * nb;
+    hipLaunchKernelGGL(k_nl_project_finish, dim3((total + 255) / 256), dim3(256), 0, s, work, total, chunks, dv, C);
     LIRA_HIP(hipGetLastError());
 }
 
@@ -493,8 +520,8 @@ void nl_force(const T *psi, int nb, long N, int n, double h, const NlIon *ions, 
                                   double, hipStream_t);                                                               \
     template void ionic_fields<T>(T *, T *, int, double, const IonicCentre *, int, const double *, hipStream_t);      \
     template void nl_beta<T>(T *, const NlIon &, int, double, const double *, hipStream_t);                          \
-    template void nl_project<T>(const T *, int, long, int, const T *, const NlIon *, int, double, double *,          \
-                                hipStream_t);                                                                         \
+    template void nl_project<T>(const T *, int, long, int, const T *, const NlIon *, int, int, long, double,         \
+                                double *, double *, hipStream_t);                                                     \
     template void nl_apply<T>(T *, int, long, int, const T *, const NlIon *, const int *, int, long, const double *, \
                               const double *, hipStream_t);                                                           \
     template void ionic_forces<T>(const T *, const T *, const T *, int, double, const IonicCentre *, const double *, \

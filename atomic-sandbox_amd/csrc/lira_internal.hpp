@@ -190,10 +190,13 @@ struct NlIon {
 template <typename T>
 void ionic_fields(T *vloc, T *pc, int n, double h, const IonicCentre *c, int nat, const double *tab, hipStream_t s);
 template <typename T> void nl_beta(T *beta, const NlIon &ion, int n, double h, const double *tab, hipStream_t s);
-// C[(row_off + r) * nb + b] = dv sum_box beta_r x_b for every ion with projectors
+// C[(row_off + r) * nb + b] = dv sum_box beta_r x_b for every ion with projectors (rows in all); the box sums run in
+// chunks of max_pts / nl_project_chunks points, their partials in work (rows * nb * nl_project_chunks(max_pts) doubles)
+// summed in a fixed order
+int nl_project_chunks(long max_pts);
 template <typename T>
-void nl_project(const T *x, int nb, long N, int n, const T *beta, const NlIon *ions, int nions, double dv, double *C,
-                hipStream_t s);
+void nl_project(const T *x, int nb, long N, int n, const T *beta, const NlIon *ions, int nions, int rows,
+                long max_pts, double dv, double *C, double *work, hipStream_t s);
 // out_b += sum_r erow_r C_rb beta_r on the boxes of the listed ions (boxes must be disjoint)
 template <typename T>
 void nl_apply(T *out, int nb, long N, int n, const T *beta, const NlIon *ions, const int *members, int nmembers,

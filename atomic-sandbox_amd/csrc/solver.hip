@@ -605,8 +605,8 @@ template <typename T> class Solver final : public SolverIface {
     // pseudised ions: the setup (tables copied at lira_set_ionic) and the terms of the current geometry
     std::vector<SpeciesHost> species_;
     std::vector<double> tab_h_;
-    DevBuf tab_, centres_, nl_ions_, beta_, nl_C_, nl_erow_, nl_members_, pc_, qf_, fpart_, fout_;
-    size_t nl_C_cap_ = 0;
+    DevBuf tab_, centres_, nl_ions_, beta_, nl_C_, nl_erow_, nl_members_, pc_, qf_, fpart_, fout_, nl_P_;
+    size_t nl_C_cap_ = 0, nl_P_cap_ = 0;
     int ionic_version_ = 0, state_ionic_version_ = -1;
     std::vector<NlIon> nl_h_;
     std::vector<int> nl_atom_;  // the nucleus of each NlIon
@@ -640,6 +640,7 @@ template <typename T> class Solver final : public SolverIface {
     }
     void ionic_geometry(double sigma);
     void nl_apply_block(const T *x, T *out, int nb);
+    void nl_projections(const T *x, int nb);
     std::vector<double> nl_energy_terms(Channel<T> &c, bool fd);
 
     void ensure_host(size_t count) {
@@ -1846,14 +1847,25 @@ template <typename T> void Solver<T>::ionic_geometry(double sigma) {
 }
 
 // out_b += V_NL x_b for a block: the projections C (f64, rows x nb), then the scatter colour by colour
-template <typename T> void Solver<T>::nl_apply_block(const T *x, T *out, int nb) {
+template <typename T> void Solver<T>::nl_projections(const T *x, int nb) {
     const size_t need = (size_t)nl_rows_ * nb * 4;
     if (need > nl_C_cap_) {
         nl_C_.alloc(need * sizeof(double));
         nl_C_cap_ = need;
     }
-    nl_project<T>(x, nb, N_, n_, beta_.template as<T>(), nl_ions_.as<NlIon>(), (int)nl_h_.size(), dv_,
-                  nl_C_.as<double>(), stream_);
+    long max_pts = 0;
+    for (const NlIon &ion : nl_h_) max_pts = std::max(max_pts, ion.pts);
+    const size_t work = (size_t)nl_rows_ * nb * nl_project_chunks(max_pts);
+    if (work > nl_P_cap_) {
+        nl_P_.alloc(work * sizeof(double));
+        nl_P_cap_ = work;
+    }
+    nl_project<T>(x, nb, N_, n_, beta_.template as<T>(), nl_ions_.as<NlIon>(), (int)nl_h_.size(), nl_rows_, max_pts,
+                  dv_, nl_C_.as<double>(), nl_P_.as<double>(), stream_);
+}
+
+template <typename T> void Solver<T>::nl_apply_block(const T *x, T *out, int nb) {
+    nl_projections(x, nb);
     for (const Colour &c : colours_)
         nl_apply<T>(out, nb, N_, n_, beta_.template as<T>(), nl_ions_.as<NlIon>(), nl_members_.as<int>() + c.off,
                     c.count, c.max_pts, nl_C_.as<double>(), nl_erow_.as<double>(), stream_);
@@ -1864,13 +1876,7 @@ template <typename T> void Solver<T>::nl_apply_block(const T *x, T *out, int nb)
 template <typename T> std::vector<double> Solver<T>::nl_energy_terms(Channel<T> &c, bool fd) {
     std::vector<double> terms;
     if (nl_rows_ == 0 || c.k == 0) return terms;
-    const size_t need = (size_t)nl_rows_ * c.k * 4;
-    if (need > nl_C_cap_) {
-        nl_C_.alloc(need * sizeof(double));
-        nl_C_cap_ = need;
-    }
-    nl_project<T>(c.X.template as<T>(), c.k, N_, n_, beta_.template as<T>(), nl_ions_.as<NlIon>(), (int)nl_h_.size(),
-                  dv_, nl_C_.as<double>(), stream_);
+    nl_projections(c.X.template as<T>(), c.k);
     const double *C = readback(nl_C_.as<double>(), (size_t)nl_rows_ * c.k);
     for (int b = 0; b < c.k; b++) {
         const double f = fd ? c.occ[b] : 1.0;

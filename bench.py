@@ -108,16 +108,16 @@ def poisson_256(lira, n=256, h=0.15, seed=3):
 
 
 def pseudo_water(lira, n=95, h=0.15):
-    """Water with the O pseudised by the closed-form O-like tables (Z_v = 6, s + p projectors, partial core) and
-    all-electron H: 8 valence electrons, 4 + 4 bands plus witnesses; a cold MIXED and a cold FP64 solve (each after an
-    untimed one), Fermi-Dirac kT 5 mHa, delta_rho < 1e-6."""
-    import synthetic_tables as T
+    """Water with the O pseudised by the generated O table filtered for h (lira_pseudo_generate: Z_v = 6, s projector,
+    p local, partial core) and all-electron H: 8 valence electrons, 4 + 4 bands plus witnesses; a cold MIXED and a cold
+    FP64 solve (each after an untimed one), Fermi-Dirac kT 5 mHa, delta_rho < 1e-6."""
     pos, z, _, _ = water()
     settings = lira.default_settings(h, lira.FERMI_DIRAC)
-    out = {}
+    oxygen = lira.generate_pseudo(8, h)
+    out = {"o_table": {k: oxygen.record[k] for k in ("f", "local", "z_v", "r_nlc", "r_core")}}
     for name, prec in (("mixed", lira.MIXED), ("fp64", lira.FP64)):
         s = lira.GpuSolver(n, h, prec)
-        s.set_ionic([T.oxygen_like(h)])
+        s.set_ionic([oxygen])
         s.solve(pos, z, 4, 4, settings)
         t0 = time.perf_counter()
         st = s.solve(pos, z, 4, 4, settings)
@@ -174,16 +174,15 @@ def batch_scf(lira, count=256, workers=4, seed=0, seq_count=32):
 
 def aspirin_leg(lira, n=160, h=0.2, seed=0, fd_components=1, max_iter=80):
     """Aspirin C9H8O4 (the reference's scene geometry with seeded +-0.05 A per nucleus and coordinate): C and O
-    pseudised by the closed-form C-like and O-like tables, H all-electron, 68 valence electrons (34 + 34), 160^3 at
+    pseudised by the generated tables filtered for h, H all-electron, 68 valence electrons (34 + 34), 160^3 at
     h = 0.2 Bohr, MIXED, Fermi-Dirac kT 5 mHa, the SCF tolerance raised to the f32 floor of 68 electrons as the
     reference's provider does (max(1e-6, 2 * 5e-8 * 68) = 6.8e-6). Cold SCF time, steps and block size, then the
     Hellmann-Feynman force (the hand-written dE/dR kernels) of the nucleus with the largest force against a central
     difference of the converged energy at +-1e-3 A (two warm solves per component)."""
     import molecules as M
-    import synthetic_tables as T
     pos, z = M.aspirin(seed)
     s = lira.GpuSolver(n, h, lira.MIXED)
-    s.set_ionic([T.carbon_like(h), T.oxygen_like(h)])
+    s.set_ionic([lira.generate_pseudo(6, h), lira.generate_pseudo(8, h)])
     tol = lira.effective_gpu_tol(1e-6, 68)
     st = lira.default_settings(h, lira.FERMI_DIRAC, tol=tol, max_iter=max_iter)
     steps = [0]
@@ -372,6 +371,7 @@ def main():
     ap.add_argument("--aspirin-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--roofline-only", action="store_true", help="only the H.X roofline measurement (for profiling)")
     ap.add_argument("--poisson-only", action="store_true", help="only the Poisson 256^3 measurement (for profiling)")
+    ap.add_argument("--pseudo-only", action="store_true", help="only the pseudised-water measurement")
     ap.add_argument("--variant", type=int, default=None, help="stencil tile variant of the roofline (default: tuned)")
     ap.add_argument("--traffic-probe", nargs=2, type=int, metavar=("ESIZE", "VARIANT"), help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -396,6 +396,9 @@ def main():
         return
     if args.poisson_only:
         print(json.dumps({"poisson_only": True, **poisson_256(lira)}))
+        return
+    if args.pseudo_only:
+        print(json.dumps({"pseudo_only": True, **pseudo_water(lira)}, default=float))
         return
     if args.roofline_only:
         r = hx_roofline(torch, lira, variant=args.variant)
@@ -481,9 +484,9 @@ def main():
         print(f"# H.X 192^3 x 64 f32: {r['ms']:.3f} ms, {r['gbs']:.0f} GB/s ({100 * r['frac']:.1f} % of peak); "
               f"f64 {r64['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
     if not args.no_extra:
-        result["pseudo_water_synthetic"] = pseudo_water(lira)
-        print(f"# pseudised water (synthetic O table): mixed {result['pseudo_water_synthetic']['mixed']['seconds']:.3f} s,"
-              f" fp64 {result['pseudo_water_synthetic']['fp64']['seconds']:.3f} s", file=sys.stderr, flush=True)
+        result["pseudo_water"] = pseudo_water(lira)
+        print(f"# pseudised water (generated O table): mixed {result['pseudo_water']['mixed']['seconds']:.3f} s,"
+              f" fp64 {result['pseudo_water']['fp64']['seconds']:.3f} s", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
         result["poisson_256"] = poisson_256(lira)
         print(f"# Poisson 256^3 to 1e-8: {result['poisson_256']['seconds'] * 1e3:.1f} ms, "
@@ -499,7 +502,7 @@ def main():
               file=sys.stderr, flush=True)
     if args.aspirin:
         torch.cuda.empty_cache()
-        result["aspirin_synthetic"] = aspirin_leg(lira)
+        result["aspirin"] = aspirin_leg(lira)
     if not args.no_cpu and gpu_fp64 is not None:
         result["cpu_baseline"] = cpu_baseline(state, n, h, pos, {
             "steps": gpu_fp64["steps"], "lobpcg_iterations": gpu_fp64["lobpcg_iterations"],
