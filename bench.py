@@ -1,9 +1,11 @@
 """Benchmark of the real-space Kohn–Sham SCF hot path on one MI355X.
 
-One step = one cold SCF of water (H2O, all-electron, O at the origin, O-H 0.96 A, H-O-H 104.5 deg) on a 95^3 grid at
-h = 0.15 Bohr, LSDA, Fermi–Dirac occupations at kT = 5 mHa, converged to delta_rho < 1e-6 in MIXED precision (f32
-grid fields, f64 Rayleigh–Ritz / DIIS / energies). The same SCF in FP64 is timed once for the mixed-vs-FP64 ratio
-and the per-result errors. The roofline object is the block stencil H.X on 192^3 x 64 bands (f32, V ~ U[-5, 0] Ha,
+One step = one cold SCF of water (H2O, O at the origin, O-H 0.96 A, H-O-H 104.5 deg; the O pseudised by the generated
+pseudopotential filtered for the grid, the two H all-electron: 8 valence electrons, 4 + 4 bands plus Fermi-Dirac
+witnesses) on a 95^3 grid at h = 0.15 Bohr, LSDA, Fermi–Dirac occupations at kT = 5 mHa, converged to
+delta_rho < 1e-6 in MIXED precision (f32 grid fields, f64 Rayleigh–Ritz / DIIS / energies). The same SCF in FP64 is
+timed once for the mixed-vs-FP64 ratio and the per-result errors; the all-electron water of the same grid is timed
+beside it. The roofline object is the block stencil H.X on 192^3 x 64 bands (f32, V ~ U[-5, 0] Ha,
 X ~ N(0, 1)), timed with HIP events on the stream it runs on. The CPU baseline times the f64 oracle (a restatement of
 the reference's single-threaded CPU solver) on a bounded sample of the same water problem.
 
@@ -30,6 +32,7 @@ HBM_PEAK_GBS = 8000.0
 
 
 def water():
+    """All-electron water: positions, Z, electrons per spin."""
     r = 0.96 * BOHR_PER_ANGSTROM
     half = math.radians(104.5 / 2)
     pos = [[0.0, 0.0, 0.0], [r * math.sin(half), r * math.cos(half), 0.0], [-r * math.sin(half), r * math.cos(half), 0.0]]
@@ -107,20 +110,28 @@ def poisson_256(lira, n=256, h=0.15, seed=3):
             "grid": n, "h_bohr": h, "gbs_algorithmic": gbs}
 
 
-def pseudo_water(lira, n=95, h=0.15):
-    """Water with the O pseudised by the generated O table filtered for h (lira_pseudo_generate: Z_v = 6, s projector,
-    p local, partial core) and all-electron H: 8 valence electrons, 4 + 4 bands plus witnesses; a cold MIXED and a cold
-    FP64 solve (each after an untimed one), Fermi-Dirac kT 5 mHa, delta_rho < 1e-6."""
-    pos, z, _, _ = water()
+def pseudo_species(lira, h):
+    """The ionic setup of the headline water: the O generated and filtered for h (lira_pseudo_generate: Z_v = 6, s
+    projector, p local, LFC partial core); H stays all-electron."""
+    return [lira.generate_pseudo(8, h)]
+
+
+def water_pair(lira, n=95, h=0.15, pseudised=False):
+    """A cold MIXED and a cold FP64 solve of water (each after an untimed one), Fermi-Dirac kT 5 mHa,
+    delta_rho < 1e-6: all-electron (5 + 5 electrons), or with the O pseudised (4 + 4)."""
+    pos, z, nu, nd = water()
     settings = lira.default_settings(h, lira.FERMI_DIRAC)
-    oxygen = lira.generate_pseudo(8, h)
-    out = {"o_table": {k: oxygen.record[k] for k in ("f", "local", "z_v", "r_nlc", "r_core")}}
+    species = pseudo_species(lira, h) if pseudised else []
+    if pseudised:
+        nu, nd = 4, 4
+    out = {"o_table": {k: species[0].record[k] for k in ("f", "local", "z_v", "r_nlc", "r_core")}} if pseudised else {}
     for name, prec in (("mixed", lira.MIXED), ("fp64", lira.FP64)):
         s = lira.GpuSolver(n, h, prec)
-        s.set_ionic([oxygen])
-        s.solve(pos, z, 4, 4, settings)
+        if species:
+            s.set_ionic(species)
+        s.solve(pos, z, nu, nd, settings)
         t0 = time.perf_counter()
-        st = s.solve(pos, z, 4, 4, settings)
+        st = s.solve(pos, z, nu, nd, settings)
         out[name] = {"seconds": time.perf_counter() - t0, "steps": st.iterations, "energy": st.energy.total,
                      "lobpcg_iterations": st.lobpcg_iterations}
         out[name + "_forces"] = s.forces(pos, z, h)
@@ -299,16 +310,16 @@ def cpu_share():
         return os.cpu_count() or 1
 
 
-def cpu_baseline(state, n, h, pos, fp64_counts):
+def cpu_baseline(state, n, h, pos, z, species, fp64_counts):
     """The oracle (the C f64 restatement of the reference's single-threaded CPU solver, crates/qm) on this host: a
     bounded sample of the same water problem, one f64 Poisson solve of the converged density to the SCF's Poisson
-    tolerance and LOBPCG at the oracle's own block size on the converged potential (setup alone, then two iterations),
-    timed on 1 thread (the reference's CPU solver) and on the host's CPU share, and scaled to one whole SCF with the
-    oracle's own counts on this workload (tests/golden/oracle_water95_counts.json: its steps, LOBPCG iterations and
-    Poisson CG iterations, from one full oracle solve; when that file is absent, the FP64 GPU run's counts, and the
-    sample says so)."""
+    tolerance and LOBPCG (with the pseudised O's nonlocal operator) at the oracle's own block size on the converged
+    potential (setup alone, then two iterations), timed on 1 thread (the reference's CPU solver) and on the host's CPU
+    share, and scaled to one whole SCF with the oracle's own counts on this workload
+    (tests/golden/oracle_pseudo_water95_counts.json: its steps, LOBPCG iterations and Poisson CG iterations, from one
+    full oracle solve; when that file is absent, the FP64 GPU run's counts, and the sample says so)."""
     import oracle as O
-    path = os.path.join(ROOT, "tests", "golden", "oracle_water95_counts.json")
+    path = os.path.join(ROOT, "tests", "golden", "oracle_pseudo_water95_counts.json")
     if os.path.exists(path):
         with open(path) as fh:
             counts = json.load(fh)
@@ -321,6 +332,7 @@ def cpu_baseline(state, n, h, pos, fp64_counts):
     v = vnuc + vh
     k = int(counts["k_up"])
     guess = O.initial_orbitals(n, h, pos, k, 1)
+    nl = O.Nonlocal(n, h, pos, z, species)
     runs = {}
     share = cpu_share()
     for label, threads in (("threads_1", 1), ("threads_all", share)):
@@ -329,10 +341,10 @@ def cpu_baseline(state, n, h, pos, fp64_counts):
         _, it_p, _ = O.poisson_solve(n, h, rho, tol=1e-10, max_iter=5000)
         t_poisson = time.perf_counter() - t0
         t0 = time.perf_counter()
-        O.lobpcg(n, h, v, guess, 1e-30, 0, precond_lambda=-1.0)
+        nl.lobpcg(v, guess, 1e-30, 0, precond_lambda=-1.0)
         t_setup = time.perf_counter() - t0
         t0 = time.perf_counter()
-        O.lobpcg(n, h, v, guess, 1e-30, 2, precond_lambda=-1.0)
+        nl.lobpcg(v, guess, 1e-30, 2, precond_lambda=-1.0)
         t_iter = max(time.perf_counter() - t0 - t_setup, 0.0) / 2
         per_cg = t_poisson / max(it_p, 1)
         calls = 2 * counts["steps"]  # one eigensolve per spin channel and step
@@ -345,7 +357,7 @@ def cpu_baseline(state, n, h, pos, fp64_counts):
     O.set_threads(share)
     top = runs["threads_all"]
     return {"value": top["seconds_per_scf"], "unit": "s", "cores": share, "kind": "port",
-            "sample": f"oracle (C f64 restatement of crates/qm) on the same water 95^3 problem: one Poisson solve and "
+            "sample": f"oracle (C f64 restatement of crates/qm) on the same pseudised water 95^3 problem: one Poisson solve and "
                       f"LOBPCG setup + 2 iterations at k = {k}, timed on 1 and {share} threads, scaled with {source} "
                       f"({counts['steps']} SCF steps, {counts['lobpcg_iterations']} LOBPCG iterations, "
                       f"{counts['cg_iterations']} CG iterations)",
@@ -361,7 +373,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-roofline", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the pseudised-water and Poisson 256^3 measurements")
+    ap.add_argument("--no-extra", action="store_true", help="skip the all-electron water and Poisson 256^3 measurements")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc passes")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched small-molecule SCF measurement")
     ap.add_argument("--batch-count", type=int, default=256)
@@ -398,7 +410,7 @@ def main():
         print(json.dumps({"poisson_only": True, **poisson_256(lira)}))
         return
     if args.pseudo_only:
-        print(json.dumps({"pseudo_only": True, **pseudo_water(lira)}, default=float))
+        print(json.dumps({"pseudo_only": True, **water_pair(lira, pseudised=True)}, default=float))
         return
     if args.roofline_only:
         r = hx_roofline(torch, lira, variant=args.variant)
@@ -407,9 +419,12 @@ def main():
                           "stream_gbs": r["stream_gbs"]}))
         return
     n, h = 95, 0.15
-    pos, z, nu, nd = water()
+    pos, z, _, _ = water()
+    nu, nd = 4, 4  # valence electrons per spin: 6 of the pseudised O, 1 + 1 of the H
+    species = pseudo_species(lira, h)
     settings = lira.default_settings(h, lira.FERMI_DIRAC)
     solver = lira.GpuSolver(n, h, lira.MIXED)
+    solver.set_ionic(species)
     for _ in range(args.warmup):
         solver.solve(pos, z, nu, nd, settings)
     torch.cuda.synchronize()
@@ -429,9 +444,12 @@ def main():
     result = {"metric": METRIC, "value": mixed_s, "unit": "s", "n_gpus": 1, "steps": args.steps,
               "warmup": args.warmup, "ms_per_step": mixed_s * 1e3, "higher_is_better": False, "scaling": "strong",
               "vs_baseline": None, "dtype": "f32 grid / f64 control", "data": "synthetic",
-              "config": {"workload": "water H2O all-electron, 95^3 grid at h = 0.15 Bohr, LSDA, FD kT 5 mHa, "
-                                     "delta_rho < 1e-6", "grid": n, "h_bohr": h, "nuclei": 3,
-                         "electrons": nu + nd, "precision": "mixed", "parallelism": "single GPU"},
+              "config": {"workload": "water H2O, O pseudised by the generated pseudopotential (f = 1.4, p local, LFC "
+                                     "partial core, filtered for h) and H all-electron, 95^3 grid at h = 0.15 Bohr, "
+                                     "LSDA, 4 + 4 bands plus witnesses, FD kT 5 mHa, delta_rho < 1e-6",
+                         "grid": n, "h_bohr": h, "nuclei": 3, "valence_electrons": nu + nd,
+                         "o_table": {k: species[0].record[k] for k in ("f", "local", "z_v", "r_nlc", "r_core")},
+                         "precision": "mixed", "parallelism": "single GPU"},
               "mixed": {"seconds": mixed_s, "steps": stats.iterations, "energy": stats.energy.total,
                         "lobpcg_iterations": stats.lobpcg_iterations, "cg_iterations": stats.cg_iterations,
                         "readbacks": stats.readbacks, "seconds_poisson": stats.seconds_poisson,
@@ -439,6 +457,7 @@ def main():
     gpu_fp64 = None
     if not args.no_fp64:
         s64 = lira.GpuSolver(n, h, lira.FP64)
+        s64.set_ionic(species)
         for _ in range(args.warmup):  # as the mixed solver: the one-time stencil tuning stays out of the timing
             s64.solve(pos, z, nu, nd, settings)
         torch.cuda.synchronize()
@@ -484,9 +503,9 @@ def main():
         print(f"# H.X 192^3 x 64 f32: {r['ms']:.3f} ms, {r['gbs']:.0f} GB/s ({100 * r['frac']:.1f} % of peak); "
               f"f64 {r64['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
     if not args.no_extra:
-        result["pseudo_water"] = pseudo_water(lira)
-        print(f"# pseudised water (generated O table): mixed {result['pseudo_water']['mixed']['seconds']:.3f} s,"
-              f" fp64 {result['pseudo_water']['fp64']['seconds']:.3f} s", file=sys.stderr, flush=True)
+        result["water_all_electron"] = water_pair(lira)
+        print(f"# all-electron water: mixed {result['water_all_electron']['mixed']['seconds']:.3f} s,"
+              f" fp64 {result['water_all_electron']['fp64']['seconds']:.3f} s", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
         result["poisson_256"] = poisson_256(lira)
         print(f"# Poisson 256^3 to 1e-8: {result['poisson_256']['seconds'] * 1e3:.1f} ms, "
@@ -504,7 +523,7 @@ def main():
         torch.cuda.empty_cache()
         result["aspirin"] = aspirin_leg(lira)
     if not args.no_cpu and gpu_fp64 is not None:
-        result["cpu_baseline"] = cpu_baseline(state, n, h, pos, {
+        result["cpu_baseline"] = cpu_baseline(state, n, h, pos, z, species, {
             "steps": gpu_fp64["steps"], "lobpcg_iterations": gpu_fp64["lobpcg_iterations"],
             "cg_iterations": gpu_fp64["cg_iterations"], "k_up": gpu_fp64["k"]})
     print(json.dumps(result))
