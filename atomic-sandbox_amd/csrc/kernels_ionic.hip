@@ -268,19 +268,23 @@ __device__ inline void block_sum_rows(double *acc, int rows, double (*lds)[RMAX]
     __syncthreads();
 }
 
-// Partial projections over one chunk of an ion's box: P[((row_off + r) * nb + b) * chunks + chunk] = sum over the
-// chunk's points of beta_r(p) x_b(p). Grid (chunks, ion, band); a chunk is NL_CHUNK consecutive box points, 8 per
-// thread, so the box is spread over many workgroups; chunks past an ion's box write zeros.
+// Projections over chunks of each ion's box. Block (chunk, ion, band) writes the partial sums of its chunk,
+// P[((row_off + r) * nb + b) * chunks + chunk] = sum over the chunk's points of beta_r(p) x_b(p) (a chunk is NL_CHUNK
+// consecutive box points, 8 per thread; chunks past an ion's box write zeros); the last of the chunks of one
+// (ion, band) to finish (a counter per (ion, band), reset by that block) sums them in chunk order into
+// C[(row_off + r) * nb + b] = dv * sum. Bands b < n1 read x1, the others x2 (two blocks in one launch).
 constexpr int NL_CHUNK = 2048;
 template <typename T>
-__global__ __launch_bounds__(256) void k_nl_project_partial(const T *x, int nb, long N, int n, const T *beta,
-                                                            const NlIon *ions, int chunks, double *P) {
+__global__ __launch_bounds__(256) void k_nl_project(const T *x1, int n1, const T *x2, int nb, long N, int n,
+                                                    const T *beta, const NlIon *ions, int chunks, double *P,
+                                                    double dv, double *C, unsigned *counters) {
     __shared__ double lds[4][NL_MAX_ROWS];
+    __shared__ bool last;
     const NlIon &ion = ions[blockIdx.y];
     const int b = blockIdx.z, chunk = blockIdx.x;
     const int rows = ion.rows;
     const int pts = (int)ion.pts, dx = ion.dim[0], dy = ion.dim[1];
-    const T *xb = x + (long)b * N;
+    const T *xb = b < n1 ? x1 + (long)b * N : x2 + (long)(b - n1) * N;
     const T *bb = beta + ion.beta_off;
     double acc[NL_MAX_ROWS];
 #pragma unroll
@@ -303,25 +307,30 @@ __global__ __launch_bounds__(256) void k_nl_project_partial(const T *x, int nb, 
     if ((int)threadIdx.x < rows) {
         const int r = threadIdx.x;
         P[((long)(ion.row_off + r) * nb + b) * chunks + chunk] = ((lds[0][r] + lds[1][r]) + lds[2][r]) + lds[3][r];
+        __threadfence();
     }
-}
-
-// C[row * nb + b] = dv * (the row's chunk partials summed in chunk order), one thread per (row, band)
-__global__ __launch_bounds__(256) void k_nl_project_finish(const double *P, int total, int chunks, double dv,
-                                                           double *C) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= total) return;
-    const double *p = P + (long)i * chunks;
-    double s = 0.0;
-    for (int c = 0; c < chunks; c++) s += p[c];
-    C[i] = dv * s;
+    __syncthreads();
+    unsigned *counter = counters + (long)blockIdx.y * nb + b;
+    if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == (unsigned)chunks - 1u;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if ((int)threadIdx.x < rows) {
+        const long row = ion.row_off + threadIdx.x;
+        const double *pp = P + (row * nb + b) * chunks;
+        double sum = 0.0;
+        for (int c = 0; c < chunks; c++) sum += pp[c];
+        C[row * nb + b] = dv * sum;
+    }
+    if (threadIdx.x == 0) *counter = 0u;
 }
 
 // out_b(p) += sum_r E_r C[r][b] beta_r(p) over the boxes of the ions of one colour (disjoint boxes: no two threads
-// touch one point); blockIdx.y picks the ion of the colour, blockIdx.z the band
+// touch one point); blockIdx.y picks the ion of the colour, blockIdx.z the band (b < n1 in out1, the rest in out2)
 template <typename T>
-__global__ __launch_bounds__(256) void k_nl_apply(T *out, int nb, long N, int n, const T *beta, const NlIon *ions,
-                                                  const int *members, const double *C, const double *erow) {
+__global__ __launch_bounds__(256) void k_nl_apply(T *out1, int n1, T *out2, int nb, long N, int n, const T *beta,
+                                                  const NlIon *ions, const int *members, const double *C,
+                                                  const double *erow) {
     const NlIon &ion = ions[members[blockIdx.y]];
     const long p = blockIdx.x * 256L + threadIdx.x;
     if (p >= ion.pts) return;
@@ -334,31 +343,30 @@ __global__ __launch_bounds__(256) void k_nl_apply(T *out, int nb, long N, int n,
         const long row = ion.row_off + r;
         acc += erow[row] * C[row * nb + b] * (double)beta[ion.beta_off + (long)r * ion.pts + p];
     }
-    T *ob = out + (long)b * N;
+    T *ob = b < n1 ? out1 + (long)b * N : out2 + (long)(b - n1) * N;
     ob[g] = (T)((double)ob[g] + acc);
 }
 
 int nl_project_chunks(long max_pts) { return (int)std::max(1L, (max_pts + NL_CHUNK - 1) / NL_CHUNK); }
 
 template <typename T>
-void nl_project(const T *x, int nb, long N, int n, const T *beta, const NlIon *ions, int nions, int rows,
-                long max_pts, double dv, double *C, double *work, hipStream_t s) {
-    if (nions <= 0 || nb <= 0 || rows <= 0) return;
+void nl_project(const T *x1, int n1, const T *x2, int n2, long N, int n, const T *beta, const NlIon *ions, int nions,
+                long max_pts, double dv, double *C, double *work, unsigned *counters, hipStream_t s) {
+    const int nb = n1 + n2;
+    if (nions <= 0 || nb <= 0) return;
     const int chunks = nl_project_chunks(max_pts);
-    hipLaunchKernelGGL(k_nl_project_partial<T>, dim3(chunks, nions, nb), dim3(256), 0, s, x, nb, N, n, beta, ions,
-                       chunks, work);
-    LIRA_HIP(hipGetLastError());
-    const int total = rows * nb;
-    hipLaunchKernelGGL(k_nl_project_finish, dim3((total + 255) / 256), dim3(256), 0, s, work, total, chunks, dv, C);
+    hipLaunchKernelGGL(k_nl_project<T>, dim3(chunks, nions, nb), dim3(256), 0, s, x1, n1, x2, nb, N, n, beta, ions,
+                       chunks, work, dv, C, counters);
     LIRA_HIP(hipGetLastError());
 }
 
 template <typename T>
-void nl_apply(T *out, int nb, long N, int n, const T *beta, const NlIon *ions, const int *members, int nmembers,
-              long max_pts, const double *C, const double *erow, hipStream_t s) {
+void nl_apply(T *out1, int n1, T *out2, int n2, long N, int n, const T *beta, const NlIon *ions, const int *members,
+              int nmembers, long max_pts, const double *C, const double *erow, hipStream_t s) {
+    const int nb = n1 + n2;
     if (nmembers <= 0 || nb <= 0 || max_pts <= 0) return;
-    hipLaunchKernelGGL(k_nl_apply<T>, dim3((unsigned)((max_pts + 255) / 256), nmembers, nb), dim3(256), 0, s, out, nb,
-                       N, n, beta, ions, members, C, erow);
+    hipLaunchKernelGGL(k_nl_apply<T>, dim3((unsigned)((max_pts + 255) / 256), nmembers, nb), dim3(256), 0, s, out1, n1,
+                       out2, nb, N, n, beta, ions, members, C, erow);
     LIRA_HIP(hipGetLastError());
 }
 
@@ -520,10 +528,10 @@ void nl_force(const T *psi, int nb, long N, int n, double h, const NlIon *ions, 
                                   double, hipStream_t);                                                               \
     template void ionic_fields<T>(T *, T *, int, double, const IonicCentre *, int, const double *, hipStream_t);      \
     template void nl_beta<T>(T *, const NlIon &, int, double, const double *, hipStream_t);                          \
-    template void nl_project<T>(const T *, int, long, int, const T *, const NlIon *, int, int, long, double,         \
-                                double *, double *, hipStream_t);                                                     \
-    template void nl_apply<T>(T *, int, long, int, const T *, const NlIon *, const int *, int, long, const double *, \
-                              const double *, hipStream_t);                                                           \
+    template void nl_project<T>(const T *, int, const T *, int, long, int, const T *, const NlIon *, int, long,       \
+                                double, double *, double *, unsigned *, hipStream_t);                                 \
+    template void nl_apply<T>(T *, int, T *, int, long, int, const T *, const NlIon *, const int *, int, long,          \
+                              const double *, const double *, hipStream_t);                                           \
     template void ionic_forces<T>(const T *, const T *, const T *, int, double, const IonicCentre *, const double *, \
                                   int, double *, double *, hipStream_t);                                              \
     template void nl_force<T>(const T *, int, long, int, double, const NlIon *, int, const double *, double *,       \

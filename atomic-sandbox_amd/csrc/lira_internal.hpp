@@ -190,17 +190,19 @@ struct NlIon {
 template <typename T>
 void ionic_fields(T *vloc, T *pc, int n, double h, const IonicCentre *c, int nat, const double *tab, hipStream_t s);
 template <typename T> void nl_beta(T *beta, const NlIon &ion, int n, double h, const double *tab, hipStream_t s);
-// C[(row_off + r) * nb + b] = dv sum_box beta_r x_b for every ion with projectors (rows in all); the box sums run in
-// chunks of max_pts / nl_project_chunks points, their partials in work (rows * nb * nl_project_chunks(max_pts) doubles)
-// summed in a fixed order
+// C[(row_off + r) * nb + b] = dv sum_box beta_r x_b for every ion with projectors, nb = n1 + n2 bands (the first n1
+// from x1, the rest from x2); the box sums run in chunks (nl_project_chunks(max_pts) per box), their partials in work
+// (rows * nb * chunks doubles) summed in a fixed order by the last chunk of each (ion, band), whose counter
+// (counters[ion * nb + b], zero between launches) it resets
 int nl_project_chunks(long max_pts);
 template <typename T>
-void nl_project(const T *x, int nb, long N, int n, const T *beta, const NlIon *ions, int nions, int rows,
-                long max_pts, double dv, double *C, double *work, hipStream_t s);
-// out_b += sum_r erow_r C_rb beta_r on the boxes of the listed ions (boxes must be disjoint)
+void nl_project(const T *x1, int n1, const T *x2, int n2, long N, int n, const T *beta, const NlIon *ions, int nions,
+                long max_pts, double dv, double *C, double *work, unsigned *counters, hipStream_t s);
+// out_b += sum_r erow_r C_rb beta_r on the boxes of the listed ions (boxes must be disjoint), bands b < n1 in out1,
+// the rest in out2
 template <typename T>
-void nl_apply(T *out, int nb, long N, int n, const T *beta, const NlIon *ions, const int *members, int nmembers,
-              long max_pts, const double *C, const double *erow, hipStream_t s);
+void nl_apply(T *out1, int n1, T *out2, int n2, long N, int n, const T *beta, const NlIon *ions, const int *members,
+              int nmembers, long max_pts, const double *C, const double *erow, hipStream_t s);
 // out[(term * nat + I) * 3 + a], term 0 local LR, 1 local SR, 2 partial core (qm/src/forces.rs:281-333)
 template <typename T>
 void ionic_forces(const T *rho, const T *vu, const T *vd, int n, double h, const IonicCentre *cs, const double *tab,

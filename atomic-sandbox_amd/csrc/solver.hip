@@ -605,8 +605,8 @@ template <typename T> class Solver final : public SolverIface {
     // pseudised ions: the setup (tables copied at lira_set_ionic) and the terms of the current geometry
     std::vector<SpeciesHost> species_;
     std::vector<double> tab_h_;
-    DevBuf tab_, centres_, nl_ions_, beta_, nl_C_, nl_erow_, nl_members_, pc_, qf_, fpart_, fout_, nl_P_;
-    size_t nl_C_cap_ = 0, nl_P_cap_ = 0;
+    DevBuf tab_, centres_, nl_ions_, beta_, nl_C_, nl_erow_, nl_members_, pc_, qf_, fpart_, fout_, nl_P_, nl_count_;
+    size_t nl_C_cap_ = 0, nl_P_cap_ = 0, nl_count_cap_ = 0;
     int ionic_version_ = 0, state_ionic_version_ = -1;
     std::vector<NlIon> nl_h_;
     std::vector<int> nl_atom_;  // the nucleus of each NlIon
@@ -639,8 +639,8 @@ template <typename T> class Solver final : public SolverIface {
         return nullptr;
     }
     void ionic_geometry(double sigma);
-    void nl_apply_block(const T *x, T *out, int nb);
-    void nl_projections(const T *x, int nb);
+    void nl_apply_block(const T *x, T *out, int nb, const T *x2 = nullptr, T *out2 = nullptr, int nb2 = 0);
+    void nl_projections(const T *x, int nb, const T *x2 = nullptr, int nb2 = 0);
     std::vector<double> nl_energy_terms(Channel<T> &c, bool fd);
 
     void ensure_host(size_t count) {
@@ -749,10 +749,7 @@ template <typename T> class Solver final : public SolverIface {
         sp.x2 = x2;
         sp.out2 = out2;
         stencil<T>(ST_HAM, sp, x1, out1, v, nullptr, nullptr, stream_, variant(ST_HAM, n1 + n2));
-        if (nl_rows_ > 0) {
-            nl_apply_block(x1, out1, n1);
-            nl_apply_block(x2, out2, n2);
-        }
+        if (nl_rows_ > 0) nl_apply_block(x1, out1, n1, x2, out2, n2);
     }
     void tune_stencil(int nb) {
         if (nb <= 0) return;
@@ -1847,7 +1844,8 @@ template <typename T> void Solver<T>::ionic_geometry(double sigma) {
 }
 
 // out_b += V_NL x_b for a block: the projections C (f64, rows x nb), then the scatter colour by colour
-template <typename T> void Solver<T>::nl_projections(const T *x, int nb) {
+template <typename T> void Solver<T>::nl_projections(const T *x, int n1, const T *x2, int n2) {
+    const int nb = n1 + n2;
     const size_t need = (size_t)nl_rows_ * nb * 4;
     if (need > nl_C_cap_) {
         nl_C_.alloc(need * sizeof(double));
@@ -1860,15 +1858,22 @@ template <typename T> void Solver<T>::nl_projections(const T *x, int nb) {
         nl_P_.alloc(work * sizeof(double));
         nl_P_cap_ = work;
     }
-    nl_project<T>(x, nb, N_, n_, beta_.template as<T>(), nl_ions_.as<NlIon>(), (int)nl_h_.size(), nl_rows_, max_pts,
-                  dv_, nl_C_.as<double>(), nl_P_.as<double>(), stream_);
+    // one counter per (ion, band), zero between launches (the last chunk of each resets its own)
+    const size_t counters = nl_h_.size() * (size_t)nb;
+    if (counters > nl_count_cap_) {
+        nl_count_.alloc(counters * sizeof(unsigned));
+        LIRA_HIP(hipMemsetAsync(nl_count_.ptr, 0, counters * sizeof(unsigned), stream_));
+        nl_count_cap_ = counters;
+    }
+    nl_project<T>(x, n1, x2, n2, N_, n_, beta_.template as<T>(), nl_ions_.as<NlIon>(), (int)nl_h_.size(), max_pts, dv_,
+                  nl_C_.as<double>(), nl_P_.as<double>(), nl_count_.as<unsigned>(), stream_);
 }
 
-template <typename T> void Solver<T>::nl_apply_block(const T *x, T *out, int nb) {
-    nl_projections(x, nb);
+template <typename T> void Solver<T>::nl_apply_block(const T *x, T *out, int nb, const T *x2, T *out2, int nb2) {
+    nl_projections(x, nb, x2, nb2);
     for (const Colour &c : colours_)
-        nl_apply<T>(out, nb, N_, n_, beta_.template as<T>(), nl_ions_.as<NlIon>(), nl_members_.as<int>() + c.off,
-                    c.count, c.max_pts, nl_C_.as<double>(), nl_erow_.as<double>(), stream_);
+        nl_apply<T>(out, nb, out2, nb2, N_, n_, beta_.template as<T>(), nl_ions_.as<NlIon>(),
+                    nl_members_.as<int>() + c.off, c.count, c.max_pts, nl_C_.as<double>(), nl_erow_.as<double>(), stream_);
 }
 
 // E_NL terms f_b E_r c_rb^2 of a channel in (orbital, sphere, m) order; Aufbau: every orbital at f = 1
