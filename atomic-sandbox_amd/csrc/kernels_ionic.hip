@@ -268,12 +268,12 @@ __device__ inline void block_sum_rows(double *acc, int rows, double (*lds)[RMAX]
     __syncthreads();
 }
 
-// Projections over chunks of each ion's box. Block (chunk, ion, band) writes the partial sums of its chunk,
-// P[((row_off + r) * nb + b) * chunks + chunk] = sum over the chunk's points of beta_r(p) x_b(p) (a chunk is NL_CHUNK
-// consecutive box points, 8 per thread; chunks past an ion's box write zeros); the last of the chunks of one
-// (ion, band) to finish (a counter per (ion, band), reset by that block) sums them in chunk order into
+// Projections over chunks of each ion's box. Block (chunk, ion, band group) writes the partial sums of its chunk for
+// NL_BANDS bands, P[((row_off + r) * nb + b) * chunks + chunk] = sum over the chunk's points of beta_r(p) x_b(p) (a
+// chunk is NL_CHUNK consecutive box points, 8 per thread; chunks past an ion's box write zeros); the last of the chunks
+// of one (ion, band group) to finish (a counter per (ion, group), reset by that block) sums them in chunk order into
 // C[(row_off + r) * nb + b] = dv * sum. Bands b < n1 read x1, the others x2 (two blocks in one launch).
-constexpr int NL_CHUNK = 2048;
+constexpr int NL_CHUNK = 2048, NL_BANDS = 4;
 template <typename T>
 __global__ __launch_bounds__(256) void k_nl_project(const T *x1, int n1, const T *x2, int nb, long N, int n,
                                                     const T *beta, const NlIon *ions, int chunks, double *P,
@@ -281,46 +281,75 @@ __global__ __launch_bounds__(256) void k_nl_project(const T *x1, int n1, const T
     __shared__ double lds[4][NL_MAX_ROWS];
     __shared__ bool last;
     const NlIon &ion = ions[blockIdx.y];
-    const int b = blockIdx.z, chunk = blockIdx.x;
+    const int b0 = blockIdx.z * NL_BANDS, chunk = blockIdx.x;
+    const int bands = nb - b0 < NL_BANDS ? nb - b0 : NL_BANDS;
     const int rows = ion.rows;
     const int pts = (int)ion.pts, dx = ion.dim[0], dy = ion.dim[1];
-    const T *xb = b < n1 ? x1 + (long)b * N : x2 + (long)(b - n1) * N;
+    const T *xb[NL_BANDS];
+#pragma unroll
+    for (int j = 0; j < NL_BANDS; j++) {
+        const int b = b0 + (j < bands ? j : 0);
+        xb[j] = b < n1 ? x1 + (long)b * N : x2 + (long)(b - n1) * N;
+    }
     const T *bb = beta + ion.beta_off;
-    double acc[NL_MAX_ROWS];
+    double acc[NL_BANDS][NL_MAX_ROWS];
 #pragma unroll
-    for (int r = 0; r < NL_MAX_ROWS; r++) acc[r] = 0.0;
+    for (int j = 0; j < NL_BANDS; j++)
+#pragma unroll
+        for (int r = 0; r < NL_MAX_ROWS; r++) acc[j][r] = 0.0;
     const int p0 = chunk * NL_CHUNK + (int)threadIdx.x;
-#pragma unroll 2
-    for (int k = 0; k < NL_CHUNK / 256; k++) {
+    // points past the box read the box's last point with weight 0: the loads carry no branch, so the unrolled
+    // iterations issue them together
+#pragma unroll 4
+    for (int k = 0; k < (pts > 0 ? NL_CHUNK / 256 : 0); k++) {
         const int p = p0 + k * 256;
-        if (p < pts) {
-            const int px = p % dx, t = p / dx;
-            const int py = t % dy, pz = t / dy;
-            const long g = (ion.lo[0] + px) + (long)n * ((ion.lo[1] + py) + (long)n * (ion.lo[2] + pz));
-            const double xv = (double)xb[g];
+        const bool in = p < pts;
+        const int q = in ? p : pts - 1;
+        const double w = in ? 1.0 : 0.0;
+        const int px = q % dx, t = q / dx;
+        const int py = t % dy, pz = t / dy;
+        const long g = (ion.lo[0] + px) + (long)n * ((ion.lo[1] + py) + (long)n * (ion.lo[2] + pz));
+        double xv[NL_BANDS];
 #pragma unroll
-            for (int r = 0; r < NL_MAX_ROWS; r++)
-                if (r < rows) acc[r] += (double)bb[(long)r * pts + p] * xv;
+        for (int j = 0; j < NL_BANDS; j++) xv[j] = w * (double)xb[j][g];
+#pragma unroll
+        for (int r = 0; r < NL_MAX_ROWS; r++)
+            if (r < rows) {
+                const double bv = (double)bb[(long)r * pts + q];
+#pragma unroll
+                for (int j = 0; j < NL_BANDS; j++) acc[j][r] += bv * xv[j];
+            }
+    }
+#pragma unroll
+    for (int j = 0; j < NL_BANDS; j++) {
+        if (j < bands) {  // uniform over the block
+            block_sum_rows<NL_MAX_ROWS>(acc[j], rows, lds);
+            if ((int)threadIdx.x < rows) {
+                const int r = threadIdx.x;
+                P[((long)(ion.row_off + r) * nb + b0 + j) * chunks + chunk] =
+                    ((lds[0][r] + lds[1][r]) + lds[2][r]) + lds[3][r];
+                __threadfence();
+            }
+            __syncthreads();
         }
     }
-    block_sum_rows<NL_MAX_ROWS>(acc, rows, lds);
-    if ((int)threadIdx.x < rows) {
-        const int r = threadIdx.x;
-        P[((long)(ion.row_off + r) * nb + b) * chunks + chunk] = ((lds[0][r] + lds[1][r]) + lds[2][r]) + lds[3][r];
-        __threadfence();
-    }
-    __syncthreads();
-    unsigned *counter = counters + (long)blockIdx.y * nb + b;
+    const int groups = (nb + NL_BANDS - 1) / NL_BANDS;
+    unsigned *counter = counters + (long)blockIdx.y * groups + blockIdx.z;
     if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == (unsigned)chunks - 1u;
     __syncthreads();
     if (!last) return;
     __threadfence();
-    if ((int)threadIdx.x < rows) {
-        const long row = ion.row_off + threadIdx.x;
-        const double *pp = P + (row * nb + b) * chunks;
-        double sum = 0.0;
-        for (int c = 0; c < chunks; c++) sum += pp[c];
-        C[row * nb + b] = dv * sum;
+    // one wave per (row, band): lane l sums chunks l, l + 64, ... in order, then the wave's fixed shuffle tree
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int pair = wave; pair < rows * bands; pair += 4) {
+        const int r = pair % rows, j = pair / rows;
+        const long row = ion.row_off + r;
+        const double *pp = P + (row * nb + b0 + j) * chunks;
+        double v = 0.0;
+        for (int c = lane; c < chunks; c += 64) v += pp[c];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) C[row * nb + b0 + j] = dv * v;
     }
     if (threadIdx.x == 0) *counter = 0u;
 }
@@ -355,8 +384,8 @@ void nl_project(const T *x1, int n1, const T *x2, int n2, long N, int n, const T
     const int nb = n1 + n2;
     if (nions <= 0 || nb <= 0) return;
     const int chunks = nl_project_chunks(max_pts);
-    hipLaunchKernelGGL(k_nl_project<T>, dim3(chunks, nions, nb), dim3(256), 0, s, x1, n1, x2, nb, N, n, beta, ions,
-                       chunks, work, dv, C, counters);
+    hipLaunchKernelGGL(k_nl_project<T>, dim3(chunks, nions, (nb + NL_BANDS - 1) / NL_BANDS), dim3(256), 0, s, x1, n1,
+                       x2, nb, N, n, beta, ions, chunks, work, dv, C, counters);
     LIRA_HIP(hipGetLastError());
 }
 
