@@ -6,20 +6,27 @@
 //   spin-restricted LSDA reference atom on the universal log grid (atom.rs) -> core / valence partition
 //   (partition.rs) -> LFC partial core at the ledger's crossover (nlcc.rs) -> for each (f, local) of the ladder:
 //   Troullier-Martins channels at r_c = f r_peak (tm.rs), unscreening, the Kleinman-Bylander form with its derived
-//   support r_beta (kb.rs), the GSS ghost test G1 (kb.rs:611-634) and the log-derivative gate T5 (gates.rs:182-217,
-//   313-325) -> the band-limited tables for spacing h (filter.rs: King-Smith-Payne-Lin completion, uniform 3-D tables).
+//   support r_beta (kb.rs), the ghost gates G1 (GSS) and G2 (the radial KB atom's exact level count, kb.rs:611-663),
+//   the transferability gates T1-T7 (gates.rs: the KB atom solved self-consistently in the cation, s -> p, anion and
+//   Hund configurations, log derivatives, hardness) -> the band-limited tables for spacing h (filter.rs:
+//   King-Smith-Payne-Lin completion, uniform 3-D tables).
 //
-// Gates of the reference ladder that are not restated here: G2 (the KB atom's inertia count) and T1-T4, T6, T7 (the
-// KB atom solved self-consistently in the transfer configurations). At the reference configuration the KB atom's
-// screening is the unscreening's own, so its valence density is the TM pseudo-density (kb.rs module doc, "The radial
-// KB atom"); the filter's partial-core support uses that density directly.
+// At the reference configuration the KB atom's screening is the unscreening's own, so its valence density is the TM
+// pseudo-density (kb.rs module doc, "The radial KB atom"); the filter's partial-core support uses that density
+// directly. The dimer gate on f for Z <= 2 (a 3-D computation the reference's providers run) and T8's h_req ladder are
+// not restated: H and He come out as the reference's ungated generate() makes them.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <optional>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "lira.h"
@@ -1006,7 +1013,7 @@ std::vector<double> kb_regular(const LogGrid &grid, const std::vector<double> &v
 // T5 (gates.rs:182-217): delta E_equiv(E) = 1/2 W(u_AE, u_PP)(r_d), each regular solution normalised inside r_d,
 // r_d = 1.2 max r_c; the maximum over eps_l +- 0.5 Ha on a 10 mHa mesh with both end points exact
 double t5_channel(const LogGrid &g, const AtomResult &ref, const std::vector<TmChannel> &ch, const KbPotential &kb,
-                  const std::vector<double> &screen, int l, double eps) {
+                  const std::vector<double> &screen, int l, double eps, const std::vector<double> &extra) {
     double rmax = 0.0;
     for (const auto &c : ch) rmax = std::max(rmax, c.r_c);
     const int i_d = g.nearest(T5_RD_FACTOR * rmax);
@@ -1041,7 +1048,641 @@ double t5_channel(const LogGrid &g, const AtomResult &ref, const std::vector<TmC
         const double e = k == n ? hi : std::min(lo + k * T5_MESH, hi);
         worst = std::max(worst, std::fabs(de_equiv(e)));
     }
+    for (double e : extra) worst = std::max(worst, std::fabs(de_equiv(e)));
     return worst;
+}
+
+
+// ---- the general all-electron atom: spin-polarised and fixed occupations, the total energy (atom.rs) ----
+struct ShellOcc {
+    int n, l;
+    double occ[2];
+};
+
+struct FullOrbital {
+    int n, l;
+    double eps[2] = {NAN, NAN};
+    double occ[2] = {0.0, 0.0};
+    std::vector<double> u[2];
+};
+
+struct FullAtom {
+    int z = 0;
+    bool restricted = true;
+    int multiplicity = 1;
+    std::vector<FullOrbital> orbitals;
+    std::vector<double> rho[2], v[2];
+    double total = 0.0;
+};
+
+struct AtomOpts {
+    bool restricted = true;
+    bool aufbau = true;
+    int multiplicity = 1;
+    std::vector<ShellOcc> fixed;
+};
+
+// one spin channel's aufbau, cap(l) = factor (2l + 1) (atom.rs:197-226)
+std::vector<Shell> aufbau_cap(const LogGrid &grid, const std::vector<double> &v, int z, double electrons, double factor,
+                              const std::vector<FullOrbital> &prev, int spin) {
+    auto guess = [&](int n, int l) -> std::optional<double> {
+        for (const auto &o : prev)
+            if (o.n == n && o.l == l && std::isfinite(o.eps[spin])) return o.eps[spin];
+        return std::nullopt;
+    };
+    std::optional<BoundState> next[4];
+    int next_n[4];
+    for (int l = 0; l < 4; l++) {
+        next[l] = bound_state(grid, v, l, 0, z, guess(l + 1, l));
+        next_n[l] = l + 1;
+    }
+    std::vector<Shell> out;
+    double left = electrons;
+    while (left > 1e-12) {
+        int pick = -1;
+        for (int l = 0; l < 4; l++)
+            if (next[l] && (pick < 0 || next[l]->e < next[pick]->e)) pick = l;
+        if (pick < 0) throw Unbound("Z = " + std::to_string(z) + ": no bound shell left for the remaining electrons");
+        const int n = next_n[pick];
+        const double occ = std::min(left, factor * (2 * pick + 1));
+        left -= occ;
+        out.push_back({n, pick, occ, std::move(*next[pick])});
+        next_n[pick] = n + 1;
+        next[pick] = bound_state(grid, v, pick, n - pick, z, guess(n + 1, pick));
+    }
+    return out;
+}
+
+std::vector<FullOrbital> occupy_full(const LogGrid &grid, int z, const std::vector<double> *v, const AtomOpts &o,
+                                     const std::vector<FullOrbital> &prev, std::vector<double> *rho) {
+    std::vector<Shell> ch[2];
+    if (o.aufbau && o.restricted) {
+        ch[0] = aufbau_cap(grid, v[0], z, z, 2.0, prev, 0);
+        for (auto &s : ch[0]) s.occ *= 0.5;
+        ch[1] = ch[0];
+    } else if (o.aufbau) {
+        const int m = o.multiplicity;
+        if (m < 1 || m - 1 > z || (z - (m - 1)) % 2 != 0) throw Fail("multiplicity impossible for the electron count");
+        const int up = (z + m - 1) / 2, dn = (z + 1 - m) / 2;
+        ch[0] = aufbau_cap(grid, v[0], z, up, 1.0, prev, 0);
+        ch[1] = aufbau_cap(grid, v[1], z, dn, 1.0, prev, 1);
+    } else {
+        for (int sp = 0; sp < 2; sp++)
+            for (const auto &sh : o.fixed) {
+                if (sh.occ[sp] <= 0.0) continue;
+                const int src = o.restricted ? 0 : sp;
+                std::optional<double> g;
+                for (const auto &q : prev)
+                    if (q.n == sh.n && q.l == sh.l && std::isfinite(q.eps[src])) g = q.eps[src];
+                auto b = bound_state(grid, v[src], sh.l, sh.n - sh.l - 1, z, g);
+                if (!b) throw Unbound("Z = " + std::to_string(z) + ": a fixed shell is unbound");
+                ch[sp].push_back({sh.n, sh.l, sh.occ[sp], std::move(*b)});
+            }
+    }
+    for (int sp = 0; sp < 2; sp++) rho[sp] = shell_density(grid, ch[sp]);
+    std::vector<FullOrbital> out;
+    for (int sp = 0; sp < 2; sp++)
+        for (auto &sh : ch[sp]) {
+            FullOrbital *o2 = nullptr;
+            for (auto &q : out)
+                if (q.n == sh.n && q.l == sh.l) o2 = &q;
+            if (!o2) {
+                out.push_back(FullOrbital{sh.n, sh.l});
+                o2 = &out.back();
+            }
+            o2->eps[sp] = sh.b.e;
+            o2->occ[sp] = sh.occ;
+            o2->u[sp] = std::move(sh.b.u);
+        }
+    std::sort(out.begin(), out.end(), [](const FullOrbital &a, const FullOrbital &b) {
+        return a.n != b.n ? a.n < b.n : a.l < b.l;
+    });
+    return out;
+}
+
+void full_potentials(const LogGrid &grid, int z, const std::vector<double> *rho, std::vector<double> *v) {
+    const int n = grid.size();
+    std::vector<double> total(n);
+    for (int i = 0; i < n; i++) total[i] = rho[0][i] + rho[1][i];
+    const auto vh = hartree_potential(grid, total);
+    v[0].resize(n);
+    v[1].resize(n);
+    for (int i = 0; i < n; i++) {
+        double vu, vd;
+        pw92(rho[0][i], rho[1][i], &vu, &vd);
+        const double c = -z / grid.r[i] + vh[i];
+        v[0][i] = c + vu;
+        v[1][i] = c + vd;
+    }
+}
+
+double shell_integral(const LogGrid &g, const std::vector<double> &f) {
+    std::vector<double> w(g.size());
+    for (int i = 0; i < g.size(); i++) w[i] = 4.0 * M_PI * g.r[i] * g.r[i] * f[i];
+    return g.integrate(w);
+}
+
+// the energy from the output density and orbitals: T_s = sum f eps - sum_s int rho_s V_s^in (atom.rs:387-417)
+double full_energy(const LogGrid &grid, int z, const std::vector<FullOrbital> &orb, const std::vector<double> *rho,
+                   const std::vector<double> *v_in) {
+    const int n = grid.size();
+    Neumaier band;
+    for (const auto &o : orb)
+        for (int s = 0; s < 2; s++)
+            if (o.occ[s] > 0.0) band.add(o.occ[s] * o.eps[s]);
+    Neumaier pot;
+    for (int s = 0; s < 2; s++) {
+        std::vector<double> f(n);
+        for (int i = 0; i < n; i++) f[i] = rho[s][i] * v_in[s][i];
+        pot.add(shell_integral(grid, f));
+    }
+    const double kinetic = band.sum() - pot.sum();
+    std::vector<double> total(n), f(n);
+    for (int i = 0; i < n; i++) total[i] = rho[0][i] + rho[1][i];
+    for (int i = 0; i < n; i++) f[i] = 4.0 * M_PI * grid.r[i] * total[i];
+    const double nuclear = -z * grid.integrate(f);
+    const auto vh = hartree_potential(grid, total);
+    for (int i = 0; i < n; i++) f[i] = total[i] * vh[i];
+    const double hartree = 0.5 * shell_integral(grid, f);
+    for (int i = 0; i < n; i++) f[i] = pw92(rho[0][i], rho[1][i]);
+    const double xc = shell_integral(grid, f);
+    Neumaier e;
+    e.add(kinetic);
+    e.add(nuclear);
+    e.add(hartree);
+    e.add(xc);
+    return e.sum();
+}
+
+// one SCF pass from rho = 0 (atom.rs:331-384) on the stacked spin densities
+FullAtom full_pass(int z, const AtomOpts &o, bool use_diis, bool &exhausted) {
+    const LogGrid &grid = universal_grid();
+    const int n = grid.size();
+    std::vector<double> w(2 * n);
+    for (int i = 0; i < n; i++) w[i] = w[n + i] = grid.r[i] * grid.r[i] * grid.r[i] * grid.delta;
+    std::vector<double> rho_in[2] = {std::vector<double>(n, 0.0), std::vector<double>(n, 0.0)};
+    std::vector<FullOrbital> prev, g_orb;
+    Diis diis(use_diis ? DIIS_HISTORY : 1);
+    bool have_good = false;
+    std::vector<double> g_in[2], g_out[2];
+    int halvings = 0;
+    exhausted = false;
+    double delta = INFINITY;
+    for (int it = 1; it <= SCF_MAX_ITER; it++) {
+        std::vector<double> v[2], rho_out[2];
+        full_potentials(grid, z, rho_in, v);
+        std::vector<FullOrbital> orb;
+        bool ok = true;
+        try {
+            orb = occupy_full(grid, z, v, o, prev, rho_out);
+        } catch (const Unbound &) {
+            ok = false;
+        }
+        if (ok) {
+            if (use_diis) halvings = 0;
+            delta = 0.0;
+            for (int s = 0; s < 2; s++)
+                for (int i = 0; i < n; i++) delta += std::fabs(rho_out[s][i] - rho_in[s][i]) * w[i];
+            if (delta < SCF_TOL) {
+                FullAtom a;
+                a.z = z;
+                a.restricted = o.restricted;
+                a.total = full_energy(grid, z, orb, rho_out, v);
+                double up = 0.0, dn = 0.0;
+                for (const auto &q : orb) up += q.occ[0], dn += q.occ[1];
+                a.multiplicity = (int)std::lround(std::fabs(up - dn)) + 1;
+                a.orbitals = std::move(orb);
+                a.rho[0] = rho_out[0];
+                a.rho[1] = rho_out[1];
+                full_potentials(grid, z, rho_out, a.v);
+                return a;
+            }
+            have_good = true;
+            for (int s = 0; s < 2; s++) {
+                g_in[s] = rho_in[s];
+                g_out[s] = rho_out[s];
+            }
+            g_orb = orb;
+            prev = orb;
+            std::vector<double> x(2 * n), y(2 * n);
+            for (int i = 0; i < n; i++) {
+                x[i] = rho_in[0][i], x[n + i] = rho_in[1][i];
+                y[i] = rho_out[0][i], y[n + i] = rho_out[1][i];
+            }
+            const auto mixed = diis.mix(x, y, w, MIXING / (double)(1ull << halvings));
+            for (int i = 0; i < n; i++) {
+                rho_in[0][i] = mixed[i];
+                rho_in[1][i] = mixed[n + i];
+            }
+        } else {
+            if (!have_good) throw Unbound("Z = " + std::to_string(z) + ": the first step binds no shell");
+            if (halvings == BACKTRACK_MAX) {
+                exhausted = true;
+                throw Unbound("Z = " + std::to_string(z) + ": an occupied shell is unbound after backtracking");
+            }
+            halvings++;
+            const double beta = MIXING / (double)(1ull << halvings);
+            diis.clear();
+            for (int s = 0; s < 2; s++)
+                for (int i = 0; i < n; i++) rho_in[s][i] = g_in[s][i] + beta * (g_out[s][i] - g_in[s][i]);
+            prev = g_orb;
+        }
+    }
+    throw Fail("Z = " + std::to_string(z) + ": the radial SCF did not converge");
+}
+
+FullAtom solve_atom_full(int z, const AtomOpts &o) {
+    bool exhausted = false;
+    try {
+        return full_pass(z, o, true, exhausted);
+    } catch (const Unbound &) {
+        if (!exhausted) throw;
+    }
+    return full_pass(z, o, false, exhausted);
+}
+
+// the spin-polarised ground state: the lowest total energy over the allowed multiplicities (the parity of N, up to 5,
+// while M - 1 <= N; atom.rs:172-175, 541-575)
+FullAtom ground_state_polarised(int z) {
+    std::optional<FullAtom> best;
+    std::string last;
+    for (int m = 1 + z % 2; m <= 5; m += 2) {
+        if (m - 1 > z) break;
+        AtomOpts o;
+        o.restricted = false;
+        o.multiplicity = m;
+        try {
+            FullAtom a = solve_atom_full(z, o);
+            if (!best || a.total < best->total) best = std::move(a);
+        } catch (const Fail &e) {
+            last = e.what();
+        }
+    }
+    if (!best) throw Fail("Hund atom: " + last);
+    return std::move(*best);
+}
+
+// ---- the radial KB atom (kb.rs) ----
+struct KbProj {
+    int l;
+    double e_kb;
+    std::vector<double> beta;
+};
+
+struct KbInput {
+    int z = 0;
+    std::vector<double> v_loc, rho_pc;
+    std::vector<KbProj> proj;
+    const KbProj *of(int l) const {
+        for (const auto &p : proj)
+            if (p.l == l) return &p;
+        return nullptr;
+    }
+};
+
+struct KbAtom {
+    std::vector<std::array<double, 2>> eps;
+    std::vector<std::array<std::vector<double>, 2>> u;
+    std::vector<double> rho[2], v[2];
+    double energy = 0.0;
+};
+
+int support_index(const std::vector<double> &beta) {
+    double m = 0.0;
+    for (double b : beta) m = std::max(m, std::fabs(b));
+    for (int i = (int)beta.size() - 1; i >= 0; i--)
+        if (std::fabs(beta[i]) > 1e-12 * m) return i;
+    return 0;
+}
+
+int matching_index(const std::vector<double> &beta) { return support_index(beta) + 3; }
+
+// the solution vanishing at r_{i_start}, run inward to the first point (kb.rs:330-338)
+std::vector<double> inward_dirichlet(const std::vector<double> &g, int i_start, double delta) {
+    const double k = delta * delta / 12.0;
+    std::vector<double> y(i_start + 1, 0.0);
+    y[i_start - 1] = 1e-20;
+    for (int i = i_start - 1; i >= 1; i--) {
+        y[i - 1] = (2.0 * y[i] * (1.0 + 5.0 * k * g[i]) - y[i + 1] * (1.0 - k * g[i + 1])) / (1.0 - k * g[i - 1]);
+        if (std::fabs(y[i - 1]) > 1e150)
+            for (int q = i - 1; q <= i_start; q++) y[q] *= 1e-150;
+    }
+    return y;
+}
+
+int kb_wall(const std::vector<double> &g, double delta, int i_m) {
+    return std::min(std::max(shooting_end(g, delta), i_m + 4), (int)g.size() - 1);
+}
+
+struct Green {
+    std::vector<double> uh, ui, ch;
+    double w = 0.0, w_scale = 0.0;
+};
+
+// the pieces of the Green-function form at energy e (kb.rs:295-328)
+Green green(const LogGrid &grid, const std::vector<double> &v, const std::vector<double> &beta, int l, double e, int i_m) {
+    const int n = grid.size();
+    const auto g = g_coeff(grid, v, l, e);
+    double y0, y1;
+    regular_start(grid, l, 0.0, y0, y1);
+    const auto yh = outward(g, y0, y1, i_m + 2, grid.delta, nullptr);
+    const int ie = kb_wall(g, grid.delta, i_m);
+    const auto yi = inward_dirichlet(g, ie, grid.delta);
+    Green gr;
+    gr.uh.resize(i_m + 3);
+    for (int i = 0; i <= i_m + 2; i++) gr.uh[i] = yh[i] * std::sqrt(grid.r[i]);
+    gr.ui.assign(n, 0.0);
+    for (int i = 0; i <= ie; i++) gr.ui[i] = yi[i] * std::sqrt(grid.r[i]);
+    auto dx = [&](const std::vector<double> &f, int i) {
+        return (f[i - 2] - 8.0 * f[i - 1] + 8.0 * f[i + 1] - f[i + 2]) / (12.0 * grid.delta);
+    };
+    int iw = 2;
+    for (int i = 2; i <= i_m - 2; i++)
+        if (std::fabs(g[i]) < std::fabs(g[iw])) iw = i;
+    const double a = gr.uh[iw] * dx(gr.ui, iw), b = dx(gr.uh, iw) * gr.ui[iw];
+    std::vector<double> f(n, 0.0);
+    for (int i = 0; i <= i_m; i++) f[i] = beta[i] * gr.uh[i];
+    gr.ch = grid.cumulative(f);
+    gr.w = (a - b) / grid.r[iw];
+    gr.w_scale = (std::fabs(a) + std::fabs(b)) / grid.r[iw];
+    return gr;
+}
+
+// E_KB W s(e) = -W + 4 E_KB int_0^{r_m} beta u_i C_h dr, and W (kb.rs:341-346)
+std::pair<double, double> kb_green(const LogGrid &grid, const std::vector<double> &v, double e_kb,
+                                   const std::vector<double> &beta, int l, double e, int i_m) {
+    const Green gr = green(grid, v, beta, l, e, i_m);
+    std::vector<double> f(grid.size(), 0.0);
+    for (int i = 0; i <= i_m; i++) f[i] = beta[i] * gr.ui[i] * gr.ch[i];
+    const double j = grid.partial(f, i_m);
+    return {-gr.w + 4.0 * e_kb * j, gr.w};
+}
+
+double kb_lower_bound(const LogGrid &grid, const std::vector<double> &v, int l, const KbProj *p) {
+    double floor = INFINITY;
+    for (int i = 0; i < grid.size(); i++) floor = std::min(floor, v[i] + l * (l + 1.0) / (2.0 * grid.r[i] * grid.r[i]));
+    if (!p) return floor;
+    std::vector<double> bb(grid.size());
+    for (int i = 0; i < grid.size(); i++) bb[i] = p->beta[i] * p->beta[i];
+    return floor + std::min(p->e_kb * grid.integrate(bb), 0.0);
+}
+
+int local_count(const LogGrid &grid, const std::vector<double> &g, int l, int i_end) {
+    double y0, y1;
+    regular_start(grid, l, 0.0, y0, y1);
+    return outward_nodes(g, y0, y1, i_end, grid.delta);
+}
+
+// the exact number of KB eigenvalues of channel l below e: n_loc(e) + [s(e) < 0] - [E_KB > 0] (kb.rs:371-383)
+int kb_count_below(const LogGrid &grid, const std::vector<double> &v, const KbProj *p, int l, double e) {
+    const auto g = g_coeff(grid, v, l, e);
+    if (!p) return local_count(grid, g, l, shooting_end(g, grid.delta));
+    const int i_m = matching_index(p->beta);
+    const int n_loc = local_count(grid, g, l, kb_wall(g, grid.delta, i_m));
+    const auto dw = kb_green(grid, v, p->e_kb, p->beta, l, e, i_m);
+    const double sv = dw.first / (p->e_kb * dw.second);
+    return std::max(n_loc + (sv < 0.0 ? 1 : 0) - (p->e_kb > 0.0 ? 1 : 0), 0);
+}
+
+// every eigenvalue of channel l in [e_lo, e_hi] (kb.rs:390-418)
+std::vector<double> kb_eigenvalues(const LogGrid &grid, const std::vector<double> &v, const KbProj *p, int l, double e_lo,
+                                   double e_hi, double mesh) {
+    std::vector<double> roots;
+    if (!p) {
+        for (int k = 0;; k++) {
+            auto b = bound_state(grid, v, l, k, 0.0, std::nullopt);
+            if (!b || b->e >= e_hi) break;
+            if (b->e >= e_lo) roots.push_back(b->e);
+        }
+        return roots;
+    }
+    const int i_m = matching_index(p->beta);
+    auto det = [&](double e) { return kb_green(grid, v, p->e_kb, p->beta, l, e, i_m).first; };
+    const int steps = std::max(1, (int)std::ceil((e_hi - e_lo) / mesh));
+    double a = e_lo, fa = det(e_lo);
+    for (int k = 1; k <= steps; k++) {
+        const double b = std::min(e_lo + k * mesh, e_hi), fb = det(b);
+        if (fa * fb < 0.0) {
+            double lo = a, hi = b, flo = fa;
+            while (hi - lo > 1e-13 * std::max(std::fabs(lo), 1.0)) {
+                const double mid = 0.5 * (lo + hi), fm = det(mid);
+                if (flo * fm <= 0.0) hi = mid;
+                else {
+                    lo = mid;
+                    flo = fm;
+                }
+            }
+            roots.push_back(0.5 * (lo + hi));
+        }
+        a = b;
+        fa = fb;
+    }
+    return roots;
+}
+
+// the normalised eigenfunction at an eigenvalue, u proportional to G beta (kb.rs:429-446)
+std::vector<double> kb_state(const LogGrid &grid, const std::vector<double> &v, const KbProj *p, int l, double e) {
+    if (!p) return eigenfunction(grid, v, l, e, 0.0).u;
+    const int n = grid.size();
+    const int i_m = matching_index(p->beta);
+    const Green gr = green(grid, v, p->beta, l, e, i_m);
+    if (std::fabs(gr.w) < 1e-6 * gr.w_scale) return eigenfunction(grid, v, l, e, 0.0).u;
+    std::vector<double> f(n, 0.0);
+    for (int i = 0; i <= i_m; i++) f[i] = gr.ui[i] * p->beta[i];
+    const auto di = grid.cumulative(f);
+    const double d_tot = di[i_m];
+    std::vector<double> u(n);
+    for (int i = 0; i < n; i++)
+        u[i] = i <= i_m ? gr.ui[i] * gr.ch[i] + gr.uh[i] * (d_tot - di[i]) : gr.ui[i] * gr.ch[i_m];
+    std::vector<double> sq(n);
+    double umax = 0.0;
+    for (int i = 0; i < n; i++) {
+        sq[i] = u[i] * u[i];
+        umax = std::max(umax, std::fabs(u[i]));
+    }
+    const double norm = std::sqrt(grid.integrate(sq));
+    int ip = 0;
+    for (int i = n - 2; i >= 1; i--)
+        if (std::fabs(u[i]) >= std::fabs(u[i - 1]) && std::fabs(u[i]) >= std::fabs(u[i + 1]) &&
+            std::fabs(u[i]) > R_PEAK_FLOOR * umax) {
+            ip = i;
+            break;
+        }
+    const double sgn = u[ip] < 0.0 ? -1.0 : 1.0;
+    for (auto &x : u) x = sgn * x / norm;
+    return u;
+}
+
+// the lowest eigenvalue of channel l by bisection on the exact count (kb.rs:497-516)
+std::optional<double> kb_lowest(const LogGrid &grid, const std::vector<double> &v, const KbProj *p, int l, double prev) {
+    const std::optional<double> guess = std::isfinite(prev) ? std::optional<double>(prev) : std::nullopt;
+    if (!p) {
+        auto b = bound_state(grid, v, l, 0, 0.0, guess);
+        return b ? std::optional<double>(b->e) : std::nullopt;
+    }
+    auto count = [&](double e) { return kb_count_below(grid, v, p, l, e); };
+    double hi = 0.0;
+    if (count(hi) == 0) return std::nullopt;
+    double lo = kb_lower_bound(grid, v, l, p);
+    if (guess) {
+        const double gs = *guess, w = 0.1 * std::max(std::fabs(gs), 0.1);
+        if (gs - w > lo && count(gs - w) == 0) lo = gs - w;
+        if (gs + w < hi && count(gs + w) >= 1) hi = gs + w;
+    }
+    while (hi - lo > 1e-13 * std::max(std::fabs(lo), 1.0)) {
+        const double mid = 0.5 * (lo + hi);
+        if (count(mid) >= 1) hi = mid;
+        else lo = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+void kb_potential(const LogGrid &grid, const KbInput &in, const std::vector<double> *rho, std::vector<double> *v) {
+    const int n = grid.size();
+    std::vector<double> total(n);
+    for (int i = 0; i < n; i++) total[i] = rho[0][i] + rho[1][i];
+    const auto vh = hartree_potential(grid, total);
+    v[0].resize(n);
+    v[1].resize(n);
+    for (int i = 0; i < n; i++) {
+        const double c = 0.5 * in.rho_pc[i];
+        double vu, vd;
+        pw92(rho[0][i] + c, rho[1][i] + c, &vu, &vd);
+        v[0][i] = in.v_loc[i] + vh[i] + vu;
+        v[1][i] = in.v_loc[i] + vh[i] + vd;
+    }
+}
+
+using ValenceConfig = std::vector<std::array<double, 2>>;
+
+// one SCF pass of the KB atom (kb.rs:532-609)
+KbAtom kb_scf_pass(const LogGrid &grid, const KbInput &in, const ValenceConfig &cfg, bool restricted, bool linear,
+                   bool &exhausted) {
+    const int n = grid.size(), nl = (int)cfg.size();
+    std::vector<double> w(2 * n);
+    for (int i = 0; i < n; i++) w[i] = w[n + i] = grid.r[i] * grid.r[i] * grid.r[i] * grid.delta;
+    std::vector<std::array<double, 2>> eps(nl, {NAN, NAN});
+    std::vector<std::array<std::vector<double>, 2>> us(nl);
+    auto occupy = [&](const std::vector<double> *v, std::vector<double> *rho) {
+        rho[0].assign(n, 0.0);
+        rho[1].assign(n, 0.0);
+        for (int l = 0; l < nl; l++) {
+            const KbProj *p = in.of(l);
+            for (int s = 0; s < 2; s++) {
+                if (cfg[l][s] <= 0.0) continue;
+                if (restricted && s == 1 && cfg[l][0] > 0.0) {
+                    eps[l][1] = eps[l][0];
+                    us[l][1] = us[l][0];
+                } else {
+                    const int src = restricted ? 0 : s;
+                    const auto e = kb_lowest(grid, v[src], p, l, eps[l][s]);
+                    if (!e) throw Unbound("the KB atom does not bind l = " + std::to_string(l));
+                    us[l][s] = kb_state(grid, v[src], p, l, *e);
+                    eps[l][s] = *e;
+                }
+                for (int i = 0; i < n; i++)
+                    rho[s][i] += cfg[l][s] * us[l][s][i] * us[l][s][i] / (4.0 * M_PI * grid.r[i] * grid.r[i]);
+            }
+        }
+    };
+    std::vector<double> rho_in[2] = {std::vector<double>(n, 0.0), std::vector<double>(n, 0.0)};
+    Diis diis(linear ? 1 : DIIS_HISTORY);
+    bool have_good = false;
+    std::vector<double> g_in[2], g_out[2];
+    std::vector<std::array<double, 2>> g_eps;
+    std::vector<std::array<std::vector<double>, 2>> g_us;
+    int halvings = 0;
+    exhausted = false;
+    for (int it = 0; it < SCF_MAX_ITER; it++) {
+        std::vector<double> v[2], rho_out[2];
+        kb_potential(grid, in, rho_in, v);
+        try {
+            occupy(v, rho_out);
+        } catch (const Unbound &) {
+            if (!have_good) throw;
+            if (halvings == BACKTRACK_MAX) {
+                exhausted = true;
+                throw;
+            }
+            halvings++;
+            const double beta = MIXING / (double)(1ull << halvings);
+            diis.clear();
+            for (int s = 0; s < 2; s++)
+                for (int i = 0; i < n; i++) rho_in[s][i] = g_in[s][i] + beta * (g_out[s][i] - g_in[s][i]);
+            eps = g_eps;
+            us = g_us;
+            continue;
+        }
+        if (!linear) halvings = 0;
+        double delta = 0.0;
+        for (int s = 0; s < 2; s++)
+            for (int i = 0; i < n; i++) delta += std::fabs(rho_out[s][i] - rho_in[s][i]) * w[i];
+        if (delta < SCF_TOL) {
+            Neumaier band, pot;
+            for (int l = 0; l < nl; l++)
+                for (int s = 0; s < 2; s++)
+                    if (cfg[l][s] > 0.0) band.add(cfg[l][s] * eps[l][s]);
+            std::vector<double> f(n), total(n);
+            for (int s = 0; s < 2; s++) {
+                for (int i = 0; i < n; i++) f[i] = rho_out[s][i] * v[s][i];
+                pot.add(shell_integral(grid, f));
+            }
+            for (int i = 0; i < n; i++) total[i] = rho_out[0][i] + rho_out[1][i];
+            const auto vh = hartree_potential(grid, total);
+            for (int i = 0; i < n; i++) f[i] = total[i] * in.v_loc[i];
+            const double e_loc = shell_integral(grid, f);
+            for (int i = 0; i < n; i++) f[i] = total[i] * vh[i];
+            const double e_h = 0.5 * shell_integral(grid, f);
+            for (int i = 0; i < n; i++) {
+                const double c = 0.5 * in.rho_pc[i];
+                f[i] = pw92(rho_out[0][i] + c, rho_out[1][i] + c);
+            }
+            const double e_xc = shell_integral(grid, f);
+            Neumaier e;
+            e.add(band.sum());
+            e.add(-pot.sum());
+            e.add(e_loc);
+            e.add(e_h);
+            e.add(e_xc);
+            KbAtom a;
+            a.eps = eps;
+            a.u = us;
+            a.rho[0] = rho_out[0];
+            a.rho[1] = rho_out[1];
+            kb_potential(grid, in, rho_out, a.v);
+            a.energy = e.sum();
+            return a;
+        }
+        have_good = true;
+        for (int s = 0; s < 2; s++) {
+            g_in[s] = rho_in[s];
+            g_out[s] = rho_out[s];
+        }
+        g_eps = eps;
+        g_us = us;
+        std::vector<double> x(2 * n), y(2 * n);
+        for (int i = 0; i < n; i++) {
+            x[i] = rho_in[0][i], x[n + i] = rho_in[1][i];
+            y[i] = rho_out[0][i], y[n + i] = rho_out[1][i];
+        }
+        const auto mixed = diis.mix(x, y, w, MIXING / (double)(1ull << halvings));
+        for (int i = 0; i < n; i++) {
+            rho_in[0][i] = mixed[i];
+            rho_in[1][i] = mixed[n + i];
+        }
+    }
+    throw Fail("the KB atom's SCF did not converge");
+}
+
+KbAtom solve_kb_atom(const KbInput &in, const ValenceConfig &cfg, bool restricted) {
+    const LogGrid &grid = universal_grid();
+    bool exhausted = false;
+    try {
+        return kb_scf_pass(grid, in, cfg, restricted, false, exhausted);
+    } catch (const Unbound &) {
+        if (!exhausted) throw;
+    }
+    return kb_scf_pass(grid, in, cfg, restricted, true, exhausted);
 }
 
 // ---- the band-limiting filter (filter.rs) ----
@@ -1273,24 +1914,338 @@ struct Candidate {
     double t5[4] = {0, 0, 0, 0};
 };
 
-Candidate candidate(const AtomResult &ref, const Partition &part, const PartialCore &pc, const std::vector<double> &tau,
-                    double f, int local) {
+// What the gates need of an element, independent of the ladder entry: the all-electron bound spectrum above the core
+// per l (partition.rs:118-121), the restricted reference's total energy, and the all-electron atoms of the transfer
+// configurations and of T6's shifted occupations, solved once and kept.
+struct ElementData {
+    const AtomResult &ref;
+    const Partition &part;
+    const PartialCore &pc;
+    std::vector<std::vector<double>> spectrum;
+    double ref_energy = 0.0;
+    std::optional<FullAtom> hund;
+    std::string hund_error;
+    bool hund_tried = false;
+    std::deque<std::pair<std::string, std::optional<FullAtom>>> ae_cache;  // key -> atom (nullopt: unbound); a deque
+                                                                            // keeps references to its entries valid
+    std::vector<std::pair<std::string, std::string>> ae_errors;             // key -> a failure that is not unbound
+};
+
+ValenceConfig reference_config(const Partition &part) {
+    ValenceConfig c(part.l_max + 1, std::array<double, 2>{0.0, 0.0});
+    for (const auto &ch : part.channels)
+        if (ch.occupied) c[ch.l] = {0.5 * ch.occ, 0.5 * ch.occ};
+    return c;
+}
+
+// the core full in both spins plus the valence channels' (n, l) at the configuration's occupations (gates.rs:120-129)
+std::vector<ShellOcc> ae_shells(const Partition &part, const ValenceConfig &val) {
+    std::vector<ShellOcc> v;
+    for (const Shell *s : part.core) v.push_back({s->n, s->l, {0.5 * s->occ, 0.5 * s->occ}});
+    for (int l = 0; l < (int)val.size(); l++) {
+        if (val[l][0] + val[l][1] <= 0.0) continue;
+        int n = -1;
+        for (const auto &c : part.channels)
+            if (c.l == l) n = c.n;
+        if (n < 0) throw Fail("a configuration occupies an l without a channel");
+        v.push_back({n, l, {val[l][0], val[l][1]}});
+    }
+    return v;
+}
+
+std::string config_key(const ValenceConfig &c) {
+    std::string k;
+    for (const auto &o : c) k += std::to_string(o[0]) + "/" + std::to_string(o[1]) + ";";
+    return k;
+}
+
+// the restricted all-electron atom at fixed occupations, cached per configuration; nullopt when it is unbound
+const std::optional<FullAtom> &ae_fixed(ElementData &d, const ValenceConfig &val) {
+    const std::string key = config_key(val);
+    for (auto &e : d.ae_cache)
+        if (e.first == key) return e.second;
+    for (auto &e : d.ae_errors)
+        if (e.first == key) throw Fail(e.second);
+    AtomOpts o;
+    o.restricted = true;
+    o.aufbau = false;
+    o.fixed = ae_shells(d.part, val);
+    try {
+        d.ae_cache.emplace_back(key, solve_atom_full(d.part.z, o));
+    } catch (const Unbound &) {
+        d.ae_cache.emplace_back(key, std::nullopt);
+    } catch (const Fail &e) {
+        d.ae_errors.emplace_back(key, e.what());
+        throw;
+    }
+    return d.ae_cache.back().second;
+}
+
+const FullAtom &hund_atom(ElementData &d) {
+    if (!d.hund_tried) {
+        d.hund_tried = true;
+        try {
+            d.hund = ground_state_polarised(d.part.z);
+        } catch (const Fail &e) {
+            d.hund_error = e.what();
+        }
+    }
+    if (!d.hund) throw Fail("Hund atom: " + d.hund_error);
+    return *d.hund;
+}
+
+KbInput kb_input(const KbPotential &kb, const PartialCore &pc, int z) {
+    KbInput in;
+    in.z = z;
+    in.v_loc = kb.v_loc;
+    in.rho_pc = pc.rho_pc;
+    for (const auto &p : kb.projectors) in.proj.push_back({p.l, p.e_kb, p.beta});
+    return in;
+}
+
+struct Missing {
+    int l;
+    std::vector<double> ae, roots;
+};
+
+// G2 (kb.rs:635-663): the reference KB atom reproduces the occupied eigenvalues to 1 mHa, and per l <= l_max the exact
+// count of KB levels below top = min(0, eps_v,max + 0.5) neither exceeds the all-electron atom's levels of l above its
+// core (more: a ghost) nor disagrees with the roots located on the 1 mHa mesh; fewer is recorded for T5
+std::vector<Missing> ghost_g2(const KbInput &in, const KbAtom &atom, const ElementData &d) {
     const LogGrid &g = universal_grid();
+    double err = 0.0, eps_v_max = -INFINITY;
+    for (const auto &c : d.part.channels)
+        if (c.occupied) {
+            err = std::max(err, std::fabs(atom.eps[c.l][0] - c.eps));
+            eps_v_max = std::max(eps_v_max, c.eps);
+        }
+    if (err >= 1e-3) throw Fail("G2: KB eigenvalues miss the reference by " + std::to_string(err * 1e3) + " mHa");
+    const double top = std::min(eps_v_max + 0.5, 0.0);
+    std::vector<Missing> missing;
+    for (int l = 0; l <= d.part.l_max; l++) {
+        const KbProj *p = in.of(l);
+        const int count = kb_count_below(g, atom.v[0], p, l, top);
+        const auto roots = kb_eigenvalues(g, atom.v[0], p, l, kb_lower_bound(g, atom.v[0], l, p), top, 1e-3);
+        if ((int)roots.size() != count)
+            throw Fail("G2: l = " + std::to_string(l) + ": the mesh located " + std::to_string(roots.size()) +
+                       " roots but the inertia count is " + std::to_string(count));
+        std::vector<double> ae;
+        for (double e : d.spectrum[l])
+            if (e < top) ae.push_back(e);
+        if (count > (int)ae.size()) throw Fail("G2: l = " + std::to_string(l) + " has an extra bound state (ghost)");
+        if (count < (int)ae.size()) missing.push_back({l, ae, roots});
+    }
+    return missing;
+}
+
+// T7 in one configuration (gates.rs:221-245): per spin and l <= l_max the KB count below top must not exceed the
+// all-electron levels of l above the core there
+void t7_counts(const KbInput &in, const ElementData &d, const FullAtom &ae, const KbAtom &pp, bool restricted,
+               const std::string &name) {
+    const LogGrid &g = universal_grid();
+    double eps_v_max = -INFINITY;
+    for (const auto &o : ae.orbitals) {
+        bool valence = false;
+        for (const Shell *v : d.part.valence)
+            if (v->n == o.n && v->l == o.l) valence = true;
+        if (!valence) continue;
+        for (int s = 0; s < 2; s++)
+            if (o.occ[s] > 0.0) eps_v_max = std::max(eps_v_max, o.eps[s]);
+    }
+    const double top = std::min(eps_v_max + 0.5, 0.0);
+    for (int s = 0; s < (restricted ? 1 : 2); s++)
+        for (int l = 0; l <= d.part.l_max; l++) {
+            int n_core_l = 0;
+            for (const Shell *c : d.part.core)
+                if (c->l == l) n_core_l++;
+            int levels = 0;
+            for (int k = n_core_l;; k++) {
+                auto b = bound_state(g, ae.v[s], l, k, d.part.z, std::nullopt);
+                if (!b || b->e >= top) break;
+                levels++;
+            }
+            const int n_kb = kb_count_below(g, pp.v[s], in.of(l), l, top);
+            if (n_kb > levels)
+                throw Fail("T7 " + name + " spin " + std::to_string(s) + ", l = " + std::to_string(l) +
+                           ": an extra KB bound state (ghost)");
+        }
+}
+
+// T1-T7 on a candidate that passed G1/G2 (gates.rs:251-371); T5 on the channels with G2's missing AE states
+void run_gates(Candidate &c, const KbInput &in, const KbAtom &pp_ref, const std::vector<Missing> &missing,
+               ElementData &d) {
+    const LogGrid &g = universal_grid();
+    const Partition &part = d.part;
+    // T1: every channel with a reference state, an unoccupied one through its lowest KB level
+    double t1 = 0.0;
+    for (const auto &ch : part.channels) {
+        double e;
+        if (ch.occupied) e = pp_ref.eps[ch.l][0];
+        else {
+            const auto lo = kb_lowest(g, pp_ref.v[0], in.of(ch.l), ch.l, NAN);
+            if (!lo) throw Fail("T1: the unoccupied l = " + std::to_string(ch.l) + " reference state is not bound");
+            e = *lo;
+        }
+        t1 = std::max(t1, std::fabs(e - ch.eps));
+    }
+    if (t1 >= 1e-3) throw Fail("T1: " + std::to_string(t1 * 1e3) + " mHa");
+    // T2: norm conservation as a regression
+    for (size_t k = 0; k < c.channels.size(); k++) {
+        const TmChannel &tm = c.channels[k];
+        std::vector<double> a(g.size()), b(g.size());
+        for (int i = 0; i < g.size(); i++) {
+            a[i] = tm.u[i] * tm.u[i];
+            b[i] = part.channels[k].u[i] * part.channels[k].u[i];
+        }
+        if (std::fabs(g.partial(a, tm.i_c) - g.partial(b, tm.i_c)) >= 1e-10) throw Fail("T2: norm error");
+    }
+    // T3 / T4 / T7 over the transfer configurations: cation, s -> p, anion, Hund
+    const ValenceConfig base = reference_config(part);
+    auto total = [](const ValenceConfig &v, int l) { return v[l][0] + v[l][1]; };
+    auto cap = [](int l) { return 2.0 * (2 * l + 1); };
+    auto moved = [&](int from, int to) {
+        ValenceConfig v = base;
+        if (from >= 0) v[from][0] -= 0.5, v[from][1] -= 0.5;
+        if (to >= 0) v[to][0] += 0.5, v[to][1] += 0.5;
+        return v;
+    };
+    auto has = [&](int l) {
+        for (const auto &ch : part.channels)
+            if (ch.l == l) return true;
+        return false;
+    };
+    auto eps_of = [&](int l) -> double {
+        for (const auto &ch : part.channels)
+            if (ch.l == l) return ch.eps;
+        return INFINITY;
+    };
+    int homo = -1;
+    double homo_e = -INFINITY;
+    for (const auto &ch : part.channels)
+        if (ch.occupied && ch.eps > homo_e) homo_e = ch.eps, homo = ch.l;
+    struct Config {
+        std::string name;
+        ValenceConfig pp;
+        bool restricted;
+    };
+    std::vector<Config> configs{{"cation", moved(homo, -1), true}};
+    if (has(0) && has(1) && total(base, 0) >= 1.0 && total(base, 1) <= cap(1) - 1.0)
+        configs.push_back({"s->p", moved(0, 1), true});
+    int room = -1;
+    for (int l = 0; l < (int)base.size(); l++)
+        if (has(l) && total(base, l) < cap(l) && (room < 0 || eps_of(l) < eps_of(room))) room = l;
+    if (room >= 0) configs.push_back({"anion", moved(-1, room), true});
+    const FullAtom &hund = hund_atom(d);
+    ValenceConfig hund_pp(base.size(), std::array<double, 2>{0.0, 0.0});
+    for (const auto &o : hund.orbitals)
+        for (const Shell *v : part.valence)
+            if (v->n == o.n && v->l == o.l) hund_pp[o.l] = {o.occ[0], o.occ[1]};
+    configs.push_back({"Hund", hund_pp, false});
+    for (const auto &cfg : configs) {
+        if (cfg.name == "Hund") {
+            double n_v = 0.0;
+            for (const auto &o : cfg.pp) n_v += o[0] + o[1];
+            if (std::fabs(n_v - part.z_v) > 1e-9) throw Fail("T3 Hund: the polarised ground state's valence count differs");
+        }
+        const FullAtom *ae = nullptr;
+        if (cfg.name == "Hund") ae = &hund;
+        else {
+            const auto &r = ae_fixed(d, cfg.pp);
+            if (r) ae = &*r;
+        }
+        std::optional<KbAtom> pp;
+        try {
+            pp = solve_kb_atom(in, cfg.pp, cfg.restricted);
+        } catch (const Unbound &) {
+        }
+        if (!ae || !pp) {
+            if (cfg.name == "anion") continue;  // not bound in both: an LSDA property, recorded by the reference
+            throw Fail("T3 " + cfg.name + ": an atom of the configuration is unbound");
+        }
+        const double de_ae = ae->total - d.ref_energy, de_pp = pp->energy - pp_ref.energy;
+        double worst = 0.0;
+        for (const auto &o : ae->orbitals) {
+            bool valence = false;
+            for (const Shell *v : part.valence)
+                if (v->n == o.n && v->l == o.l) valence = true;
+            if (!valence) continue;
+            for (int s = 0; s < 2; s++)
+                if (o.occ[s] > 0.0) worst = std::max(worst, std::fabs(o.eps[s] - pp->eps[o.l][s]));
+        }
+        if (std::fabs(de_ae - de_pp) >= 5e-3)
+            throw Fail("T3 " + cfg.name + ": dE_AE " + std::to_string(de_ae) + " vs dE_PP " + std::to_string(de_pp));
+        if (worst >= 5e-3) throw Fail("T4 " + cfg.name + ": eigenvalue error " + std::to_string(worst * 1e3) + " mHa");
+        t7_counts(in, d, *ae, *pp, cfg.restricted, cfg.name);
+    }
+    // T5: every generated channel on eps_l +- 0.5 Ha plus the AE states of its l the KB atom does not bind
+    for (size_t k = 0; k < part.channels.size() && k < 4; k++) {
+        const int l = part.channels[k].l;
+        std::vector<double> extra;
+        for (const auto &m : missing)
+            if (m.l == l)
+                for (size_t q = m.roots.size(); q < m.ae.size(); q++) extra.push_back(m.ae[q]);
+        c.t5[k] = t5_channel(g, d.ref, c.channels, c.kb, c.screen, l, part.channels[k].eps, extra);
+        if (c.t5[k] >= T5_DE_EQUIV)
+            throw Fail("T5 l = " + std::to_string(l) + ": max delta E_equiv " + std::to_string(c.t5[k] * 1e3) + " mHa");
+    }
+    // T6: d eps_l / d f_l' by +-0.25 e (restricted), AE vs PP; a full shell one-sided over [f - 0.5, f]
+    std::vector<int> occupied;
+    for (const auto &ch : part.channels)
+        if (ch.occupied) occupied.push_back(ch.l);
+    for (int lp : occupied) {
+        const bool full = total(base, lp) + 0.25 > cap(lp) + 1e-12;
+        const double hi = full ? 0.0 : 0.25, lo = full ? -0.5 : -0.25;
+        ValenceConfig cp = base, cm = base;
+        cp[lp][0] += 0.5 * hi, cp[lp][1] += 0.5 * hi;
+        cm[lp][0] += 0.5 * lo, cm[lp][1] += 0.5 * lo;
+        const auto &ae_p = ae_fixed(d, cp);
+        const auto &ae_m = ae_fixed(d, cm);
+        if (!ae_p || !ae_m) throw Fail("T6: an all-electron atom is unbound");
+        KbAtom pp_p, pp_m;
+        try {
+            pp_p = solve_kb_atom(in, cp, true);
+            pp_m = solve_kb_atom(in, cm, true);
+        } catch (const Unbound &) {
+            throw Fail("T6: a KB atom is unbound");
+        }
+        for (int l : occupied) {
+            int n = 0;
+            for (const auto &ch : part.channels)
+                if (ch.l == l) n = ch.n;
+            auto ae_eps = [&](const FullAtom &a) -> double {
+                for (const auto &o : a.orbitals)
+                    if (o.n == n && o.l == l) return o.eps[0];
+                return NAN;
+            };
+            const double slope_ae = (ae_eps(*ae_p) - ae_eps(*ae_m)) / (hi - lo);
+            const double slope_pp = (pp_p.eps[l][0] - pp_m.eps[l][0]) / (hi - lo);
+            if (!(std::fabs(slope_ae - slope_pp) < 10e-3))
+                throw Fail("T6 d eps_" + std::to_string(l) + " / d f_" + std::to_string(lp));
+        }
+    }
+}
+
+Candidate candidate(ElementData &d, const std::vector<double> &tau, double f, int local) {
+    const LogGrid &g = universal_grid();
+    const Partition &part = d.part;
     Candidate c;
-    c.channels = tm_channels(ref, part, f);
+    c.channels = tm_channels(d.ref, part, f);
     std::vector<double> occ;
     for (const auto &ch : part.channels) occ.push_back(ch.occ);
-    c.un = unscreen(g, part.z_v, c.channels, occ, pc.rho_pc);
+    c.un = unscreen(g, part.z_v, c.channels, occ, d.pc.rho_pc);
     c.screen.resize(g.size());
     for (int i = 0; i < g.size(); i++) c.screen[i] = c.channels[0].v_scr[i] - c.un.v_ion[0][i];
     c.kb = kb_form(g, part.z_v, c.channels, c.un, local, tau);
     ghost_g1(g, c.kb, c.screen, part);
-    for (size_t k = 0; k < part.channels.size() && k < 4; k++) {
-        c.t5[k] = t5_channel(g, ref, c.channels, c.kb, c.screen, part.channels[k].l, part.channels[k].eps);
-        if (c.t5[k] >= T5_DE_EQUIV)
-            throw Fail("T5 l = " + std::to_string(part.channels[k].l) + ": max delta E_equiv " +
-                       std::to_string(c.t5[k] * 1e3) + " mHa");
+    const KbInput in = kb_input(c.kb, d.pc, part.z);
+    KbAtom pp_ref;
+    try {
+        pp_ref = solve_kb_atom(in, reference_config(part), true);
+    } catch (const Fail &e) {
+        throw Fail(std::string("G2: ") + e.what());
     }
+    const auto missing = ghost_g2(in, pp_ref, d);
+    run_gates(c, in, pp_ref, missing, d);
     return c;
 }
 
@@ -1361,29 +2316,60 @@ std::string g_pseudo_error;
 
 }  // namespace
 
-Table *generate(int z, double h, int f_index, int local) {
-    if (z < 1 || z > Z_MAX) throw Fail("Z = " + std::to_string(z) + " is outside 1..18");
+// The accepted ladder entry of one element, independent of h: the reference atom, its partition and partial core, the
+// candidate and its record. Kept for the process (a generation runs every gate: seconds of host time), keyed by
+// (Z, f_index, local); a refusal is kept as its reason.
+struct Generated {
+    AtomResult ref;
+    std::unique_ptr<Partition> part;
+    PartialCore pc;
+    Candidate c;
+    lira_pseudo_record rec{};
+};
+
+namespace {
+std::mutex g_gen_mutex;
+std::map<std::tuple<int, int, int>, std::shared_ptr<Generated>> g_generated;
+std::map<std::tuple<int, int, int>, std::string> g_refused;
+}  // namespace
+
+std::shared_ptr<Generated> generate_entry(int z, int f_index, int local) {
     const LogGrid &g = universal_grid();
-    const AtomResult ref = ground_state_restricted(z);
-    const Partition part = partition(ref);
+    auto gen = std::make_shared<Generated>();
+    gen->ref = ground_state_restricted(z);
+    gen->part = std::make_unique<Partition>(partition(gen->ref));
+    const Partition &part = *gen->part;
     const auto rho_c = density_of(part.core), rho_v = density_of(part.valence);
-    const PartialCore pc = partial_core(g, rho_c, rho_v);
+    gen->pc = partial_core(g, rho_c, rho_v);
     const auto tau = core_tail(g, rho_c, part.n_core);
+    ElementData d{gen->ref, part, gen->pc};
+    for (int l = 0; l <= part.l_max; l++) {
+        int n_core_l = 0;
+        for (const Shell *c : part.core)
+            if (c->l == l) n_core_l++;
+        std::vector<double> levels;
+        for (int k = n_core_l;; k++) {
+            auto b = bound_state(g, gen->ref.v, l, k, z, std::nullopt);
+            if (!b) break;
+            levels.push_back(b->e);
+        }
+        d.spectrum.push_back(std::move(levels));
+    }
+    d.ref_energy = solve_atom_full(z, AtomOpts{}).total;
     std::string tried;
     for (int fi = 0; fi < 4; fi++) {
         if (f_index >= 0 && fi != f_index) continue;
         for (int lo = 0; lo < 2; lo++) {
             if (local >= 0 && lo != local) continue;
-            Candidate c;
             try {
-                c = candidate(ref, part, pc, tau, F_LADDER[fi], lo);
+                gen->c = candidate(d, tau, F_LADDER[fi], lo);
             } catch (const Fail &e) {
                 tried += std::string(tried.empty() ? "" : "; ") + "f = " + std::to_string(F_LADDER[fi]).substr(0, 3) +
                          (lo == 0 ? " highest-channel local: " : " polynomial local: ") + e.what();
                 continue;
             }
-            auto t = std::make_unique<Table>();
-            lira_pseudo_record &r = t->rec;
+            const Candidate &c = gen->c;
+            lira_pseudo_record &r = gen->rec;
             r.z = z;
             r.f = F_LADDER[fi];
             r.local = lo;
@@ -1404,15 +2390,40 @@ Table *generate(int z, double h, int f_index, int local) {
                 r.gss[k] = c.kb.projectors[k].gss;
                 r.r_beta[k] = g.r[c.kb.projectors[k].i_beta];
             }
-            r.r_nlc = pc.r_nlc;
-            r.r_core = pc.r_core;
+            r.r_nlc = gen->pc.r_nlc;
+            r.r_core = gen->pc.r_core;
             r.r_cmp = c.kb.r_cmp;
-            r.h = h;
-            filter_tables(*t, part, pc, c, h);
-            return t.release();
+            return gen;
         }
     }
     throw Fail("Z = " + std::to_string(z) + ": no ladder entry passes: " + tried);
+}
+
+Table *generate(int z, double h, int f_index, int local) {
+    if (z < 1 || z > Z_MAX) throw Fail("Z = " + std::to_string(z) + " is outside 1..18");
+    const auto key = std::make_tuple(z, f_index, local);
+    std::shared_ptr<Generated> gen;
+    {
+        std::lock_guard<std::mutex> lock(g_gen_mutex);
+        auto it = g_generated.find(key);
+        if (it != g_generated.end()) gen = it->second;
+        else {
+            auto bad = g_refused.find(key);
+            if (bad != g_refused.end()) throw Fail(bad->second);
+            try {
+                gen = generate_entry(z, f_index, local);
+            } catch (const Fail &e) {
+                g_refused[key] = e.what();
+                throw;
+            }
+            g_generated[key] = gen;
+        }
+    }
+    auto t = std::make_unique<Table>();
+    t->rec = gen->rec;
+    t->rec.h = h;
+    filter_tables(*t, *gen->part, gen->pc, gen->c, h);
+    return t.release();
 }
 
 const std::string &last_error() { return g_pseudo_error; }
