@@ -217,8 +217,10 @@ lira_status lira_set_ionic(lira_solver *s, const lira_species *species, int n_sp
  * Z in, a norm-conserving Troullier-Martins / Kleinman-Bylander ion with the LFC partial core out, filtered for a grid
  * of spacing h (qm::radial::filter::filtered, filter.rs:384). The ladder tries f in {1.6, 1.4, 1.2, 1.0} (softest
  * first) and, per f, the highest generated channel as the local potential, then the polynomial local
- * (ledger.rs:181-182); the first entry that passes the gates restated here (G1, kb.rs:611-634; T5, gates.rs:313-325)
- * is kept. f_index (0..3) and local (0, 1) pin one entry instead (-1: the ladder). Z in 1..18. */
+ * (ledger.rs:181-182); the first entry that passes the ghost gates G1/G2 (kb.rs:611-663) and the transferability
+ * gates T1-T7 (gates.rs:251-371) is kept. The dimer gate of Z <= 2 is not run: H and He come out as the reference's
+ * ungated generate() makes them. f_index (0..3) and local (0, 1) pin one entry instead (-1: the ladder). Z in 1..18.
+ * The accepted entry of each (Z, f_index, local) is kept for the process; later calls only filter it for h. */
 typedef enum { LIRA_PSEUDO_LOCAL_HIGHEST = 0, LIRA_PSEUDO_LOCAL_POLYNOMIAL = 1 } lira_pseudo_local;
 
 /* The generation record (qm::radial::record::GenerationRecord, crates/qm/src/radial/record.rs; the columns of the
