@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -2316,6 +2317,126 @@ std::string g_pseudo_error;
 
 }  // namespace
 
+// ---- the dimer gate on f for Z <= 2 (crates/providers/src/pseudo.rs:798-1100, recipe.rs:152-176) ----
+constexpr double DIMER_H = 0.15, DIMER_BOX = 12.0, DIMER_RE = 0.02, DIMER_OMEGA_REL = 0.05, DIMER_WALL_REL = 0.05;
+constexpr double DIMER_BRACKET[2] = {1.0, 4.0}, DIMER_SIGMAS[4] = {1.0, 1.2, 1.4, 1.6};
+constexpr double AMU_IN_ME = 1822.888486209;  // the unified atomic mass unit in electron masses (CODATA 2018)
+
+// the most abundant isotope's mass (u) of the elements the gate applies to: 1H, 4He (AME2020)
+double isotope_mass_amu(int z) { return z == 1 ? 1.00782503223 : 4.00260325413; }
+
+// the lira_species view of a filtered table (pointers into t)
+void species_of(const Table &t, lira_species &sp, lira_projector *proj) {
+    std::memset(&sp, 0, sizeof(sp));
+    sp.z = t.rec.z;
+    sp.z_v = t.rec.z_v;
+    sp.sigma_lr = t.sigma_lr;
+    sp.du = t.du;
+    sp.v_sr_u = t.v_sr_u.data();
+    sp.n_v_sr = (int)t.v_sr_u.size();
+    sp.r_sr = t.r_sr;
+    const bool core = t.pc_radius > 0.0;
+    sp.rho_pc_u = core ? t.rho_pc_u.data() : nullptr;
+    sp.n_rho_pc = core ? (int)t.rho_pc_u.size() : 0;
+    sp.pc_radius = t.pc_radius;
+    for (int k = 0; k < t.rec.n_projectors; k++) {
+        proj[k].l = t.rec.proj_l[k];
+        proj[k].e_kb = t.rec.e_kb[k];
+        proj[k].radius = t.proj_radius[k];
+        proj[k].t_u = t.t_u[k].data();
+        proj[k].n_t = (int)t.t_u[k].size();
+    }
+    sp.projectors = t.rec.n_projectors > 0 ? proj : nullptr;
+    sp.n_projectors = t.rec.n_projectors;
+    sp.r_core = t.r_core;
+}
+
+// One gate's solver: the bond force 1/2 (F_1,x - F_0,x) of a cold dimer solve (nucleus 1 at +R/2) through the C ABI
+struct DimerSolver {
+    lira_solver *s = nullptr;
+    int solves = 0;
+    DimerSolver() {
+        const int n = 2 * (int)std::lround(DIMER_BOX / (2.0 * DIMER_H)) + 1;
+        if (lira_solver_create(n, DIMER_H, LIRA_MIXED, &s) != LIRA_OK) throw Fail("dimer gate: no GPU solver");
+    }
+    ~DimerSolver() {
+        if (s) lira_solver_destroy(s);
+    }
+    DimerSolver(const DimerSolver &) = delete;
+    DimerSolver &operator=(const DimerSolver &) = delete;
+    // electrons per spin: a singlet of 2 Z (all-electron) or 2 Z_v (pseudised) electrons
+    double force(int z, double r, const lira_species *sp, double sigma) {
+        if (lira_set_ionic(s, sp, sp ? 1 : 0) != LIRA_OK) throw Fail("dimer gate: lira_set_ionic failed");
+        lira_nucleus nuc[2] = {{{-0.5 * r, 0.0, 0.0}, z}, {{0.5 * r, 0.0, 0.0}, z}};
+        const int per_spin = sp ? (int)std::lround(sp->z_v) : z;
+        lira_system sys{nuc, 2, per_spin, per_spin};
+        lira_settings st;
+        lira_default_settings(DIMER_H, &st);
+        st.sigma = sigma;
+        st.tol = 1e-6;
+        st.eig_tol = 1e-7;
+        st.occupations = LIRA_FERMI_DIRAC;
+        st.kt = 5e-3;
+        lira_scf_stats stats;
+        const lira_status status = lira_solve(s, &sys, &st, 0, nullptr, nullptr, &stats);
+        solves++;
+        if (status != LIRA_OK)
+            throw Fail(std::string("dimer gate: the solve at R = ") + std::to_string(r) + " failed: " + lira_last_error(s));
+        double f[6];
+        if (lira_forces(s, nuc, 2, sigma, f) != LIRA_OK) throw Fail("dimer gate: forces failed");
+        return 0.5 * (f[3] - f[0]);
+    }
+    // the all-electron force at sigma -> 0: a + b s^2 + c s^3 + d s^4 through the four widths (qm::extrapolate_sigma)
+    double all_electron(int z, double r) {
+        std::vector<double> a(16), b(4), x;
+        for (int k = 0; k < 4; k++) {
+            const double sg = DIMER_SIGMAS[k] * DIMER_H;
+            b[k] = force(z, r, nullptr, sg);
+            a[k * 4 + 0] = 1.0;
+            a[k * 4 + 1] = sg * sg;
+            a[k * 4 + 2] = sg * sg * sg;
+            a[k * 4 + 3] = sg * sg * sg * sg;
+        }
+        if (!Diis::lu_solve(a, b, 4, x)) throw Fail("dimer gate: the sigma extrapolation is singular");
+        return x[0];
+    }
+};
+
+struct DimerFit {
+    bool bracketed = false;
+    double r_e = NAN, omega = NAN;
+    double lo = 0.0, hi = 0.0, f_lo = 0.0, f_hi = 0.0;
+};
+
+// R_e by bisection of the force's sign to 0.1 x 0.02 Bohr, then linear interpolation; k = -dF/dR by +-h (pseudo.rs:
+// 912-948)
+DimerFit dimer_fit(int z, double lo, double hi, const std::function<double(double)> &force) {
+    DimerFit out;
+    out.lo = lo, out.hi = hi;
+    double a = lo, b = hi, fa = force(a), fb = force(b);
+    out.f_lo = fa, out.f_hi = fb;
+    if (!(fa > 0.0 && fb < 0.0)) return out;
+    while (b - a > 0.1 * DIMER_RE) {
+        const double m = 0.5 * (a + b), fm = force(m);
+        if (fm > 0.0) a = m, fa = fm;
+        else b = m, fb = fm;
+    }
+    out.bracketed = true;
+    out.r_e = a + (b - a) * fa / (fa - fb);
+    const double d = DIMER_H;
+    const double k = -(force(out.r_e + d) - force(out.r_e - d)) / (2.0 * d);
+    if (!(k > 0.0)) throw Fail("dimer gate: k <= 0 at R_e");
+    const double mu = 0.5 * isotope_mass_amu(z) * AMU_IN_ME;
+    out.omega = std::sqrt(k / mu);
+    return out;
+}
+
+struct AeDimer {
+    bool done = false;
+    DimerFit fit;
+    std::string error;
+};
+
 // The accepted ladder entry of one element, independent of h: the reference atom, its partition and partial core, the
 // candidate and its record. Kept for the process (a generation runs every gate: seconds of host time), keyed by
 // (Z, f_index, local); a refusal is kept as its reason.
@@ -2329,11 +2450,15 @@ struct Generated {
 
 namespace {
 std::mutex g_gen_mutex;
-std::map<std::tuple<int, int, int>, std::shared_ptr<Generated>> g_generated;
-std::map<std::tuple<int, int, int>, std::string> g_refused;
+std::map<std::tuple<int, int, int, bool>, std::shared_ptr<Generated>> g_generated;
+std::map<std::tuple<int, int, int, bool>, std::string> g_refused;
+AeDimer g_ae_dimer[3];  // per Z <= 2: the all-electron dimer does not depend on the ladder entry
 }  // namespace
 
-std::shared_ptr<Generated> generate_entry(int z, int f_index, int local) {
+// gate (optional): run on every entry that passes the atomic gates; false refuses the entry with the reason in why
+using EntryGate = std::function<bool(Generated &, std::string &why)>;
+
+std::shared_ptr<Generated> generate_entry(int z, int f_index, int local, const EntryGate &gate = nullptr) {
     const LogGrid &g = universal_grid();
     auto gen = std::make_shared<Generated>();
     gen->ref = ground_state_restricted(z);
@@ -2393,15 +2518,92 @@ std::shared_ptr<Generated> generate_entry(int z, int f_index, int local) {
             r.r_nlc = gen->pc.r_nlc;
             r.r_core = gen->pc.r_core;
             r.r_cmp = c.kb.r_cmp;
+            r.dimer_pp[0] = r.dimer_pp[1] = r.dimer_ae[0] = r.dimer_ae[1] = NAN;
+            r.provisional = z <= 2 ? 1 : 0;
+            if (gate && z <= 2) {
+                std::string why;
+                if (!gate(*gen, why)) {
+                    tried += std::string(tried.empty() ? "" : "; ") + "f = " + std::to_string(F_LADDER[fi]).substr(0, 3) +
+                             (lo == 0 ? " highest-channel local: " : " polynomial local: ") + "dimer gate " + why;
+                    continue;
+                }
+                r.provisional = 0;
+            }
             return gen;
         }
     }
     throw Fail("Z = " + std::to_string(z) + ": no ladder entry passes: " + tried);
 }
 
-Table *generate(int z, double h, int f_index, int local) {
+// the dimer gate of one ladder entry (recipe.rs:152-176 dimer_verdict, pseudo.rs:1018-1098 GridDimerGate::dimers)
+bool dimer_gate(DimerSolver &solver, Generated &gen, std::string &why) {
+    const Partition &part = *gen.part;
+    const int z = part.z;
+    double r_peak = 0.0;
+    for (const auto &c : part.channels)
+        if (c.occupied) r_peak = std::max(r_peak, c.r_peak);
+    const double lo = DIMER_BRACKET[0] * r_peak, hi = DIMER_BRACKET[1] * r_peak;
+    AeDimer &ae = g_ae_dimer[z];
+    if (!ae.done) {
+        ae.done = true;
+        try {
+            ae.fit = dimer_fit(z, lo, hi, [&](double r) { return solver.all_electron(z, r); });
+        } catch (const Fail &e) {
+            ae.error = e.what();
+        }
+    }
+    if (!ae.error.empty()) {
+        why = "could not run: " + ae.error;
+        return false;
+    }
+    Table t;
+    t.rec = gen.rec;
+    filter_tables(t, part, gen.pc, gen.c, DIMER_H);
+    lira_species sp;
+    lira_projector proj[3];
+    species_of(t, sp, proj);
+    lira_pseudo_record &r = gen.rec;
+    r.dimer_ae[0] = ae.fit.r_e;
+    r.dimer_ae[1] = ae.fit.omega;
+    try {
+        if (ae.fit.bracketed) {
+            const DimerFit pp = dimer_fit(z, lo, hi, [&](double x) { return solver.force(z, x, &sp, DIMER_H); });
+            r.dimer_pp[0] = pp.r_e;
+            r.dimer_pp[1] = pp.omega;
+            if (!pp.bracketed) {
+                why = "fail: the pseudised dimer is unbound where the all-electron one is bound";
+                return false;
+            }
+            const double dr = pp.r_e - ae.fit.r_e, dw = pp.omega / ae.fit.omega - 1.0;
+            if (!(std::fabs(dr) < DIMER_RE && std::fabs(dw) < DIMER_OMEGA_REL)) {
+                why = "fail: PP R_e " + std::to_string(pp.r_e) + " Bohr, omega " + std::to_string(pp.omega) +
+                      " a.u. vs AE R_e " + std::to_string(ae.fit.r_e) + ", omega " + std::to_string(ae.fit.omega);
+                return false;
+            }
+            r.dimer = 1;
+            return true;
+        }
+        // no all-electron force zero in the bracket: not applicable if the pseudised repulsive wall agrees at the outer end
+        solver.force(z, lo, &sp, DIMER_H);
+        const double p_hi = solver.force(z, hi, &sp, DIMER_H);
+        const double wall = p_hi / ae.fit.f_hi - 1.0;
+        if (std::fabs(wall) > DIMER_WALL_REL) {
+            why = "fail: no AE minimum in the bracket and the pseudised repulsive wall differs by " +
+                  std::to_string(100.0 * wall) + " %";
+            return false;
+        }
+        r.dimer = 2;
+        return true;
+    } catch (const Fail &e) {
+        why = std::string("could not run: ") + e.what();
+        return false;
+    }
+}
+
+Table *generate(int z, double h, int f_index, int local, bool gated = false) {
     if (z < 1 || z > Z_MAX) throw Fail("Z = " + std::to_string(z) + " is outside 1..18");
-    const auto key = std::make_tuple(z, f_index, local);
+    gated = gated && z <= 2;
+    const auto key = std::make_tuple(z, f_index, local, gated);
     std::shared_ptr<Generated> gen;
     {
         std::lock_guard<std::mutex> lock(g_gen_mutex);
@@ -2411,7 +2613,14 @@ Table *generate(int z, double h, int f_index, int local) {
             auto bad = g_refused.find(key);
             if (bad != g_refused.end()) throw Fail(bad->second);
             try {
-                gen = generate_entry(z, f_index, local);
+                if (gated) {
+                    DimerSolver solver;
+                    gen = generate_entry(z, f_index, local,
+                                         [&](Generated &g2, std::string &why) { return dimer_gate(solver, g2, why); });
+                    gen->rec.dimer_solves = solver.solves;
+                } else {
+                    gen = generate_entry(z, f_index, local);
+                }
             } catch (const Fail &e) {
                 g_refused[key] = e.what();
                 throw;
@@ -2441,6 +2650,20 @@ extern "C" lira_status lira_pseudo_generate(int z, double h, int f_index, int lo
     *out = nullptr;
     try {
         radial::Table *t = radial::generate(z, h, f_index, local);
+        *out = static_cast<lira_pseudo *>(t);
+        radial::set_error("");
+        return LIRA_OK;
+    } catch (const std::exception &e) {
+        radial::set_error(e.what());
+        return LIRA_NUMERICAL;
+    }
+}
+
+extern "C" lira_status lira_pseudo_generate_gated(int z, double h, lira_pseudo **out) {
+    if (!out || !(h > 0.0)) return LIRA_INVALID;
+    *out = nullptr;
+    try {
+        radial::Table *t = radial::generate(z, h, -1, -1, true);
         *out = static_cast<lira_pseudo *>(t);
         radial::set_error("");
         return LIRA_OK;

@@ -35,7 +35,7 @@ EXPORTS = [
     "lira_inject_readback_fault", "lira_gram_noalloc", "lira_mg_vcycle", "lira_mg_level_op",
     "lira_solve_footprint_bytes", "lira_witness_cap_for_budget", "lira_unoccupied", "lira_harris",
     "lira_stream_bandwidth", "lira_rayleigh_ritz_device", "lira_set_device_rr", "lira_pseudo_generate",
-    "lira_pseudo_error", "lira_pseudo_record_of", "lira_pseudo_species", "lira_pseudo_free",
+    "lira_pseudo_error", "lira_pseudo_record_of", "lira_pseudo_species", "lira_pseudo_free", "lira_pseudo_generate_gated",
 ]
 
 
@@ -69,7 +69,9 @@ class PseudoRecord(C.Structure):
                 ("n_core", C.c_int), ("n_channels", C.c_int), ("channel_l", C.c_int * 4), ("r_c", C.c_double * 4),
                 ("eps", C.c_double * 4), ("t5", C.c_double * 4), ("n_projectors", C.c_int), ("proj_l", C.c_int * 3),
                 ("e_kb", C.c_double * 3), ("gss", C.c_double * 3), ("r_beta", C.c_double * 3), ("r_nlc", C.c_double),
-                ("r_core", C.c_double), ("r_cmp", C.c_double), ("r_pc_support", C.c_double), ("h", C.c_double)]
+                ("r_core", C.c_double), ("r_cmp", C.c_double), ("r_pc_support", C.c_double), ("h", C.c_double),
+                ("provisional", C.c_int), ("dimer", C.c_int), ("dimer_pp", C.c_double * 2), ("dimer_ae", C.c_double * 2),
+                ("dimer_solves", C.c_int)]
 
 
 class QmError(RuntimeError):
@@ -269,6 +271,7 @@ def lib():
         L.lira_set_ionic.argtypes = [vp, vp, C.c_int]
         L.lira_free_atom_levels.argtypes = [C.c_int, _dp, C.POINTER(C.c_double)]
         L.lira_pseudo_generate.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(vp)]
+        L.lira_pseudo_generate_gated.argtypes = [C.c_int, C.c_double, C.POINTER(vp)]
         L.lira_pseudo_error.argtypes = []
         L.lira_pseudo_error.restype = C.c_char_p
         L.lira_pseudo_record_of.argtypes = [vp, C.POINTER(PseudoRecord)]
@@ -735,16 +738,23 @@ class PseudoSpecies:
             "channels": [(rec.channel_l[k], rec.r_c[k], rec.eps[k], rec.t5[k]) for k in range(nc)],
             "projectors": [(rec.proj_l[k], rec.e_kb[k], rec.gss[k], rec.r_beta[k]) for k in range(npj)],
             "r_nlc": rec.r_nlc if rec.r_nlc > 0 else None, "r_core": rec.r_core, "r_cmp": rec.r_cmp,
-            "r_pc_support": rec.r_pc_support, "h": rec.h,
+            "r_pc_support": rec.r_pc_support, "h": rec.h, "provisional": bool(rec.provisional),
+            "dimer": {0: None, 1: "pass", 2: "not applicable"}[rec.dimer],
+            "dimer_pp": (rec.dimer_pp[0], rec.dimer_pp[1]), "dimer_ae": (rec.dimer_ae[0], rec.dimer_ae[1]),
+            "dimer_solves": rec.dimer_solves,
         }
         self.record["local"] = "polynomial" if rec.local == PSEUDO_LOCAL_POLYNOMIAL else "spdf"[rec.local_l] + "-local"
 
 
-def generate_pseudo(z: int, h: float, f_index: int = -1, local: int = -1) -> PseudoSpecies:
-    """qm::radial::generate(z) filtered for spacing h (lira_pseudo_generate). Raises Numerical with the reason when
-    the element is refused."""
+def generate_pseudo(z: int, h: float, f_index: int = -1, local: int = -1, gated: bool = False) -> PseudoSpecies:
+    """qm::radial::generate(z) filtered for spacing h (lira_pseudo_generate); gated: with the dimer gate on f for
+    Z <= 2 (lira_pseudo_generate_gated, solves on the current GPU). Raises Numerical with the reason when the element
+    is refused."""
     handle = C.c_void_p()
-    st = lib().lira_pseudo_generate(int(z), float(h), int(f_index), int(local), C.byref(handle))
+    if gated:
+        st = lib().lira_pseudo_generate_gated(int(z), float(h), C.byref(handle))
+    else:
+        st = lib().lira_pseudo_generate(int(z), float(h), int(f_index), int(local), C.byref(handle))
     if st != 0:
         why = lib().lira_pseudo_error().decode()
         raise {4: Numerical, 6: Invalid}.get(st, QmError)(f"lira_pseudo_generate: {why}")

@@ -248,12 +248,29 @@ typedef struct {
     double r_cmp;
     double r_pc_support;
     double h;
+    int provisional;     /* Z <= 2 generated without the dimer gate (the reference's ungated generate: usable by no
+                            setup there); 0 for every other table */
+    int dimer;           /* the dimer gate's verdict: 0 not run, 1 pass, 2 not applicable (no all-electron force zero in
+                            the bracket, the pseudised repulsive wall within 5 % of the all-electron one) */
+    double dimer_pp[2];  /* the pseudised dimer's R_e (Bohr) and omega (a.u.) */
+    double dimer_ae[2];  /* the all-electron dimer's at sigma -> 0; NaN when not located */
+    int dimer_solves;    /* SCF solves the gate ran for this generation */
 } lira_pseudo_record;
 
 typedef struct lira_pseudo lira_pseudo;
 /* qm::radial::generate + filtered (recipe.rs:116, filter.rs:384). LIRA_NUMERICAL with lira_pseudo_error()'s reason
  * when the element is refused (no ladder entry passes, two valence states of one l, a failed radial solve). */
 lira_status lira_pseudo_generate(int z, double h, int f_index, int local, lira_pseudo **out);
+/* qm::radial::generate_with and providers::pseudo::GridDimerGate (recipe.rs:116-176, crates/providers/src/pseudo.rs:
+ * 798-1100): the ladder with the molecular gate on f for Z <= 2. For each entry that passes the atomic gates the
+ * homonuclear dimer (two nuclei at +-R/2 on x, singlet) is solved on the current GPU on 81^3 x 0.15 (a 12 Bohr cube),
+ * mixed precision, cold, Fermi-Dirac kT 5 mHa, tol 1e-6, eig_tol 1e-7: R_e where the bond force changes sign in
+ * [1, 4] r_peak (bisection to 0.002 Bohr, then linear interpolation), k = -dF/dR by +-h, omega = sqrt(2k/m) with m the
+ * most abundant isotope's mass; the all-electron dimer's force at each R extrapolated sigma -> 0 from sigma = h x {1,
+ * 1.2, 1.4, 1.6}. An entry passes when |R_e - R_e,AE| < 0.02 Bohr and |omega/omega_AE - 1| < 0.05, or when the
+ * all-electron dimer has no force zero in the bracket and the pseudised force at its outer end is within 5 % of the
+ * all-electron one. For Z > 2 it is lira_pseudo_generate (the gate does not apply). */
+lira_status lira_pseudo_generate_gated(int z, double h, lira_pseudo **out);
 const char *lira_pseudo_error(void);
 lira_status lira_pseudo_record_of(const lira_pseudo *p, lira_pseudo_record *rec);
 /* The filtered tables as a lira_species for lira_set_ionic (IonicSetup::from_tables, crates/qm/src/ionic.rs); sp and
