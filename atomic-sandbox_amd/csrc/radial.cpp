@@ -1903,9 +1903,19 @@ struct Table {
     std::vector<double> v_sr_u, rho_pc_u;
     std::vector<std::vector<double>> t_u;
     std::vector<double> proj_radius;
+    // the same band-limited functions at the log grid's nodes (filled for the filtered re-run of T8)
+    std::vector<double> v_sr_log, rho_pc_log;
+    std::vector<std::vector<double>> t_log;
 };
 
 namespace {
+
+struct T3Config {
+    std::string name;
+    ValenceConfig pp;
+    bool restricted;
+    double de_ae;
+};
 
 struct Candidate {
     std::vector<TmChannel> channels;
@@ -1913,6 +1923,8 @@ struct Candidate {
     KbPotential kb;
     std::vector<double> screen;
     double t5[4] = {0, 0, 0, 0};
+    KbAtom ref_atom;              // the reference KB atom (unfiltered)
+    std::vector<T3Config> t3;     // the transfer configurations both atoms bind, with the all-electron dE
 };
 
 // What the gates need of an element, independent of the ladder entry: the all-electron bound spectrum above the core
@@ -2175,6 +2187,7 @@ void run_gates(Candidate &c, const KbInput &in, const KbAtom &pp_ref, const std:
         }
         if (std::fabs(de_ae - de_pp) >= 5e-3)
             throw Fail("T3 " + cfg.name + ": dE_AE " + std::to_string(de_ae) + " vs dE_PP " + std::to_string(de_pp));
+        c.t3.push_back({cfg.name, cfg.pp, cfg.restricted, de_ae});
         if (worst >= 5e-3) throw Fail("T4 " + cfg.name + ": eigenvalue error " + std::to_string(worst * 1e3) + " mHa");
         t7_counts(in, d, *ae, *pp, cfg.restricted, cfg.name);
     }
@@ -2247,12 +2260,14 @@ Candidate candidate(ElementData &d, const std::vector<double> &tau, double f, in
     }
     const auto missing = ghost_g2(in, pp_ref, d);
     run_gates(c, in, pp_ref, missing, d);
+    c.ref_atom = std::move(pp_ref);
     return c;
 }
 
 // the filtered tables at spacing h (filter.rs:384-478): V_SR = V_loc + Z_v erf(d/a)/d on R_SR = max(r_cmp, r_floor), every
 // beta_l / r on r_beta, rho_pc on its XC-weight support; each completed and sampled on the uniform 3-D table
-void filter_tables(Table &t, const Partition &part, const PartialCore &pc, const Candidate &c, double h) {
+void filter_tables(Table &t, const Partition &part, const PartialCore &pc, const Candidate &c, double h,
+                   bool with_log = false) {
     const LogGrid &g = universal_grid();
     const int n = g.size();
     const double a = SIGMA_LR_OVER_H * h * std::sqrt(2.0);
@@ -2272,6 +2287,12 @@ void filter_tables(Table &t, const Partition &part, const PartialCore &pc, const
     const Completion cv = complete(g, v_sr, 0, q_c, q_max, GAMMA_R * r_sr, width);
     t.r_sr = GAMMA_R * r_sr + width;
     t.v_sr_u = uniform_fill(t.du, t.r_sr, cv);
+    auto on_log = [&](const Completion &cc) {
+        std::vector<double> out(n);
+        for (int i = 0; i < n; i++) out[i] = cc.value_at(g.r[i]);
+        return out;
+    };
+    if (with_log) t.v_sr_log = on_log(cv);
     // projectors: F_l = beta_u / r on r_beta
     for (const auto &p : c.kb.projectors) {
         std::vector<double> fl(n);
@@ -2280,18 +2301,19 @@ void filter_tables(Table &t, const Partition &part, const PartialCore &pc, const
         const Completion cb = complete(g, fl, p.l, q_c, q_max, GAMMA_R * rb, width);
         t.proj_radius.push_back(GAMMA_R * rb + width);
         t.t_u.push_back(uniform_fill(t.du, t.proj_radius.back(), cb));
+        if (with_log) t.t_log.push_back(on_log(cb));
     }
     // the partial core on its XC-weight support: the smallest r with 4 pi int_r^inf |v_xc| rho_pc r'^2 dr' <= v_tail_floor,
-    // v_xc the mean spin potential on rho_v + rho_pc of the reference KB atom (its valence density: the TM one)
+    // v_xc the mean spin potential on rho_v + rho_pc of the reference KB atom (filter.rs:400-415)
     bool core = false;
     for (double x : pc.rho_pc)
         if (x != 0.0) core = true;
     if (core) {
         std::vector<double> wv(n);
         for (int i = 0; i < n; i++) {
-            const double s = 0.5 * c.un.rho_v[i] + 0.5 * pc.rho_pc[i];
+            const double cc = 0.5 * pc.rho_pc[i];
             double vu, vd;
-            pw92(s, s, &vu, &vd);
+            pw92(c.ref_atom.rho[0][i] + cc, c.ref_atom.rho[1][i] + cc, &vu, &vd);
             wv[i] = 4.0 * M_PI * g.r[i] * g.r[i] * std::fabs(0.5 * (vu + vd)) * pc.rho_pc[i];
         }
         const auto cum = g.cumulative(wv);
@@ -2306,11 +2328,313 @@ void filter_tables(Table &t, const Partition &part, const PartialCore &pc, const
         t.pc_radius = GAMMA_R * r_pc + width;
         t.rho_pc_u = uniform_fill(t.du, t.pc_radius, cp);
         t.rec.r_pc_support = r_pc;
+        if (with_log) t.rho_pc_log = on_log(cp);
     } else {
         t.rho_pc_u.assign(4, 0.0);
         t.pc_radius = 0.0;
+        if (with_log) t.rho_pc_log.assign(n, 0.0);
     }
     t.r_core = pc.r_core;
+}
+
+// ---- T8: the filtered re-run at every ladder spacing, h_req and h_eb (filter.rs:480-676) ----
+constexpr double H_LADDER[5] = {0.35, 0.30, 0.25, 0.20, 0.15};
+constexpr double H_REQ_TOL = 1e-3, H_REQ_FORCE = 2e-6, T3_DELTA_E = 5e-3;
+
+// 4-point Lagrange on a uniform table of an even function, index -1 mirroring index 1: (value, d/dd); 0 beyond it
+std::pair<double, double> uniform_lagrange(const std::vector<double> &f, double du, double d) {
+    const double s = d / du;
+    const long i0 = (long)std::floor(s) - 1;
+    if (i0 + 3 >= (long)f.size()) return {0.0, 0.0};
+    auto at = [&](long i) { return f[(size_t)std::labs(i)]; };
+    const double t = s - (double)i0;
+    const double a = at(i0), b = at(i0 + 1), c = at(i0 + 2), e = at(i0 + 3);
+    const double l0 = -(t - 1.0) * (t - 2.0) * (t - 3.0) / 6.0, l1 = t * (t - 2.0) * (t - 3.0) / 2.0,
+                 l2 = -t * (t - 1.0) * (t - 3.0) / 2.0, l3 = t * (t - 1.0) * (t - 2.0) / 6.0;
+    const double d0 = -((t - 2.0) * (t - 3.0) + (t - 1.0) * (t - 3.0) + (t - 1.0) * (t - 2.0)) / 6.0,
+                 d1 = ((t - 2.0) * (t - 3.0) + t * (t - 3.0) + t * (t - 2.0)) / 2.0,
+                 d2 = -((t - 1.0) * (t - 3.0) + t * (t - 3.0) + t * (t - 1.0)) / 2.0,
+                 d3 = ((t - 1.0) * (t - 2.0) + t * (t - 2.0) + t * (t - 1.0)) / 6.0;
+    return {a * l0 + b * l1 + c * l2 + e * l3, (a * d0 + b * d1 + c * d2 + e * d3) / du};
+}
+
+std::pair<double, double> table_read(const std::vector<double> &f, double du, double radius, double d) {
+    if (d > radius) return {0.0, 0.0};
+    return uniform_lagrange(f, du, d);
+}
+
+// the hard cut at q_max (no completion, no step): the band-limited weight fields of the egg-box probe
+Completion hard_cut(const LogGrid &grid, const std::vector<double> &f, int l, double q_max) {
+    const int n = grid.size();
+    std::vector<double> q, w, tmp(n);
+    q_mesh(0.0, q_max, q, w);
+    Completion c;
+    c.l = l, c.r_cut = INFINITY, c.width = 1.0;
+    for (size_t k = 0; k < q.size(); k++) {
+        for (int i = 0; i < n; i++) tmp[i] = f[i] * sph_bessel(l, q[k] * grid.r[i]) * grid.r[i] * grid.r[i];
+        c.q.push_back(q[k]);
+        c.c.push_back(w[k] * 4.0 * M_PI * grid.integrate(tmp));
+    }
+    return c;
+}
+
+std::vector<double> hard_cut_table(const std::vector<double> &f, int l, double h, double du, double radius) {
+    return uniform_fill(du, radius, hard_cut(universal_grid(), f, l, M_PI / h));
+}
+
+// the lattice points r_p - R (spacing h) within radius of R
+template <typename F> void for_lattice(double h, double radius, const double *r, F &&fn) {
+    const long k = (long)std::ceil(radius / h) + 1;
+    for (long i = -k; i <= k; i++)
+        for (long j = -k; j <= k; j++)
+            for (long l = -k; l <= k; l++) {
+                const double d[3] = {i * h - r[0], j * h - r[1], l * h - r[2]};
+                if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] <= radius * radius) fn(d);
+            }
+}
+
+// max over R in {+-h/4}^3 of |grad_R E(R)| by central differences of step 2e-4 Bohr
+template <typename E> double probe(double h, E &&e) {
+    const double q = h / 4.0, step = 2e-4;
+    double worst = 0.0;
+    for (double sx : {-q, q})
+        for (double sy : {-q, q})
+            for (double sz : {-q, q}) {
+                const double r[3] = {sx, sy, sz};
+                double g2 = 0.0;
+                for (int a = 0; a < 3; a++) {
+                    double p[3] = {r[0], r[1], r[2]}, m[3] = {r[0], r[1], r[2]};
+                    p[a] += step;
+                    m[a] -= step;
+                    const double ga = (e(p) - e(m)) / (2.0 * step);
+                    g2 += ga * ga;
+                }
+                worst = std::max(worst, std::sqrt(g2));
+            }
+    return worst;
+}
+
+double norm3(const double *d) { return std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]); }
+
+// h^3 sum_p w(|r_p - R|) g(|r_p - R|) for a spherical table g with a spherical weight w on a uniform table
+template <typename G> double spherical_probe(double h, double du, const std::vector<double> &w, double radius, G &&g) {
+    const double h3 = h * h * h;
+    return probe(h, [&](const double *r) {
+        Neumaier s;
+        for_lattice(h, radius, r, [&](const double *d) {
+            const double x = norm3(d);
+            s.add(uniform_lagrange(w, du, x).first * g(x));
+        });
+        return h3 * s.sum();
+    });
+}
+
+std::vector<double> mean_vxc(const std::vector<double> *rho, const std::vector<double> &rho_pc) {
+    std::vector<double> v(rho_pc.size());
+    for (size_t i = 0; i < rho_pc.size(); i++) {
+        const double c = 0.5 * rho_pc[i];
+        double vu, vd;
+        pw92(rho[0][i] + c, rho[1][i] + c, &vu, &vd);
+        v[i] = 0.5 * (vu + vd);
+    }
+    return v;
+}
+
+int xc_domain_violations(const std::vector<double> *rho, const std::vector<double> &rho_pc) {
+    int n = 0;
+    for (size_t i = 0; i < rho_pc.size(); i++)
+        if (rho_pc[i] < 0.0)
+            for (int s = 0; s < 2; s++)
+                if (rho[s][i] + 0.5 * rho_pc[i] < 0.0) n++;
+    return n;
+}
+
+// the KB atom's input on the filtered tables at the log nodes: V_loc = V_LR + V_SR, beta_u = r^{l+1} t_l, rho_pc
+KbInput filtered_input(const Table &t, int z) {
+    const LogGrid &g = universal_grid();
+    const int n = g.size();
+    const double a = t.sigma_lr * std::sqrt(2.0), z_v = t.rec.z_v;
+    KbInput in;
+    in.z = z;
+    in.v_loc.resize(n);
+    for (int i = 0; i < n; i++) {
+        const double d = g.r[i];
+        const double v_lr = d < 1e-10 ? -z_v * 2.0 / (std::sqrt(M_PI) * a) : -z_v * std::erf(d / a) / d;
+        in.v_loc[i] = v_lr + t.v_sr_log[i];
+    }
+    for (int k = 0; k < t.rec.n_projectors; k++) {
+        KbProj p{t.rec.proj_l[k], t.rec.e_kb[k], std::vector<double>(n)};
+        for (int i = 0; i < n; i++) p.beta[i] = std::pow(g.r[i], p.l + 1) * t.t_log[k][i];
+        in.proj.push_back(std::move(p));
+    }
+    in.rho_pc = t.rho_pc_log;
+    return in;
+}
+
+// T8(d) (filter.rs:573-611): [F_SR, F_NL, F_pc] of the filtered KB atom `atom`
+std::array<double, 3> table_eggbox(const Table &t, const Partition &part, const Candidate &c, const KbAtom &atom,
+                                   double h) {
+    const LogGrid &g = universal_grid();
+    const int n = g.size();
+    const double du = t.du, h3 = h * h * h;
+    std::vector<double> rho_v(n);
+    for (int i = 0; i < n; i++) rho_v[i] = atom.rho[0][i] + atom.rho[1][i];
+    const auto w_sr = hard_cut_table(rho_v, 0, h, du, t.r_sr);
+    const double f_sr =
+        spherical_probe(h, du, w_sr, t.r_sr, [&](double x) { return table_read(t.v_sr_u, du, t.r_sr, x).first; });
+    double f_pc = 0.0;
+    if (t.pc_radius > 0.0) {
+        const auto w_pc = hard_cut_table(mean_vxc(atom.rho, t.rho_pc_log), 0, h, du, t.pc_radius);
+        f_pc = spherical_probe(h, du, w_pc, t.pc_radius,
+                               [&](double x) { return table_read(t.rho_pc_u, du, t.pc_radius, x).first; });
+    }
+    // E_NL = sum_k sum_m (f_l / (2l+1)) E_KB c_km^2, c_km = h^3 sum_p beta_klm phi_lm
+    struct P {
+        int k, l;
+        double f, e_kb;
+        std::vector<double> phi;
+    };
+    std::vector<P> projs;
+    const ValenceConfig base = reference_config(part);
+    for (int k = 0; k < t.rec.n_projectors; k++) {
+        const int l = t.rec.proj_l[k];
+        if (l > 1) throw Fail("T8: projectors above l = 1 are not probed");
+        const double occ = base[l][0] + base[l][1];
+        const std::vector<double> *u = nullptr;
+        double f = 1.0;
+        if (occ > 0.0) {
+            u = &atom.u[l][0];
+            f = occ;
+        } else {
+            for (const auto &ch : c.channels)
+                if (ch.l == l) u = &ch.u;
+        }
+        std::vector<double> ur(n);
+        for (int i = 0; i < n; i++) ur[i] = (*u)[i] / g.r[i];
+        projs.push_back({k, l, f, t.rec.e_kb[k], hard_cut_table(ur, l, h, du, t.proj_radius[k])});
+    }
+    double f_nl = 0.0;
+    if (!projs.empty())
+        f_nl = probe(h, [&](const double *r) {
+            Neumaier e;
+            for (const P &p : projs) {
+                const double radius = t.proj_radius[p.k];
+                const double nrm2 = (2.0 * p.l + 1.0) / (4.0 * M_PI);
+                Neumaier cm[3];
+                for_lattice(h, radius, r, [&](const double *d) {
+                    const double x = norm3(d);
+                    const double tp = table_read(t.t_u[p.k], du, radius, x).first * uniform_lagrange(p.phi, du, x).first * nrm2;
+                    if (p.l == 0) cm[0].add(tp);
+                    else
+                        for (int m = 0; m < 3; m++) cm[m].add(tp * d[m] * d[m]);  // real solid harmonics of l = 1: x, y, z
+                });
+                const double a = p.f / (2.0 * p.l + 1.0) * p.e_kb;
+                for (int m = 0; m < 2 * p.l + 1; m++) {
+                    const double sm = h3 * cm[m].sum();
+                    e.add(a * sm * sm);
+                }
+            }
+            return e.sum();
+        });
+    return {f_sr, f_nl, f_pc};
+}
+
+// T8(e), reported (filter.rs:528-561): the nonlinear partial-core force F_pc(R) = h^3 sum_p vbar(d) rho_pc'(|d|) d/|d|
+double pc_alias(const Table &t, const KbAtom &atom, double h) {
+    if (t.pc_radius <= 0.0) return 0.0;
+    const double du = t.du, radius = t.pc_radius, q = h / 4.0, h3 = h * h * h;
+    const auto ru = hard_cut_table(atom.rho[0], 0, h, du, radius), rd = hard_cut_table(atom.rho[1], 0, h, du, radius);
+    double worst = 0.0;
+    for (double sx : {-q, q})
+        for (double sy : {-q, q})
+            for (double sz : {-q, q}) {
+                const double r[3] = {sx, sy, sz};
+                Neumaier f[3];
+                for_lattice(h, radius, r, [&](const double *d) {
+                    const double x = norm3(d);
+                    if (x == 0.0) return;
+                    const double u = uniform_lagrange(ru, du, x).first, dn = uniform_lagrange(rd, du, x).first;
+                    const auto pc = table_read(t.rho_pc_u, du, radius, x);
+                    double vu, vd;
+                    pw92(u + 0.5 * pc.first, dn + 0.5 * pc.first, &vu, &vd);
+                    const double w = 0.5 * (vu + vd) * pc.second / x;
+                    for (int a = 0; a < 3; a++) f[a].add(w * d[a]);
+                });
+                const double fx = h3 * f[0].sum(), fy = h3 * f[1].sum(), fz = h3 * f[2].sum();
+                worst = std::max(worst, std::sqrt(fx * fx + fy * fy + fz * fz));
+            }
+    return worst;
+}
+
+struct FilterRow {
+    double h, de, max_eig, t3_max;
+    int clamped;
+    std::array<double, 3> f_table;
+    double f_pc_nl;
+    bool passes() const {
+        return std::fabs(de) < H_REQ_TOL && max_eig < H_REQ_TOL && t3_max < T3_DELTA_E && clamped == 0 &&
+               f_table[0] + f_table[1] + f_table[2] < H_REQ_FORCE;
+    }
+    bool eggbox_certified() const { return f_table[0] + f_table[1] + f_pc_nl < H_REQ_FORCE; }
+};
+
+// the filtered re-run at one spacing (filter.rs:630-658)
+FilterRow filter_row(const Partition &part, const PartialCore &pc, const Candidate &c, double h) {
+    Table t;
+    t.rec.z = part.z;
+    t.rec.z_v = part.z_v;
+    t.rec.n_projectors = (int)c.kb.projectors.size();
+    for (int k = 0; k < t.rec.n_projectors; k++) {
+        t.rec.proj_l[k] = c.kb.projectors[k].l;
+        t.rec.e_kb[k] = c.kb.projectors[k].e_kb;
+    }
+    filter_tables(t, part, pc, c, h, true);
+    const KbInput in = filtered_input(t, part.z);
+    FilterRow row{h, INFINITY, INFINITY, INFINITY, 0, {INFINITY, INFINITY, INFINITY}, INFINITY};
+    KbAtom fa;
+    try {
+        fa = solve_kb_atom(in, reference_config(part), true);
+    } catch (const Fail &) {
+        return row;
+    }
+    row.de = fa.energy - c.ref_atom.energy;
+    row.max_eig = 0.0;
+    for (const auto &ch : part.channels)
+        if (ch.occupied) row.max_eig = std::max(row.max_eig, std::fabs(fa.eps[ch.l][0] - c.ref_atom.eps[ch.l][0]));
+    row.clamped = xc_domain_violations(fa.rho, t.rho_pc_log);
+    row.t3_max = 0.0;
+    std::optional<KbAtom> hund;
+    for (const auto &cfg : c.t3) {
+        try {
+            KbAtom pp = solve_kb_atom(in, cfg.pp, cfg.restricted);
+            row.clamped += xc_domain_violations(pp.rho, t.rho_pc_log);
+            row.t3_max = std::max(row.t3_max, std::fabs((pp.energy - fa.energy) - cfg.de_ae));
+            if (cfg.name == "Hund") hund = std::move(pp);
+        } catch (const Fail &) {
+            row.t3_max = INFINITY;
+        }
+    }
+    row.f_pc_nl = pc_alias(t, fa, h);
+    if (hund) row.f_pc_nl = std::max(row.f_pc_nl, pc_alias(t, *hund, h));
+    row.f_table = table_eggbox(t, part, c, fa, h);
+    return row;
+}
+
+// h_req (filter.rs:661-669) and h_eb (671-676): the coarsest ladder spacing from which every finer row passes; NaN
+// when even the finest fails
+void h_req_of(const Partition &part, const PartialCore &pc, const Candidate &c, double &h_req, double &h_eb) {
+    std::vector<FilterRow> rows;
+    for (double h : H_LADDER) rows.push_back(filter_row(part, pc, c, h));
+    h_req = h_eb = NAN;
+    for (size_t k = 0; k < rows.size(); k++) {
+        bool all = true, all_eb = true;
+        for (size_t j = k; j < rows.size(); j++) {
+            all = all && rows[j].passes();
+            all_eb = all_eb && rows[j].eggbox_certified();
+        }
+        if (all && std::isnan(h_req)) h_req = rows[k].h;
+        if (all_eb && std::isnan(h_eb)) h_eb = rows[k].h;
+    }
 }
 
 std::string g_pseudo_error;
@@ -2520,6 +2844,7 @@ std::shared_ptr<Generated> generate_entry(int z, int f_index, int local, const E
             r.r_cmp = c.kb.r_cmp;
             r.dimer_pp[0] = r.dimer_pp[1] = r.dimer_ae[0] = r.dimer_ae[1] = NAN;
             r.provisional = z <= 2 ? 1 : 0;
+            r.h_req = r.h_eb = NAN;
             if (gate && z <= 2) {
                 std::string why;
                 if (!gate(*gen, why)) {
@@ -2529,6 +2854,7 @@ std::shared_ptr<Generated> generate_entry(int z, int f_index, int local, const E
                 }
                 r.provisional = 0;
             }
+            h_req_of(part, gen->pc, gen->c, r.h_req, r.h_eb);  // T8 on the accepted entry (recipe.rs:166-170)
             return gen;
         }
     }

@@ -9,6 +9,7 @@ module only marshals arguments. It refuses to run without the built library.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 from dataclasses import dataclass, field
 
@@ -71,7 +72,7 @@ class PseudoRecord(C.Structure):
                 ("e_kb", C.c_double * 3), ("gss", C.c_double * 3), ("r_beta", C.c_double * 3), ("r_nlc", C.c_double),
                 ("r_core", C.c_double), ("r_cmp", C.c_double), ("r_pc_support", C.c_double), ("h", C.c_double),
                 ("provisional", C.c_int), ("dimer", C.c_int), ("dimer_pp", C.c_double * 2), ("dimer_ae", C.c_double * 2),
-                ("dimer_solves", C.c_int)]
+                ("dimer_solves", C.c_int), ("h_req", C.c_double), ("h_eb", C.c_double)]
 
 
 class QmError(RuntimeError):
@@ -742,6 +743,7 @@ class PseudoSpecies:
             "dimer": {0: None, 1: "pass", 2: "not applicable"}[rec.dimer],
             "dimer_pp": (rec.dimer_pp[0], rec.dimer_pp[1]), "dimer_ae": (rec.dimer_ae[0], rec.dimer_ae[1]),
             "dimer_solves": rec.dimer_solves,
+            "h_req": None if math.isnan(rec.h_req) else rec.h_req, "h_eb": None if math.isnan(rec.h_eb) else rec.h_eb,
         }
         self.record["local"] = "polynomial" if rec.local == PSEUDO_LOCAL_POLYNOMIAL else "spdf"[rec.local_l] + "-local"
 

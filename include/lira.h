@@ -255,6 +255,9 @@ typedef struct {
     double dimer_pp[2];  /* the pseudised dimer's R_e (Bohr) and omega (a.u.) */
     double dimer_ae[2];  /* the all-electron dimer's at sigma -> 0; NaN when not located */
     int dimer_solves;    /* SCF solves the gate ran for this generation */
+    double h_req;        /* T8: the coarsest spacing of {0.35, 0.30, 0.25, 0.20, 0.15} from which the filtered tables
+                            pass (a)-(d) at every finer one (NaN: none) */
+    double h_eb;         /* T8(e): the same for the egg-box force with the nonlinear partial-core term (reported) */
 } lira_pseudo_record;
 
 typedef struct lira_pseudo lira_pseudo;

@@ -1,8 +1,8 @@
 """The pseudopotential generator (lira_pseudo_generate, host code) against the reference's pinned generation records
 (crates/validate/references.md:2147-2170, compared verbatim by crates/validate/tests/pseudo_gen.rs
 generation_record_table_matches_references_md): per element the ladder entry kept (f, local), r_c per channel, Z_v,
-E_KB and GSS per projector, r_nlc and r_core, printed to 3-4 decimals there and matched here to their last digit; the
-refusals with their reasons, and the T5 value of every refused ladder entry of Be and Al. Runs on the CPU; the
+E_KB and GSS per projector, r_nlc and r_core, printed to 3-4 decimals there and matched here to their last digit, and
+h_req and h_eb (T8's filtered re-run at every ladder spacing); the refusals with their reasons, and the T5 value of every refused ladder entry of Be and Al. Runs on the CPU; the
 generations (each runs every gate of the ladder: seconds of host time) are spread over a process pool.
 """
 import json
@@ -72,6 +72,8 @@ def _check_row(sp, row):
     else:
         assert round(rec["r_nlc"], 3) == pytest.approx(row["r_nlc"], abs=1e-9)
     assert round(rec["r_core"], 3) == pytest.approx(row["r_core"], abs=1e-9)
+    # T8: the coarsest ladder spacing the filtered tables pass from (h_req) and its egg-box twin (h_eb)
+    assert rec["h_req"] == row["h_req"] and rec["h_eb"] == row["h_eb"], (rec["h_req"], rec["h_eb"])
 
 
 @pytest.mark.parametrize("z", sorted(int(z) for z in GOLD["accepted"]))
