@@ -8,13 +8,13 @@
 //   Troullier-Martins channels at r_c = f r_peak (tm.rs), unscreening, the Kleinman-Bylander form with its derived
 //   support r_beta (kb.rs), the ghost gates G1 (GSS) and G2 (the radial KB atom's exact level count, kb.rs:611-663),
 //   the transferability gates T1-T7 (gates.rs: the KB atom solved self-consistently in the cation, s -> p, anion and
-//   Hund configurations, log derivatives, hardness) -> the band-limited tables for spacing h (filter.rs:
-//   King-Smith-Payne-Lin completion, uniform 3-D tables).
+//   Hund configurations, log derivatives, hardness) -> T8 on the accepted entry (the filtered re-run at every ladder
+//   spacing: h_req, h_eb) -> the band-limited tables for spacing h (filter.rs: King-Smith-Payne-Lin completion, uniform
+//   3-D tables).
 //
-// At the reference configuration the KB atom's screening is the unscreening's own, so its valence density is the TM
-// pseudo-density (kb.rs module doc, "The radial KB atom"); the filter's partial-core support uses that density
-// directly. The dimer gate on f for Z <= 2 (a 3-D computation the reference's providers run) and T8's h_req ladder are
-// not restated: H and He come out as the reference's ungated generate() makes them.
+// The molecular gate on f for Z <= 2 (the reference providers' GridDimerGate) runs through the library's own solver
+// when asked for (lira_pseudo_generate_gated); without it H and He come out as the reference's ungated generate()
+// makes them.
 #include <algorithm>
 #include <array>
 #include <cmath>
