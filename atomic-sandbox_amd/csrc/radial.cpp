@@ -1897,8 +1897,11 @@ std::vector<double> uniform_fill(double du, double radius, const Completion &c) 
 }  // namespace
 
 // ---- a generated, filtered pseudopotential ----
+struct Generated;
+
 struct Table {
     lira_pseudo_record rec{};
+    std::shared_ptr<const Generated> gen;  // the accepted entry the tables were filtered from
     double du = 0.0, sigma_lr = 0.0, r_sr = 0.0, pc_radius = 0.0, r_core = 0.0;
     std::vector<double> v_sr_u, rho_pc_u;
     std::vector<std::vector<double>> t_u;
@@ -2957,8 +2960,30 @@ Table *generate(int z, double h, int f_index, int local, bool gated = false) {
     auto t = std::make_unique<Table>();
     t->rec = gen->rec;
     t->rec.h = h;
+    t->gen = gen;
     filter_tables(*t, *gen->part, gen->pc, gen->c, h);
     return t.release();
+}
+
+// ionic::check (crates/qm/src/ionic.rs:84-102): the table may be used at its own spacing h when it is not provisional,
+// h <= h_req (or, with no h_req on the ladder, h finer than the whole ladder), and its filtered re-run at h passes T8
+// (a)-(d). Returns the refusal, empty when the table passes.
+std::string check_table(const Table &t) {
+    const lira_pseudo_record &r = t.rec;
+    const std::string el = "Z = " + std::to_string(r.z);
+    if (r.provisional) return el + ": the pseudopotential is provisional (generated without the dimer gate)";
+    const double finest = H_LADDER[4];
+    const bool fine_enough = std::isnan(r.h_req) ? r.h < finest - 1e-12 : r.h <= r.h_req + 1e-12;
+    if (!fine_enough)
+        return el + ": h = " + std::to_string(r.h) + " Bohr is coarser than the pseudopotential allows (h_req = " +
+               (std::isnan(r.h_req) ? std::string("none on the ladder") : std::to_string(r.h_req) + " Bohr") + ")";
+    const FilterRow row = filter_row(*t.gen->part, t.gen->pc, t.gen->c, r.h);
+    if (!row.passes())
+        return el + ": the filtered pseudopotential fails the filtered gates at h = " + std::to_string(r.h) + ": dE " +
+               std::to_string(row.de * 1e3) + " mHa, d eps " + std::to_string(row.max_eig * 1e3) + " mHa, T3 " +
+               std::to_string(row.t3_max * 1e3) + " mHa, XC clamp at " + std::to_string(row.clamped) +
+               " points, table egg-box force " + std::to_string(row.f_table[0] + row.f_table[1] + row.f_table[2]);
+    return "";
 }
 
 const std::string &last_error() { return g_pseudo_error; }
@@ -3000,6 +3025,18 @@ extern "C" lira_status lira_pseudo_generate_gated(int z, double h, lira_pseudo *
 }
 
 extern "C" const char *lira_pseudo_error(void) { return radial::last_error().c_str(); }
+
+extern "C" lira_status lira_pseudo_check(const lira_pseudo *p) {
+    if (!p) return LIRA_INVALID;
+    try {
+        const std::string why = radial::check_table(*p);
+        radial::set_error(why);
+        return why.empty() ? LIRA_OK : LIRA_NUMERICAL;
+    } catch (const std::exception &e) {
+        radial::set_error(e.what());
+        return LIRA_NUMERICAL;
+    }
+}
 
 extern "C" lira_status lira_pseudo_record_of(const lira_pseudo *p, lira_pseudo_record *rec) {
     if (!p || !rec) return LIRA_INVALID;

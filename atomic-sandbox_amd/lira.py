@@ -37,6 +37,7 @@ EXPORTS = [
     "lira_solve_footprint_bytes", "lira_witness_cap_for_budget", "lira_unoccupied", "lira_harris",
     "lira_stream_bandwidth", "lira_rayleigh_ritz_device", "lira_set_device_rr", "lira_pseudo_generate",
     "lira_pseudo_error", "lira_pseudo_record_of", "lira_pseudo_species", "lira_pseudo_free", "lira_pseudo_generate_gated",
+    "lira_pseudo_check",
 ]
 
 
@@ -273,6 +274,7 @@ def lib():
         L.lira_free_atom_levels.argtypes = [C.c_int, _dp, C.POINTER(C.c_double)]
         L.lira_pseudo_generate.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(vp)]
         L.lira_pseudo_generate_gated.argtypes = [C.c_int, C.c_double, C.POINTER(vp)]
+        L.lira_pseudo_check.argtypes = [vp]
         L.lira_pseudo_error.argtypes = []
         L.lira_pseudo_error.restype = C.c_char_p
         L.lira_pseudo_record_of.argtypes = [vp, C.POINTER(PseudoRecord)]
@@ -717,9 +719,12 @@ class PseudoProjector:
 
 class PseudoSpecies:
     """A generated ion filtered for spacing h (qm::radial::filter::FilteredTable, filter.rs:330-338), in the form
-    GpuSolver.set_ionic takes, with its generation record (`record`, a dict of lira_pseudo_record's fields)."""
+    GpuSolver.set_ionic takes, with its generation record (`record`, a dict of lira_pseudo_record's fields) and the
+    verdict of qm::ionic::check at h (`usable`, and `refusal`: the reason when it is not)."""
 
     def __init__(self, handle):
+        st = lib().lira_pseudo_check(handle)  # ionic::check at the table's own spacing
+        self.usable, self.refusal = st == 0, ("" if st == 0 else lib().lira_pseudo_error().decode())
         rec = PseudoRecord()
         _raise(lib().lira_pseudo_record_of(handle, C.byref(rec)), None, "lira_pseudo_record_of")
         sp = Species()

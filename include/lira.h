@@ -276,6 +276,12 @@ lira_status lira_pseudo_generate(int z, double h, int f_index, int local, lira_p
 lira_status lira_pseudo_generate_gated(int z, double h, lira_pseudo **out);
 const char *lira_pseudo_error(void);
 lira_status lira_pseudo_record_of(const lira_pseudo *p, lira_pseudo_record *rec);
+/* qm::ionic::check, what IonicSetup::build applies to every pseudised species (crates/qm/src/ionic.rs:84-102): LIRA_OK
+ * when the table may be used at the spacing it was filtered for: not provisional, h <= h_req (with no h_req on the
+ * ladder: h finer than 0.15), and its filtered re-run at h passing T8 (a)-(d); else LIRA_NUMERICAL with the reason
+ * (the reference's Provisional, TooCoarse, FilteredGate) in lira_pseudo_error(). lira_set_ionic itself takes tables
+ * unchecked, as IonicSetup::from_tables does. */
+lira_status lira_pseudo_check(const lira_pseudo *p);
 /* The filtered tables as a lira_species for lira_set_ionic (IonicSetup::from_tables, crates/qm/src/ionic.rs); sp and
  * proj[0..n_projectors) point into p, which must outlive them (lira_set_ionic copies them). */
 lira_status lira_pseudo_species(const lira_pseudo *p, lira_species *sp, lira_projector *proj, int proj_cap);

@@ -20,10 +20,11 @@ H = 0.25
 
 
 def _generate(args):
-    z, fi, lo = args
+    z, fi, lo = args[:3]
+    h = args[3] if len(args) > 3 else H
     try:
-        sp = lira.generate_pseudo(z, H, fi, lo)
-        return {"record": sp.record, "radius": [p.radius for p in sp.projectors], "pc_radius": sp.pc_radius,
+        sp = lira.generate_pseudo(z, h, fi, lo)
+        return {"record": sp.record, "usable": sp.usable, "refusal": sp.refusal, "radius": [p.radius for p in sp.projectors], "pc_radius": sp.pc_radius,
                 "sigma_lr": sp.sigma_lr, "du": sp.du,
                 "tables": [sp.v_sr_u[:3].tolist() + [float(sp.v_sr_u[-1])],
                            None if sp.rho_pc_u is None else sp.rho_pc_u[:3].tolist() + [float(sp.rho_pc_u[-1])]]
@@ -40,6 +41,7 @@ def generated():
             jobs.append((int(z), lira.F_LADDER.index(float(f)),
                          lira.PSEUDO_LOCAL_HIGHEST if local == "highest" else lira.PSEUDO_LOCAL_POLYNOMIAL))
     jobs.append((6, 2, lira.PSEUDO_LOCAL_HIGHEST))
+    jobs.append((8, -1, -1, 0.20))
     # spawned workers: a forked child of a process whose OpenMP runtime already started threads (the oracle's tests)
     # can deadlock in it
     ctx = multiprocessing.get_context("spawn")
@@ -134,3 +136,12 @@ def test_pinning_an_entry_and_bad_arguments(generated):
         lira.generate_pseudo(8, H, 7, -1)
     with pytest.raises(lira.Numerical, match="outside 1..18"):
         lira.generate_pseudo(19, H)
+
+
+def test_ionic_check_of_a_table_at_its_spacing(generated):
+    # qm::ionic::check (crates/qm/src/ionic.rs:84-102; its test at :256-262): O's h_req is 0.20, so its table filtered
+    # for 0.25 is too coarse and the one for 0.20 passes its own filtered re-run; the ungated H is provisional
+    o25, o20, h25 = generated[(8, -1, -1)], generated[(8, -1, -1, 0.20)], generated[(1, -1, -1)]
+    assert not o25["usable"] and "coarser than the pseudopotential allows" in o25["refusal"]
+    assert o20["usable"] and o20["refusal"] == ""
+    assert not h25["usable"] and "provisional" in h25["refusal"]
