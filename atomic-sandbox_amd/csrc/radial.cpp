@@ -2640,7 +2640,7 @@ void h_req_of(const Partition &part, const PartialCore &pc, const Candidate &c, 
     }
 }
 
-std::string g_pseudo_error;
+thread_local std::string g_pseudo_error;  // per calling thread, as lira_last_error is per solver
 
 }  // namespace
 
@@ -2996,8 +2996,11 @@ struct lira_pseudo : lira::radial::Table {};
 
 using namespace lira;
 
+// the spacings a table is filtered for: its uniform tables hold radius * 256 / h points
+static bool spacing_ok(double h) { return h >= 0.02 && h <= 1.0; }
+
 extern "C" lira_status lira_pseudo_generate(int z, double h, int f_index, int local, lira_pseudo **out) {
-    if (!out || !(h > 0.0) || f_index < -1 || f_index > 3 || local < -1 || local > 1) return LIRA_INVALID;
+    if (!out || !spacing_ok(h) || f_index < -1 || f_index > 3 || local < -1 || local > 1) return LIRA_INVALID;
     *out = nullptr;
     try {
         radial::Table *t = radial::generate(z, h, f_index, local);
@@ -3011,7 +3014,7 @@ extern "C" lira_status lira_pseudo_generate(int z, double h, int f_index, int lo
 }
 
 extern "C" lira_status lira_pseudo_generate_gated(int z, double h, lira_pseudo **out) {
-    if (!out || !(h > 0.0)) return LIRA_INVALID;
+    if (!out || !spacing_ok(h)) return LIRA_INVALID;
     *out = nullptr;
     try {
         radial::Table *t = radial::generate(z, h, -1, -1, true);

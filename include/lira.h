@@ -219,7 +219,8 @@ lira_status lira_set_ionic(lira_solver *s, const lira_species *species, int n_sp
  * first) and, per f, the highest generated channel as the local potential, then the polynomial local
  * (ledger.rs:181-182); the first entry that passes the ghost gates G1/G2 (kb.rs:611-663) and the transferability
  * gates T1-T7 (gates.rs:251-371) is kept. The dimer gate of Z <= 2 is not run: H and He come out as the reference's
- * ungated generate() makes them. f_index (0..3) and local (0, 1) pin one entry instead (-1: the ladder). Z in 1..18.
+ * ungated generate() makes them. f_index (0..3) and local (0, 1) pin one entry instead (-1: the ladder). Z in 1..18,
+ * h in [0.02, 1] Bohr.
  * The accepted entry of each (Z, f_index, local) is kept for the process; later calls only filter it for h. */
 typedef enum { LIRA_PSEUDO_LOCAL_HIGHEST = 0, LIRA_PSEUDO_LOCAL_POLYNOMIAL = 1 } lira_pseudo_local;
 
