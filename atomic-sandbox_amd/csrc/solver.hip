@@ -526,6 +526,7 @@ template <typename T> class Solver final : public SolverIface {
         for (hipEvent_t &e : rr_ev_) LIRA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
     GraphSlot pcg_graph_[2], pcg_start_graph_;
+    GraphSlot lobpcg_graph_[2];  // one LOBPCG iteration's device segment, per arrangement of the W / TMP buffers
     static void drop_graph(GraphSlot &g) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
         if (g.graph) (void)hipGraphDestroy(g.graph);
@@ -534,7 +535,8 @@ template <typename T> class Solver final : public SolverIface {
         g.key.clear();
     }
     void drop_graphs() {
-        for (GraphSlot *g : {&pcg_graph_[0], &pcg_graph_[1], &pcg_start_graph_}) drop_graph(*g);
+        for (GraphSlot *g : {&pcg_graph_[0], &pcg_graph_[1], &pcg_start_graph_, &lobpcg_graph_[0], &lobpcg_graph_[1]})
+            drop_graph(*g);
     }
     // runs body's launches on stream_: replayed from g when g was captured under key, else captured into g first
     template <typename F> void run_graph(GraphSlot &g, const std::vector<uintptr_t> &key, F &&body) {
@@ -641,6 +643,7 @@ template <typename T> class Solver final : public SolverIface {
     void ionic_geometry(double sigma);
     void nl_apply_block(const T *x, T *out, int nb, const T *x2 = nullptr, T *out2 = nullptr, int nb2 = 0);
     void nl_projections(const T *x, int nb, const T *x2 = nullptr, int nb2 = 0);
+    void ensure_nl(int nb);  // the projections' buffers for nb bands (allocations and the counters' first clear)
     std::vector<double> nl_energy_terms(Channel<T> &c, bool fd);
 
     void ensure_host(size_t count) {
@@ -1313,37 +1316,54 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
             dth_cur = dth;
         }
         const double *dshift = per_column ? dth_cur : dlam;
-        // 1. R = HX - theta X for every column (the head's residuals are measured too), W = T R for the moving ones
-        residual_precond<T>(c.R.template as<T>(), c.W.template as<T>(), c.HX.template as<T>(), X(), dth_cur, v, dshift,
-                            diag_t, k, N_, stream_);
-        StencilParams jp{n_, na, h_, 1.0, omega, diag_t};
-        for (int s = 0; s < 3; s++) {
-            stencil<T>(ST_JAC, jp, col(c.W, locked), col(c.TMP, locked), v, col(c.R, locked), dshift + locked, stream_,
-                       variant(ST_JAC, na));
-            std::swap(c.W, c.TMP);
-        }
-        hamiltonian2(v, col(c.W, locked), col(c.HW, locked), na, col(c.P, locked), col(c.HP, locked), np);
-        // 2. the readback: B = S^T S and S^T T with S = [X, W, P], T = [R, HW, HP] in one Gram product, and ||r_i||^2
+        // 1. R = HX - theta X for every column (the head's residuals are measured too), W = T R for the moving ones;
+        // three preconditioner sweeps W -> TMP -> W -> TMP (the result in TMP's buffer, which then becomes W); H W and
+        // H P; 2. the Gram B = S^T S and S^T T with S = [X, W, P], T = [R, HW, HP] and ||r_i||^2 for the readback.
+        // This device segment is one fixed launch sequence: with graphs on it is captured once per arrangement of
+        // its buffers and sizes and replayed (every allocation and tuning it could need done before).
         const int m = k + na + np, q = 2 * m;
         ensure_gram(m, q);
-        Cols S{{{X(), k}, {col(c.W, locked), na}, {col(c.P, locked), np}}, np > 0 ? 3 : 2};
-        Cols U{{{X(), k}, {col(c.W, locked), na}, {col(c.P, locked), np}, {c.R.ptr, k}, {col(c.HW, locked), na},
-                {col(c.HP, locked), np}}, 6};
-        if (np == 0) U = Cols{{{X(), k}, {col(c.W, locked), na}, {c.R.ptr, k}, {col(c.HW, locked), na}}, 4};
+        const int vj = variant(ST_JAC, na);
+        variant(ST_HAM, np > 0 ? na + np : na);
+        if (nl_rows_ > 0) ensure_nl(na + np);
+        T *W0 = c.W.template as<T>(), *T0 = c.TMP.template as<T>();
         double *dG = dG_.as<double>();
-        gram<T>(S, U, N_, dv_, gpart_.as<double>(), gpart_cap_, dG, stream_);
-        RedJobs<T> jr{};
-        for (int i = 0; i < k; i++) {
-            jr.a[i % MAX_JOBS] = col(c.R, i);
-            jr.b[i % MAX_JOBS] = jr.a[i % MAX_JOBS];
-            jr.kind[i % MAX_JOBS] = RED_DOT;
-            jr.count = i % MAX_JOBS + 1;
-            if (jr.count == MAX_JOBS || i == k - 1) {
-                reduce_jobs<T>(jr, n_, dv_, partials_.as<double>(), dG + (size_t)m * q + (i / MAX_JOBS) * MAX_JOBS,
-                               stream_);
-                jr.count = 0;
+        auto segment = [&] {
+            residual_precond<T>(c.R.template as<T>(), W0, c.HX.template as<T>(), X(), dth_cur, v, dshift, diag_t, k, N_,
+                                stream_);
+            StencilParams jp{n_, na, h_, 1.0, omega, diag_t};
+            T *const wl = W0 + (long)locked * N_, *const tl = T0 + (long)locked * N_;
+            const T *const rl = c.R.template as<T>() + (long)locked * N_;
+            stencil<T>(ST_JAC, jp, wl, tl, v, rl, dshift + locked, stream_, vj);
+            stencil<T>(ST_JAC, jp, tl, wl, v, rl, dshift + locked, stream_, vj);
+            stencil<T>(ST_JAC, jp, wl, tl, v, rl, dshift + locked, stream_, vj);
+            hamiltonian2(v, tl, col(c.HW, locked), na, col(c.P, locked), col(c.HP, locked), np);
+            Cols S{{{X(), k}, {tl, na}, {col(c.P, locked), np}}, np > 0 ? 3 : 2};
+            Cols U{{{X(), k}, {tl, na}, {col(c.P, locked), np}, {c.R.ptr, k}, {col(c.HW, locked), na},
+                    {col(c.HP, locked), np}}, 6};
+            if (np == 0) U = Cols{{{X(), k}, {tl, na}, {c.R.ptr, k}, {col(c.HW, locked), na}}, 4};
+            gram<T>(S, U, N_, dv_, gpart_.as<double>(), gpart_cap_, dG, stream_);
+            RedJobs<T> jr{};
+            for (int i = 0; i < k; i++) {
+                jr.a[i % MAX_JOBS] = col(c.R, i);
+                jr.b[i % MAX_JOBS] = jr.a[i % MAX_JOBS];
+                jr.kind[i % MAX_JOBS] = RED_DOT;
+                jr.count = i % MAX_JOBS + 1;
+                if (jr.count == MAX_JOBS || i == k - 1) {
+                    reduce_jobs<T>(jr, n_, dv_, partials_.as<double>(), dG + (size_t)m * q + (i / MAX_JOBS) * MAX_JOBS,
+                                   stream_);
+                    jr.count = 0;
+                }
             }
-        }
+        };
+        const std::vector<uintptr_t> seg_key{
+            (uintptr_t)X(), (uintptr_t)W0, (uintptr_t)T0, (uintptr_t)c.R.ptr, (uintptr_t)c.HX.ptr, (uintptr_t)c.HW.ptr,
+            (uintptr_t)c.HP.ptr, (uintptr_t)c.P.ptr, (uintptr_t)v, (uintptr_t)dth_cur, (uintptr_t)dshift, (uintptr_t)dG,
+            (uintptr_t)gpart_.ptr, (uintptr_t)partials_.ptr, (uintptr_t)nl_C_.ptr, (uintptr_t)nl_P_.ptr,
+            (uintptr_t)nl_count_.ptr, (uintptr_t)beta_.ptr, (uintptr_t)k, (uintptr_t)na, (uintptr_t)np,
+            (uintptr_t)locked, (uintptr_t)vj, (uintptr_t)nl_rows_, (uintptr_t)ionic_version_};
+        run_graph(lobpcg_graph_[W0 < T0 ? 0 : 1], seg_key, segment);
+        std::swap(c.W, c.TMP);  // the sweeps' result sits in TMP's buffer
         const double *g = readback(dG, (size_t)m * q + k);
         // A's X columns come through R: A_ij = S_i^T R_j + theta_j B_ij
         std::vector<double> Bm((size_t)m * m), Am((size_t)m * m);
@@ -1844,8 +1864,7 @@ template <typename T> void Solver<T>::ionic_geometry(double sigma) {
 }
 
 // out_b += V_NL x_b for a block: the projections C (f64, rows x nb), then the scatter colour by colour
-template <typename T> void Solver<T>::nl_projections(const T *x, int n1, const T *x2, int n2) {
-    const int nb = n1 + n2;
+template <typename T> void Solver<T>::ensure_nl(int nb) {
     const size_t need = (size_t)nl_rows_ * nb * 4;
     if (need > nl_C_cap_) {
         nl_C_.alloc(need * sizeof(double));
@@ -1865,6 +1884,12 @@ template <typename T> void Solver<T>::nl_projections(const T *x, int n1, const T
         LIRA_HIP(hipMemsetAsync(nl_count_.ptr, 0, counters * sizeof(unsigned), stream_));
         nl_count_cap_ = counters;
     }
+}
+
+template <typename T> void Solver<T>::nl_projections(const T *x, int n1, const T *x2, int n2) {
+    ensure_nl(n1 + n2);
+    long max_pts = 0;
+    for (const NlIon &ion : nl_h_) max_pts = std::max(max_pts, ion.pts);
     nl_project<T>(x, n1, x2, n2, N_, n_, beta_.template as<T>(), nl_ions_.as<NlIon>(), (int)nl_h_.size(), max_pts, dv_,
                   nl_C_.as<double>(), nl_P_.as<double>(), nl_count_.as<unsigned>(), stream_);
 }

@@ -435,7 +435,9 @@ lira_status lira_solve_batch(const lira_batch_job *jobs, int n_jobs, int precisi
 
 /* Dispatch control, no counterpart in the reference (its recorder batches dispatches instead,
  * crates/qm-gpu/src/batch.rs:55-233): on, the fixed kernel sequences between two host syncs (the Poisson MG-PCG
- * iteration, in its two forms, and its start) are captured once as HIP graphs and replayed; off (the default), every
+ * iteration, in its two forms, and its start; each LOBPCG iteration's segment from the residuals through the
+ * preconditioner, H W, H P and the KB projections to the Gram product) are captured once per buffer arrangement and
+ * block size as HIP graphs and replayed; off (the default), every
  * kernel is launched one by one. Results are bitwise identical either way. Turn it on only where no other host thread
  * allocates device memory or launches work while this solver captures (a capture is open during the first call of
  * each sequence); with solvers on several threads keep it off. */

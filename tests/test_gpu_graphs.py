@@ -61,3 +61,28 @@ def test_scf_with_graphs_is_bitwise_the_eager_scf(lira_mod):
         assert a.lobpcg_iterations == b.lobpcg_iterations and a.cg_iterations == b.cg_iterations
         assert np.array_equal(g.field(2), e.field(2))
         assert np.array_equal(g.forces(H2[0], H2[1], h), e.forces(H2[0], H2[1], h))
+
+
+def test_pseudised_scf_with_graphs_is_bitwise_the_eager_scf(lira_mod):
+    # the captured LOBPCG segment with the KB projections and scatter in it (a generated O table, water at 33^3)
+    lira = lira_mod
+    n, h = 33, 0.25
+    r, half = 0.96 * 1.8897261, np.radians(104.5 / 2)
+    pos = [[0.0, 0.0, 0.0], [r * np.sin(half), r * np.cos(half), 0.0], [-r * np.sin(half), r * np.cos(half), 0.0]]
+    z = [8, 1, 1]
+    oxygen = lira.generate_pseudo(8, h)
+    st = lira.default_settings(h, 1)
+    for precision in (1, 0):
+        g = lira.GpuSolver(n, h, precision)
+        g.set_graphs(True)
+        g.set_ionic([oxygen])
+        e = lira.GpuSolver(n, h, precision)
+        e.set_graphs(False)
+        e.set_ionic([oxygen])
+        for _ in range(2):
+            a = g.solve(pos, z, 4, 4, st)
+            b = e.solve(pos, z, 4, 4, st)
+            assert a.energy.total == b.energy.total and a.iterations == b.iterations, precision
+            assert a.lobpcg_iterations == b.lobpcg_iterations and a.cg_iterations == b.cg_iterations
+            assert np.array_equal(g.field(2), e.field(2))
+            assert np.array_equal(g.forces(pos, z, h), e.forces(pos, z, h))
