@@ -422,6 +422,22 @@ template <typename T> struct Channel {
     std::vector<double> evals, occ, res;
 };
 
+// Waits for e by polling it. The solve's host syncs are short (a Gram product, one CG iteration: tens of
+// microseconds) and a blocking wait adds an interrupt's wake-up of about that order to each; past 2 ms of polling the
+// wait blocks instead.
+static void wait_event(hipEvent_t e) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned i = 1;; i++) {
+        const hipError_t r = hipEventQuery(e);
+        if (r == hipSuccess) return;
+        if (r != hipErrorNotReady) LIRA_HIP(r);
+        if ((i & 63) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) {
+            LIRA_HIP(hipEventSynchronize(e));
+            return;
+        }
+    }
+}
+
 template <typename T> class Solver final : public SolverIface {
   public:
     Solver(int n, double h);
@@ -430,6 +446,7 @@ template <typename T> class Solver final : public SolverIface {
         drop_graphs();
         for (hipEvent_t &e : rr_ev_)
             if (e) (void)hipEventDestroy(e);
+        if (rb_ev_) (void)hipEventDestroy(rb_ev_);
         if (rr_host_) (void)hipHostFree(rr_host_);
         if (hbuf_) (void)hipHostFree(hbuf_);
         if (up_) (void)hipHostFree(up_);
@@ -568,6 +585,7 @@ template <typename T> class Solver final : public SolverIface {
     hipStream_t stream_ = nullptr;
     double *hbuf_ = nullptr;
     size_t hcap_ = 0;
+    hipEvent_t rb_ev_ = nullptr;  // marks a readback's copy
     double *up_ = nullptr;  // pinned upload arena: small host->device copies stay asynchronous
     size_t up_cap_ = 0, up_pos_ = 0;
     DevBuf partials_, dres_, gpart_, dG_, dC_, dsmall_;
@@ -657,8 +675,10 @@ template <typename T> class Solver final : public SolverIface {
     // device failure (the reference's readback-fault latch, crates/qm-gpu/src/grid.rs:113-157): no kernel is disturbed.
     const double *readback(const double *d, size_t count) {
         ensure_host(count);
+        if (!rb_ev_) LIRA_HIP(hipEventCreateWithFlags(&rb_ev_, hipEventDisableTiming));
         LIRA_HIP(hipMemcpyAsync(hbuf_, d, count * sizeof(double), hipMemcpyDeviceToHost, stream_));
-        LIRA_HIP(hipStreamSynchronize(stream_));
+        LIRA_HIP(hipEventRecord(rb_ev_, stream_));
+        wait_event(rb_ev_);  // the copy is the stream's last command: the stream is idle
         up_pos_ = 0;  // the stream is idle: every queued upload has been consumed
         readbacks_++;
         if (fault_after_ >= 0 && fault_after_-- == 0) throw DeviceError("a readback failed (injected)");
@@ -1073,7 +1093,7 @@ template <typename T> int Solver<T>::pcg(const T *r0, T *e, double rel_tol, int 
         // check the previous iteration's residual while this one runs
         const int prev = slot ^ 1;
         if (pending[prev]) {
-            LIRA_HIP(hipEventSynchronize(ev[prev]));
+            wait_event(ev[prev]);
             pending[prev] = false;
             rel = std::sqrt(std::max(hrr[prev], 0.0)) / r0n;
             if (!std::isfinite(rel)) throw NumericalError("poisson: non-finite CG residual");
@@ -1280,7 +1300,7 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
             enqueue(t);
             for (;;) {
                 if (queued == t && t < max_iter) enqueue(++queued);
-                LIRA_HIP(hipEventSynchronize(rr_ev_[t & 1]));
+                wait_event(rr_ev_[t & 1]);
                 readbacks_++;
                 if (fault_after_ >= 0 && fault_after_-- == 0) throw DeviceError("a readback failed (injected)");
                 const double *r = rr_host_ + (t & 1) * RR_REC_STRIDE;
