@@ -1091,7 +1091,10 @@ constexpr int G2_TI = 32, G2_TJ = 64;
 // one stage ahead so their latency hides behind the 32 MFMAs of the current stage. The four waves take interleaved
 // stages of the workgroup's chunk; their 32 x 64 accumulators are summed in a fixed order at the end.
 constexpr int G2_S = 16;  // points per wave stage
-constexpr int G2_ZERO_ROW = G2_TI + G2_TJ;  // a slab row that stays zero (the B operand of a U column past q)
+// A wave's slab slice holds the 4 LOADS rows a stage loads (the tile's 32 S rows, then the packed U rows) and, last,
+// a row that stays zero (the B operand of a U column past q): sized by LOADS, the slab of the common 16-load shape
+// takes 35 KB instead of the 53 KB of the widest, and four workgroups fit a CU's LDS instead of two.
+template <int LOADS> constexpr int g2_rows() { return 4 * LOADS + 1; }
 
 // Every load is issued unconditionally and its value is not touched until the stage is staged: a missing column or a
 // point past the chunk reads a zero constant instead. (A load under a branch, or a select on the loaded value, makes
@@ -1129,10 +1132,15 @@ __host__ __device__ inline bool gram_mirrored(int sym, int m, int i, int j) {
 // q nor one of the tile's S columns (U = [S, R, HW, HP] in LOBPCG): each U column reads its B operand from its packed
 // row, its S row, or the zero row. LOADS = 8 (the S rows) + a quarter of the packed U columns, rounded up to a
 // multiple of 4: the elements each lane loads per 16-point stage.
+// At least three waves per SIMD: the compiler then keeps the accumulators in VGPRs (128 registers at 16 loads, four
+// waves) instead of 138 VGPRs + 64 AGPRs (two waves), with no spills; the loads of more waves are in flight at once.
 template <typename T, int LOADS>
-__global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q, long N, long chunk,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_gram(ColPtrs S, ColPtrs U, int m,
+                                                                                     int q, long N, long chunk,
                                               double *partials, int sym) {
-    __shared__ double slab[4][G2_TI + G2_TJ + 1][G2_S + 1];  // one slice per wave; reused for the final reduction
+    constexpr int ROWS = g2_rows<LOADS>(), ZERO_ROW = ROWS - 1;
+    static_assert(4 * ROWS * (G2_S + 1) >= 3 * 16 * 64, "the final reduction's half pass must fit the slab");
+    __shared__ double slab[4][ROWS][G2_S + 1];  // one slice per wave; reused for the final reduction
     __shared__ const T *cols[G2_TI + G2_TJ];
     __shared__ int urow[G2_TJ];  // the slab row each U column's B operand is read from
     const int nbj = (q + G2_TJ - 1) / G2_TJ;
@@ -1169,11 +1177,11 @@ __global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q
         const bool real = uc != nullptr && srow < 0;
         const unsigned long long mask = __ballot(real);
         const int rank = __popcll(mask & ((1ull << c) - 1ull));
-        urow[c] = srow >= 0 ? srow : real ? G2_TI + rank : G2_ZERO_ROW;
+        urow[c] = srow >= 0 ? srow : real ? G2_TI + rank : ZERO_ROW;
         if (real) cols[G2_TI + rank] = uc;
     }
     // the zero row of every wave's slice
-    if (lane < G2_S) slab[wave][G2_ZERO_ROW][lane] = 0.0;
+    if (lane < G2_S) slab[wave][ZERO_ROW][lane] = 0.0;
     lira_d4 acc[2][4];
 #pragma unroll
     for (int a = 0; a < 2; a++)
@@ -1222,33 +1230,33 @@ __global__ __launch_bounds__(256) void k_gram(ColPtrs S, ColPtrs U, int m, int q
         stage(va, p0 + 8 * G2_S);
         stage(vb, p0 + 12 * G2_S);
     }
-    // sum the four waves' accumulators in wave order through LDS (32 doubles per lane per wave)
-    __syncthreads();
+    // sum the four waves' accumulators in wave order through LDS, one 16 x 64 half (acc[a]) per pass: 16 doubles per
+    // lane per wave
     double *red = &slab[0][0][0];
-    if (wave > 0) {
+    double *dst = partials + (long)blockIdx.x * m * q;
+    const int jl = lane & 15, ib = lane >> 4;
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+    for (int a = 0; a < 2; a++) {
+        __syncthreads();
+        if (wave > 0) {
 #pragma unroll
             for (int b = 0; b < 4; b++)
 #pragma unroll
-                for (int r = 0; r < 4; r++) red[(((wave - 1) * 32) + (a * 16 + b * 4 + r)) * 64 + lane] = acc[a][b][r];
-    }
-    __syncthreads();
-    if (wave == 0) {
-        double *dst = partials + (long)blockIdx.x * m * q;
-        const int jl = lane & 15, ib = lane >> 4;
-#pragma unroll
-        for (int a = 0; a < 2; a++)
+                for (int r = 0; r < 4; r++) red[(((wave - 1) * 16) + (b * 4 + r)) * 64 + lane] = acc[a][b][r];
+        }
+        __syncthreads();
+        if (wave == 0) {
 #pragma unroll
             for (int b = 0; b < 4; b++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int slot = a * 16 + b * 4 + r;
-                    const double sum = ((acc[a][b][r] + red[(0 * 32 + slot) * 64 + lane]) +
-                                        red[(1 * 32 + slot) * 64 + lane]) + red[(2 * 32 + slot) * 64 + lane];
+                    const int slot = b * 4 + r;
+                    const double sum = ((acc[a][b][r] + red[(0 * 16 + slot) * 64 + lane]) +
+                                        red[(1 * 16 + slot) * 64 + lane]) + red[(2 * 16 + slot) * 64 + lane];
                     const int i = i0 + a * 16 + ib + 4 * r, j = j0 + b * 16 + jl;
                     if (i < m && j < q) dst[(long)i * q + j] = sum;
                 }
+        }
     }
 }
 
@@ -1273,9 +1281,11 @@ __global__ __launch_bounds__(256) void k_gram_finish(const double *partials, int
     if (lane == 0 && out < mq) G[out] = acc * scale;
 }
 
+// about 768 workgroups in all: three per CU in one round (the 16-load shape fits four per CU; 512 or 1024 measured
+// slower at the LOBPCG shapes of a 95^3 grid)
 static int gram_chunks(int m, int q, long N) {
     const int blocks = ((m + G2_TI - 1) / G2_TI) * ((q + G2_TJ - 1) / G2_TJ);
-    int chunks = (512 + blocks - 1) / blocks;
+    int chunks = (768 + blocks - 1) / blocks;
     const long max_chunks = (N + 4 * G2_S - 1) / (4 * G2_S);
     if (chunks > max_chunks) chunks = (int)max_chunks;
     if (chunks < 1) chunks = 1;
