@@ -270,16 +270,15 @@ __device__ inline void block_sum_rows(double *acc, int rows, double (*lds)[RMAX]
 
 // Projections over chunks of each ion's box. Block (chunk, ion, band group) writes the partial sums of its chunk for
 // NL_BANDS bands, P[((row_off + r) * nb + b) * chunks + chunk] = sum over the chunk's points of beta_r(p) x_b(p) (a
-// chunk is NL_CHUNK consecutive box points, 8 per thread; chunks past an ion's box write zeros); the last of the chunks
-// of one (ion, band group) to finish (a counter per (ion, group), reset by that block) sums them in chunk order into
-// C[(row_off + r) * nb + b] = dv * sum. Bands b < n1 read x1, the others x2 (two blocks in one launch).
+// chunk is NL_CHUNK consecutive box points, 8 per thread; chunks past an ion's box write zeros); k_nl_finish then sums
+// each (row, band)'s chunks in a fixed order into C[(row_off + r) * nb + b] = dv * sum. Bands b < n1 read x1, the
+// others x2 (two blocks in one launch). (Folding the finish into the last chunk block to arrive costs more: each
+// block would pay a device-scope release fence, a write-back of its XCD's dirty L2 lines.)
 constexpr int NL_CHUNK = 2048, NL_BANDS = 4;
 template <typename T>
 __global__ __launch_bounds__(256) void k_nl_project(const T *x1, int n1, const T *x2, int nb, long N, int n,
-                                                    const T *beta, const NlIon *ions, int chunks, double *P,
-                                                    double dv, double *C, unsigned *counters) {
+                                                    const T *beta, const NlIon *ions, int chunks, double *P) {
     __shared__ double lds[4][NL_MAX_ROWS];
-    __shared__ bool last;
     const NlIon &ion = ions[blockIdx.y];
     const int b0 = blockIdx.z * NL_BANDS, chunk = blockIdx.x;
     const int bands = nb - b0 < NL_BANDS ? nb - b0 : NL_BANDS;
@@ -328,30 +327,24 @@ __global__ __launch_bounds__(256) void k_nl_project(const T *x1, int n1, const T
                 const int r = threadIdx.x;
                 P[((long)(ion.row_off + r) * nb + b0 + j) * chunks + chunk] =
                     ((lds[0][r] + lds[1][r]) + lds[2][r]) + lds[3][r];
-                __threadfence();
             }
             __syncthreads();
         }
     }
-    const int groups = (nb + NL_BANDS - 1) / NL_BANDS;
-    unsigned *counter = counters + (long)blockIdx.y * groups + blockIdx.z;
-    if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == (unsigned)chunks - 1u;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    // one wave per (row, band): lane l sums chunks l, l + 64, ... in order, then the wave's fixed shuffle tree
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int pair = wave; pair < rows * bands; pair += 4) {
-        const int r = pair % rows, j = pair / rows;
-        const long row = ion.row_off + r;
-        const double *pp = P + (row * nb + b0 + j) * chunks;
-        double v = 0.0;
-        for (int c = lane; c < chunks; c += 64) v += pp[c];
+}
+
+// C[pair] = dv * the sum of P[pair * chunks + c] over the chunks, pair = row * nb + band, one wave per pair: lane l
+// sums chunks l, l + 64, ... in order, then the wave's fixed shuffle tree
+__global__ __launch_bounds__(256) void k_nl_finish(const double *P, int chunks, long pairs, double dv, double *C) {
+    const int lane = threadIdx.x & 63;
+    const long pair = blockIdx.x * 4L + (threadIdx.x >> 6);
+    if (pair >= pairs) return;  // uniform over the wave
+    const double *pp = P + pair * chunks;
+    double v = 0.0;
+    for (int c = lane; c < chunks; c += 64) v += pp[c];
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-        if (lane == 0) C[row * nb + b0 + j] = dv * v;
-    }
-    if (threadIdx.x == 0) *counter = 0u;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) C[pair] = dv * v;
 }
 
 // out_b(p) += sum_r E_r C[r][b] beta_r(p) over the boxes of the ions of one colour (disjoint boxes: no two threads
@@ -380,12 +373,15 @@ int nl_project_chunks(long max_pts) { return (int)std::max(1L, (max_pts + NL_CHU
 
 template <typename T>
 void nl_project(const T *x1, int n1, const T *x2, int n2, long N, int n, const T *beta, const NlIon *ions, int nions,
-                long max_pts, double dv, double *C, double *work, unsigned *counters, hipStream_t s) {
+                long max_pts, int rows, double dv, double *C, double *work, hipStream_t s) {
     const int nb = n1 + n2;
     if (nions <= 0 || nb <= 0) return;
     const int chunks = nl_project_chunks(max_pts);
     hipLaunchKernelGGL(k_nl_project<T>, dim3(chunks, nions, (nb + NL_BANDS - 1) / NL_BANDS), dim3(256), 0, s, x1, n1,
-                       x2, nb, N, n, beta, ions, chunks, work, dv, C, counters);
+                       x2, nb, N, n, beta, ions, chunks, work);
+    LIRA_HIP(hipGetLastError());
+    const long pairs = (long)rows * nb;
+    hipLaunchKernelGGL(k_nl_finish, dim3((unsigned)((pairs + 3) / 4)), dim3(256), 0, s, work, chunks, pairs, dv, C);
     LIRA_HIP(hipGetLastError());
 }
 
@@ -557,8 +553,8 @@ void nl_force(const T *psi, int nb, long N, int n, double h, const NlIon *ions, 
                                   double, hipStream_t);                                                               \
     template void ionic_fields<T>(T *, T *, int, double, const IonicCentre *, int, const double *, hipStream_t);      \
     template void nl_beta<T>(T *, const NlIon &, int, double, const double *, hipStream_t);                          \
-    template void nl_project<T>(const T *, int, const T *, int, long, int, const T *, const NlIon *, int, long,       \
-                                double, double *, double *, unsigned *, hipStream_t);                                 \
+    template void nl_project<T>(const T *, int, const T *, int, long, int, const T *, const NlIon *, int, long, int,  \
+                                double, double *, double *, hipStream_t);                                             \
     template void nl_apply<T>(T *, int, T *, int, long, int, const T *, const NlIon *, const int *, int, long,          \
                               const double *, const double *, hipStream_t);                                           \
     template void ionic_forces<T>(const T *, const T *, const T *, int, double, const IonicCentre *, const double *, \

@@ -192,12 +192,12 @@ void ionic_fields(T *vloc, T *pc, int n, double h, const IonicCentre *c, int nat
 template <typename T> void nl_beta(T *beta, const NlIon &ion, int n, double h, const double *tab, hipStream_t s);
 // C[(row_off + r) * nb + b] = dv sum_box beta_r x_b for every ion with projectors, nb = n1 + n2 bands (the first n1
 // from x1, the rest from x2); the box sums run in chunks (nl_project_chunks(max_pts) per box), their partials in work
-// (rows * nb * chunks doubles) summed in a fixed order by the last chunk of each (ion, band), whose counter
-// (counters[ion * nb + b], zero between launches) it resets
+// (rows * nb * chunks doubles, rows = all ions' projector rows) summed per (row, band) in a fixed order by a second
+// launch
 int nl_project_chunks(long max_pts);
 template <typename T>
 void nl_project(const T *x1, int n1, const T *x2, int n2, long N, int n, const T *beta, const NlIon *ions, int nions,
-                long max_pts, double dv, double *C, double *work, unsigned *counters, hipStream_t s);
+                long max_pts, int rows, double dv, double *C, double *work, hipStream_t s);
 // out_b += sum_r erow_r C_rb beta_r on the boxes of the listed ions (boxes must be disjoint), bands b < n1 in out1,
 // the rest in out2
 template <typename T>

@@ -625,8 +625,8 @@ template <typename T> class Solver final : public SolverIface {
     // pseudised ions: the setup (tables copied at lira_set_ionic) and the terms of the current geometry
     std::vector<SpeciesHost> species_;
     std::vector<double> tab_h_;
-    DevBuf tab_, centres_, nl_ions_, beta_, nl_C_, nl_erow_, nl_members_, pc_, qf_, fpart_, fout_, nl_P_, nl_count_;
-    size_t nl_C_cap_ = 0, nl_P_cap_ = 0, nl_count_cap_ = 0;
+    DevBuf tab_, centres_, nl_ions_, beta_, nl_C_, nl_erow_, nl_members_, pc_, qf_, fpart_, fout_, nl_P_;
+    size_t nl_C_cap_ = 0, nl_P_cap_ = 0;
     int ionic_version_ = 0, state_ionic_version_ = -1;
     std::vector<NlIon> nl_h_;
     std::vector<int> nl_atom_;  // the nucleus of each NlIon
@@ -661,7 +661,7 @@ template <typename T> class Solver final : public SolverIface {
     void ionic_geometry(double sigma);
     void nl_apply_block(const T *x, T *out, int nb, const T *x2 = nullptr, T *out2 = nullptr, int nb2 = 0);
     void nl_projections(const T *x, int nb, const T *x2 = nullptr, int nb2 = 0);
-    void ensure_nl(int nb);  // the projections' buffers for nb bands (allocations and the counters' first clear)
+    void ensure_nl(int nb);  // the projections' buffers for nb bands
     std::vector<double> nl_energy_terms(Channel<T> &c, bool fd);
 
     void ensure_host(size_t count) {
@@ -1380,7 +1380,7 @@ EigenOut Solver<T>::lobpcg(Channel<T> &c, const T *v, int k, double lam, double 
             (uintptr_t)X(), (uintptr_t)W0, (uintptr_t)T0, (uintptr_t)c.R.ptr, (uintptr_t)c.HX.ptr, (uintptr_t)c.HW.ptr,
             (uintptr_t)c.HP.ptr, (uintptr_t)c.P.ptr, (uintptr_t)v, (uintptr_t)dth_cur, (uintptr_t)dshift, (uintptr_t)dG,
             (uintptr_t)gpart_.ptr, (uintptr_t)partials_.ptr, (uintptr_t)nl_C_.ptr, (uintptr_t)nl_P_.ptr,
-            (uintptr_t)nl_count_.ptr, (uintptr_t)beta_.ptr, (uintptr_t)k, (uintptr_t)na, (uintptr_t)np,
+            (uintptr_t)beta_.ptr, (uintptr_t)k, (uintptr_t)na, (uintptr_t)np,
             (uintptr_t)locked, (uintptr_t)vj, (uintptr_t)nl_rows_, (uintptr_t)ionic_version_};
         run_graph(lobpcg_graph_[W0 < T0 ? 0 : 1], seg_key, segment);
         std::swap(c.W, c.TMP);  // the sweeps' result sits in TMP's buffer
@@ -1897,21 +1897,14 @@ template <typename T> void Solver<T>::ensure_nl(int nb) {
         nl_P_.alloc(work * sizeof(double));
         nl_P_cap_ = work;
     }
-    // one counter per (ion, band), zero between launches (the last chunk of each resets its own)
-    const size_t counters = nl_h_.size() * (size_t)nb;
-    if (counters > nl_count_cap_) {
-        nl_count_.alloc(counters * sizeof(unsigned));
-        LIRA_HIP(hipMemsetAsync(nl_count_.ptr, 0, counters * sizeof(unsigned), stream_));
-        nl_count_cap_ = counters;
-    }
 }
 
 template <typename T> void Solver<T>::nl_projections(const T *x, int n1, const T *x2, int n2) {
     ensure_nl(n1 + n2);
     long max_pts = 0;
     for (const NlIon &ion : nl_h_) max_pts = std::max(max_pts, ion.pts);
-    nl_project<T>(x, n1, x2, n2, N_, n_, beta_.template as<T>(), nl_ions_.as<NlIon>(), (int)nl_h_.size(), max_pts, dv_,
-                  nl_C_.as<double>(), nl_P_.as<double>(), nl_count_.as<unsigned>(), stream_);
+    nl_project<T>(x, n1, x2, n2, N_, n_, beta_.template as<T>(), nl_ions_.as<NlIon>(), (int)nl_h_.size(), max_pts,
+                  nl_rows_, dv_, nl_C_.as<double>(), nl_P_.as<double>(), stream_);
 }
 
 template <typename T> void Solver<T>::nl_apply_block(const T *x, T *out, int nb, const T *x2, T *out2, int nb2) {
